@@ -1,0 +1,291 @@
+/*
+ * naer.h — C ABI of the MI355X (gfx950) spiking-network simulation step.
+ *
+ * The reference has no FFI layer: its simulation step sits behind the C++
+ * class Naer::Network (include/naer/Network.h:15-56 of the reference) and the
+ * positional OpenCL kernel argument lists bound in Network::prepareKernels
+ * (src/Network.cpp:571-652) and Network::step (src/Network.cpp:122-256).
+ * This header is the drop-in boundary that replaces those: plain C types,
+ * plain pointers and sizes, no HIP or C++ types in any signature.  The C++
+ * Naer::Network in Network.h is implemented on top of it.
+ *
+ * Conventions
+ *   - every function returns NAER_OK (0) or a negative NAER_ERR_* code; the
+ *     message of the last failure on the calling thread is naer_last_error().
+ *   - a naer_net is used by one host thread at a time.  Work is enqueued on the
+ *     net's own HIP stream; naer_net_step() is asynchronous, naer_net_sync()
+ *     is the synchronisation point (reference Network::finish,
+ *     src/Network.cpp:258-265).
+ *   - the library owns all device memory of a net and frees it in
+ *     naer_net_destroy() (the reference never releases its buffers,
+ *     src/Network.cpp:49-75).
+ *
+ * Simulation scheme (clock-driven, fixed step dt)
+ *   step t covers [t*dt, (t+1)*dt).  Per step:
+ *     1. LIF update of every neuron: read and clear the delay-ring slot t,
+ *        add the counter-based Poisson drive, exact propagator, refractory
+ *        countdown, threshold and reset; fired neurons are compacted into
+ *        the step's spike list.  POISSON source neurons emit the reference's
+ *        exponential-ISI trains instead (input_neuron_group.cl:6-45).
+ *     2. spike scatter: the CSR row of every spiking source is streamed and
+ *        each synapse (target, delay d, weight w) adds w into ring slot t+d of
+ *        its target.  NAER_MODE_FAST accumulates with fp32 hardware atomics;
+ *        NAER_MODE_DETERMINISTIC accumulates the weights as 32.32 fixed-point
+ *        int64 values (w rounded once to a multiple of 2^-32 mV), whose sum is
+ *        exact and therefore independent of event order.
+ *     3. STDP (plastic synapses only): pair-based additive STDP with per-neuron
+ *        pre/post traces; depression at pre-synaptic spikes (fused into the
+ *        scatter, the depressed weight is the one delivered), potentiation at
+ *        post-synaptic spikes through a reverse (incoming) index.
+ *
+ * Synapse storage: outgoing CSR.  row_ptr is uint64[N+1]; each synapse has a
+ * uint32 word  target | delay_steps << 24 | plastic << 31  and an fp32 weight.
+ * Rows are sorted ascending by (word, weight bits), so the layout is a pure
+ * function of the network description and seeds.
+ * Limits: N <= 2^24 neurons per net, delays 1..127 steps.
+ */
+#ifndef NAER_H
+#define NAER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------------- */
+#define NAER_OK 0
+#define NAER_ERR_INVALID (-1)  /* bad argument / description                 */
+#define NAER_ERR_HIP (-2)      /* HIP runtime failure                          */
+#define NAER_ERR_NOMEM (-3)    /* device allocation failed                     */
+#define NAER_ERR_CAPACITY (-4) /* a documented size limit was exceeded         */
+#define NAER_ERR_STATE (-5)    /* call not valid in the object's current state */
+#define NAER_ERR_NODEVICE (-6) /* no HIP device / device index out of range    */
+
+/* ---- random streams --------------------------------------------------------
+ * Every random quantity is a pure function of (seed, index), drawn from
+ * Philox4x32-10 with key = {low 32 bits, high 32 bits} of the seed:
+ *   initial potential of neuron j:        ctr = {j, 0, NAER_STREAM_V0, 0}, word 0
+ *   Poisson drive of neuron j at step t:   ctr = {j, t >> 2, NAER_STREAM_DRIVE, 0},
+ *                                          word t & 3
+ *   draw k of target j, FIXED_INDEGREE connection c (connection seed):
+ *                                          ctr = {j, k, NAER_STREAM_CONN, c};
+ *                                          word 0 -> source, 1 -> delay, 2 -> weight
+ * A uniform u in [0,1) is (word >> 8) * 2^-24 in fp32; an index in [0,n) is
+ * (word * n) >> 32.  BERNOULLI connections use the reference's own map
+ * instead (random.cl:6-24, network.cl:34-41): key = {src*32 + dst%32, seed32},
+ * ctr = {dst,0,0,0}, connected iff word 2 / 4294967295.0 < p; word 1 draws the
+ * delay and word 3 the random weight. */
+#define NAER_STREAM_V0 1u
+#define NAER_STREAM_DRIVE 2u
+#define NAER_STREAM_CONN 3u
+
+/* The drive count k of one step is found by inversion of a per-population
+ * CDF table: c_0 = e^-lambda, c_k = c_{k-1} + e^-lambda lambda^k / k! summed
+ * in fp64 and rounded to fp32; the first entry that rounds to >= 1.0f (or entry
+ * NAER_DRIVE_TABLE-1) is replaced by 2.0f; k = min{k : u < cdf[k]}.
+ * lambda = ext_rate_hz * dt_ms * 1e-3 must be <= 20. */
+#define NAER_DRIVE_TABLE 64
+
+/* ---- description enums ---------------------------------------------------- */
+/* Population kinds.  The reference's NeuronType::NOP and NeuronType::STDP are
+ * both leaky integrate-and-fire neurons (nop.cl, stdp.cl); plasticity is a
+ * property of connections here.  NeuronType::POISSON is a spike source. */
+#define NAER_POP_LIF 0u
+#define NAER_POP_POISSON 1u
+
+/* Connection rules. */
+#define NAER_RULE_BERNOULLI 0u      /* pairwise Bernoulli(p), network.cl:14-107 */
+#define NAER_RULE_FIXED_INDEGREE 1u /* K sources per target, drawn uniformly     */
+
+/* Accumulation modes of the spike scatter. */
+#define NAER_MODE_FAST 0u
+#define NAER_MODE_DETERMINISTIC 1u
+
+/* ---- network description -------------------------------------------------- */
+typedef struct naer_pop_desc {
+    uint32_t kind; /* NAER_POP_LIF or NAER_POP_POISSON */
+    uint32_t size; /* number of neurons                 */
+    /* LIF parameters (ignored for POISSON) */
+    float tau_m_ms;   /* membrane time constant                     */
+    float v_th;       /* threshold (mV); a neuron fires when V > v_th */
+    float v_reset;    /* reset potential (mV)                       */
+    float v_rest;     /* resting potential E_L (mV)                 */
+    float t_ref_ms;   /* absolute refractory period                 */
+    float v0_min;     /* initial V drawn uniformly in [v0_min, v0_max) */
+    float v0_max;
+    float ext_rate_hz; /* independent Poisson drive per neuron (0 = none) */
+    float ext_weight;  /* jump per drive event (mV)                        */
+    /* POISSON source parameter: rate of the exponential-ISI train (Hz);
+     * the reference hard-codes 20 Hz (input_neuron_group.cl:4). */
+    float src_rate_hz;
+} naer_pop_desc;
+
+typedef struct naer_conn_desc {
+    uint32_t src_pop; /* index into naer_net_desc.pops */
+    uint32_t dst_pop;
+    uint32_t rule;     /* NAER_RULE_*                                        */
+    uint32_t indegree; /* FIXED_INDEGREE: sources per target                 */
+    double probability; /* BERNOULLI: connection probability                 */
+    /* weights: `weight` for every synapse, or, when weight == 0.0f (the
+     * reference's convention, network.cl:97-100), uniform in [wmin, wmax). */
+    float weight;
+    float wmin;
+    float wmax;
+    /* delays in steps: delay_min + m * delay_quantum, m uniform in
+     * {0, ..., (delay_max - delay_min) / delay_quantum}. */
+    uint32_t delay_min;
+    uint32_t delay_max;
+    uint32_t delay_quantum;
+    uint32_t plastic; /* 1: STDP on this connection (source and target LIF) */
+    uint32_t _pad;
+    /* The reference truncates the 64-bit seed to 32 bits
+     * (src/Network.cpp:447,532); BERNOULLI uses the low 32 bits exactly as
+     * the reference does, FIXED_INDEGREE uses all 64. */
+    uint64_t seed;
+} naer_conn_desc;
+
+typedef struct naer_stdp_desc {
+    float tau_plus_ms;  /* pre-synaptic trace time constant  */
+    float tau_minus_ms; /* post-synaptic trace time constant */
+    float a_plus;       /* potentiation step (mV)            */
+    float a_minus;      /* depression step (mV)              */
+    float w_min;        /* hard bounds (mV)                  */
+    float w_max;
+} naer_stdp_desc;
+
+typedef struct naer_net_desc {
+    const naer_pop_desc* pops;
+    uint32_t n_pops;
+    uint32_t n_conns;
+    const naer_conn_desc* conns;
+    naer_stdp_desc stdp;
+    float dt_ms;   /* step; dt_ms*1000 must be an integer number of us     */
+    uint32_t mode; /* NAER_MODE_FAST or NAER_MODE_DETERMINISTIC           */
+    uint64_t seed; /* initial potentials and Poisson drive                */
+    /* seed of the POISSON source trains; the reference passes
+     * (uint32_t)0xbf785e3215ac7ebd (src/Network.cpp:124). */
+    uint32_t poisson_seed;
+    /* spikes kept on the device for naer_net_read_spikes (0 = no recording) */
+    uint32_t record_capacity;
+} naer_net_desc;
+
+/* One recorded spike.  For POISSON sources `time_us` is the exact train time
+ * (integer microseconds, input_neuron_group.cl:29-37); for LIF neurons it is
+ * the start of the step in which the neuron fired. */
+typedef struct naer_spike {
+    uint32_t step;
+    uint32_t neuron;
+    uint32_t time_us;
+    uint32_t _pad;
+} naer_spike;
+
+typedef struct naer_stats {
+    uint64_t n_neurons;
+    uint64_t n_synapses;
+    uint64_t n_plastic;
+    uint64_t step;       /* steps simulated so far                  */
+    uint64_t time_us;    /* simulated time (reference getTime())    */
+    uint64_t spikes;     /* spikes emitted (recording independent)  */
+    uint64_t events;     /* synaptic events delivered (sum of out-degrees of emitted spikes) */
+    uint64_t dropped;    /* spikes lost to a full record buffer     */
+    uint32_t ring_depth; /* delay-ring slots                        */
+    uint32_t batch;      /* steps per update launch                 */
+    uint32_t max_row;    /* longest CSR row                         */
+    uint32_t kernels;    /* number of entries in kernel_ms/kernel_calls */
+    /* per-kernel device time accumulated while profiling is on (ms) and
+     * launch counts; index with NAER_K_* */
+    double kernel_ms[8];
+    uint64_t kernel_calls[8];
+    uint64_t device_bytes; /* device memory held by the net */
+} naer_stats;
+
+#define NAER_K_UPDATE 0
+#define NAER_K_SCATTER 1
+#define NAER_K_POTENTIATE 2
+#define NAER_K_CLOCK 3
+
+/* ---- context ---------------------------------------------------------------
+ * Replaces the OpenCL platform/device/context/queue set-up of
+ * Network::Network (src/Network.cpp:14-47). */
+typedef struct naer_ctx naer_ctx;
+int naer_ctx_create(int device, naer_ctx** out);
+int naer_ctx_destroy(naer_ctx* ctx);
+
+/* ---- network ---------------------------------------------------------------
+ * naer_net_create copies the description (pops/conns arrays may be freed
+ * afterwards).  It replaces addPopulation/addConnection bookkeeping
+ * (src/Network.cpp:76-104). */
+typedef struct naer_net naer_net;
+int naer_net_create(naer_ctx* ctx, const naer_net_desc* desc, naer_net** out);
+
+/* Device connectivity build + state initialisation; replaces
+ * Network::compile (src/Network.cpp:267-308) and allocateConnections
+ * (src/Network.cpp:421-569).  Synchronous. */
+int naer_net_build(naer_net* net);
+
+/* Enqueue n_steps simulation steps on the net's stream (asynchronous).
+ * Network::step (src/Network.cpp:122-256) advances one 1.5 ms epoch; the C++
+ * wrapper calls this with n_steps = 1500 us / dt. */
+int naer_net_step(naer_net* net, uint32_t n_steps);
+
+/* Wait for all enqueued work (Network::finish, src/Network.cpp:258-265). */
+int naer_net_sync(naer_net* net);
+
+/* Counters and timers (getTime/getSynapses/getNeurons and the public
+ * *_time fields, include/naer/Network.h:44-55).  Synchronises the net. */
+int naer_net_query(naer_net* net, naer_stats* out);
+
+/* Per-kernel HIP-event timing on the net's stream (off by default). */
+int naer_net_set_profiling(naer_net* net, int enable);
+
+/* Copy out and drain the recorded spikes (unsorted).  *n_out receives the
+ * number written (<= cap).  Spikes not returned (beyond record_capacity or
+ * beyond cap) are discarded and counted in naer_stats.dropped.  Synchronises
+ * the net. */
+int naer_net_read_spikes(naer_net* net, naer_spike* out, uint64_t cap, uint64_t* n_out);
+
+/* Copy neuron state.  Any pointer may be NULL.  Arrays have n_neurons
+ * entries; POISSON neurons report v = 0 and refractory = 0.  pre/post are the
+ * STDP traces (zeros when the net has no plastic connection). */
+int naer_net_read_state(naer_net* net, float* v, int32_t* refractory, float* pre_trace,
+                        float* post_trace);
+
+/* Copy the CSR (after build).  row_ptr has n_neurons+1 entries, col and
+ * weight n_synapses entries; any pointer may be NULL. */
+int naer_net_read_csr(naer_net* net, uint64_t* row_ptr, uint32_t* col, float* weight);
+
+/* Record the synaptic input I_syn (mV, as used by the update) of every neuron
+ * for steps [t_begin, t_begin + n_steps) into a device buffer; read it with
+ * naer_net_read_probe (n_steps * n_neurons floats, step-major). */
+int naer_net_set_probe(naer_net* net, uint64_t t_begin, uint32_t n_steps);
+int naer_net_read_probe(naer_net* net, float* out);
+
+int naer_net_destroy(naer_net* net);
+
+/* ---- generators exposed for parity checks ----------------------------------
+ * Run on the device with the same device functions the simulation uses. */
+
+/* Philox4x32-10 blocks: out[4*i..4*i+3] = philox(ctr[4*i..], key[2*i..]). */
+int naer_philox(naer_ctx* ctx, const uint32_t* ctr, const uint32_t* key, uint32_t* out,
+                uint64_t n);
+
+/* The reference's Poisson train of one 1500 us epoch
+ * (simulate_poisson_neurons, input_neuron_group.cl:6-45) for POISSON neurons
+ * idx in [0, n): times[10*idx + m], count[idx] (<= 10). */
+int naer_ref_poisson_epoch(naer_ctx* ctx, uint32_t n, uint32_t n_pois_total, uint32_t seed,
+                           float rate_hz, uint32_t t0, uint32_t t1, uint32_t* times,
+                           uint32_t* count);
+
+/* Message of the last failure on the calling thread ("" if none). */
+const char* naer_last_error(void);
+
+/* Library version string. */
+const char* naer_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NAER_H */

@@ -1,0 +1,678 @@
+// kernels.hip — gfx950 kernels of the simulation step and the connectivity
+// build.  Host launch wrappers at the bottom (declared in naer_kernels.hpp).
+//
+// Reference kernels these replace (include/naer/kernels/ of the reference):
+//   k_update     <- simulate_nop_neurons (nop.cl:7-90), simulate_stdp_neurons'
+//                   neuron part (stdp.cl:32-140), neuron_dynamics (lif.cl:15-34)
+//                   and simulate_poisson_neurons (input_neuron_group.cl:6-45);
+//                   its ballot compaction replaces histogram_scan/scatter
+//                   (spike_aggregation.cl:3-65): with a fixed step there is no
+//                   time sort, only a per-step spike list.
+//   k_scatter    <- the per-spike synapse segment walk of nop.cl:51-82 /
+//                   stdp.cl:86-105 plus distribute_offsets
+//                   (spike_aggregation.cl:20-32): the CSR row of a source is
+//                   row_ptr[s]..row_ptr[s+1], no per-group offset table.
+//   k_potentiate <- lazy LTP of update_synapse (stdp.cl:19-23), here eager at
+//                   the post-synaptic spike.
+//   k_conn_*     <- count_connections / integrate_connections /
+//                   generate_connections (network.cl:14-107).
+#include "naer_device.hpp"
+#include "naer_kernels.hpp"
+
+namespace naer {
+
+namespace {
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ long long to_fixed(float w)
+{
+    return __double2ll_rn((double)w * 4294967296.0);
+}
+
+__device__ __forceinline__ float from_fixed(long long a) { return (float)((double)a * 0x1p-32); }
+
+__device__ __forceinline__ float clip(float w, float lo, float hi)
+{
+    return w < lo ? lo : (w > hi ? hi : w);
+}
+
+__device__ __forceinline__ uint32_t find_pop(const UpdateArgs& a, uint32_t j)
+{
+    uint32_t p = 0;
+    for (uint32_t q = 1; q < a.n_pops; ++q)
+        if (j >= a.pops[q].begin)
+            p = q;
+    return p;
+}
+
+// Block-wide spike staging: wave ballots reserve LDS slots, one global
+// reservation per block at the end.
+struct Stage {
+    naer_spike* list;  // LDS
+    uint32_t* n;       // LDS counter
+};
+
+__device__ __forceinline__ void put_global(const UpdateArgs& a, naer_spike* glist,
+                                           unsigned long long spikes_before, uint32_t g,
+                                           const naer_spike& sp)
+{
+    if (g < a.list_cap)
+        glist[g] = sp;
+    else
+        atomicAdd(&a.st->overflow, 1u);
+    const unsigned long long r = spikes_before + g - a.st->rec_base;
+    if (r < a.rec_cap)
+        a.rec[r] = sp;
+}
+
+// Ballot compaction of the lanes with `fire` set into the LDS stage.
+__device__ __forceinline__ void emit(const UpdateArgs& a, Stage& sg, naer_spike* glist,
+                                     uint32_t* gcount, unsigned long long spikes_before, bool fire,
+                                     uint32_t neuron, uint32_t step, uint32_t time_us)
+{
+    const unsigned long long mask = __ballot(fire);
+    if (mask == 0)
+        return;
+    const uint32_t leader = __ffsll((long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane_id() == leader)
+        base = atomicAdd(sg.n, (uint32_t)__popcll(mask));
+    base = __builtin_amdgcn_readlane(base, leader);
+    if (fire) {
+        const uint32_t lt = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+        const uint32_t slot = base + lt;
+        naer_spike sp;
+        sp.step = step;
+        sp.neuron = neuron;
+        sp.time_us = time_us;
+        sp._pad = 0;
+        if (slot < kLdsSpikes) {
+            sg.list[slot] = sp;
+        } else {
+            const uint32_t g = atomicAdd(gcount, 1u);
+            put_global(a, glist, spikes_before, g, sp);
+        }
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Update: one thread per neuron, B consecutive steps per launch.  The ring
+// slots of all B steps are loaded up front (they cannot receive anything
+// during the batch because B <= the minimum delay), so each thread has B
+// independent loads in flight.
+// ---------------------------------------------------------------------------
+template <bool DET, bool PLASTIC>
+__global__ __launch_bounds__(kBlock) void k_update(const UpdateArgs a)
+{
+    using RingT = typename std::conditional<DET, long long, float>::type;
+    __shared__ naer_spike s_list[kLdsSpikes];
+    __shared__ uint32_t s_n;
+    __shared__ unsigned long long s_events;
+    __shared__ uint32_t s_base;
+    if (threadIdx.x == 0) {
+        s_n = 0;
+        s_events = 0;
+    }
+    __syncthreads();
+
+    DevState* st = a.st;
+    const unsigned long long t0 = st->clock;
+    const uint32_t pl = st->batch & 1u;
+    const unsigned long long spikes_before = st->spikes;
+    naer_spike* glist = pl ? a.list1 : a.list0;
+    uint32_t* gcount = &st->count[pl];
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        st->cur = pl;
+    Stage sg{ s_list, &s_n };
+
+    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    unsigned long long my_events = 0;
+    if (j < a.N) {
+        const uint32_t p = find_pop(a, j);
+        const PopParams& pp = a.pops[p];
+        if (pp.kind == NAER_POP_LIF) {
+            RingT* ring = (RingT*)a.ring;
+            uint32_t slot0 = (uint32_t)(t0 % a.D);
+            RingT in[kMaxBatch];
+#pragma unroll
+            for (uint32_t s = 0; s < kMaxBatch; ++s) {
+                if (s < a.B) {
+                    uint32_t sl = slot0 + s;
+                    sl = sl >= a.D ? sl - a.D : sl;
+                    in[s] = ring[(size_t)sl * a.N + j];
+                }
+            }
+#pragma unroll
+            for (uint32_t s = 0; s < kMaxBatch; ++s) {
+                if (s < a.B) {
+                    uint32_t sl = slot0 + s;
+                    sl = sl >= a.D ? sl - a.D : sl;
+                    ring[(size_t)sl * a.N + j] = (RingT)0;
+                }
+            }
+            float V = a.V[j];
+            int32_t ref = a.ref[j];
+            int32_t last = a.last[j];
+            float xh = 0.0f, yh = 0.0f;
+            if (PLASTIC) {
+                xh = a.xh[j];
+                yh = a.yh[j];
+            }
+            const float* cdf = a.cdf + pp.cdf_offset;
+            const uint32_t s_lo = (uint32_t)a.seed, s_hi = (uint32_t)(a.seed >> 32);
+            U4 rnd{ 0, 0, 0, 0 };
+            uint32_t rnd_blk = 0xFFFFFFFFu;
+#pragma nounroll
+            for (uint32_t s = 0; s < a.B; ++s) {
+                const unsigned long long t = t0 + s;
+                RingT cur = in[0];
+#pragma unroll
+                for (uint32_t q = 0; q + 1 < kMaxBatch; ++q)
+                    in[q] = in[q + 1];
+                float I;
+                if (DET)
+                    I = from_fixed((long long)cur);
+                else
+                    I = (float)cur;
+                if (a.probe && t - a.probe_t0 < a.probe_n)
+                    a.probe[(size_t)(t - a.probe_t0) * a.N + j] = I;
+                if (PLASTIC) {
+                    const float sp = ((long long)last == (long long)t - 1) ? 1.0f : 0.0f;
+                    const float xa = xh + sp;
+                    xh = xa * a.dec_p;
+                    const float ya = yh + sp;
+                    yh = ya * a.dec_m;
+                }
+                bool fired = false;
+                if (ref > 0) {
+                    --ref;
+                } else {
+                    const float x = V - pp.v_rest;
+                    const float xp = x * pp.P;
+                    float v = xp + pp.v_rest;
+                    v = v + I;
+                    if (pp.has_drive) {
+                        const uint32_t blk = (uint32_t)(t >> 2);
+                        if (blk != rnd_blk) {
+                            rnd = philox(U4{ j, blk, NAER_STREAM_DRIVE, 0u }, s_lo, s_hi);
+                            rnd_blk = blk;
+                        }
+                        const float u = unit24(word_of(rnd, (uint32_t)t & 3u));
+                        int k = 0;
+                        while (u >= cdf[k])
+                            ++k;
+                        const float e = (float)k * pp.w_ext;
+                        v = v + e;
+                    }
+                    fired = v > pp.v_th;
+                    V = fired ? pp.v_reset : v;
+                    if (fired) {
+                        ref = pp.n_ref;
+                        last = (int32_t)t;
+                        my_events += a.row_ptr[j + 1] - a.row_ptr[j];
+                    }
+                }
+                emit(a, sg, glist, gcount, spikes_before, fired, j, (uint32_t)t,
+                     (uint32_t)t * a.dt_us);
+            }
+            a.V[j] = V;
+            a.ref[j] = ref;
+            a.last[j] = last;
+            if (PLASTIC) {
+                a.xh[j] = xh;
+                a.yh[j] = yh;
+            }
+        } else {
+            // POISSON source: the reference train (input_neuron_group.cl:26-39)
+            // evaluated window by window; state = next spike time and the
+            // number already emitted in the current 1500 us epoch.
+            uint32_t nxt = a.pnext[j], cnt = a.pcnt[j];
+            const uint32_t idx = pp.pois_base + (j - pp.begin);
+            const uint32_t gid = ref_poisson_gid(idx);
+#pragma nounroll
+            for (uint32_t s = 0; s < a.B; ++s) {
+                const unsigned long long t = t0 + s;
+                const uint32_t T = (uint32_t)t * a.dt_us;
+                if (T % kEpochUs == 0) {
+                    nxt = T + ref_isi_us(pp.neg_rate_per_us, gid, a.pois_seed, T * a.n_pois + idx);
+                    cnt = 0;
+                }
+                const uint32_t end = T + a.dt_us;
+                bool pend = cnt < kMaxPoissonPerEpoch && nxt < end;
+                while (__any(pend)) {
+                    emit(a, sg, glist, gcount, spikes_before, pend, j, (uint32_t)t, nxt);
+                    if (pend) {
+                        my_events += a.row_ptr[j + 1] - a.row_ptr[j];
+                        ++cnt;
+                        nxt = nxt + ref_isi_us(pp.neg_rate_per_us, gid, a.pois_seed, nxt * a.n_pois + idx);
+                        pend = cnt < kMaxPoissonPerEpoch && nxt < end;
+                    }
+                }
+            }
+            a.pnext[j] = nxt;
+            a.pcnt[j] = cnt;
+        }
+    }
+    if (my_events)
+        atomicAdd(&s_events, my_events);
+    __syncthreads();
+    const uint32_t n = s_n < kLdsSpikes ? s_n : kLdsSpikes;
+    if (threadIdx.x == 0) {
+        s_base = n ? atomicAdd(gcount, n) : 0u;
+        if (s_events)
+            atomicAdd(&st->events, s_events);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += kBlock)
+        put_global(a, glist, spikes_before, s_base + i, s_list[i]);
+}
+
+// ---------------------------------------------------------------------------
+// Scatter: grid (x: waves striding over the batch's spikes, y: 256-synapse
+// chunk of the row).  Each lane loads 4 synapses before issuing its 4
+// accumulations.  The last kernel of a batch advances the device clock.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void batch_epilogue(DevState* st, uint32_t cur, uint32_t M, uint32_t B)
+{
+    st->spikes += M;
+    st->count[cur ^ 1u] = 0;
+    st->clock += B;
+    st->batch += 1;
+}
+
+template <bool DET, bool PLASTIC>
+__global__ __launch_bounds__(kBlock) void k_scatter(const ScatterArgs a)
+{
+    using RingT = typename std::conditional<DET, unsigned long long, float>::type;
+    DevState* st = a.st;
+    const uint32_t cur = st->cur;
+    const uint32_t M = st->count[cur];
+    const naer_spike* list = cur ? a.list1 : a.list0;
+    if (a.is_last && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0)
+        batch_epilogue(st, cur, M, a.B);
+
+    RingT* ring = (RingT*)a.ring;
+    const uint32_t lane = lane_id();
+    const uint32_t wave =
+        blockIdx.x * (kBlock / kWave) + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const uint32_t nw = gridDim.x * (kBlock / kWave);
+    const unsigned long long chunk_off = (unsigned long long)blockIdx.y * kChunk;
+    for (uint32_t i = wave; i < M; i += nw) {
+        const naer_spike sp = list[i];
+        const uint32_t s = sp.neuron;
+        const unsigned long long re = a.row_ptr[s + 1];
+        const unsigned long long beg = a.row_ptr[s] + chunk_off;
+        if (beg >= re)
+            continue;
+        const uint32_t tD = sp.step % a.D;
+        uint32_t c[kChunkUnroll];
+        float wv[kChunkUnroll];
+#pragma unroll
+        for (uint32_t u = 0; u < kChunkUnroll; ++u) {
+            const unsigned long long q = beg + u * kWave + lane;
+            c[u] = q < re ? a.col[q] : 0u;
+            wv[u] = q < re ? a.w[q] : 0.0f;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kChunkUnroll; ++u) {
+            const unsigned long long q = beg + u * kWave + lane;
+            if (q < re) {
+                const uint32_t tgt = c[u] & kTargetMask;
+                uint32_t sl = tD + ((c[u] >> 24) & 0x7Fu);
+                sl = sl >= a.D ? sl - a.D : sl;
+                float x = wv[u];
+                if (PLASTIC && (c[u] & kPlasticBit)) {
+                    const float dw = a.a_minus * a.yh[tgt];
+                    x = clip(x - dw, a.w_min, a.w_max);
+                    a.w[q] = x;
+                }
+                RingT* dst = ring + (size_t)sl * a.N + tgt;
+                if (DET)
+                    atomicAdd((unsigned long long*)dst, (unsigned long long)to_fixed(x));
+                else
+                    unsafeAtomicAdd((float*)dst, x);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Potentiation at post-synaptic spikes through the reverse index; always the
+// last kernel of a plastic batch.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_potentiate(const PotentiateArgs a)
+{
+    DevState* st = a.st;
+    const uint32_t cur = st->cur;
+    const uint32_t M = st->count[cur];
+    const naer_spike* list = cur ? a.list1 : a.list0;
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        batch_epilogue(st, cur, M, a.B);
+    const uint32_t lane = lane_id();
+    const uint32_t wave =
+        blockIdx.x * (kBlock / kWave) + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const uint32_t nw = gridDim.x * (kBlock / kWave);
+    for (uint32_t i = wave; i < M; i += nw) {
+        const uint32_t post = list[i].neuron;
+        const unsigned long long rb = a.rev_ptr[post], re = a.rev_ptr[post + 1];
+        for (unsigned long long r = rb + lane; r < re; r += kWave) {
+            const uint32_t pos = a.rev_pos[r];
+            const float dw = a.a_plus * a.xh[a.rev_src[r]];
+            a.w[pos] = clip(a.w[pos] + dw, a.w_min, a.w_max);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Connectivity build
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float draw_weight(const ConnArgs& c, uint32_t word)
+{
+    if (c.weight != 0.0f)
+        return c.weight;
+    const float span = c.wmax - c.wmin;
+    const float x = span * unit24(word);
+    return c.wmin + x;
+}
+
+template <bool PLACE>
+__global__ __launch_bounds__(kBlock) void k_conn_fixed(const ConnArgs c, uint32_t* deg,
+                                                       unsigned long long* cursor, uint32_t* col,
+                                                       float* w)
+{
+    const uint32_t k0 = (uint32_t)c.seed, k1 = (uint32_t)(c.seed >> 32);
+    for (uint32_t j = blockIdx.x; j < c.n_dst; j += gridDim.x) {
+        for (uint32_t k = threadIdx.x; k < c.K; k += kBlock) {
+            const U4 r = philox(U4{ j, k, NAER_STREAM_CONN, c.conn_index }, k0, k1);
+            const uint32_t src = c.src_begin + below(r.x, c.n_src);
+            if (!PLACE) {
+                atomicAdd(&deg[src], 1u);
+            } else {
+                const uint32_t del = c.dmin + c.dq * below(r.y, c.dchoices);
+                const unsigned long long pos = atomicAdd(&cursor[src], 1ull);
+                col[pos] = (c.dst_begin + j) | (del << 24) | c.plastic_bit;
+                w[pos] = draw_weight(c, r.z);
+            }
+        }
+    }
+}
+
+template <bool PLACE>
+__global__ __launch_bounds__(kBlock) void k_conn_bernoulli(const ConnArgs c, uint32_t* deg,
+                                                           unsigned long long* cursor,
+                                                           uint32_t* col, float* w)
+{
+    const uint32_t seed32 = (uint32_t)c.seed;
+    for (uint32_t s = blockIdx.x; s < c.n_src; s += gridDim.x) {
+        uint32_t cnt = 0;
+        for (uint32_t j = threadIdx.x; j < c.n_dst; j += kBlock) {
+            const U4 r = philox(U4{ j, 0u, 0u, 0u }, s * 32u + j % 32u, seed32);
+            if (!((double)r.z / 4294967295.0 < c.p))
+                continue;
+            if (!PLACE) {
+                ++cnt;
+            } else {
+                const uint32_t del = c.dmin + c.dq * below(r.y, c.dchoices);
+                const unsigned long long pos = atomicAdd(&cursor[c.src_begin + s], 1ull);
+                col[pos] = (c.dst_begin + j) | (del << 24) | c.plastic_bit;
+                w[pos] = draw_weight(c, r.w);
+            }
+        }
+        if (!PLACE && cnt)
+            atomicAdd(&deg[c.src_begin + s], cnt);
+    }
+}
+
+// Canonical row order: ascending (word << 32 | weight bits), bitonic sort of
+// one row per block in LDS.
+__global__ __launch_bounds__(512) void k_sort_rows(const unsigned long long* row_ptr, uint32_t* col,
+                                                   float* w, uint32_t N)
+{
+    extern __shared__ unsigned long long s_keys[];
+    for (uint32_t r = blockIdx.x; r < N; r += gridDim.x) {
+        const unsigned long long beg = row_ptr[r];
+        const uint32_t len = (uint32_t)(row_ptr[r + 1] - beg);
+        if (len <= 1)
+            continue;
+        uint32_t L = 1;
+        while (L < len)
+            L <<= 1;
+        for (uint32_t i = threadIdx.x; i < L; i += blockDim.x)
+            s_keys[i] = i < len ? ((unsigned long long)col[beg + i] << 32) |
+                                      (unsigned long long)__float_as_uint(w[beg + i])
+                                : ~0ull;
+        __syncthreads();
+        for (uint32_t k = 2; k <= L; k <<= 1) {
+            for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
+                for (uint32_t i = threadIdx.x; i < L; i += blockDim.x) {
+                    const uint32_t ixj = i ^ jj;
+                    if (ixj > i) {
+                        const unsigned long long x = s_keys[i], y = s_keys[ixj];
+                        const bool up = (i & k) == 0;
+                        if ((x > y) == up) {
+                            s_keys[i] = y;
+                            s_keys[ixj] = x;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) {
+            col[beg + i] = (uint32_t)(s_keys[i] >> 32);
+            w[beg + i] = __uint_as_float((uint32_t)s_keys[i]);
+        }
+        __syncthreads();
+    }
+}
+
+template <bool PLACE>
+__global__ __launch_bounds__(kBlock) void k_reverse(const unsigned long long* row_ptr,
+                                                    const uint32_t* col, uint32_t N, uint32_t* rdeg,
+                                                    unsigned long long* rcursor, uint32_t* rev_pos,
+                                                    uint32_t* rev_src)
+{
+    for (uint32_t s = blockIdx.x; s < N; s += gridDim.x) {
+        const unsigned long long re = row_ptr[s + 1];
+        for (unsigned long long i = row_ptr[s] + threadIdx.x; i < re; i += kBlock) {
+            const uint32_t c = col[i];
+            if (!(c & kPlasticBit))
+                continue;
+            const uint32_t tgt = c & kTargetMask;
+            if (!PLACE) {
+                atomicAdd(&rdeg[tgt], 1u);
+            } else {
+                const unsigned long long r = atomicAdd(&rcursor[tgt], 1ull);
+                rev_pos[r] = (uint32_t)i;
+                rev_src[r] = s;
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_init_state(const UpdateArgs a)
+{
+    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= a.N)
+        return;
+    const PopParams& pp = a.pops[find_pop(a, j)];
+    float v = 0.0f;
+    if (pp.kind == NAER_POP_LIF) {
+        const U4 r = philox(U4{ j, 0u, NAER_STREAM_V0, 0u }, (uint32_t)a.seed,
+                            (uint32_t)(a.seed >> 32));
+        const float span = pp.v0_max - pp.v0_min;
+        const float x = span * unit24(r.x);
+        v = pp.v0_min + x;
+    }
+    a.V[j] = v;
+    a.ref[j] = 0;
+    a.last[j] = -1;
+    if (a.xh) {
+        a.xh[j] = 0.0f;
+        a.yh[j] = 0.0f;
+    }
+    a.pnext[j] = 0;
+    a.pcnt[j] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Generators exposed for parity checks
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_philox(const uint32_t* ctr, const uint32_t* key,
+                                                   uint32_t* out, unsigned long long n)
+{
+    const unsigned long long i = (unsigned long long)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n)
+        return;
+    const U4 r = philox(U4{ ctr[4 * i], ctr[4 * i + 1], ctr[4 * i + 2], ctr[4 * i + 3] },
+                        key[2 * i], key[2 * i + 1]);
+    out[4 * i] = r.x;
+    out[4 * i + 1] = r.y;
+    out[4 * i + 2] = r.z;
+    out[4 * i + 3] = r.w;
+}
+
+__global__ __launch_bounds__(kBlock) void k_ref_poisson_epoch(uint32_t n, uint32_t n_pois,
+                                                              uint32_t seed, double neg_rate,
+                                                              uint32_t t0, uint32_t t1,
+                                                              uint32_t* times, uint32_t* count)
+{
+    const uint32_t idx = blockIdx.x * kBlock + threadIdx.x;
+    if (idx >= n)
+        return;
+    const uint32_t gid = ref_poisson_gid(idx);
+    uint32_t time = t0 + ref_isi_us(neg_rate, gid, seed, t0 * n_pois + idx);
+    uint32_t c = 0;
+    for (uint32_t m = 0; m < kMaxPoissonPerEpoch; ++m) {
+        if (time < t1) {
+            times[(size_t)kMaxPoissonPerEpoch * idx + c++] = time;
+            time += ref_isi_us(neg_rate, gid, seed, time * n_pois + idx);
+        }
+    }
+    count[idx] = c;
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+hipError_t launch_update(const UpdateArgs& a, bool det, bool plastic, hipStream_t s)
+{
+    const dim3 grid(cdiv(a.N, kBlock));
+    if (det) {
+        if (plastic)
+            hipLaunchKernelGGL((k_update<true, true>), grid, dim3(kBlock), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_update<true, false>), grid, dim3(kBlock), 0, s, a);
+    } else {
+        if (plastic)
+            hipLaunchKernelGGL((k_update<false, true>), grid, dim3(kBlock), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_update<false, false>), grid, dim3(kBlock), 0, s, a);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter(const ScatterArgs& a, bool det, bool plastic, uint32_t grid_x,
+                          uint32_t chunks, hipStream_t s)
+{
+    const dim3 grid(grid_x, chunks);
+    if (det) {
+        if (plastic)
+            hipLaunchKernelGGL((k_scatter<true, true>), grid, dim3(kBlock), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_scatter<true, false>), grid, dim3(kBlock), 0, s, a);
+    } else {
+        if (plastic)
+            hipLaunchKernelGGL((k_scatter<false, true>), grid, dim3(kBlock), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_scatter<false, false>), grid, dim3(kBlock), 0, s, a);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_potentiate(const PotentiateArgs& a, uint32_t grid_x, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_potentiate, dim3(grid_x), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_conn(const ConnArgs& c, uint32_t rule, bool place, uint32_t* deg,
+                       unsigned long long* cursor, uint32_t* col, float* w, hipStream_t s)
+{
+    if (rule == NAER_RULE_FIXED_INDEGREE) {
+        const dim3 grid(c.n_dst < 65536u ? (c.n_dst ? c.n_dst : 1u) : 65536u);
+        if (place)
+            hipLaunchKernelGGL((k_conn_fixed<true>), grid, dim3(kBlock), 0, s, c, deg, cursor, col, w);
+        else
+            hipLaunchKernelGGL((k_conn_fixed<false>), grid, dim3(kBlock), 0, s, c, deg, cursor, col, w);
+    } else {
+        const dim3 grid(c.n_src < 65536u ? (c.n_src ? c.n_src : 1u) : 65536u);
+        if (place)
+            hipLaunchKernelGGL((k_conn_bernoulli<true>), grid, dim3(kBlock), 0, s, c, deg, cursor, col, w);
+        else
+            hipLaunchKernelGGL((k_conn_bernoulli<false>), grid, dim3(kBlock), 0, s, c, deg, cursor, col, w);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_sort_rows(const unsigned long long* row_ptr, uint32_t* col, float* w, uint32_t N,
+                            uint32_t max_row, hipStream_t s)
+{
+    uint32_t L = 1;
+    while (L < max_row)
+        L <<= 1;
+    const size_t lds = (size_t)L * sizeof(unsigned long long);
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_sort_rows,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess)
+            return e;
+    }
+    const dim3 grid(N < 65536u ? N : 65536u);
+    hipLaunchKernelGGL(k_sort_rows, grid, dim3(512), lds, s, row_ptr, col, w, N);
+    return hipGetLastError();
+}
+
+hipError_t launch_reverse(bool place, const unsigned long long* row_ptr, const uint32_t* col,
+                          uint32_t N, uint32_t* rdeg, unsigned long long* rcursor, uint32_t* rev_pos,
+                          uint32_t* rev_src, hipStream_t s)
+{
+    const dim3 grid(N < 65536u ? N : 65536u);
+    if (place)
+        hipLaunchKernelGGL((k_reverse<true>), grid, dim3(kBlock), 0, s, row_ptr, col, N, rdeg,
+                           rcursor, rev_pos, rev_src);
+    else
+        hipLaunchKernelGGL((k_reverse<false>), grid, dim3(kBlock), 0, s, row_ptr, col, N, rdeg,
+                           rcursor, rev_pos, rev_src);
+    return hipGetLastError();
+}
+
+hipError_t launch_init_state(const UpdateArgs& a, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_init_state, dim3(cdiv(a.N, kBlock)), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out,
+                         unsigned long long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_philox, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, s, ctr, key, out, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_ref_poisson_epoch(uint32_t n, uint32_t n_pois, uint32_t seed, double neg_rate,
+                                    uint32_t t0, uint32_t t1, uint32_t* times, uint32_t* count,
+                                    hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ref_poisson_epoch, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, s, n, n_pois,
+                       seed, neg_rate, t0, t1, times, count);
+    return hipGetLastError();
+}
+
+}  // namespace naer

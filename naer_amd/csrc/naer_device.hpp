@@ -1,0 +1,192 @@
+// naer_device.hpp — device-side building blocks and the kernel argument
+// records shared by the kernels (kernels.hip) and the host runtime
+// (naer_runtime.hip).  gfx950 only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/naer/naer.h"
+
+namespace naer {
+
+constexpr uint32_t kMaxPops = 16;      // populations per net passed by value
+constexpr uint32_t kMaxBatch = 16;     // steps per update launch
+constexpr uint32_t kBlock = 256;       // threads per block of the step kernels
+constexpr uint32_t kWave = 64;         // CDNA wavefront
+constexpr uint32_t kLdsSpikes = 512;   // per-block spike staging in LDS
+constexpr uint32_t kChunkUnroll = 4;   // synapses per lane per scatter task
+constexpr uint32_t kChunk = kWave * kChunkUnroll;
+constexpr uint32_t kTargetMask = 0x00FFFFFFu;
+constexpr uint32_t kPlasticBit = 0x80000000u;
+constexpr uint32_t kEpochUs = 1500u;   // reference epoch, naer_types.h:4
+constexpr uint32_t kMaxPoissonPerEpoch = 10u;  // input_neuron_group.cl:30
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11), the generator behind the reference's
+// uniform_rand (random.cl:8-16).  Round: (x0,x1,x2,x3) ->
+// (hi(M1 x2) ^ x1 ^ k0, lo(M1 x2), hi(M0 x0) ^ x3 ^ k1, lo(M0 x0)).
+// ---------------------------------------------------------------------------
+struct U4 {
+    uint32_t x, y, z, w;
+};
+
+__host__ __device__ __forceinline__ uint32_t mulhi32(uint32_t a, uint32_t b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umulhi(a, b);
+#else
+    return (uint32_t)(((uint64_t)a * b) >> 32);
+#endif
+}
+
+__host__ __device__ __forceinline__ U4 philox(U4 c, uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t h0 = mulhi32(0xD2511F53u, c.x), l0 = 0xD2511F53u * c.x;
+        const uint32_t h1 = mulhi32(0xCD9E8D57u, c.z), l1 = 0xCD9E8D57u * c.z;
+        c = U4{ h1 ^ c.y ^ k0, l1, h0 ^ c.w ^ k1, l0 };
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return c;
+}
+
+__host__ __device__ __forceinline__ uint32_t word_of(const U4& r, uint32_t i)
+{
+    return i == 0 ? r.x : (i == 1 ? r.y : (i == 2 ? r.z : r.w));
+}
+
+// fp32 uniform on the 2^-24 grid in [0,1)
+__host__ __device__ __forceinline__ float unit24(uint32_t w) { return (float)(w >> 8) * 0x1p-24f; }
+// index in [0,n) by multiply-shift
+__host__ __device__ __forceinline__ uint32_t below(uint32_t w, uint32_t n) { return mulhi32(w, n); }
+
+// (uint)exprnd_fast(rate, seed, iteration), random.cl:6-44 with the cast of
+// input_neuron_group.cl:29,37: third Philox word over UINT_MAX in fp64,
+// log(1-u) / -(rate/1e6), truncated; out-of-range values saturate.
+__device__ __forceinline__ uint32_t ref_isi_us(double neg_rate_per_us, uint32_t gid, uint32_t seed,
+                                               uint32_t iteration)
+{
+    const U4 r = philox(U4{ iteration, 0u, 0u, 0u }, gid, seed);
+    const double u = (double)r.z / 4294967295.0;
+    const double isi = log(1.0 - u) / neg_rate_per_us;
+    return isi < 4294967296.0 ? (uint32_t)isi : 0xFFFFFFFFu;
+}
+
+// get_global_id(0) of POISSON neuron idx under the reference launch shape
+// (128 work-items per 2048 neurons, Network.cpp:181-184).
+__host__ __device__ __forceinline__ uint32_t ref_poisson_gid(uint32_t idx)
+{
+    return (idx / 2048u) * 128u + idx % 128u;
+}
+
+// ---------------------------------------------------------------------------
+// Records in device memory
+// ---------------------------------------------------------------------------
+struct PopParams {
+    uint32_t begin, end;  // neuron range [begin, end)
+    uint32_t kind;        // NAER_POP_*
+    uint32_t pois_base;   // POISSON: index of `begin` among all POISSON neurons
+    float P;              // exp(-dt/tau_m)
+    float v_th, v_reset, v_rest;
+    float v0_min, v0_max;  // initial potential range
+    float w_ext;          // drive jump
+    int32_t n_ref;        // refractory steps
+    uint32_t has_drive;
+    double neg_rate_per_us;  // POISSON: -(rate/1e6)
+    uint32_t cdf_offset;  // drive CDF table offset (floats)
+    uint32_t _pad;
+};
+
+// Step bookkeeping that lives on the device so that launches (and captured
+// graphs) need no per-step host arguments.
+struct DevState {
+    unsigned long long clock;   // first step of the next batch
+    unsigned long long spikes;  // spikes emitted so far
+    unsigned long long events;  // synaptic events so far
+    unsigned long long rec_base;  // spike index where the record buffer starts
+    uint32_t batch;             // batches done; the update writes list[batch & 1]
+    uint32_t cur;               // list of the batch in flight
+    uint32_t count[2];          // spikes in list[0], list[1]
+    uint32_t overflow;          // spikes lost to a full batch list
+    uint32_t _pad;
+};
+
+struct UpdateArgs {
+    float* V;
+    int32_t* ref;
+    int32_t* last;
+    float* xh;
+    float* yh;
+    uint32_t* pnext;
+    uint32_t* pcnt;
+    void* ring;
+    const unsigned long long* row_ptr;
+    naer_spike* list0;
+    naer_spike* list1;
+    naer_spike* rec;
+    DevState* st;
+    const float* cdf;
+    float* probe;
+    unsigned long long rec_cap;
+    unsigned long long probe_t0;
+    unsigned long long seed;
+    uint32_t probe_n;
+    uint32_t list_cap;
+    uint32_t N, D;
+    uint32_t B;
+    uint32_t dt_us;
+    uint32_t n_pois;
+    uint32_t pois_seed;
+    float dec_p, dec_m;
+    uint32_t n_pops;
+    uint32_t _pad;
+    PopParams pops[kMaxPops];
+};
+
+struct ScatterArgs {
+    const naer_spike* list0;
+    const naer_spike* list1;
+    DevState* st;
+    const unsigned long long* row_ptr;
+    const uint32_t* col;
+    float* w;
+    void* ring;
+    const float* yh;
+    float a_minus, w_min, w_max;
+    uint32_t N, D;
+    uint32_t B;
+    uint32_t is_last;
+    uint32_t _pad;
+};
+
+struct PotentiateArgs {
+    const naer_spike* list0;
+    const naer_spike* list1;
+    DevState* st;
+    const unsigned long long* rev_ptr;
+    const uint32_t* rev_pos;
+    const uint32_t* rev_src;
+    float* w;
+    const float* xh;
+    float a_plus, w_min, w_max;
+    uint32_t B;
+};
+
+// One connection of the device connectivity build.
+struct ConnArgs {
+    uint32_t src_begin, n_src;
+    uint32_t dst_begin, n_dst;
+    uint32_t K;            // FIXED_INDEGREE in-degree
+    uint32_t conn_index;   // position in the description (stream word)
+    unsigned long long seed;
+    double p;              // BERNOULLI probability
+    float weight, wmin, wmax;
+    uint32_t dmin, dq, dchoices;
+    uint32_t plastic_bit;
+    uint32_t _pad;
+};
+
+}  // namespace naer

@@ -1,0 +1,29 @@
+// naer_kernels.hpp — host launch wrappers of the kernels in kernels.hip.
+#pragma once
+
+#include "naer_device.hpp"
+
+namespace naer {
+
+hipError_t launch_update(const UpdateArgs& a, bool det, bool plastic, hipStream_t s);
+hipError_t launch_scatter(const ScatterArgs& a, bool det, bool plastic, uint32_t grid_x,
+                          uint32_t chunks, hipStream_t s);
+hipError_t launch_potentiate(const PotentiateArgs& a, uint32_t grid_x, hipStream_t s);
+hipError_t launch_conn(const ConnArgs& c, uint32_t rule, bool place, uint32_t* deg,
+                       unsigned long long* cursor, uint32_t* col, float* w, hipStream_t s);
+hipError_t launch_sort_rows(const unsigned long long* row_ptr, uint32_t* col, float* w, uint32_t N,
+                            uint32_t max_row, hipStream_t s);
+hipError_t launch_reverse(bool place, const unsigned long long* row_ptr, const uint32_t* col,
+                          uint32_t N, uint32_t* rdeg, unsigned long long* rcursor, uint32_t* rev_pos,
+                          uint32_t* rev_src, hipStream_t s);
+hipError_t launch_init_state(const UpdateArgs& a, hipStream_t s);
+hipError_t launch_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out,
+                         unsigned long long n, hipStream_t s);
+hipError_t launch_ref_poisson_epoch(uint32_t n, uint32_t n_pois, uint32_t seed, double neg_rate,
+                                    uint32_t t0, uint32_t t1, uint32_t* times, uint32_t* count,
+                                    hipStream_t s);
+
+// Longest CSR row the LDS row sort accepts.
+constexpr uint32_t kMaxSortedRow = 8192;
+
+}  // namespace naer
