@@ -71,7 +71,7 @@ def build_driver():
         return None
     os.makedirs(BIN_DIR, exist_ok=True)
     out = os.path.join(BIN_DIR, "naer")
-    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), *srcs,
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), "-I", os.path.join(ROOT, "include", "naer"), *srcs,
           "-L", LIB_DIR, "-lnaer", "-Wl,-rpath,$ORIGIN/../lib", "-o", out])
     return out
 
