@@ -46,30 +46,20 @@ __device__ __forceinline__ uint32_t find_pop(const UpdateArgs& a, uint32_t j)
     return p;
 }
 
-// Block-wide spike staging: wave ballots reserve LDS slots, one global
-// reservation per block at the end.
+// Block-wide spike staging: wave ballots reserve slots of the block's list
+// segment through an LDS counter; the first kLdsSpikes entries are staged in
+// LDS and copied out coalesced at the end, the rest go straight to the
+// segment.  No global atomics.
 struct Stage {
     naer_spike* list;  // LDS
     uint32_t* n;       // LDS counter
+    naer_spike* seg;   // this block's global list segment
+    uint32_t seg_cap;
 };
 
-__device__ __forceinline__ void put_global(const UpdateArgs& a, naer_spike* glist,
-                                           unsigned long long spikes_before, uint32_t g,
-                                           const naer_spike& sp)
-{
-    if (g < a.list_cap)
-        glist[g] = sp;
-    else
-        atomicAdd(&a.st->overflow, 1u);
-    const unsigned long long r = spikes_before + g - a.st->rec_base;
-    if (r < a.rec_cap)
-        a.rec[r] = sp;
-}
-
-// Ballot compaction of the lanes with `fire` set into the LDS stage.
-__device__ __forceinline__ void emit(const UpdateArgs& a, Stage& sg, naer_spike* glist,
-                                     uint32_t* gcount, unsigned long long spikes_before, bool fire,
-                                     uint32_t neuron, uint32_t step, uint32_t time_us)
+// Ballot compaction of the lanes with `fire` set.
+__device__ __forceinline__ void emit(Stage& sg, bool fire, uint32_t neuron, uint32_t step,
+                                     uint32_t time_us)
 {
     const unsigned long long mask = __ballot(fire);
     if (mask == 0)
@@ -88,12 +78,10 @@ __device__ __forceinline__ void emit(const UpdateArgs& a, Stage& sg, naer_spike*
         sp.neuron = neuron;
         sp.time_us = time_us;
         sp._pad = 0;
-        if (slot < kLdsSpikes) {
+        if (slot < kLdsSpikes)
             sg.list[slot] = sp;
-        } else {
-            const uint32_t g = atomicAdd(gcount, 1u);
-            put_global(a, glist, spikes_before, g, sp);
-        }
+        else if (slot < sg.seg_cap)
+            sg.seg[slot] = sp;
     }
 }
 
@@ -110,24 +98,25 @@ __global__ __launch_bounds__(kBlock) void k_update(const UpdateArgs a)
 {
     using RingT = typename std::conditional<DET, long long, float>::type;
     __shared__ naer_spike s_list[kLdsSpikes];
+    __shared__ float s_cdf[kMaxPops * NAER_DRIVE_TABLE];
     __shared__ uint32_t s_n;
     __shared__ unsigned long long s_events;
-    __shared__ uint32_t s_base;
+    __shared__ unsigned long long s_base;
     if (threadIdx.x == 0) {
         s_n = 0;
         s_events = 0;
     }
+    for (uint32_t i = threadIdx.x; i < a.n_pops * NAER_DRIVE_TABLE; i += kBlock)
+        s_cdf[i] = a.cdf[i];
     __syncthreads();
 
     DevState* st = a.st;
     const unsigned long long t0 = st->clock;
     const uint32_t pl = st->batch & 1u;
-    const unsigned long long spikes_before = st->spikes;
-    naer_spike* glist = pl ? a.list1 : a.list0;
-    uint32_t* gcount = &st->count[pl];
+    naer_spike* seg = (pl ? a.list1 : a.list0) + (size_t)blockIdx.x * a.seg_cap;
     if (blockIdx.x == 0 && threadIdx.x == 0)
         st->cur = pl;
-    Stage sg{ s_list, &s_n };
+    Stage sg{ s_list, &s_n, seg, a.seg_cap };
 
     const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
     unsigned long long my_events = 0;
@@ -162,7 +151,12 @@ __global__ __launch_bounds__(kBlock) void k_update(const UpdateArgs a)
                 xh = a.xh[j];
                 yh = a.yh[j];
             }
-            const float* cdf = a.cdf + pp.cdf_offset;
+            // population constants in registers for the whole batch
+            const float P = pp.P, v_th = pp.v_th, v_reset = pp.v_reset, v_rest = pp.v_rest;
+            const float w_ext = pp.w_ext;
+            const int32_t n_ref = pp.n_ref;
+            const bool has_drive = pp.has_drive != 0;
+            const float* cdf = s_cdf + pp.cdf_offset;
             const uint32_t s_lo = (uint32_t)a.seed, s_hi = (uint32_t)(a.seed >> 32);
             U4 rnd{ 0, 0, 0, 0 };
             uint32_t rnd_blk = 0xFFFFFFFFu;
@@ -191,11 +185,11 @@ __global__ __launch_bounds__(kBlock) void k_update(const UpdateArgs a)
                 if (ref > 0) {
                     --ref;
                 } else {
-                    const float x = V - pp.v_rest;
-                    const float xp = x * pp.P;
-                    float v = xp + pp.v_rest;
+                    const float x = V - v_rest;
+                    const float xp = x * P;
+                    float v = xp + v_rest;
                     v = v + I;
-                    if (pp.has_drive) {
+                    if (has_drive) {
                         const uint32_t blk = (uint32_t)(t >> 2);
                         if (blk != rnd_blk) {
                             rnd = philox(U4{ j, blk, NAER_STREAM_DRIVE, 0u }, s_lo, s_hi);
@@ -205,19 +199,18 @@ __global__ __launch_bounds__(kBlock) void k_update(const UpdateArgs a)
                         int k = 0;
                         while (u >= cdf[k])
                             ++k;
-                        const float e = (float)k * pp.w_ext;
+                        const float e = (float)k * w_ext;
                         v = v + e;
                     }
-                    fired = v > pp.v_th;
-                    V = fired ? pp.v_reset : v;
+                    fired = v > v_th;
+                    V = fired ? v_reset : v;
                     if (fired) {
-                        ref = pp.n_ref;
+                        ref = n_ref;
                         last = (int32_t)t;
                         my_events += a.row_ptr[j + 1] - a.row_ptr[j];
                     }
                 }
-                emit(a, sg, glist, gcount, spikes_before, fired, j, (uint32_t)t,
-                     (uint32_t)t * a.dt_us);
+                emit(sg, fired, j, (uint32_t)t, (uint32_t)t * a.dt_us);
             }
             a.V[j] = V;
             a.ref[j] = ref;
@@ -244,7 +237,7 @@ __global__ __launch_bounds__(kBlock) void k_update(const UpdateArgs a)
                 const uint32_t end = T + a.dt_us;
                 bool pend = cnt < kMaxPoissonPerEpoch && nxt < end;
                 while (__any(pend)) {
-                    emit(a, sg, glist, gcount, spikes_before, pend, j, (uint32_t)t, nxt);
+                    emit(sg, pend, j, (uint32_t)t, nxt);
                     if (pend) {
                         my_events += a.row_ptr[j + 1] - a.row_ptr[j];
                         ++cnt;
@@ -260,28 +253,45 @@ __global__ __launch_bounds__(kBlock) void k_update(const UpdateArgs a)
     if (my_events)
         atomicAdd(&s_events, my_events);
     __syncthreads();
-    const uint32_t n = s_n < kLdsSpikes ? s_n : kLdsSpikes;
+    const uint32_t total = s_n;
+    const uint32_t n = total < a.seg_cap ? total : a.seg_cap;
+    const uint32_t staged = n < kLdsSpikes ? n : kLdsSpikes;
+    for (uint32_t i = threadIdx.x; i < staged; i += kBlock)
+        seg[i] = s_list[i];
     if (threadIdx.x == 0) {
-        s_base = n ? atomicAdd(gcount, n) : 0u;
-        if (s_events)
-            atomicAdd(&st->events, s_events);
+        a.seg_count[pl * a.n_seg + blockIdx.x] = n;
+        a.spk_part[blockIdx.x] += total;
+        a.ev_part[blockIdx.x] += s_events;
+        if (total > n)
+            atomicAdd(&st->overflow, total - n);
+        s_base = (a.rec_cap && n) ? atomicAdd(&st->rec_n, (unsigned long long)n) : 0ull;
     }
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < n; i += kBlock)
-        put_global(a, glist, spikes_before, s_base + i, s_list[i]);
+    if (a.rec_cap && n) {
+        // spikes beyond the LDS stage were written to the segment directly
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < n; i += kBlock) {
+            const unsigned long long r = s_base + i;
+            if (r < a.rec_cap)
+                a.rec[r] = i < kLdsSpikes ? s_list[i] : seg[i];
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
-// Scatter: grid (x: waves striding over the batch's spikes, y: 256-synapse
-// chunk of the row).  Each lane loads 4 synapses before issuing its 4
-// accumulations.  The last kernel of a batch advances the device clock.
+// Scatter: grid (x: waves striding over the update blocks' list segments,
+// y: 256-synapse chunk of the row).  Each lane loads 4 synapses before
+// issuing its 4 accumulations.  The last kernel of a batch advances the
+// device clock.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void batch_epilogue(DevState* st, uint32_t cur, uint32_t M, uint32_t B)
+__device__ __forceinline__ void batch_epilogue(DevState* st, uint32_t B)
 {
-    st->spikes += M;
-    st->count[cur ^ 1u] = 0;
     st->clock += B;
     st->batch += 1;
+}
+
+__device__ __forceinline__ uint32_t wave_global()
+{
+    return blockIdx.x * (kBlock / kWave) + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
 }
 
 template <bool DET, bool PLASTIC>
@@ -290,51 +300,53 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const ScatterArgs a)
     using RingT = typename std::conditional<DET, unsigned long long, float>::type;
     DevState* st = a.st;
     const uint32_t cur = st->cur;
-    const uint32_t M = st->count[cur];
     const naer_spike* list = cur ? a.list1 : a.list0;
+    const uint32_t* counts = a.seg_count + cur * a.n_seg;
     if (a.is_last && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0)
-        batch_epilogue(st, cur, M, a.B);
+        batch_epilogue(st, a.B);
 
     RingT* ring = (RingT*)a.ring;
     const uint32_t lane = lane_id();
-    const uint32_t wave =
-        blockIdx.x * (kBlock / kWave) + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const uint32_t nw = gridDim.x * (kBlock / kWave);
     const unsigned long long chunk_off = (unsigned long long)blockIdx.y * kChunk;
-    for (uint32_t i = wave; i < M; i += nw) {
-        const naer_spike sp = list[i];
-        const uint32_t s = sp.neuron;
-        const unsigned long long re = a.row_ptr[s + 1];
-        const unsigned long long beg = a.row_ptr[s] + chunk_off;
-        if (beg >= re)
-            continue;
-        const uint32_t tD = sp.step % a.D;
-        uint32_t c[kChunkUnroll];
-        float wv[kChunkUnroll];
+    for (uint32_t g = wave_global(); g < a.n_seg; g += nw) {
+        const uint32_t cnt = counts[g];
+        const naer_spike* seg = list + (size_t)g * a.seg_cap;
+        for (uint32_t i = 0; i < cnt; ++i) {
+            const naer_spike sp = seg[i];
+            const uint32_t s = sp.neuron;
+            const unsigned long long re = a.row_ptr[s + 1];
+            const unsigned long long beg = a.row_ptr[s] + chunk_off;
+            if (beg >= re)
+                continue;
+            const uint32_t tD = sp.step % a.D;
+            uint32_t c[kChunkUnroll];
+            float wv[kChunkUnroll];
 #pragma unroll
-        for (uint32_t u = 0; u < kChunkUnroll; ++u) {
-            const unsigned long long q = beg + u * kWave + lane;
-            c[u] = q < re ? a.col[q] : 0u;
-            wv[u] = q < re ? a.w[q] : 0.0f;
-        }
+            for (uint32_t u = 0; u < kChunkUnroll; ++u) {
+                const unsigned long long q = beg + u * kWave + lane;
+                c[u] = q < re ? a.col[q] : 0u;
+                wv[u] = q < re ? a.w[q] : 0.0f;
+            }
 #pragma unroll
-        for (uint32_t u = 0; u < kChunkUnroll; ++u) {
-            const unsigned long long q = beg + u * kWave + lane;
-            if (q < re) {
-                const uint32_t tgt = c[u] & kTargetMask;
-                uint32_t sl = tD + ((c[u] >> 24) & 0x7Fu);
-                sl = sl >= a.D ? sl - a.D : sl;
-                float x = wv[u];
-                if (PLASTIC && (c[u] & kPlasticBit)) {
-                    const float dw = a.a_minus * a.yh[tgt];
-                    x = clip(x - dw, a.w_min, a.w_max);
-                    a.w[q] = x;
+            for (uint32_t u = 0; u < kChunkUnroll; ++u) {
+                const unsigned long long q = beg + u * kWave + lane;
+                if (q < re) {
+                    const uint32_t tgt = c[u] & kTargetMask;
+                    uint32_t sl = tD + ((c[u] >> 24) & 0x7Fu);
+                    sl = sl >= a.D ? sl - a.D : sl;
+                    float x = wv[u];
+                    if (PLASTIC && (c[u] & kPlasticBit)) {
+                        const float dw = a.a_minus * a.yh[tgt];
+                        x = clip(x - dw, a.w_min, a.w_max);
+                        a.w[q] = x;
+                    }
+                    RingT* dst = ring + (size_t)sl * a.N + tgt;
+                    if (DET)
+                        atomicAdd((unsigned long long*)dst, (unsigned long long)to_fixed(x));
+                    else
+                        unsafeAtomicAdd((float*)dst, x);
                 }
-                RingT* dst = ring + (size_t)sl * a.N + tgt;
-                if (DET)
-                    atomicAdd((unsigned long long*)dst, (unsigned long long)to_fixed(x));
-                else
-                    unsafeAtomicAdd((float*)dst, x);
             }
         }
     }
@@ -348,21 +360,23 @@ __global__ __launch_bounds__(kBlock) void k_potentiate(const PotentiateArgs a)
 {
     DevState* st = a.st;
     const uint32_t cur = st->cur;
-    const uint32_t M = st->count[cur];
     const naer_spike* list = cur ? a.list1 : a.list0;
+    const uint32_t* counts = a.seg_count + cur * a.n_seg;
     if (blockIdx.x == 0 && threadIdx.x == 0)
-        batch_epilogue(st, cur, M, a.B);
+        batch_epilogue(st, a.B);
     const uint32_t lane = lane_id();
-    const uint32_t wave =
-        blockIdx.x * (kBlock / kWave) + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const uint32_t nw = gridDim.x * (kBlock / kWave);
-    for (uint32_t i = wave; i < M; i += nw) {
-        const uint32_t post = list[i].neuron;
-        const unsigned long long rb = a.rev_ptr[post], re = a.rev_ptr[post + 1];
-        for (unsigned long long r = rb + lane; r < re; r += kWave) {
-            const uint32_t pos = a.rev_pos[r];
-            const float dw = a.a_plus * a.xh[a.rev_src[r]];
-            a.w[pos] = clip(a.w[pos] + dw, a.w_min, a.w_max);
+    for (uint32_t g = wave_global(); g < a.n_seg; g += nw) {
+        const uint32_t cnt = counts[g];
+        const naer_spike* seg = list + (size_t)g * a.seg_cap;
+        for (uint32_t i = 0; i < cnt; ++i) {
+            const uint32_t post = seg[i].neuron;
+            const unsigned long long rb = a.rev_ptr[post], re = a.rev_ptr[post + 1];
+            for (unsigned long long r = rb + lane; r < re; r += kWave) {
+                const uint32_t pos = a.rev_pos[r];
+                const float dw = a.a_plus * a.xh[a.rev_src[r]];
+                a.w[pos] = clip(a.w[pos] + dw, a.w_min, a.w_max);
+            }
         }
     }
 }

@@ -104,16 +104,17 @@ struct PopParams {
 // graphs) need no per-step host arguments.
 struct DevState {
     unsigned long long clock;   // first step of the next batch
-    unsigned long long spikes;  // spikes emitted so far
-    unsigned long long events;  // synaptic events so far
-    unsigned long long rec_base;  // spike index where the record buffer starts
+    unsigned long long rec_n;   // spikes offered to the record buffer since the last drain
     uint32_t batch;             // batches done; the update writes list[batch & 1]
     uint32_t cur;               // list of the batch in flight
-    uint32_t count[2];          // spikes in list[0], list[1]
-    uint32_t overflow;          // spikes lost to a full batch list
+    uint32_t overflow;          // spikes lost to a full list segment
     uint32_t _pad;
 };
 
+// Spike lists are segmented by update block: block b of the update kernel
+// owns list[b * seg_cap, (b+1) * seg_cap) and seg_count[b], so compaction
+// needs no global atomics.  Per-block spike and event totals accumulate in
+// spk_part / ev_part (each block owns its entry).
 struct UpdateArgs {
     float* V;
     int32_t* ref;
@@ -126,6 +127,9 @@ struct UpdateArgs {
     const unsigned long long* row_ptr;
     naer_spike* list0;
     naer_spike* list1;
+    uint32_t* seg_count;               // [2][n_seg]
+    unsigned long long* spk_part;      // [n_seg]
+    unsigned long long* ev_part;       // [n_seg]
     naer_spike* rec;
     DevState* st;
     const float* cdf;
@@ -134,7 +138,9 @@ struct UpdateArgs {
     unsigned long long probe_t0;
     unsigned long long seed;
     uint32_t probe_n;
-    uint32_t list_cap;
+    uint32_t seg_cap;
+    uint32_t n_seg;
+    uint32_t _pad0;
     uint32_t N, D;
     uint32_t B;
     uint32_t dt_us;
@@ -149,6 +155,8 @@ struct UpdateArgs {
 struct ScatterArgs {
     const naer_spike* list0;
     const naer_spike* list1;
+    const uint32_t* seg_count;
+    uint32_t seg_cap, n_seg;
     DevState* st;
     const unsigned long long* row_ptr;
     const uint32_t* col;
@@ -165,6 +173,8 @@ struct ScatterArgs {
 struct PotentiateArgs {
     const naer_spike* list0;
     const naer_spike* list1;
+    const uint32_t* seg_count;
+    uint32_t seg_cap, n_seg;
     DevState* st;
     const unsigned long long* rev_ptr;
     const uint32_t* rev_pos;

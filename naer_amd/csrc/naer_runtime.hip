@@ -76,7 +76,9 @@ struct naer_net {
     uint32_t *rev_pos = nullptr, *rev_src = nullptr;
     uint64_t n_plastic = 0;
     naer_spike *list0 = nullptr, *list1 = nullptr, *rec = nullptr;
-    uint32_t list_cap = 0;
+    uint32_t seg_cap = 0, n_seg = 0;
+    uint32_t* seg_count = nullptr;
+    unsigned long long *spk_part = nullptr, *ev_part = nullptr;
     uint64_t rec_cap = 0, dropped = 0;
     DevState* st = nullptr;
     float* d_cdf = nullptr;
@@ -284,6 +286,9 @@ UpdateArgs make_update_args(naer_net* n)
     a.row_ptr = n->row_ptr;
     a.list0 = n->list0;
     a.list1 = n->list1;
+    a.seg_count = n->seg_count;
+    a.spk_part = n->spk_part;
+    a.ev_part = n->ev_part;
     a.rec = n->rec;
     a.st = n->st;
     a.cdf = n->d_cdf;
@@ -292,7 +297,8 @@ UpdateArgs make_update_args(naer_net* n)
     a.probe_t0 = n->probe_t0;
     a.probe_n = n->probe_n;
     a.seed = n->desc.seed;
-    a.list_cap = n->list_cap;
+    a.seg_cap = n->seg_cap;
+    a.n_seg = n->n_seg;
     a.N = n->N;
     a.D = n->D;
     a.B = n->B;
@@ -508,14 +514,20 @@ int naer_net_build(naer_net* n)
     if ((rc = n->alloc((char**)&n->ring, ring_bytes)))
         return rc;
     HIP_TRY(hipMemsetAsync(n->ring, 0, ring_bytes, n->stream));
-    uint64_t n_lif = N - n->n_pois;
+    // list segment of one update block: every LIF neuron fires at most once
+    // per step, a POISSON source at most 10 times per 1500 us epoch
     const uint64_t epochs = ((uint64_t)n->B * n->dt_us + kEpochUs - 1) / kEpochUs + 1;
-    const uint64_t cap = n_lif * n->B + (uint64_t)n->n_pois * kMaxPoissonPerEpoch * epochs;
-    if (cap > 0xFFFFFFF0ull)
-        return fail(NAER_ERR_CAPACITY, "spike list of a batch would exceed 2^32 entries");
-    n->list_cap = (uint32_t)cap;
-    if ((rc = n->alloc(&n->list0, cap)) || (rc = n->alloc(&n->list1, cap)))
+    const uint64_t per_neuron = n->n_pois ? std::max<uint64_t>(n->B, kMaxPoissonPerEpoch * epochs) : n->B;
+    n->seg_cap = (uint32_t)(kBlock * per_neuron);
+    n->n_seg = (N + kBlock - 1) / kBlock;
+    const size_t list_elems = (size_t)n->seg_cap * n->n_seg;
+    if ((rc = n->alloc(&n->list0, list_elems)) || (rc = n->alloc(&n->list1, list_elems)) ||
+        (rc = n->alloc(&n->seg_count, 2 * (size_t)n->n_seg)) || (rc = n->alloc(&n->spk_part, n->n_seg)) ||
+        (rc = n->alloc(&n->ev_part, n->n_seg)))
         return rc;
+    HIP_TRY(hipMemsetAsync(n->seg_count, 0, 2 * (size_t)n->n_seg * sizeof(uint32_t), n->stream));
+    HIP_TRY(hipMemsetAsync(n->spk_part, 0, n->n_seg * sizeof(unsigned long long), n->stream));
+    HIP_TRY(hipMemsetAsync(n->ev_part, 0, n->n_seg * sizeof(unsigned long long), n->stream));
     n->rec_cap = n->desc.record_capacity;
     if ((rc = n->alloc(&n->rec, n->rec_cap ? n->rec_cap : 1)))
         return rc;
@@ -563,6 +575,9 @@ int naer_net_step(naer_net* n, uint32_t n_steps)
     std::memset(&sa, 0, sizeof sa);
     sa.list0 = n->list0;
     sa.list1 = n->list1;
+    sa.seg_count = n->seg_count;
+    sa.seg_cap = n->seg_cap;
+    sa.n_seg = n->n_seg;
     sa.st = n->st;
     sa.row_ptr = n->row_ptr;
     sa.col = n->col;
@@ -579,6 +594,9 @@ int naer_net_step(naer_net* n, uint32_t n_steps)
     std::memset(&pa, 0, sizeof pa);
     pa.list0 = n->list0;
     pa.list1 = n->list1;
+    pa.seg_count = n->seg_count;
+    pa.seg_cap = n->seg_cap;
+    pa.n_seg = n->n_seg;
     pa.st = n->st;
     pa.rev_ptr = n->rev_ptr;
     pa.rev_pos = n->rev_pos;
@@ -645,12 +663,16 @@ int naer_net_query(naer_net* n, naer_stats* out)
     if (n->built) {
         DevState s;
         HIP_TRY(hipMemcpy(&s, n->st, sizeof s, hipMemcpyDeviceToHost));
+        std::vector<unsigned long long> part(n->n_seg);
+        HIP_TRY(hipMemcpy(part.data(), n->spk_part, n->n_seg * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        for (auto v : part)
+            out->spikes += v;
+        HIP_TRY(hipMemcpy(part.data(), n->ev_part, n->n_seg * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        for (auto v : part)
+            out->events += v;
         out->step = s.clock;
         out->time_us = s.clock * n->dt_us;
-        out->spikes = s.spikes;
-        out->events = s.events;
-        const uint64_t pend = s.spikes - s.rec_base;
-        out->dropped = n->dropped + (pend > n->rec_cap ? pend - n->rec_cap : 0) + s.overflow;
+        out->dropped = n->dropped + (s.rec_n > n->rec_cap ? s.rec_n - n->rec_cap : 0) + s.overflow;
     }
     return NAER_OK;
 }
@@ -666,7 +688,7 @@ int naer_net_read_spikes(naer_net* n, naer_spike* out, uint64_t cap, uint64_t* n
         return rc;
     DevState s;
     HIP_TRY(hipMemcpy(&s, n->st, sizeof s, hipMemcpyDeviceToHost));
-    const uint64_t pend = s.spikes - s.rec_base;
+    const uint64_t pend = s.rec_n;
     const uint64_t avail = std::min<uint64_t>(pend, n->rec_cap);
     const uint64_t k = std::min<uint64_t>(avail, cap);
     if (k)
@@ -674,8 +696,8 @@ int naer_net_read_spikes(naer_net* n, naer_spike* out, uint64_t cap, uint64_t* n
     // the buffer is drained: spikes not returned (beyond the record capacity
     // or beyond `cap`) are counted as dropped
     n->dropped += pend - k;
-    s.rec_base = s.spikes;
-    HIP_TRY(hipMemcpy(&n->st->rec_base, &s.rec_base, sizeof s.rec_base, hipMemcpyHostToDevice));
+    const unsigned long long zero = 0;
+    HIP_TRY(hipMemcpy(&n->st->rec_n, &zero, sizeof zero, hipMemcpyHostToDevice));
     *n_out = k;
     return NAER_OK;
 }
