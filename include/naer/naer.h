@@ -65,7 +65,9 @@ extern "C" {
 
 /* ---- random streams --------------------------------------------------------
  * Every random quantity is a pure function of (seed, index), drawn from
- * Philox4x32-10 with key = {low 32 bits, high 32 bits} of the seed:
+ * Philox4x32-10 with key = {low 32 bits, high 32 bits} of the seed.  j is the
+ * neuron index within its network and t the network's own step count (both
+ * independent of how networks are packed, see naer_pack_create):
  *   initial potential of neuron j:        ctr = {j, 0, NAER_STREAM_V0, 0}, word 0
  *   Poisson drive of neuron j at step t:   ctr = {j, t >> 2, NAER_STREAM_DRIVE, 0},
  *                                          word t & 3
@@ -220,6 +222,18 @@ int naer_ctx_destroy(naer_ctx* ctx);
 typedef struct naer_net naer_net;
 int naer_net_create(naer_ctx* ctx, const naer_net_desc* desc, naer_net** out);
 
+/* A pack: n_nets independent networks simulated by the same launches (the
+ * batched parameter sweep).  Networks share dt and mode (and the STDP
+ * parameters when plastic); each keeps its own seeds, local neuron ids and
+ * local clock, so its results do not depend on what it is packed with
+ * (bit for bit in NAER_MODE_DETERMINISTIC).  Neurons of network i occupy
+ * [neuron_begin[i], neuron_begin[i] + n_neurons[i]) of the pack; recorded
+ * spikes carry pack neuron ids and pack steps.  All naer_net_* calls accept
+ * a pack; naer_net_create is naer_pack_create with one network.  No
+ * counterpart in the reference (one network per process, src/main.cpp). */
+int naer_pack_create(naer_ctx* ctx, const naer_net_desc* descs, uint32_t n_nets, naer_net** out);
+int naer_pack_layout(naer_net* pack, uint32_t* neuron_begin, uint32_t* n_neurons, uint64_t* t_start);
+
 /* Device connectivity build + state initialisation; replaces
  * Network::compile (src/Network.cpp:267-308) and allocateConnections
  * (src/Network.cpp:421-569).  Synchronous. */
@@ -229,6 +243,13 @@ int naer_net_build(naer_net* net);
  * Network::step (src/Network.cpp:122-256) advances one 1.5 ms epoch; the C++
  * wrapper calls this with n_steps = 1500 us / dt. */
 int naer_net_step(naer_net* net, uint32_t n_steps);
+
+/* Enqueue steps as replays of one captured HIP graph of chunk_steps steps
+ * (0 = plain launches, the default).  The kernels read the step counter
+ * from device memory, so one graph serves every chunk; steps that do not
+ * fill a chunk are launched plainly.  Kernel timers do not cover graph
+ * replays.  No counterpart in the reference. */
+int naer_net_set_graph(naer_net* net, uint32_t chunk_steps);
 
 /* Wait for all enqueued work (Network::finish, src/Network.cpp:258-265). */
 int naer_net_sync(naer_net* net);

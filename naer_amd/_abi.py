@@ -100,9 +100,12 @@ _SIGS = {
     "naer_ctx_create": [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)],
     "naer_ctx_destroy": [ctypes.c_void_p],
     "naer_net_create": [ctypes.c_void_p, ctypes.POINTER(NetDesc), ctypes.POINTER(ctypes.c_void_p)],
+    "naer_pack_create": [ctypes.c_void_p, ctypes.POINTER(NetDesc), u32, ctypes.POINTER(ctypes.c_void_p)],
+    "naer_pack_layout": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     "naer_net_build": [ctypes.c_void_p],
     "naer_net_step": [ctypes.c_void_p, u32],
     "naer_net_sync": [ctypes.c_void_p],
+    "naer_net_set_graph": [ctypes.c_void_p, u32],
     "naer_net_query": [ctypes.c_void_p, ctypes.POINTER(Stats)],
     "naer_net_set_profiling": [ctypes.c_void_p, ctypes.c_int],
     "naer_net_read_spikes": [ctypes.c_void_p, ctypes.c_void_p, u64, ctypes.POINTER(u64)],

@@ -37,12 +37,14 @@ __device__ __forceinline__ float clip(float w, float lo, float hi)
     return w < lo ? lo : (w > hi ? hi : w);
 }
 
-__device__ __forceinline__ uint32_t find_pop(const UpdateArgs& a, uint32_t j)
+// Population of neuron j: populations are contiguous ranges; the host gives
+// the first population of every update block.
+__device__ __forceinline__ uint32_t find_pop(const PopParams* pops, uint32_t n_pops, uint32_t first,
+                                             uint32_t j)
 {
-    uint32_t p = 0;
-    for (uint32_t q = 1; q < a.n_pops; ++q)
-        if (j >= a.pops[q].begin)
-            p = q;
+    uint32_t p = first;
+    while (p + 1 < n_pops && j >= pops[p + 1].begin)
+        ++p;
     return p;
 }
 
@@ -98,7 +100,7 @@ __global__ __launch_bounds__(kBlock) void k_update(const UpdateArgs a)
 {
     using RingT = typename std::conditional<DET, long long, float>::type;
     __shared__ naer_spike s_list[kLdsSpikes];
-    __shared__ float s_cdf[kMaxPops * NAER_DRIVE_TABLE];
+    __shared__ float s_cdf[kStagePops * NAER_DRIVE_TABLE];
     __shared__ uint32_t s_n;
     __shared__ unsigned long long s_events;
     __shared__ unsigned long long s_base;
@@ -106,8 +108,12 @@ __global__ __launch_bounds__(kBlock) void k_update(const UpdateArgs a)
         s_n = 0;
         s_events = 0;
     }
-    for (uint32_t i = threadIdx.x; i < a.n_pops * NAER_DRIVE_TABLE; i += kBlock)
-        s_cdf[i] = a.cdf[i];
+    const uint32_t p_first = a.blk_pop[blockIdx.x];
+    {
+        const uint32_t np = min(kStagePops, a.n_pops - p_first);
+        for (uint32_t i = threadIdx.x; i < np * NAER_DRIVE_TABLE; i += kBlock)
+            s_cdf[i] = a.cdf[(size_t)p_first * NAER_DRIVE_TABLE + i];
+    }
     __syncthreads();
 
     DevState* st = a.st;
@@ -121,8 +127,10 @@ __global__ __launch_bounds__(kBlock) void k_update(const UpdateArgs a)
     const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
     unsigned long long my_events = 0;
     if (j < a.N) {
-        const uint32_t p = find_pop(a, j);
-        const PopParams& pp = a.pops[p];
+        const uint32_t p = find_pop(a.pops, a.n_pops, p_first, j);
+        const PopParams pp = a.pops[p];
+        const uint32_t jl = j - pp.net_base;                 // neuron id within its net
+        const unsigned long long t0l = t0 - pp.t_start;      // net-local step of t0
         if (pp.kind == NAER_POP_LIF) {
             RingT* ring = (RingT*)a.ring;
             uint32_t slot0 = (uint32_t)(t0 % a.D);
@@ -156,8 +164,9 @@ __global__ __launch_bounds__(kBlock) void k_update(const UpdateArgs a)
             const float w_ext = pp.w_ext;
             const int32_t n_ref = pp.n_ref;
             const bool has_drive = pp.has_drive != 0;
-            const float* cdf = s_cdf + pp.cdf_offset;
-            const uint32_t s_lo = (uint32_t)a.seed, s_hi = (uint32_t)(a.seed >> 32);
+            const float* cdf = p - p_first < kStagePops ? s_cdf + (p - p_first) * NAER_DRIVE_TABLE
+                                                        : a.cdf + pp.cdf_offset;
+            const uint32_t s_lo = pp.seed_lo, s_hi = pp.seed_hi;
             U4 rnd{ 0, 0, 0, 0 };
             uint32_t rnd_blk = 0xFFFFFFFFu;
 #pragma nounroll
@@ -190,12 +199,13 @@ __global__ __launch_bounds__(kBlock) void k_update(const UpdateArgs a)
                     float v = xp + v_rest;
                     v = v + I;
                     if (has_drive) {
-                        const uint32_t blk = (uint32_t)(t >> 2);
+                        const unsigned long long tl = t0l + s;
+                        const uint32_t blk = (uint32_t)(tl >> 2);
                         if (blk != rnd_blk) {
-                            rnd = philox(U4{ j, blk, NAER_STREAM_DRIVE, 0u }, s_lo, s_hi);
+                            rnd = philox(U4{ jl, blk, NAER_STREAM_DRIVE, 0u }, s_lo, s_hi);
                             rnd_blk = blk;
                         }
-                        const float u = unit24(word_of(rnd, (uint32_t)t & 3u));
+                        const float u = unit24(word_of(rnd, (uint32_t)tl & 3u));
                         int k = 0;
                         while (u >= cdf[k])
                             ++k;
@@ -210,7 +220,7 @@ __global__ __launch_bounds__(kBlock) void k_update(const UpdateArgs a)
                         my_events += a.row_ptr[j + 1] - a.row_ptr[j];
                     }
                 }
-                emit(sg, fired, j, (uint32_t)t, (uint32_t)t * a.dt_us);
+                emit(sg, fired, j, (uint32_t)t, (uint32_t)(t0l + s) * a.dt_us);
             }
             a.V[j] = V;
             a.ref[j] = ref;
@@ -229,9 +239,9 @@ __global__ __launch_bounds__(kBlock) void k_update(const UpdateArgs a)
 #pragma nounroll
             for (uint32_t s = 0; s < a.B; ++s) {
                 const unsigned long long t = t0 + s;
-                const uint32_t T = (uint32_t)t * a.dt_us;
+                const uint32_t T = (uint32_t)(t0l + s) * a.dt_us;
                 if (T % kEpochUs == 0) {
-                    nxt = T + ref_isi_us(pp.neg_rate_per_us, gid, a.pois_seed, T * a.n_pois + idx);
+                    nxt = T + ref_isi_us(pp.neg_rate_per_us, gid, pp.pois_seed, T * pp.n_pois + idx);
                     cnt = 0;
                 }
                 const uint32_t end = T + a.dt_us;
@@ -241,7 +251,7 @@ __global__ __launch_bounds__(kBlock) void k_update(const UpdateArgs a)
                     if (pend) {
                         my_events += a.row_ptr[j + 1] - a.row_ptr[j];
                         ++cnt;
-                        nxt = nxt + ref_isi_us(pp.neg_rate_per_us, gid, a.pois_seed, nxt * a.n_pois + idx);
+                        nxt = nxt + ref_isi_us(pp.neg_rate_per_us, gid, pp.pois_seed, nxt * pp.n_pois + idx);
                         pend = cnt < kMaxPoissonPerEpoch && nxt < end;
                     }
                 }
@@ -513,11 +523,10 @@ __global__ __launch_bounds__(kBlock) void k_init_state(const UpdateArgs a)
     const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
     if (j >= a.N)
         return;
-    const PopParams& pp = a.pops[find_pop(a, j)];
+    const PopParams pp = a.pops[find_pop(a.pops, a.n_pops, a.blk_pop[blockIdx.x], j)];
     float v = 0.0f;
     if (pp.kind == NAER_POP_LIF) {
-        const U4 r = philox(U4{ j, 0u, NAER_STREAM_V0, 0u }, (uint32_t)a.seed,
-                            (uint32_t)(a.seed >> 32));
+        const U4 r = philox(U4{ j - pp.net_base, 0u, NAER_STREAM_V0, 0u }, pp.seed_lo, pp.seed_hi);
         const float span = pp.v0_max - pp.v0_min;
         const float x = span * unit24(r.x);
         v = pp.v0_min + x;

@@ -10,7 +10,7 @@
 
 namespace naer {
 
-constexpr uint32_t kMaxPops = 16;      // populations per net passed by value
+constexpr uint32_t kStagePops = 4;     // drive tables of a block's first populations staged in LDS
 constexpr uint32_t kMaxBatch = 16;     // steps per update launch
 constexpr uint32_t kBlock = 256;       // threads per block of the step kernels
 constexpr uint32_t kWave = 64;         // CDNA wavefront
@@ -86,18 +86,25 @@ __host__ __device__ __forceinline__ uint32_t ref_poisson_gid(uint32_t idx)
 // Records in device memory
 // ---------------------------------------------------------------------------
 struct PopParams {
-    uint32_t begin, end;  // neuron range [begin, end)
+    uint32_t begin, end;  // neuron range [begin, end) in the pack
     uint32_t kind;        // NAER_POP_*
-    uint32_t pois_base;   // POISSON: index of `begin` among all POISSON neurons
+    uint32_t pois_base;   // POISSON: index of `begin` among the POISSON neurons of its net
     float P;              // exp(-dt/tau_m)
     float v_th, v_reset, v_rest;
     float v0_min, v0_max;  // initial potential range
     float w_ext;          // drive jump
     int32_t n_ref;        // refractory steps
     uint32_t has_drive;
-    double neg_rate_per_us;  // POISSON: -(rate/1e6)
     uint32_t cdf_offset;  // drive CDF table offset (floats)
+    double neg_rate_per_us;  // POISSON: -(rate/1e6)
+    // the network the population belongs to: random draws and clocks are
+    // local to it, so a network gives the same results alone or packed
+    uint32_t net_base;    // first neuron of the net in the pack
+    uint32_t n_pois;      // POISSON neurons of the net
+    uint32_t seed_lo, seed_hi;
+    uint32_t pois_seed;
     uint32_t _pad;
+    unsigned long long t_start;  // pack step at which the net's local time is 0
 };
 
 // Step bookkeeping that lives on the device so that launches (and captured
@@ -132,24 +139,20 @@ struct UpdateArgs {
     unsigned long long* ev_part;       // [n_seg]
     naer_spike* rec;
     DevState* st;
-    const float* cdf;
+    const float* cdf;                  // NAER_DRIVE_TABLE floats per population
+    const PopParams* pops;             // [n_pops]
+    const uint32_t* blk_pop;           // first population of each update block
     float* probe;
     unsigned long long rec_cap;
     unsigned long long probe_t0;
-    unsigned long long seed;
     uint32_t probe_n;
     uint32_t seg_cap;
     uint32_t n_seg;
-    uint32_t _pad0;
     uint32_t N, D;
     uint32_t B;
     uint32_t dt_us;
-    uint32_t n_pois;
-    uint32_t pois_seed;
-    float dec_p, dec_m;
     uint32_t n_pops;
-    uint32_t _pad;
-    PopParams pops[kMaxPops];
+    float dec_p, dec_m;
 };
 
 struct ScatterArgs {

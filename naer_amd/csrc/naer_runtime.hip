@@ -49,11 +49,24 @@ struct PendingTimer {
     hipEvent_t a, b;
 };
 
+// One network of a pack.
+struct NetInfo {
+    uint32_t neuron_begin = 0, n_neurons = 0;
+    uint32_t pop_begin = 0, n_pops = 0;
+    uint32_t conn_begin = 0, n_conns = 0;
+    uint64_t t_start = 0;  // pack step at which its local time is 0
+};
+
 struct naer_net {
     naer_ctx* ctx = nullptr;
-    naer_net_desc desc{};
-    std::vector<naer_pop_desc> pops;
-    std::vector<naer_conn_desc> conns;
+    naer_net_desc desc{};               // pack-wide settings (dt, mode, STDP) of the first net
+    std::vector<naer_net_desc> net_descs;
+    std::vector<NetInfo> nets;
+    std::vector<naer_pop_desc> pops;    // all populations, pack order
+    std::vector<naer_conn_desc> conns;  // all connections, population indices into `pops`
+    std::vector<uint32_t> conn_local;   // index of each connection within its net
+    std::vector<uint32_t> pop_net;      // net of each population
+    std::vector<uint32_t> blk_pop_h;    // first population of each update block
     uint32_t N = 0, n_pois = 0, dt_us = 0, D = 0, B = 0, dmin = 0, dmax = 0;
     bool plastic = false, det = false, built = false;
     std::vector<PopParams> pp;
@@ -82,6 +95,8 @@ struct naer_net {
     uint64_t rec_cap = 0, dropped = 0;
     DevState* st = nullptr;
     float* d_cdf = nullptr;
+    PopParams* d_pops = nullptr;
+    uint32_t* d_blk_pop = nullptr;
     float* probe = nullptr;
     uint64_t probe_t0 = 0;
     uint32_t probe_n = 0;
@@ -90,6 +105,8 @@ struct naer_net {
     uint64_t steps_enqueued = 0;
 
     bool profiling = false;
+    uint32_t graph_chunk = 0;         // steps per captured graph (0 = plain launches)
+    hipGraphExec_t graph_exec = nullptr;
     std::vector<PendingTimer> pending;
     std::vector<hipEvent_t> event_pool;
     double kernel_ms[8] = {};
@@ -99,6 +116,8 @@ struct naer_net {
     {
         if (stream)
             (void)hipStreamSynchronize(stream);
+        if (graph_exec)
+            (void)hipGraphExecDestroy(graph_exec);
         for (auto& p : pending) {
             (void)hipEventDestroy(p.a);
             (void)hipEventDestroy(p.b);
@@ -167,107 +186,175 @@ namespace {
 
 int validate_and_derive(naer_net* n)
 {
-    const naer_net_desc& d = n->desc;
-    if (d.n_pops == 0 || d.n_pops > kMaxPops)
-        return fail(NAER_ERR_INVALID, "a net needs 1..%u populations (got %u)", kMaxPops, d.n_pops);
-    const double dtus = (double)d.dt_ms * 1000.0;
+    if (n->net_descs.empty())
+        return fail(NAER_ERR_INVALID, "a pack needs at least one network");
+    n->desc = n->net_descs[0];
+    const naer_net_desc& d0 = n->desc;
+    const double dtus = (double)d0.dt_ms * 1000.0;
     n->dt_us = (uint32_t)std::llround(dtus);
-    if (!(d.dt_ms > 0.0f) || std::fabs(dtus - (double)n->dt_us) > 1e-3 || n->dt_us == 0)
+    if (!(d0.dt_ms > 0.0f) || std::fabs(dtus - (double)n->dt_us) > 1e-3 || n->dt_us == 0)
         return fail(NAER_ERR_INVALID, "dt must be a positive whole number of microseconds");
-    if (d.mode != NAER_MODE_FAST && d.mode != NAER_MODE_DETERMINISTIC)
-        return fail(NAER_ERR_INVALID, "unknown mode %u", d.mode);
-    n->det = d.mode == NAER_MODE_DETERMINISTIC;
+    if (d0.mode != NAER_MODE_FAST && d0.mode != NAER_MODE_DETERMINISTIC)
+        return fail(NAER_ERR_INVALID, "unknown mode %u", d0.mode);
+    n->det = d0.mode == NAER_MODE_DETERMINISTIC;
 
-    uint64_t N = 0, pois = 0;
-    n->pp.resize(d.n_pops);
-    n->cdf.assign((size_t)d.n_pops * NAER_DRIVE_TABLE, 2.0f);
-    for (uint32_t p = 0; p < d.n_pops; ++p) {
+    uint64_t N = 0;
+    n->dmin = 0xFFFFFFFFu;
+    n->dmax = 1;
+    bool stdp_seen = false;
+    naer_stdp_desc stdp{};
+    uint64_t rec = 0;
+    for (uint32_t ni = 0; ni < n->net_descs.size(); ++ni) {
+        const naer_net_desc& d = n->net_descs[ni];
+        if (d.n_pops == 0 || !d.pops)
+            return fail(NAER_ERR_INVALID, "network %u has no populations", ni);
+        if (d.n_conns && !d.conns)
+            return fail(NAER_ERR_INVALID, "network %u: conns is NULL", ni);
+        if (d.dt_ms != d0.dt_ms || d.mode != d0.mode)
+            return fail(NAER_ERR_INVALID, "network %u: all networks of a pack share dt and mode", ni);
+        NetInfo info;
+        info.neuron_begin = (uint32_t)N;
+        info.pop_begin = (uint32_t)n->pops.size();
+        info.n_pops = d.n_pops;
+        info.conn_begin = (uint32_t)n->conns.size();
+        info.n_conns = d.n_conns;
+        uint64_t nn = 0;
+        for (uint32_t p = 0; p < d.n_pops; ++p) {
+            n->pops.push_back(d.pops[p]);
+            n->pop_net.push_back(ni);
+            nn += d.pops[p].size;
+        }
+        if (nn == 0 || nn > (1ull << 24))
+            return fail(NAER_ERR_CAPACITY, "network %u: neuron count must be in [1, 2^24] (got %llu)", ni,
+                        (unsigned long long)nn);
+        info.n_neurons = (uint32_t)nn;
+        N += nn;
+        for (uint32_t c = 0; c < d.n_conns; ++c) {
+            naer_conn_desc k = d.conns[c];
+            if (k.src_pop >= d.n_pops || k.dst_pop >= d.n_pops)
+                return fail(NAER_ERR_INVALID, "network %u connection %u: population index out of range", ni, c);
+            if (d.pops[k.dst_pop].kind != NAER_POP_LIF)
+                return fail(NAER_ERR_INVALID, "network %u connection %u: target population must be LIF", ni, c);
+            const uint32_t q = k.delay_quantum ? k.delay_quantum : 1;
+            if (k.delay_min < 1 || k.delay_max < k.delay_min || k.delay_max > 127 ||
+                (k.delay_max - k.delay_min) % q)
+                return fail(NAER_ERR_INVALID,
+                            "network %u connection %u: delays must satisfy 1 <= min <= max <= 127 on the quantum grid",
+                            ni, c);
+            if (k.plastic && d.pops[k.src_pop].kind != NAER_POP_LIF)
+                return fail(NAER_ERR_INVALID, "network %u connection %u: plastic connections need a LIF source",
+                            ni, c);
+            if (k.rule != NAER_RULE_BERNOULLI && k.rule != NAER_RULE_FIXED_INDEGREE)
+                return fail(NAER_ERR_INVALID, "network %u connection %u: unknown rule %u", ni, c, k.rule);
+            if (k.rule == NAER_RULE_BERNOULLI && !(k.probability >= 0.0 && k.probability <= 1.0))
+                return fail(NAER_ERR_INVALID, "network %u connection %u: probability must be in [0, 1]", ni, c);
+            if (k.plastic) {
+                if (stdp_seen && std::memcmp(&stdp, &d.stdp, sizeof stdp) != 0)
+                    return fail(NAER_ERR_INVALID, "plastic networks of a pack must share the STDP parameters");
+                stdp = d.stdp;
+                stdp_seen = true;
+                n->plastic = true;
+            }
+            n->dmin = std::min(n->dmin, k.delay_min);
+            n->dmax = std::max(n->dmax, k.delay_max);
+            k.src_pop += info.pop_begin;
+            k.dst_pop += info.pop_begin;
+            n->conns.push_back(k);
+            n->conn_local.push_back(c);
+        }
+        rec += d.record_capacity;
+        n->nets.push_back(info);
+    }
+    if (N > 0xFFFFFFF0ull)
+        return fail(NAER_ERR_CAPACITY, "a pack is limited to 2^32 neurons");
+    n->N = (uint32_t)N;
+    if (stdp_seen) {
+        n->desc.stdp = stdp;
+        if (!(stdp.tau_plus_ms > 0.0f) || !(stdp.tau_minus_ms > 0.0f) || !(stdp.w_max >= stdp.w_min))
+            return fail(NAER_ERR_INVALID, "STDP needs tau_plus, tau_minus > 0 and w_max >= w_min");
+    }
+    n->desc.record_capacity = (uint32_t)std::min<uint64_t>(rec, 0xFFFFFFFFull);
+
+    const uint32_t P = (uint32_t)n->pops.size();
+    n->pp.resize(P);
+    n->cdf.assign((size_t)P * NAER_DRIVE_TABLE, 2.0f);
+    std::vector<uint32_t> pois_in_net(n->nets.size(), 0);
+    uint64_t acc = 0;
+    for (uint32_t p = 0; p < P; ++p) {
         const naer_pop_desc& q = n->pops[p];
+        const uint32_t ni = n->pop_net[p];
+        const naer_net_desc& d = n->net_descs[ni];
         PopParams& r = n->pp[p];
         std::memset(&r, 0, sizeof r);
-        r.begin = (uint32_t)N;
-        N += q.size;
-        r.end = (uint32_t)std::min<uint64_t>(N, 0xFFFFFFFFu);
+        r.begin = (uint32_t)acc;
+        acc += q.size;
+        r.end = (uint32_t)acc;
         r.kind = q.kind;
         r.cdf_offset = p * NAER_DRIVE_TABLE;
+        r.net_base = n->nets[ni].neuron_begin;
+        r.seed_lo = (uint32_t)d.seed;
+        r.seed_hi = (uint32_t)(d.seed >> 32);
+        r.pois_seed = d.poisson_seed;
+        r.t_start = n->nets[ni].t_start;
         if (q.kind == NAER_POP_POISSON) {
-            r.pois_base = (uint32_t)pois;
-            pois += q.size;
+            if (kEpochUs % n->dt_us != 0)
+                return fail(NAER_ERR_INVALID, "POISSON sources need dt dividing the 1500 us epoch");
+            r.pois_base = pois_in_net[ni];
+            pois_in_net[ni] += q.size;
             r.neg_rate_per_us = -((double)q.src_rate_hz / 1000000.0);
         } else if (q.kind == NAER_POP_LIF) {
             if (!(q.tau_m_ms > 0.0f))
                 return fail(NAER_ERR_INVALID, "population %u: tau_m must be > 0", p);
-            r.P = (float)std::exp(-(double)d.dt_ms / (double)q.tau_m_ms);
+            r.P = (float)std::exp(-(double)d0.dt_ms / (double)q.tau_m_ms);
             r.v_th = q.v_th;
             r.v_reset = q.v_reset;
             r.v_rest = q.v_rest;
             r.v0_min = q.v0_min;
             r.v0_max = q.v0_max;
             r.w_ext = q.ext_weight;
-            r.n_ref = (int32_t)std::llround((double)q.t_ref_ms / (double)d.dt_ms);
+            r.n_ref = (int32_t)std::llround((double)q.t_ref_ms / (double)d0.dt_ms);
             r.has_drive = q.ext_rate_hz > 0.0f ? 1u : 0u;
-            const double lam = (double)q.ext_rate_hz * (double)d.dt_ms * 1e-3;
+            const double lam = (double)q.ext_rate_hz * (double)d0.dt_ms * 1e-3;
             if (!(lam >= 0.0 && lam <= 20.0))
                 return fail(NAER_ERR_INVALID, "population %u: drive rate*dt must be in [0, 20]", p);
             // inversion table, see NAER_DRIVE_TABLE in naer.h
             float* c = &n->cdf[(size_t)p * NAER_DRIVE_TABLE];
-            double pk = std::exp(-lam), acc = pk;
+            double pk = std::exp(-lam), cum = pk;
             for (uint32_t k = 0;; ++k) {
-                const float f = (float)acc;
+                const float f = (float)cum;
                 if (f >= 1.0f || k == NAER_DRIVE_TABLE - 1) {
                     c[k] = 2.0f;
                     break;
                 }
                 c[k] = f;
                 pk = pk * lam / (double)(k + 1);
-                acc = acc + pk;
+                cum = cum + pk;
             }
         } else {
             return fail(NAER_ERR_INVALID, "population %u: unknown kind %u", p, q.kind);
         }
     }
-    if (N == 0 || N > (1ull << 24))
-        return fail(NAER_ERR_CAPACITY, "neuron count must be in [1, 2^24] (got %llu)",
-                    (unsigned long long)N);
-    n->N = (uint32_t)N;
-    n->n_pois = (uint32_t)pois;
-    if (pois && kEpochUs % n->dt_us != 0)
-        return fail(NAER_ERR_INVALID, "POISSON sources need dt dividing the 1500 us epoch");
-
-    n->dmin = 0xFFFFFFFFu;
-    n->dmax = 1;
-    for (uint32_t c = 0; c < d.n_conns; ++c) {
-        const naer_conn_desc& k = n->conns[c];
-        if (k.src_pop >= d.n_pops || k.dst_pop >= d.n_pops)
-            return fail(NAER_ERR_INVALID, "connection %u: population index out of range", c);
-        if (n->pops[k.dst_pop].kind != NAER_POP_LIF)
-            return fail(NAER_ERR_INVALID, "connection %u: target population must be LIF", c);
-        const uint32_t q = k.delay_quantum ? k.delay_quantum : 1;
-        if (k.delay_min < 1 || k.delay_max < k.delay_min || k.delay_max > 127 ||
-            (k.delay_max - k.delay_min) % q)
-            return fail(NAER_ERR_INVALID,
-                        "connection %u: delays must satisfy 1 <= min <= max <= 127 on the quantum grid", c);
-        if (k.plastic && n->pops[k.src_pop].kind != NAER_POP_LIF)
-            return fail(NAER_ERR_INVALID, "connection %u: plastic connections need a LIF source", c);
-        if (k.rule != NAER_RULE_BERNOULLI && k.rule != NAER_RULE_FIXED_INDEGREE)
-            return fail(NAER_ERR_INVALID, "connection %u: unknown rule %u", c, k.rule);
-        if (k.rule == NAER_RULE_BERNOULLI && !(k.probability >= 0.0 && k.probability <= 1.0))
-            return fail(NAER_ERR_INVALID, "connection %u: probability must be in [0, 1]", c);
-        n->dmin = std::min(n->dmin, k.delay_min);
-        n->dmax = std::max(n->dmax, k.delay_max);
-        if (k.plastic)
-            n->plastic = true;
+    for (uint32_t p = 0; p < P; ++p)
+        n->pp[p].n_pois = pois_in_net[n->pop_net[p]];
+    for (uint32_t p = 0; p < P; ++p)
+        if (n->pops[p].kind == NAER_POP_POISSON)
+            n->n_pois += n->pops[p].size;
+    // first population of every update block
+    const uint32_t nblk = (n->N + kBlock - 1) / kBlock;
+    n->blk_pop_h.resize(nblk);
+    uint32_t p = 0;
+    for (uint32_t b = 0; b < nblk; ++b) {
+        const uint32_t j = b * kBlock;
+        while (p + 1 < P && j >= n->pp[p + 1].begin)
+            ++p;
+        // skip empty populations that start at the same neuron
+        n->blk_pop_h[b] = p;
     }
     n->D = n->dmax + 1;
     // Steps per update launch: inputs of a batch were all scattered before
     // it starts when B <= minimum delay.  Plastic nets interleave scatter and
     // potentiation every step.
     n->B = n->plastic ? 1u : std::min<uint32_t>(kMaxBatch, n->dmin == 0xFFFFFFFFu ? kMaxBatch : n->dmin);
-    if (n->plastic) {
-        const naer_stdp_desc& s = d.stdp;
-        if (!(s.tau_plus_ms > 0.0f) || !(s.tau_minus_ms > 0.0f) || !(s.w_max >= s.w_min))
-            return fail(NAER_ERR_INVALID, "STDP needs tau_plus, tau_minus > 0 and w_max >= w_min");
-    }
     return NAER_OK;
 }
 
@@ -292,26 +379,23 @@ UpdateArgs make_update_args(naer_net* n)
     a.rec = n->rec;
     a.st = n->st;
     a.cdf = n->d_cdf;
+    a.pops = n->d_pops;
+    a.blk_pop = n->d_blk_pop;
     a.probe = n->probe;
     a.rec_cap = n->rec_cap;
     a.probe_t0 = n->probe_t0;
     a.probe_n = n->probe_n;
-    a.seed = n->desc.seed;
     a.seg_cap = n->seg_cap;
     a.n_seg = n->n_seg;
     a.N = n->N;
     a.D = n->D;
     a.B = n->B;
     a.dt_us = n->dt_us;
-    a.n_pois = n->n_pois;
-    a.pois_seed = n->desc.poisson_seed;
+    a.n_pops = (uint32_t)n->pp.size();
     if (n->plastic) {
         a.dec_p = (float)std::exp(-(double)n->desc.dt_ms / (double)n->desc.stdp.tau_plus_ms);
         a.dec_m = (float)std::exp(-(double)n->desc.dt_ms / (double)n->desc.stdp.tau_minus_ms);
     }
-    a.n_pops = n->desc.n_pops;
-    for (uint32_t p = 0; p < n->desc.n_pops; ++p)
-        a.pops[p] = n->pp[p];
     return a;
 }
 
@@ -325,7 +409,7 @@ ConnArgs make_conn_args(naer_net* n, uint32_t c)
     a.dst_begin = n->pp[k.dst_pop].begin;
     a.n_dst = n->pops[k.dst_pop].size;
     a.K = k.indegree;
-    a.conn_index = c;
+    a.conn_index = n->conn_local[c];
     a.seed = k.seed;
     a.p = k.probability;
     a.weight = k.weight;
@@ -363,10 +447,11 @@ int scan_rows(naer_net* n, const uint32_t* d_deg, unsigned long long* d_ptr, uin
     return NAER_OK;
 }
 
-int launch_timed(naer_net* n, int kind, hipError_t (*fn)(naer_net*, void*), void* arg)
+int launch_timed(naer_net* n, int kind, hipError_t (*fn)(naer_net*, void*), void* arg, bool timed)
 {
     hipEvent_t a = nullptr, b = nullptr;
-    if (n->profiling) {
+    timed = timed && n->profiling;
+    if (timed) {
         a = n->take_event();
         b = n->take_event();
         HIP_TRY(hipEventRecord(a, n->stream));
@@ -375,7 +460,7 @@ int launch_timed(naer_net* n, int kind, hipError_t (*fn)(naer_net*, void*), void
     if (e != hipSuccess)
         return fail(NAER_ERR_HIP, "kernel launch failed: %s", hipGetErrorString(e));
     n->kernel_calls[kind] += 1;
-    if (n->profiling) {
+    if (timed) {
         HIP_TRY(hipEventRecord(b, n->stream));
         n->pending.push_back(PendingTimer{ kind, a, b });
     }
@@ -413,23 +498,23 @@ int naer_ctx_destroy(naer_ctx* ctx)
     return NAER_OK;
 }
 
-int naer_net_create(naer_ctx* ctx, const naer_net_desc* desc, naer_net** out)
+int naer_pack_create(naer_ctx* ctx, const naer_net_desc* descs, uint32_t n_nets, naer_net** out)
 {
-    if (!ctx || !desc || !out)
-        return fail(NAER_ERR_INVALID, "NULL argument");
-    if (desc->n_pops && !desc->pops)
-        return fail(NAER_ERR_INVALID, "pops is NULL");
-    if (desc->n_conns && !desc->conns)
-        return fail(NAER_ERR_INVALID, "conns is NULL");
+    if (!ctx || !descs || !out || n_nets == 0)
+        return fail(NAER_ERR_INVALID, "NULL argument or empty pack");
     naer_net* n = new naer_net;
     n->ctx = ctx;
-    n->desc = *desc;
-    n->pops.assign(desc->pops, desc->pops + desc->n_pops);
-    if (desc->n_conns)
-        n->conns.assign(desc->conns, desc->conns + desc->n_conns);
+    n->net_descs.assign(descs, descs + n_nets);
+    int rc = validate_and_derive(n);
+    // the caller's arrays are not referenced after creation
+    for (auto& d : n->net_descs) {
+        d.pops = nullptr;
+        d.conns = nullptr;
+    }
     n->desc.pops = n->pops.data();
     n->desc.conns = n->conns.data();
-    int rc = validate_and_derive(n);
+    n->desc.n_pops = (uint32_t)n->pops.size();
+    n->desc.n_conns = (uint32_t)n->conns.size();
     if (rc != NAER_OK) {
         delete n;
         return rc;
@@ -442,6 +527,28 @@ int naer_net_create(naer_ctx* ctx, const naer_net_desc* desc, naer_net** out)
         return fail(NAER_ERR_HIP, "stream creation failed: %s", hipGetErrorString(e));
     }
     *out = n;
+    return NAER_OK;
+}
+
+int naer_net_create(naer_ctx* ctx, const naer_net_desc* desc, naer_net** out)
+{
+    if (!desc)
+        return fail(NAER_ERR_INVALID, "NULL argument");
+    return naer_pack_create(ctx, desc, 1, out);
+}
+
+int naer_pack_layout(naer_net* n, uint32_t* neuron_begin, uint32_t* n_neurons, uint64_t* t_start)
+{
+    if (!n)
+        return fail(NAER_ERR_INVALID, "net is NULL");
+    for (size_t i = 0; i < n->nets.size(); ++i) {
+        if (neuron_begin)
+            neuron_begin[i] = n->nets[i].neuron_begin;
+        if (n_neurons)
+            n_neurons[i] = n->nets[i].n_neurons;
+        if (t_start)
+            t_start[i] = n->nets[i].t_start;
+    }
     return NAER_OK;
 }
 
@@ -461,7 +568,7 @@ int naer_net_build(naer_net* n)
     if ((rc = n->alloc(&deg, N)) || (rc = n->alloc(&n->row_ptr, (size_t)N + 1)))
         return rc;
     HIP_TRY(hipMemsetAsync(deg, 0, N * sizeof(uint32_t), n->stream));
-    for (uint32_t c = 0; c < n->desc.n_conns; ++c) {
+    for (uint32_t c = 0; c < n->conns.size(); ++c) {
         ConnArgs a = make_conn_args(n, c);
         HIP_TRY(launch_conn(a, n->conns[c].rule, false, deg, nullptr, nullptr, nullptr, n->stream));
     }
@@ -478,7 +585,7 @@ int naer_net_build(naer_net* n)
         return rc;
     HIP_TRY(hipMemcpyAsync(cursor, n->row_ptr, N * sizeof(unsigned long long), hipMemcpyDeviceToDevice,
                            n->stream));
-    for (uint32_t c = 0; c < n->desc.n_conns; ++c) {
+    for (uint32_t c = 0; c < n->conns.size(); ++c) {
         ConnArgs a = make_conn_args(n, c);
         HIP_TRY(launch_conn(a, n->conns[c].rule, true, nullptr, cursor, n->col, n->w, n->stream));
     }
@@ -531,8 +638,13 @@ int naer_net_build(naer_net* n)
     n->rec_cap = n->desc.record_capacity;
     if ((rc = n->alloc(&n->rec, n->rec_cap ? n->rec_cap : 1)))
         return rc;
-    if ((rc = n->alloc(&n->st, 1)) || (rc = n->alloc(&n->d_cdf, n->cdf.size())))
+    if ((rc = n->alloc(&n->st, 1)) || (rc = n->alloc(&n->d_cdf, n->cdf.size())) ||
+        (rc = n->alloc(&n->d_pops, n->pp.size())) || (rc = n->alloc(&n->d_blk_pop, n->blk_pop_h.size())))
         return rc;
+    HIP_TRY(hipMemcpyAsync(n->d_pops, n->pp.data(), n->pp.size() * sizeof(PopParams), hipMemcpyHostToDevice,
+                           n->stream));
+    HIP_TRY(hipMemcpyAsync(n->d_blk_pop, n->blk_pop_h.data(), n->blk_pop_h.size() * sizeof(uint32_t),
+                           hipMemcpyHostToDevice, n->stream));
     HIP_TRY(hipMemsetAsync(n->st, 0, sizeof(DevState), n->stream));
     HIP_TRY(hipMemcpyAsync(n->d_cdf, n->cdf.data(), n->cdf.size() * sizeof(float), hipMemcpyHostToDevice,
                            n->stream));
@@ -564,12 +676,8 @@ static hipError_t do_potentiate(naer_net* n, void* arg)
     return launch_potentiate(*a, 256, n->stream);
 }
 
-int naer_net_step(naer_net* n, uint32_t n_steps)
+static int enqueue_steps(naer_net* n, uint32_t n_steps, bool timed)
 {
-    if (!n)
-        return fail(NAER_ERR_INVALID, "net is NULL");
-    if (!n->built)
-        return fail(NAER_ERR_STATE, "net not built");
     UpdateArgs ua = make_update_args(n);
     ScatterArgs sa;
     std::memset(&sa, 0, sizeof sa);
@@ -612,15 +720,64 @@ int naer_net_step(naer_net* n, uint32_t n_steps)
         ua.B = b;
         sa.B = b;
         pa.B = b;
-        if ((rc = launch_timed(n, NAER_K_UPDATE, do_update, &ua)))
+        if ((rc = launch_timed(n, NAER_K_UPDATE, do_update, &ua, timed)))
             return rc;
-        if ((rc = launch_timed(n, NAER_K_SCATTER, do_scatter, &sa)))
+        if ((rc = launch_timed(n, NAER_K_SCATTER, do_scatter, &sa, timed)))
             return rc;
-        if (n->plastic && (rc = launch_timed(n, NAER_K_POTENTIATE, do_potentiate, &pa)))
+        if (n->plastic && (rc = launch_timed(n, NAER_K_POTENTIATE, do_potentiate, &pa, timed)))
             return rc;
         n_steps -= b;
         n->steps_enqueued += b;
     }
+    return NAER_OK;
+}
+
+int naer_net_step(naer_net* n, uint32_t n_steps)
+{
+    if (!n)
+        return fail(NAER_ERR_INVALID, "net is NULL");
+    if (!n->built)
+        return fail(NAER_ERR_STATE, "net not built");
+    if (n->graph_chunk) {
+        if (!n->graph_exec && n_steps >= n->graph_chunk) {
+            // capture one chunk; its kernels read the clock from the device,
+            // so the same graph replays for every chunk
+            hipGraph_t g = nullptr;
+            HIP_TRY(hipStreamBeginCapture(n->stream, hipStreamCaptureModeThreadLocal));
+            const uint64_t before = n->steps_enqueued;
+            int rc = enqueue_steps(n, n->graph_chunk, false);
+            hipError_t e = hipStreamEndCapture(n->stream, &g);
+            n->steps_enqueued = before;
+            if (rc)
+                return rc;
+            if (e != hipSuccess)
+                return fail(NAER_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+            e = hipGraphInstantiate(&n->graph_exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess)
+                return fail(NAER_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
+        }
+        while (n->graph_exec && n_steps >= n->graph_chunk) {
+            HIP_TRY(hipGraphLaunch(n->graph_exec, n->stream));
+            n_steps -= n->graph_chunk;
+            n->steps_enqueued += n->graph_chunk;
+        }
+    }
+    return n_steps ? enqueue_steps(n, n_steps, true) : NAER_OK;
+}
+
+int naer_net_set_graph(naer_net* n, uint32_t chunk_steps)
+{
+    if (!n)
+        return fail(NAER_ERR_INVALID, "net is NULL");
+    int rc = naer_net_sync(n);
+    if (rc)
+        return rc;
+    if (n->graph_exec) {
+        HIP_TRY(hipGraphExecDestroy(n->graph_exec));
+        n->graph_exec = nullptr;
+    }
+    n->graph_chunk = chunk_steps;
     return NAER_OK;
 }
 
@@ -758,6 +915,10 @@ int naer_net_set_probe(naer_net* n, uint64_t t_begin, uint32_t n_steps)
     int rc = naer_net_sync(n);
     if (rc)
         return rc;
+    if (n->graph_exec) {
+        HIP_TRY(hipGraphExecDestroy(n->graph_exec));
+        n->graph_exec = nullptr;
+    }
     if (n->probe) {
         n->release(n->probe, (size_t)n->probe_n * n->N * sizeof(float));
         n->probe = nullptr;

@@ -248,6 +248,10 @@ class Network:
         check(_abi.lib().naer_net_sync(self._h()))
         self._refresh_timers()
 
+    def set_graph(self, chunk_steps):
+        """Replay steps as captured HIP graphs of chunk_steps steps (0 = off)."""
+        check(_abi.lib().naer_net_set_graph(self._h(), int(chunk_steps)))
+
     def set_profiling(self, on=True):
         check(_abi.lib().naer_net_set_profiling(self._h(), 1 if on else 0))
 

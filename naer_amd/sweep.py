@@ -1,0 +1,139 @@
+"""Batched parameter sweep: many independent Brunel networks per launch.
+
+The sweep workload: n_nets balanced networks with sizes log-uniform in
+[n_min, n_max], in-degree 10% of N (80% from E), J_E = 0.1 mV, g uniform in
+[3, 7], Poisson drive uniform in [10, 25] kHz x 0.1 mV, delay 1.5 ms, seed
+base+i, every even-numbered network with pair STDP on E->E; each runs
+`steps` steps of 0.1 ms.
+
+Two ways to run it:
+  packed      all static networks in one pack and all plastic networks in a
+              second pack (plasticity forces one step per launch), the two packs
+              on their own streams, steps enqueued in graph-captured chunks;
+  sequential  the plain loop: one network per launch sequence, one after the
+              other.
+In deterministic mode every network's raster is bit-identical either way.
+
+    python -m naer_amd.sweep --nets 256 --steps 5000 [--mode det|fast]
+"""
+import argparse
+import json
+import math
+import time
+
+import numpy as np
+
+from . import _abi, models
+from .pack import Pack
+
+SWEEP_SEED = 0x5EED5EED
+
+
+def sweep_specs(n_nets=256, n_min=1_000, n_max=50_000, seed=SWEEP_SEED):
+    """Parameters of every network of the sweep (host-side draws, fixed seed)."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(n_nets):
+        n = int(round(math.exp(rng.uniform(math.log(n_min), math.log(n_max)))))
+        specs.append(dict(index=i, n=n, indegree=max(10, int(round(0.1 * n))), g=float(rng.uniform(3.0, 7.0)),
+                          drive_hz=float(rng.uniform(10_000.0, 25_000.0)), plastic=(i % 2 == 0),
+                          seed=(seed * 1000003 + i) & 0xFFFFFFFFFFFFFFFF))
+    return specs
+
+
+def make_network(spec, mode=_abi.MODE_DETERMINISTIC, record_capacity=0):
+    return models.balanced(spec["n"], spec["indegree"], 0.1, spec["g"], spec["drive_hz"], 0.1,
+                           delay_ms=1.5, plastic=spec["plastic"], seed=spec["seed"], mode=mode,
+                           record_capacity=record_capacity)
+
+
+def expected_record(spec, steps, rate_bound_hz=500.0):
+    """Record buffer large enough for a refractory-limited rate."""
+    return int(spec["n"] * rate_bound_hz * steps * 1e-4) + 1024
+
+
+def run_packed(specs, steps, mode=_abi.MODE_DETERMINISTIC, chunk=100, record=True, graphs=True):
+    groups = [[s for s in specs if not s["plastic"]], [s for s in specs if s["plastic"]]]
+    packs = []
+    t_build = time.perf_counter()
+    for g in groups:
+        if not g:
+            continue
+        nets = [make_network(s, mode, expected_record(s, steps) if record else 0) for s in g]
+        p = Pack(nets)
+        p.build()
+        if graphs:
+            p.set_graph(chunk)
+        packs.append((g, p))
+    for _, p in packs:
+        p.sync()
+    build_s = time.perf_counter() - t_build
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps:
+        c = min(chunk, steps - done)
+        for _, p in packs:
+            p.run(c)
+        done += c
+    for _, p in packs:
+        p.sync()
+    run_s = time.perf_counter() - t0
+    rasters = {}
+    stats = []
+    for g, p in packs:
+        st = p.stats()
+        stats.append(dict(nets=len(g), neurons=int(st.n_neurons), synapses=int(st.n_synapses),
+                          spikes=int(st.spikes), events=int(st.events), batch=int(st.batch)))
+        if record:
+            for s, r in zip(g, p.spikes()):
+                rasters[s["index"]] = r
+        p.close()
+    return dict(build_s=build_s, run_s=run_s, packs=stats, rasters=rasters)
+
+
+def run_sequential(specs, steps, mode=_abi.MODE_DETERMINISTIC, record=True):
+    rasters = {}
+    build_s = run_s = 0.0
+    events = 0
+    for s in specs:
+        t = time.perf_counter()
+        net = make_network(s, mode, expected_record(s, steps) if record else 0)
+        net.compile()
+        net.finish()
+        build_s += time.perf_counter() - t
+        t = time.perf_counter()
+        net.run(steps)
+        net.finish()
+        run_s += time.perf_counter() - t
+        events += int(net.stats().events)
+        if record:
+            rasters[s["index"]] = net.spikes()
+        net.close()
+    return dict(build_s=build_s, run_s=run_s, events=events, rasters=rasters)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--nets", type=int, default=256)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--n-max", type=int, default=50_000)
+    ap.add_argument("--mode", choices=["det", "fast"], default="det")
+    ap.add_argument("--no-sequential", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true")
+    a = ap.parse_args()
+    mode = _abi.MODE_DETERMINISTIC if a.mode == "det" else _abi.MODE_FAST
+    specs = sweep_specs(a.nets, n_max=a.n_max)
+    packed = run_packed(specs, a.steps, mode, graphs=not a.no_graphs)
+    out = dict(nets=a.nets, steps=a.steps, mode=a.mode, packed_run_s=packed["run_s"],
+               packed_build_s=packed["build_s"], packed_jobs_per_s=a.nets / packed["run_s"],
+               packs=packed["packs"])
+    if not a.no_sequential:
+        seq = run_sequential(specs, a.steps, mode)
+        ident = sum(int(np.array_equal(packed["rasters"][i], seq["rasters"][i])) for i in range(a.nets))
+        out.update(seq_run_s=seq["run_s"], seq_build_s=seq["build_s"], seq_jobs_per_s=a.nets / seq["run_s"],
+                   speedup=seq["run_s"] / packed["run_s"], rasters_identical=ident)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
