@@ -232,7 +232,39 @@ int naer_net_create(naer_ctx* ctx, const naer_net_desc* desc, naer_net** out);
  * a pack; naer_net_create is naer_pack_create with one network.  No
  * counterpart in the reference (one network per process, src/main.cpp). */
 int naer_pack_create(naer_ctx* ctx, const naer_net_desc* descs, uint32_t n_nets, naer_net** out);
+/* Per network (pack) or slot (arena): first neuron, neuron count (0 for a
+ * retired slot) and the pack step at which its clock started. */
 int naer_pack_layout(naer_net* pack, uint32_t* neuron_begin, uint32_t* n_neurons, uint64_t* t_start);
+
+/* An arena: preallocated capacity in which networks (serving jobs) are
+ * admitted and retired between steps; the form used for continuous batching.
+ * Each admitted job is built on the device in free neuron and synapse ranges,
+ * starts at the arena's next step with its own clock at 0, and gives the same
+ * results as when simulated alone (bit for bit in NAER_MODE_DETERMINISTIC).
+ * Jobs must use dt and mode of the arena, delays within [delay_min,
+ * delay_max] steps, rows of at most max_row synapses and, when plastic, the
+ * arena's STDP parameters.  No counterpart in the reference. */
+typedef struct naer_arena_desc {
+    uint64_t max_neurons;  /* neuron capacity (each job is rounded up to 256) */
+    uint64_t max_synapses; /* synapse capacity                               */
+    uint64_t max_plastic;  /* plastic-synapse capacity (reverse index)       */
+    float dt_ms;
+    uint32_t mode;
+    uint32_t delay_min; /* steps */
+    uint32_t delay_max;
+    uint32_t max_row;   /* longest CSR row a job may have (<= 8192)        */
+    uint32_t plastic;   /* 1: jobs may carry STDP; the arena then steps one dt per launch */
+    naer_stdp_desc stdp;
+    uint32_t record_capacity;
+    uint32_t poisson;   /* 1: jobs may contain POISSON sources (larger spike lists) */
+} naer_arena_desc;
+
+int naer_arena_create(naer_ctx* ctx, const naer_arena_desc* desc, naer_net** out);
+/* NAER_ERR_CAPACITY when the job does not fit now (the caller may retry
+ * after retiring jobs); *slot identifies the job in naer_pack_layout and
+ * naer_arena_retire.  Synchronises the arena. */
+int naer_arena_admit(naer_net* arena, const naer_net_desc* job, uint32_t* slot);
+int naer_arena_retire(naer_net* arena, uint32_t slot);
 
 /* Device connectivity build + state initialisation; replaces
  * Network::compile (src/Network.cpp:267-308) and allocateConnections

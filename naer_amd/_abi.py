@@ -70,6 +70,14 @@ class NetDesc(ctypes.Structure):
     ]
 
 
+class ArenaDesc(ctypes.Structure):
+    _fields_ = [
+        ("max_neurons", u64), ("max_synapses", u64), ("max_plastic", u64),
+        ("dt_ms", f32), ("mode", u32), ("delay_min", u32), ("delay_max", u32), ("max_row", u32),
+        ("plastic", u32), ("stdp", StdpDesc), ("record_capacity", u32), ("poisson", u32),
+    ]
+
+
 class Spike(ctypes.Structure):
     _fields_ = [("step", u32), ("neuron", u32), ("time_us", u32), ("_pad", u32)]
 
@@ -102,6 +110,9 @@ _SIGS = {
     "naer_net_create": [ctypes.c_void_p, ctypes.POINTER(NetDesc), ctypes.POINTER(ctypes.c_void_p)],
     "naer_pack_create": [ctypes.c_void_p, ctypes.POINTER(NetDesc), u32, ctypes.POINTER(ctypes.c_void_p)],
     "naer_pack_layout": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+    "naer_arena_create": [ctypes.c_void_p, ctypes.POINTER(ArenaDesc), ctypes.POINTER(ctypes.c_void_p)],
+    "naer_arena_admit": [ctypes.c_void_p, ctypes.POINTER(NetDesc), ctypes.POINTER(u32)],
+    "naer_arena_retire": [ctypes.c_void_p, u32],
     "naer_net_build": [ctypes.c_void_p],
     "naer_net_step": [ctypes.c_void_p, u32],
     "naer_net_sync": [ctypes.c_void_p],

@@ -108,9 +108,9 @@ __global__ __launch_bounds__(kBlock) void k_update(const UpdateArgs a)
         s_n = 0;
         s_events = 0;
     }
-    const uint32_t p_first = a.blk_pop[blockIdx.x];
+    const uint32_t p_first = a.n_pops ? a.blk_pop[blockIdx.x] : 0u;
     {
-        const uint32_t np = min(kStagePops, a.n_pops - p_first);
+        const uint32_t np = a.n_pops ? min(kStagePops, a.n_pops - p_first) : 0u;
         for (uint32_t i = threadIdx.x; i < np * NAER_DRIVE_TABLE; i += kBlock)
             s_cdf[i] = a.cdf[(size_t)p_first * NAER_DRIVE_TABLE + i];
     }
@@ -126,8 +126,10 @@ __global__ __launch_bounds__(kBlock) void k_update(const UpdateArgs a)
 
     const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
     unsigned long long my_events = 0;
-    if (j < a.N) {
-        const uint32_t p = find_pop(a.pops, a.n_pops, p_first, j);
+    // neurons outside every population (free ranges of an arena) stay idle
+    const uint32_t p = (j < a.N && a.n_pops) ? find_pop(a.pops, a.n_pops, p_first, j) : 0u;
+    const bool active = j < a.N && a.n_pops && j >= a.pops[p].begin && j < a.pops[p].end;
+    if (active) {
         const PopParams pp = a.pops[p];
         const uint32_t jl = j - pp.net_base;                 // neuron id within its net
         const unsigned long long t0l = t0 - pp.t_start;      // net-local step of t0
@@ -184,7 +186,8 @@ __global__ __launch_bounds__(kBlock) void k_update(const UpdateArgs a)
                 if (a.probe && t - a.probe_t0 < a.probe_n)
                     a.probe[(size_t)(t - a.probe_t0) * a.N + j] = I;
                 if (PLASTIC) {
-                    const float sp = ((long long)last == (long long)t - 1) ? 1.0f : 0.0f;
+                    // last < 0: never fired
+                    const float sp = (last >= 0 && (long long)last == (long long)t - 1) ? 1.0f : 0.0f;
                     const float xa = xh + sp;
                     xh = xa * a.dec_p;
                     const float ya = yh + sp;
@@ -454,10 +457,10 @@ __global__ __launch_bounds__(kBlock) void k_conn_bernoulli(const ConnArgs c, uin
 // Canonical row order: ascending (word << 32 | weight bits), bitonic sort of
 // one row per block in LDS.
 __global__ __launch_bounds__(512) void k_sort_rows(const unsigned long long* row_ptr, uint32_t* col,
-                                                   float* w, uint32_t N)
+                                                   float* w, uint32_t row_begin, uint32_t row_end)
 {
     extern __shared__ unsigned long long s_keys[];
-    for (uint32_t r = blockIdx.x; r < N; r += gridDim.x) {
+    for (uint32_t r = row_begin + blockIdx.x; r < row_end; r += gridDim.x) {
         const unsigned long long beg = row_ptr[r];
         const uint32_t len = (uint32_t)(row_ptr[r + 1] - beg);
         if (len <= 1)
@@ -496,11 +499,12 @@ __global__ __launch_bounds__(512) void k_sort_rows(const unsigned long long* row
 
 template <bool PLACE>
 __global__ __launch_bounds__(kBlock) void k_reverse(const unsigned long long* row_ptr,
-                                                    const uint32_t* col, uint32_t N, uint32_t* rdeg,
+                                                    const uint32_t* col, uint32_t row_begin,
+                                                    uint32_t row_end, uint32_t* rdeg,
                                                     unsigned long long* rcursor, uint32_t* rev_pos,
                                                     uint32_t* rev_src)
 {
-    for (uint32_t s = blockIdx.x; s < N; s += gridDim.x) {
+    for (uint32_t s = row_begin + blockIdx.x; s < row_end; s += gridDim.x) {
         const unsigned long long re = row_ptr[s + 1];
         for (unsigned long long i = row_ptr[s] + threadIdx.x; i < re; i += kBlock) {
             const uint32_t c = col[i];
@@ -518,18 +522,24 @@ __global__ __launch_bounds__(kBlock) void k_reverse(const unsigned long long* ro
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_init_state(const UpdateArgs a)
+// Initial state of neurons [j_begin, j_end): potentials from the V0 stream,
+// empty refractory/trace/POISSON state and cleared ring cells.
+template <bool DET>
+__global__ __launch_bounds__(kBlock) void k_init_state(const UpdateArgs a, uint32_t j_begin, uint32_t j_end)
 {
-    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
-    if (j >= a.N)
+    const uint32_t j = j_begin + blockIdx.x * kBlock + threadIdx.x;
+    if (j >= j_end)
         return;
-    const PopParams pp = a.pops[find_pop(a.pops, a.n_pops, a.blk_pop[blockIdx.x], j)];
     float v = 0.0f;
-    if (pp.kind == NAER_POP_LIF) {
-        const U4 r = philox(U4{ j - pp.net_base, 0u, NAER_STREAM_V0, 0u }, pp.seed_lo, pp.seed_hi);
-        const float span = pp.v0_max - pp.v0_min;
-        const float x = span * unit24(r.x);
-        v = pp.v0_min + x;
+    if (a.n_pops) {
+        const uint32_t p = find_pop(a.pops, a.n_pops, a.blk_pop[j / kBlock], j);
+        const PopParams pp = a.pops[p];
+        if (pp.kind == NAER_POP_LIF && j >= pp.begin && j < pp.end) {
+            const U4 r = philox(U4{ j - pp.net_base, 0u, NAER_STREAM_V0, 0u }, pp.seed_lo, pp.seed_hi);
+            const float span = pp.v0_max - pp.v0_min;
+            const float x = span * unit24(r.x);
+            v = pp.v0_min + x;
+        }
     }
     a.V[j] = v;
     a.ref[j] = 0;
@@ -540,6 +550,10 @@ __global__ __launch_bounds__(kBlock) void k_init_state(const UpdateArgs a)
     }
     a.pnext[j] = 0;
     a.pcnt[j] = 0;
+    using RingT = typename std::conditional<DET, long long, float>::type;
+    RingT* ring = (RingT*)a.ring;
+    for (uint32_t sl = 0; sl < a.D; ++sl)
+        ring[(size_t)sl * a.N + j] = (RingT)0;
 }
 
 // ---------------------------------------------------------------------------
@@ -644,9 +658,11 @@ hipError_t launch_conn(const ConnArgs& c, uint32_t rule, bool place, uint32_t* d
     return hipGetLastError();
 }
 
-hipError_t launch_sort_rows(const unsigned long long* row_ptr, uint32_t* col, float* w, uint32_t N,
-                            uint32_t max_row, hipStream_t s)
+hipError_t launch_sort_rows(const unsigned long long* row_ptr, uint32_t* col, float* w, uint32_t row_begin,
+                            uint32_t row_end, uint32_t max_row, hipStream_t s)
 {
+    if (row_end <= row_begin || max_row <= 1)
+        return hipSuccess;
     uint32_t L = 1;
     while (L < max_row)
         L <<= 1;
@@ -657,28 +673,38 @@ hipError_t launch_sort_rows(const unsigned long long* row_ptr, uint32_t* col, fl
         if (e != hipSuccess)
             return e;
     }
-    const dim3 grid(N < 65536u ? N : 65536u);
-    hipLaunchKernelGGL(k_sort_rows, grid, dim3(512), lds, s, row_ptr, col, w, N);
+    const uint32_t rows = row_end - row_begin;
+    const dim3 grid(rows < 65536u ? rows : 65536u);
+    hipLaunchKernelGGL(k_sort_rows, grid, dim3(512), lds, s, row_ptr, col, w, row_begin, row_end);
     return hipGetLastError();
 }
 
 hipError_t launch_reverse(bool place, const unsigned long long* row_ptr, const uint32_t* col,
-                          uint32_t N, uint32_t* rdeg, unsigned long long* rcursor, uint32_t* rev_pos,
-                          uint32_t* rev_src, hipStream_t s)
+                          uint32_t row_begin, uint32_t row_end, uint32_t* rdeg, unsigned long long* rcursor,
+                          uint32_t* rev_pos, uint32_t* rev_src, hipStream_t s)
 {
-    const dim3 grid(N < 65536u ? N : 65536u);
+    if (row_end <= row_begin)
+        return hipSuccess;
+    const uint32_t rows = row_end - row_begin;
+    const dim3 grid(rows < 65536u ? rows : 65536u);
     if (place)
-        hipLaunchKernelGGL((k_reverse<true>), grid, dim3(kBlock), 0, s, row_ptr, col, N, rdeg,
-                           rcursor, rev_pos, rev_src);
+        hipLaunchKernelGGL((k_reverse<true>), grid, dim3(kBlock), 0, s, row_ptr, col, row_begin, row_end,
+                           rdeg, rcursor, rev_pos, rev_src);
     else
-        hipLaunchKernelGGL((k_reverse<false>), grid, dim3(kBlock), 0, s, row_ptr, col, N, rdeg,
-                           rcursor, rev_pos, rev_src);
+        hipLaunchKernelGGL((k_reverse<false>), grid, dim3(kBlock), 0, s, row_ptr, col, row_begin, row_end,
+                           rdeg, rcursor, rev_pos, rev_src);
     return hipGetLastError();
 }
 
-hipError_t launch_init_state(const UpdateArgs& a, hipStream_t s)
+hipError_t launch_init_state(const UpdateArgs& a, bool det, uint32_t j_begin, uint32_t j_end, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_init_state, dim3(cdiv(a.N, kBlock)), dim3(kBlock), 0, s, a);
+    if (j_end <= j_begin)
+        return hipSuccess;
+    const dim3 grid(cdiv(j_end - j_begin, kBlock));
+    if (det)
+        hipLaunchKernelGGL((k_init_state<true>), grid, dim3(kBlock), 0, s, a, j_begin, j_end);
+    else
+        hipLaunchKernelGGL((k_init_state<false>), grid, dim3(kBlock), 0, s, a, j_begin, j_end);
     return hipGetLastError();
 }
 

@@ -1,12 +1,22 @@
-// naer_runtime.hip — host side of the C ABI (include/naer/naer.h): context
-// and network objects, the device connectivity build (Network::compile,
-// src/Network.cpp:267-569 of the reference) and the step loop
-// (Network::step, src/Network.cpp:122-256).
+// naer_runtime.hip — host side of the C ABI (include/naer/naer.h): context,
+// network/pack/arena objects, the device connectivity build (Network::compile,
+// src/Network.cpp:267-569 of the reference) and the step loop (Network::step,
+// src/Network.cpp:122-256).
+//
+// One object type serves the three forms of the ABI:
+//   net    one network (a pack of one)
+//   pack   several networks fixed at creation, laid out back to back
+//   arena  preallocated capacity; networks (jobs) are admitted and retired
+//          between steps, each built in place in free neuron and synapse
+//          ranges
+// Every network keeps its own seeds, local neuron ids and local clock, so its
+// results do not depend on the form or on its neighbours.
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -30,13 +40,67 @@ int fail(int code, const char* fmt, ...)
     return code;
 }
 
-#define HIP_TRY(expr)                                                                        \
-    do {                                                                                     \
-        hipError_t e_ = (expr);                                                              \
-        if (e_ != hipSuccess)                                                                \
-            return fail(e_ == hipErrorOutOfMemory ? NAER_ERR_NOMEM : NAER_ERR_HIP, "%s: %s (%s:%d)", \
-                        #expr, hipGetErrorString(e_), __FILE__, __LINE__);                   \
+#define HIP_TRY(expr)                                                                                  \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            return fail(e_ == hipErrorOutOfMemory ? NAER_ERR_NOMEM : NAER_ERR_HIP, "%s: %s (%s:%d)", #expr, \
+                        hipGetErrorString(e_), __FILE__, __LINE__);                                    \
     } while (0)
+
+// First-fit allocator of [0, capacity) ranges with coalescing.
+class RangeAlloc {
+public:
+    void reset(uint64_t capacity)
+    {
+        free_.clear();
+        if (capacity)
+            free_[0] = capacity;
+    }
+    bool take(uint64_t size, uint64_t align, uint64_t* begin)
+    {
+        if (size == 0) {
+            *begin = 0;
+            return true;
+        }
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            const uint64_t b = (it->first + align - 1) / align * align;
+            const uint64_t end = it->first + it->second;
+            if (b + size <= end) {
+                const uint64_t lo = it->first;
+                free_.erase(it);
+                if (b > lo)
+                    free_[lo] = b - lo;
+                if (b + size < end)
+                    free_[b + size] = end - (b + size);
+                *begin = b;
+                return true;
+            }
+        }
+        return false;
+    }
+    void give(uint64_t begin, uint64_t size)
+    {
+        if (size == 0)
+            return;
+        auto it = free_.emplace(begin, size).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+
+private:
+    std::map<uint64_t, uint64_t> free_;
+};
 
 }  // namespace
 
@@ -49,30 +113,50 @@ struct PendingTimer {
     hipEvent_t a, b;
 };
 
-// One network of a pack.
+// One network of a pack or arena.
 struct NetInfo {
+    naer_net_desc desc{};
+    std::vector<naer_pop_desc> pops;
+    std::vector<naer_conn_desc> conns;  // population indices local to the network
     uint32_t neuron_begin = 0, n_neurons = 0;
-    uint32_t pop_begin = 0, n_pops = 0;
-    uint32_t conn_begin = 0, n_conns = 0;
-    uint64_t t_start = 0;  // pack step at which its local time is 0
+    uint64_t syn_begin = 0, n_syn = 0;
+    uint64_t rev_begin = 0, n_rev = 0;
+    uint32_t max_row = 0;
+    uint32_t dmin = 0xFFFFFFFFu, dmax = 0;
+    bool plastic = false;
+    bool active = false;
+    uint64_t t_start = 0;
 };
 
 struct naer_net {
     naer_ctx* ctx = nullptr;
-    naer_net_desc desc{};               // pack-wide settings (dt, mode, STDP) of the first net
-    std::vector<naer_net_desc> net_descs;
+    bool arena = false;
+    bool built = false;
+    bool poisson_ok = true;  // arena: jobs may contain POISSON sources
+
+    // settings shared by every network
+    float dt_ms = 0.0f;
+    uint32_t dt_us = 0, mode = 0;
+    bool det = false;
+    bool plastic = false;  // plasticity present (pack) or allowed (arena)
+    bool stdp_set = false;
+    naer_stdp_desc stdp{};
+    uint32_t D = 0, B = 0, dmin = 0, dmax = 0;
+    uint32_t N = 0;      // neuron capacity
+    uint64_t S_cap = 0;  // synapse capacity
+    uint64_t R_cap = 0;  // plastic-synapse capacity
+    uint32_t max_row = 0;
+
     std::vector<NetInfo> nets;
-    std::vector<naer_pop_desc> pops;    // all populations, pack order
-    std::vector<naer_conn_desc> conns;  // all connections, population indices into `pops`
-    std::vector<uint32_t> conn_local;   // index of each connection within its net
-    std::vector<uint32_t> pop_net;      // net of each population
-    std::vector<uint32_t> blk_pop_h;    // first population of each update block
-    uint32_t N = 0, n_pois = 0, dt_us = 0, D = 0, B = 0, dmin = 0, dmax = 0;
-    bool plastic = false, det = false, built = false;
+    RangeAlloc neuron_alloc, syn_alloc, rev_alloc;
+
+    // population table of the active networks
     std::vector<PopParams> pp;
     std::vector<float> cdf;
-    hipStream_t stream = nullptr;
+    std::vector<uint32_t> blk_pop_h;
+    uint32_t pop_cap = 0;
 
+    hipStream_t stream = nullptr;
     std::vector<void*> allocs;
     uint64_t dev_bytes = 0;
 
@@ -83,11 +167,10 @@ struct naer_net {
     unsigned long long* row_ptr = nullptr;
     uint32_t* col = nullptr;
     float* w = nullptr;
-    uint64_t S = 0;
-    uint32_t max_row = 0;
     unsigned long long* rev_ptr = nullptr;
     uint32_t *rev_pos = nullptr, *rev_src = nullptr;
-    uint64_t n_plastic = 0;
+    uint32_t* deg = nullptr;               // build scratch, N entries
+    unsigned long long* cursor = nullptr;  // build scratch, N entries
     naer_spike *list0 = nullptr, *list1 = nullptr, *rec = nullptr;
     uint32_t seg_cap = 0, n_seg = 0;
     uint32_t* seg_count = nullptr;
@@ -105,7 +188,7 @@ struct naer_net {
     uint64_t steps_enqueued = 0;
 
     bool profiling = false;
-    uint32_t graph_chunk = 0;         // steps per captured graph (0 = plain launches)
+    uint32_t graph_chunk = 0;  // steps per captured graph (0 = plain launches)
     hipGraphExec_t graph_exec = nullptr;
     std::vector<PendingTimer> pending;
     std::vector<hipEvent_t> event_pool;
@@ -180,182 +263,210 @@ struct naer_net {
         pending.clear();
         return NAER_OK;
     }
+
+    int drop_graph()
+    {
+        if (graph_exec) {
+            HIP_TRY(hipGraphExecDestroy(graph_exec));
+            graph_exec = nullptr;
+        }
+        return NAER_OK;
+    }
 };
 
 namespace {
 
-int validate_and_derive(naer_net* n)
+// ---------------------------------------------------------------------------
+// description checks
+// ---------------------------------------------------------------------------
+int check_settings(naer_net* n, float dt_ms, uint32_t mode)
 {
-    if (n->net_descs.empty())
-        return fail(NAER_ERR_INVALID, "a pack needs at least one network");
-    n->desc = n->net_descs[0];
-    const naer_net_desc& d0 = n->desc;
-    const double dtus = (double)d0.dt_ms * 1000.0;
-    n->dt_us = (uint32_t)std::llround(dtus);
-    if (!(d0.dt_ms > 0.0f) || std::fabs(dtus - (double)n->dt_us) > 1e-3 || n->dt_us == 0)
+    const double dtus = (double)dt_ms * 1000.0;
+    const uint32_t dt_us = (uint32_t)std::llround(dtus);
+    if (!(dt_ms > 0.0f) || std::fabs(dtus - (double)dt_us) > 1e-3 || dt_us == 0)
         return fail(NAER_ERR_INVALID, "dt must be a positive whole number of microseconds");
-    if (d0.mode != NAER_MODE_FAST && d0.mode != NAER_MODE_DETERMINISTIC)
-        return fail(NAER_ERR_INVALID, "unknown mode %u", d0.mode);
-    n->det = d0.mode == NAER_MODE_DETERMINISTIC;
+    if (mode != NAER_MODE_FAST && mode != NAER_MODE_DETERMINISTIC)
+        return fail(NAER_ERR_INVALID, "unknown mode %u", mode);
+    n->dt_ms = dt_ms;
+    n->dt_us = dt_us;
+    n->mode = mode;
+    n->det = mode == NAER_MODE_DETERMINISTIC;
+    return NAER_OK;
+}
 
-    uint64_t N = 0;
-    n->dmin = 0xFFFFFFFFu;
-    n->dmax = 1;
-    bool stdp_seen = false;
-    naer_stdp_desc stdp{};
-    uint64_t rec = 0;
-    for (uint32_t ni = 0; ni < n->net_descs.size(); ++ni) {
-        const naer_net_desc& d = n->net_descs[ni];
-        if (d.n_pops == 0 || !d.pops)
-            return fail(NAER_ERR_INVALID, "network %u has no populations", ni);
-        if (d.n_conns && !d.conns)
-            return fail(NAER_ERR_INVALID, "network %u: conns is NULL", ni);
-        if (d.dt_ms != d0.dt_ms || d.mode != d0.mode)
-            return fail(NAER_ERR_INVALID, "network %u: all networks of a pack share dt and mode", ni);
-        NetInfo info;
-        info.neuron_begin = (uint32_t)N;
-        info.pop_begin = (uint32_t)n->pops.size();
-        info.n_pops = d.n_pops;
-        info.conn_begin = (uint32_t)n->conns.size();
-        info.n_conns = d.n_conns;
-        uint64_t nn = 0;
-        for (uint32_t p = 0; p < d.n_pops; ++p) {
-            n->pops.push_back(d.pops[p]);
-            n->pop_net.push_back(ni);
-            nn += d.pops[p].size;
-        }
-        if (nn == 0 || nn > (1ull << 24))
-            return fail(NAER_ERR_CAPACITY, "network %u: neuron count must be in [1, 2^24] (got %llu)", ni,
-                        (unsigned long long)nn);
-        info.n_neurons = (uint32_t)nn;
-        N += nn;
-        for (uint32_t c = 0; c < d.n_conns; ++c) {
-            naer_conn_desc k = d.conns[c];
-            if (k.src_pop >= d.n_pops || k.dst_pop >= d.n_pops)
-                return fail(NAER_ERR_INVALID, "network %u connection %u: population index out of range", ni, c);
-            if (d.pops[k.dst_pop].kind != NAER_POP_LIF)
-                return fail(NAER_ERR_INVALID, "network %u connection %u: target population must be LIF", ni, c);
-            const uint32_t q = k.delay_quantum ? k.delay_quantum : 1;
-            if (k.delay_min < 1 || k.delay_max < k.delay_min || k.delay_max > 127 ||
-                (k.delay_max - k.delay_min) % q)
-                return fail(NAER_ERR_INVALID,
-                            "network %u connection %u: delays must satisfy 1 <= min <= max <= 127 on the quantum grid",
-                            ni, c);
-            if (k.plastic && d.pops[k.src_pop].kind != NAER_POP_LIF)
-                return fail(NAER_ERR_INVALID, "network %u connection %u: plastic connections need a LIF source",
-                            ni, c);
-            if (k.rule != NAER_RULE_BERNOULLI && k.rule != NAER_RULE_FIXED_INDEGREE)
-                return fail(NAER_ERR_INVALID, "network %u connection %u: unknown rule %u", ni, c, k.rule);
-            if (k.rule == NAER_RULE_BERNOULLI && !(k.probability >= 0.0 && k.probability <= 1.0))
-                return fail(NAER_ERR_INVALID, "network %u connection %u: probability must be in [0, 1]", ni, c);
-            if (k.plastic) {
-                if (stdp_seen && std::memcmp(&stdp, &d.stdp, sizeof stdp) != 0)
-                    return fail(NAER_ERR_INVALID, "plastic networks of a pack must share the STDP parameters");
-                stdp = d.stdp;
-                stdp_seen = true;
-                n->plastic = true;
-            }
-            n->dmin = std::min(n->dmin, k.delay_min);
-            n->dmax = std::max(n->dmax, k.delay_max);
-            k.src_pop += info.pop_begin;
-            k.dst_pop += info.pop_begin;
-            n->conns.push_back(k);
-            n->conn_local.push_back(c);
-        }
-        rec += d.record_capacity;
-        n->nets.push_back(info);
-    }
-    if (N > 0xFFFFFFF0ull)
-        return fail(NAER_ERR_CAPACITY, "a pack is limited to 2^32 neurons");
-    n->N = (uint32_t)N;
-    if (stdp_seen) {
-        n->desc.stdp = stdp;
-        if (!(stdp.tau_plus_ms > 0.0f) || !(stdp.tau_minus_ms > 0.0f) || !(stdp.w_max >= stdp.w_min))
-            return fail(NAER_ERR_INVALID, "STDP needs tau_plus, tau_minus > 0 and w_max >= w_min");
-    }
-    n->desc.record_capacity = (uint32_t)std::min<uint64_t>(rec, 0xFFFFFFFFull);
-
-    const uint32_t P = (uint32_t)n->pops.size();
-    n->pp.resize(P);
-    n->cdf.assign((size_t)P * NAER_DRIVE_TABLE, 2.0f);
-    std::vector<uint32_t> pois_in_net(n->nets.size(), 0);
-    uint64_t acc = 0;
-    for (uint32_t p = 0; p < P; ++p) {
-        const naer_pop_desc& q = n->pops[p];
-        const uint32_t ni = n->pop_net[p];
-        const naer_net_desc& d = n->net_descs[ni];
-        PopParams& r = n->pp[p];
-        std::memset(&r, 0, sizeof r);
-        r.begin = (uint32_t)acc;
-        acc += q.size;
-        r.end = (uint32_t)acc;
-        r.kind = q.kind;
-        r.cdf_offset = p * NAER_DRIVE_TABLE;
-        r.net_base = n->nets[ni].neuron_begin;
-        r.seed_lo = (uint32_t)d.seed;
-        r.seed_hi = (uint32_t)(d.seed >> 32);
-        r.pois_seed = d.poisson_seed;
-        r.t_start = n->nets[ni].t_start;
+// Validates one network and fills its NetInfo (ranges excluded).
+int check_net(naer_net* n, const naer_net_desc& d, NetInfo* info)
+{
+    if (d.n_pops == 0 || !d.pops)
+        return fail(NAER_ERR_INVALID, "a network needs at least one population");
+    if (d.n_conns && !d.conns)
+        return fail(NAER_ERR_INVALID, "conns is NULL");
+    if (d.dt_ms != n->dt_ms || d.mode != n->mode)
+        return fail(NAER_ERR_INVALID, "all networks of a pack or arena share dt and mode");
+    info->desc = d;
+    info->pops.assign(d.pops, d.pops + d.n_pops);
+    info->conns.assign(d.conns, d.conns + d.n_conns);
+    info->desc.pops = nullptr;
+    info->desc.conns = nullptr;
+    uint64_t nn = 0;
+    for (uint32_t p = 0; p < d.n_pops; ++p) {
+        const naer_pop_desc& q = d.pops[p];
+        nn += q.size;
         if (q.kind == NAER_POP_POISSON) {
             if (kEpochUs % n->dt_us != 0)
                 return fail(NAER_ERR_INVALID, "POISSON sources need dt dividing the 1500 us epoch");
-            r.pois_base = pois_in_net[ni];
-            pois_in_net[ni] += q.size;
-            r.neg_rate_per_us = -((double)q.src_rate_hz / 1000000.0);
         } else if (q.kind == NAER_POP_LIF) {
             if (!(q.tau_m_ms > 0.0f))
                 return fail(NAER_ERR_INVALID, "population %u: tau_m must be > 0", p);
-            r.P = (float)std::exp(-(double)d0.dt_ms / (double)q.tau_m_ms);
-            r.v_th = q.v_th;
-            r.v_reset = q.v_reset;
-            r.v_rest = q.v_rest;
-            r.v0_min = q.v0_min;
-            r.v0_max = q.v0_max;
-            r.w_ext = q.ext_weight;
-            r.n_ref = (int32_t)std::llround((double)q.t_ref_ms / (double)d0.dt_ms);
-            r.has_drive = q.ext_rate_hz > 0.0f ? 1u : 0u;
-            const double lam = (double)q.ext_rate_hz * (double)d0.dt_ms * 1e-3;
+            const double lam = (double)q.ext_rate_hz * (double)d.dt_ms * 1e-3;
             if (!(lam >= 0.0 && lam <= 20.0))
                 return fail(NAER_ERR_INVALID, "population %u: drive rate*dt must be in [0, 20]", p);
-            // inversion table, see NAER_DRIVE_TABLE in naer.h
-            float* c = &n->cdf[(size_t)p * NAER_DRIVE_TABLE];
-            double pk = std::exp(-lam), cum = pk;
-            for (uint32_t k = 0;; ++k) {
-                const float f = (float)cum;
-                if (f >= 1.0f || k == NAER_DRIVE_TABLE - 1) {
-                    c[k] = 2.0f;
-                    break;
-                }
-                c[k] = f;
-                pk = pk * lam / (double)(k + 1);
-                cum = cum + pk;
-            }
         } else {
             return fail(NAER_ERR_INVALID, "population %u: unknown kind %u", p, q.kind);
         }
     }
-    for (uint32_t p = 0; p < P; ++p)
-        n->pp[p].n_pois = pois_in_net[n->pop_net[p]];
-    for (uint32_t p = 0; p < P; ++p)
-        if (n->pops[p].kind == NAER_POP_POISSON)
-            n->n_pois += n->pops[p].size;
-    // first population of every update block
-    const uint32_t nblk = (n->N + kBlock - 1) / kBlock;
-    n->blk_pop_h.resize(nblk);
+    if (nn == 0 || nn > (1ull << 24))
+        return fail(NAER_ERR_CAPACITY, "neuron count must be in [1, 2^24] (got %llu)", (unsigned long long)nn);
+    info->n_neurons = (uint32_t)nn;
+    for (uint32_t c = 0; c < d.n_conns; ++c) {
+        const naer_conn_desc& k = d.conns[c];
+        if (k.src_pop >= d.n_pops || k.dst_pop >= d.n_pops)
+            return fail(NAER_ERR_INVALID, "connection %u: population index out of range", c);
+        if (d.pops[k.dst_pop].kind != NAER_POP_LIF)
+            return fail(NAER_ERR_INVALID, "connection %u: target population must be LIF", c);
+        const uint32_t q = k.delay_quantum ? k.delay_quantum : 1;
+        if (k.delay_min < 1 || k.delay_max < k.delay_min || k.delay_max > 127 || (k.delay_max - k.delay_min) % q)
+            return fail(NAER_ERR_INVALID,
+                        "connection %u: delays must satisfy 1 <= min <= max <= 127 on the quantum grid", c);
+        if (k.plastic && d.pops[k.src_pop].kind != NAER_POP_LIF)
+            return fail(NAER_ERR_INVALID, "connection %u: plastic connections need a LIF source", c);
+        if (k.rule != NAER_RULE_BERNOULLI && k.rule != NAER_RULE_FIXED_INDEGREE)
+            return fail(NAER_ERR_INVALID, "connection %u: unknown rule %u", c, k.rule);
+        if (k.rule == NAER_RULE_BERNOULLI && !(k.probability >= 0.0 && k.probability <= 1.0))
+            return fail(NAER_ERR_INVALID, "connection %u: probability must be in [0, 1]", c);
+        info->dmin = std::min(info->dmin, k.delay_min);
+        info->dmax = std::max(info->dmax, k.delay_max);
+        if (k.plastic) {
+            const naer_stdp_desc& s = d.stdp;
+            if (!(s.tau_plus_ms > 0.0f) || !(s.tau_minus_ms > 0.0f) || !(s.w_max >= s.w_min))
+                return fail(NAER_ERR_INVALID, "STDP needs tau_plus, tau_minus > 0 and w_max >= w_min");
+            info->plastic = true;
+        }
+    }
+    return NAER_OK;
+}
+
+// ---------------------------------------------------------------------------
+// population table of the active networks (sorted by neuron range)
+// ---------------------------------------------------------------------------
+void fill_pop(const naer_net* n, const NetInfo& net, uint32_t p_local, uint32_t begin, uint32_t pois_base,
+              uint32_t n_pois, PopParams& r, float* cdf)
+{
+    const naer_pop_desc& q = net.pops[p_local];
+    std::memset(&r, 0, sizeof r);
+    r.begin = begin;
+    r.end = begin + q.size;
+    r.kind = q.kind;
+    r.net_base = net.neuron_begin;
+    r.seed_lo = (uint32_t)net.desc.seed;
+    r.seed_hi = (uint32_t)(net.desc.seed >> 32);
+    r.pois_seed = net.desc.poisson_seed;
+    r.t_start = net.t_start;
+    r.n_pois = n_pois;
+    for (uint32_t k = 0; k < NAER_DRIVE_TABLE; ++k)
+        cdf[k] = 2.0f;
+    if (q.kind == NAER_POP_POISSON) {
+        r.pois_base = pois_base;
+        r.neg_rate_per_us = -((double)q.src_rate_hz / 1000000.0);
+        return;
+    }
+    r.P = (float)std::exp(-(double)n->dt_ms / (double)q.tau_m_ms);
+    r.v_th = q.v_th;
+    r.v_reset = q.v_reset;
+    r.v_rest = q.v_rest;
+    r.v0_min = q.v0_min;
+    r.v0_max = q.v0_max;
+    r.w_ext = q.ext_weight;
+    r.n_ref = (int32_t)std::llround((double)q.t_ref_ms / (double)n->dt_ms);
+    r.has_drive = q.ext_rate_hz > 0.0f ? 1u : 0u;
+    // inversion table, see NAER_DRIVE_TABLE in naer.h
+    const double lam = (double)q.ext_rate_hz * (double)n->dt_ms * 1e-3;
+    double pk = std::exp(-lam), cum = pk;
+    for (uint32_t k = 0;; ++k) {
+        const float f = (float)cum;
+        if (f >= 1.0f || k == NAER_DRIVE_TABLE - 1) {
+            cdf[k] = 2.0f;
+            break;
+        }
+        cdf[k] = f;
+        pk = pk * lam / (double)(k + 1);
+        cum = cum + pk;
+    }
+}
+
+int upload_table(naer_net* n)
+{
+    std::vector<uint32_t> order;
+    for (uint32_t i = 0; i < n->nets.size(); ++i)
+        if (n->nets[i].active)
+            order.push_back(i);
+    std::sort(order.begin(), order.end(),
+              [&](uint32_t a, uint32_t b) { return n->nets[a].neuron_begin < n->nets[b].neuron_begin; });
+    n->pp.clear();
+    n->cdf.clear();
+    for (uint32_t i : order) {
+        const NetInfo& net = n->nets[i];
+        uint32_t n_pois = 0;
+        for (auto& q : net.pops)
+            if (q.kind == NAER_POP_POISSON)
+                n_pois += q.size;
+        uint32_t begin = net.neuron_begin, pois = 0;
+        for (uint32_t p = 0; p < net.pops.size(); ++p) {
+            if (net.pops[p].size == 0)
+                continue;
+            PopParams r;
+            n->cdf.resize(n->cdf.size() + NAER_DRIVE_TABLE);
+            fill_pop(n, net, p, begin, pois, n_pois, r, &n->cdf[n->cdf.size() - NAER_DRIVE_TABLE]);
+            r.cdf_offset = (uint32_t)(n->cdf.size() - NAER_DRIVE_TABLE);
+            if (net.pops[p].kind == NAER_POP_POISSON)
+                pois += net.pops[p].size;
+            begin += net.pops[p].size;
+            n->pp.push_back(r);
+        }
+    }
+    const uint32_t P = (uint32_t)n->pp.size();
+    if (P > n->pop_cap) {
+        if (n->d_pops)
+            n->release(n->d_pops, (size_t)n->pop_cap * sizeof(PopParams));
+        if (n->d_cdf)
+            n->release(n->d_cdf, (size_t)n->pop_cap * NAER_DRIVE_TABLE * sizeof(float));
+        n->pop_cap = std::max<uint32_t>(P, 2 * n->pop_cap);
+        int rc;
+        if ((rc = n->alloc(&n->d_pops, n->pop_cap)) ||
+            (rc = n->alloc(&n->d_cdf, (size_t)n->pop_cap * NAER_DRIVE_TABLE)))
+            return rc;
+    }
+    // first population of every update block: the last one starting at or
+    // before the block's first neuron
+    n->blk_pop_h.assign(n->n_seg, 0);
     uint32_t p = 0;
-    for (uint32_t b = 0; b < nblk; ++b) {
+    for (uint32_t b = 0; b < n->n_seg; ++b) {
         const uint32_t j = b * kBlock;
-        while (p + 1 < P && j >= n->pp[p + 1].begin)
+        while (p + 1 < P && n->pp[p + 1].begin <= j)
             ++p;
-        // skip empty populations that start at the same neuron
         n->blk_pop_h[b] = p;
     }
-    n->D = n->dmax + 1;
-    // Steps per update launch: inputs of a batch were all scattered before
-    // it starts when B <= minimum delay.  Plastic nets interleave scatter and
-    // potentiation every step.
-    n->B = n->plastic ? 1u : std::min<uint32_t>(kMaxBatch, n->dmin == 0xFFFFFFFFu ? kMaxBatch : n->dmin);
-    return NAER_OK;
+    if (P) {
+        HIP_TRY(hipMemcpyAsync(n->d_pops, n->pp.data(), P * sizeof(PopParams), hipMemcpyHostToDevice, n->stream));
+        HIP_TRY(hipMemcpyAsync(n->d_cdf, n->cdf.data(), n->cdf.size() * sizeof(float), hipMemcpyHostToDevice,
+                               n->stream));
+    }
+    HIP_TRY(hipMemcpyAsync(n->d_blk_pop, n->blk_pop_h.data(), n->n_seg * sizeof(uint32_t), hipMemcpyHostToDevice,
+                           n->stream));
+    // the copies read host vectors that may change at the next upload
+    HIP_TRY(hipStreamSynchronize(n->stream));
+    // captured graphs hold the population count in their arguments
+    return n->drop_graph();
 }
 
 UpdateArgs make_update_args(naer_net* n)
@@ -393,23 +504,30 @@ UpdateArgs make_update_args(naer_net* n)
     a.dt_us = n->dt_us;
     a.n_pops = (uint32_t)n->pp.size();
     if (n->plastic) {
-        a.dec_p = (float)std::exp(-(double)n->desc.dt_ms / (double)n->desc.stdp.tau_plus_ms);
-        a.dec_m = (float)std::exp(-(double)n->desc.dt_ms / (double)n->desc.stdp.tau_minus_ms);
+        a.dec_p = (float)std::exp(-(double)n->dt_ms / (double)n->stdp.tau_plus_ms);
+        a.dec_m = (float)std::exp(-(double)n->dt_ms / (double)n->stdp.tau_minus_ms);
     }
     return a;
 }
 
-ConnArgs make_conn_args(naer_net* n, uint32_t c)
+ConnArgs make_conn_args(const NetInfo& net, uint32_t c)
 {
-    const naer_conn_desc& k = n->conns[c];
+    const naer_conn_desc& k = net.conns[c];
+    // population starts within the network (populations are contiguous, in order)
+    std::vector<uint32_t> pb(net.pops.size());
+    uint32_t acc = net.neuron_begin;
+    for (size_t p = 0; p < net.pops.size(); ++p) {
+        pb[p] = acc;
+        acc += net.pops[p].size;
+    }
     ConnArgs a;
     std::memset(&a, 0, sizeof a);
-    a.src_begin = n->pp[k.src_pop].begin;
-    a.n_src = n->pops[k.src_pop].size;
-    a.dst_begin = n->pp[k.dst_pop].begin;
-    a.n_dst = n->pops[k.dst_pop].size;
+    a.src_begin = pb[k.src_pop];
+    a.n_src = net.pops[k.src_pop].size;
+    a.dst_begin = pb[k.dst_pop];
+    a.n_dst = net.pops[k.dst_pop].size;
     a.K = k.indegree;
-    a.conn_index = n->conn_local[c];
+    a.conn_index = c;
     a.seed = k.seed;
     a.p = k.probability;
     a.weight = k.weight;
@@ -422,29 +540,140 @@ ConnArgs make_conn_args(naer_net* n, uint32_t c)
     return a;
 }
 
-// exclusive scan of per-row counts into a device row_ptr
-int scan_rows(naer_net* n, const uint32_t* d_deg, unsigned long long* d_ptr, uint64_t* total,
-              uint32_t* max_len)
+// ---------------------------------------------------------------------------
+// device build of one network
+// ---------------------------------------------------------------------------
+int count_net(naer_net* n, const NetInfo& net)
 {
-    std::vector<uint32_t> deg(n->N);
-    HIP_TRY(hipMemcpyAsync(deg.data(), d_deg, n->N * sizeof(uint32_t), hipMemcpyDeviceToHost, n->stream));
+    HIP_TRY(hipMemsetAsync(n->deg + net.neuron_begin, 0, (size_t)net.n_neurons * sizeof(uint32_t), n->stream));
+    for (uint32_t c = 0; c < net.conns.size(); ++c) {
+        ConnArgs a = make_conn_args(net, c);
+        HIP_TRY(launch_conn(a, net.conns[c].rule, false, n->deg, nullptr, nullptr, nullptr, n->stream));
+    }
+    return NAER_OK;
+}
+
+// Reads the counts of [begin, begin+count) and returns their prefix sums.
+int read_prefix(naer_net* n, uint32_t begin, uint32_t count, std::vector<unsigned long long>* prefix,
+                uint32_t* max_len)
+{
+    std::vector<uint32_t> d(count);
+    if (count)
+        HIP_TRY(hipMemcpyAsync(d.data(), n->deg + begin, count * sizeof(uint32_t), hipMemcpyDeviceToHost, n->stream));
     HIP_TRY(hipStreamSynchronize(n->stream));
-    std::vector<unsigned long long> ptr(n->N + 1);
+    prefix->resize((size_t)count + 1);
     unsigned long long acc = 0;
     uint32_t mx = 0;
-    for (uint32_t j = 0; j < n->N; ++j) {
-        ptr[j] = acc;
-        acc += deg[j];
-        mx = std::max(mx, deg[j]);
+    for (uint32_t j = 0; j < count; ++j) {
+        (*prefix)[j] = acc;
+        acc += d[j];
+        mx = std::max(mx, d[j]);
     }
-    ptr[n->N] = acc;
-    HIP_TRY(hipMemcpyAsync(d_ptr, ptr.data(), (n->N + 1) * sizeof(unsigned long long),
-                           hipMemcpyHostToDevice, n->stream));
-    HIP_TRY(hipStreamSynchronize(n->stream));
-    *total = acc;
+    (*prefix)[count] = acc;
     if (max_len)
         *max_len = mx;
     return NAER_OK;
+}
+
+// Writes ptr[begin .. begin+count] = base + prefix.
+int write_ptr(naer_net* n, unsigned long long* d_ptr, uint32_t begin, uint64_t base,
+              std::vector<unsigned long long>& prefix)
+{
+    for (auto& v : prefix)
+        v += base;
+    HIP_TRY(hipMemcpyAsync(d_ptr + begin, prefix.data(), prefix.size() * sizeof(unsigned long long),
+                           hipMemcpyHostToDevice, n->stream));
+    HIP_TRY(hipStreamSynchronize(n->stream));
+    return NAER_OK;
+}
+
+int place_net(naer_net* n, const NetInfo& net)
+{
+    HIP_TRY(hipMemcpyAsync(n->cursor + net.neuron_begin, n->row_ptr + net.neuron_begin,
+                           (size_t)net.n_neurons * sizeof(unsigned long long), hipMemcpyDeviceToDevice, n->stream));
+    for (uint32_t c = 0; c < net.conns.size(); ++c) {
+        ConnArgs a = make_conn_args(net, c);
+        HIP_TRY(launch_conn(a, net.conns[c].rule, true, nullptr, n->cursor, n->col, n->w, n->stream));
+    }
+    HIP_TRY(launch_sort_rows(n->row_ptr, n->col, n->w, net.neuron_begin, net.neuron_begin + net.n_neurons,
+                             net.max_row, n->stream));
+    return NAER_OK;
+}
+
+// Counts the plastic synapses onto every target of a network.
+int reverse_count(naer_net* n, const NetInfo& net, std::vector<unsigned long long>* prefix)
+{
+    HIP_TRY(hipMemsetAsync(n->deg + net.neuron_begin, 0, (size_t)net.n_neurons * sizeof(uint32_t), n->stream));
+    HIP_TRY(launch_reverse(false, n->row_ptr, n->col, net.neuron_begin, net.neuron_begin + net.n_neurons, n->deg,
+                           nullptr, nullptr, nullptr, n->stream));
+    return read_prefix(n, net.neuron_begin, net.n_neurons, prefix, nullptr);
+}
+
+// Writes the reverse index of a network at net.rev_begin.
+int reverse_place(naer_net* n, const NetInfo& net, std::vector<unsigned long long>& prefix)
+{
+    int rc = write_ptr(n, n->rev_ptr, net.neuron_begin, net.rev_begin, prefix);
+    if (rc)
+        return rc;
+    HIP_TRY(hipMemcpyAsync(n->cursor + net.neuron_begin, n->rev_ptr + net.neuron_begin,
+                           (size_t)net.n_neurons * sizeof(unsigned long long), hipMemcpyDeviceToDevice, n->stream));
+    HIP_TRY(launch_reverse(true, n->row_ptr, n->col, net.neuron_begin, net.neuron_begin + net.n_neurons, nullptr,
+                           n->cursor, n->rev_pos, n->rev_src, n->stream));
+    return NAER_OK;
+}
+
+int init_net(naer_net* n, const NetInfo& net)
+{
+    UpdateArgs ua = make_update_args(n);
+    HIP_TRY(launch_init_state(ua, n->det, net.neuron_begin, net.neuron_begin + net.n_neurons, n->stream));
+    return NAER_OK;
+}
+
+// Allocation of everything sized by the neuron capacity.
+int alloc_neurons(naer_net* n, bool any_poisson)
+{
+    const uint32_t N = n->N;
+    int rc;
+    if ((rc = n->alloc(&n->deg, N)) || (rc = n->alloc(&n->cursor, N)) ||
+        (rc = n->alloc(&n->row_ptr, (size_t)N + 1)))
+        return rc;
+    HIP_TRY(hipMemsetAsync(n->row_ptr, 0, ((size_t)N + 1) * sizeof(unsigned long long), n->stream));
+    if (n->plastic) {
+        if ((rc = n->alloc(&n->rev_ptr, (size_t)N + 1)) || (rc = n->alloc(&n->xh, N)) || (rc = n->alloc(&n->yh, N)))
+            return rc;
+        HIP_TRY(hipMemsetAsync(n->rev_ptr, 0, ((size_t)N + 1) * sizeof(unsigned long long), n->stream));
+    }
+    if ((rc = n->alloc(&n->V, N)) || (rc = n->alloc(&n->ref, N)) || (rc = n->alloc(&n->last, N)) ||
+        (rc = n->alloc(&n->pnext, N)) || (rc = n->alloc(&n->pcnt, N)))
+        return rc;
+    const size_t ring_bytes = (size_t)n->D * N * (n->det ? sizeof(long long) : sizeof(float));
+    if ((rc = n->alloc((char**)&n->ring, ring_bytes)))
+        return rc;
+    HIP_TRY(hipMemsetAsync(n->ring, 0, ring_bytes, n->stream));
+    // list segment of one update block: every LIF neuron fires at most once
+    // per step, a POISSON source at most 10 times per 1500 us epoch
+    const uint64_t epochs = ((uint64_t)n->B * n->dt_us + kEpochUs - 1) / kEpochUs + 1;
+    const uint64_t per_neuron = any_poisson ? std::max<uint64_t>(n->B, kMaxPoissonPerEpoch * epochs) : n->B;
+    n->seg_cap = (uint32_t)(kBlock * per_neuron);
+    n->n_seg = (N + kBlock - 1) / kBlock;
+    const size_t list_elems = (size_t)n->seg_cap * n->n_seg;
+    if ((rc = n->alloc(&n->list0, list_elems)) || (rc = n->alloc(&n->list1, list_elems)) ||
+        (rc = n->alloc(&n->seg_count, 2 * (size_t)n->n_seg)) || (rc = n->alloc(&n->spk_part, n->n_seg)) ||
+        (rc = n->alloc(&n->ev_part, n->n_seg)) || (rc = n->alloc(&n->d_blk_pop, n->n_seg)))
+        return rc;
+    HIP_TRY(hipMemsetAsync(n->seg_count, 0, 2 * (size_t)n->n_seg * sizeof(uint32_t), n->stream));
+    HIP_TRY(hipMemsetAsync(n->spk_part, 0, n->n_seg * sizeof(unsigned long long), n->stream));
+    HIP_TRY(hipMemsetAsync(n->ev_part, 0, n->n_seg * sizeof(unsigned long long), n->stream));
+    if ((rc = n->alloc(&n->rec, n->rec_cap ? n->rec_cap : 1)) || (rc = n->alloc(&n->st, 1)))
+        return rc;
+    HIP_TRY(hipMemsetAsync(n->st, 0, sizeof(DevState), n->stream));
+    return NAER_OK;
+}
+
+void set_scatter_shape(naer_net* n)
+{
+    n->scatter_chunks = std::max<uint32_t>(1, (n->max_row + kChunk - 1) / kChunk);
+    n->scatter_grid = std::max<uint32_t>(4, std::min<uint32_t>(1024, 2048 / n->scatter_chunks));
 }
 
 int launch_timed(naer_net* n, int kind, hipError_t (*fn)(naer_net*, void*), void* arg, bool timed)
@@ -467,13 +696,23 @@ int launch_timed(naer_net* n, int kind, hipError_t (*fn)(naer_net*, void*), void
     return NAER_OK;
 }
 
+int make_stream(naer_net* n)
+{
+    hipError_t e = hipSetDevice(n->ctx->device);
+    if (e == hipSuccess)
+        e = hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking);
+    if (e != hipSuccess)
+        return fail(NAER_ERR_HIP, "stream creation failed: %s", hipGetErrorString(e));
+    return NAER_OK;
+}
+
 }  // namespace
 
 extern "C" {
 
 const char* naer_last_error(void) { return g_err.c_str(); }
 
-const char* naer_version(void) { return "naer-mi355x 0.1 (gfx950)"; }
+const char* naer_version(void) { return "naer-mi355x 0.2 (gfx950)"; }
 
 int naer_ctx_create(int device, naer_ctx** out)
 {
@@ -504,27 +743,46 @@ int naer_pack_create(naer_ctx* ctx, const naer_net_desc* descs, uint32_t n_nets,
         return fail(NAER_ERR_INVALID, "NULL argument or empty pack");
     naer_net* n = new naer_net;
     n->ctx = ctx;
-    n->net_descs.assign(descs, descs + n_nets);
-    int rc = validate_and_derive(n);
-    // the caller's arrays are not referenced after creation
-    for (auto& d : n->net_descs) {
-        d.pops = nullptr;
-        d.conns = nullptr;
+    int rc = check_settings(n, descs[0].dt_ms, descs[0].mode);
+    uint64_t N = 0, rec = 0;
+    n->dmin = 0xFFFFFFFFu;
+    n->dmax = 1;
+    for (uint32_t i = 0; rc == NAER_OK && i < n_nets; ++i) {
+        NetInfo info;
+        if ((rc = check_net(n, descs[i], &info)))
+            break;
+        if (info.plastic) {
+            if (n->stdp_set && std::memcmp(&n->stdp, &descs[i].stdp, sizeof n->stdp) != 0) {
+                rc = fail(NAER_ERR_INVALID, "plastic networks of a pack must share the STDP parameters");
+                break;
+            }
+            n->stdp = descs[i].stdp;
+            n->stdp_set = true;
+            n->plastic = true;
+        }
+        info.neuron_begin = (uint32_t)N;
+        info.active = true;
+        N += info.n_neurons;
+        n->dmin = std::min(n->dmin, info.dmin);
+        n->dmax = std::max(n->dmax, info.dmax);
+        rec += descs[i].record_capacity;
+        n->nets.push_back(std::move(info));
+        if (N > 0xFFFFFF00ull)
+            rc = fail(NAER_ERR_CAPACITY, "a pack is limited to 2^32 neurons");
     }
-    n->desc.pops = n->pops.data();
-    n->desc.conns = n->conns.data();
-    n->desc.n_pops = (uint32_t)n->pops.size();
-    n->desc.n_conns = (uint32_t)n->conns.size();
+    if (rc == NAER_OK) {
+        n->N = (uint32_t)N;
+        n->D = n->dmax + 1;
+        // Steps per update launch: inputs of a batch were all scattered
+        // before it starts when B <= minimum delay.  Plastic nets interleave
+        // scatter and potentiation every step.
+        n->B = n->plastic ? 1u : std::min<uint32_t>(kMaxBatch, n->dmin == 0xFFFFFFFFu ? kMaxBatch : n->dmin);
+        n->rec_cap = rec;
+        rc = make_stream(n);
+    }
     if (rc != NAER_OK) {
         delete n;
         return rc;
-    }
-    hipError_t e = hipSetDevice(ctx->device);
-    if (e == hipSuccess)
-        e = hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking);
-    if (e != hipSuccess) {
-        delete n;
-        return fail(NAER_ERR_HIP, "stream creation failed: %s", hipGetErrorString(e));
     }
     *out = n;
     return NAER_OK;
@@ -545,7 +803,7 @@ int naer_pack_layout(naer_net* n, uint32_t* neuron_begin, uint32_t* n_neurons, u
         if (neuron_begin)
             neuron_begin[i] = n->nets[i].neuron_begin;
         if (n_neurons)
-            n_neurons[i] = n->nets[i].n_neurons;
+            n_neurons[i] = n->nets[i].active ? n->nets[i].n_neurons : 0u;
         if (t_start)
             t_start[i] = n->nets[i].t_start;
     }
@@ -556,106 +814,246 @@ int naer_net_build(naer_net* n)
 {
     if (!n)
         return fail(NAER_ERR_INVALID, "net is NULL");
-    if (n->built)
+    if (n->built || n->arena)
         return fail(NAER_ERR_STATE, "net already built");
     HIP_TRY(hipSetDevice(n->ctx->device));
-    const uint32_t N = n->N;
+    bool any_pois = false;
+    for (auto& net : n->nets)
+        for (auto& q : net.pops)
+            any_pois = any_pois || q.kind == NAER_POP_POISSON;
     int rc;
-
-    // 1. out-degree of every source over all connections
-    uint32_t* deg = nullptr;
-    unsigned long long* cursor = nullptr;
-    if ((rc = n->alloc(&deg, N)) || (rc = n->alloc(&n->row_ptr, (size_t)N + 1)))
+    if ((rc = alloc_neurons(n, any_pois)))
         return rc;
-    HIP_TRY(hipMemsetAsync(deg, 0, N * sizeof(uint32_t), n->stream));
-    for (uint32_t c = 0; c < n->conns.size(); ++c) {
-        ConnArgs a = make_conn_args(n, c);
-        HIP_TRY(launch_conn(a, n->conns[c].rule, false, deg, nullptr, nullptr, nullptr, n->stream));
+    // 1. out-degrees of every network, rows back to back
+    std::vector<std::vector<unsigned long long>> prefix(n->nets.size());
+    uint64_t S = 0;
+    for (size_t i = 0; i < n->nets.size(); ++i) {
+        NetInfo& net = n->nets[i];
+        if ((rc = count_net(n, net)) ||
+            (rc = read_prefix(n, net.neuron_begin, net.n_neurons, &prefix[i], &net.max_row)))
+            return rc;
+        if (net.max_row > kMaxSortedRow)
+            return fail(NAER_ERR_CAPACITY, "a source has %u synapses; rows are limited to %u", net.max_row,
+                        kMaxSortedRow);
+        net.syn_begin = S;
+        net.n_syn = prefix[i].back();
+        S += net.n_syn;
+        n->max_row = std::max(n->max_row, net.max_row);
     }
-    // 2. row pointers
-    if ((rc = scan_rows(n, deg, n->row_ptr, &n->S, &n->max_row)))
-        return rc;
-    if (n->max_row > kMaxSortedRow)
-        return fail(NAER_ERR_CAPACITY, "a source has %u synapses; rows are limited to %u",
-                    n->max_row, kMaxSortedRow);
-    if (n->plastic && n->S >= 0xFFFFFFFFull)
+    if (n->plastic && S >= 0xFFFFFFFFull)
         return fail(NAER_ERR_CAPACITY, "plastic nets are limited to 2^32-1 synapses");
-    // 3. placement, then canonical row order
-    if ((rc = n->alloc(&n->col, n->S)) || (rc = n->alloc(&n->w, n->S)) || (rc = n->alloc(&cursor, N)))
+    n->S_cap = S;
+    if ((rc = n->alloc(&n->col, S)) || (rc = n->alloc(&n->w, S)))
         return rc;
-    HIP_TRY(hipMemcpyAsync(cursor, n->row_ptr, N * sizeof(unsigned long long), hipMemcpyDeviceToDevice,
-                           n->stream));
-    for (uint32_t c = 0; c < n->conns.size(); ++c) {
-        ConnArgs a = make_conn_args(n, c);
-        HIP_TRY(launch_conn(a, n->conns[c].rule, true, nullptr, cursor, n->col, n->w, n->stream));
+    // 2. placement and canonical row order
+    for (size_t i = 0; i < n->nets.size(); ++i) {
+        NetInfo& net = n->nets[i];
+        if ((rc = write_ptr(n, n->row_ptr, net.neuron_begin, net.syn_begin, prefix[i])) ||
+            (rc = place_net(n, net)))
+            return rc;
     }
-    if (n->S)
-        HIP_TRY(launch_sort_rows(n->row_ptr, n->col, n->w, N, n->max_row, n->stream));
-    // 4. reverse index of plastic synapses
+    // 3. reverse index of plastic synapses, networks back to back
     if (n->plastic) {
-        HIP_TRY(hipMemsetAsync(deg, 0, N * sizeof(uint32_t), n->stream));
-        HIP_TRY(launch_reverse(false, n->row_ptr, n->col, N, deg, nullptr, nullptr, nullptr, n->stream));
-        if ((rc = n->alloc(&n->rev_ptr, (size_t)N + 1)))
+        std::vector<std::vector<unsigned long long>> rp(n->nets.size());
+        uint64_t R = 0;
+        for (size_t i = 0; i < n->nets.size(); ++i) {
+            NetInfo& net = n->nets[i];
+            if ((rc = reverse_count(n, net, &rp[i])))
+                return rc;
+            net.rev_begin = R;
+            net.n_rev = rp[i].back();
+            R += net.n_rev;
+        }
+        n->R_cap = R;
+        if ((rc = n->alloc(&n->rev_pos, R)) || (rc = n->alloc(&n->rev_src, R)))
             return rc;
-        if ((rc = scan_rows(n, deg, n->rev_ptr, &n->n_plastic, nullptr)))
-            return rc;
-        if ((rc = n->alloc(&n->rev_pos, n->n_plastic)) || (rc = n->alloc(&n->rev_src, n->n_plastic)))
-            return rc;
-        HIP_TRY(hipMemcpyAsync(cursor, n->rev_ptr, N * sizeof(unsigned long long),
-                               hipMemcpyDeviceToDevice, n->stream));
-        HIP_TRY(launch_reverse(true, n->row_ptr, n->col, N, nullptr, cursor, n->rev_pos, n->rev_src,
-                               n->stream));
+        for (size_t i = 0; i < n->nets.size(); ++i)
+            if ((rc = reverse_place(n, n->nets[i], rp[i])))
+                return rc;
     }
+    // 4. neuron state
+    if ((rc = upload_table(n)))
+        return rc;
+    for (auto& net : n->nets)
+        if ((rc = init_net(n, net)))
+            return rc;
     HIP_TRY(hipStreamSynchronize(n->stream));
-    n->release(deg, N * sizeof(uint32_t));
-    n->release(cursor, N * sizeof(unsigned long long));
-
-    // 5. neuron state, delay ring, spike lists
-    if ((rc = n->alloc(&n->V, N)) || (rc = n->alloc(&n->ref, N)) || (rc = n->alloc(&n->last, N)) ||
-        (rc = n->alloc(&n->pnext, N)) || (rc = n->alloc(&n->pcnt, N)))
-        return rc;
-    if (n->plastic && ((rc = n->alloc(&n->xh, N)) || (rc = n->alloc(&n->yh, N))))
-        return rc;
-    const size_t ring_elems = (size_t)n->D * N;
-    const size_t ring_bytes = ring_elems * (n->det ? sizeof(long long) : sizeof(float));
-    if ((rc = n->alloc((char**)&n->ring, ring_bytes)))
-        return rc;
-    HIP_TRY(hipMemsetAsync(n->ring, 0, ring_bytes, n->stream));
-    // list segment of one update block: every LIF neuron fires at most once
-    // per step, a POISSON source at most 10 times per 1500 us epoch
-    const uint64_t epochs = ((uint64_t)n->B * n->dt_us + kEpochUs - 1) / kEpochUs + 1;
-    const uint64_t per_neuron = n->n_pois ? std::max<uint64_t>(n->B, kMaxPoissonPerEpoch * epochs) : n->B;
-    n->seg_cap = (uint32_t)(kBlock * per_neuron);
-    n->n_seg = (N + kBlock - 1) / kBlock;
-    const size_t list_elems = (size_t)n->seg_cap * n->n_seg;
-    if ((rc = n->alloc(&n->list0, list_elems)) || (rc = n->alloc(&n->list1, list_elems)) ||
-        (rc = n->alloc(&n->seg_count, 2 * (size_t)n->n_seg)) || (rc = n->alloc(&n->spk_part, n->n_seg)) ||
-        (rc = n->alloc(&n->ev_part, n->n_seg)))
-        return rc;
-    HIP_TRY(hipMemsetAsync(n->seg_count, 0, 2 * (size_t)n->n_seg * sizeof(uint32_t), n->stream));
-    HIP_TRY(hipMemsetAsync(n->spk_part, 0, n->n_seg * sizeof(unsigned long long), n->stream));
-    HIP_TRY(hipMemsetAsync(n->ev_part, 0, n->n_seg * sizeof(unsigned long long), n->stream));
-    n->rec_cap = n->desc.record_capacity;
-    if ((rc = n->alloc(&n->rec, n->rec_cap ? n->rec_cap : 1)))
-        return rc;
-    if ((rc = n->alloc(&n->st, 1)) || (rc = n->alloc(&n->d_cdf, n->cdf.size())) ||
-        (rc = n->alloc(&n->d_pops, n->pp.size())) || (rc = n->alloc(&n->d_blk_pop, n->blk_pop_h.size())))
-        return rc;
-    HIP_TRY(hipMemcpyAsync(n->d_pops, n->pp.data(), n->pp.size() * sizeof(PopParams), hipMemcpyHostToDevice,
-                           n->stream));
-    HIP_TRY(hipMemcpyAsync(n->d_blk_pop, n->blk_pop_h.data(), n->blk_pop_h.size() * sizeof(uint32_t),
-                           hipMemcpyHostToDevice, n->stream));
-    HIP_TRY(hipMemsetAsync(n->st, 0, sizeof(DevState), n->stream));
-    HIP_TRY(hipMemcpyAsync(n->d_cdf, n->cdf.data(), n->cdf.size() * sizeof(float), hipMemcpyHostToDevice,
-                           n->stream));
-    UpdateArgs ua = make_update_args(n);
-    HIP_TRY(launch_init_state(ua, n->stream));
-    HIP_TRY(hipStreamSynchronize(n->stream));
-
-    n->scatter_chunks = std::max<uint32_t>(1, (n->max_row + kChunk - 1) / kChunk);
-    n->scatter_grid = std::max<uint32_t>(4, std::min<uint32_t>(1024, 2048 / n->scatter_chunks));
+    set_scatter_shape(n);
     n->built = true;
     return NAER_OK;
+}
+
+int naer_arena_create(naer_ctx* ctx, const naer_arena_desc* a, naer_net** out)
+{
+    if (!ctx || !a || !out)
+        return fail(NAER_ERR_INVALID, "NULL argument");
+    naer_net* n = new naer_net;
+    n->ctx = ctx;
+    n->arena = true;
+    int rc = check_settings(n, a->dt_ms, a->mode);
+    if (rc == NAER_OK && (a->max_neurons == 0 || a->max_neurons > 0xFFFFFF00ull))
+        rc = fail(NAER_ERR_CAPACITY, "arena neuron capacity must be in [1, 2^32)");
+    if (rc == NAER_OK && (a->delay_min < 1 || a->delay_max < a->delay_min || a->delay_max > 127))
+        rc = fail(NAER_ERR_INVALID, "arena delays must satisfy 1 <= min <= max <= 127");
+    if (rc == NAER_OK && (a->max_row == 0 || a->max_row > kMaxSortedRow))
+        rc = fail(NAER_ERR_CAPACITY, "arena max_row must be in [1, %u]", kMaxSortedRow);
+    if (rc == NAER_OK && a->plastic && a->max_synapses >= 0xFFFFFFFFull)
+        rc = fail(NAER_ERR_CAPACITY, "plastic arenas are limited to 2^32-1 synapses");
+    if (rc == NAER_OK && a->plastic &&
+        (!(a->stdp.tau_plus_ms > 0.0f) || !(a->stdp.tau_minus_ms > 0.0f) || !(a->stdp.w_max >= a->stdp.w_min)))
+        rc = fail(NAER_ERR_INVALID, "STDP needs tau_plus, tau_minus > 0 and w_max >= w_min");
+    if (rc == NAER_OK) {
+        n->N = (uint32_t)((a->max_neurons + kBlock - 1) / kBlock * kBlock);
+        n->S_cap = a->max_synapses;
+        n->R_cap = a->plastic ? a->max_plastic : 0;
+        n->plastic = a->plastic != 0;
+        n->stdp = a->stdp;
+        n->stdp_set = n->plastic;
+        n->dmin = a->delay_min;
+        n->dmax = a->delay_max;
+        n->D = n->dmax + 1;
+        n->B = n->plastic ? 1u : std::min<uint32_t>(kMaxBatch, n->dmin);
+        n->max_row = a->max_row;
+        n->rec_cap = a->record_capacity;
+        n->neuron_alloc.reset(n->N);
+        n->syn_alloc.reset(n->S_cap);
+        n->rev_alloc.reset(n->R_cap);
+        rc = make_stream(n);
+    }
+    n->poisson_ok = a->poisson != 0;
+    if (rc == NAER_OK)
+        rc = alloc_neurons(n, n->poisson_ok);
+    if (rc == NAER_OK)
+        rc = n->alloc(&n->col, n->S_cap);
+    if (rc == NAER_OK)
+        rc = n->alloc(&n->w, n->S_cap);
+    if (rc == NAER_OK && n->plastic)
+        rc = n->alloc(&n->rev_pos, n->R_cap);
+    if (rc == NAER_OK && n->plastic)
+        rc = n->alloc(&n->rev_src, n->R_cap);
+    if (rc == NAER_OK)
+        rc = upload_table(n);
+    if (rc == NAER_OK) {
+        set_scatter_shape(n);
+        n->built = true;
+        hipError_t e = hipStreamSynchronize(n->stream);
+        if (e != hipSuccess)
+            rc = fail(NAER_ERR_HIP, "arena set-up failed: %s", hipGetErrorString(e));
+    }
+    if (rc != NAER_OK) {
+        delete n;
+        return rc;
+    }
+    *out = n;
+    return NAER_OK;
+}
+
+int naer_arena_admit(naer_net* n, const naer_net_desc* desc, uint32_t* slot)
+{
+    if (!n || !desc || !slot)
+        return fail(NAER_ERR_INVALID, "NULL argument");
+    if (!n->arena)
+        return fail(NAER_ERR_STATE, "not an arena");
+    NetInfo net;
+    int rc;
+    if ((rc = check_net(n, *desc, &net)))
+        return rc;
+    if (net.dmin < n->dmin || net.dmax > n->dmax)
+        return fail(NAER_ERR_INVALID, "job delays [%u, %u] outside the arena's [%u, %u]", net.dmin, net.dmax,
+                    n->dmin, n->dmax);
+    if (net.plastic && (!n->plastic || std::memcmp(&n->stdp, &desc->stdp, sizeof n->stdp) != 0))
+        return fail(NAER_ERR_INVALID, "job needs STDP parameters the arena was not created with");
+    for (auto& q : net.pops)
+        if (q.kind == NAER_POP_POISSON && !n->poisson_ok)
+            return fail(NAER_ERR_INVALID, "the arena was created without POISSON sources");
+    if ((rc = naer_net_sync(n)))
+        return rc;
+    // one extra entry: row_ptr/rev_ptr[begin + n] (the end of the job's last
+    // row) must not be shared with a job placed right behind it
+    uint64_t nb = 0;
+    if (!n->neuron_alloc.take((uint64_t)net.n_neurons + 1, kBlock, &nb))
+        return fail(NAER_ERR_CAPACITY, "no free range of %u neurons", net.n_neurons);
+    net.neuron_begin = (uint32_t)nb;
+    // a job starts at the next step of the arena clock
+    net.t_start = n->steps_enqueued;
+    net.active = true;
+    bool syn_taken = false, rev_taken = false;
+    auto undo = [&](int code) {
+        n->neuron_alloc.give(net.neuron_begin, (uint64_t)net.n_neurons + 1);
+        if (syn_taken)
+            n->syn_alloc.give(net.syn_begin, net.n_syn);
+        if (rev_taken)
+            n->rev_alloc.give(net.rev_begin, net.n_rev);
+        return code;
+    };
+    std::vector<unsigned long long> prefix;
+    if ((rc = count_net(n, net)) || (rc = read_prefix(n, net.neuron_begin, net.n_neurons, &prefix, &net.max_row)))
+        return undo(rc);
+    if (net.max_row > n->max_row)
+        return undo(fail(NAER_ERR_CAPACITY, "a source has %u synapses; the arena admits rows up to %u", net.max_row,
+                         n->max_row));
+    net.n_syn = prefix.back();
+    if (!n->syn_alloc.take(net.n_syn, 64, &net.syn_begin))
+        return undo(fail(NAER_ERR_CAPACITY, "no free range of %llu synapses", (unsigned long long)net.n_syn));
+    syn_taken = true;
+    if ((rc = write_ptr(n, n->row_ptr, net.neuron_begin, net.syn_begin, prefix)) || (rc = place_net(n, net)))
+        return undo(rc);
+    if (n->plastic) {
+        // static jobs get an empty reverse index (a flat rev_ptr over their range)
+        std::vector<unsigned long long> rp((size_t)net.n_neurons + 1, 0);
+        if (net.plastic && (rc = reverse_count(n, net, &rp)))
+            return undo(rc);
+        net.n_rev = rp.back();
+        if (!n->rev_alloc.take(net.n_rev, 64, &net.rev_begin))
+            return undo(fail(NAER_ERR_CAPACITY, "no free range of %llu plastic synapses", (unsigned long long)net.n_rev));
+        rev_taken = true;
+        if (net.plastic) {
+            if ((rc = reverse_place(n, net, rp)))
+                return undo(rc);
+        } else if ((rc = write_ptr(n, n->rev_ptr, net.neuron_begin, net.rev_begin, rp))) {
+            return undo(rc);
+        }
+    }
+    // slot: reuse a retired one
+    uint32_t s = (uint32_t)n->nets.size();
+    for (uint32_t i = 0; i < n->nets.size(); ++i)
+        if (!n->nets[i].active) {
+            s = i;
+            break;
+        }
+    if (s == n->nets.size())
+        n->nets.push_back(net);
+    else
+        n->nets[s] = net;
+    if ((rc = upload_table(n)) || (rc = init_net(n, n->nets[s]))) {
+        n->nets[s].active = false;
+        (void)upload_table(n);
+        return undo(rc);
+    }
+    HIP_TRY(hipStreamSynchronize(n->stream));
+    *slot = s;
+    return NAER_OK;
+}
+
+int naer_arena_retire(naer_net* n, uint32_t slot)
+{
+    if (!n)
+        return fail(NAER_ERR_INVALID, "net is NULL");
+    if (!n->arena)
+        return fail(NAER_ERR_STATE, "not an arena");
+    if (slot >= n->nets.size() || !n->nets[slot].active)
+        return fail(NAER_ERR_INVALID, "slot %u holds no job", slot);
+    int rc;
+    if ((rc = naer_net_sync(n)))
+        return rc;
+    NetInfo& net = n->nets[slot];
+    net.active = false;
+    n->neuron_alloc.give(net.neuron_begin, (uint64_t)net.n_neurons + 1);
+    n->syn_alloc.give(net.syn_begin, net.n_syn);
+    if (n->plastic)
+        n->rev_alloc.give(net.rev_begin, net.n_rev);
+    net.n_syn = net.n_rev = 0;
+    return upload_table(n);
 }
 
 static hipError_t do_update(naer_net* n, void* arg)
@@ -692,9 +1090,9 @@ static int enqueue_steps(naer_net* n, uint32_t n_steps, bool timed)
     sa.w = n->w;
     sa.ring = n->ring;
     sa.yh = n->yh;
-    sa.a_minus = n->desc.stdp.a_minus;
-    sa.w_min = n->desc.stdp.w_min;
-    sa.w_max = n->desc.stdp.w_max;
+    sa.a_minus = n->stdp.a_minus;
+    sa.w_min = n->stdp.w_min;
+    sa.w_max = n->stdp.w_max;
     sa.N = n->N;
     sa.D = n->D;
     sa.is_last = n->plastic ? 0u : 1u;
@@ -711,9 +1109,9 @@ static int enqueue_steps(naer_net* n, uint32_t n_steps, bool timed)
     pa.rev_src = n->rev_src;
     pa.w = n->w;
     pa.xh = n->xh;
-    pa.a_plus = n->desc.stdp.a_plus;
-    pa.w_min = n->desc.stdp.w_min;
-    pa.w_max = n->desc.stdp.w_max;
+    pa.a_plus = n->stdp.a_plus;
+    pa.w_min = n->stdp.w_min;
+    pa.w_max = n->stdp.w_max;
     int rc;
     while (n_steps > 0) {
         const uint32_t b = std::min(n_steps, n->B);
@@ -773,10 +1171,8 @@ int naer_net_set_graph(naer_net* n, uint32_t chunk_steps)
     int rc = naer_net_sync(n);
     if (rc)
         return rc;
-    if (n->graph_exec) {
-        HIP_TRY(hipGraphExecDestroy(n->graph_exec));
-        n->graph_exec = nullptr;
-    }
+    if ((rc = n->drop_graph()))
+        return rc;
     n->graph_chunk = chunk_steps;
     return NAER_OK;
 }
@@ -805,9 +1201,12 @@ int naer_net_query(naer_net* n, naer_stats* out)
     if (rc)
         return rc;
     std::memset(out, 0, sizeof *out);
-    out->n_neurons = n->N;
-    out->n_synapses = n->S;
-    out->n_plastic = n->n_plastic;
+    for (auto& net : n->nets)
+        if (net.active) {
+            out->n_neurons += net.n_neurons;
+            out->n_synapses += net.n_syn;
+            out->n_plastic += net.n_rev;
+        }
     out->ring_depth = n->D;
     out->batch = n->B;
     out->max_row = n->max_row;
@@ -899,10 +1298,10 @@ int naer_net_read_csr(naer_net* n, uint64_t* row_ptr, uint32_t* col, float* w)
         return rc;
     if (row_ptr)
         HIP_TRY(hipMemcpy(row_ptr, n->row_ptr, ((size_t)n->N + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost));
-    if (col && n->S)
-        HIP_TRY(hipMemcpy(col, n->col, n->S * sizeof(uint32_t), hipMemcpyDeviceToHost));
-    if (w && n->S)
-        HIP_TRY(hipMemcpy(w, n->w, n->S * sizeof(float), hipMemcpyDeviceToHost));
+    if (col && n->S_cap)
+        HIP_TRY(hipMemcpy(col, n->col, n->S_cap * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (w && n->S_cap)
+        HIP_TRY(hipMemcpy(w, n->w, n->S_cap * sizeof(float), hipMemcpyDeviceToHost));
     return NAER_OK;
 }
 
@@ -915,10 +1314,9 @@ int naer_net_set_probe(naer_net* n, uint64_t t_begin, uint32_t n_steps)
     int rc = naer_net_sync(n);
     if (rc)
         return rc;
-    if (n->graph_exec) {
-        HIP_TRY(hipGraphExecDestroy(n->graph_exec));
-        n->graph_exec = nullptr;
-    }
+    // captured graphs hold the probe pointer in their arguments
+    if ((rc = n->drop_graph()))
+        return rc;
     if (n->probe) {
         n->release(n->probe, (size_t)n->probe_n * n->N * sizeof(float));
         n->probe = nullptr;
@@ -980,8 +1378,8 @@ int naer_philox(naer_ctx* ctx, const uint32_t* ctr, const uint32_t* key, uint32_
     return NAER_OK;
 }
 
-int naer_ref_poisson_epoch(naer_ctx* ctx, uint32_t n, uint32_t n_pois_total, uint32_t seed,
-                           float rate_hz, uint32_t t0, uint32_t t1, uint32_t* times, uint32_t* count)
+int naer_ref_poisson_epoch(naer_ctx* ctx, uint32_t n, uint32_t n_pois_total, uint32_t seed, float rate_hz,
+                           uint32_t t0, uint32_t t1, uint32_t* times, uint32_t* count)
 {
     if (!ctx || (n && (!times || !count)))
         return fail(NAER_ERR_INVALID, "NULL argument");
@@ -994,8 +1392,7 @@ int naer_ref_poisson_epoch(naer_ctx* ctx, uint32_t n, uint32_t n_pois_total, uin
     if (e == hipSuccess)
         e = hipMemset(dt, 0, (size_t)n * kMaxPoissonPerEpoch * 4);
     if (e == hipSuccess)
-        e = launch_ref_poisson_epoch(n, n_pois_total, seed, -((double)rate_hz / 1000000.0), t0, t1, dt,
-                                     dc, nullptr);
+        e = launch_ref_poisson_epoch(n, n_pois_total, seed, -((double)rate_hz / 1000000.0), t0, t1, dt, dc, nullptr);
     if (e == hipSuccess)
         e = hipMemcpy(times, dt, (size_t)n * kMaxPoissonPerEpoch * 4, hipMemcpyDeviceToHost);
     if (e == hipSuccess)

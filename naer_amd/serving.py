@@ -1,0 +1,276 @@
+"""Serving: a request loop with continuous batching of simulation jobs.
+
+Jobs arrive as a Poisson process.  Each job is a balanced LIF network
+(N log-uniform in [2k, 200k], in-degree uniform in [500, 2000], duration
+uniform in [0.2, 2] s of simulated time, 30% with pair STDP on E->E; J_E =
+0.1 mV, g = 5, 20 kHz x 0.1 mV drive, delay 1.5 ms).
+
+The server keeps two arenas (static jobs, plastic jobs) whose capacities split
+the HBM budget.  Between chunks of `chunk` steps it admits waiting jobs that fit
+(first come first served, later jobs may pass a head that does not fit yet),
+runs every resident job in the same launches (the two arenas on their own
+streams, so their work overlaps on the device), drains the recorded spikes and
+retires the jobs that have reached their duration.  A job's raster depends only
+on its own description: alone or in any mix it is the same, bit for bit in
+deterministic mode.
+
+    python -m naer_amd.serving --rates 2,5,10,20 --jobs 48
+"""
+import argparse
+import collections
+import json
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi, models
+
+DELAY_STEPS = 15       # 1.5 ms at dt = 0.1 ms
+MAX_ROW = 8192
+BYTES_PER_SYNAPSE = 8  # uint32 target/delay word + fp32 weight
+BYTES_PER_PLASTIC = 8  # reverse index: position + source
+
+
+@dataclass
+class JobSpec:
+    index: int
+    arrival_s: float
+    n: int
+    indegree: int
+    duration_s: float
+    plastic: bool
+    seed: int
+
+    def steps(self, dt_ms=0.1):
+        return int(round(self.duration_s * 1000.0 / dt_ms))
+
+    def n_synapses(self):
+        return self.n * self.indegree
+
+    def n_plastic(self):
+        if not self.plastic:
+            return 0
+        n_e = int(round(self.n * 0.8))
+        return n_e * int(round(self.indegree * 0.8))
+
+    def network(self, mode=_abi.MODE_DETERMINISTIC, record_capacity=0):
+        return models.balanced(self.n, self.indegree, 0.1, 5.0, 20_000.0, 0.1, delay_ms=1.5,
+                               plastic=self.plastic, seed=self.seed, mode=mode,
+                               record_capacity=record_capacity)
+
+
+def served_jobs(n_jobs, rate_per_s, seed=0x5E11, n_range=(2_000, 200_000), k_range=(500, 2000),
+                dur_range=(0.2, 2.0), stdp_fraction=0.3):
+    """Job stream with Poisson arrivals at rate_per_s jobs per second."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    jobs = []
+    for i in range(n_jobs):
+        t += rng.exponential(1.0 / rate_per_s) if rate_per_s > 0 else 0.0
+        n = int(round(math.exp(rng.uniform(math.log(n_range[0]), math.log(n_range[1])))))
+        jobs.append(JobSpec(index=i, arrival_s=t, n=n, indegree=int(rng.integers(k_range[0], k_range[1] + 1)),
+                            duration_s=float(rng.uniform(*dur_range)), plastic=bool(rng.random() < stdp_fraction),
+                            seed=(seed * 7919 + i) & 0xFFFFFFFFFFFFFFFF))
+    return jobs
+
+
+def neuron_bytes(mode, plastic):
+    """Device bytes per neuron of arena capacity: ring (delay+1 slots), state,
+    row pointers, build scratch and two spike-list segments (B entries each)."""
+    ring = (DELAY_STEPS + 1) * (8 if mode == _abi.MODE_DETERMINISTIC else 4)
+    b = 1 if plastic else DELAY_STEPS
+    lists = 2 * b * 16
+    state = 4 * 5 + 8 + 4 + 8 + 16  # V ref last pnext pcnt, row_ptr, deg, cursor, counters
+    if plastic:
+        state += 8 + 8  # traces, reverse row pointers
+    return ring + lists + state
+
+
+def footprint(job, mode=_abi.MODE_DETERMINISTIC):
+    """Device bytes a job occupies in its arena."""
+    nb = neuron_bytes(mode, job.plastic)
+    n = (job.n + 1 + 255) // 256 * 256
+    return n * nb + job.n_synapses() * BYTES_PER_SYNAPSE + job.n_plastic() * BYTES_PER_PLASTIC
+
+
+@dataclass
+class Capacity:
+    neurons: int
+    synapses: int
+    plastic: int
+
+
+def split_budget(budget_bytes, mode=_abi.MODE_DETERMINISTIC, stdp_fraction=0.3, synapses_per_neuron=700):
+    """Capacities of the static and the plastic arena for a total budget,
+    shared in proportion to the expected footprint of the job mix."""
+    sb_static = BYTES_PER_SYNAPSE
+    sb_plastic = BYTES_PER_SYNAPSE + 0.64 * BYTES_PER_PLASTIC
+    w_s = (1 - stdp_fraction) * sb_static
+    w_p = stdp_fraction * sb_plastic
+    caps = []
+    for frac, plastic, sb in ((w_s / (w_s + w_p), False, sb_static), (w_p / (w_s + w_p), True, sb_plastic)):
+        b = budget_bytes * frac
+        per_syn = sb + neuron_bytes(mode, plastic) / synapses_per_neuron
+        syn = int(b / per_syn)
+        caps.append(Capacity(neurons=max(256, syn // synapses_per_neuron), synapses=syn,
+                             plastic=int(0.64 * syn) if plastic else 0))
+    return caps[0], caps[1]
+
+
+@dataclass
+class JobResult:
+    index: int
+    arrival_s: float
+    admitted_s: float = -1.0
+    done_s: float = -1.0
+    slot: int = -1
+    arena: int = -1
+    spikes: int = 0
+    raster: list = field(default_factory=list)
+
+    @property
+    def latency_s(self):
+        return self.done_s - self.arrival_s
+
+
+class Server:
+    """Continuous-batching request loop over two arenas (static, plastic).
+
+    arena_factory(capacity, plastic) builds an arena; the default is the device
+    Arena, tests pass a host stand-in to exercise admission logic without a GPU.
+    """
+
+    def __init__(self, budget_bytes=200e9, mode=_abi.MODE_DETERMINISTIC, chunk=100, graphs=True,
+                 keep_rasters=False, record_per_chunk=1 << 22, arena_factory=None, dt_ms=0.1,
+                 synapses_per_neuron=700):
+        self.mode = mode
+        self.chunk = chunk
+        self.dt_ms = dt_ms
+        self.keep = keep_rasters
+        static_cap, plastic_cap = split_budget(budget_bytes, mode, synapses_per_neuron=synapses_per_neuron)
+        if arena_factory is None:
+            from .arena import Arena
+
+            def arena_factory(cap, plastic):
+                return Arena(cap.neurons, cap.synapses, cap.plastic, dt_ms=dt_ms, mode=mode,
+                             delay_steps=(DELAY_STEPS, DELAY_STEPS), max_row=MAX_ROW, plastic=plastic,
+                             stdp=models.PAIR_STDP if plastic else None, record_capacity=record_per_chunk)
+        self.arenas = [arena_factory(static_cap, False), arena_factory(plastic_cap, True)]
+        self.capacity = (static_cap, plastic_cap)
+        if graphs:
+            for a in self.arenas:
+                if hasattr(a, "set_graph"):
+                    a.set_graph(chunk)
+
+    def close(self):
+        for a in self.arenas:
+            a.close()
+
+    def serve(self, jobs, clock=time.perf_counter, sleep=time.sleep):
+        """Serve the job stream (arrival times relative to the start) and
+        return per-job results and the summary."""
+        jobs = sorted(jobs, key=lambda j: j.arrival_s)
+        res = {j.index: JobResult(j.index, j.arrival_s) for j in jobs}
+        pending = collections.deque(jobs)
+        waiting = []
+        active = [dict(), dict()]  # per arena: slot -> (job, steps done)
+        rejected = []
+        t0 = clock()
+        while pending or waiting or any(active):
+            now = clock() - t0
+            while pending and pending[0].arrival_s <= now:
+                waiting.append(pending.popleft())
+            # admission: first come first served; a job that does not fit yet
+            # waits while later jobs may pass it
+            still = []
+            for job in waiting:
+                ai = 1 if job.plastic else 0
+                cap = self.capacity[ai]
+                if (job.n > cap.neurons or job.n_synapses() > cap.synapses or job.n_plastic() > cap.plastic
+                        or job.indegree * 4 > MAX_ROW):
+                    rejected.append(job.index)  # can never fit
+                    continue
+                slot = self.arenas[ai].admit(job.network(self.mode))
+                if slot is None:
+                    still.append(job)
+                    continue
+                active[ai][slot] = [job, 0]
+                r = res[job.index]
+                r.admitted_s, r.slot, r.arena = clock() - t0, slot, ai
+            waiting = still
+            if not any(active):
+                if pending:
+                    sleep(max(0.0, min(0.05, pending[0].arrival_s - (clock() - t0))))
+                continue
+            for ai, arena in enumerate(self.arenas):
+                if active[ai]:
+                    arena.run(self.chunk)
+            for ai, arena in enumerate(self.arenas):
+                if not active[ai]:
+                    continue
+                drained = arena.drain()
+                done = []
+                for slot, entry in active[ai].items():
+                    job, steps = entry
+                    total = job.steps(self.dt_ms)
+                    sp = drained.get(slot)
+                    if sp is not None and len(sp):
+                        sp = sp[sp["step"] < total]
+                        res[job.index].spikes += len(sp)
+                        if self.keep:
+                            res[job.index].raster.append(sp)
+                    entry[1] = steps + self.chunk
+                    if entry[1] >= total:
+                        done.append(slot)
+                for slot in done:
+                    job = active[ai].pop(slot)[0]
+                    arena.retire(slot)
+                    res[job.index].done_s = clock() - t0
+        makespan = clock() - t0
+        finished = [r for r in res.values() if r.done_s >= 0]
+        lat = np.array([r.latency_s for r in finished]) if finished else np.zeros(1)
+        for r in finished:
+            if self.keep:
+                r.raster = (np.sort(np.concatenate(r.raster), order=["step", "neuron", "time_us"])
+                            if r.raster else np.zeros(0, dtype=_abi.SPIKE_DTYPE))
+        return dict(results=res, rejected=rejected, makespan_s=makespan, completed=len(finished),
+                    jobs_per_s=len(finished) / makespan if makespan > 0 else 0.0,
+                    p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
+                    mean_s=float(lat.mean()))
+
+
+def run_alone(job, mode=_abi.MODE_DETERMINISTIC, dt_ms=0.1):
+    """The job simulated as a single network (reference for mix-vs-alone)."""
+    steps = job.steps(dt_ms)
+    net = job.network(mode, record_capacity=int(job.n * steps * 0.05) + 4096)
+    net.compile()
+    net.run(steps)
+    sp = net.spikes()
+    net.close()
+    return sp
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--rates", default="2,5,10,20")
+    ap.add_argument("--jobs", type=int, default=48)
+    ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--mode", choices=["det", "fast"], default="det")
+    ap.add_argument("--chunk", type=int, default=100)
+    a = ap.parse_args()
+    mode = _abi.MODE_DETERMINISTIC if a.mode == "det" else _abi.MODE_FAST
+    srv = Server(a.budget_gb * 1e9, mode=mode, chunk=a.chunk)
+    out = []
+    for rate in [float(x) for x in a.rates.split(",")]:
+        r = srv.serve(served_jobs(a.jobs, rate))
+        out.append(dict(rate=rate, jobs=a.jobs, completed=r["completed"], jobs_per_s=r["jobs_per_s"],
+                        p50_s=r["p50_s"], p99_s=r["p99_s"], makespan_s=r["makespan_s"],
+                        rejected=len(r["rejected"])))
+        print(json.dumps(out[-1]), flush=True)
+    srv.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -734,7 +734,7 @@ static void update_neuron(orc_net* n, uint32_t j, uint64_t t, uint32_t* list, ui
     n->ring_abs[slot] = 0.0;
 
     if (n->has_plastic) {
-        int prev = (int64_t)n->last[j] == (int64_t)t - 1;
+        int prev = n->last[j] >= 0 && (int64_t)n->last[j] == (int64_t)t - 1;  /* last < 0: never fired */
         if (n->prec == ORC_DOUBLE) {
             n->xhd[j] = (n->xhd[j] + (prev ? 1.0 : 0.0)) * n->dec_pd;
             n->yhd[j] = (n->yhd[j] + (prev ? 1.0 : 0.0)) * n->dec_md;
