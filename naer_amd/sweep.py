@@ -19,6 +19,7 @@ In deterministic mode every network's raster is bit-identical either way.
 import argparse
 import json
 import math
+import sys
 import time
 
 import numpy as np
@@ -52,7 +53,11 @@ def expected_record(spec, steps, rate_bound_hz=500.0):
     return int(spec["n"] * rate_bound_hz * steps * 1e-4) + 1024
 
 
-def run_packed(specs, steps, mode=_abi.MODE_DETERMINISTIC, chunk=100, record=True, graphs=True):
+def _progress(msg):
+    print("# " + msg, file=sys.stderr, flush=True)
+
+
+def run_packed(specs, steps, mode=_abi.MODE_DETERMINISTIC, chunk=100, record=True, graphs=True, verbose=False):
     groups = [[s for s in specs if not s["plastic"]], [s for s in specs if s["plastic"]]]
     packs = []
     t_build = time.perf_counter()
@@ -65,16 +70,24 @@ def run_packed(specs, steps, mode=_abi.MODE_DETERMINISTIC, chunk=100, record=Tru
         if graphs:
             p.set_graph(chunk)
         packs.append((g, p))
+        if verbose:
+            _progress(f"packed: built {len(g)} networks ({time.perf_counter() - t_build:.1f} s)")
     for _, p in packs:
         p.sync()
     build_s = time.perf_counter() - t_build
     t0 = time.perf_counter()
     done = 0
+    last = t0
     while done < steps:
         c = min(chunk, steps - done)
         for _, p in packs:
             p.run(c)
         done += c
+        if verbose and time.perf_counter() - last > 10:
+            for _, p in packs:
+                p.sync()
+            last = time.perf_counter()
+            _progress(f"packed: {done}/{steps} steps ({last - t0:.1f} s)")
     for _, p in packs:
         p.sync()
     run_s = time.perf_counter() - t0
@@ -91,7 +104,7 @@ def run_packed(specs, steps, mode=_abi.MODE_DETERMINISTIC, chunk=100, record=Tru
     return dict(build_s=build_s, run_s=run_s, packs=stats, rasters=rasters)
 
 
-def run_sequential(specs, steps, mode=_abi.MODE_DETERMINISTIC, record=True):
+def run_sequential(specs, steps, mode=_abi.MODE_DETERMINISTIC, record=True, verbose=False):
     rasters = {}
     build_s = run_s = 0.0
     events = 0
@@ -109,6 +122,8 @@ def run_sequential(specs, steps, mode=_abi.MODE_DETERMINISTIC, record=True):
         if record:
             rasters[s["index"]] = net.spikes()
         net.close()
+        if verbose and (s["index"] % 16 == 15):
+            _progress(f"sequential: {s['index'] + 1}/{len(specs)} networks (run {run_s:.1f} s)")
     return dict(build_s=build_s, run_s=run_s, events=events, rasters=rasters)
 
 
@@ -123,12 +138,12 @@ def main():
     a = ap.parse_args()
     mode = _abi.MODE_DETERMINISTIC if a.mode == "det" else _abi.MODE_FAST
     specs = sweep_specs(a.nets, n_max=a.n_max)
-    packed = run_packed(specs, a.steps, mode, graphs=not a.no_graphs)
+    packed = run_packed(specs, a.steps, mode, graphs=not a.no_graphs, verbose=True)
     out = dict(nets=a.nets, steps=a.steps, mode=a.mode, packed_run_s=packed["run_s"],
                packed_build_s=packed["build_s"], packed_jobs_per_s=a.nets / packed["run_s"],
                packs=packed["packs"])
     if not a.no_sequential:
-        seq = run_sequential(specs, a.steps, mode)
+        seq = run_sequential(specs, a.steps, mode, verbose=True)
         ident = sum(int(np.array_equal(packed["rasters"][i], seq["rasters"][i])) for i in range(a.nets))
         out.update(seq_run_s=seq["run_s"], seq_build_s=seq["build_s"], seq_jobs_per_s=a.nets / seq["run_s"],
                    speedup=seq["run_s"] / packed["run_s"], rasters_identical=ident)
