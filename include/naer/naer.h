@@ -233,8 +233,11 @@ int naer_net_create(naer_ctx* ctx, const naer_net_desc* desc, naer_net** out);
  * counterpart in the reference (one network per process, src/main.cpp). */
 int naer_pack_create(naer_ctx* ctx, const naer_net_desc* descs, uint32_t n_nets, naer_net** out);
 /* Per network (pack) or slot (arena): first neuron, neuron count (0 for a
- * retired slot) and the pack step at which its clock started. */
-int naer_pack_layout(naer_net* pack, uint32_t* neuron_begin, uint32_t* n_neurons, uint64_t* t_start);
+ * retired slot) and the pack step at which its clock started.  Writes the
+ * first min(cap, slots) entries of each non-NULL array and sets *n_slots
+ * (if non-NULL) to the number of slots. */
+int naer_pack_layout(naer_net* pack, uint32_t cap, uint32_t* neuron_begin, uint32_t* n_neurons,
+                     uint64_t* t_start, uint32_t* n_slots);
 
 /* An arena: preallocated capacity in which networks (serving jobs) are
  * admitted and retired between steps; the form used for continuous batching.

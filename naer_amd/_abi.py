@@ -8,7 +8,8 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libnaer.so")
+# NAER_LIB: load another build of the library (e.g. a variant under test)
+LIB_PATH = os.environ.get("NAER_LIB") or os.path.join(_HERE, "lib", "libnaer.so")
 
 NAER_OK = 0
 NAER_ERR_INVALID = -1
@@ -109,7 +110,8 @@ _SIGS = {
     "naer_ctx_destroy": [ctypes.c_void_p],
     "naer_net_create": [ctypes.c_void_p, ctypes.POINTER(NetDesc), ctypes.POINTER(ctypes.c_void_p)],
     "naer_pack_create": [ctypes.c_void_p, ctypes.POINTER(NetDesc), u32, ctypes.POINTER(ctypes.c_void_p)],
-    "naer_pack_layout": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+    "naer_pack_layout": [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                         ctypes.c_void_p],
     "naer_arena_create": [ctypes.c_void_p, ctypes.POINTER(ArenaDesc), ctypes.POINTER(ctypes.c_void_p)],
     "naer_arena_admit": [ctypes.c_void_p, ctypes.POINTER(NetDesc), ctypes.POINTER(u32)],
     "naer_arena_retire": [ctypes.c_void_p, u32],

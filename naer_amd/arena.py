@@ -62,12 +62,15 @@ class Arena:
         self.slots.pop(slot, None)
 
     def layout(self):
-        n = max(self.slots, default=-1) + 1
+        """(neuron_begin, n_neurons, t_start) per slot, retired slots included."""
+        n = ctypes.c_uint32()
+        check(_abi.lib().naer_pack_layout(self._handle, 0, None, None, None, ctypes.byref(n)))
+        n = n.value
         begin = np.zeros(max(1, n), np.uint32)
         size = np.zeros(max(1, n), np.uint32)
         t0 = np.zeros(max(1, n), np.uint64)
-        if n:
-            check(_abi.lib().naer_pack_layout(self._handle, begin.ctypes.data, size.ctypes.data, t0.ctypes.data))
+        check(_abi.lib().naer_pack_layout(self._handle, n, begin.ctypes.data, size.ctypes.data, t0.ctypes.data,
+                                          None))
         return begin[:n], size[:n], t0[:n]
 
     def set_graph(self, chunk_steps):

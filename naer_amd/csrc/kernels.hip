@@ -90,15 +90,20 @@ __device__ __forceinline__ void emit(Stage& sg, bool fire, uint32_t neuron, uint
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// Update: one thread per neuron, B consecutive steps per launch.  The ring
-// slots of all B steps are loaded up front (they cannot receive anything
-// during the batch because B <= the minimum delay), so each thread has B
-// independent loads in flight.
+// Update: one block per tile of TW neurons, one thread per neuron, B
+// consecutive steps per launch.  Synaptic input of a step = the neuron's
+// delay-ring slot (events delivered by atomics) + the events of the tile's
+// buckets (events sorted by the bucketed delivery, summed here in LDS).  Ring
+// slots and buckets of all B steps are read up front: nothing can arrive
+// during the batch because B <= the minimum delay.
 // ---------------------------------------------------------------------------
-template <bool DET, bool PLASTIC>
-__global__ __launch_bounds__(kBlock) void k_update(const UpdateArgs a)
+template <bool DET, bool PLASTIC, bool BUCKET, uint32_t TW>
+__global__ __launch_bounds__(TW, (BUCKET && !DET && TW == 1024) ? 8 : 1) void k_update(const UpdateArgs a)
 {
     using RingT = typename std::conditional<DET, long long, float>::type;
+    using AccT = typename std::conditional<DET, unsigned long long, float>::type;
+    extern __shared__ unsigned long long s_dyn[];
+    AccT* acc = (AccT*)s_dyn;  // [B][TW] when BUCKET
     __shared__ naer_spike s_list[kLdsSpikes];
     __shared__ float s_cdf[kStagePops * NAER_DRIVE_TABLE];
     __shared__ uint32_t s_n;
@@ -111,20 +116,98 @@ __global__ __launch_bounds__(kBlock) void k_update(const UpdateArgs a)
     const uint32_t p_first = a.n_pops ? a.blk_pop[blockIdx.x] : 0u;
     {
         const uint32_t np = a.n_pops ? min(kStagePops, a.n_pops - p_first) : 0u;
-        for (uint32_t i = threadIdx.x; i < np * NAER_DRIVE_TABLE; i += kBlock)
+        for (uint32_t i = threadIdx.x; i < np * NAER_DRIVE_TABLE; i += TW)
             s_cdf[i] = a.cdf[(size_t)p_first * NAER_DRIVE_TABLE + i];
     }
-    __syncthreads();
-
     DevState* st = a.st;
     const unsigned long long t0 = st->clock;
+    // bucketed events: the M buckets (batches A-1 .. A-M) of this tile,
+    // flattened: event i of the tile lies in bucket m with
+    // s_pre[m] <= i < s_pre[m + 1], at ev + s_evbase[m] + i
+    __shared__ uint32_t s_pre[kMaxArrivalBatches + 1];
+    __shared__ long long s_evbase[kMaxArrivalBatches];
+    __shared__ uint32_t s_ring_live;
+    const unsigned long long A = t0 / a.Bfull;
+    const uint32_t o0 = (uint32_t)(t0 - A * a.Bfull);
+    if (BUCKET) {
+        for (uint32_t i = threadIdx.x; i < a.B * TW; i += TW)
+            acc[i] = (AccT)0;
+        if (threadIdx.x == 0)
+            s_ring_live = 0;
+        __syncthreads();
+        if (threadIdx.x < a.M) {
+            const uint32_t m = threadIdx.x + 1;
+            uint32_t lo = 0, hi = 0;
+            uint32_t buf = 0;
+            if (A >= m) {
+                buf = (uint32_t)((A - m) % a.R);
+                if (a.flags[buf]) {
+                    const uint32_t* bs = a.bs + (size_t)buf * (a.NBK + 1);
+                    const uint32_t k = (m - 1) * a.NT + blockIdx.x;
+                    lo = bs[k];
+                    hi = bs[k + 1];
+                } else {
+                    s_ring_live = 1;  // that batch delivered through the ring
+                }
+            }
+            s_pre[m] = hi - lo;
+            s_evbase[m - 1] = (long long)buf * (long long)a.e_cap + lo;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            s_pre[0] = 0;
+            for (uint32_t m = 0; m < a.M; ++m) {
+                const uint32_t n = s_pre[m + 1];
+                s_evbase[m] -= s_pre[m];
+                s_pre[m + 1] = s_pre[m] + n;
+            }
+        }
+        __syncthreads();
+        const uint32_t total = s_pre[a.M];
+        constexpr uint32_t U = 4;
+        for (uint32_t i0 = threadIdx.x; i0 < total; i0 += TW * U) {
+            uint2 e[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u) {
+                const uint32_t i = i0 + u * TW;
+                e[u] = make_uint2(0xFFFFFFFFu, 0u);
+                if (i < total) {
+                    uint32_t m = 0;
+                    while (i >= s_pre[m + 1])
+                        ++m;
+                    e[u] = a.ev[s_evbase[m] + i];
+                }
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u) {
+                const uint32_t o = (e[u].x >> 16) - o0;
+                if (o < a.B) {
+                    const float w = __uint_as_float(e[u].y);
+                    if (DET)
+                        atomicAdd((unsigned long long*)&acc[o * TW + (e[u].x & 0xFFFFu)],
+                                  (unsigned long long)to_fixed(w));
+                    else
+                        atomicAdd((float*)&acc[o * TW + (e[u].x & 0xFFFFu)], w);
+                }
+            }
+        }
+        __syncthreads();
+    } else {
+        __syncthreads();
+    }
+    // ring slots of the batch hold events only if a batch they can come
+    // from went through the ring
+    const bool ring_live = !BUCKET || s_ring_live != 0;
+
     const uint32_t pl = st->batch & 1u;
     naer_spike* seg = (pl ? a.list1 : a.list0) + (size_t)blockIdx.x * a.seg_cap;
-    if (blockIdx.x == 0 && threadIdx.x == 0)
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         st->cur = pl;
+        st->t0_cur = t0;
+    }
     Stage sg{ s_list, &s_n, seg, a.seg_cap };
 
-    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t j = blockIdx.x * TW + threadIdx.x;
     unsigned long long my_events = 0;
     // neurons outside every population (free ranges of an arena) stay idle
     const uint32_t p = (j < a.N && a.n_pops) ? find_pop(a.pops, a.n_pops, p_first, j) : 0u;
@@ -136,18 +219,27 @@ __global__ __launch_bounds__(kBlock) void k_update(const UpdateArgs a)
         if (pp.kind == NAER_POP_LIF) {
             RingT* ring = (RingT*)a.ring;
             uint32_t slot0 = (uint32_t)(t0 % a.D);
-            RingT in[kMaxBatch];
+            // ring slots of the batch: into registers, or (bucketed) added to
+            // the thread's own LDS accumulators
+            RingT in[BUCKET ? 1 : kMaxBatch];
+            if (!BUCKET) {
 #pragma unroll
-            for (uint32_t s = 0; s < kMaxBatch; ++s) {
-                if (s < a.B) {
+                for (uint32_t s = 0; s < kMaxBatch; ++s) {
+                    if (s < a.B) {
+                        uint32_t sl = slot0 + s;
+                        sl = sl >= a.D ? sl - a.D : sl;
+                        in[s] = ring[(size_t)sl * a.N + j];
+                    }
+                }
+            } else if (ring_live) {
+                for (uint32_t s = 0; s < a.B; ++s) {
                     uint32_t sl = slot0 + s;
                     sl = sl >= a.D ? sl - a.D : sl;
-                    in[s] = ring[(size_t)sl * a.N + j];
+                    acc[s * TW + threadIdx.x] += (AccT)ring[(size_t)sl * a.N + j];
                 }
             }
-#pragma unroll
-            for (uint32_t s = 0; s < kMaxBatch; ++s) {
-                if (s < a.B) {
+            if (ring_live) {
+                for (uint32_t s = 0; s < a.B; ++s) {
                     uint32_t sl = slot0 + s;
                     sl = sl >= a.D ? sl - a.D : sl;
                     ring[(size_t)sl * a.N + j] = (RingT)0;
@@ -174,15 +266,19 @@ __global__ __launch_bounds__(kBlock) void k_update(const UpdateArgs a)
 #pragma nounroll
             for (uint32_t s = 0; s < a.B; ++s) {
                 const unsigned long long t = t0 + s;
-                RingT cur = in[0];
-#pragma unroll
-                for (uint32_t q = 0; q + 1 < kMaxBatch; ++q)
-                    in[q] = in[q + 1];
                 float I;
-                if (DET)
-                    I = from_fixed((long long)cur);
-                else
-                    I = (float)cur;
+                if (BUCKET) {
+                    if (DET)
+                        I = from_fixed((long long)acc[s * TW + threadIdx.x]);
+                    else
+                        I = (float)acc[s * TW + threadIdx.x];
+                } else {
+                    const RingT cur = in[0];
+#pragma unroll
+                    for (uint32_t q = 0; q + 1 < (BUCKET ? 1 : kMaxBatch); ++q)
+                        in[q] = in[q + 1];
+                    I = DET ? from_fixed((long long)cur) : (float)cur;
+                }
                 if (a.probe && t - a.probe_t0 < a.probe_n)
                     a.probe[(size_t)(t - a.probe_t0) * a.N + j] = I;
                 if (PLASTIC) {
@@ -269,7 +365,7 @@ __global__ __launch_bounds__(kBlock) void k_update(const UpdateArgs a)
     const uint32_t total = s_n;
     const uint32_t n = total < a.seg_cap ? total : a.seg_cap;
     const uint32_t staged = n < kLdsSpikes ? n : kLdsSpikes;
-    for (uint32_t i = threadIdx.x; i < staged; i += kBlock)
+    for (uint32_t i = threadIdx.x; i < staged; i += TW)
         seg[i] = s_list[i];
     if (threadIdx.x == 0) {
         a.seg_count[pl * a.n_seg + blockIdx.x] = n;
@@ -282,7 +378,7 @@ __global__ __launch_bounds__(kBlock) void k_update(const UpdateArgs a)
     if (a.rec_cap && n) {
         // spikes beyond the LDS stage were written to the segment directly
         __syncthreads();
-        for (uint32_t i = threadIdx.x; i < n; i += kBlock) {
+        for (uint32_t i = threadIdx.x; i < n; i += TW) {
             const unsigned long long r = s_base + i;
             if (r < a.rec_cap)
                 a.rec[r] = i < kLdsSpikes ? s_list[i] : seg[i];
@@ -315,8 +411,12 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const ScatterArgs a)
     const uint32_t cur = st->cur;
     const naer_spike* list = cur ? a.list1 : a.list0;
     const uint32_t* counts = a.seg_count + cur * a.n_seg;
-    if (a.is_last && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0)
-        batch_epilogue(st, a.B);
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+        if (a.flags)  // this batch's events go through the ring, not the buckets
+            a.flags[(st->t0_cur / a.Bfull) % a.R] = 0;
+        if (a.is_last)
+            batch_epilogue(st, a.B);
+    }
 
     RingT* ring = (RingT*)a.ring;
     const uint32_t lane = lane_id();
@@ -391,6 +491,351 @@ __global__ __launch_bounds__(kBlock) void k_potentiate(const PotentiateArgs a)
                 a.w[pos] = clip(a.w[pos] + dw, a.w_min, a.w_max);
             }
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Bucketed delivery (see BucketArgs): histogram, column scan, bucket starts,
+// placement.  Block b walks the spikes of list segments [b*G, (b+1)*G) as
+// tasks t = i*C + c (spike i, 256-synapse chunk c of its row); wave w takes
+// tasks w, w + waves, ...  Both passes use the same split, so each block
+// places exactly the events it counted.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kBucketThreads = 1024;
+constexpr uint32_t kArrivalLut = 256;  // offsets r + d < kMaxBatch + 128
+
+__device__ __forceinline__ void bucket_setup(const BucketArgs& a, uint32_t cur, uint8_t* lut_m, uint8_t* lut_o,
+                                             uint32_t* s_off)
+{
+    for (uint32_t v = threadIdx.x; v < kArrivalLut; v += blockDim.x) {
+        lut_m[v] = (uint8_t)(v / a.Bfull);
+        lut_o[v] = (uint8_t)(v % a.Bfull);
+    }
+    const uint32_t g0 = blockIdx.x * a.G;
+    const uint32_t G = min(a.G, a.n_seg - g0);
+    if (threadIdx.x < G)
+        s_off[threadIdx.x + 1] = a.seg_count[cur * a.n_seg + g0 + threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        s_off[0] = 0;
+        for (uint32_t g = 1; g <= G; ++g)
+            s_off[g] += s_off[g - 1];
+    }
+}
+
+struct Task {
+    naer_spike sp;
+    unsigned long long beg, end;  // synapse range of the chunk
+};
+
+// Task t of this block (wave-uniform); false when its chunk is empty.
+__device__ __forceinline__ bool block_task(const BucketArgs& a, const naer_spike* list, const uint32_t* s_off,
+                                           uint32_t G, uint32_t t, Task& k)
+{
+    const uint32_t i = t / a.C, c = t - i * a.C;
+    uint32_t lo = 0, hi = G;  // segment g with s_off[g] <= i < s_off[g + 1]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s_off[mid] <= i)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    k.sp = list[(size_t)(blockIdx.x * a.G + lo) * a.seg_cap + (i - s_off[lo])];
+    const unsigned long long rb = a.row_ptr[k.sp.neuron], re = a.row_ptr[k.sp.neuron + 1];
+    k.beg = rb + (unsigned long long)c * kChunk;
+    k.end = min(re, k.beg + kChunk);
+    return k.beg < k.end;
+}
+
+__global__ __launch_bounds__(kBucketThreads) void k_bucket_hist(const BucketArgs a)
+{
+    extern __shared__ uint32_t s_hist[];
+    __shared__ uint8_t lut_m[kArrivalLut], lut_o[kArrivalLut];
+    __shared__ uint32_t s_off[kMaxSegPerBlock + 1];
+    const DevState* st = a.st;
+    const uint32_t cur = st->cur;
+    const unsigned long long t0 = st->t0_cur;
+    for (uint32_t i = threadIdx.x; i < a.NBK; i += blockDim.x)
+        s_hist[i] = 0;
+    bucket_setup(a, cur, lut_m, lut_o, s_off);
+    __syncthreads();
+    const naer_spike* list = cur ? a.list1 : a.list0;
+    const uint32_t G = min(a.G, a.n_seg - blockIdx.x * a.G);
+    const uint32_t n_tasks = s_off[G] * a.C;
+    const uint32_t lane = lane_id();
+    const uint32_t nwv = blockDim.x / kWave;
+    for (uint32_t t = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave); t < n_tasks; t += nwv) {
+        Task k;
+        if (!block_task(a, list, s_off, G, t, k))
+            continue;
+        const uint32_t r = (uint32_t)(k.sp.step - t0);
+        uint32_t c[kChunkUnroll];
+#pragma unroll
+        for (uint32_t u = 0; u < kChunkUnroll; ++u) {
+            const unsigned long long q = k.beg + u * kWave + lane;
+            c[u] = q < k.end ? a.col[q] : 0u;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kChunkUnroll; ++u) {
+            const unsigned long long q = k.beg + u * kWave + lane;
+            if (q < k.end) {
+                const uint32_t v = r + ((c[u] >> 24) & 0x7Fu);
+                const uint32_t T = (c[u] & kTargetMask) >> a.tw_shift;
+                atomicAdd(&s_hist[(lut_m[v] - 1u) * a.NT + T], 1u);
+            }
+        }
+    }
+    __syncthreads();
+    uint32_t* h = a.hist + (size_t)blockIdx.x * a.NBK;
+    for (uint32_t i = threadIdx.x; i < a.NBK; i += blockDim.x)
+        h[i] = s_hist[i];
+    const uint32_t nbc = a.NBK >> kGroupShift;
+    uint32_t* hc = a.hist_c + (size_t)blockIdx.x * nbc;
+    for (uint32_t i = threadIdx.x; i < nbc; i += blockDim.x) {
+        uint32_t sum = 0;
+#pragma unroll
+        for (uint32_t t = 0; t < kGroupTiles; ++t)
+            sum += s_hist[(i << kGroupShift) + t];
+        hc[i] = sum;
+    }
+}
+
+// Blocks [0, ceil(NBK/kGroupTiles / 64)): hist_c[b][kc] <- sum of
+// hist_c[b'][kc] for b' < b.  The remaining blocks: totals[k] <- sum over b
+// of hist_f[b][k].  A block covers 64 columns with 16 row groups.
+constexpr uint32_t kScanCols = 64, kScanGroups = 16;
+
+__global__ __launch_bounds__(kScanCols * kScanGroups) void k_bucket_colscan(const BucketArgs a)
+{
+    __shared__ uint32_t s_part[kScanGroups][kScanCols];
+    const uint32_t nbc = a.NBK >> kGroupShift;
+    const uint32_t coarse_blocks = (nbc + kScanCols - 1) / kScanCols;
+    const bool coarse = blockIdx.x < coarse_blocks;
+    uint32_t* hist = coarse ? a.hist_c : a.hist;
+    const uint32_t width = coarse ? nbc : a.NBK;
+    const uint32_t col = threadIdx.x % kScanCols, grp = threadIdx.x / kScanCols;
+    const uint32_t k = (coarse ? blockIdx.x : blockIdx.x - coarse_blocks) * kScanCols + col;
+    const uint32_t rows = (a.NB + kScanGroups - 1) / kScanGroups;
+    const uint32_t b0 = min(grp * rows, a.NB), b1 = min(b0 + rows, a.NB);
+    uint32_t sum = 0;
+    if (k < width) {
+        for (uint32_t b = b0; b < b1; b += 8) {
+            uint32_t c[8];
+#pragma unroll
+            for (uint32_t u = 0; u < 8; ++u)
+                c[u] = b + u < b1 ? hist[(size_t)(b + u) * width + k] : 0u;
+#pragma unroll
+            for (uint32_t u = 0; u < 8; ++u)
+                sum += c[u];
+        }
+    }
+    s_part[grp][col] = sum;
+    __syncthreads();
+    if (grp == 0) {
+        uint32_t run = 0;
+        for (uint32_t g = 0; g < kScanGroups; ++g) {
+            const uint32_t x = s_part[g][col];
+            s_part[g][col] = run;
+            run += x;
+        }
+        if (!coarse && k < width)
+            a.totals[k] = run;
+    }
+    if (!coarse)
+        return;
+    __syncthreads();
+    if (k >= width)
+        return;
+    uint32_t run = s_part[grp][col];
+    for (uint32_t b = b0; b < b1; b += 8) {
+        uint32_t c[8];
+#pragma unroll
+        for (uint32_t u = 0; u < 8; ++u)
+            c[u] = b + u < b1 ? hist[(size_t)(b + u) * width + k] : 0u;
+#pragma unroll
+        for (uint32_t u = 0; u < 8; ++u) {
+            if (b + u < b1)
+                hist[(size_t)(b + u) * width + k] = run;
+            run += c[u];
+        }
+    }
+}
+
+// Exclusive scan of the bucket totals (one block, staged in LDS) into the
+// bucket starts of the batch's buffer, and the flag saying whether the batch
+// fits the buffer.
+__global__ __launch_bounds__(1024) void k_bucket_starts(const BucketArgs a)
+{
+    extern __shared__ uint32_t s_tot[];  // [NBK]
+    __shared__ unsigned long long s_part[1024];
+    for (uint32_t k = threadIdx.x; k < a.NBK; k += 1024)
+        s_tot[k] = a.totals[k];
+    __syncthreads();
+    const uint32_t per = (a.NBK + 1023) / 1024;
+    const uint32_t lo = min(threadIdx.x * per, a.NBK), hi = min(lo + per, a.NBK);
+    unsigned long long sum = 0;
+    for (uint32_t k = lo; k < hi; ++k)
+        sum += s_tot[k];
+    s_part[threadIdx.x] = sum;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        const unsigned long long v = threadIdx.x >= off ? s_part[threadIdx.x - off] : 0ull;
+        __syncthreads();
+        s_part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    unsigned long long run = s_part[threadIdx.x] - sum;  // exclusive
+    for (uint32_t k = lo; k < hi; ++k) {
+        const uint32_t x = s_tot[k];
+        s_tot[k] = (uint32_t)min(run, 0xFFFFFFFFull);
+        run += x;
+    }
+    __syncthreads();
+    const unsigned long long t0 = a.st->t0_cur;
+    const uint32_t buf = (uint32_t)((t0 / a.Bfull) % a.R);
+    uint32_t* bs = a.bs + (size_t)buf * (a.NBK + 1);
+    for (uint32_t k = threadIdx.x; k < a.NBK; k += 1024)
+        bs[k] = s_tot[k];
+    if (threadIdx.x == 1023) {
+        const unsigned long long total = s_part[1023];
+        bs[a.NBK] = (uint32_t)min(total, 0xFFFFFFFFull);
+        a.flags[buf] = total <= a.e_cap ? 1u : 0u;
+    }
+}
+
+template <bool DET>
+__global__ __launch_bounds__(kBucketThreads) void k_bucket_place(const BucketArgs a)
+{
+    using RingT = typename std::conditional<DET, unsigned long long, float>::type;
+    extern __shared__ uint32_t s_cur[];
+    __shared__ uint8_t lut_m[kArrivalLut], lut_o[kArrivalLut];
+    __shared__ uint32_t s_off[kMaxSegPerBlock + 1];
+    DevState* st = a.st;
+    const uint32_t cur = st->cur;
+    const unsigned long long t0 = st->t0_cur;
+    const uint32_t buf = (uint32_t)((t0 / a.Bfull) % a.R);
+    const bool ok = a.flags[buf] != 0;
+    const uint32_t nbc = a.NBK >> kGroupShift;
+    const uint32_t ng = a.NT >> kGroupShift;
+    const uint32_t g_shift = a.tw_shift + kGroupShift;
+    if (ok) {
+        const uint32_t* bs = a.bs + (size_t)buf * (a.NBK + 1);
+        const uint32_t* h = a.hist_c + (size_t)blockIdx.x * nbc;
+        for (uint32_t i = threadIdx.x; i < nbc; i += blockDim.x)
+            s_cur[i] = bs[i << kGroupShift] + h[i];
+    }
+    bucket_setup(a, cur, lut_m, lut_o, s_off);
+    __syncthreads();
+    if (a.is_last && blockIdx.x == 0 && threadIdx.x == 0)
+        batch_epilogue(st, a.B);
+    uint2* stage = a.stage;
+    RingT* ring = (RingT*)a.ring;
+    const naer_spike* list = cur ? a.list1 : a.list0;
+    const uint32_t G = min(a.G, a.n_seg - blockIdx.x * a.G);
+    const uint32_t n_tasks = s_off[G] * a.C;
+    const uint32_t lane = lane_id();
+    const uint32_t nwv = blockDim.x / kWave;
+    for (uint32_t t = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave); t < n_tasks; t += nwv) {
+        Task k;
+        if (!block_task(a, list, s_off, G, t, k))
+            continue;
+        const uint32_t r = (uint32_t)(k.sp.step - t0);
+        uint32_t c[kChunkUnroll];
+        float w[kChunkUnroll];
+#pragma unroll
+        for (uint32_t u = 0; u < kChunkUnroll; ++u) {
+            const unsigned long long q = k.beg + u * kWave + lane;
+            c[u] = q < k.end ? a.col[q] : 0u;
+            w[u] = q < k.end ? a.w[q] : 0.0f;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kChunkUnroll; ++u) {
+            const unsigned long long q = k.beg + u * kWave + lane;
+            if (q < k.end) {
+                const uint32_t d = (c[u] >> 24) & 0x7Fu;
+                const uint32_t tgt = c[u] & kTargetMask;
+                if (ok) {
+                    const uint32_t v = r + d;
+                    const uint32_t g = tgt >> g_shift;
+                    const uint32_t pos = atomicAdd(&s_cur[(lut_m[v] - 1u) * ng + g], 1u);
+                    stage[pos] = make_uint2((tgt - (g << g_shift)) | ((uint32_t)lut_o[v] << 16),
+                                            __float_as_uint(w[u]));
+                } else {
+                    // the batch does not fit the event buffer: deliver through the ring
+                    uint32_t sl = k.sp.step % a.D + d;
+                    sl = sl >= a.D ? sl - a.D : sl;
+                    RingT* dst = ring + (size_t)sl * a.N + tgt;
+                    if (DET)
+                        atomicAdd((unsigned long long*)dst, (unsigned long long)to_fixed(w[u]));
+                    else
+                        unsafeAtomicAdd((float*)dst, w[u]);
+                }
+            }
+        }
+    }
+}
+
+// One block per group bucket (m, g): its events, in the staging buffer at the
+// range of buckets [kc * kGroupTiles, (kc + 1) * kGroupTiles), go to their
+// tile's bucket of ev[E % R] (same positions overall, tile order within).
+constexpr uint32_t kRefineThreads = 512;
+
+__global__ __launch_bounds__(kRefineThreads) void k_bucket_refine(const BucketArgs a)
+{
+    __shared__ uint32_t s_cur[kGroupTiles];
+    const unsigned long long t0 = a.st->t0_cur;
+    const uint32_t buf = (uint32_t)((t0 / a.Bfull) % a.R);
+    if (!a.flags[buf])
+        return;
+    const uint32_t* bs = a.bs + (size_t)buf * (a.NBK + 1);
+    const uint32_t k0 = blockIdx.x << kGroupShift;
+    if (threadIdx.x < kGroupTiles)
+        s_cur[threadIdx.x] = bs[k0 + threadIdx.x];
+    const uint32_t lo = bs[k0], hi = bs[k0 + kGroupTiles];
+    __syncthreads();
+    uint2* ev = a.ev + (size_t)buf * a.e_cap;
+    const uint32_t tmask = (1u << a.tw_shift) - 1u;
+    constexpr uint32_t U = 4;
+    for (uint32_t i0 = lo + threadIdx.x; i0 < hi; i0 += kRefineThreads * U) {
+        uint2 e[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            const uint32_t i = i0 + u * kRefineThreads;
+            e[u] = i < hi ? a.stage[i] : make_uint2(0u, 0u);
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            if (i0 + u * kRefineThreads < hi) {
+                const uint32_t t = (e[u].x & 0xFFFFu) >> a.tw_shift;
+                const uint32_t pos = atomicAdd(&s_cur[t], 1u);
+                ev[pos] = make_uint2((e[u].x & tmask) | (e[u].x & 0xFFFF0000u), e[u].y);
+            }
+        }
+    }
+}
+
+// Zeroes the weight of every bucketed event whose target lies in [j_begin,
+// j_end): run when a network is (re)initialised on that range, so events of
+// a network that held the range before cannot reach the new one (the ring
+// cells of the range are cleared by k_init_state).  Grid: (tiles overlapping
+// the range, R * M buckets of each tile).
+__global__ __launch_bounds__(kBlock) void k_bucket_scrub(const UpdateArgs a, uint32_t j_begin, uint32_t j_end)
+{
+    const uint32_t T = (j_begin >> a.tw_shift) + blockIdx.x;
+    const uint32_t buf = blockIdx.y / a.M;
+    const uint32_t m = blockIdx.y % a.M;
+    if (!a.flags[buf])
+        return;
+    const uint32_t* bs = a.bs + (size_t)buf * (a.NBK + 1);
+    const uint32_t k = m * a.NT + T;
+    const uint32_t lo = bs[k], hi = bs[k + 1];
+    uint2* ev = (uint2*)a.ev + (size_t)buf * a.e_cap;
+    const uint32_t base = T << a.tw_shift;
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += kBlock) {
+        const uint32_t j = base + (ev[i].x & 0xFFFFu);
+        if (j >= j_begin && j < j_end)
+            ev[i].y = 0u;
     }
 }
 
@@ -532,7 +977,7 @@ __global__ __launch_bounds__(kBlock) void k_init_state(const UpdateArgs a, uint3
         return;
     float v = 0.0f;
     if (a.n_pops) {
-        const uint32_t p = find_pop(a.pops, a.n_pops, a.blk_pop[j / kBlock], j);
+        const uint32_t p = find_pop(a.pops, a.n_pops, a.blk_pop[j >> a.tw_shift], j);
         const PopParams pp = a.pops[p];
         if (pp.kind == NAER_POP_LIF && j >= pp.begin && j < pp.end) {
             const U4 r = philox(U4{ j - pp.net_base, 0u, NAER_STREAM_V0, 0u }, pp.seed_lo, pp.seed_hi);
@@ -598,20 +1043,79 @@ __global__ __launch_bounds__(kBlock) void k_ref_poisson_epoch(uint32_t n, uint32
 // ---------------------------------------------------------------------------
 static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
-hipError_t launch_update(const UpdateArgs& a, bool det, bool plastic, hipStream_t s)
+template <bool DET, bool PLASTIC, bool BUCKET, uint32_t TW>
+static hipError_t launch_update_t(const UpdateArgs& a, hipStream_t s)
 {
-    const dim3 grid(cdiv(a.N, kBlock));
-    if (det) {
-        if (plastic)
-            hipLaunchKernelGGL((k_update<true, true>), grid, dim3(kBlock), 0, s, a);
-        else
-            hipLaunchKernelGGL((k_update<true, false>), grid, dim3(kBlock), 0, s, a);
-    } else {
-        if (plastic)
-            hipLaunchKernelGGL((k_update<false, true>), grid, dim3(kBlock), 0, s, a);
-        else
-            hipLaunchKernelGGL((k_update<false, false>), grid, dim3(kBlock), 0, s, a);
+    const size_t lds = BUCKET ? (size_t)a.B * TW * (DET ? 8 : 4) : 0;
+    static size_t lds_set = 0;  // per instantiation: largest size allowed so far
+    if (lds > 32768 && lds > lds_set) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_update<DET, PLASTIC, BUCKET, TW>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess)
+            return e;
+        lds_set = lds;
     }
+    hipLaunchKernelGGL((k_update<DET, PLASTIC, BUCKET, TW>), dim3(a.n_seg), dim3(TW), lds, s, a);
+    return hipGetLastError();
+}
+
+template <bool DET, bool PLASTIC, bool BUCKET>
+static hipError_t launch_update_tw(const UpdateArgs& a, hipStream_t s)
+{
+    return (1u << a.tw_shift) == 1024u ? launch_update_t<DET, PLASTIC, BUCKET, 1024>(a, s)
+                                        : launch_update_t<DET, PLASTIC, BUCKET, 256>(a, s);
+}
+
+hipError_t launch_update(const UpdateArgs& a, bool det, bool plastic, bool bucket, hipStream_t s)
+{
+    if ((1u << a.tw_shift) != 1024u && (1u << a.tw_shift) != 256u)
+        return hipErrorInvalidValue;
+    if (plastic)  // plastic nets deliver through the ring only
+        return det ? launch_update_tw<true, true, false>(a, s) : launch_update_tw<false, true, false>(a, s);
+    if (bucket)
+        return det ? launch_update_tw<true, false, true>(a, s) : launch_update_tw<false, false, true>(a, s);
+    return det ? launch_update_tw<true, false, false>(a, s) : launch_update_tw<false, false, false>(a, s);
+}
+
+hipError_t launch_bucketed(const BucketArgs& a, bool det, hipStream_t s)
+{
+    const size_t lds = (size_t)a.NBK * sizeof(uint32_t);
+    static size_t lds_set = 0;
+    if (lds > 32768 && lds > lds_set) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_bucket_hist, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)lds);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void*)k_bucket_starts, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void*)k_bucket_place<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void*)k_bucket_place<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds);
+        if (e != hipSuccess)
+            return e;
+        lds_set = lds;
+    }
+    hipLaunchKernelGGL(k_bucket_hist, dim3(a.NB), dim3(kBucketThreads), lds, s, a);
+    hipLaunchKernelGGL(k_bucket_colscan, dim3(cdiv(a.NBK >> kGroupShift, kScanCols) + cdiv(a.NBK, kScanCols)),
+                       dim3(kScanCols * kScanGroups), 0, s, a);
+    hipLaunchKernelGGL(k_bucket_starts, dim3(1), dim3(1024), lds, s, a);
+    const size_t lds_c = (size_t)(a.NBK >> kGroupShift) * sizeof(uint32_t);
+    if (det)
+        hipLaunchKernelGGL((k_bucket_place<true>), dim3(a.NB), dim3(kBucketThreads), lds_c, s, a);
+    else
+        hipLaunchKernelGGL((k_bucket_place<false>), dim3(a.NB), dim3(kBucketThreads), lds_c, s, a);
+    hipLaunchKernelGGL(k_bucket_refine, dim3(a.NBK >> kGroupShift), dim3(kRefineThreads), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_bucket_scrub(const UpdateArgs& a, uint32_t j_begin, uint32_t j_end, hipStream_t s)
+{
+    if (j_end <= j_begin)
+        return hipSuccess;
+    const uint32_t t0 = j_begin >> a.tw_shift, t1 = (j_end - 1) >> a.tw_shift;
+    hipLaunchKernelGGL(k_bucket_scrub, dim3(t1 - t0 + 1, a.R * a.M), dim3(kBlock), 0, s, a, j_begin, j_end);
     return hipGetLastError();
 }
 

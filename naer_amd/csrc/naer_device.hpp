@@ -111,6 +111,7 @@ struct PopParams {
 // graphs) need no per-step host arguments.
 struct DevState {
     unsigned long long clock;   // first step of the next batch
+    unsigned long long t0_cur;  // first step of the batch in flight (set by the update)
     unsigned long long rec_n;   // spikes offered to the record buffer since the last drain
     uint32_t batch;             // batches done; the update writes list[batch & 1]
     uint32_t cur;               // list of the batch in flight
@@ -153,7 +154,61 @@ struct UpdateArgs {
     uint32_t dt_us;
     uint32_t n_pops;
     float dec_p, dec_m;
+    uint32_t tw_shift;   // update block = one tile of 1 << tw_shift neurons
+    uint32_t Bfull;      // steps of an aligned batch
+    // bucketed delivery (see BucketArgs): events of earlier batches sorted by
+    // (arrival batch offset m, target tile)
+    const uint2* ev;          // [R][e_cap]
+    const uint32_t* bs;       // [R][NBK + 1] bucket starts
+    const uint32_t* flags;    // [R] 1: that batch's events are in the buckets
+    unsigned long long e_cap;
+    uint32_t R, M, NT, NBK;
 };
+
+// Bucketed delivery of one aligned batch E (steps [E*Bfull, (E+1)*Bfull)):
+// every synaptic event (target j, arrival step a = t + d) ends up in bucket
+// k = (m - 1) * NT + T, m = a/Bfull - E in [1, M], T = the target's tile, as
+// {j - T*TW | (a mod Bfull) << 16, weight bits}.  NT is the tile count
+// rounded up to a multiple of kGroupTiles, so the kGroupTiles buckets of a
+// tile group (m, g) are adjacent.  Five kernels:
+//   hist     per block: counts per bucket (LDS), written as a row of hist_f
+//            and, summed per group, as a row of hist_c
+//   colscan  hist_c columns: exclusive scan over blocks; hist_f columns: totals
+//   starts   exclusive scan of the bucket totals -> bs[E % R], flags[E % R]
+//   place    events to their group's range of the staging buffer (runs of a
+//            block's events per group are contiguous, so stores coalesce)
+//   refine   one block per group: staging -> ev[E % R] in bucket order
+// The update of batch E+m reads bucket (m, T) into LDS.
+constexpr uint32_t kGroupShift = 4;
+constexpr uint32_t kGroupTiles = 1u << kGroupShift;
+
+struct BucketArgs {
+    const naer_spike* list0;
+    const naer_spike* list1;
+    const uint32_t* seg_count;
+    uint32_t seg_cap, n_seg;
+    DevState* st;
+    const unsigned long long* row_ptr;
+    const uint32_t* col;
+    const float* w;
+    uint32_t* hist;    // hist_f [NB][NBK]
+    uint32_t* hist_c;  // [NB][NBK / kGroupTiles]
+    uint32_t* totals;  // [NBK]
+    uint32_t* bs;      // [R][NBK + 1]
+    uint32_t* flags;   // [R]
+    uint2* ev;         // [R][e_cap]
+    uint2* stage;      // [e_cap] events of the batch in group order
+    void* ring;        // fallback when the batch exceeds e_cap
+    unsigned long long e_cap;
+    uint32_t N, D, Bfull, B, R, M, NT, NBK, tw_shift;
+    uint32_t G;        // list segments per histogram/placement block (<= kMaxSegPerBlock)
+    uint32_t NB;       // histogram/placement blocks
+    uint32_t C;        // 256-synapse chunks of the longest row
+    uint32_t is_last;
+};
+
+constexpr uint32_t kMaxSegPerBlock = 64;  // G bound: NT <= 16384 tiles over <= 256 blocks
+constexpr uint32_t kMaxArrivalBatches = 64;  // M bound: (B - 1 + 127) / B for B >= 2
 
 struct ScatterArgs {
     const naer_spike* list0;
@@ -170,7 +225,8 @@ struct ScatterArgs {
     uint32_t N, D;
     uint32_t B;
     uint32_t is_last;
-    uint32_t _pad;
+    uint32_t* flags;   // bucketed delivery: the batch is marked as not bucketed
+    uint32_t R, Bfull;
 };
 
 struct PotentiateArgs {

@@ -5,7 +5,10 @@
 
 namespace naer {
 
-hipError_t launch_update(const UpdateArgs& a, bool det, bool plastic, hipStream_t s);
+hipError_t launch_update(const UpdateArgs& a, bool det, bool plastic, bool bucket, hipStream_t s);
+// histogram, column scan, bucket starts and placement of one aligned batch
+hipError_t launch_bucketed(const BucketArgs& a, bool det, hipStream_t s);
+hipError_t launch_bucket_scrub(const UpdateArgs& a, uint32_t j_begin, uint32_t j_end, hipStream_t s);
 hipError_t launch_scatter(const ScatterArgs& a, bool det, bool plastic, uint32_t grid_x,
                           uint32_t chunks, hipStream_t s);
 hipError_t launch_potentiate(const PotentiateArgs& a, uint32_t grid_x, hipStream_t s);

@@ -187,6 +187,15 @@ struct naer_net {
     uint32_t scatter_chunks = 1, scatter_grid = 1;
     uint64_t steps_enqueued = 0;
 
+    // update tiles and bucketed delivery
+    uint32_t tw_shift = 8;   // update block = tile of 1 << tw_shift neurons
+    bool bucketed = false;   // full aligned batches are delivered through buckets
+    uint32_t R = 0, M = 0, NT = 0, NBK = 0, NB = 0, G = 1;
+    uint64_t e_cap = 0;
+    uint2* ev = nullptr;
+    uint2* stage = nullptr;
+    uint32_t *bs = nullptr, *flags = nullptr, *hist = nullptr, *hist_c = nullptr, *totals = nullptr;
+
     bool profiling = false;
     uint32_t graph_chunk = 0;  // steps per captured graph (0 = plain launches)
     hipGraphExec_t graph_exec = nullptr;
@@ -451,7 +460,7 @@ int upload_table(naer_net* n)
     n->blk_pop_h.assign(n->n_seg, 0);
     uint32_t p = 0;
     for (uint32_t b = 0; b < n->n_seg; ++b) {
-        const uint32_t j = b * kBlock;
+        const uint32_t j = b << n->tw_shift;
         while (p + 1 < P && n->pp[p + 1].begin <= j)
             ++p;
         n->blk_pop_h[b] = p;
@@ -507,6 +516,16 @@ UpdateArgs make_update_args(naer_net* n)
         a.dec_p = (float)std::exp(-(double)n->dt_ms / (double)n->stdp.tau_plus_ms);
         a.dec_m = (float)std::exp(-(double)n->dt_ms / (double)n->stdp.tau_minus_ms);
     }
+    a.tw_shift = n->tw_shift;
+    a.Bfull = n->B;
+    a.ev = n->ev;
+    a.bs = n->bs;
+    a.flags = n->flags;
+    a.e_cap = n->e_cap;
+    a.R = n->R;
+    a.M = n->M;
+    a.NT = n->NT;
+    a.NBK = n->NBK;
     return a;
 }
 
@@ -626,6 +645,8 @@ int init_net(naer_net* n, const NetInfo& net)
 {
     UpdateArgs ua = make_update_args(n);
     HIP_TRY(launch_init_state(ua, n->det, net.neuron_begin, net.neuron_begin + net.n_neurons, n->stream));
+    if (n->bucketed)
+        HIP_TRY(launch_bucket_scrub(ua, net.neuron_begin, net.neuron_begin + net.n_neurons, n->stream));
     return NAER_OK;
 }
 
@@ -654,8 +675,11 @@ int alloc_neurons(naer_net* n, bool any_poisson)
     // per step, a POISSON source at most 10 times per 1500 us epoch
     const uint64_t epochs = ((uint64_t)n->B * n->dt_us + kEpochUs - 1) / kEpochUs + 1;
     const uint64_t per_neuron = any_poisson ? std::max<uint64_t>(n->B, kMaxPoissonPerEpoch * epochs) : n->B;
-    n->seg_cap = (uint32_t)(kBlock * per_neuron);
-    n->n_seg = (N + kBlock - 1) / kBlock;
+    // update tiles: 1024 neurons per block for large nets, 256 otherwise
+    n->tw_shift = N >= (1u << 18) ? 10u : 8u;
+    const uint32_t TW = 1u << n->tw_shift;
+    n->seg_cap = (uint32_t)(TW * per_neuron);
+    n->n_seg = (N + TW - 1) / TW;
     const size_t list_elems = (size_t)n->seg_cap * n->n_seg;
     if ((rc = n->alloc(&n->list0, list_elems)) || (rc = n->alloc(&n->list1, list_elems)) ||
         (rc = n->alloc(&n->seg_count, 2 * (size_t)n->n_seg)) || (rc = n->alloc(&n->spk_part, n->n_seg)) ||
@@ -667,6 +691,40 @@ int alloc_neurons(naer_net* n, bool any_poisson)
     if ((rc = n->alloc(&n->rec, n->rec_cap ? n->rec_cap : 1)) || (rc = n->alloc(&n->st, 1)))
         return rc;
     HIP_TRY(hipMemsetAsync(n->st, 0, sizeof(DevState), n->stream));
+    return NAER_OK;
+}
+
+// Buffers of the bucketed delivery: R = M + 1 batches of events in flight
+// (a batch's events are read during the M batches that follow it), e_cap
+// events each, sized for a mean rate of kBucketRateHz over all synapses.
+constexpr double kBucketRateHz = 80.0;
+constexpr uint32_t kMaxBuckets = 16384;
+
+int alloc_buckets(naer_net* n)
+{
+    n->bucketed = false;
+    if (n->plastic || n->B < 2 || n->S_cap == 0)
+        return NAER_OK;
+    n->M = (n->B - 1 + n->dmax) / n->B;
+    n->NT = (n->n_seg + kGroupTiles - 1) / kGroupTiles * kGroupTiles;
+    n->NBK = n->M * n->NT;
+    if (n->NBK > kMaxBuckets)
+        return NAER_OK;
+    n->R = n->M + 1;
+    const double expect = (double)n->S_cap * n->B * n->dt_ms * 1e-3 * kBucketRateHz;
+    n->e_cap = (uint64_t)std::min(std::max(expect, (double)(1u << 20)), (double)0x7FFFFFFFu);
+    n->G = std::max<uint32_t>(1, (n->n_seg + 255) / 256);
+    n->NB = (n->n_seg + n->G - 1) / n->G;
+    if (n->M > kMaxArrivalBatches || n->G > kMaxSegPerBlock)
+        return NAER_OK;
+    int rc;
+    if ((rc = n->alloc(&n->ev, (size_t)n->R * n->e_cap)) || (rc = n->alloc(&n->stage, n->e_cap)) ||
+        (rc = n->alloc(&n->bs, (size_t)n->R * (n->NBK + 1))) || (rc = n->alloc(&n->flags, n->R)) ||
+        (rc = n->alloc(&n->hist, (size_t)n->NB * n->NBK)) ||
+        (rc = n->alloc(&n->hist_c, (size_t)n->NB * (n->NBK / kGroupTiles))) || (rc = n->alloc(&n->totals, n->NBK)))
+        return rc;
+    HIP_TRY(hipMemsetAsync(n->flags, 0, n->R * sizeof(uint32_t), n->stream));
+    n->bucketed = true;
     return NAER_OK;
 }
 
@@ -795,11 +853,14 @@ int naer_net_create(naer_ctx* ctx, const naer_net_desc* desc, naer_net** out)
     return naer_pack_create(ctx, desc, 1, out);
 }
 
-int naer_pack_layout(naer_net* n, uint32_t* neuron_begin, uint32_t* n_neurons, uint64_t* t_start)
+int naer_pack_layout(naer_net* n, uint32_t cap, uint32_t* neuron_begin, uint32_t* n_neurons, uint64_t* t_start,
+                     uint32_t* n_slots)
 {
     if (!n)
         return fail(NAER_ERR_INVALID, "net is NULL");
-    for (size_t i = 0; i < n->nets.size(); ++i) {
+    if (n_slots)
+        *n_slots = (uint32_t)n->nets.size();
+    for (size_t i = 0; i < std::min<size_t>(cap, n->nets.size()); ++i) {
         if (neuron_begin)
             neuron_begin[i] = n->nets[i].neuron_begin;
         if (n_neurons)
@@ -871,8 +932,8 @@ int naer_net_build(naer_net* n)
             if ((rc = reverse_place(n, n->nets[i], rp[i])))
                 return rc;
     }
-    // 4. neuron state
-    if ((rc = upload_table(n)))
+    // 4. delivery buffers, neuron state
+    if ((rc = alloc_buckets(n)) || (rc = upload_table(n)))
         return rc;
     for (auto& net : n->nets)
         if ((rc = init_net(n, net)))
@@ -931,6 +992,8 @@ int naer_arena_create(naer_ctx* ctx, const naer_arena_desc* a, naer_net** out)
         rc = n->alloc(&n->rev_pos, n->R_cap);
     if (rc == NAER_OK && n->plastic)
         rc = n->alloc(&n->rev_src, n->R_cap);
+    if (rc == NAER_OK)
+        rc = alloc_buckets(n);
     if (rc == NAER_OK)
         rc = upload_table(n);
     if (rc == NAER_OK) {
@@ -1059,7 +1122,7 @@ int naer_arena_retire(naer_net* n, uint32_t slot)
 static hipError_t do_update(naer_net* n, void* arg)
 {
     UpdateArgs* a = (UpdateArgs*)arg;
-    return launch_update(*a, n->det, n->plastic, n->stream);
+    return launch_update(*a, n->det, n->plastic, n->bucketed, n->stream);
 }
 
 static hipError_t do_scatter(naer_net* n, void* arg)
@@ -1074,6 +1137,15 @@ static hipError_t do_potentiate(naer_net* n, void* arg)
     return launch_potentiate(*a, 256, n->stream);
 }
 
+static hipError_t do_bucketed(naer_net* n, void* arg)
+{
+    BucketArgs* a = (BucketArgs*)arg;
+    return launch_bucketed(*a, n->det, n->stream);
+}
+
+// Enqueues steps batch by batch.  Batches are aligned to multiples of B
+// steps; a full aligned batch of a bucketed net delivers its spikes through
+// the buckets, any other batch (partial, plastic) through the ring.
 static int enqueue_steps(naer_net* n, uint32_t n_steps, bool timed)
 {
     UpdateArgs ua = make_update_args(n);
@@ -1096,6 +1168,41 @@ static int enqueue_steps(naer_net* n, uint32_t n_steps, bool timed)
     sa.N = n->N;
     sa.D = n->D;
     sa.is_last = n->plastic ? 0u : 1u;
+    sa.flags = n->bucketed ? n->flags : nullptr;
+    sa.R = n->R;
+    sa.Bfull = n->B;
+    BucketArgs ba;
+    std::memset(&ba, 0, sizeof ba);
+    ba.list0 = n->list0;
+    ba.list1 = n->list1;
+    ba.seg_count = n->seg_count;
+    ba.seg_cap = n->seg_cap;
+    ba.n_seg = n->n_seg;
+    ba.st = n->st;
+    ba.row_ptr = n->row_ptr;
+    ba.col = n->col;
+    ba.w = n->w;
+    ba.hist = n->hist;
+    ba.hist_c = n->hist_c;
+    ba.stage = n->stage;
+    ba.totals = n->totals;
+    ba.bs = n->bs;
+    ba.flags = n->flags;
+    ba.ev = n->ev;
+    ba.ring = n->ring;
+    ba.e_cap = n->e_cap;
+    ba.N = n->N;
+    ba.D = n->D;
+    ba.Bfull = n->B;
+    ba.R = n->R;
+    ba.M = n->M;
+    ba.NT = n->NT;
+    ba.NBK = n->NBK;
+    ba.tw_shift = n->tw_shift;
+    ba.G = n->G;
+    ba.NB = n->NB;
+    ba.C = n->scatter_chunks;
+    ba.is_last = 1;
     PotentiateArgs pa;
     std::memset(&pa, 0, sizeof pa);
     pa.list0 = n->list0;
@@ -1114,14 +1221,20 @@ static int enqueue_steps(naer_net* n, uint32_t n_steps, bool timed)
     pa.w_max = n->stdp.w_max;
     int rc;
     while (n_steps > 0) {
-        const uint32_t b = std::min(n_steps, n->B);
+        const uint32_t off = (uint32_t)(n->steps_enqueued % n->B);
+        const uint32_t b = std::min(n_steps, n->B - off);
         ua.B = b;
         sa.B = b;
         pa.B = b;
+        ba.B = b;
         if ((rc = launch_timed(n, NAER_K_UPDATE, do_update, &ua, timed)))
             return rc;
-        if ((rc = launch_timed(n, NAER_K_SCATTER, do_scatter, &sa, timed)))
+        if (n->bucketed && off == 0 && b == n->B) {
+            if ((rc = launch_timed(n, NAER_K_SCATTER, do_bucketed, &ba, timed)))
+                return rc;
+        } else if ((rc = launch_timed(n, NAER_K_SCATTER, do_scatter, &sa, timed))) {
             return rc;
+        }
         if (n->plastic && (rc = launch_timed(n, NAER_K_POTENTIATE, do_potentiate, &pa, timed)))
             return rc;
         n_steps -= b;
@@ -1137,6 +1250,17 @@ int naer_net_step(naer_net* n, uint32_t n_steps)
     if (!n->built)
         return fail(NAER_ERR_STATE, "net not built");
     if (n->graph_chunk) {
+        // a captured chunk is a whole number of aligned batches, so every
+        // replay sees the same batch pattern: align plainly first
+        const uint32_t misalign = (uint32_t)((n->B - n->steps_enqueued % n->B) % n->B);
+        if (misalign && n_steps > misalign) {
+            int rc = enqueue_steps(n, misalign, true);
+            if (rc)
+                return rc;
+            n_steps -= misalign;
+        }
+    }
+    if (n->graph_chunk && n->steps_enqueued % n->B == 0) {
         if (!n->graph_exec && n_steps >= n->graph_chunk) {
             // capture one chunk; its kernels read the clock from the device,
             // so the same graph replays for every chunk
@@ -1173,7 +1297,8 @@ int naer_net_set_graph(naer_net* n, uint32_t chunk_steps)
         return rc;
     if ((rc = n->drop_graph()))
         return rc;
-    n->graph_chunk = chunk_steps;
+    // whole aligned batches per chunk
+    n->graph_chunk = chunk_steps ? (chunk_steps + n->B - 1) / n->B * n->B : 0;
     return NAER_OK;
 }
 

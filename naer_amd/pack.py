@@ -34,8 +34,8 @@ class Pack:
         self.begin = np.zeros(n, np.uint32)
         self.size = np.zeros(n, np.uint32)
         self.t_start = np.zeros(n, np.uint64)
-        check(_abi.lib().naer_pack_layout(h, self.begin.ctypes.data, self.size.ctypes.data,
-                                          self.t_start.ctypes.data))
+        check(_abi.lib().naer_pack_layout(h, n, self.begin.ctypes.data, self.size.ctypes.data,
+                                          self.t_start.ctypes.data, None))
         if profiling:
             check(_abi.lib().naer_net_set_profiling(h, 1))
 
