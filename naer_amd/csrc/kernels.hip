@@ -97,8 +97,8 @@ __device__ __forceinline__ void emit(Stage& sg, bool fire, uint32_t neuron, uint
 // slots and buckets of all B steps are read up front: nothing can arrive
 // during the batch because B <= the minimum delay.
 // ---------------------------------------------------------------------------
-template <bool DET, bool PLASTIC, bool BUCKET, uint32_t TW>
-__global__ __launch_bounds__(TW, (BUCKET && !DET && TW == 1024) ? 8 : 1) void k_update(const UpdateArgs a)
+template <bool DET, bool PLASTIC, bool BUCKET, bool POIS, uint32_t TW>
+__global__ __launch_bounds__(TW, (BUCKET && !DET && !POIS && TW == 1024) ? 8 : 1) void k_update(const UpdateArgs a)
 {
     using RingT = typename std::conditional<DET, long long, float>::type;
     using AccT = typename std::conditional<DET, unsigned long long, float>::type;
@@ -165,6 +165,7 @@ __global__ __launch_bounds__(TW, (BUCKET && !DET && TW == 1024) ? 8 : 1) void k_
         __syncthreads();
         const uint32_t total = s_pre[a.M];
         constexpr uint32_t U = 4;
+        uint32_t m = 0;  // bucket of the thread's current event (its indexes only grow)
         for (uint32_t i0 = threadIdx.x; i0 < total; i0 += TW * U) {
             uint2 e[U];
 #pragma unroll
@@ -172,7 +173,6 @@ __global__ __launch_bounds__(TW, (BUCKET && !DET && TW == 1024) ? 8 : 1) void k_
                 const uint32_t i = i0 + u * TW;
                 e[u] = make_uint2(0xFFFFFFFFu, 0u);
                 if (i < total) {
-                    uint32_t m = 0;
                     while (i >= s_pre[m + 1])
                         ++m;
                     e[u] = a.ev[s_evbase[m] + i];
@@ -208,14 +208,14 @@ __global__ __launch_bounds__(TW, (BUCKET && !DET && TW == 1024) ? 8 : 1) void k_
     Stage sg{ s_list, &s_n, seg, a.seg_cap };
 
     const uint32_t j = blockIdx.x * TW + threadIdx.x;
-    unsigned long long my_events = 0;
+    uint32_t my_events = 0;  // <= B * longest row
     // neurons outside every population (free ranges of an arena) stay idle
     const uint32_t p = (j < a.N && a.n_pops) ? find_pop(a.pops, a.n_pops, p_first, j) : 0u;
     const bool active = j < a.N && a.n_pops && j >= a.pops[p].begin && j < a.pops[p].end;
     if (active) {
         const PopParams pp = a.pops[p];
         const uint32_t jl = j - pp.net_base;                 // neuron id within its net
-        const unsigned long long t0l = t0 - pp.t_start;      // net-local step of t0
+        const uint32_t t0l = (uint32_t)(t0 - pp.t_start);    // net-local step of t0
         if (pp.kind == NAER_POP_LIF) {
             RingT* ring = (RingT*)a.ring;
             uint32_t slot0 = (uint32_t)(t0 % a.D);
@@ -298,7 +298,7 @@ __global__ __launch_bounds__(TW, (BUCKET && !DET && TW == 1024) ? 8 : 1) void k_
                     float v = xp + v_rest;
                     v = v + I;
                     if (has_drive) {
-                        const unsigned long long tl = t0l + s;
+                        const uint32_t tl = t0l + s;
                         const uint32_t blk = (uint32_t)(tl >> 2);
                         if (blk != rnd_blk) {
                             rnd = philox(U4{ jl, blk, NAER_STREAM_DRIVE, 0u }, s_lo, s_hi);
@@ -316,7 +316,7 @@ __global__ __launch_bounds__(TW, (BUCKET && !DET && TW == 1024) ? 8 : 1) void k_
                     if (fired) {
                         ref = n_ref;
                         last = (int32_t)t;
-                        my_events += a.row_ptr[j + 1] - a.row_ptr[j];
+                        my_events += (uint32_t)(a.row_ptr[j + 1] - a.row_ptr[j]);
                     }
                 }
                 emit(sg, fired, j, (uint32_t)t, (uint32_t)(t0l + s) * a.dt_us);
@@ -328,7 +328,7 @@ __global__ __launch_bounds__(TW, (BUCKET && !DET && TW == 1024) ? 8 : 1) void k_
                 a.xh[j] = xh;
                 a.yh[j] = yh;
             }
-        } else {
+        } else if (POIS) {
             // POISSON source: the reference train (input_neuron_group.cl:26-39)
             // evaluated window by window; state = next spike time and the
             // number already emitted in the current 1500 us epoch.
@@ -348,7 +348,7 @@ __global__ __launch_bounds__(TW, (BUCKET && !DET && TW == 1024) ? 8 : 1) void k_
                 while (__any(pend)) {
                     emit(sg, pend, j, (uint32_t)t, nxt);
                     if (pend) {
-                        my_events += a.row_ptr[j + 1] - a.row_ptr[j];
+                        my_events += (uint32_t)(a.row_ptr[j + 1] - a.row_ptr[j]);
                         ++cnt;
                         nxt = nxt + ref_isi_us(pp.neg_rate_per_us, gid, pp.pois_seed, nxt * pp.n_pois + idx);
                         pend = cnt < kMaxPoissonPerEpoch && nxt < end;
@@ -360,7 +360,7 @@ __global__ __launch_bounds__(TW, (BUCKET && !DET && TW == 1024) ? 8 : 1) void k_
         }
     }
     if (my_events)
-        atomicAdd(&s_events, my_events);
+        atomicAdd(&s_events, (unsigned long long)my_events);
     __syncthreads();
     const uint32_t total = s_n;
     const uint32_t n = total < a.seg_cap ? total : a.seg_cap;
@@ -523,29 +523,53 @@ __device__ __forceinline__ void bucket_setup(const BucketArgs& a, uint32_t cur, 
     }
 }
 
-struct Task {
-    naer_spike sp;
-    unsigned long long beg, end;  // synapse range of the chunk
+// Rows of the block's spikes, staged in LDS kRowRound at a time: one round
+// of parallel loads (spike, row bounds), then the chunk tasks read LDS only.
+constexpr uint32_t kRowRound = 1024;
+
+struct RowStage {
+    unsigned long long beg[kRowRound];
+    uint32_t len[kRowRound];
+    uint32_t r[kRowRound];  // spike step - first step of the batch
 };
 
-// Task t of this block (wave-uniform); false when its chunk is empty.
-__device__ __forceinline__ bool block_task(const BucketArgs& a, const naer_spike* list, const uint32_t* s_off,
-                                           uint32_t G, uint32_t t, Task& k)
+// Calls f(r, q_begin, q_end) for every non-empty 256-synapse chunk of the
+// rows of the block's spikes; f runs on a whole wave (q range wave-uniform).
+template <typename F>
+__device__ __forceinline__ void for_block_chunks(const BucketArgs& a, const naer_spike* list, const uint32_t* s_off,
+                                                 unsigned long long t0, RowStage& rs, F f)
 {
-    const uint32_t i = t / a.C, c = t - i * a.C;
-    uint32_t lo = 0, hi = G;  // segment g with s_off[g] <= i < s_off[g + 1]
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (s_off[mid] <= i)
-            lo = mid;
-        else
-            hi = mid;
+    const uint32_t G = min(a.G, a.n_seg - blockIdx.x * a.G);
+    const uint32_t n_spk = s_off[G];
+    const uint32_t nwv = blockDim.x / kWave;
+    for (uint32_t base = 0; base < n_spk; base += kRowRound) {
+        const uint32_t cnt = min(kRowRound, n_spk - base);
+        for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
+            const uint32_t x = base + i;
+            uint32_t lo = 0, hi = G;  // segment g with s_off[g] <= x < s_off[g + 1]
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (s_off[mid] <= x)
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            const naer_spike sp = list[(size_t)(blockIdx.x * a.G + lo) * a.seg_cap + (x - s_off[lo])];
+            const unsigned long long rb = a.row_ptr[sp.neuron];
+            rs.beg[i] = rb;
+            rs.len[i] = (uint32_t)(a.row_ptr[sp.neuron + 1] - rb);
+            rs.r[i] = (uint32_t)(sp.step - t0);
+        }
+        __syncthreads();
+        const uint32_t n_tasks = cnt * a.C;
+        for (uint32_t t = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave); t < n_tasks; t += nwv) {
+            const uint32_t i = t / a.C, c = t - i * a.C;
+            const uint32_t off = c * kChunk, len = rs.len[i];
+            if (off < len)
+                f(rs.r[i], rs.beg[i] + off, rs.beg[i] + min(len, off + kChunk));
+        }
+        __syncthreads();
     }
-    k.sp = list[(size_t)(blockIdx.x * a.G + lo) * a.seg_cap + (i - s_off[lo])];
-    const unsigned long long rb = a.row_ptr[k.sp.neuron], re = a.row_ptr[k.sp.neuron + 1];
-    k.beg = rb + (unsigned long long)c * kChunk;
-    k.end = min(re, k.beg + kChunk);
-    return k.beg < k.end;
 }
 
 __global__ __launch_bounds__(kBucketThreads) void k_bucket_hist(const BucketArgs a)
@@ -553,6 +577,7 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_hist(const BucketArgs
     extern __shared__ uint32_t s_hist[];
     __shared__ uint8_t lut_m[kArrivalLut], lut_o[kArrivalLut];
     __shared__ uint32_t s_off[kMaxSegPerBlock + 1];
+    __shared__ RowStage rs;
     const DevState* st = a.st;
     const uint32_t cur = st->cur;
     const unsigned long long t0 = st->t0_cur;
@@ -561,47 +586,38 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_hist(const BucketArgs
     bucket_setup(a, cur, lut_m, lut_o, s_off);
     __syncthreads();
     const naer_spike* list = cur ? a.list1 : a.list0;
-    const uint32_t G = min(a.G, a.n_seg - blockIdx.x * a.G);
-    const uint32_t n_tasks = s_off[G] * a.C;
     const uint32_t lane = lane_id();
-    const uint32_t nwv = blockDim.x / kWave;
-    for (uint32_t t = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave); t < n_tasks; t += nwv) {
-        Task k;
-        if (!block_task(a, list, s_off, G, t, k))
-            continue;
-        const uint32_t r = (uint32_t)(k.sp.step - t0);
+    for_block_chunks(a, list, s_off, t0, rs, [&](uint32_t r, unsigned long long beg, unsigned long long end) {
         uint32_t c[kChunkUnroll];
 #pragma unroll
         for (uint32_t u = 0; u < kChunkUnroll; ++u) {
-            const unsigned long long q = k.beg + u * kWave + lane;
-            c[u] = q < k.end ? a.col[q] : 0u;
+            const unsigned long long q = beg + u * kWave + lane;
+            c[u] = q < end ? a.col[q] : 0u;
         }
 #pragma unroll
         for (uint32_t u = 0; u < kChunkUnroll; ++u) {
-            const unsigned long long q = k.beg + u * kWave + lane;
-            if (q < k.end) {
+            const unsigned long long q = beg + u * kWave + lane;
+            if (q < end) {
                 const uint32_t v = r + ((c[u] >> 24) & 0x7Fu);
                 const uint32_t T = (c[u] & kTargetMask) >> a.tw_shift;
                 atomicAdd(&s_hist[(lut_m[v] - 1u) * a.NT + T], 1u);
             }
         }
-    }
-    __syncthreads();
+    });
     uint32_t* h = a.hist + (size_t)blockIdx.x * a.NBK;
     for (uint32_t i = threadIdx.x; i < a.NBK; i += blockDim.x)
         h[i] = s_hist[i];
-    const uint32_t nbc = a.NBK >> kGroupShift;
+    const uint32_t nbc = a.NBK >> a.gt_shift;
     uint32_t* hc = a.hist_c + (size_t)blockIdx.x * nbc;
     for (uint32_t i = threadIdx.x; i < nbc; i += blockDim.x) {
         uint32_t sum = 0;
-#pragma unroll
-        for (uint32_t t = 0; t < kGroupTiles; ++t)
-            sum += s_hist[(i << kGroupShift) + t];
+        for (uint32_t t = 0; t < (1u << a.gt_shift); ++t)
+            sum += s_hist[(i << a.gt_shift) + t];
         hc[i] = sum;
     }
 }
 
-// Blocks [0, ceil(NBK/kGroupTiles / 64)): hist_c[b][kc] <- sum of
+// Blocks [0, ceil((NBK >> gt_shift) / 64)): hist_c[b][kc] <- sum of
 // hist_c[b'][kc] for b' < b.  The remaining blocks: totals[k] <- sum over b
 // of hist_f[b][k].  A block covers 64 columns with 16 row groups.
 constexpr uint32_t kScanCols = 64, kScanGroups = 16;
@@ -609,7 +625,7 @@ constexpr uint32_t kScanCols = 64, kScanGroups = 16;
 __global__ __launch_bounds__(kScanCols * kScanGroups) void k_bucket_colscan(const BucketArgs a)
 {
     __shared__ uint32_t s_part[kScanGroups][kScanCols];
-    const uint32_t nbc = a.NBK >> kGroupShift;
+    const uint32_t nbc = a.NBK >> a.gt_shift;
     const uint32_t coarse_blocks = (nbc + kScanCols - 1) / kScanCols;
     const bool coarse = blockIdx.x < coarse_blocks;
     uint32_t* hist = coarse ? a.hist_c : a.hist;
@@ -711,19 +727,20 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_place(const BucketArg
     extern __shared__ uint32_t s_cur[];
     __shared__ uint8_t lut_m[kArrivalLut], lut_o[kArrivalLut];
     __shared__ uint32_t s_off[kMaxSegPerBlock + 1];
+    __shared__ RowStage rs;
     DevState* st = a.st;
     const uint32_t cur = st->cur;
     const unsigned long long t0 = st->t0_cur;
     const uint32_t buf = (uint32_t)((t0 / a.Bfull) % a.R);
     const bool ok = a.flags[buf] != 0;
-    const uint32_t nbc = a.NBK >> kGroupShift;
-    const uint32_t ng = a.NT >> kGroupShift;
-    const uint32_t g_shift = a.tw_shift + kGroupShift;
+    const uint32_t nbc = a.NBK >> a.gt_shift;
+    const uint32_t ng = a.NT >> a.gt_shift;
+    const uint32_t g_shift = a.tw_shift + a.gt_shift;
     if (ok) {
         const uint32_t* bs = a.bs + (size_t)buf * (a.NBK + 1);
         const uint32_t* h = a.hist_c + (size_t)blockIdx.x * nbc;
         for (uint32_t i = threadIdx.x; i < nbc; i += blockDim.x)
-            s_cur[i] = bs[i << kGroupShift] + h[i];
+            s_cur[i] = bs[i << a.gt_shift] + h[i];
     }
     bucket_setup(a, cur, lut_m, lut_o, s_off);
     __syncthreads();
@@ -732,27 +749,20 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_place(const BucketArg
     uint2* stage = a.stage;
     RingT* ring = (RingT*)a.ring;
     const naer_spike* list = cur ? a.list1 : a.list0;
-    const uint32_t G = min(a.G, a.n_seg - blockIdx.x * a.G);
-    const uint32_t n_tasks = s_off[G] * a.C;
     const uint32_t lane = lane_id();
-    const uint32_t nwv = blockDim.x / kWave;
-    for (uint32_t t = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave); t < n_tasks; t += nwv) {
-        Task k;
-        if (!block_task(a, list, s_off, G, t, k))
-            continue;
-        const uint32_t r = (uint32_t)(k.sp.step - t0);
+    for_block_chunks(a, list, s_off, t0, rs, [&](uint32_t r, unsigned long long beg, unsigned long long end) {
         uint32_t c[kChunkUnroll];
         float w[kChunkUnroll];
 #pragma unroll
         for (uint32_t u = 0; u < kChunkUnroll; ++u) {
-            const unsigned long long q = k.beg + u * kWave + lane;
-            c[u] = q < k.end ? a.col[q] : 0u;
-            w[u] = q < k.end ? a.w[q] : 0.0f;
+            const unsigned long long q = beg + u * kWave + lane;
+            c[u] = q < end ? a.col[q] : 0u;
+            w[u] = q < end ? a.w[q] : 0.0f;
         }
 #pragma unroll
         for (uint32_t u = 0; u < kChunkUnroll; ++u) {
-            const unsigned long long q = k.beg + u * kWave + lane;
-            if (q < k.end) {
+            const unsigned long long q = beg + u * kWave + lane;
+            if (q < end) {
                 const uint32_t d = (c[u] >> 24) & 0x7Fu;
                 const uint32_t tgt = c[u] & kTargetMask;
                 if (ok) {
@@ -763,7 +773,7 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_place(const BucketArg
                                             __float_as_uint(w[u]));
                 } else {
                     // the batch does not fit the event buffer: deliver through the ring
-                    uint32_t sl = k.sp.step % a.D + d;
+                    uint32_t sl = (uint32_t)((t0 + r) % a.D) + d;
                     sl = sl >= a.D ? sl - a.D : sl;
                     RingT* dst = ring + (size_t)sl * a.N + tgt;
                     if (DET)
@@ -773,26 +783,26 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_place(const BucketArg
                 }
             }
         }
-    }
+    });
 }
 
 // One block per group bucket (m, g): its events, in the staging buffer at the
-// range of buckets [kc * kGroupTiles, (kc + 1) * kGroupTiles), go to their
+// range of buckets [kc << gt_shift, (kc + 1) << gt_shift), go to their
 // tile's bucket of ev[E % R] (same positions overall, tile order within).
 constexpr uint32_t kRefineThreads = 512;
 
 __global__ __launch_bounds__(kRefineThreads) void k_bucket_refine(const BucketArgs a)
 {
-    __shared__ uint32_t s_cur[kGroupTiles];
+    __shared__ uint32_t s_cur[kMaxGroupTiles];
     const unsigned long long t0 = a.st->t0_cur;
     const uint32_t buf = (uint32_t)((t0 / a.Bfull) % a.R);
     if (!a.flags[buf])
         return;
     const uint32_t* bs = a.bs + (size_t)buf * (a.NBK + 1);
-    const uint32_t k0 = blockIdx.x << kGroupShift;
-    if (threadIdx.x < kGroupTiles)
+    const uint32_t k0 = blockIdx.x << a.gt_shift;
+    if (threadIdx.x < (1u << a.gt_shift))
         s_cur[threadIdx.x] = bs[k0 + threadIdx.x];
-    const uint32_t lo = bs[k0], hi = bs[k0 + kGroupTiles];
+    const uint32_t lo = bs[k0], hi = bs[k0 + (1u << a.gt_shift)];
     __syncthreads();
     uint2* ev = a.ev + (size_t)buf * a.e_cap;
     const uint32_t tmask = (1u << a.tw_shift) - 1u;
@@ -1043,38 +1053,41 @@ __global__ __launch_bounds__(kBlock) void k_ref_poisson_epoch(uint32_t n, uint32
 // ---------------------------------------------------------------------------
 static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
-template <bool DET, bool PLASTIC, bool BUCKET, uint32_t TW>
+template <bool DET, bool PLASTIC, bool BUCKET, bool POIS, uint32_t TW>
 static hipError_t launch_update_t(const UpdateArgs& a, hipStream_t s)
 {
     const size_t lds = BUCKET ? (size_t)a.B * TW * (DET ? 8 : 4) : 0;
     static size_t lds_set = 0;  // per instantiation: largest size allowed so far
     if (lds > 32768 && lds > lds_set) {
-        hipError_t e = hipFuncSetAttribute((const void*)k_update<DET, PLASTIC, BUCKET, TW>,
+        hipError_t e = hipFuncSetAttribute((const void*)k_update<DET, PLASTIC, BUCKET, POIS, TW>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess)
             return e;
         lds_set = lds;
     }
-    hipLaunchKernelGGL((k_update<DET, PLASTIC, BUCKET, TW>), dim3(a.n_seg), dim3(TW), lds, s, a);
+    hipLaunchKernelGGL((k_update<DET, PLASTIC, BUCKET, POIS, TW>), dim3(a.n_seg), dim3(TW), lds, s, a);
     return hipGetLastError();
 }
 
 template <bool DET, bool PLASTIC, bool BUCKET>
-static hipError_t launch_update_tw(const UpdateArgs& a, hipStream_t s)
+static hipError_t launch_update_tw(const UpdateArgs& a, bool pois, hipStream_t s)
 {
-    return (1u << a.tw_shift) == 1024u ? launch_update_t<DET, PLASTIC, BUCKET, 1024>(a, s)
-                                        : launch_update_t<DET, PLASTIC, BUCKET, 256>(a, s);
+    if (a.tw_shift == 10)
+        return pois ? launch_update_t<DET, PLASTIC, BUCKET, true, 1024>(a, s)
+                    : launch_update_t<DET, PLASTIC, BUCKET, false, 1024>(a, s);
+    return pois ? launch_update_t<DET, PLASTIC, BUCKET, true, 256>(a, s)
+                : launch_update_t<DET, PLASTIC, BUCKET, false, 256>(a, s);
 }
 
-hipError_t launch_update(const UpdateArgs& a, bool det, bool plastic, bool bucket, hipStream_t s)
+hipError_t launch_update(const UpdateArgs& a, bool det, bool plastic, bool bucket, bool pois, hipStream_t s)
 {
-    if ((1u << a.tw_shift) != 1024u && (1u << a.tw_shift) != 256u)
+    if (a.tw_shift != 8 && a.tw_shift != 10)
         return hipErrorInvalidValue;
     if (plastic)  // plastic nets deliver through the ring only
-        return det ? launch_update_tw<true, true, false>(a, s) : launch_update_tw<false, true, false>(a, s);
+        return det ? launch_update_tw<true, true, false>(a, pois, s) : launch_update_tw<false, true, false>(a, pois, s);
     if (bucket)
-        return det ? launch_update_tw<true, false, true>(a, s) : launch_update_tw<false, false, true>(a, s);
-    return det ? launch_update_tw<true, false, false>(a, s) : launch_update_tw<false, false, false>(a, s);
+        return det ? launch_update_tw<true, false, true>(a, pois, s) : launch_update_tw<false, false, true>(a, pois, s);
+    return det ? launch_update_tw<true, false, false>(a, pois, s) : launch_update_tw<false, false, false>(a, pois, s);
 }
 
 hipError_t launch_bucketed(const BucketArgs& a, bool det, hipStream_t s)
@@ -1098,15 +1111,15 @@ hipError_t launch_bucketed(const BucketArgs& a, bool det, hipStream_t s)
         lds_set = lds;
     }
     hipLaunchKernelGGL(k_bucket_hist, dim3(a.NB), dim3(kBucketThreads), lds, s, a);
-    hipLaunchKernelGGL(k_bucket_colscan, dim3(cdiv(a.NBK >> kGroupShift, kScanCols) + cdiv(a.NBK, kScanCols)),
+    hipLaunchKernelGGL(k_bucket_colscan, dim3(cdiv(a.NBK >> a.gt_shift, kScanCols) + cdiv(a.NBK, kScanCols)),
                        dim3(kScanCols * kScanGroups), 0, s, a);
     hipLaunchKernelGGL(k_bucket_starts, dim3(1), dim3(1024), lds, s, a);
-    const size_t lds_c = (size_t)(a.NBK >> kGroupShift) * sizeof(uint32_t);
+    const size_t lds_c = (size_t)(a.NBK >> a.gt_shift) * sizeof(uint32_t);
     if (det)
         hipLaunchKernelGGL((k_bucket_place<true>), dim3(a.NB), dim3(kBucketThreads), lds_c, s, a);
     else
         hipLaunchKernelGGL((k_bucket_place<false>), dim3(a.NB), dim3(kBucketThreads), lds_c, s, a);
-    hipLaunchKernelGGL(k_bucket_refine, dim3(a.NBK >> kGroupShift), dim3(kRefineThreads), 0, s, a);
+    hipLaunchKernelGGL(k_bucket_refine, dim3(a.NBK >> a.gt_shift), dim3(kRefineThreads), 0, s, a);
     return hipGetLastError();
 }
 

@@ -44,8 +44,11 @@ __host__ __device__ __forceinline__ U4 philox(U4 c, uint32_t k0, uint32_t k1)
 {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
-        const uint32_t h0 = mulhi32(0xD2511F53u, c.x), l0 = 0xD2511F53u * c.x;
-        const uint32_t h1 = mulhi32(0xCD9E8D57u, c.z), l1 = 0xCD9E8D57u * c.z;
+        // one 32x32->64 product per multiplier (v_mad_u64_u32 on the device)
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
+        const uint32_t h0 = (uint32_t)(p0 >> 32), l0 = (uint32_t)p0;
+        const uint32_t h1 = (uint32_t)(p1 >> 32), l1 = (uint32_t)p1;
         c = U4{ h1 ^ c.y ^ k0, l1, h0 ^ c.w ^ k1, l0 };
         k0 += 0x9E3779B9u;
         k1 += 0xBB67AE85u;
@@ -168,9 +171,9 @@ struct UpdateArgs {
 // Bucketed delivery of one aligned batch E (steps [E*Bfull, (E+1)*Bfull)):
 // every synaptic event (target j, arrival step a = t + d) ends up in bucket
 // k = (m - 1) * NT + T, m = a/Bfull - E in [1, M], T = the target's tile, as
-// {j - T*TW | (a mod Bfull) << 16, weight bits}.  NT is the tile count
-// rounded up to a multiple of kGroupTiles, so the kGroupTiles buckets of a
-// tile group (m, g) are adjacent.  Five kernels:
+// {j - T*TW | (a mod Bfull) << 16, weight bits}.  A group is 2^14 neurons
+// (1 << gt_shift tiles); NT is the tile count rounded up to whole groups, so
+// the buckets of a group (m, g) are adjacent.  Five kernels:
 //   hist     per block: counts per bucket (LDS), written as a row of hist_f
 //            and, summed per group, as a row of hist_c
 //   colscan  hist_c columns: exclusive scan over blocks; hist_f columns: totals
@@ -179,8 +182,8 @@ struct UpdateArgs {
 //            block's events per group are contiguous, so stores coalesce)
 //   refine   one block per group: staging -> ev[E % R] in bucket order
 // The update of batch E+m reads bucket (m, T) into LDS.
-constexpr uint32_t kGroupShift = 4;
-constexpr uint32_t kGroupTiles = 1u << kGroupShift;
+constexpr uint32_t kGroupNeuronShift = 14;  // neurons per group: 16384
+constexpr uint32_t kMaxGroupTiles = 64;      // 16384 / the smallest tile (256)
 
 struct BucketArgs {
     const naer_spike* list0;
@@ -192,7 +195,7 @@ struct BucketArgs {
     const uint32_t* col;
     const float* w;
     uint32_t* hist;    // hist_f [NB][NBK]
-    uint32_t* hist_c;  // [NB][NBK / kGroupTiles]
+    uint32_t* hist_c;  // [NB][NBK >> gt_shift]
     uint32_t* totals;  // [NBK]
     uint32_t* bs;      // [R][NBK + 1]
     uint32_t* flags;   // [R]
@@ -201,6 +204,7 @@ struct BucketArgs {
     void* ring;        // fallback when the batch exceeds e_cap
     unsigned long long e_cap;
     uint32_t N, D, Bfull, B, R, M, NT, NBK, tw_shift;
+    uint32_t gt_shift;  // tiles per group = 1 << gt_shift = 2^14 >> tw_shift
     uint32_t G;        // list segments per histogram/placement block (<= kMaxSegPerBlock)
     uint32_t NB;       // histogram/placement blocks
     uint32_t C;        // 256-synapse chunks of the longest row

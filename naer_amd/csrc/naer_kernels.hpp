@@ -5,7 +5,8 @@
 
 namespace naer {
 
-hipError_t launch_update(const UpdateArgs& a, bool det, bool plastic, bool bucket, hipStream_t s);
+// pois: the net may contain POISSON sources (otherwise their code is left out)
+hipError_t launch_update(const UpdateArgs& a, bool det, bool plastic, bool bucket, bool pois, hipStream_t s);
 // histogram, column scan, bucket starts and placement of one aligned batch
 hipError_t launch_bucketed(const BucketArgs& a, bool det, hipStream_t s);
 hipError_t launch_bucket_scrub(const UpdateArgs& a, uint32_t j_begin, uint32_t j_end, hipStream_t s);

@@ -189,6 +189,7 @@ struct naer_net {
 
     // update tiles and bucketed delivery
     uint32_t tw_shift = 8;   // update block = tile of 1 << tw_shift neurons
+    uint32_t gt_shift = 6;   // tiles per bucket group
     bool bucketed = false;   // full aligned batches are delivered through buckets
     uint32_t R = 0, M = 0, NT = 0, NBK = 0, NB = 0, G = 1;
     uint64_t e_cap = 0;
@@ -706,7 +707,8 @@ int alloc_buckets(naer_net* n)
     if (n->plastic || n->B < 2 || n->S_cap == 0)
         return NAER_OK;
     n->M = (n->B - 1 + n->dmax) / n->B;
-    n->NT = (n->n_seg + kGroupTiles - 1) / kGroupTiles * kGroupTiles;
+    n->gt_shift = kGroupNeuronShift - n->tw_shift;
+    n->NT = (n->n_seg + (1u << n->gt_shift) - 1) >> n->gt_shift << n->gt_shift;
     n->NBK = n->M * n->NT;
     if (n->NBK > kMaxBuckets)
         return NAER_OK;
@@ -721,7 +723,7 @@ int alloc_buckets(naer_net* n)
     if ((rc = n->alloc(&n->ev, (size_t)n->R * n->e_cap)) || (rc = n->alloc(&n->stage, n->e_cap)) ||
         (rc = n->alloc(&n->bs, (size_t)n->R * (n->NBK + 1))) || (rc = n->alloc(&n->flags, n->R)) ||
         (rc = n->alloc(&n->hist, (size_t)n->NB * n->NBK)) ||
-        (rc = n->alloc(&n->hist_c, (size_t)n->NB * (n->NBK / kGroupTiles))) || (rc = n->alloc(&n->totals, n->NBK)))
+        (rc = n->alloc(&n->hist_c, (size_t)n->NB * (n->NBK >> n->gt_shift))) || (rc = n->alloc(&n->totals, n->NBK)))
         return rc;
     HIP_TRY(hipMemsetAsync(n->flags, 0, n->R * sizeof(uint32_t), n->stream));
     n->bucketed = true;
@@ -883,6 +885,7 @@ int naer_net_build(naer_net* n)
         for (auto& q : net.pops)
             any_pois = any_pois || q.kind == NAER_POP_POISSON;
     int rc;
+    n->poisson_ok = any_pois;
     if ((rc = alloc_neurons(n, any_pois)))
         return rc;
     // 1. out-degrees of every network, rows back to back
@@ -1122,7 +1125,7 @@ int naer_arena_retire(naer_net* n, uint32_t slot)
 static hipError_t do_update(naer_net* n, void* arg)
 {
     UpdateArgs* a = (UpdateArgs*)arg;
-    return launch_update(*a, n->det, n->plastic, n->bucketed, n->stream);
+    return launch_update(*a, n->det, n->plastic, n->bucketed, n->poisson_ok, n->stream);
 }
 
 static hipError_t do_scatter(naer_net* n, void* arg)
@@ -1199,6 +1202,7 @@ static int enqueue_steps(naer_net* n, uint32_t n_steps, bool timed)
     ba.NT = n->NT;
     ba.NBK = n->NBK;
     ba.tw_shift = n->tw_shift;
+    ba.gt_shift = n->gt_shift;
     ba.G = n->G;
     ba.NB = n->NB;
     ba.C = n->scatter_chunks;
