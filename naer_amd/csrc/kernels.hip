@@ -30,6 +30,12 @@ __device__ __forceinline__ long long to_fixed(float w)
     return __double2ll_rn((double)w * 4294967296.0);
 }
 
+// value of the first active lane (wave-uniform values into scalar registers)
+__device__ __forceinline__ uint32_t rfl(uint32_t x) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)x); }
+__device__ __forceinline__ float rflf(float x) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x))); }
+
+constexpr uint32_t kCdfHead = 8;  // drive table entries compared in registers
+
 __device__ __forceinline__ float from_fixed(long long a) { return (float)((double)a * 0x1p-32); }
 
 __device__ __forceinline__ float clip(float w, float lo, float hi)
@@ -101,7 +107,9 @@ template <bool DET, bool PLASTIC, bool BUCKET, bool POIS, uint32_t TW>
 __global__ __launch_bounds__(TW, (BUCKET && !DET && !POIS && TW == 1024) ? 8 : 1) void k_update(const UpdateArgs a)
 {
     using RingT = typename std::conditional<DET, long long, float>::type;
-    using AccT = typename std::conditional<DET, unsigned long long, float>::type;
+    // bucketed input accumulates in 32.32 fixed point in both modes: integer
+    // LDS atomics run at several times the rate of fp32 ones on gfx950
+    using AccT = unsigned long long;
     extern __shared__ unsigned long long s_dyn[];
     AccT* acc = (AccT*)s_dyn;  // [B][TW] when BUCKET
     __shared__ naer_spike s_list[kLdsSpikes];
@@ -126,45 +134,61 @@ __global__ __launch_bounds__(TW, (BUCKET && !DET && !POIS && TW == 1024) ? 8 : 1
     // s_pre[m] <= i < s_pre[m + 1], at ev + s_evbase[m] + i
     __shared__ uint32_t s_pre[kMaxArrivalBatches + 1];
     __shared__ long long s_evbase[kMaxArrivalBatches];
+    __shared__ uint32_t s_live[kMaxArrivalBatches];
     __shared__ uint32_t s_ring_live;
     const unsigned long long A = t0 / a.Bfull;
     const uint32_t o0 = (uint32_t)(t0 - A * a.Bfull);
+    // the thread's neuron state, loaded first so that its latency overlaps
+    // the event phase
+    const uint32_t j = blockIdx.x * TW + threadIdx.x;
+    float V_in = 0.0f;
+    int32_t ref_in = 0, last_in = 0;
+    if (j < a.N) {
+        V_in = a.V[j];
+        ref_in = a.ref[j];
+        last_in = a.last[j];
+    }
     if (BUCKET) {
         for (uint32_t i = threadIdx.x; i < a.B * TW; i += TW)
             acc[i] = (AccT)0;
-        if (threadIdx.x == 0)
-            s_ring_live = 0;
-        __syncthreads();
         if (threadIdx.x < a.M) {
             const uint32_t m = threadIdx.x + 1;
-            uint32_t lo = 0, hi = 0;
+            uint32_t lo = 0, hi = 0, live = 0;
             uint32_t buf = 0;
             if (A >= m) {
                 buf = (uint32_t)((A - m) % a.R);
-                if (a.flags[buf]) {
-                    const uint32_t* bs = a.bs + (size_t)buf * (a.NBK + 1);
-                    const uint32_t k = (m - 1) * a.NT + blockIdx.x;
-                    lo = bs[k];
-                    hi = bs[k + 1];
+                // bounds are read together with the flag (in bounds either
+                // way) and dropped when the batch went through the ring
+                const uint32_t* bs = a.bs + (size_t)buf * (a.NBK + 1);
+                const uint32_t k = (m - 1) * a.NT + blockIdx.x;
+                const uint32_t f = a.flags[buf];
+                const uint32_t b_lo = bs[k], b_hi = bs[k + 1];
+                if (f) {
+                    lo = b_lo;
+                    hi = b_hi;
                 } else {
-                    s_ring_live = 1;  // that batch delivered through the ring
+                    live = 1;
                 }
             }
             s_pre[m] = hi - lo;
             s_evbase[m - 1] = (long long)buf * (long long)a.e_cap + lo;
+            s_live[m - 1] = live;
         }
         __syncthreads();
         if (threadIdx.x == 0) {
+            uint32_t live = 0;
             s_pre[0] = 0;
             for (uint32_t m = 0; m < a.M; ++m) {
                 const uint32_t n = s_pre[m + 1];
                 s_evbase[m] -= s_pre[m];
                 s_pre[m + 1] = s_pre[m] + n;
+                live |= s_live[m];
             }
+            s_ring_live = live;
         }
         __syncthreads();
         const uint32_t total = s_pre[a.M];
-        constexpr uint32_t U = 4;
+        constexpr uint32_t U = 8;
         uint32_t m = 0;  // bucket of the thread's current event (its indexes only grow)
         for (uint32_t i0 = threadIdx.x; i0 < total; i0 += TW * U) {
             uint2 e[U];
@@ -183,11 +207,7 @@ __global__ __launch_bounds__(TW, (BUCKET && !DET && !POIS && TW == 1024) ? 8 : 1
                 const uint32_t o = (e[u].x >> 16) - o0;
                 if (o < a.B) {
                     const float w = __uint_as_float(e[u].y);
-                    if (DET)
-                        atomicAdd((unsigned long long*)&acc[o * TW + (e[u].x & 0xFFFFu)],
-                                  (unsigned long long)to_fixed(w));
-                    else
-                        atomicAdd((float*)&acc[o * TW + (e[u].x & 0xFFFFu)], w);
+                    atomicAdd(&acc[o * TW + (e[u].x & 0xFFFFu)], (unsigned long long)to_fixed(w));
                 }
             }
         }
@@ -207,16 +227,25 @@ __global__ __launch_bounds__(TW, (BUCKET && !DET && !POIS && TW == 1024) ? 8 : 1
     }
     Stage sg{ s_list, &s_n, seg, a.seg_cap };
 
-    const uint32_t j = blockIdx.x * TW + threadIdx.x;
     uint32_t my_events = 0;  // <= B * longest row
     // neurons outside every population (free ranges of an arena) stay idle
     const uint32_t p = (j < a.N && a.n_pops) ? find_pop(a.pops, a.n_pops, p_first, j) : 0u;
-    const bool active = j < a.N && a.n_pops && j >= a.pops[p].begin && j < a.pops[p].end;
-    if (active) {
-        const PopParams pp = a.pops[p];
-        const uint32_t jl = j - pp.net_base;                 // neuron id within its net
-        const uint32_t t0l = (uint32_t)(t0 - pp.t_start);    // net-local step of t0
-        if (pp.kind == NAER_POP_LIF) {
+    bool todo = j < a.N && a.n_pops && j >= a.pops[p].begin && j < a.pops[p].end;
+    // The lanes of a wave are processed population by population, so the
+    // population's constants are wave-uniform (scalar registers); a wave
+    // almost always lies in one population.
+    for (;;) {
+        const unsigned long long pending = __ballot(todo);
+        if (pending == 0)
+            break;
+        const uint32_t pu = (uint32_t)__builtin_amdgcn_readlane((int)p, (int)(__ffsll((long long)pending) - 1));
+        if (!(todo && p == pu))
+            continue;
+        todo = false;
+        const PopParams& pp = a.pops[pu];
+        const uint32_t jl = j - rfl(pp.net_base);                          // neuron id within its net
+        const uint32_t t0l = (uint32_t)(t0 - ((unsigned long long)rfl(pp.t_start_hi) << 32 | rfl(pp.t_start_lo)));
+        if (rfl(pp.kind) == NAER_POP_LIF) {
             RingT* ring = (RingT*)a.ring;
             uint32_t slot0 = (uint32_t)(t0 % a.D);
             // ring slots of the batch: into registers, or (bucketed) added to
@@ -235,7 +264,8 @@ __global__ __launch_bounds__(TW, (BUCKET && !DET && !POIS && TW == 1024) ? 8 : 1
                 for (uint32_t s = 0; s < a.B; ++s) {
                     uint32_t sl = slot0 + s;
                     sl = sl >= a.D ? sl - a.D : sl;
-                    acc[s * TW + threadIdx.x] += (AccT)ring[(size_t)sl * a.N + j];
+                    const RingT r = ring[(size_t)sl * a.N + j];
+                    acc[s * TW + threadIdx.x] += DET ? (AccT)r : (AccT)to_fixed((float)r);
                 }
             }
             if (ring_live) {
@@ -245,22 +275,29 @@ __global__ __launch_bounds__(TW, (BUCKET && !DET && !POIS && TW == 1024) ? 8 : 1
                     ring[(size_t)sl * a.N + j] = (RingT)0;
                 }
             }
-            float V = a.V[j];
-            int32_t ref = a.ref[j];
-            int32_t last = a.last[j];
+            float V = V_in;
+            int32_t ref = ref_in;
+            int32_t last = last_in;
             float xh = 0.0f, yh = 0.0f;
             if (PLASTIC) {
                 xh = a.xh[j];
                 yh = a.yh[j];
             }
-            // population constants in registers for the whole batch
-            const float P = pp.P, v_th = pp.v_th, v_reset = pp.v_reset, v_rest = pp.v_rest;
-            const float w_ext = pp.w_ext;
-            const int32_t n_ref = pp.n_ref;
-            const bool has_drive = pp.has_drive != 0;
-            const float* cdf = p - p_first < kStagePops ? s_cdf + (p - p_first) * NAER_DRIVE_TABLE
-                                                        : a.cdf + pp.cdf_offset;
-            const uint32_t s_lo = pp.seed_lo, s_hi = pp.seed_hi;
+            // population constants in scalar registers for the whole batch
+            const float P = rflf(pp.P), v_th = rflf(pp.v_th), v_reset = rflf(pp.v_reset);
+            const float v_rest = rflf(pp.v_rest), w_ext = rflf(pp.w_ext);
+            const int32_t n_ref = (int32_t)rfl((uint32_t)pp.n_ref);
+            const bool has_drive = rfl(pp.has_drive) != 0;
+            const float* cdf = pu - p_first < kStagePops ? s_cdf + (pu - p_first) * NAER_DRIVE_TABLE
+                                                         : a.cdf + rfl(pp.cdf_offset);
+            // the table's first kCdfHead entries, also in scalar registers: the
+            // count k = #{i < kCdfHead : u >= cdf[i]} is the search result
+            // whenever it is below kCdfHead (the table is nondecreasing)
+            float head[kCdfHead];
+#pragma unroll
+            for (uint32_t i = 0; i < kCdfHead; ++i)
+                head[i] = rflf(cdf[i]);
+            const uint32_t s_lo = rfl(pp.seed_lo), s_hi = rfl(pp.seed_hi);
             U4 rnd{ 0, 0, 0, 0 };
             uint32_t rnd_blk = 0xFFFFFFFFu;
 #pragma nounroll
@@ -268,10 +305,7 @@ __global__ __launch_bounds__(TW, (BUCKET && !DET && !POIS && TW == 1024) ? 8 : 1
                 const unsigned long long t = t0 + s;
                 float I;
                 if (BUCKET) {
-                    if (DET)
-                        I = from_fixed((long long)acc[s * TW + threadIdx.x]);
-                    else
-                        I = (float)acc[s * TW + threadIdx.x];
+                    I = from_fixed((long long)acc[s * TW + threadIdx.x]);
                 } else {
                     const RingT cur = in[0];
 #pragma unroll
@@ -305,9 +339,14 @@ __global__ __launch_bounds__(TW, (BUCKET && !DET && !POIS && TW == 1024) ? 8 : 1
                             rnd_blk = blk;
                         }
                         const float u = unit24(word_of(rnd, (uint32_t)tl & 3u));
-                        int k = 0;
-                        while (u >= cdf[k])
-                            ++k;
+                        uint32_t k = 0;
+#pragma unroll
+                        for (uint32_t i = 0; i < kCdfHead; ++i)
+                            k += u >= head[i] ? 1u : 0u;
+                        if (k == kCdfHead) {
+                            while (u >= cdf[k])
+                                ++k;
+                        }
                         const float e = (float)k * w_ext;
                         v = v + e;
                     }
@@ -333,14 +372,16 @@ __global__ __launch_bounds__(TW, (BUCKET && !DET && !POIS && TW == 1024) ? 8 : 1
             // evaluated window by window; state = next spike time and the
             // number already emitted in the current 1500 us epoch.
             uint32_t nxt = a.pnext[j], cnt = a.pcnt[j];
-            const uint32_t idx = pp.pois_base + (j - pp.begin);
+            const uint32_t idx = rfl(pp.pois_base) + (j - rfl(pp.begin));
             const uint32_t gid = ref_poisson_gid(idx);
+            const double neg_rate = pp.neg_rate_per_us;
+            const uint32_t pois_seed = rfl(pp.pois_seed), n_pois = rfl(pp.n_pois);
 #pragma nounroll
             for (uint32_t s = 0; s < a.B; ++s) {
                 const unsigned long long t = t0 + s;
                 const uint32_t T = (uint32_t)(t0l + s) * a.dt_us;
                 if (T % kEpochUs == 0) {
-                    nxt = T + ref_isi_us(pp.neg_rate_per_us, gid, pp.pois_seed, T * pp.n_pois + idx);
+                    nxt = T + ref_isi_us(neg_rate, gid, pois_seed, T * n_pois + idx);
                     cnt = 0;
                 }
                 const uint32_t end = T + a.dt_us;
@@ -350,7 +391,7 @@ __global__ __launch_bounds__(TW, (BUCKET && !DET && !POIS && TW == 1024) ? 8 : 1
                     if (pend) {
                         my_events += (uint32_t)(a.row_ptr[j + 1] - a.row_ptr[j]);
                         ++cnt;
-                        nxt = nxt + ref_isi_us(pp.neg_rate_per_us, gid, pp.pois_seed, nxt * pp.n_pois + idx);
+                        nxt = nxt + ref_isi_us(neg_rate, gid, pois_seed, nxt * n_pois + idx);
                         pend = cnt < kMaxPoissonPerEpoch && nxt < end;
                     }
                 }
@@ -1056,7 +1097,7 @@ static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b -
 template <bool DET, bool PLASTIC, bool BUCKET, bool POIS, uint32_t TW>
 static hipError_t launch_update_t(const UpdateArgs& a, hipStream_t s)
 {
-    const size_t lds = BUCKET ? (size_t)a.B * TW * (DET ? 8 : 4) : 0;
+    const size_t lds = BUCKET ? (size_t)a.B * TW * sizeof(unsigned long long) : 0;
     static size_t lds_set = 0;  // per instantiation: largest size allowed so far
     if (lds > 32768 && lds > lds_set) {
         hipError_t e = hipFuncSetAttribute((const void*)k_update<DET, PLASTIC, BUCKET, POIS, TW>,
@@ -1075,13 +1116,16 @@ static hipError_t launch_update_tw(const UpdateArgs& a, bool pois, hipStream_t s
     if (a.tw_shift == 10)
         return pois ? launch_update_t<DET, PLASTIC, BUCKET, true, 1024>(a, s)
                     : launch_update_t<DET, PLASTIC, BUCKET, false, 1024>(a, s);
+    if (a.tw_shift == 9)
+        return pois ? launch_update_t<DET, PLASTIC, BUCKET, true, 512>(a, s)
+                    : launch_update_t<DET, PLASTIC, BUCKET, false, 512>(a, s);
     return pois ? launch_update_t<DET, PLASTIC, BUCKET, true, 256>(a, s)
                 : launch_update_t<DET, PLASTIC, BUCKET, false, 256>(a, s);
 }
 
 hipError_t launch_update(const UpdateArgs& a, bool det, bool plastic, bool bucket, bool pois, hipStream_t s)
 {
-    if (a.tw_shift != 8 && a.tw_shift != 10)
+    if (a.tw_shift < 8 || a.tw_shift > 10)
         return hipErrorInvalidValue;
     if (plastic)  // plastic nets deliver through the ring only
         return det ? launch_update_tw<true, true, false>(a, pois, s) : launch_update_tw<false, true, false>(a, pois, s);

@@ -107,7 +107,7 @@ struct PopParams {
     uint32_t seed_lo, seed_hi;
     uint32_t pois_seed;
     uint32_t _pad;
-    unsigned long long t_start;  // pack step at which the net's local time is 0
+    uint32_t t_start_lo, t_start_hi;  // pack step at which the net's local time is 0
 };
 
 // Step bookkeeping that lives on the device so that launches (and captured

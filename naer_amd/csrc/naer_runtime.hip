@@ -381,7 +381,8 @@ void fill_pop(const naer_net* n, const NetInfo& net, uint32_t p_local, uint32_t 
     r.seed_lo = (uint32_t)net.desc.seed;
     r.seed_hi = (uint32_t)(net.desc.seed >> 32);
     r.pois_seed = net.desc.poisson_seed;
-    r.t_start = net.t_start;
+    r.t_start_lo = (uint32_t)net.t_start;
+    r.t_start_hi = (uint32_t)(net.t_start >> 32);
     r.n_pois = n_pois;
     for (uint32_t k = 0; k < NAER_DRIVE_TABLE; ++k)
         cdf[k] = 2.0f;
