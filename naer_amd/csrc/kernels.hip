@@ -538,7 +538,7 @@ __global__ __launch_bounds__(kBlock) void k_potentiate(const PotentiateArgs a)
 // ---------------------------------------------------------------------------
 // Bucketed delivery (see BucketArgs): histogram, column scan, bucket starts,
 // placement.  Block b walks the spikes of list segments [b*G, (b+1)*G) as
-// tasks t = i*C + c (spike i, 256-synapse chunk c of its row); wave w takes
+// tasks t = i*C + c (spike i, kBucketChunk-synapse chunk c of its row); wave w takes
 // tasks w, w + waves, ...  Both passes use the same split, so each block
 // places exactly the events it counted.
 // ---------------------------------------------------------------------------
@@ -574,7 +574,7 @@ struct RowStage {
     uint32_t r[kRowRound];  // spike step - first step of the batch
 };
 
-// Calls f(r, q_begin, q_end) for every non-empty 256-synapse chunk of the
+// Calls f(r, q_begin, q_end) for every non-empty kBucketChunk-synapse chunk of the
 // rows of the block's spikes; f runs on a whole wave (q range wave-uniform).
 template <typename F>
 __device__ __forceinline__ void for_block_chunks(const BucketArgs& a, const naer_spike* list, const uint32_t* s_off,
@@ -605,9 +605,9 @@ __device__ __forceinline__ void for_block_chunks(const BucketArgs& a, const naer
         const uint32_t n_tasks = cnt * a.C;
         for (uint32_t t = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave); t < n_tasks; t += nwv) {
             const uint32_t i = t / a.C, c = t - i * a.C;
-            const uint32_t off = c * kChunk, len = rs.len[i];
+            const uint32_t off = c * kBucketChunk, len = rs.len[i];
             if (off < len)
-                f(rs.r[i], rs.beg[i] + off, rs.beg[i] + min(len, off + kChunk));
+                f(rs.r[i], rs.beg[i] + off, rs.beg[i] + min(len, off + kBucketChunk));
         }
         __syncthreads();
     }
@@ -629,14 +629,14 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_hist(const BucketArgs
     const naer_spike* list = cur ? a.list1 : a.list0;
     const uint32_t lane = lane_id();
     for_block_chunks(a, list, s_off, t0, rs, [&](uint32_t r, unsigned long long beg, unsigned long long end) {
-        uint32_t c[kChunkUnroll];
+        uint32_t c[kBucketUnroll];
 #pragma unroll
-        for (uint32_t u = 0; u < kChunkUnroll; ++u) {
+        for (uint32_t u = 0; u < kBucketUnroll; ++u) {
             const unsigned long long q = beg + u * kWave + lane;
             c[u] = q < end ? a.col[q] : 0u;
         }
 #pragma unroll
-        for (uint32_t u = 0; u < kChunkUnroll; ++u) {
+        for (uint32_t u = 0; u < kBucketUnroll; ++u) {
             const unsigned long long q = beg + u * kWave + lane;
             if (q < end) {
                 const uint32_t v = r + ((c[u] >> 24) & 0x7Fu);
@@ -792,16 +792,16 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_place(const BucketArg
     const naer_spike* list = cur ? a.list1 : a.list0;
     const uint32_t lane = lane_id();
     for_block_chunks(a, list, s_off, t0, rs, [&](uint32_t r, unsigned long long beg, unsigned long long end) {
-        uint32_t c[kChunkUnroll];
-        float w[kChunkUnroll];
+        uint32_t c[kBucketUnroll];
+        float w[kBucketUnroll];
 #pragma unroll
-        for (uint32_t u = 0; u < kChunkUnroll; ++u) {
+        for (uint32_t u = 0; u < kBucketUnroll; ++u) {
             const unsigned long long q = beg + u * kWave + lane;
             c[u] = q < end ? a.col[q] : 0u;
             w[u] = q < end ? a.w[q] : 0.0f;
         }
 #pragma unroll
-        for (uint32_t u = 0; u < kChunkUnroll; ++u) {
+        for (uint32_t u = 0; u < kBucketUnroll; ++u) {
             const unsigned long long q = beg + u * kWave + lane;
             if (q < end) {
                 const uint32_t d = (c[u] >> 24) & 0x7Fu;

@@ -17,6 +17,8 @@ constexpr uint32_t kWave = 64;         // CDNA wavefront
 constexpr uint32_t kLdsSpikes = 512;   // per-block spike staging in LDS
 constexpr uint32_t kChunkUnroll = 4;   // synapses per lane per scatter task
 constexpr uint32_t kChunk = kWave * kChunkUnroll;
+constexpr uint32_t kBucketUnroll = 8;  // synapses per lane per task of the bucketed passes
+constexpr uint32_t kBucketChunk = kWave * kBucketUnroll;
 constexpr uint32_t kTargetMask = 0x00FFFFFFu;
 constexpr uint32_t kPlasticBit = 0x80000000u;
 constexpr uint32_t kEpochUs = 1500u;   // reference epoch, naer_types.h:4
@@ -207,7 +209,7 @@ struct BucketArgs {
     uint32_t gt_shift;  // tiles per group = 1 << gt_shift = 2^14 >> tw_shift
     uint32_t G;        // list segments per histogram/placement block (<= kMaxSegPerBlock)
     uint32_t NB;       // histogram/placement blocks
-    uint32_t C;        // 256-synapse chunks of the longest row
+    uint32_t C;        // kBucketChunk-synapse chunks of the longest row
     uint32_t is_last;
 };
 

@@ -1206,7 +1206,7 @@ static int enqueue_steps(naer_net* n, uint32_t n_steps, bool timed)
     ba.gt_shift = n->gt_shift;
     ba.G = n->G;
     ba.NB = n->NB;
-    ba.C = n->scatter_chunks;
+    ba.C = std::max<uint32_t>(1, (n->max_row + kBucketChunk - 1) / kBucketChunk);
     ba.is_last = 1;
     PotentiateArgs pa;
     std::memset(&pa, 0, sizeof pa);
