@@ -10,8 +10,12 @@ resident in HBM.  A "step" is one dt = 0.1 ms update of all neurons plus the
 scatter of their spikes; K steps are timed (default 10,000 = 1 s biological).
 
 Prints one JSON line: synaptic events/s (value), realtime factor, the roofline
-of the scatter kernel measured with HIP events on the net's stream, and the
-CPU oracle timed on a bounded sample of the same network on the host cores.
+of the spike delivery (the kernels that carry a batch's spikes to their
+targets) measured with HIP events on the net's stream, per-kernel times, and
+the CPU oracle timed on a bounded sample of the same network on the host cores.
+
+    python bench.py --copy-probe   # a plain device copy of known size, the
+                                   # calibration run for rocprofv3 --pmc counters
 """
 import argparse
 import json
@@ -23,7 +27,37 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-SCATTER_BYTES_PER_EVENT = 12  # col 4 + weight 4 + ring accumulate 4 (fast mode)
+# algorithmic bytes: a static synaptic event = col 4 + weight 4 + accumulate 4;
+# a neuron-step = V r/w 8 + refractory r/w 8 + input slot read+clear 8
+BYTES_PER_EVENT = 12
+BYTES_PER_NEURON_STEP = 24
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "pmc_large_1m.md")
+COPY_PROBE_BYTES = 512 << 20
+
+
+def measured_traffic():
+    """HBM bytes per delivered batch of the delivery kernels, from the
+    calibrated rocprofv3 FETCH_SIZE/WRITE_SIZE summary (None if absent)."""
+    try:
+        with open(PMC_SUMMARY) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_batch:"):
+                    return int(float(line.split(":", 1)[1]))
+    except OSError:
+        pass
+    return None
+
+
+def copy_probe():
+    """A plain copy of COPY_PROBE_BYTES on the device, five times: what
+    FETCH_SIZE / WRITE_SIZE read for it calibrates the counters."""
+    import torch
+    src = torch.ones(COPY_PROBE_BYTES // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": COPY_PROBE_BYTES, "launches": 5}), flush=True)
 
 
 def cpu_baseline(args):
@@ -62,11 +96,16 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=2_000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the simulation step of one network is not "
               "split across GPUs in this build (--gpus must be 1)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe()
+        return
 
     from naer_amd import _abi, models
 
@@ -81,8 +120,6 @@ def main():
     net.run(args.warmup)
     net.finish()
     s0 = net.stats()
-    if not args.no_profile:
-        net.set_profiling(True)
     t0 = time.perf_counter()
     net.run(args.steps)
     net.finish()
@@ -101,26 +138,46 @@ def main():
     roofline = None
     kernels = {}
     if not args.no_profile:
-        sc_ms = s1.kernel_ms[_abi.K_SCATTER] - s0.kernel_ms[_abi.K_SCATTER]
-        sc_n = s1.kernel_calls[_abi.K_SCATTER] - s0.kernel_calls[_abi.K_SCATTER]
-        up_ms = s1.kernel_ms[_abi.K_UPDATE] - s0.kernel_ms[_abi.K_UPDATE]
-        up_n = s1.kernel_calls[_abi.K_UPDATE] - s0.kernel_calls[_abi.K_UPDATE]
-        ring_b = 8 if mode == _abi.MODE_FAST else 16
+        # kernel times: a further profiled stretch (HIP events around every
+        # launch on the net's stream), kept out of the timed region above
+        prof_steps = min(steps, args.profile_steps)
+        net.set_profiling(True)
+        s0 = net.stats()
+        net.run(prof_steps)
+        net.finish()
+        s1 = net.stats()
+        net.set_profiling(False)
+        prof_events = s1.events - s0.events
+
+        def part(k):
+            ms = s1.kernel_ms[k] - s0.kernel_ms[k]
+            n = s1.kernel_calls[k] - s0.kernel_calls[k]
+            return ms, n
+
         B = s1.batch
-        upd_bytes = N * steps * ring_b + N * (steps / B) * 24  # ring r+w per step, V/ref/last r+w per launch
-        sc_bytes = events * (SCATTER_BYTES_PER_EVENT + (4 if mode == _abi.MODE_DETERMINISTIC else 0))
-        ach = sc_bytes / (sc_ms * 1e-3) / 1e9 if sc_ms > 0 else 0.0
+        up_ms, up_n = part(_abi.K_UPDATE)
+        dl_ms, dl_n = part(_abi.K_SCATTER)  # delivered batches, all delivery kernels
+        kernels["k_update"] = {"avg_us": round(up_ms * 1e3 / max(up_n, 1), 2), "launches": int(up_n),
+                               "alg_bytes_per_launch": N * B * BYTES_PER_NEURON_STEP}
+        names = {_abi.K_BUCKET_HIST: "k_bucket_hist", _abi.K_BUCKET_SCAN: "k_bucket_colscan+k_bucket_starts",
+                 _abi.K_BUCKET_PLACE: "k_bucket_place", _abi.K_BUCKET_REFINE: "k_bucket_refine"}
+        bucketed = False
+        for k, name in names.items():
+            ms, n = part(k)
+            if n:
+                bucketed = True
+                kernels[name] = {"avg_us": round(ms * 1e3 / n, 2), "launches": int(n)}
+        if not bucketed:
+            kernels["k_scatter"] = {"avg_us": round(dl_ms * 1e3 / max(dl_n, 1), 2), "launches": int(dl_n)}
+        dl_bytes = prof_events * BYTES_PER_EVENT / max(dl_n, 1)  # per delivered batch
+        dl_us = dl_ms * 1e3 / max(dl_n, 1)
+        ach = dl_bytes / (dl_us * 1e-6) / 1e9 if dl_us > 0 else 0.0
         roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                    "kernel": "k_scatter", "avg_us": round(sc_ms * 1e3 / max(sc_n, 1), 2),
-                    "bytes_per_launch": round(sc_bytes / max(sc_n, 1))}
-        kernels = {
-            "update": {"avg_us": round(up_ms * 1e3 / max(up_n, 1), 2), "launches": int(up_n),
-                       "ms": round(up_ms, 2),
-                       "gbs": round(upd_bytes / (up_ms * 1e-3) / 1e9, 1) if up_ms > 0 else None},
-            "scatter": {"avg_us": round(sc_ms * 1e3 / max(sc_n, 1), 2), "launches": int(sc_n),
-                        "ms": round(sc_ms, 2)},
-        }
+                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
+                    "kernel": ("spike delivery: k_bucket_hist + k_bucket_colscan + k_bucket_starts + "
+                               "k_bucket_place + k_bucket_refine") if bucketed else "k_scatter",
+                    "avg_us": round(dl_us, 2), "bytes_per_launch": round(dl_bytes),
+                    "unit_of_work": f"{BYTES_PER_EVENT} B per synaptic event, one launch = one batch of {B} steps"}
 
     cpu = None
     if not args.no_cpu:

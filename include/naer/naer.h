@@ -204,9 +204,13 @@ typedef struct naer_stats {
 } naer_stats;
 
 #define NAER_K_UPDATE 0
-#define NAER_K_SCATTER 1
+#define NAER_K_SCATTER 1       /* spike delivery of a batch, all its kernels */
 #define NAER_K_POTENTIATE 2
-#define NAER_K_CLOCK 3
+/* the parts of NAER_K_SCATTER when a batch is delivered through buckets */
+#define NAER_K_BUCKET_HIST 3
+#define NAER_K_BUCKET_SCAN 4
+#define NAER_K_BUCKET_PLACE 5
+#define NAER_K_BUCKET_REFINE 6
 
 /* ---- context ---------------------------------------------------------------
  * Replaces the OpenCL platform/device/context/queue set-up of

@@ -31,7 +31,8 @@ STREAM_DRIVE = 2
 STREAM_CONN = 3
 DRIVE_TABLE = 64
 
-K_UPDATE, K_SCATTER, K_POTENTIATE, K_CLOCK = 0, 1, 2, 3
+K_UPDATE, K_SCATTER, K_POTENTIATE = 0, 1, 2
+K_BUCKET_HIST, K_BUCKET_SCAN, K_BUCKET_PLACE, K_BUCKET_REFINE = 3, 4, 5, 6
 
 u32, u64, f32, f64 = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_float, ctypes.c_double
 

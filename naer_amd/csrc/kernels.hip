@@ -1134,7 +1134,7 @@ hipError_t launch_update(const UpdateArgs& a, bool det, bool plastic, bool bucke
     return det ? launch_update_tw<true, false, false>(a, pois, s) : launch_update_tw<false, false, false>(a, pois, s);
 }
 
-hipError_t launch_bucketed(const BucketArgs& a, bool det, hipStream_t s)
+static hipError_t bucket_lds_attributes(const BucketArgs& a)
 {
     const size_t lds = (size_t)a.NBK * sizeof(uint32_t);
     static size_t lds_set = 0;
@@ -1144,27 +1144,52 @@ hipError_t launch_bucketed(const BucketArgs& a, bool det, hipStream_t s)
         if (e == hipSuccess)
             e = hipFuncSetAttribute((const void*)k_bucket_starts, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)lds);
-        if (e == hipSuccess)
-            e = hipFuncSetAttribute((const void*)k_bucket_place<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)lds);
-        if (e == hipSuccess)
-            e = hipFuncSetAttribute((const void*)k_bucket_place<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)lds);
         if (e != hipSuccess)
             return e;
         lds_set = lds;
     }
-    hipLaunchKernelGGL(k_bucket_hist, dim3(a.NB), dim3(kBucketThreads), lds, s, a);
-    hipLaunchKernelGGL(k_bucket_colscan, dim3(cdiv(a.NBK >> a.gt_shift, kScanCols) + cdiv(a.NBK, kScanCols)),
-                       dim3(kScanCols * kScanGroups), 0, s, a);
-    hipLaunchKernelGGL(k_bucket_starts, dim3(1), dim3(1024), lds, s, a);
+    return hipSuccess;
+}
+
+hipError_t launch_bucket_step(const BucketArgs& a, int step, bool det, hipStream_t s)
+{
+    hipError_t e = bucket_lds_attributes(a);
+    if (e != hipSuccess)
+        return e;
+    const size_t lds = (size_t)a.NBK * sizeof(uint32_t);
     const size_t lds_c = (size_t)(a.NBK >> a.gt_shift) * sizeof(uint32_t);
-    if (det)
-        hipLaunchKernelGGL((k_bucket_place<true>), dim3(a.NB), dim3(kBucketThreads), lds_c, s, a);
-    else
-        hipLaunchKernelGGL((k_bucket_place<false>), dim3(a.NB), dim3(kBucketThreads), lds_c, s, a);
-    hipLaunchKernelGGL(k_bucket_refine, dim3(a.NBK >> a.gt_shift), dim3(kRefineThreads), 0, s, a);
+    switch (step) {
+    case 0:
+        hipLaunchKernelGGL(k_bucket_hist, dim3(a.NB), dim3(kBucketThreads), lds, s, a);
+        break;
+    case 1:
+        hipLaunchKernelGGL(k_bucket_colscan, dim3(cdiv(a.NBK >> a.gt_shift, kScanCols) + cdiv(a.NBK, kScanCols)),
+                           dim3(kScanCols * kScanGroups), 0, s, a);
+        hipLaunchKernelGGL(k_bucket_starts, dim3(1), dim3(1024), lds, s, a);
+        break;
+    case 2:
+        if (det)
+            hipLaunchKernelGGL((k_bucket_place<true>), dim3(a.NB), dim3(kBucketThreads), lds_c, s, a);
+        else
+            hipLaunchKernelGGL((k_bucket_place<false>), dim3(a.NB), dim3(kBucketThreads), lds_c, s, a);
+        break;
+    case 3:
+        hipLaunchKernelGGL(k_bucket_refine, dim3(a.NBK >> a.gt_shift), dim3(kRefineThreads), 0, s, a);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
     return hipGetLastError();
+}
+
+hipError_t launch_bucketed(const BucketArgs& a, bool det, hipStream_t s)
+{
+    for (int step = 0; step < kBucketSteps; ++step) {
+        hipError_t e = launch_bucket_step(a, step, det, s);
+        if (e != hipSuccess)
+            return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_bucket_scrub(const UpdateArgs& a, uint32_t j_begin, uint32_t j_end, hipStream_t s)

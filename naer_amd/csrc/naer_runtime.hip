@@ -267,6 +267,8 @@ struct naer_net {
             float ms = 0.0f;
             HIP_TRY(hipEventElapsedTime(&ms, p.a, p.b));
             kernel_ms[p.kind] += ms;
+            if (p.kind >= NAER_K_BUCKET_HIST && p.kind <= NAER_K_BUCKET_REFINE)
+                kernel_ms[NAER_K_SCATTER] += ms;
             event_pool.push_back(p.a);
             event_pool.push_back(p.b);
         }
@@ -1141,10 +1143,15 @@ static hipError_t do_potentiate(naer_net* n, void* arg)
     return launch_potentiate(*a, 256, n->stream);
 }
 
-static hipError_t do_bucketed(naer_net* n, void* arg)
+struct BucketStep {
+    const BucketArgs* a;
+    int step;
+};
+
+static hipError_t do_bucket_step(naer_net* n, void* arg)
 {
-    BucketArgs* a = (BucketArgs*)arg;
-    return launch_bucketed(*a, n->det, n->stream);
+    const BucketStep* b = (const BucketStep*)arg;
+    return launch_bucket_step(*b->a, b->step, n->det, n->stream);
 }
 
 // Enqueues steps batch by batch.  Batches are aligned to multiples of B
@@ -1235,8 +1242,13 @@ static int enqueue_steps(naer_net* n, uint32_t n_steps, bool timed)
         if ((rc = launch_timed(n, NAER_K_UPDATE, do_update, &ua, timed)))
             return rc;
         if (n->bucketed && off == 0 && b == n->B) {
-            if ((rc = launch_timed(n, NAER_K_SCATTER, do_bucketed, &ba, timed)))
-                return rc;
+            // timed per part; the parts add up to NAER_K_SCATTER
+            for (int k = 0; k < kBucketSteps; ++k) {
+                BucketStep bst{ &ba, k };
+                if ((rc = launch_timed(n, NAER_K_BUCKET_HIST + k, do_bucket_step, &bst, timed)))
+                    return rc;
+            }
+            n->kernel_calls[NAER_K_SCATTER] += 1;
         } else if ((rc = launch_timed(n, NAER_K_SCATTER, do_scatter, &sa, timed))) {
             return rc;
         }
@@ -1341,7 +1353,7 @@ int naer_net_query(naer_net* n, naer_stats* out)
     out->batch = n->B;
     out->max_row = n->max_row;
     out->device_bytes = n->dev_bytes;
-    out->kernels = 4;
+    out->kernels = 7;
     for (int k = 0; k < 8; ++k) {
         out->kernel_ms[k] = n->kernel_ms[k];
         out->kernel_calls[k] = n->kernel_calls[k];
