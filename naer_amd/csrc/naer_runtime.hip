@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -718,6 +719,13 @@ int alloc_buckets(naer_net* n)
     n->R = n->M + 1;
     const double expect = (double)n->S_cap * n->B * n->dt_ms * 1e-3 * kBucketRateHz;
     n->e_cap = (uint64_t)std::min(std::max(expect, (double)(1u << 20)), (double)0x7FFFFFFFu);
+    // NAER_BUCKET_EVENTS overrides the capacity (tests use a small one to
+    // drive batches through the ring fallback)
+    if (const char* env = std::getenv("NAER_BUCKET_EVENTS")) {
+        const unsigned long long v = std::strtoull(env, nullptr, 10);
+        if (v > 0)
+            n->e_cap = std::min<unsigned long long>(v, 0x7FFFFFFFull);
+    }
     n->G = std::max<uint32_t>(1, (n->n_seg + 255) / 256);
     n->NB = (n->n_seg + n->G - 1) / n->G;
     if (n->M > kMaxArrivalBatches || n->G > kMaxSegPerBlock)

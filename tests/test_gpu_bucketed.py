@@ -1,0 +1,124 @@
+"""Bucketed spike delivery (aligned batches of B steps whose events are sorted
+by arrival batch and target tile, then summed in LDS by the update) against
+the oracle, in the cases where it hands over to the delay ring: partial
+batches from run lengths that are not multiples of B, batches that exceed the
+event buffer, graph-captured chunks, and at the full 1M-neuron size the
+conservation of events and run-to-run determinism.
+
+Bar: NAER_MODE_DETERMINISTIC rasters, potentials and counters bit-exact with
+the oracle's 32.32 fixed-point step.
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import first_divergence, oracle_of
+from naer_amd import _abi, models
+
+pytestmark = pytest.mark.gpu
+
+
+def multi_delay_net(record=2_000_000):
+    """8,000 neurons, in-degree 800, delays 1.5-6 ms on a 0.5 ms grid: B = 15
+    steps per batch, events spread over M = 4 later batches."""
+    net = models.balanced(8000, 800, 0.1, 5.0, 20_000.0, 0.1, delay_ms=(1.5, 6.0, 0.5),
+                          mode=_abi.MODE_DETERMINISTIC, seed=0xB0C4E7)
+    net.record_capacity = record
+    return net
+
+
+def assert_same(net, o):
+    gs, os_ = net.spikes(), o.spikes()
+    assert len(gs) > 500
+    assert first_divergence(gs, os_) is None
+    st, ost = net.state(), o.state()
+    np.testing.assert_array_equal(st["v"].view(np.uint32), ost["v"].view(np.uint32))
+    np.testing.assert_array_equal(st["ref"], ost["ref"])
+    s = net.stats()
+    assert s.spikes == o.spike_count and s.events == o.event_count
+
+
+def test_batch_shape():
+    net = multi_delay_net()
+    net.compile()
+    s = net.stats()
+    assert s.batch == 15 and s.ring_depth == 61
+    net.close()
+
+
+def test_irregular_run_lengths_bit_exact():
+    net = multi_delay_net()
+    net.compile()
+    o = oracle_of(net, O.FIXED)
+    chunks = [1, 14, 15, 7, 30, 23, 45, 3, 12, 100, 50, 15, 15, 29, 1, 140]
+    for c in chunks:
+        net.run(c)
+    o.run(sum(chunks))
+    assert_same(net, o)
+    net.close()
+
+
+@pytest.mark.parametrize("cap", [2_000, 150_000])
+def test_event_buffer_overflow_falls_back_to_ring(cap):
+    # a small event buffer: every batch (2_000) or the busier ones (150_000)
+    # go through the ring from the placement kernel
+    os.environ["NAER_BUCKET_EVENTS"] = str(cap)
+    try:
+        net = multi_delay_net()
+        net.compile()
+    finally:
+        del os.environ["NAER_BUCKET_EVENTS"]
+    o = oracle_of(net, O.FIXED)
+    net.run(450)
+    o.run(450)
+    if cap < 10_000:
+        assert net.stats().events / 30 > cap  # no batch fits
+    assert_same(net, o)
+    net.close()
+
+
+def test_graph_chunks_with_unaligned_runs():
+    net = multi_delay_net()
+    net.compile()
+    net.set_graph(25)  # captured chunks round up to whole batches (30 steps)
+    o = oracle_of(net, O.FIXED)
+    chunks = [37, 61, 100, 8, 94]
+    for c in chunks:
+        net.run(c)
+    o.run(sum(chunks))
+    assert_same(net, o)
+    net.close()
+
+
+def _row_ptr(net):
+    rp = np.zeros(net.getNeurons() + 1, np.uint64)
+    _abi.check(_abi.lib().naer_net_read_csr(net._h(), rp.ctypes.data, None, None))
+    return rp
+
+
+def test_full_size_events_conserved_and_deterministic():
+    """1M neurons, 2e9 synapses: every recorded spike delivers exactly its
+    out-degree, and two runs give the same raster bit for bit."""
+    rasters = []
+    for _ in range(2):
+        net = models.large_1m(mode=_abi.MODE_DETERMINISTIC, record_capacity=2_000_000)
+        net.compile()
+        net.run(400)
+        net.finish()
+        net.spikes()  # drain the warm-up
+        e0 = net.stats().events
+        net.run(300)
+        net.finish()
+        sp = net.spikes()
+        de = net.stats().events - e0
+        rp = _row_ptr(net)
+        deg = (rp[1:] - rp[:-1]).astype(np.int64)
+        assert len(sp) > 10_000
+        assert int(deg[sp["neuron"]].sum()) == de
+        assert deg.sum() == 2_000_000_000
+        rasters.append(sp)
+        net.close()
+    assert first_divergence(rasters[0], rasters[1]) is None
