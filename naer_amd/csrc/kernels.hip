@@ -516,20 +516,42 @@ __global__ __launch_bounds__(kBlock) void k_potentiate(const PotentiateArgs a)
     const uint32_t cur = st->cur;
     const naer_spike* list = cur ? a.list1 : a.list0;
     const uint32_t* counts = a.seg_count + cur * a.n_seg;
-    if (blockIdx.x == 0 && threadIdx.x == 0)
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0)
         batch_epilogue(st, a.B);
     const uint32_t lane = lane_id();
     const uint32_t nw = gridDim.x * (kBlock / kWave);
+    const unsigned long long chunk_off = (unsigned long long)blockIdx.y * kChunk;
     for (uint32_t g = wave_global(); g < a.n_seg; g += nw) {
         const uint32_t cnt = counts[g];
         const naer_spike* seg = list + (size_t)g * a.seg_cap;
         for (uint32_t i = 0; i < cnt; ++i) {
             const uint32_t post = seg[i].neuron;
-            const unsigned long long rb = a.rev_ptr[post], re = a.rev_ptr[post + 1];
-            for (unsigned long long r = rb + lane; r < re; r += kWave) {
-                const uint32_t pos = a.rev_pos[r];
-                const float dw = a.a_plus * a.xh[a.rev_src[r]];
-                a.w[pos] = clip(a.w[pos] + dw, a.w_min, a.w_max);
+            const unsigned long long re = a.rev_ptr[post + 1];
+            const unsigned long long rb = a.rev_ptr[post] + chunk_off;
+            if (rb >= re)
+                continue;
+            // kChunkUnroll synapses per lane: index loads, then gathers, then updates
+            uint32_t pos[kChunkUnroll], src[kChunkUnroll];
+#pragma unroll
+            for (uint32_t u = 0; u < kChunkUnroll; ++u) {
+                const unsigned long long r = rb + u * kWave + lane;
+                pos[u] = r < re ? a.rev_pos[r] : 0u;
+                src[u] = r < re ? a.rev_src[r] : 0u;
+            }
+            float x[kChunkUnroll], w[kChunkUnroll];
+#pragma unroll
+            for (uint32_t u = 0; u < kChunkUnroll; ++u) {
+                const unsigned long long r = rb + u * kWave + lane;
+                x[u] = r < re ? a.xh[src[u]] : 0.0f;
+                w[u] = r < re ? a.w[pos[u]] : 0.0f;
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kChunkUnroll; ++u) {
+                const unsigned long long r = rb + u * kWave + lane;
+                if (r < re) {
+                    const float dw = a.a_plus * x[u];
+                    a.w[pos[u]] = clip(w[u] + dw, a.w_min, a.w_max);
+                }
             }
         }
     }
@@ -752,8 +774,10 @@ __global__ __launch_bounds__(1024) void k_bucket_starts(const BucketArgs a)
     const unsigned long long t0 = a.st->t0_cur;
     const uint32_t buf = (uint32_t)((t0 / a.Bfull) % a.R);
     uint32_t* bs = a.bs + (size_t)buf * (a.NBK + 1);
-    for (uint32_t k = threadIdx.x; k < a.NBK; k += 1024)
+    for (uint32_t k = threadIdx.x; k < a.NBK; k += 1024) {
         bs[k] = s_tot[k];
+        a.gcur[k] = s_tot[k];
+    }
     if (threadIdx.x == 1023) {
         const unsigned long long total = s_part[1023];
         bs[a.NBK] = (uint32_t)min(total, 0xFFFFFFFFull);
@@ -827,42 +851,53 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_place(const BucketArg
     });
 }
 
-// One block per group bucket (m, g): its events, in the staging buffer at the
-// range of buckets [kc << gt_shift, (kc + 1) << gt_shift), go to their
-// tile's bucket of ev[E % R] (same positions overall, tile order within).
+// Group bucket (m, g) = blockIdx.x: its events lie in the staging buffer at
+// the range of buckets [kc << gt_shift, (kc + 1) << gt_shift); blockIdx.y
+// takes the y-th of RS equal parts of that range and moves its events to
+// their tile's bucket of ev[E % R], kRefineChunk at a time: counted per tile
+// in LDS, a base per tile taken from the buckets' global cursors, written.
 constexpr uint32_t kRefineThreads = 512;
+constexpr uint32_t kRefineChunk = 4096;
 
 __global__ __launch_bounds__(kRefineThreads) void k_bucket_refine(const BucketArgs a)
 {
-    __shared__ uint32_t s_cur[kMaxGroupTiles];
+    __shared__ uint2 s_ev[kRefineChunk];
+    __shared__ uint16_t s_rank[kRefineChunk];
+    __shared__ uint32_t s_cnt[kMaxGroupTiles];
+    __shared__ uint32_t s_base[kMaxGroupTiles];
     const unsigned long long t0 = a.st->t0_cur;
     const uint32_t buf = (uint32_t)((t0 / a.Bfull) % a.R);
     if (!a.flags[buf])
         return;
     const uint32_t* bs = a.bs + (size_t)buf * (a.NBK + 1);
+    const uint32_t gt = 1u << a.gt_shift;
     const uint32_t k0 = blockIdx.x << a.gt_shift;
-    if (threadIdx.x < (1u << a.gt_shift))
-        s_cur[threadIdx.x] = bs[k0 + threadIdx.x];
-    const uint32_t lo = bs[k0], hi = bs[k0 + (1u << a.gt_shift)];
-    __syncthreads();
+    const uint32_t lo = bs[k0], hi = bs[k0 + gt];
+    const uint32_t len = hi - lo;
+    const uint32_t a0 = lo + (uint32_t)(((unsigned long long)len * blockIdx.y) / a.RS);
+    const uint32_t a1 = lo + (uint32_t)(((unsigned long long)len * (blockIdx.y + 1)) / a.RS);
     uint2* ev = a.ev + (size_t)buf * a.e_cap;
     const uint32_t tmask = (1u << a.tw_shift) - 1u;
-    constexpr uint32_t U = 4;
-    for (uint32_t i0 = lo + threadIdx.x; i0 < hi; i0 += kRefineThreads * U) {
-        uint2 e[U];
-#pragma unroll
-        for (uint32_t u = 0; u < U; ++u) {
-            const uint32_t i = i0 + u * kRefineThreads;
-            e[u] = i < hi ? a.stage[i] : make_uint2(0u, 0u);
+    for (uint32_t r0 = a0; r0 < a1; r0 += kRefineChunk) {
+        const uint32_t n = min(kRefineChunk, a1 - r0);
+        if (threadIdx.x < gt)
+            s_cnt[threadIdx.x] = 0;
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < n; i += kRefineThreads) {
+            const uint2 e = a.stage[r0 + i];
+            s_ev[i] = e;
+            s_rank[i] = (uint16_t)atomicAdd(&s_cnt[(e.x & 0xFFFFu) >> a.tw_shift], 1u);
         }
-#pragma unroll
-        for (uint32_t u = 0; u < U; ++u) {
-            if (i0 + u * kRefineThreads < hi) {
-                const uint32_t t = (e[u].x & 0xFFFFu) >> a.tw_shift;
-                const uint32_t pos = atomicAdd(&s_cur[t], 1u);
-                ev[pos] = make_uint2((e[u].x & tmask) | (e[u].x & 0xFFFF0000u), e[u].y);
-            }
+        __syncthreads();
+        if (threadIdx.x < gt && s_cnt[threadIdx.x])
+            s_base[threadIdx.x] = atomicAdd(&a.gcur[k0 + threadIdx.x], s_cnt[threadIdx.x]);
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < n; i += kRefineThreads) {
+            const uint2 e = s_ev[i];
+            const uint32_t t = (e.x & 0xFFFFu) >> a.tw_shift;
+            ev[s_base[t] + s_rank[i]] = make_uint2((e.x & tmask) | (e.x & 0xFFFF0000u), e.y);
         }
+        __syncthreads();
     }
 }
 
@@ -1174,7 +1209,7 @@ hipError_t launch_bucket_step(const BucketArgs& a, int step, bool det, hipStream
             hipLaunchKernelGGL((k_bucket_place<false>), dim3(a.NB), dim3(kBucketThreads), lds_c, s, a);
         break;
     case 3:
-        hipLaunchKernelGGL(k_bucket_refine, dim3(a.NBK >> a.gt_shift), dim3(kRefineThreads), 0, s, a);
+        hipLaunchKernelGGL(k_bucket_refine, dim3(a.NBK >> a.gt_shift, a.RS), dim3(kRefineThreads), 0, s, a);
         break;
     default:
         return hipErrorInvalidValue;
@@ -1219,9 +1254,9 @@ hipError_t launch_scatter(const ScatterArgs& a, bool det, bool plastic, uint32_t
     return hipGetLastError();
 }
 
-hipError_t launch_potentiate(const PotentiateArgs& a, uint32_t grid_x, hipStream_t s)
+hipError_t launch_potentiate(const PotentiateArgs& a, uint32_t grid_x, uint32_t chunks, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_potentiate, dim3(grid_x), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_potentiate, dim3(grid_x, chunks), dim3(kBlock), 0, s, a);
     return hipGetLastError();
 }
 

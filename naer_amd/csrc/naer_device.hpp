@@ -182,7 +182,7 @@ struct UpdateArgs {
 //   starts   exclusive scan of the bucket totals -> bs[E % R], flags[E % R]
 //   place    events to their group's range of the staging buffer (runs of a
 //            block's events per group are contiguous, so stores coalesce)
-//   refine   one block per group: staging -> ev[E % R] in bucket order
+//   refine   RS blocks per group: staging -> ev[E % R] in bucket order
 // The update of batch E+m reads bucket (m, T) into LDS.
 constexpr uint32_t kGroupNeuronShift = 14;  // neurons per group: 16384
 constexpr uint32_t kMaxGroupTiles = 64;      // 16384 / the smallest tile (256)
@@ -200,6 +200,7 @@ struct BucketArgs {
     uint32_t* hist_c;  // [NB][NBK >> gt_shift]
     uint32_t* totals;  // [NBK]
     uint32_t* bs;      // [R][NBK + 1]
+    uint32_t* gcur;    // [NBK] refine cursors of the batch's buckets
     uint32_t* flags;   // [R]
     uint2* ev;         // [R][e_cap]
     uint2* stage;      // [e_cap] events of the batch in group order
@@ -210,6 +211,7 @@ struct BucketArgs {
     uint32_t G;        // list segments per histogram/placement block (<= kMaxSegPerBlock)
     uint32_t NB;       // histogram/placement blocks
     uint32_t C;        // kBucketChunk-synapse chunks of the longest row
+    uint32_t RS;       // refine blocks per group bucket
     uint32_t is_last;
 };
 

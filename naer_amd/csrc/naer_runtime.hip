@@ -123,6 +123,7 @@ struct NetInfo {
     uint64_t syn_begin = 0, n_syn = 0;
     uint64_t rev_begin = 0, n_rev = 0;
     uint32_t max_row = 0;
+    uint32_t max_rev = 0;  // longest reverse (plastic in-synapse) row
     uint32_t dmin = 0xFFFFFFFFu, dmax = 0;
     bool plastic = false;
     bool active = false;
@@ -186,6 +187,7 @@ struct naer_net {
     uint32_t probe_n = 0;
 
     uint32_t scatter_chunks = 1, scatter_grid = 1;
+    uint32_t rev_chunks = 1;  // potentiation grid rows: longest reverse row / kChunk
     uint64_t steps_enqueued = 0;
 
     // update tiles and bucketed delivery
@@ -197,6 +199,7 @@ struct naer_net {
     uint2* ev = nullptr;
     uint2* stage = nullptr;
     uint32_t *bs = nullptr, *flags = nullptr, *hist = nullptr, *hist_c = nullptr, *totals = nullptr;
+    uint32_t* gcur = nullptr;
 
     bool profiling = false;
     uint32_t graph_chunk = 0;  // steps per captured graph (0 = plain launches)
@@ -625,12 +628,15 @@ int place_net(naer_net* n, const NetInfo& net)
 }
 
 // Counts the plastic synapses onto every target of a network.
-int reverse_count(naer_net* n, const NetInfo& net, std::vector<unsigned long long>* prefix)
+int reverse_count(naer_net* n, NetInfo& net, std::vector<unsigned long long>* prefix)
 {
     HIP_TRY(hipMemsetAsync(n->deg + net.neuron_begin, 0, (size_t)net.n_neurons * sizeof(uint32_t), n->stream));
     HIP_TRY(launch_reverse(false, n->row_ptr, n->col, net.neuron_begin, net.neuron_begin + net.n_neurons, n->deg,
                            nullptr, nullptr, nullptr, n->stream));
-    return read_prefix(n, net.neuron_begin, net.n_neurons, prefix, nullptr);
+    int rc = read_prefix(n, net.neuron_begin, net.n_neurons, prefix, &net.max_rev);
+    // reverse rows are walked in kChunk-synapse chunks (one grid row each)
+    n->rev_chunks = std::max<uint32_t>(n->rev_chunks, (net.max_rev + kChunk - 1) / kChunk);
+    return rc;
 }
 
 // Writes the reverse index of a network at net.rev_begin.
@@ -702,7 +708,7 @@ int alloc_neurons(naer_net* n, bool any_poisson)
 // Buffers of the bucketed delivery: R = M + 1 batches of events in flight
 // (a batch's events are read during the M batches that follow it), e_cap
 // events each, sized for a mean rate of kBucketRateHz over all synapses.
-constexpr double kBucketRateHz = 80.0;
+constexpr double kBucketRateHz = 200.0;
 constexpr uint32_t kMaxBuckets = 16384;
 
 int alloc_buckets(naer_net* n)
@@ -718,7 +724,11 @@ int alloc_buckets(naer_net* n)
         return NAER_OK;
     n->R = n->M + 1;
     const double expect = (double)n->S_cap * n->B * n->dt_ms * 1e-3 * kBucketRateHz;
-    n->e_cap = (uint64_t)std::min(std::max(expect, (double)(1u << 20)), (double)0x7FFFFFFFu);
+    // the R + 1 event buffers take at most a quarter of the free device memory
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    const double fit = (double)free_b / 4.0 / (8.0 * (n->R + 1));
+    n->e_cap = (uint64_t)std::min(std::min(std::max(expect, (double)(1u << 20)), fit), (double)0x7FFFFFFFu);
     // NAER_BUCKET_EVENTS overrides the capacity (tests use a small one to
     // drive batches through the ring fallback)
     if (const char* env = std::getenv("NAER_BUCKET_EVENTS")) {
@@ -734,7 +744,8 @@ int alloc_buckets(naer_net* n)
     if ((rc = n->alloc(&n->ev, (size_t)n->R * n->e_cap)) || (rc = n->alloc(&n->stage, n->e_cap)) ||
         (rc = n->alloc(&n->bs, (size_t)n->R * (n->NBK + 1))) || (rc = n->alloc(&n->flags, n->R)) ||
         (rc = n->alloc(&n->hist, (size_t)n->NB * n->NBK)) ||
-        (rc = n->alloc(&n->hist_c, (size_t)n->NB * (n->NBK >> n->gt_shift))) || (rc = n->alloc(&n->totals, n->NBK)))
+        (rc = n->alloc(&n->hist_c, (size_t)n->NB * (n->NBK >> n->gt_shift))) || (rc = n->alloc(&n->totals, n->NBK)) ||
+        (rc = n->alloc(&n->gcur, n->NBK)))
         return rc;
     HIP_TRY(hipMemsetAsync(n->flags, 0, n->R * sizeof(uint32_t), n->stream));
     n->bucketed = true;
@@ -1148,7 +1159,7 @@ static hipError_t do_scatter(naer_net* n, void* arg)
 static hipError_t do_potentiate(naer_net* n, void* arg)
 {
     PotentiateArgs* a = (PotentiateArgs*)arg;
-    return launch_potentiate(*a, 256, n->stream);
+    return launch_potentiate(*a, n->scatter_grid, std::max<uint32_t>(1, n->rev_chunks), n->stream);
 }
 
 struct BucketStep {
@@ -1222,6 +1233,9 @@ static int enqueue_steps(naer_net* n, uint32_t n_steps, bool timed)
     ba.G = n->G;
     ba.NB = n->NB;
     ba.C = std::max<uint32_t>(1, (n->max_row + kBucketChunk - 1) / kBucketChunk);
+    ba.gcur = n->gcur;
+    // about 2048 refine blocks over the group buckets
+    ba.RS = std::max<uint32_t>(1, std::min<uint32_t>(64, 2048 / std::max<uint32_t>(1, n->NBK >> n->gt_shift)));
     ba.is_last = 1;
     PotentiateArgs pa;
     std::memset(&pa, 0, sizeof pa);

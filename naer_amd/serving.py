@@ -31,6 +31,7 @@ DELAY_STEPS = 15       # 1.5 ms at dt = 0.1 ms
 MAX_ROW = 8192
 BYTES_PER_SYNAPSE = 8  # uint32 target/delay word + fp32 weight
 BYTES_PER_PLASTIC = 8  # reverse index: position + source
+MAX_PLASTIC_ARENA_SYNAPSES = (1 << 32) - (1 << 20)
 
 
 @dataclass
@@ -114,6 +115,8 @@ def split_budget(budget_bytes, mode=_abi.MODE_DETERMINISTIC, stdp_fraction=0.3, 
         b = budget_bytes * frac
         per_syn = sb + neuron_bytes(mode, plastic) / synapses_per_neuron
         syn = int(b / per_syn)
+        if plastic:  # the reverse index of a plastic arena holds 32-bit synapse positions
+            syn = min(syn, MAX_PLASTIC_ARENA_SYNAPSES)
         caps.append(Capacity(neurons=max(256, syn // synapses_per_neuron), synapses=syn,
                              plastic=int(0.64 * syn) if plastic else 0))
     return caps[0], caps[1]
