@@ -738,7 +738,7 @@ int alloc_buckets(naer_net* n)
     }
     n->G = std::max<uint32_t>(1, (n->n_seg + 255) / 256);
     n->NB = (n->n_seg + n->G - 1) / n->G;
-    if (n->M > kMaxArrivalBatches || n->G > kMaxSegPerBlock)
+    if (n->M > kMaxArrivalBatches || n->G > kMaxSegPerBlock || (n->NBK >> n->gt_shift) > 1024)
         return NAER_OK;
     int rc;
     if ((rc = n->alloc(&n->ev, (size_t)n->R * n->e_cap)) || (rc = n->alloc(&n->stage, n->e_cap)) ||
