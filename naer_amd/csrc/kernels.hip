@@ -428,8 +428,8 @@ __global__ __launch_bounds__(TW, (BUCKET && !DET && !POIS && TW == 1024) ? 8 : 1
 }
 
 // ---------------------------------------------------------------------------
-// Scatter: grid (x: waves striding over the update blocks' list segments,
-// y: 256-synapse chunk of the row).  Each lane loads 4 synapses before
+// Scatter: grid (x: the update blocks' list segments, one block each, its
+// waves taking the segment's spikes in turn; y: 256-synapse chunk of the row).  Each lane loads 4 synapses before
 // issuing its 4 accumulations.  The last kernel of a batch advances the
 // device clock.
 // ---------------------------------------------------------------------------
@@ -439,10 +439,6 @@ __device__ __forceinline__ void batch_epilogue(DevState* st, uint32_t B)
     st->batch += 1;
 }
 
-__device__ __forceinline__ uint32_t wave_global()
-{
-    return blockIdx.x * (kBlock / kWave) + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-}
 
 template <bool DET, bool PLASTIC>
 __global__ __launch_bounds__(kBlock) void k_scatter(const ScatterArgs a)
@@ -461,12 +457,12 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const ScatterArgs a)
 
     RingT* ring = (RingT*)a.ring;
     const uint32_t lane = lane_id();
-    const uint32_t nw = gridDim.x * (kBlock / kWave);
     const unsigned long long chunk_off = (unsigned long long)blockIdx.y * kChunk;
-    for (uint32_t g = wave_global(); g < a.n_seg; g += nw) {
+    {
+        const uint32_t g = blockIdx.x;  // one block per list segment, its waves over the spikes
         const uint32_t cnt = counts[g];
         const naer_spike* seg = list + (size_t)g * a.seg_cap;
-        for (uint32_t i = 0; i < cnt; ++i) {
+        for (uint32_t i = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave); i < cnt; i += kBlock / kWave) {
             const naer_spike sp = seg[i];
             const uint32_t s = sp.neuron;
             const unsigned long long re = a.row_ptr[s + 1];
@@ -519,12 +515,12 @@ __global__ __launch_bounds__(kBlock) void k_potentiate(const PotentiateArgs a)
     if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0)
         batch_epilogue(st, a.B);
     const uint32_t lane = lane_id();
-    const uint32_t nw = gridDim.x * (kBlock / kWave);
     const unsigned long long chunk_off = (unsigned long long)blockIdx.y * kChunk;
-    for (uint32_t g = wave_global(); g < a.n_seg; g += nw) {
+    {
+        const uint32_t g = blockIdx.x;  // one block per list segment, its waves over the spikes
         const uint32_t cnt = counts[g];
         const naer_spike* seg = list + (size_t)g * a.seg_cap;
-        for (uint32_t i = 0; i < cnt; ++i) {
+        for (uint32_t i = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave); i < cnt; i += kBlock / kWave) {
             const uint32_t post = seg[i].neuron;
             const unsigned long long re = a.rev_ptr[post + 1];
             const unsigned long long rb = a.rev_ptr[post] + chunk_off;

@@ -1153,13 +1153,13 @@ static hipError_t do_update(naer_net* n, void* arg)
 static hipError_t do_scatter(naer_net* n, void* arg)
 {
     ScatterArgs* a = (ScatterArgs*)arg;
-    return launch_scatter(*a, n->det, n->plastic, n->scatter_grid, n->scatter_chunks, n->stream);
+    return launch_scatter(*a, n->det, n->plastic, n->n_seg, n->scatter_chunks, n->stream);
 }
 
 static hipError_t do_potentiate(naer_net* n, void* arg)
 {
     PotentiateArgs* a = (PotentiateArgs*)arg;
-    return launch_potentiate(*a, n->scatter_grid, std::max<uint32_t>(1, n->rev_chunks), n->stream);
+    return launch_potentiate(*a, n->n_seg, std::max<uint32_t>(1, n->rev_chunks), n->stream);
 }
 
 struct BucketStep {
