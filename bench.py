@@ -93,7 +93,7 @@ def main():
     ap.add_argument("--indegree", type=int, default=2000)
     ap.add_argument("--mode", choices=["fast", "det"], default="fast")
     ap.add_argument("--cpu-n", type=int, default=50_000)
-    ap.add_argument("--cpu-steps", type=int, default=2_000)
+    ap.add_argument("--cpu-steps", type=int, default=30_000)  # ~10-15 s of oracle work
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--copy-probe", action="store_true")
