@@ -104,7 +104,7 @@ __device__ __forceinline__ void emit(Stage& sg, bool fire, uint32_t neuron, uint
 // during the batch because B <= the minimum delay.
 // ---------------------------------------------------------------------------
 template <bool DET, bool PLASTIC, bool BUCKET, bool POIS, uint32_t TW>
-__global__ __launch_bounds__(TW, (BUCKET && !DET && !POIS && TW == 1024) ? 8 : 1) void k_update(const UpdateArgs a)
+__global__ __launch_bounds__(TW, (BUCKET && !PLASTIC && !POIS && TW == 1024) ? 8 : 1) void k_update(const UpdateArgs a)
 {
     using RingT = typename std::conditional<DET, long long, float>::type;
     // bucketed input accumulates in 32.32 fixed point in both modes: integer

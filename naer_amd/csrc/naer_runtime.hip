@@ -662,6 +662,21 @@ int init_net(naer_net* n, const NetInfo& net)
 }
 
 // Allocation of everything sized by the neuron capacity.
+// Steps per update launch.  The inputs of a batch were all delivered before
+// it starts when B <= the minimum delay; plastic nets interleave delivery and
+// potentiation every step.  Nets with 1024-neuron tiles (N >= 2^18) keep
+// B <= kMaxBatchLargeTile so that the update's [B][1024] 64-bit LDS
+// accumulators of two tiles fit one CU.
+uint32_t batch_steps(bool plastic, uint32_t dmin, uint64_t N)
+{
+    if (plastic)
+        return 1;
+    uint32_t b = std::min<uint32_t>(kMaxBatch, dmin == 0xFFFFFFFFu ? kMaxBatch : dmin);
+    if (N >= (1u << 18))
+        b = std::min(b, kMaxBatchLargeTile);
+    return std::max(b, 1u);
+}
+
 int alloc_neurons(naer_net* n, bool any_poisson)
 {
     const uint32_t N = n->N;
@@ -855,10 +870,7 @@ int naer_pack_create(naer_ctx* ctx, const naer_net_desc* descs, uint32_t n_nets,
     if (rc == NAER_OK) {
         n->N = (uint32_t)N;
         n->D = n->dmax + 1;
-        // Steps per update launch: inputs of a batch were all scattered
-        // before it starts when B <= minimum delay.  Plastic nets interleave
-        // scatter and potentiation every step.
-        n->B = n->plastic ? 1u : std::min<uint32_t>(kMaxBatch, n->dmin == 0xFFFFFFFFu ? kMaxBatch : n->dmin);
+        n->B = batch_steps(n->plastic, n->dmin, n->N);
         n->rec_cap = rec;
         rc = make_stream(n);
     }
@@ -998,7 +1010,7 @@ int naer_arena_create(naer_ctx* ctx, const naer_arena_desc* a, naer_net** out)
         n->dmin = a->delay_min;
         n->dmax = a->delay_max;
         n->D = n->dmax + 1;
-        n->B = n->plastic ? 1u : std::min<uint32_t>(kMaxBatch, n->dmin);
+        n->B = batch_steps(n->plastic, n->dmin, n->N);
         n->max_row = a->max_row;
         n->rec_cap = a->record_capacity;
         n->neuron_alloc.reset(n->N);
