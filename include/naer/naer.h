@@ -306,6 +306,22 @@ int naer_net_set_profiling(naer_net* net, int enable);
  * the net. */
 int naer_net_read_spikes(naer_net* net, naer_spike* out, uint64_t cap, uint64_t* n_out);
 
+/* Spike raster files.  No counterpart in the reference (it records nothing);
+ * the format:
+ *   header (40 bytes, little endian): "NAERRST1", uint32 version (1),
+ *   uint32 bytes per record (8), float64 dt_ms, uint64 neurons,
+ *   uint64 records (written by naer_net_raster_close);
+ *   records: {uint32 step, uint32 neuron}, ordered by (step, neuron).
+ * Steps and neuron ids are those of the net (pack steps and pack ids for a
+ * pack or an arena).  naer_net_raster_drain appends the spikes recorded since
+ * the previous drain (the same spikes naer_net_read_spikes would return)
+ * through a pinned host buffer; drains happen between steps, so the file stays
+ * ordered.  Each synchronises the net. */
+#define NAER_RASTER_HEADER_BYTES 40
+int naer_net_raster_open(naer_net* net, const char* path);
+int naer_net_raster_drain(naer_net* net, uint64_t* n_written);
+int naer_net_raster_close(naer_net* net, uint64_t* n_total);
+
 /* Copy neuron state.  Any pointer may be NULL.  Arrays have n_neurons
  * entries; POISSON neurons report v = 0 and refractory = 0.  pre/post are the
  * STDP traces (zeros when the net has no plastic connection). */

@@ -111,6 +111,9 @@ _SIGS = {
     "naer_ctx_destroy": [ctypes.c_void_p],
     "naer_net_create": [ctypes.c_void_p, ctypes.POINTER(NetDesc), ctypes.POINTER(ctypes.c_void_p)],
     "naer_pack_create": [ctypes.c_void_p, ctypes.POINTER(NetDesc), u32, ctypes.POINTER(ctypes.c_void_p)],
+    "naer_net_raster_open": [ctypes.c_void_p, ctypes.c_char_p],
+    "naer_net_raster_drain": [ctypes.c_void_p, ctypes.c_void_p],
+    "naer_net_raster_close": [ctypes.c_void_p, ctypes.c_void_p],
     "naer_pack_layout": [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                          ctypes.c_void_p],
     "naer_arena_create": [ctypes.c_void_p, ctypes.POINTER(ArenaDesc), ctypes.POINTER(ctypes.c_void_p)],

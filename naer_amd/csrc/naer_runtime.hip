@@ -188,6 +188,11 @@ struct naer_net {
 
     uint32_t scatter_chunks = 1, scatter_grid = 1;
     uint32_t rev_chunks = 1;  // potentiation grid rows: longest reverse row / kChunk
+
+    // spike raster file (naer_net_raster_*)
+    FILE* raster = nullptr;
+    naer_spike* raster_stage = nullptr;  // pinned host copy of the record buffer
+    uint64_t raster_records = 0;
     uint64_t steps_enqueued = 0;
 
     // update tiles and bucketed delivery
@@ -223,6 +228,10 @@ struct naer_net {
             (void)hipEventDestroy(e);
         for (void* p : allocs)
             (void)hipFree(p);
+        if (raster)
+            std::fclose(raster);  // unfinished: the header keeps 0 records
+        if (raster_stage)
+            (void)hipHostFree(raster_stage);
         if (stream)
             (void)hipStreamDestroy(stream);
     }
@@ -1431,6 +1440,88 @@ int naer_net_read_spikes(naer_net* n, naer_spike* out, uint64_t cap, uint64_t* n
     const unsigned long long zero = 0;
     HIP_TRY(hipMemcpy(&n->st->rec_n, &zero, sizeof zero, hipMemcpyHostToDevice));
     *n_out = k;
+    return NAER_OK;
+}
+
+int naer_net_raster_open(naer_net* n, const char* path)
+{
+    if (!n || !path)
+        return fail(NAER_ERR_INVALID, "NULL argument");
+    if (!n->built)
+        return fail(NAER_ERR_STATE, "net not built");
+    if (n->raster)
+        return fail(NAER_ERR_STATE, "a raster file is already open");
+    if (n->rec_cap == 0)
+        return fail(NAER_ERR_STATE, "the net records no spikes (record_capacity 0)");
+    if (!n->raster_stage)
+        HIP_TRY(hipHostMalloc((void**)&n->raster_stage, n->rec_cap * sizeof(naer_spike), hipHostMallocDefault));
+    FILE* f = std::fopen(path, "wb");
+    if (!f)
+        return fail(NAER_ERR_INVALID, "cannot open %s for writing", path);
+    unsigned char h[NAER_RASTER_HEADER_BYTES] = {};
+    std::memcpy(h, "NAERRST1", 8);
+    const uint32_t version = 1, rec_bytes = 8;
+    const double dt = n->dt_ms;
+    const uint64_t neurons = n->N, records = 0;
+    std::memcpy(h + 8, &version, 4);
+    std::memcpy(h + 12, &rec_bytes, 4);
+    std::memcpy(h + 16, &dt, 8);
+    std::memcpy(h + 24, &neurons, 8);
+    std::memcpy(h + 32, &records, 8);
+    if (std::fwrite(h, 1, sizeof h, f) != sizeof h) {
+        std::fclose(f);
+        return fail(NAER_ERR_INVALID, "cannot write %s", path);
+    }
+    n->raster = f;
+    n->raster_records = 0;
+    return NAER_OK;
+}
+
+int naer_net_raster_drain(naer_net* n, uint64_t* n_written)
+{
+    if (!n)
+        return fail(NAER_ERR_INVALID, "net is NULL");
+    if (!n->raster)
+        return fail(NAER_ERR_STATE, "no raster file is open");
+    uint64_t k = 0;
+    int rc = naer_net_read_spikes(n, n->raster_stage, n->rec_cap, &k);
+    if (rc)
+        return rc;
+    std::vector<uint64_t> keys(k);
+    for (uint64_t i = 0; i < k; ++i)
+        keys[i] = (uint64_t)n->raster_stage[i].step << 32 | n->raster_stage[i].neuron;
+    std::sort(keys.begin(), keys.end());
+    std::vector<uint32_t> rec(2 * k);
+    for (uint64_t i = 0; i < k; ++i) {
+        rec[2 * i] = (uint32_t)(keys[i] >> 32);
+        rec[2 * i + 1] = (uint32_t)keys[i];
+    }
+    if (k && std::fwrite(rec.data(), 8, k, n->raster) != k)
+        return fail(NAER_ERR_INVALID, "raster write failed");
+    n->raster_records += k;
+    if (n_written)
+        *n_written = k;
+    return NAER_OK;
+}
+
+int naer_net_raster_close(naer_net* n, uint64_t* n_total)
+{
+    if (!n)
+        return fail(NAER_ERR_INVALID, "net is NULL");
+    if (!n->raster)
+        return fail(NAER_ERR_STATE, "no raster file is open");
+    int rc = naer_net_raster_drain(n, nullptr);
+    FILE* f = n->raster;
+    n->raster = nullptr;
+    bool ok = rc == NAER_OK && std::fseek(f, 32, SEEK_SET) == 0 &&
+              std::fwrite(&n->raster_records, 8, 1, f) == 1;
+    ok = (std::fclose(f) == 0) && ok;
+    if (rc)
+        return rc;
+    if (!ok)
+        return fail(NAER_ERR_INVALID, "raster file could not be finished");
+    if (n_total)
+        *n_total = n->raster_records;
     return NAER_OK;
 }
 

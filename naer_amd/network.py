@@ -289,6 +289,24 @@ class Network:
         out = out[: n.value]
         return np.sort(out, order=["step", "neuron", "time_us"])
 
+    # --- spike raster files (naer_net_raster_*) ---------------------------
+    def raster_open(self, path):
+        """Start a raster file; needs record_capacity > 0 (naer_amd/raster.py
+        reads it)."""
+        check(_abi.lib().naer_net_raster_open(self._h(), str(path).encode()))
+
+    def raster_drain(self):
+        """Append the spikes recorded since the last drain; returns their number."""
+        n = ctypes.c_uint64()
+        check(_abi.lib().naer_net_raster_drain(self._h(), ctypes.byref(n)))
+        return n.value
+
+    def raster_close(self):
+        """Drain and finish the file; returns the total number of records."""
+        n = ctypes.c_uint64()
+        check(_abi.lib().naer_net_raster_close(self._h(), ctypes.byref(n)))
+        return n.value
+
     def state(self):
         N = self.getNeurons()
         v = np.zeros(N, np.float32)

@@ -122,3 +122,4 @@ def test_full_size_events_conserved_and_deterministic():
         rasters.append(sp)
         net.close()
     assert first_divergence(rasters[0], rasters[1]) is None
+
