@@ -318,6 +318,17 @@ int naer_net_read_spikes(naer_net* net, naer_spike* out, uint64_t cap, uint64_t*
  * through a pinned host buffer; drains happen between steps, so the file stays
  * ordered.  Each synchronises the net. */
 #define NAER_RASTER_HEADER_BYTES 40
+
+/* Checkpoint and resume (no counterpart in the reference).  naer_net_save
+ * writes the dynamic state of a built net at a step boundary (potentials,
+ * refractory counters, traces, Poisson source state, plastic weights, the
+ * delay ring, the in-flight bucketed events, the clock, counters and
+ * undrained recorded spikes); naer_net_load restores it into a net built from
+ * the same description, which then continues bit for bit as the saved one
+ * would have (deterministic mode).  Arenas are not checkpointed.  Both
+ * synchronise the net. */
+int naer_net_save(naer_net* net, const char* path);
+int naer_net_load(naer_net* net, const char* path);
 int naer_net_raster_open(naer_net* net, const char* path);
 int naer_net_raster_drain(naer_net* net, uint64_t* n_written);
 int naer_net_raster_close(naer_net* net, uint64_t* n_total);

@@ -111,6 +111,8 @@ _SIGS = {
     "naer_ctx_destroy": [ctypes.c_void_p],
     "naer_net_create": [ctypes.c_void_p, ctypes.POINTER(NetDesc), ctypes.POINTER(ctypes.c_void_p)],
     "naer_pack_create": [ctypes.c_void_p, ctypes.POINTER(NetDesc), u32, ctypes.POINTER(ctypes.c_void_p)],
+    "naer_net_save": [ctypes.c_void_p, ctypes.c_char_p],
+    "naer_net_load": [ctypes.c_void_p, ctypes.c_char_p],
     "naer_net_raster_open": [ctypes.c_void_p, ctypes.c_char_p],
     "naer_net_raster_drain": [ctypes.c_void_p, ctypes.c_void_p],
     "naer_net_raster_close": [ctypes.c_void_p, ctypes.c_void_p],

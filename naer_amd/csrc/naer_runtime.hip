@@ -1525,6 +1525,164 @@ int naer_net_raster_close(naer_net* n, uint64_t* n_total)
     return NAER_OK;
 }
 
+// --- checkpoint / resume ---------------------------------------------------
+namespace {
+
+constexpr char kCheckpointMagic[8] = { 'N', 'A', 'E', 'R', 'C', 'K', 'P', '1' };
+
+// Structure of a net that a checkpoint must match to be loaded.
+struct CheckpointShape {
+    uint64_t N, S_cap, R_cap, D, B, R, NBK, e_cap, n_seg, rec_cap;
+    uint32_t det, plastic, poisson, bucketed;
+};
+
+CheckpointShape shape_of(const naer_net* n)
+{
+    CheckpointShape c;
+    std::memset(&c, 0, sizeof c);
+    c.N = n->N;
+    c.S_cap = n->S_cap;
+    c.R_cap = n->R_cap;
+    c.D = n->D;
+    c.B = n->B;
+    c.R = n->R;
+    c.NBK = n->NBK;
+    c.e_cap = n->e_cap;
+    c.n_seg = n->n_seg;
+    c.rec_cap = n->rec_cap;
+    c.det = n->det;
+    c.plastic = n->plastic;
+    c.poisson = n->poisson_ok;
+    c.bucketed = n->bucketed;
+    return c;
+}
+
+// Device <-> file copies through a bounded host buffer.
+struct CheckpointIO {
+    FILE* f;
+    bool writing;
+    std::vector<char> buf = std::vector<char>(64u << 20);
+    int copy(void* dev, uint64_t bytes)
+    {
+        for (uint64_t off = 0; off < bytes;) {
+            const size_t k = (size_t)std::min<uint64_t>(buf.size(), bytes - off);
+            if (writing) {
+                HIP_TRY(hipMemcpy(buf.data(), (char*)dev + off, k, hipMemcpyDeviceToHost));
+                if (std::fwrite(buf.data(), 1, k, f) != k)
+                    return fail(NAER_ERR_INVALID, "checkpoint write failed");
+            } else {
+                if (std::fread(buf.data(), 1, k, f) != k)
+                    return fail(NAER_ERR_INVALID, "checkpoint truncated");
+                HIP_TRY(hipMemcpy((char*)dev + off, buf.data(), k, hipMemcpyHostToDevice));
+            }
+            off += k;
+        }
+        return NAER_OK;
+    }
+    int scalar(void* host, size_t bytes)
+    {
+        const size_t k = writing ? std::fwrite(host, 1, bytes, f) : std::fread(host, 1, bytes, f);
+        return k == bytes ? NAER_OK : fail(NAER_ERR_INVALID, "checkpoint %s failed", writing ? "write" : "read");
+    }
+};
+
+// The dynamic state in a fixed order; used for both directions.
+int checkpoint_body(naer_net* n, CheckpointIO& io)
+{
+    const uint64_t N = n->N;
+    int rc;
+    if ((rc = io.copy(n->V, N * 4)) || (rc = io.copy(n->ref, N * 4)) || (rc = io.copy(n->last, N * 4)))
+        return rc;
+    if (n->plastic && ((rc = io.copy(n->xh, N * 4)) || (rc = io.copy(n->yh, N * 4)) ||
+                       (rc = io.copy(n->w, n->S_cap * 4))))
+        return rc;
+    if (n->pnext && ((rc = io.copy(n->pnext, N * 4)) || (rc = io.copy(n->pcnt, N * 4))))
+        return rc;
+    if ((rc = io.copy(n->ring, (uint64_t)n->D * N * (n->det ? 8 : 4))) || (rc = io.copy(n->st, sizeof(DevState))) ||
+        (rc = io.copy(n->spk_part, (uint64_t)n->n_seg * 8)) || (rc = io.copy(n->ev_part, (uint64_t)n->n_seg * 8)))
+        return rc;
+    if ((rc = io.scalar(&n->steps_enqueued, 8)) || (rc = io.scalar(&n->dropped, 8)))
+        return rc;
+    // undrained recorded spikes
+    DevState s;
+    HIP_TRY(hipMemcpy(&s, n->st, sizeof s, hipMemcpyDeviceToHost));
+    if ((rc = io.copy(n->rec, std::min<uint64_t>(s.rec_n, n->rec_cap) * sizeof(naer_spike))))
+        return rc;
+    if (n->bucketed) {
+        // flags and starts of every buffer, then the events of the batches in flight
+        if ((rc = io.copy(n->flags, (uint64_t)n->R * 4)) || (rc = io.copy(n->bs, (uint64_t)n->R * (n->NBK + 1) * 4)))
+            return rc;
+        std::vector<uint32_t> flags(n->R), used(n->R);
+        HIP_TRY(hipMemcpy(flags.data(), n->flags, n->R * 4, hipMemcpyDeviceToHost));
+        for (uint32_t r = 0; r < n->R; ++r) {
+            used[r] = 0;
+            if (flags[r])
+                HIP_TRY(hipMemcpy(&used[r], n->bs + (size_t)r * (n->NBK + 1) + n->NBK, 4, hipMemcpyDeviceToHost));
+            if ((rc = io.copy(n->ev + (size_t)r * n->e_cap, (uint64_t)used[r] * sizeof(uint2))))
+                return rc;
+        }
+    }
+    return NAER_OK;
+}
+
+}  // namespace
+
+int naer_net_save(naer_net* n, const char* path)
+{
+    if (!n || !path)
+        return fail(NAER_ERR_INVALID, "NULL argument");
+    if (!n->built)
+        return fail(NAER_ERR_STATE, "net not built");
+    if (n->arena)
+        return fail(NAER_ERR_STATE, "arenas are not checkpointed");
+    int rc = naer_net_sync(n);
+    if (rc)
+        return rc;
+    FILE* f = std::fopen(path, "wb");
+    if (!f)
+        return fail(NAER_ERR_INVALID, "cannot open %s for writing", path);
+    CheckpointIO io{ f, true };
+    CheckpointShape c = shape_of(n);
+    rc = io.scalar((void*)kCheckpointMagic, 8);
+    if (!rc)
+        rc = io.scalar(&c, sizeof c);
+    if (!rc)
+        rc = checkpoint_body(n, io);
+    if (std::fclose(f) != 0 && !rc)
+        rc = fail(NAER_ERR_INVALID, "cannot finish %s", path);
+    return rc;
+}
+
+int naer_net_load(naer_net* n, const char* path)
+{
+    if (!n || !path)
+        return fail(NAER_ERR_INVALID, "NULL argument");
+    if (!n->built)
+        return fail(NAER_ERR_STATE, "net not built");
+    if (n->arena)
+        return fail(NAER_ERR_STATE, "arenas are not checkpointed");
+    int rc = naer_net_sync(n);
+    if (rc)
+        return rc;
+    FILE* f = std::fopen(path, "rb");
+    if (!f)
+        return fail(NAER_ERR_INVALID, "cannot open %s", path);
+    CheckpointIO io{ f, false };
+    char magic[8];
+    CheckpointShape c, mine = shape_of(n);
+    rc = io.scalar(magic, 8);
+    if (!rc && std::memcmp(magic, kCheckpointMagic, 8) != 0)
+        rc = fail(NAER_ERR_INVALID, "%s is not a checkpoint", path);
+    if (!rc)
+        rc = io.scalar(&c, sizeof c);
+    if (!rc && std::memcmp(&c, &mine, sizeof c) != 0)
+        rc = fail(NAER_ERR_INVALID, "the checkpoint was written by a net of another shape");
+    if (!rc && (rc = n->drop_graph()) == NAER_OK)
+        rc = checkpoint_body(n, io);
+    std::fclose(f);
+    return rc;
+}
+
 int naer_net_read_state(naer_net* n, float* v, int32_t* refr, float* pre, float* post)
 {
     if (!n)

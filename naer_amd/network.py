@@ -289,6 +289,15 @@ class Network:
         out = out[: n.value]
         return np.sort(out, order=["step", "neuron", "time_us"])
 
+    # --- checkpoint / resume (naer_net_save / naer_net_load) ----------------
+    def save(self, path):
+        """Write the dynamic state (at the current step) to `path`."""
+        check(_abi.lib().naer_net_save(self._h(), str(path).encode()))
+
+    def load(self, path):
+        """Restore a state saved by a net built from the same description."""
+        check(_abi.lib().naer_net_load(self._h(), str(path).encode()))
+
     # --- spike raster files (naer_net_raster_*) ---------------------------
     def raster_open(self, path):
         """Start a raster file; needs record_capacity > 0 (naer_amd/raster.py

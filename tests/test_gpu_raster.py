@@ -9,6 +9,7 @@ from test_gpu_bucketed import multi_delay_net
 
 pytestmark = pytest.mark.gpu
 
+
 def test_raster_file_equals_oracle(tmp_path):
     """Spikes written through naer_net_raster_* (drains between runs) equal
     the oracle's raster, in (step, neuron) order."""
