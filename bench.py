@@ -98,6 +98,7 @@ def main():
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=2_000)
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the simulation step of one network is not "
@@ -117,7 +118,12 @@ def main():
     print(f"# built N={net.getNeurons()} S={net.getSynapses()} in {build_s:.1f} s", file=sys.stderr,
           flush=True)
 
-    net.run(args.warmup)
+    # warm-up: the autotuner picks the delivery schedule on real steps, then
+    # the rest of the warm-up steps run with it
+    schedule = net.autotune(2) if args.warmup > 0 and not args.no_autotune else net.schedule()
+    done = int(net.stats().step)
+    if args.warmup > done:
+        net.run(args.warmup - done)
     net.finish()
     s0 = net.stats()
     t0 = time.perf_counter()
@@ -207,6 +213,7 @@ def main():
         "rate_hz": round(spikes / N / bio_s, 3),
         "events": int(events),
         "build_s": round(build_s, 2),
+        "schedule": schedule,
         "roofline": roofline,
         "kernels": kernels,
         "cpu_baseline": cpu,

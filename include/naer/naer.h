@@ -319,6 +319,23 @@ int naer_net_read_spikes(naer_net* net, naer_spike* out, uint64_t cap, uint64_t*
  * ordered.  Each synchronises the net. */
 #define NAER_RASTER_HEADER_BYTES 40
 
+/* Schedule of the bucketed delivery (no counterpart in the reference): how
+ * the histogram/placement kernels cut the batch's spike rows into wave tasks
+ * and blocks, and how many blocks refine each group bucket.  Any schedule
+ * gives the same results.  naer_net_autotune tries candidates on the running
+ * simulation (each for `batches` aligned batches of real steps, timed with HIP
+ * events, scored per synaptic event) and keeps the fastest. */
+typedef struct naer_schedule {
+    uint32_t bucketed;           /* out: 1 if the net delivers through buckets */
+    uint32_t task_synapses;      /* synapses of a row per wave task: 256 or 512 */
+    uint32_t segments_per_block; /* spike-list segments per histogram/placement block, 1..64 */
+    uint32_t refine_split;       /* refine blocks per group bucket, 1..64 */
+    double delivery_us;          /* out: delivery time per batch measured by the autotuner */
+} naer_schedule;
+int naer_net_get_schedule(naer_net* net, naer_schedule* out);
+int naer_net_set_schedule(naer_net* net, const naer_schedule* s);
+int naer_net_autotune(naer_net* net, uint32_t batches, naer_schedule* chosen);
+
 /* Checkpoint and resume (no counterpart in the reference).  naer_net_save
  * writes the dynamic state of a built net at a step boundary (potentials,
  * refractory counters, traces, Poisson source state, plastic weights, the

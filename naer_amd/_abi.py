@@ -84,6 +84,15 @@ class Spike(ctypes.Structure):
     _fields_ = [("step", u32), ("neuron", u32), ("time_us", u32), ("_pad", u32)]
 
 
+class Schedule(ctypes.Structure):
+    _fields_ = [("bucketed", ctypes.c_uint32), ("task_synapses", ctypes.c_uint32),
+                ("segments_per_block", ctypes.c_uint32), ("refine_split", ctypes.c_uint32),
+                ("delivery_us", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class Stats(ctypes.Structure):
     _fields_ = [
         ("n_neurons", u64), ("n_synapses", u64), ("n_plastic", u64), ("step", u64),
@@ -111,6 +120,9 @@ _SIGS = {
     "naer_ctx_destroy": [ctypes.c_void_p],
     "naer_net_create": [ctypes.c_void_p, ctypes.POINTER(NetDesc), ctypes.POINTER(ctypes.c_void_p)],
     "naer_pack_create": [ctypes.c_void_p, ctypes.POINTER(NetDesc), u32, ctypes.POINTER(ctypes.c_void_p)],
+    "naer_net_get_schedule": [ctypes.c_void_p, ctypes.c_void_p],
+    "naer_net_set_schedule": [ctypes.c_void_p, ctypes.c_void_p],
+    "naer_net_autotune": [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p],
     "naer_net_save": [ctypes.c_void_p, ctypes.c_char_p],
     "naer_net_load": [ctypes.c_void_p, ctypes.c_char_p],
     "naer_net_raster_open": [ctypes.c_void_p, ctypes.c_char_p],
@@ -161,6 +173,23 @@ def lib():
         L.naer_version.argtypes = []
         _lib = L
     return _lib
+
+
+def get_schedule(handle):
+    s = Schedule()
+    check(lib().naer_net_get_schedule(handle, ctypes.byref(s)))
+    return s.as_dict()
+
+
+def set_schedule(handle, task_synapses, segments_per_block, refine_split):
+    s = Schedule(0, int(task_synapses), int(segments_per_block), int(refine_split), 0.0)
+    check(lib().naer_net_set_schedule(handle, ctypes.byref(s)))
+
+
+def autotune(handle, batches=2):
+    s = Schedule()
+    check(lib().naer_net_autotune(handle, int(batches), ctypes.byref(s)))
+    return s.as_dict()
 
 
 def check(rc):

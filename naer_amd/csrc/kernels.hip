@@ -556,7 +556,8 @@ __global__ __launch_bounds__(kBlock) void k_potentiate(const PotentiateArgs a)
 // ---------------------------------------------------------------------------
 // Bucketed delivery (see BucketArgs): histogram, column scan, bucket starts,
 // placement.  Block b walks the spikes of list segments [b*G, (b+1)*G) as
-// tasks t = i*C + c (spike i, kBucketChunk-synapse chunk c of its row); wave w takes
+// tasks t = i*C + c (spike i, (64 U)-synapse chunk c of its row; U = the
+// schedule's synapses per lane, 4 or 8); wave w takes
 // tasks w, w + waves, ...  Both passes use the same split, so each block
 // places exactly the events it counted.
 // ---------------------------------------------------------------------------
@@ -592,12 +593,13 @@ struct RowStage {
     uint32_t r[kRowRound];  // spike step - first step of the batch
 };
 
-// Calls f(r, q_begin, q_end) for every non-empty kBucketChunk-synapse chunk of the
+// Calls f(r, q_begin, q_end) for every non-empty (64 U)-synapse chunk of the
 // rows of the block's spikes; f runs on a whole wave (q range wave-uniform).
-template <typename F>
+template <uint32_t U, typename F>
 __device__ __forceinline__ void for_block_chunks(const BucketArgs& a, const naer_spike* list, const uint32_t* s_off,
                                                  unsigned long long t0, RowStage& rs, F f)
 {
+    constexpr uint32_t kTask = kWave * U;  // synapses of a row per task
     const uint32_t G = min(a.G, a.n_seg - blockIdx.x * a.G);
     const uint32_t n_spk = s_off[G];
     const uint32_t nwv = blockDim.x / kWave;
@@ -623,14 +625,15 @@ __device__ __forceinline__ void for_block_chunks(const BucketArgs& a, const naer
         const uint32_t n_tasks = cnt * a.C;
         for (uint32_t t = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave); t < n_tasks; t += nwv) {
             const uint32_t i = t / a.C, c = t - i * a.C;
-            const uint32_t off = c * kBucketChunk, len = rs.len[i];
+            const uint32_t off = c * kTask, len = rs.len[i];
             if (off < len)
-                f(rs.r[i], rs.beg[i] + off, rs.beg[i] + min(len, off + kBucketChunk));
+                f(rs.r[i], rs.beg[i] + off, rs.beg[i] + min(len, off + kTask));
         }
         __syncthreads();
     }
 }
 
+template <uint32_t U>
 __global__ __launch_bounds__(kBucketThreads) void k_bucket_hist(const BucketArgs a)
 {
     extern __shared__ uint32_t s_hist[];
@@ -646,15 +649,15 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_hist(const BucketArgs
     __syncthreads();
     const naer_spike* list = cur ? a.list1 : a.list0;
     const uint32_t lane = lane_id();
-    for_block_chunks(a, list, s_off, t0, rs, [&](uint32_t r, unsigned long long beg, unsigned long long end) {
-        uint32_t c[kBucketUnroll];
+    for_block_chunks<U>(a, list, s_off, t0, rs, [&](uint32_t r, unsigned long long beg, unsigned long long end) {
+        uint32_t c[U];
 #pragma unroll
-        for (uint32_t u = 0; u < kBucketUnroll; ++u) {
+        for (uint32_t u = 0; u < U; ++u) {
             const unsigned long long q = beg + u * kWave + lane;
             c[u] = q < end ? a.col[q] : 0u;
         }
 #pragma unroll
-        for (uint32_t u = 0; u < kBucketUnroll; ++u) {
+        for (uint32_t u = 0; u < U; ++u) {
             const unsigned long long q = beg + u * kWave + lane;
             if (q < end) {
                 const uint32_t v = r + ((c[u] >> 24) & 0x7Fu);
@@ -781,7 +784,7 @@ __global__ __launch_bounds__(1024) void k_bucket_starts(const BucketArgs a)
     }
 }
 
-template <bool DET>
+template <bool DET, uint32_t U>
 __global__ __launch_bounds__(kBucketThreads) void k_bucket_place(const BucketArgs a)
 {
     using RingT = typename std::conditional<DET, unsigned long long, float>::type;
@@ -811,17 +814,17 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_place(const BucketArg
     RingT* ring = (RingT*)a.ring;
     const naer_spike* list = cur ? a.list1 : a.list0;
     const uint32_t lane = lane_id();
-    for_block_chunks(a, list, s_off, t0, rs, [&](uint32_t r, unsigned long long beg, unsigned long long end) {
-        uint32_t c[kBucketUnroll];
-        float w[kBucketUnroll];
+    for_block_chunks<U>(a, list, s_off, t0, rs, [&](uint32_t r, unsigned long long beg, unsigned long long end) {
+        uint32_t c[U];
+        float w[U];
 #pragma unroll
-        for (uint32_t u = 0; u < kBucketUnroll; ++u) {
+        for (uint32_t u = 0; u < U; ++u) {
             const unsigned long long q = beg + u * kWave + lane;
             c[u] = q < end ? a.col[q] : 0u;
             w[u] = q < end ? a.w[q] : 0.0f;
         }
 #pragma unroll
-        for (uint32_t u = 0; u < kBucketUnroll; ++u) {
+        for (uint32_t u = 0; u < U; ++u) {
             const unsigned long long q = beg + u * kWave + lane;
             if (q < end) {
                 const uint32_t d = (c[u] >> 24) & 0x7Fu;
@@ -1170,8 +1173,11 @@ static hipError_t bucket_lds_attributes(const BucketArgs& a)
     const size_t lds = (size_t)a.NBK * sizeof(uint32_t);
     static size_t lds_set = 0;
     if (lds > 32768 && lds > lds_set) {
-        hipError_t e = hipFuncSetAttribute((const void*)k_bucket_hist, hipFuncAttributeMaxDynamicSharedMemorySize,
+        hipError_t e = hipFuncSetAttribute((const void*)k_bucket_hist<4>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                            (int)lds);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void*)k_bucket_hist<8>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds);
         if (e == hipSuccess)
             e = hipFuncSetAttribute((const void*)k_bucket_starts, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)lds);
@@ -1191,7 +1197,10 @@ hipError_t launch_bucket_step(const BucketArgs& a, int step, bool det, hipStream
     const size_t lds_c = (size_t)(a.NBK >> a.gt_shift) * sizeof(uint32_t);
     switch (step) {
     case 0:
-        hipLaunchKernelGGL(k_bucket_hist, dim3(a.NB), dim3(kBucketThreads), lds, s, a);
+        if (a.U == 4)
+            hipLaunchKernelGGL(k_bucket_hist<4>, dim3(a.NB), dim3(kBucketThreads), lds, s, a);
+        else
+            hipLaunchKernelGGL(k_bucket_hist<8>, dim3(a.NB), dim3(kBucketThreads), lds, s, a);
         break;
     case 1:
         hipLaunchKernelGGL(k_bucket_colscan, dim3(cdiv(a.NBK >> a.gt_shift, kScanCols) + cdiv(a.NBK, kScanCols)),
@@ -1199,10 +1208,14 @@ hipError_t launch_bucket_step(const BucketArgs& a, int step, bool det, hipStream
         hipLaunchKernelGGL(k_bucket_starts, dim3(1), dim3(1024), lds, s, a);
         break;
     case 2:
-        if (det)
-            hipLaunchKernelGGL((k_bucket_place<true>), dim3(a.NB), dim3(kBucketThreads), lds_c, s, a);
+        if (det && a.U == 4)
+            hipLaunchKernelGGL((k_bucket_place<true, 4>), dim3(a.NB), dim3(kBucketThreads), lds_c, s, a);
+        else if (det)
+            hipLaunchKernelGGL((k_bucket_place<true, 8>), dim3(a.NB), dim3(kBucketThreads), lds_c, s, a);
+        else if (a.U == 4)
+            hipLaunchKernelGGL((k_bucket_place<false, 4>), dim3(a.NB), dim3(kBucketThreads), lds_c, s, a);
         else
-            hipLaunchKernelGGL((k_bucket_place<false>), dim3(a.NB), dim3(kBucketThreads), lds_c, s, a);
+            hipLaunchKernelGGL((k_bucket_place<false, 8>), dim3(a.NB), dim3(kBucketThreads), lds_c, s, a);
         break;
     case 3:
         hipLaunchKernelGGL(k_bucket_refine, dim3(a.NBK >> a.gt_shift, a.RS), dim3(kRefineThreads), 0, s, a);

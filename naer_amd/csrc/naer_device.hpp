@@ -18,8 +18,7 @@ constexpr uint32_t kWave = 64;         // CDNA wavefront
 constexpr uint32_t kLdsSpikes = 128;   // per-block spike staging in LDS
 constexpr uint32_t kChunkUnroll = 4;   // synapses per lane per scatter task
 constexpr uint32_t kChunk = kWave * kChunkUnroll;
-constexpr uint32_t kBucketUnroll = 8;  // synapses per lane per task of the bucketed passes
-constexpr uint32_t kBucketChunk = kWave * kBucketUnroll;
+constexpr uint32_t kBucketUnroll = 8;  // default synapses per lane per task of the bucketed passes
 constexpr uint32_t kTargetMask = 0x00FFFFFFu;
 constexpr uint32_t kPlasticBit = 0x80000000u;
 constexpr uint32_t kEpochUs = 1500u;   // reference epoch, naer_types.h:4
@@ -211,7 +210,8 @@ struct BucketArgs {
     uint32_t gt_shift;  // tiles per group = 1 << gt_shift = 2^14 >> tw_shift
     uint32_t G;        // list segments per histogram/placement block (<= kMaxSegPerBlock)
     uint32_t NB;       // histogram/placement blocks
-    uint32_t C;        // kBucketChunk-synapse chunks of the longest row
+    uint32_t C;        // (64 U)-synapse tasks of the longest row
+    uint32_t U;        // synapses per lane per task: 4 or 8
     uint32_t RS;       // refine blocks per group bucket
     uint32_t is_last;
 };

@@ -194,12 +194,15 @@ struct naer_net {
     naer_spike* raster_stage = nullptr;  // pinned host copy of the record buffer
     uint64_t raster_records = 0;
     uint64_t steps_enqueued = 0;
+    double tuned_us = 0.0;  // delivery time per batch of the schedule chosen by naer_net_autotune
 
     // update tiles and bucketed delivery
     uint32_t tw_shift = 8;   // update block = tile of 1 << tw_shift neurons
     uint32_t gt_shift = 6;   // tiles per bucket group
     bool bucketed = false;   // full aligned batches are delivered through buckets
     uint32_t R = 0, M = 0, NT = 0, NBK = 0, NB = 0, G = 1;
+    uint32_t U = kBucketUnroll;  // schedule: synapses per lane per task
+    uint32_t RS = 1;             // schedule: refine blocks per group bucket
     uint64_t e_cap = 0;
     uint2* ev = nullptr;
     uint2* stage = nullptr;
@@ -760,15 +763,19 @@ int alloc_buckets(naer_net* n)
         if (v > 0)
             n->e_cap = std::min<unsigned long long>(v, 0x7FFFFFFFull);
     }
+    // default schedule (naer_net_autotune may change it): about 256
+    // histogram/placement blocks, 512-synapse tasks, ~2048 refine blocks
     n->G = std::max<uint32_t>(1, (n->n_seg + 255) / 256);
     n->NB = (n->n_seg + n->G - 1) / n->G;
+    n->U = kBucketUnroll;
+    n->RS = std::max<uint32_t>(1, std::min<uint32_t>(64, 2048 / std::max<uint32_t>(1, n->NBK >> n->gt_shift)));
     if (n->M > kMaxArrivalBatches || n->G > kMaxSegPerBlock || (n->NBK >> n->gt_shift) > 1024)
         return NAER_OK;
     int rc;
     if ((rc = n->alloc(&n->ev, (size_t)n->R * n->e_cap)) || (rc = n->alloc(&n->stage, n->e_cap)) ||
         (rc = n->alloc(&n->bs, (size_t)n->R * (n->NBK + 1))) || (rc = n->alloc(&n->flags, n->R)) ||
-        (rc = n->alloc(&n->hist, (size_t)n->NB * n->NBK)) ||
-        (rc = n->alloc(&n->hist_c, (size_t)n->NB * (n->NBK >> n->gt_shift))) || (rc = n->alloc(&n->totals, n->NBK)) ||
+        (rc = n->alloc(&n->hist, (size_t)n->n_seg * n->NBK)) ||  // rows for any G (G = 1: n_seg blocks)
+        (rc = n->alloc(&n->hist_c, (size_t)n->n_seg * (n->NBK >> n->gt_shift))) || (rc = n->alloc(&n->totals, n->NBK)) ||
         (rc = n->alloc(&n->gcur, n->NBK)))
         return rc;
     HIP_TRY(hipMemsetAsync(n->flags, 0, n->R * sizeof(uint32_t), n->stream));
@@ -1253,10 +1260,11 @@ static int enqueue_steps(naer_net* n, uint32_t n_steps, bool timed)
     ba.gt_shift = n->gt_shift;
     ba.G = n->G;
     ba.NB = n->NB;
-    ba.C = std::max<uint32_t>(1, (n->max_row + kBucketChunk - 1) / kBucketChunk);
+    ba.U = n->U;
+    ba.C = std::max<uint32_t>(1, (n->max_row + kWave * n->U - 1) / (kWave * n->U));
     ba.gcur = n->gcur;
     // about 2048 refine blocks over the group buckets
-    ba.RS = std::max<uint32_t>(1, std::min<uint32_t>(64, 2048 / std::max<uint32_t>(1, n->NBK >> n->gt_shift)));
+    ba.RS = n->RS;
     ba.is_last = 1;
     PotentiateArgs pa;
     std::memset(&pa, 0, sizeof pa);
@@ -1359,6 +1367,115 @@ int naer_net_set_graph(naer_net* n, uint32_t chunk_steps)
         return rc;
     // whole aligned batches per chunk
     n->graph_chunk = chunk_steps ? (chunk_steps + n->B - 1) / n->B * n->B : 0;
+    return NAER_OK;
+}
+
+int naer_net_get_schedule(naer_net* n, naer_schedule* out)
+{
+    if (!n || !out)
+        return fail(NAER_ERR_INVALID, "NULL argument");
+    std::memset(out, 0, sizeof *out);
+    out->bucketed = n->bucketed ? 1u : 0u;
+    out->task_synapses = kWave * n->U;
+    out->segments_per_block = n->G;
+    out->refine_split = n->RS;
+    out->delivery_us = n->tuned_us;
+    return NAER_OK;
+}
+
+int naer_net_set_schedule(naer_net* n, const naer_schedule* s)
+{
+    if (!n || !s)
+        return fail(NAER_ERR_INVALID, "NULL argument");
+    if (s->task_synapses != 4 * kWave && s->task_synapses != 8 * kWave)
+        return fail(NAER_ERR_INVALID, "task_synapses must be %u or %u", 4 * kWave, 8 * kWave);
+    if (s->segments_per_block == 0 || s->segments_per_block > kMaxSegPerBlock)
+        return fail(NAER_ERR_INVALID, "segments_per_block must be in [1, %u]", kMaxSegPerBlock);
+    if (s->refine_split == 0 || s->refine_split > 64)
+        return fail(NAER_ERR_INVALID, "refine_split must be in [1, 64]");
+    int rc = naer_net_sync(n);
+    if (rc || (rc = n->drop_graph()))  // captured launches carry the old shapes
+        return rc;
+    n->U = s->task_synapses / kWave;
+    n->G = s->segments_per_block;
+    n->NB = (n->n_seg + n->G - 1) / n->G;
+    n->RS = s->refine_split;
+    return NAER_OK;
+}
+
+// Tries schedules of the bucketed delivery on the running simulation: each
+// candidate delivers `batches` aligned batches (real steps; in deterministic
+// mode the results do not depend on the schedule) and is scored by its
+// delivery time per synaptic event, measured with HIP events.
+int naer_net_autotune(naer_net* n, uint32_t batches, naer_schedule* chosen)
+{
+    if (!n)
+        return fail(NAER_ERR_INVALID, "net is NULL");
+    if (!n->built)
+        return fail(NAER_ERR_STATE, "net not built");
+    if (!n->bucketed || batches == 0)
+        return chosen ? naer_net_get_schedule(n, chosen) : NAER_OK;
+    int rc = naer_net_sync(n);
+    if (rc || (rc = n->drop_graph()))
+        return rc;
+    const uint32_t graph = n->graph_chunk;
+    n->graph_chunk = 0;
+    // align to a batch boundary
+    const uint32_t mis = (uint32_t)((n->B - n->steps_enqueued % n->B) % n->B);
+    if (mis && (rc = naer_net_step(n, mis)))
+        return rc;
+    const bool prof = n->profiling;
+    n->profiling = true;
+    naer_schedule base;
+    naer_net_get_schedule(n, &base);
+    std::vector<naer_schedule> cand;
+    for (uint32_t task : { 4 * kWave, 8 * kWave })
+        for (uint32_t g : { std::max<uint32_t>(1, base.segments_per_block / 2), base.segments_per_block,
+                            std::min<uint32_t>(kMaxSegPerBlock, base.segments_per_block * 2) })
+            for (uint32_t r : { std::max<uint32_t>(1, base.refine_split / 2), base.refine_split,
+                                std::min<uint32_t>(64, base.refine_split * 2) }) {
+                naer_schedule c = base;
+                c.task_synapses = task;
+                c.segments_per_block = g;
+                c.refine_split = r;
+                bool dup = false;
+                for (auto& x : cand)
+                    dup = dup || (x.task_synapses == task && x.segments_per_block == g && x.refine_split == r);
+                if (!dup)
+                    cand.push_back(c);
+            }
+    double best = 1e300;
+    naer_schedule best_s = base;
+    for (auto& c : cand) {
+        if ((rc = naer_net_set_schedule(n, &c)))
+            break;
+        naer_stats s0, s1;
+        double ms0 = 0, ms1 = 0;
+        if ((rc = naer_net_query(n, &s0)))
+            break;
+        for (int k = NAER_K_BUCKET_HIST; k <= NAER_K_BUCKET_REFINE; ++k)
+            ms0 += n->kernel_ms[k];
+        if ((rc = naer_net_step(n, batches * n->B)) || (rc = naer_net_sync(n)) || (rc = naer_net_query(n, &s1)))
+            break;
+        for (int k = NAER_K_BUCKET_HIST; k <= NAER_K_BUCKET_REFINE; ++k)
+            ms1 += n->kernel_ms[k];
+        const double events = (double)(s1.events - s0.events);
+        const double score = (ms1 - ms0) / std::max(1.0, events);
+        if (score < best) {
+            best = score;
+            best_s = c;
+            best_s.delivery_us = (ms1 - ms0) * 1e3 / batches;
+        }
+    }
+    n->profiling = prof;
+    n->graph_chunk = graph;
+    if (rc)
+        return rc;
+    if ((rc = naer_net_set_schedule(n, &best_s)))
+        return rc;
+    n->tuned_us = best_s.delivery_us;
+    if (chosen)
+        return naer_net_get_schedule(n, chosen);
     return NAER_OK;
 }
 

@@ -248,6 +248,18 @@ class Network:
         check(_abi.lib().naer_net_sync(self._h()))
         self._refresh_timers()
 
+    def schedule(self):
+        """The delivery schedule (naer_schedule) as a dict."""
+        return _abi.get_schedule(self._h())
+
+    def set_schedule(self, task_synapses, segments_per_block, refine_split):
+        _abi.set_schedule(self._h(), task_synapses, segments_per_block, refine_split)
+
+    def autotune(self, batches=2):
+        """Try delivery schedules on the running simulation (real steps) and
+        keep the fastest; returns the chosen schedule."""
+        return _abi.autotune(self._h(), batches)
+
     def set_graph(self, chunk_steps):
         """Replay steps as captured HIP graphs of chunk_steps steps (0 = off)."""
         check(_abi.lib().naer_net_set_graph(self._h(), int(chunk_steps)))

@@ -42,6 +42,14 @@ class Pack:
     def build(self):
         check(_abi.lib().naer_net_build(self._handle))
 
+    def autotune(self, batches=2):
+        """Try delivery schedules on the running pack (real steps) and keep
+        the fastest; returns the chosen schedule."""
+        return _abi.autotune(self._handle, batches)
+
+    def schedule(self):
+        return _abi.get_schedule(self._handle)
+
     def set_graph(self, chunk_steps):
         check(_abi.lib().naer_net_set_graph(self._handle, int(chunk_steps)))
 

@@ -57,7 +57,8 @@ def _progress(msg):
     print("# " + msg, file=sys.stderr, flush=True)
 
 
-def run_packed(specs, steps, mode=_abi.MODE_DETERMINISTIC, chunk=100, record=True, graphs=True, verbose=False):
+def run_packed(specs, steps, mode=_abi.MODE_DETERMINISTIC, chunk=100, record=True, graphs=True, verbose=False,
+               autotune=True):
     groups = [[s for s in specs if not s["plastic"]], [s for s in specs if s["plastic"]]]
     packs = []
     t_build = time.perf_counter()
@@ -76,18 +77,29 @@ def run_packed(specs, steps, mode=_abi.MODE_DETERMINISTIC, chunk=100, record=Tru
         p.sync()
     build_s = time.perf_counter() - t_build
     t0 = time.perf_counter()
-    done = 0
+    # the autotuner's trial batches are real steps of the run (results do not
+    # depend on the schedule); each pack then runs the rest of its steps
+    done = [0] * len(packs)
+    schedules = []
+    if autotune:
+        for k, (_, p) in enumerate(packs):
+            sched = p.autotune(2)
+            schedules.append(sched)
+            done[k] = int(p.stats().step)
+            if verbose and sched["bucketed"]:
+                _progress(f"packed: schedule {sched}")
     last = t0
-    while done < steps:
-        c = min(chunk, steps - done)
-        for _, p in packs:
-            p.run(c)
-        done += c
+    while min(done) < steps:
+        for k, (_, p) in enumerate(packs):
+            c = min(chunk, steps - done[k])
+            if c > 0:
+                p.run(c)
+                done[k] += c
         if verbose and time.perf_counter() - last > 10:
             for _, p in packs:
                 p.sync()
             last = time.perf_counter()
-            _progress(f"packed: {done}/{steps} steps ({last - t0:.1f} s)")
+            _progress(f"packed: {min(done)}/{steps} steps ({last - t0:.1f} s)")
     for _, p in packs:
         p.sync()
     run_s = time.perf_counter() - t0
@@ -101,7 +113,7 @@ def run_packed(specs, steps, mode=_abi.MODE_DETERMINISTIC, chunk=100, record=Tru
             for s, r in zip(g, p.spikes()):
                 rasters[s["index"]] = r
         p.close()
-    return dict(build_s=build_s, run_s=run_s, packs=stats, rasters=rasters)
+    return dict(build_s=build_s, run_s=run_s, packs=stats, rasters=rasters, schedules=schedules)
 
 
 def run_sequential(specs, steps, mode=_abi.MODE_DETERMINISTIC, record=True, verbose=False):
@@ -141,7 +153,7 @@ def main():
     packed = run_packed(specs, a.steps, mode, graphs=not a.no_graphs, verbose=True)
     out = dict(nets=a.nets, steps=a.steps, mode=a.mode, packed_run_s=packed["run_s"],
                packed_build_s=packed["build_s"], packed_jobs_per_s=a.nets / packed["run_s"],
-               packs=packed["packs"])
+               packs=packed["packs"], schedules=packed["schedules"])
     if not a.no_sequential:
         seq = run_sequential(specs, a.steps, mode, verbose=True)
         ident = sum(int(np.array_equal(packed["rasters"][i], seq["rasters"][i])) for i in range(a.nets))

@@ -124,3 +124,23 @@ def test_full_size_events_conserved_and_deterministic():
         net.close()
     assert first_divergence(rasters[0], rasters[1]) is None
 
+
+
+def test_autotune_and_schedules_leave_results_unchanged():
+    net = multi_delay_net()
+    net.compile()
+    o = oracle_of(net, O.FIXED)
+    net.run(100)
+    chosen = net.autotune(1)  # real steps, aligned batches
+    assert chosen["bucketed"] == 1 and chosen["task_synapses"] in (256, 512)
+    steps = int(net.stats().step)
+    assert steps > 100
+    net.set_schedule(256, 1, 3)
+    net.run(60)
+    net.set_schedule(512, 4, 1)
+    net.run(700 - steps - 60)
+    o.run(700)
+    assert_same(net, o)
+    with pytest.raises(Exception):
+        net.set_schedule(128, 1, 1)
+    net.close()
