@@ -118,14 +118,16 @@ def main():
     print(f"# built N={net.getNeurons()} S={net.getSynapses()} in {build_s:.1f} s", file=sys.stderr,
           flush=True)
 
-    # warm-up: the autotuner picks the delivery schedule on real steps, then
-    # the rest of the warm-up steps run with it
+    # warm-up: half of it to leave the initial transient, then the autotuner
+    # picks the delivery schedule on real steps, then the rest of the warm-up
+    net.run(args.warmup // 2)
     schedule = net.autotune(2) if args.warmup > 0 and not args.no_autotune else net.schedule()
     done = int(net.stats().step)
     if args.warmup > done:
         net.run(args.warmup - done)
     net.finish()
     s0 = net.stats()
+    warmup_run = int(s0.step)
     t0 = time.perf_counter()
     net.run(args.steps)
     net.finish()
@@ -214,6 +216,7 @@ def main():
         "events": int(events),
         "build_s": round(build_s, 2),
         "schedule": schedule,
+        "warmup_steps_run": warmup_run,
         "roofline": roofline,
         "kernels": kernels,
         "cpu_baseline": cpu,

@@ -81,8 +81,10 @@ def run_packed(specs, steps, mode=_abi.MODE_DETERMINISTIC, chunk=100, record=Tru
     # depend on the schedule); each pack then runs the rest of its steps
     done = [0] * len(packs)
     schedules = []
-    if autotune:
+    # the tuning trials take up to 2 passes x 18 schedules x 2 batches x 16 steps
+    if autotune and steps >= 2000:
         for k, (_, p) in enumerate(packs):
+            p.run(min(steps // 10, 500))  # past the initial transient
             sched = p.autotune(2)
             schedules.append(sched)
             done[k] = int(p.stats().step)
