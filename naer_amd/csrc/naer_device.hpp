@@ -15,7 +15,7 @@ constexpr uint32_t kMaxBatch = 16;     // steps per update launch
 constexpr uint32_t kMaxBatchLargeTile = 9;  // ... with 1024-neuron tiles
 constexpr uint32_t kBlock = 256;       // threads per block of the step kernels
 constexpr uint32_t kWave = 64;         // CDNA wavefront
-constexpr uint32_t kLdsSpikes = 128;   // per-block spike staging in LDS
+constexpr uint32_t kLdsSpikes = 256;   // per-block spike staging in LDS
 constexpr uint32_t kChunkUnroll = 4;   // synapses per lane per scatter task
 constexpr uint32_t kChunk = kWave * kChunkUnroll;
 constexpr uint32_t kBucketUnroll = 8;  // default synapses per lane per task of the bucketed passes

@@ -83,13 +83,21 @@ def run_packed(specs, steps, mode=_abi.MODE_DETERMINISTIC, chunk=100, record=Tru
     schedules = []
     # the tuning trials take up to 2 passes x 18 schedules x 2 batches x 16 steps
     if autotune and steps >= 2000:
+        # packs the tuner does not apply to (ring delivery) get all their
+        # steps enqueued first: they keep the device busy while the others tune
+        tunable = [p.schedule()["bucketed"] == 1 for _, p in packs]
         for k, (_, p) in enumerate(packs):
-            p.run(min(steps // 10, 500))  # past the initial transient
-            sched = p.autotune(2)
-            schedules.append(sched)
-            done[k] = int(p.stats().step)
-            if verbose and sched["bucketed"]:
-                _progress(f"packed: schedule {sched}")
+            if not tunable[k]:
+                p.run(steps)
+                done[k] = steps
+        for k, (_, p) in enumerate(packs):
+            if tunable[k]:
+                p.run(min(steps // 10, 500))  # past the initial transient
+                sched = p.autotune(2)
+                schedules.append(sched)
+                done[k] = int(p.stats().step)
+                if verbose:
+                    _progress(f"packed: schedule {sched}")
     last = t0
     while min(done) < steps:
         for k, (_, p) in enumerate(packs):
@@ -149,10 +157,11 @@ def main():
     ap.add_argument("--mode", choices=["det", "fast"], default="det")
     ap.add_argument("--no-sequential", action="store_true")
     ap.add_argument("--no-graphs", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     a = ap.parse_args()
     mode = _abi.MODE_DETERMINISTIC if a.mode == "det" else _abi.MODE_FAST
     specs = sweep_specs(a.nets, n_max=a.n_max)
-    packed = run_packed(specs, a.steps, mode, graphs=not a.no_graphs, verbose=True)
+    packed = run_packed(specs, a.steps, mode, graphs=not a.no_graphs, verbose=True, autotune=not a.no_autotune)
     out = dict(nets=a.nets, steps=a.steps, mode=a.mode, packed_run_s=packed["run_s"],
                packed_build_s=packed["build_s"], packed_jobs_per_s=a.nets / packed["run_s"],
                packs=packed["packs"], schedules=packed["schedules"])
