@@ -321,7 +321,8 @@ int naer_net_read_spikes(naer_net* net, naer_spike* out, uint64_t cap, uint64_t*
 
 /* Schedule of the bucketed delivery (no counterpart in the reference): how
  * the histogram/placement kernels cut the batch's spike rows into wave tasks
- * and blocks, and how many blocks refine each group bucket.  Any schedule
+ * and blocks, how many blocks refine each group bucket, and how the kernels'
+ * LDS counters are incremented.  Any schedule
  * gives the same results.  naer_net_autotune tries candidates on the running
  * simulation (each for `batches` aligned batches of real steps, timed with HIP
  * events, scored per synaptic event) and keeps the fastest. */
@@ -330,6 +331,8 @@ typedef struct naer_schedule {
     uint32_t task_synapses;      /* synapses of a row per wave task: 256 or 512 */
     uint32_t segments_per_block; /* spike-list segments per histogram/placement block, 1..64 */
     uint32_t refine_split;       /* refine blocks per group bucket, 1..64 */
+    uint32_t run_aggregate;      /* 1: one LDS counter atomic per run of equal bucket keys in a
+                                    wave (rows concentrated on few tiles), 0: one per synapse */
     double delivery_us;          /* out: delivery time per batch measured by the autotuner */
 } naer_schedule;
 int naer_net_get_schedule(naer_net* net, naer_schedule* out);

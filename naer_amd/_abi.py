@@ -87,7 +87,7 @@ class Spike(ctypes.Structure):
 class Schedule(ctypes.Structure):
     _fields_ = [("bucketed", ctypes.c_uint32), ("task_synapses", ctypes.c_uint32),
                 ("segments_per_block", ctypes.c_uint32), ("refine_split", ctypes.c_uint32),
-                ("delivery_us", ctypes.c_double)]
+                ("run_aggregate", ctypes.c_uint32), ("delivery_us", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -181,8 +181,11 @@ def get_schedule(handle):
     return s.as_dict()
 
 
-def set_schedule(handle, task_synapses, segments_per_block, refine_split):
-    s = Schedule(0, int(task_synapses), int(segments_per_block), int(refine_split), 0.0)
+def set_schedule(handle, task_synapses, segments_per_block, refine_split, run_aggregate=None):
+    """run_aggregate None keeps the current counter mode."""
+    if run_aggregate is None:
+        run_aggregate = get_schedule(handle)["run_aggregate"]
+    s = Schedule(0, int(task_synapses), int(segments_per_block), int(refine_split), int(run_aggregate), 0.0)
     check(lib().naer_net_set_schedule(handle, ctypes.byref(s)))
 
 

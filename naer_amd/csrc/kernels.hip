@@ -43,6 +43,39 @@ __device__ __forceinline__ float clip(float w, float lo, float hi)
     return w < lo ? lo : (w > hi ? hi : w);
 }
 
+// Adds 1 to cnt[key] for every active lane and returns the lane's slot (the
+// counter's old value plus the lane's rank among the lanes of that key).
+// AGG: one LDS atomic per run of equal keys on consecutive lanes (the active
+// lanes must be a prefix of the wave): sorted CSR rows of a small network put
+// whole waves on one tile, and lane-by-lane atomics on one LDS word
+// serialise.  Without AGG (rows spread over many tiles) a plain atomic per
+// lane keeps the unrolled atomics of a task in flight together.
+constexpr uint32_t kRunHeads = 16;  // more run heads per wave: lane-by-lane atomics
+
+template <bool POS, bool AGG>
+__device__ __forceinline__ uint32_t run_add(uint32_t* cnt, uint32_t key)
+{
+    if (!AGG)
+        return atomicAdd(&cnt[key], 1u);
+    const uint32_t lane = lane_id();
+    // previous lane's key (DPP wave_shr:1; lane 0 is a head regardless)
+    const uint32_t prev = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)key, 0x138, 0xF, 0xF, false);
+    const unsigned long long heads = __ballot(lane == 0 || prev != key);
+    if (__popcll(heads) > kRunHeads)
+        return atomicAdd(&cnt[key], 1u);
+    const unsigned long long act = __ballot(1);
+    const unsigned long long upto = (2ull << lane) - 1ull;  // lanes <= lane (all for lane 63)
+    const unsigned long long above = heads & ~upto;
+    const uint32_t next = above ? (uint32_t)__builtin_ctzll(above) : 64u - (uint32_t)__builtin_clzll(act);
+    uint32_t old = 0;
+    if ((heads >> lane) & 1ull)
+        old = atomicAdd(&cnt[key], next - lane);
+    if (!POS)
+        return 0;
+    const uint32_t hl = 63u - (uint32_t)__builtin_clzll(heads & upto);  // lane 0 is always a head
+    return (uint32_t)__shfl((int)old, (int)hl) + (lane - hl);
+}
+
 // Population of neuron j: populations are contiguous ranges; the host gives
 // the first population of every update block.
 __device__ __forceinline__ uint32_t find_pop(const PopParams* pops, uint32_t n_pops, uint32_t first,
@@ -633,7 +666,7 @@ __device__ __forceinline__ void for_block_chunks(const BucketArgs& a, const naer
     }
 }
 
-template <uint32_t U>
+template <uint32_t U, bool AGG>
 __global__ __launch_bounds__(kBucketThreads) void k_bucket_hist(const BucketArgs a)
 {
     extern __shared__ uint32_t s_hist[];
@@ -662,7 +695,7 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_hist(const BucketArgs
             if (q < end) {
                 const uint32_t v = r + ((c[u] >> 24) & 0x7Fu);
                 const uint32_t T = (c[u] & kTargetMask) >> a.tw_shift;
-                atomicAdd(&s_hist[(lut_m[v] - 1u) * a.NT + T], 1u);
+                run_add<false, AGG>(s_hist, (lut_m[v] - 1u) * a.NT + T);
             }
         }
     });
@@ -784,7 +817,7 @@ __global__ __launch_bounds__(1024) void k_bucket_starts(const BucketArgs a)
     }
 }
 
-template <bool DET, uint32_t U>
+template <bool DET, uint32_t U, bool AGG>
 __global__ __launch_bounds__(kBucketThreads) void k_bucket_place(const BucketArgs a)
 {
     using RingT = typename std::conditional<DET, unsigned long long, float>::type;
@@ -832,7 +865,7 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_place(const BucketArg
                 if (ok) {
                     const uint32_t v = r + d;
                     const uint32_t g = tgt >> g_shift;
-                    const uint32_t pos = atomicAdd(&s_cur[(lut_m[v] - 1u) * ng + g], 1u);
+                    const uint32_t pos = run_add<true, AGG>(s_cur, (lut_m[v] - 1u) * ng + g);
                     stage[pos] = make_uint2((tgt - (g << g_shift)) | ((uint32_t)lut_o[v] << 16),
                                             __float_as_uint(w[u]));
                 } else {
@@ -858,6 +891,7 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_place(const BucketArg
 constexpr uint32_t kRefineThreads = 512;
 constexpr uint32_t kRefineChunk = 4096;
 
+template <bool AGG>
 __global__ __launch_bounds__(kRefineThreads) void k_bucket_refine(const BucketArgs a)
 {
     __shared__ uint2 s_ev[kRefineChunk];
@@ -885,7 +919,7 @@ __global__ __launch_bounds__(kRefineThreads) void k_bucket_refine(const BucketAr
         for (uint32_t i = threadIdx.x; i < n; i += kRefineThreads) {
             const uint2 e = a.stage[r0 + i];
             s_ev[i] = e;
-            s_rank[i] = (uint16_t)atomicAdd(&s_cnt[(e.x & 0xFFFFu) >> a.tw_shift], 1u);
+            s_rank[i] = (uint16_t)run_add<true, AGG>(s_cnt, (e.x & 0xFFFFu) >> a.tw_shift);
         }
         __syncthreads();
         if (threadIdx.x < gt && s_cnt[threadIdx.x])
@@ -1168,16 +1202,25 @@ hipError_t launch_update(const UpdateArgs& a, bool det, bool plastic, bool bucke
     return det ? launch_update_tw<true, false, false>(a, pois, s) : launch_update_tw<false, false, false>(a, pois, s);
 }
 
+template <bool AGG>
+static hipError_t hist_lds_attributes(size_t lds)
+{
+    hipError_t e = hipFuncSetAttribute((const void*)k_bucket_hist<4, AGG>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)k_bucket_hist<8, AGG>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds);
+    return e;
+}
+
 static hipError_t bucket_lds_attributes(const BucketArgs& a)
 {
     const size_t lds = (size_t)a.NBK * sizeof(uint32_t);
     static size_t lds_set = 0;
     if (lds > 32768 && lds > lds_set) {
-        hipError_t e = hipFuncSetAttribute((const void*)k_bucket_hist<4>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)lds);
+        hipError_t e = hist_lds_attributes<false>(lds);
         if (e == hipSuccess)
-            e = hipFuncSetAttribute((const void*)k_bucket_hist<8>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)lds);
+            e = hist_lds_attributes<true>(lds);
         if (e == hipSuccess)
             e = hipFuncSetAttribute((const void*)k_bucket_starts, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)lds);
@@ -1188,6 +1231,33 @@ static hipError_t bucket_lds_attributes(const BucketArgs& a)
     return hipSuccess;
 }
 
+template <bool AGG>
+static void launch_bucket_pass(const BucketArgs& a, int step, bool det, size_t lds, size_t lds_c, hipStream_t s)
+{
+    const dim3 grid(a.NB), block(kBucketThreads);
+    switch (step) {
+    case 0:
+        if (a.U == 4)
+            hipLaunchKernelGGL((k_bucket_hist<4, AGG>), grid, block, lds, s, a);
+        else
+            hipLaunchKernelGGL((k_bucket_hist<8, AGG>), grid, block, lds, s, a);
+        break;
+    case 2:
+        if (det && a.U == 4)
+            hipLaunchKernelGGL((k_bucket_place<true, 4, AGG>), grid, block, lds_c, s, a);
+        else if (det)
+            hipLaunchKernelGGL((k_bucket_place<true, 8, AGG>), grid, block, lds_c, s, a);
+        else if (a.U == 4)
+            hipLaunchKernelGGL((k_bucket_place<false, 4, AGG>), grid, block, lds_c, s, a);
+        else
+            hipLaunchKernelGGL((k_bucket_place<false, 8, AGG>), grid, block, lds_c, s, a);
+        break;
+    default:
+        hipLaunchKernelGGL(k_bucket_refine<AGG>, dim3(a.NBK >> a.gt_shift, a.RS), dim3(kRefineThreads), 0, s, a);
+        break;
+    }
+}
+
 hipError_t launch_bucket_step(const BucketArgs& a, int step, bool det, hipStream_t s)
 {
     hipError_t e = bucket_lds_attributes(a);
@@ -1196,29 +1266,18 @@ hipError_t launch_bucket_step(const BucketArgs& a, int step, bool det, hipStream
     const size_t lds = (size_t)a.NBK * sizeof(uint32_t);
     const size_t lds_c = (size_t)(a.NBK >> a.gt_shift) * sizeof(uint32_t);
     switch (step) {
-    case 0:
-        if (a.U == 4)
-            hipLaunchKernelGGL(k_bucket_hist<4>, dim3(a.NB), dim3(kBucketThreads), lds, s, a);
-        else
-            hipLaunchKernelGGL(k_bucket_hist<8>, dim3(a.NB), dim3(kBucketThreads), lds, s, a);
-        break;
     case 1:
         hipLaunchKernelGGL(k_bucket_colscan, dim3(cdiv(a.NBK >> a.gt_shift, kScanCols) + cdiv(a.NBK, kScanCols)),
                            dim3(kScanCols * kScanGroups), 0, s, a);
         hipLaunchKernelGGL(k_bucket_starts, dim3(1), dim3(1024), lds, s, a);
         break;
+    case 0:
     case 2:
-        if (det && a.U == 4)
-            hipLaunchKernelGGL((k_bucket_place<true, 4>), dim3(a.NB), dim3(kBucketThreads), lds_c, s, a);
-        else if (det)
-            hipLaunchKernelGGL((k_bucket_place<true, 8>), dim3(a.NB), dim3(kBucketThreads), lds_c, s, a);
-        else if (a.U == 4)
-            hipLaunchKernelGGL((k_bucket_place<false, 4>), dim3(a.NB), dim3(kBucketThreads), lds_c, s, a);
-        else
-            hipLaunchKernelGGL((k_bucket_place<false, 8>), dim3(a.NB), dim3(kBucketThreads), lds_c, s, a);
-        break;
     case 3:
-        hipLaunchKernelGGL(k_bucket_refine, dim3(a.NBK >> a.gt_shift, a.RS), dim3(kRefineThreads), 0, s, a);
+        if (a.AG)
+            launch_bucket_pass<true>(a, step, det, lds, lds_c, s);
+        else
+            launch_bucket_pass<false>(a, step, det, lds, lds_c, s);
         break;
     default:
         return hipErrorInvalidValue;

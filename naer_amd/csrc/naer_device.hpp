@@ -213,6 +213,7 @@ struct BucketArgs {
     uint32_t C;        // (64 U)-synapse tasks of the longest row
     uint32_t U;        // synapses per lane per task: 4 or 8
     uint32_t RS;       // refine blocks per group bucket
+    uint32_t AG;       // 1: LDS counters take one atomic per run of equal keys in a wave
     uint32_t is_last;
 };
 

@@ -203,6 +203,7 @@ struct naer_net {
     uint32_t R = 0, M = 0, NT = 0, NBK = 0, NB = 0, G = 1;
     uint32_t U = kBucketUnroll;  // schedule: synapses per lane per task
     uint32_t RS = 1;             // schedule: refine blocks per group bucket
+    uint32_t AG = 1;             // schedule: run-aggregated LDS counters
     uint64_t e_cap = 0;
     uint2* ev = nullptr;
     uint2* stage = nullptr;
@@ -749,6 +750,26 @@ int alloc_neurons(naer_net* n, bool any_poisson)
 constexpr double kBucketRateHz = 200.0;
 constexpr uint32_t kMaxBuckets = 16384;
 
+// Default of the run-aggregated LDS counters: on when a wave of 64 row
+// synapses is expected to hold runs of >= 4 equal bucket keys, i.e. a row's
+// synapses per (tile, arrival batch) of its own network; weighted by synapses
+// over the networks of a pack.  Arenas (networks unknown at build) default on;
+// the kernels fall back to lane-by-lane atomics on waves of short runs.
+uint32_t run_aggregation(const naer_net* n)
+{
+    double w = 0.0, sum = 0.0;
+    for (const auto& net : n->nets) {
+        if (net.n_neurons == 0 || net.n_syn == 0)
+            continue;
+        const double tiles = std::ceil((double)net.n_neurons / (double)(1u << n->tw_shift));
+        const double batches = (double)(net.dmax - std::min(net.dmin, net.dmax)) / n->B + 1.0;
+        const double run = (double)net.n_syn / net.n_neurons / (tiles * batches);
+        w += (double)net.n_syn * run;
+        sum += (double)net.n_syn;
+    }
+    return sum == 0.0 || w / sum >= 4.0 ? 1u : 0u;
+}
+
 int alloc_buckets(naer_net* n)
 {
     n->bucketed = false;
@@ -780,6 +801,7 @@ int alloc_buckets(naer_net* n)
     n->NB = (n->n_seg + n->G - 1) / n->G;
     n->U = kBucketUnroll;
     n->RS = std::max<uint32_t>(1, std::min<uint32_t>(64, 2048 / std::max<uint32_t>(1, n->NBK >> n->gt_shift)));
+    n->AG = run_aggregation(n);
     if (n->M > kMaxArrivalBatches || n->G > kMaxSegPerBlock || (n->NBK >> n->gt_shift) > 1024)
         return NAER_OK;
     int rc;
@@ -1276,6 +1298,7 @@ static int enqueue_steps(naer_net* n, uint32_t n_steps, bool timed)
     ba.gcur = n->gcur;
     // about 2048 refine blocks over the group buckets
     ba.RS = n->RS;
+    ba.AG = n->AG;
     ba.is_last = 1;
     PotentiateArgs pa;
     std::memset(&pa, 0, sizeof pa);
@@ -1390,6 +1413,7 @@ int naer_net_get_schedule(naer_net* n, naer_schedule* out)
     out->task_synapses = kWave * n->U;
     out->segments_per_block = n->G;
     out->refine_split = n->RS;
+    out->run_aggregate = n->AG;
     out->delivery_us = n->tuned_us;
     return NAER_OK;
 }
@@ -1404,6 +1428,8 @@ int naer_net_set_schedule(naer_net* n, const naer_schedule* s)
         return fail(NAER_ERR_INVALID, "segments_per_block must be in [1, %u]", kMaxSegPerBlock);
     if (s->refine_split == 0 || s->refine_split > 64)
         return fail(NAER_ERR_INVALID, "refine_split must be in [1, 64]");
+    if (s->run_aggregate > 1)
+        return fail(NAER_ERR_INVALID, "run_aggregate must be 0 or 1");
     int rc = naer_net_sync(n);
     if (rc || (rc = n->drop_graph()))  // captured launches carry the old shapes
         return rc;
@@ -1411,6 +1437,7 @@ int naer_net_set_schedule(naer_net* n, const naer_schedule* s)
     n->G = s->segments_per_block;
     n->NB = (n->n_seg + n->G - 1) / n->G;
     n->RS = s->refine_split;
+    n->AG = s->run_aggregate;
     return NAER_OK;
 }
 
@@ -1455,6 +1482,19 @@ int naer_net_autotune(naer_net* n, uint32_t batches, naer_schedule* chosen)
                 if (!dup)
                     cand.push_back(c);
             }
+    // the base schedule with the other counter mode; the winner takes the
+    // mode that did better on the base
+    const size_t n_main = cand.size();
+    {
+        naer_schedule c = base;
+        c.run_aggregate = base.run_aggregate ? 0u : 1u;
+        cand.push_back(c);
+    }
+    size_t base_i = 0;
+    for (size_t i = 0; i < n_main; ++i)
+        if (cand[i].task_synapses == base.task_synapses && cand[i].segments_per_block == base.segments_per_block &&
+            cand[i].refine_split == base.refine_split)
+            base_i = i;
     // two passes over the candidates, each candidate keeping its better
     // score, so that a drift of the activity during the trials does not
     // decide
@@ -1482,11 +1522,16 @@ int naer_net_autotune(naer_net* n, uint32_t batches, naer_schedule* chosen)
         }
     }
     size_t bi = 0;
-    for (size_t i = 1; i < cand.size(); ++i)
+    for (size_t i = 1; i < n_main; ++i)
         if (score[i] < score[bi])
             bi = i;
     naer_schedule best_s = cand[bi];
     best_s.delivery_us = per_batch[bi];
+    if (score[n_main] < score[base_i]) {
+        best_s.run_aggregate = cand[n_main].run_aggregate;
+        if (bi == base_i)
+            best_s.delivery_us = per_batch[n_main];
+    }
     n->profiling = prof;
     n->graph_chunk = graph;
     if (rc)

@@ -252,8 +252,8 @@ class Network:
         """The delivery schedule (naer_schedule) as a dict."""
         return _abi.get_schedule(self._h())
 
-    def set_schedule(self, task_synapses, segments_per_block, refine_split):
-        _abi.set_schedule(self._h(), task_synapses, segments_per_block, refine_split)
+    def set_schedule(self, task_synapses, segments_per_block, refine_split, run_aggregate=None):
+        _abi.set_schedule(self._h(), task_synapses, segments_per_block, refine_split, run_aggregate)
 
     def autotune(self, batches=2):
         """Try delivery schedules on the running simulation (real steps) and

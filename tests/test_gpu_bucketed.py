@@ -135,12 +135,14 @@ def test_autotune_and_schedules_leave_results_unchanged():
     assert chosen["bucketed"] == 1 and chosen["task_synapses"] in (256, 512)
     steps = int(net.stats().step)
     assert steps > 100
-    net.set_schedule(256, 1, 3)
+    net.set_schedule(256, 1, 3, run_aggregate=1)
     net.run(60)
-    net.set_schedule(512, 4, 1)
+    net.set_schedule(512, 4, 1, run_aggregate=0)
     net.run(700 - steps - 60)
     o.run(700)
     assert_same(net, o)
     with pytest.raises(Exception):
         net.set_schedule(128, 1, 1)
+    with pytest.raises(Exception):
+        net.set_schedule(256, 1, 1, run_aggregate=2)
     net.close()
