@@ -443,6 +443,7 @@ __global__ __launch_bounds__(TW, (BUCKET && !PLASTIC && !POIS && TW == 1024) ? 8
         seg[i] = s_list[i];
     if (threadIdx.x == 0) {
         a.seg_count[pl * a.n_seg + blockIdx.x] = n;
+        a.seg_ev[pl * a.n_seg + blockIdx.x] = s_events;
         a.spk_part[blockIdx.x] += total;
         a.ev_part[blockIdx.x] += s_events;
         if (total > n)
@@ -597,22 +598,62 @@ __global__ __launch_bounds__(kBlock) void k_potentiate(const PotentiateArgs a)
 constexpr uint32_t kBucketThreads = 1024;
 constexpr uint32_t kArrivalLut = 256;  // offsets r + d < kMaxBatch + 128
 
-__device__ __forceinline__ void bucket_setup(const BucketArgs& a, uint32_t cur, uint8_t* lut_m, uint8_t* lut_o,
-                                             uint32_t* s_off)
+__device__ __forceinline__ void bucket_setup(const BucketArgs& a, uint8_t* lut_m, uint8_t* lut_o)
 {
     for (uint32_t v = threadIdx.x; v < kArrivalLut; v += blockDim.x) {
         lut_m[v] = (uint8_t)(v / a.Bfull);
         lut_o[v] = (uint8_t)(v % a.Bfull);
     }
-    const uint32_t g0 = blockIdx.x * a.G;
-    const uint32_t G = min(a.G, a.n_seg - g0);
-    if (threadIdx.x < G)
-        s_off[threadIdx.x + 1] = a.seg_count[cur * a.n_seg + g0 + threadIdx.x];
+}
+
+// Segment ranges of the histogram/placement blocks, balanced by the batch's
+// synaptic events: segment s goes to block
+//   min(NB - 1, floor((prefix(s) + events(s) / 2) * NB / total)),
+// a nondecreasing map, so every block gets a contiguous (possibly empty)
+// range; blk_seg[b] = first segment of block b, blk_seg[NB] = n_seg.  One
+// block of 1024 threads, each owning a contiguous run of segments.
+__global__ __launch_bounds__(1024) void k_bucket_split(const BucketArgs a)
+{
+    __shared__ unsigned long long s_part[1024];
+    const uint32_t cur = a.st->cur;
+    const unsigned long long* ev = a.seg_ev + (size_t)cur * a.n_seg;
+    const uint32_t per = (a.n_seg + 1023) / 1024;
+    const uint32_t lo = min(threadIdx.x * per, a.n_seg), hi = min(lo + per, a.n_seg);
+    unsigned long long sum = 0;
+    for (uint32_t k = lo; k < hi; ++k)
+        sum += ev[k];
+    s_part[threadIdx.x] = sum;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        s_off[0] = 0;
-        for (uint32_t g = 1; g <= G; ++g)
-            s_off[g] += s_off[g - 1];
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        const unsigned long long v = threadIdx.x >= off ? s_part[threadIdx.x - off] : 0ull;
+        __syncthreads();
+        s_part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    const unsigned long long total = s_part[1023];
+    unsigned long long run = s_part[threadIdx.x] - sum;  // exclusive prefix of segment lo
+    const unsigned long long nb = a.NB;
+    auto block_of = [&](uint32_t k, unsigned long long pre, unsigned long long e) -> uint32_t {
+        const unsigned long long b = total ? (2ull * pre + e) * nb / (2ull * total) : (unsigned long long)k * nb / a.n_seg;
+        return (uint32_t)min(b, nb - 1ull);
+    };
+    uint32_t prev = 0;
+    if (lo > 0 && lo < hi) {
+        const unsigned long long e = ev[lo - 1];
+        prev = block_of(lo - 1, run - e, e);
+    }
+    for (uint32_t k = lo; k < hi; ++k) {
+        const unsigned long long e = ev[k];
+        const uint32_t b = block_of(k, run, e);
+        // blocks (prev, b] start at k (blocks [0, b] for the first segment)
+        for (uint32_t q = k == 0 ? 0u : prev + 1u; q <= b; ++q)
+            a.blk_seg[q] = k;
+        prev = b;
+        run += e;
+    }
+    if (hi == a.n_seg && lo < hi) {
+        for (uint32_t q = prev + 1u; q <= a.NB; ++q)
+            a.blk_seg[q] = a.n_seg;
     }
 }
 
@@ -628,41 +669,56 @@ struct RowStage {
 
 // Calls f(r, q_begin, q_end) for every non-empty (64 U)-synapse chunk of the
 // rows of the block's spikes; f runs on a whole wave (q range wave-uniform).
+// The block's segments are staged kMaxSegPerBlock at a time.
 template <uint32_t U, typename F>
-__device__ __forceinline__ void for_block_chunks(const BucketArgs& a, const naer_spike* list, const uint32_t* s_off,
+__device__ __forceinline__ void for_block_chunks(const BucketArgs& a, uint32_t cur, uint32_t* s_off,
                                                  unsigned long long t0, RowStage& rs, F f)
 {
     constexpr uint32_t kTask = kWave * U;  // synapses of a row per task
-    const uint32_t G = min(a.G, a.n_seg - blockIdx.x * a.G);
-    const uint32_t n_spk = s_off[G];
+    const naer_spike* list = cur ? a.list1 : a.list0;
+    const uint32_t seg_lo = a.blk_seg[blockIdx.x], seg_hi = a.blk_seg[blockIdx.x + 1];
     const uint32_t nwv = blockDim.x / kWave;
-    for (uint32_t base = 0; base < n_spk; base += kRowRound) {
-        const uint32_t cnt = min(kRowRound, n_spk - base);
-        for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
-            const uint32_t x = base + i;
-            uint32_t lo = 0, hi = G;  // segment g with s_off[g] <= x < s_off[g + 1]
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (s_off[mid] <= x)
-                    lo = mid;
-                else
-                    hi = mid;
+    for (uint32_t w0 = seg_lo; w0 < seg_hi; w0 += kMaxSegPerBlock) {
+        const uint32_t G = min(kMaxSegPerBlock, seg_hi - w0);
+        if (threadIdx.x < G)
+            s_off[threadIdx.x + 1] = a.seg_count[cur * a.n_seg + w0 + threadIdx.x];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            s_off[0] = 0;
+            for (uint32_t g = 1; g <= G; ++g)
+                s_off[g] += s_off[g - 1];
+        }
+        __syncthreads();
+        const uint32_t n_spk = s_off[G];
+        for (uint32_t base = 0; base < n_spk; base += kRowRound) {
+            const uint32_t cnt = min(kRowRound, n_spk - base);
+            for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
+                const uint32_t x = base + i;
+                uint32_t lo = 0, hi = G;  // segment g with s_off[g] <= x < s_off[g + 1]
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (s_off[mid] <= x)
+                        lo = mid;
+                    else
+                        hi = mid;
+                }
+                const naer_spike sp = list[(size_t)(w0 + lo) * a.seg_cap + (x - s_off[lo])];
+                const unsigned long long rb = a.row_ptr[sp.neuron];
+                rs.beg[i] = rb;
+                rs.len[i] = (uint32_t)(a.row_ptr[sp.neuron + 1] - rb);
+                rs.r[i] = (uint32_t)(sp.step - t0);
             }
-            const naer_spike sp = list[(size_t)(blockIdx.x * a.G + lo) * a.seg_cap + (x - s_off[lo])];
-            const unsigned long long rb = a.row_ptr[sp.neuron];
-            rs.beg[i] = rb;
-            rs.len[i] = (uint32_t)(a.row_ptr[sp.neuron + 1] - rb);
-            rs.r[i] = (uint32_t)(sp.step - t0);
+            __syncthreads();
+            const uint32_t n_tasks = cnt * a.C;
+            for (uint32_t t = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave); t < n_tasks; t += nwv) {
+                const uint32_t i = t / a.C, c = t - i * a.C;
+                const uint32_t off = c * kTask, len = rs.len[i];
+                if (off < len)
+                    f(rs.r[i], rs.beg[i] + off, rs.beg[i] + min(len, off + kTask));
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        const uint32_t n_tasks = cnt * a.C;
-        for (uint32_t t = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave); t < n_tasks; t += nwv) {
-            const uint32_t i = t / a.C, c = t - i * a.C;
-            const uint32_t off = c * kTask, len = rs.len[i];
-            if (off < len)
-                f(rs.r[i], rs.beg[i] + off, rs.beg[i] + min(len, off + kTask));
-        }
-        __syncthreads();
+        __syncthreads();  // every thread has read s_off[G] before the next window
     }
 }
 
@@ -678,11 +734,10 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_hist(const BucketArgs
     const unsigned long long t0 = st->t0_cur;
     for (uint32_t i = threadIdx.x; i < a.NBK; i += blockDim.x)
         s_hist[i] = 0;
-    bucket_setup(a, cur, lut_m, lut_o, s_off);
+    bucket_setup(a, lut_m, lut_o);
     __syncthreads();
-    const naer_spike* list = cur ? a.list1 : a.list0;
     const uint32_t lane = lane_id();
-    for_block_chunks<U>(a, list, s_off, t0, rs, [&](uint32_t r, unsigned long long beg, unsigned long long end) {
+    for_block_chunks<U>(a, cur, s_off, t0, rs, [&](uint32_t r, unsigned long long beg, unsigned long long end) {
         uint32_t c[U];
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u) {
@@ -839,15 +894,14 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_place(const BucketArg
         for (uint32_t i = threadIdx.x; i < nbc; i += blockDim.x)
             s_cur[i] = bs[i << a.gt_shift] + h[i];
     }
-    bucket_setup(a, cur, lut_m, lut_o, s_off);
+    bucket_setup(a, lut_m, lut_o);
     __syncthreads();
     if (a.is_last && blockIdx.x == 0 && threadIdx.x == 0)
         batch_epilogue(st, a.B);
     uint2* stage = a.stage;
     RingT* ring = (RingT*)a.ring;
-    const naer_spike* list = cur ? a.list1 : a.list0;
     const uint32_t lane = lane_id();
-    for_block_chunks<U>(a, list, s_off, t0, rs, [&](uint32_t r, unsigned long long beg, unsigned long long end) {
+    for_block_chunks<U>(a, cur, s_off, t0, rs, [&](uint32_t r, unsigned long long beg, unsigned long long end) {
         uint32_t c[U];
         float w[U];
 #pragma unroll
@@ -883,11 +937,14 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_place(const BucketArg
     });
 }
 
-// Group bucket (m, g) = blockIdx.x: its events lie in the staging buffer at
-// the range of buckets [kc << gt_shift, (kc + 1) << gt_shift); blockIdx.y
-// takes the y-th of RS equal parts of that range and moves its events to
-// their tile's bucket of ev[E % R], kRefineChunk at a time: counted per tile
-// in LDS, a base per tile taken from the buckets' global cursors, written.
+// The staging buffer holds the batch's events in group-bucket order (group
+// bucket kc = (m - 1) * groups + g covers buckets [kc << gt_shift,
+// (kc + 1) << gt_shift)).  With `balance` (packs, arenas) block b takes the
+// b-th of gridDim.x equal parts of the whole buffer, whatever the spread of
+// the events over groups; otherwise block b takes part b % RS of group bucket
+// b / RS.  It moves the events of each group bucket it overlaps to their tile's bucket of
+// ev[E % R], kRefineChunk at a time: counted per tile in LDS, a base per tile
+// taken from the buckets' global cursors, written.
 constexpr uint32_t kRefineThreads = 512;
 constexpr uint32_t kRefineChunk = 4096;
 
@@ -904,33 +961,60 @@ __global__ __launch_bounds__(kRefineThreads) void k_bucket_refine(const BucketAr
         return;
     const uint32_t* bs = a.bs + (size_t)buf * (a.NBK + 1);
     const uint32_t gt = 1u << a.gt_shift;
-    const uint32_t k0 = blockIdx.x << a.gt_shift;
-    const uint32_t lo = bs[k0], hi = bs[k0 + gt];
-    const uint32_t len = hi - lo;
-    const uint32_t a0 = lo + (uint32_t)(((unsigned long long)len * blockIdx.y) / a.RS);
-    const uint32_t a1 = lo + (uint32_t)(((unsigned long long)len * (blockIdx.y + 1)) / a.RS);
+    const uint32_t ngb = a.NBK >> a.gt_shift;
+    uint32_t a0, a1, lo_k;
+    if (a.balance) {
+        const uint32_t total = bs[a.NBK];
+        a0 = (uint32_t)(((unsigned long long)total * blockIdx.x) / gridDim.x);
+        a1 = (uint32_t)(((unsigned long long)total * (blockIdx.x + 1)) / gridDim.x);
+        if (a0 >= a1)
+            return;
+        // first group bucket that ends after a0
+        uint32_t hi_k = ngb - 1;
+        lo_k = 0;
+        while (lo_k < hi_k) {
+            const uint32_t mid = (lo_k + hi_k) >> 1;
+            if (bs[(mid + 1) << a.gt_shift] > a0)
+                hi_k = mid;
+            else
+                lo_k = mid + 1;
+        }
+    } else {
+        // a single network: RS equal parts of each group bucket
+        lo_k = blockIdx.x / a.RS;
+        const uint32_t y = blockIdx.x - lo_k * a.RS;
+        const uint32_t lo = bs[lo_k << a.gt_shift], len = bs[(lo_k + 1) << a.gt_shift] - lo;
+        a0 = lo + (uint32_t)(((unsigned long long)len * y) / a.RS);
+        a1 = lo + (uint32_t)(((unsigned long long)len * (y + 1)) / a.RS);
+        if (a0 >= a1)
+            return;
+    }
     uint2* ev = a.ev + (size_t)buf * a.e_cap;
     const uint32_t tmask = (1u << a.tw_shift) - 1u;
-    for (uint32_t r0 = a0; r0 < a1; r0 += kRefineChunk) {
-        const uint32_t n = min(kRefineChunk, a1 - r0);
-        if (threadIdx.x < gt)
-            s_cnt[threadIdx.x] = 0;
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < n; i += kRefineThreads) {
-            const uint2 e = a.stage[r0 + i];
-            s_ev[i] = e;
-            s_rank[i] = (uint16_t)run_add<true, AGG>(s_cnt, (e.x & 0xFFFFu) >> a.tw_shift);
+    for (uint32_t kc = lo_k; kc < ngb && bs[kc << a.gt_shift] < a1; ++kc) {
+        const uint32_t k0 = kc << a.gt_shift;
+        const uint32_t g0 = max(bs[k0], a0), g1 = min(bs[k0 + gt], a1);
+        for (uint32_t r0 = g0; r0 < g1; r0 += kRefineChunk) {
+            const uint32_t n = min(kRefineChunk, g1 - r0);
+            if (threadIdx.x < gt)
+                s_cnt[threadIdx.x] = 0;
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < n; i += kRefineThreads) {
+                const uint2 e = a.stage[r0 + i];
+                s_ev[i] = e;
+                s_rank[i] = (uint16_t)run_add<true, AGG>(s_cnt, (e.x & 0xFFFFu) >> a.tw_shift);
+            }
+            __syncthreads();
+            if (threadIdx.x < gt && s_cnt[threadIdx.x])
+                s_base[threadIdx.x] = atomicAdd(&a.gcur[k0 + threadIdx.x], s_cnt[threadIdx.x]);
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < n; i += kRefineThreads) {
+                const uint2 e = s_ev[i];
+                const uint32_t t = (e.x & 0xFFFFu) >> a.tw_shift;
+                ev[s_base[t] + s_rank[i]] = make_uint2((e.x & tmask) | (e.x & 0xFFFF0000u), e.y);
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        if (threadIdx.x < gt && s_cnt[threadIdx.x])
-            s_base[threadIdx.x] = atomicAdd(&a.gcur[k0 + threadIdx.x], s_cnt[threadIdx.x]);
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < n; i += kRefineThreads) {
-            const uint2 e = s_ev[i];
-            const uint32_t t = (e.x & 0xFFFFu) >> a.tw_shift;
-            ev[s_base[t] + s_rank[i]] = make_uint2((e.x & tmask) | (e.x & 0xFFFF0000u), e.y);
-        }
-        __syncthreads();
     }
 }
 
@@ -1237,6 +1321,8 @@ static void launch_bucket_pass(const BucketArgs& a, int step, bool det, size_t l
     const dim3 grid(a.NB), block(kBucketThreads);
     switch (step) {
     case 0:
+        if (a.balance)
+            hipLaunchKernelGGL(k_bucket_split, dim3(1), dim3(1024), 0, s, a);
         if (a.U == 4)
             hipLaunchKernelGGL((k_bucket_hist<4, AGG>), grid, block, lds, s, a);
         else
@@ -1253,7 +1339,7 @@ static void launch_bucket_pass(const BucketArgs& a, int step, bool det, size_t l
             hipLaunchKernelGGL((k_bucket_place<false, 8, AGG>), grid, block, lds_c, s, a);
         break;
     default:
-        hipLaunchKernelGGL(k_bucket_refine<AGG>, dim3(a.NBK >> a.gt_shift, a.RS), dim3(kRefineThreads), 0, s, a);
+        hipLaunchKernelGGL(k_bucket_refine<AGG>, dim3((a.NBK >> a.gt_shift) * a.RS), dim3(kRefineThreads), 0, s, a);
         break;
     }
 }

@@ -141,6 +141,7 @@ struct UpdateArgs {
     naer_spike* list0;
     naer_spike* list1;
     uint32_t* seg_count;               // [2][n_seg]
+    unsigned long long* seg_ev;        // [2][n_seg] synaptic events of the batch's spikes per segment
     unsigned long long* spk_part;      // [n_seg]
     unsigned long long* ev_part;       // [n_seg]
     naer_spike* rec;
@@ -191,6 +192,8 @@ struct BucketArgs {
     const naer_spike* list0;
     const naer_spike* list1;
     const uint32_t* seg_count;
+    const unsigned long long* seg_ev;  // [2][n_seg]
+    uint32_t* blk_seg;                 // [NB + 1]: block b takes segments [blk_seg[b], blk_seg[b + 1])
     uint32_t seg_cap, n_seg;
     DevState* st;
     const unsigned long long* row_ptr;
@@ -208,16 +211,17 @@ struct BucketArgs {
     unsigned long long e_cap;
     uint32_t N, D, Bfull, B, R, M, NT, NBK, tw_shift;
     uint32_t gt_shift;  // tiles per group = 1 << gt_shift = 2^14 >> tw_shift
-    uint32_t G;        // list segments per histogram/placement block (<= kMaxSegPerBlock)
+    uint32_t G;        // mean list segments per histogram/placement block
     uint32_t NB;       // histogram/placement blocks
     uint32_t C;        // (64 U)-synapse tasks of the longest row
     uint32_t U;        // synapses per lane per task: 4 or 8
-    uint32_t RS;       // refine blocks per group bucket
+    uint32_t RS;       // refine blocks per group bucket (on average: they split the batch evenly)
+    uint32_t balance;  // 1: k_bucket_split sets blk_seg per batch; 0: fixed ranges of G segments
     uint32_t AG;       // 1: LDS counters take one atomic per run of equal keys in a wave
     uint32_t is_last;
 };
 
-constexpr uint32_t kMaxSegPerBlock = 64;  // G bound: NT <= 16384 tiles over <= 256 blocks
+constexpr uint32_t kMaxSegPerBlock = 64;  // G bound, and the segments a block stages at once
 constexpr uint32_t kMaxArrivalBatches = 64;  // M bound: (B - 1 + 127) / B for B >= 2
 
 struct ScatterArgs {

@@ -176,6 +176,8 @@ struct naer_net {
     naer_spike *list0 = nullptr, *list1 = nullptr, *rec = nullptr;
     uint32_t seg_cap = 0, n_seg = 0;
     uint32_t* seg_count = nullptr;
+    unsigned long long* seg_ev = nullptr;
+    uint32_t* blk_seg = nullptr;
     unsigned long long *spk_part = nullptr, *ev_part = nullptr;
     uint64_t rec_cap = 0, dropped = 0;
     DevState* st = nullptr;
@@ -204,6 +206,7 @@ struct naer_net {
     uint32_t U = kBucketUnroll;  // schedule: synapses per lane per task
     uint32_t RS = 1;             // schedule: refine blocks per group bucket
     uint32_t AG = 1;             // schedule: run-aggregated LDS counters
+    bool balance = false;        // histogram/placement blocks split each batch by events
     uint64_t e_cap = 0;
     uint2* ev = nullptr;
     uint2* stage = nullptr;
@@ -515,6 +518,7 @@ UpdateArgs make_update_args(naer_net* n)
     a.list0 = n->list0;
     a.list1 = n->list1;
     a.seg_count = n->seg_count;
+    a.seg_ev = n->seg_ev;
     a.spk_part = n->spk_part;
     a.ev_part = n->ev_part;
     a.rec = n->rec;
@@ -733,9 +737,11 @@ int alloc_neurons(naer_net* n, bool any_poisson)
     const size_t list_elems = (size_t)n->seg_cap * n->n_seg;
     if ((rc = n->alloc(&n->list0, list_elems)) || (rc = n->alloc(&n->list1, list_elems)) ||
         (rc = n->alloc(&n->seg_count, 2 * (size_t)n->n_seg)) || (rc = n->alloc(&n->spk_part, n->n_seg)) ||
-        (rc = n->alloc(&n->ev_part, n->n_seg)) || (rc = n->alloc(&n->d_blk_pop, n->n_seg)))
+        (rc = n->alloc(&n->ev_part, n->n_seg)) || (rc = n->alloc(&n->d_blk_pop, n->n_seg)) ||
+        (rc = n->alloc(&n->seg_ev, 2 * (size_t)n->n_seg)) || (rc = n->alloc(&n->blk_seg, (size_t)n->n_seg + 1)))
         return rc;
     HIP_TRY(hipMemsetAsync(n->seg_count, 0, 2 * (size_t)n->n_seg * sizeof(uint32_t), n->stream));
+    HIP_TRY(hipMemsetAsync(n->seg_ev, 0, 2 * (size_t)n->n_seg * sizeof(unsigned long long), n->stream));
     HIP_TRY(hipMemsetAsync(n->spk_part, 0, n->n_seg * sizeof(unsigned long long), n->stream));
     HIP_TRY(hipMemsetAsync(n->ev_part, 0, n->n_seg * sizeof(unsigned long long), n->stream));
     if ((rc = n->alloc(&n->rec, n->rec_cap ? n->rec_cap : 1)) || (rc = n->alloc(&n->st, 1)))
@@ -768,6 +774,21 @@ uint32_t run_aggregation(const naer_net* n)
         sum += (double)net.n_syn;
     }
     return sum == 0.0 || w / sum >= 4.0 ? 1u : 0u;
+}
+
+// Segment ranges of the histogram/placement blocks.  Packs and arenas
+// (networks of unequal size and rate side by side) split every batch by its
+// events on the device (k_bucket_split); a single network uses fixed ranges
+// of G segments, set here.
+int set_block_ranges(naer_net* n)
+{
+    if (n->balance)
+        return NAER_OK;
+    std::vector<uint32_t> h(n->NB + 1);
+    for (uint32_t b = 0; b <= n->NB; ++b)
+        h[b] = (uint32_t)std::min<uint64_t>((uint64_t)b * n->G, n->n_seg);
+    HIP_TRY(hipMemcpy(n->blk_seg, h.data(), h.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    return NAER_OK;
 }
 
 int alloc_buckets(naer_net* n)
@@ -812,6 +833,9 @@ int alloc_buckets(naer_net* n)
         (rc = n->alloc(&n->gcur, n->NBK)))
         return rc;
     HIP_TRY(hipMemsetAsync(n->flags, 0, n->R * sizeof(uint32_t), n->stream));
+    n->balance = n->arena || n->nets.size() > 1;
+    if ((rc = set_block_ranges(n)))
+        return rc;
     n->bucketed = true;
     return NAER_OK;
 }
@@ -1267,6 +1291,8 @@ static int enqueue_steps(naer_net* n, uint32_t n_steps, bool timed)
     ba.list0 = n->list0;
     ba.list1 = n->list1;
     ba.seg_count = n->seg_count;
+    ba.seg_ev = n->seg_ev;
+    ba.blk_seg = n->blk_seg;
     ba.seg_cap = n->seg_cap;
     ba.n_seg = n->n_seg;
     ba.st = n->st;
@@ -1299,6 +1325,7 @@ static int enqueue_steps(naer_net* n, uint32_t n_steps, bool timed)
     // about 2048 refine blocks over the group buckets
     ba.RS = n->RS;
     ba.AG = n->AG;
+    ba.balance = n->balance ? 1u : 0u;
     ba.is_last = 1;
     PotentiateArgs pa;
     std::memset(&pa, 0, sizeof pa);
@@ -1438,7 +1465,7 @@ int naer_net_set_schedule(naer_net* n, const naer_schedule* s)
     n->NB = (n->n_seg + n->G - 1) / n->G;
     n->RS = s->refine_split;
     n->AG = s->run_aggregate;
-    return NAER_OK;
+    return n->bucketed ? set_block_ranges(n) : NAER_OK;
 }
 
 // Tries schedules of the bucketed delivery on the running simulation: each
