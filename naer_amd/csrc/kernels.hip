@@ -76,6 +76,113 @@ __device__ __forceinline__ uint32_t run_add(uint32_t* cnt, uint32_t key)
     return (uint32_t)__shfl((int)old, (int)hl) + (lane - hl);
 }
 
+// dec^k of a trace from its table (k >= the table: below 2^-40, taken as 0)
+__device__ __forceinline__ float dec_pow(const float* tab, uint32_t n, uint32_t k)
+{
+    return k < n ? tab[k] : 0.0f;
+}
+
+// The traces' tables as a kernel reads them: the first n_l entries of each
+// from an LDS copy (the placement kernel stages them), the rest from memory.
+struct TraceTabs {
+    const float *tp_l, *tm_l;
+    uint32_t n_l;
+};
+
+__device__ __forceinline__ float dec_pow2(const float* lds, uint32_t n_l, const float* tab, uint32_t n, uint32_t k)
+{
+    return k < n_l ? lds[k] : (k < n ? tab[k] : 0.0f);
+}
+
+constexpr uint32_t kLdsTrace = 6144;  // table entries staged per trace (tau up to ~22 dt)
+
+// Lazy pair STDP of one plastic synapse i -> j (pair_stdp in the oracle is
+// the eager statement).  Applies, in time order, the potentiations of j's
+// spikes with steps in [L, T) -- pre trace from i's spike at tp with trace
+// a_s = xp (has_p), or 0 -- and with DEP the depression of i's spike at T,
+// post trace of j at T.  j has at most one spike at or after T (one spike
+// per batch), so its last spike before T is s1 or s0 of pj = pst[2 j]; s2, s3
+// come from pb = pst[2 j + 1] (both loaded by the caller, one line), older
+// spikes from the ring, and only when s3 itself is in [L, T).
+template <bool DEP>
+__device__ __forceinline__ float lazy_stdp(const StdpArgs& s, const TraceTabs& tt, uint32_t j, const uint4& pj,
+                                           const uint4& pb, float w, int32_t T, int32_t L, bool has_p, int32_t tp,
+                                           float xp)
+{
+    auto potentiate = [&](int32_t t2) {
+        float x = 0.0f;
+        if (has_p) {
+            if (t2 == tp) {
+                x = xp;
+            } else {
+                const float a = xp + 1.0f;
+                x = a * dec_pow2(tt.tp_l, tt.n_l, s.tp, s.n_tp, (uint32_t)(t2 - tp));
+            }
+        }
+        const float dw = s.a_plus * x;
+        w = clip(w + dw, s.w_min, s.w_max);
+    };
+    const bool has1 = pj.x != kNoSpike, has0 = pj.z != kNoSpike;
+    const int32_t s1 = (int32_t)pj.x, s0 = (int32_t)pj.z;
+    const bool pend1 = has1 && s1 >= T;  // j's newest spike is not before T
+    if (has0 && s0 >= L) {
+        // spikes older than s0 may lie in the window too: s2, s3 from the
+        // record's second half (same line), older ones from the ring; in order
+        const uint4& b = pb;
+        const uint32_t n = b.z;  // >= 2
+        if (n >= 3 && (int32_t)b.x >= L) {
+            if (n >= 4 && (int32_t)b.y >= L) {
+                const uint32_t* hj = s.hist + (size_t)j * kSpikeHist;
+                uint32_t lo = n - 4;  // s3
+                while (lo > 0 && lo + kSpikeHist > n && (int32_t)hj[(lo - 1) & (kSpikeHist - 1)] >= L)
+                    --lo;
+                for (uint32_t k = lo; k + 4 < n; ++k)
+                    potentiate((int32_t)hj[k & (kSpikeHist - 1)]);
+                potentiate((int32_t)b.y);
+            }
+            potentiate((int32_t)b.x);
+        }
+        potentiate(s0);
+    }
+    if (has1 && !pend1 && s1 >= L)
+        potentiate(s1);
+    if (DEP) {
+        // j's last spike before T
+        const bool has = pend1 ? has0 : has1;
+        float y = 0.0f;
+        if (has) {
+            const int32_t sy = pend1 ? s0 : s1;
+            const float a = __uint_as_float(pend1 ? pj.w : pj.y) + 1.0f;
+            y = a * dec_pow2(tt.tm_l, tt.n_l, s.tm, s.n_tm, (uint32_t)(T - sy));
+        }
+        const float dw = s.a_minus * y;
+        w = clip(w - dw, s.w_min, s.w_max);
+    }
+    return w;
+}
+
+// The pre-synaptic side of a spike of i at step t1, right after its batch's
+// update (lat[i] is that spike): potentiation is pending for post spikes from
+// L = max(i's previous spike, the last flush) on.
+struct PreSide {
+    int32_t t1, L, tp;
+    float xp;
+    bool has_p;
+};
+
+__device__ __forceinline__ PreSide pre_side(const StdpArgs& s, uint32_t i, uint32_t t1, unsigned long long t_flush)
+{
+    PreSide p;
+    const uint4 li = s.lat[i];
+    const uint32_t s0 = s.pst[2 * (size_t)i].z;
+    p.t1 = (int32_t)t1;
+    p.has_p = li.x >= 2;
+    p.tp = (int32_t)s0;
+    p.xp = __uint_as_float(li.w);
+    p.L = p.has_p ? max(p.tp, (int32_t)t_flush) : (int32_t)t_flush;
+    return p;
+}
+
 // Population of neuron j: populations are contiguous ranges; the host gives
 // the first population of every update block.
 __device__ __forceinline__ uint32_t find_pop(const PopParams* pops, uint32_t n_pops, uint32_t first,
@@ -257,6 +364,10 @@ __global__ __launch_bounds__(TW, (BUCKET && !PLASTIC && !POIS && TW == 1024) ? 8
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         st->cur = pl;
         st->t0_cur = t0;
+        // the flush kernel before this update applied the pending potentiation
+        // under the same condition
+        if (PLASTIC && t0 + a.B > st->t_flush + st->window)
+            st->t_flush = t0;
     }
     Stage sg{ s_list, &s_n, seg, a.seg_cap };
 
@@ -311,10 +422,12 @@ __global__ __launch_bounds__(TW, (BUCKET && !PLASTIC && !POIS && TW == 1024) ? 8
             float V = V_in;
             int32_t ref = ref_in;
             int32_t last = last_in;
-            float xh = 0.0f, yh = 0.0f;
+            uint4 lt = make_uint4(0u, 0u, 0u, 0u);  // newest two spikes (plastic nets)
+            uint4 ps = make_uint4(kNoSpike, 0u, kNoSpike, 0u);
+            bool lt_new = false;
             if (PLASTIC) {
-                xh = a.xh[j];
-                yh = a.yh[j];
+                lt = a.sd.lat[j];
+                ps = a.sd.pst[2 * (size_t)j];
             }
             // population constants in scalar registers for the whole batch
             const float P = rflf(pp.P), v_th = rflf(pp.v_th), v_reset = rflf(pp.v_reset);
@@ -348,14 +461,6 @@ __global__ __launch_bounds__(TW, (BUCKET && !PLASTIC && !POIS && TW == 1024) ? 8
                 }
                 if (a.probe && t - a.probe_t0 < a.probe_n)
                     a.probe[(size_t)(t - a.probe_t0) * a.N + j] = I;
-                if (PLASTIC) {
-                    // last < 0: never fired
-                    const float sp = (last >= 0 && (long long)last == (long long)t - 1) ? 1.0f : 0.0f;
-                    const float xa = xh + sp;
-                    xh = xa * a.dec_p;
-                    const float ya = yh + sp;
-                    yh = ya * a.dec_m;
-                }
                 bool fired = false;
                 if (ref > 0) {
                     --ref;
@@ -389,6 +494,25 @@ __global__ __launch_bounds__(TW, (BUCKET && !PLASTIC && !POIS && TW == 1024) ? 8
                         ref = n_ref;
                         last = (int32_t)t;
                         my_events += (uint32_t)(a.row_ptr[j + 1] - a.row_ptr[j]);
+                        if (PLASTIC) {
+                            // traces at t (spikes before t), from the previous spike
+                            const uint32_t ns = lt.x;
+                            float xa = 0.0f, ya = 0.0f;
+                            if (ns) {
+                                const uint32_t k = (uint32_t)t - lt.y;
+                                const float ax = __uint_as_float(lt.z) + 1.0f;
+                                xa = ax * dec_pow(a.sd.tp, a.sd.n_tp, k);
+                                const float ay = __uint_as_float(ps.y) + 1.0f;
+                                ya = ay * dec_pow(a.sd.tm, a.sd.n_tm, k);
+                            }
+                            a.sd.hist[(size_t)j * kSpikeHist + (ns & (kSpikeHist - 1))] = (uint32_t)t;
+                            uint4* pb = a.sd.pst + 2 * (size_t)j + 1;
+                            const uint4 b = *pb;
+                            *pb = make_uint4(ps.z, b.x, ns + 1u, 0u);  // s0 -> s2, s2 -> s3
+                            lt = make_uint4(ns + 1u, (uint32_t)t, __float_as_uint(xa), lt.z);
+                            ps = make_uint4((uint32_t)t, __float_as_uint(ya), ps.x, ps.y);
+                            lt_new = true;
+                        }
                     }
                 }
                 emit(sg, fired, j, (uint32_t)t, (uint32_t)(t0l + s) * a.dt_us);
@@ -396,9 +520,9 @@ __global__ __launch_bounds__(TW, (BUCKET && !PLASTIC && !POIS && TW == 1024) ? 8
             a.V[j] = V;
             a.ref[j] = ref;
             a.last[j] = last;
-            if (PLASTIC) {
-                a.xh[j] = xh;
-                a.yh[j] = yh;
+            if (PLASTIC && lt_new) {
+                a.sd.lat[j] = lt;
+                a.sd.pst[2 * (size_t)j] = ps;
             }
         } else if (POIS) {
             // POISSON source: the reference train (input_neuron_group.cl:26-39)
@@ -492,6 +616,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const ScatterArgs a)
     RingT* ring = (RingT*)a.ring;
     const uint32_t lane = lane_id();
     const unsigned long long chunk_off = (unsigned long long)blockIdx.y * kChunk;
+    const unsigned long long t_flush = PLASTIC ? st->t_flush : 0ull;
     {
         const uint32_t g = blockIdx.x;  // one block per list segment, its waves over the spikes
         const uint32_t cnt = counts[g];
@@ -504,6 +629,9 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const ScatterArgs a)
             if (beg >= re)
                 continue;
             const uint32_t tD = sp.step % a.D;
+            PreSide ps{};
+            if (PLASTIC)
+                ps = pre_side(a.sd, s, sp.step, t_flush);
             uint32_t c[kChunkUnroll];
             float wv[kChunkUnroll];
 #pragma unroll
@@ -521,8 +649,8 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const ScatterArgs a)
                     sl = sl >= a.D ? sl - a.D : sl;
                     float x = wv[u];
                     if (PLASTIC && (c[u] & kPlasticBit)) {
-                        const float dw = a.a_minus * a.yh[tgt];
-                        x = clip(x - dw, a.w_min, a.w_max);
+                        x = lazy_stdp<true>(a.sd, TraceTabs{ nullptr, nullptr, 0u }, tgt, a.sd.pst[2 * (size_t)tgt],
+                                            a.sd.pst[2 * (size_t)tgt + 1], x, ps.t1, ps.L, ps.has_p, ps.tp, ps.xp);
                         a.w[q] = x;
                     }
                     RingT* dst = ring + (size_t)sl * a.N + tgt;
@@ -537,51 +665,56 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const ScatterArgs a)
 }
 
 // ---------------------------------------------------------------------------
-// Potentiation at post-synaptic spikes through the reverse index; always the
-// last kernel of a plastic batch.
+// Flush of pending potentiation, the first kernel of a plastic batch: when
+// the batch would end more than `window` steps after the last flush (or
+// `force`), every plastic synapse i -> j takes the potentiations of j's
+// spikes since max(i's last spike, the last flush).  One wave per source row
+// (grid-stride); the update that follows records the new flush step under
+// the same condition.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_potentiate(const PotentiateArgs a)
+__global__ __launch_bounds__(kBlock) void k_flush(const FlushArgs a)
 {
-    DevState* st = a.st;
-    const uint32_t cur = st->cur;
-    const naer_spike* list = cur ? a.list1 : a.list0;
-    const uint32_t* counts = a.seg_count + cur * a.n_seg;
-    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0)
-        batch_epilogue(st, a.B);
+    const DevState* st = a.st;
+    const unsigned long long clock = st->clock, t_flush = st->t_flush;
+    if (!a.force && clock + a.B <= t_flush + st->window)
+        return;
     const uint32_t lane = lane_id();
-    const unsigned long long chunk_off = (unsigned long long)blockIdx.y * kChunk;
-    {
-        const uint32_t g = blockIdx.x;  // one block per list segment, its waves over the spikes
-        const uint32_t cnt = counts[g];
-        const naer_spike* seg = list + (size_t)g * a.seg_cap;
-        for (uint32_t i = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave); i < cnt; i += kBlock / kWave) {
-            const uint32_t post = seg[i].neuron;
-            const unsigned long long re = a.rev_ptr[post + 1];
-            const unsigned long long rb = a.rev_ptr[post] + chunk_off;
-            if (rb >= re)
-                continue;
-            // kChunkUnroll synapses per lane: index loads, then gathers, then updates
-            uint32_t pos[kChunkUnroll], src[kChunkUnroll];
+    const uint32_t wpb = kBlock / kWave;
+    const uint32_t waves = gridDim.x * wpb;
+    for (uint32_t i = blockIdx.x * wpb + rfl(threadIdx.x / kWave); i < a.N; i += waves) {
+        const unsigned long long rb = a.row_ptr[i], re = a.row_ptr[i + 1];
+        if (rb == re)
+            continue;
+        const uint4 li = a.sd.lat[i];
+        const bool has_p = li.x != 0;
+        const int32_t tp = (int32_t)li.y;
+        const float xp = __uint_as_float(li.z);
+        const int32_t L = has_p ? max(tp, (int32_t)t_flush) : (int32_t)t_flush;
+        constexpr uint32_t U = 4;  // synapses per lane in flight
+        for (unsigned long long q0 = rb; q0 < re; q0 += U * kWave) {
+            uint32_t c[U];
+            float w[U];
+            uint4 pj[U], pb[U];
 #pragma unroll
-            for (uint32_t u = 0; u < kChunkUnroll; ++u) {
-                const unsigned long long r = rb + u * kWave + lane;
-                pos[u] = r < re ? a.rev_pos[r] : 0u;
-                src[u] = r < re ? a.rev_src[r] : 0u;
-            }
-            float x[kChunkUnroll], w[kChunkUnroll];
-#pragma unroll
-            for (uint32_t u = 0; u < kChunkUnroll; ++u) {
-                const unsigned long long r = rb + u * kWave + lane;
-                x[u] = r < re ? a.xh[src[u]] : 0.0f;
-                w[u] = r < re ? a.w[pos[u]] : 0.0f;
+            for (uint32_t u = 0; u < U; ++u) {
+                const unsigned long long q = q0 + u * kWave + lane;
+                c[u] = q < re ? a.col[q] : 0u;
             }
 #pragma unroll
-            for (uint32_t u = 0; u < kChunkUnroll; ++u) {
-                const unsigned long long r = rb + u * kWave + lane;
-                if (r < re) {
-                    const float dw = a.a_plus * x[u];
-                    a.w[pos[u]] = clip(w[u] + dw, a.w_min, a.w_max);
-                }
+            for (uint32_t u = 0; u < U; ++u) {
+                const unsigned long long q = q0 + u * kWave + lane;
+                const bool pl = q < re && (c[u] & kPlasticBit);
+                const size_t r = 2 * (size_t)(c[u] & kTargetMask);
+                w[u] = pl ? a.w[q] : 0.0f;
+                pj[u] = pl ? a.sd.pst[r] : make_uint4(kNoSpike, 0u, kNoSpike, 0u);
+                pb[u] = pl ? a.sd.pst[r + 1] : make_uint4(kNoSpike, kNoSpike, 0u, 0u);
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u) {
+                const unsigned long long q = q0 + u * kWave + lane;
+                if (q < re && (c[u] & kPlasticBit))
+                    a.w[q] = lazy_stdp<false>(a.sd, TraceTabs{ nullptr, nullptr, 0u }, c[u] & kTargetMask, pj[u],
+                                              pb[u], w[u], 0x7FFFFFFF, L, has_p, tp, xp);
             }
         }
     }
@@ -664,19 +797,34 @@ constexpr uint32_t kRowRound = 1024;
 struct RowStage {
     unsigned long long beg[kRowRound];
     uint32_t len[kRowRound];
-    uint32_t r[kRowRound];  // spike step - first step of the batch
+    uint32_t r[kRowRound];    // spike step - first step of the batch
+    uint32_t nrn[kRowRound];  // the spiking neuron
 };
 
-// Calls f(r, q_begin, q_end) for every non-empty (64 U)-synapse chunk of the
+// Calls f(r, neuron, q_begin, q_end) for every non-empty (64 U)-synapse chunk of the
 // rows of the block's spikes; f runs on a whole wave (q range wave-uniform).
 // The block's segments are staged kMaxSegPerBlock at a time.
+// Workgroups are dealt to the 8 XCDs round-robin (block b to XCD b % 8);
+// xcd_block gives XCD x the x-th contiguous share of the logical blocks, so
+// the neighbouring segments (rows of one network, whose targets' STDP
+// records share lines) are walked under one L2.  A bijection of [0, nb).
+constexpr uint32_t kXcds = 8;
+
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb)
+{
+    const uint32_t x = b % kXcds, i = b / kXcds;
+    const uint32_t q = nb / kXcds, r = nb % kXcds;
+    return x * q + min(x, r) + i;
+}
+
 template <uint32_t U, typename F>
 __device__ __forceinline__ void for_block_chunks(const BucketArgs& a, uint32_t cur, uint32_t* s_off,
                                                  unsigned long long t0, RowStage& rs, F f)
 {
     constexpr uint32_t kTask = kWave * U;  // synapses of a row per task
     const naer_spike* list = cur ? a.list1 : a.list0;
-    const uint32_t seg_lo = a.blk_seg[blockIdx.x], seg_hi = a.blk_seg[blockIdx.x + 1];
+    const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t seg_lo = a.blk_seg[lb], seg_hi = a.blk_seg[lb + 1];
     const uint32_t nwv = blockDim.x / kWave;
     for (uint32_t w0 = seg_lo; w0 < seg_hi; w0 += kMaxSegPerBlock) {
         const uint32_t G = min(kMaxSegPerBlock, seg_hi - w0);
@@ -707,6 +855,7 @@ __device__ __forceinline__ void for_block_chunks(const BucketArgs& a, uint32_t c
                 rs.beg[i] = rb;
                 rs.len[i] = (uint32_t)(a.row_ptr[sp.neuron + 1] - rb);
                 rs.r[i] = (uint32_t)(sp.step - t0);
+                rs.nrn[i] = sp.neuron;
             }
             __syncthreads();
             const uint32_t n_tasks = cnt * a.C;
@@ -714,7 +863,7 @@ __device__ __forceinline__ void for_block_chunks(const BucketArgs& a, uint32_t c
                 const uint32_t i = t / a.C, c = t - i * a.C;
                 const uint32_t off = c * kTask, len = rs.len[i];
                 if (off < len)
-                    f(rs.r[i], rs.beg[i] + off, rs.beg[i] + min(len, off + kTask));
+                    f(rs.r[i], rs.nrn[i], rs.beg[i] + off, rs.beg[i] + min(len, off + kTask));
             }
             __syncthreads();
         }
@@ -737,7 +886,8 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_hist(const BucketArgs
     bucket_setup(a, lut_m, lut_o);
     __syncthreads();
     const uint32_t lane = lane_id();
-    for_block_chunks<U>(a, cur, s_off, t0, rs, [&](uint32_t r, unsigned long long beg, unsigned long long end) {
+    for_block_chunks<U>(a, cur, s_off, t0, rs, [&](uint32_t r, uint32_t, unsigned long long beg,
+                                                   unsigned long long end) {
         uint32_t c[U];
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u) {
@@ -872,7 +1022,7 @@ __global__ __launch_bounds__(1024) void k_bucket_starts(const BucketArgs a)
     }
 }
 
-template <bool DET, uint32_t U, bool AGG>
+template <bool DET, uint32_t U, bool AGG, bool PLASTIC>
 __global__ __launch_bounds__(kBucketThreads) void k_bucket_place(const BucketArgs a)
 {
     using RingT = typename std::conditional<DET, unsigned long long, float>::type;
@@ -895,13 +1045,25 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_place(const BucketArg
             s_cur[i] = bs[i << a.gt_shift] + h[i];
     }
     bucket_setup(a, lut_m, lut_o);
+    TraceTabs tt{ nullptr, nullptr, 0u };
+    if (PLASTIC) {
+        __shared__ float s_tp[kLdsTrace], s_tm[kLdsTrace];
+        const uint32_t n_l = min(kLdsTrace, min(a.sd.n_tp, a.sd.n_tm));
+        for (uint32_t i = threadIdx.x; i < n_l; i += blockDim.x) {
+            s_tp[i] = a.sd.tp[i];
+            s_tm[i] = a.sd.tm[i];
+        }
+        tt = TraceTabs{ s_tp, s_tm, n_l };
+    }
     __syncthreads();
     if (a.is_last && blockIdx.x == 0 && threadIdx.x == 0)
         batch_epilogue(st, a.B);
     uint2* stage = a.stage;
     RingT* ring = (RingT*)a.ring;
     const uint32_t lane = lane_id();
-    for_block_chunks<U>(a, cur, s_off, t0, rs, [&](uint32_t r, unsigned long long beg, unsigned long long end) {
+    const unsigned long long t_flush = PLASTIC ? st->t_flush : 0ull;
+    for_block_chunks<U>(a, cur, s_off, t0, rs, [&](uint32_t r, uint32_t nrn, unsigned long long beg,
+                                                   unsigned long long end) {
         uint32_t c[U];
         float w[U];
 #pragma unroll
@@ -909,6 +1071,29 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_place(const BucketArg
             const unsigned long long q = beg + u * kWave + lane;
             c[u] = q < end ? a.col[q] : 0u;
             w[u] = q < end ? a.w[q] : 0.0f;
+        }
+        if (PLASTIC) {
+            // lazy pair STDP of the row's plastic synapses before they are placed
+            const PreSide ps = pre_side(a.sd, nrn, (uint32_t)(t0 + r), t_flush);
+            // the targets' newest two spikes for all U synapses first, then the walks
+            uint4 pj[U], pb[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u) {
+                const unsigned long long q = beg + u * kWave + lane;
+                const bool pl = q < end && (c[u] & kPlasticBit);
+                const size_t rj = 2 * (size_t)(c[u] & kTargetMask);
+                pj[u] = pl ? a.sd.pst[rj] : make_uint4(kNoSpike, 0u, kNoSpike, 0u);
+                pb[u] = pl ? a.sd.pst[rj + 1] : make_uint4(kNoSpike, kNoSpike, 0u, 0u);
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u) {
+                const unsigned long long q = beg + u * kWave + lane;
+                if (q < end && (c[u] & kPlasticBit)) {
+                    w[u] = lazy_stdp<true>(a.sd, tt, c[u] & kTargetMask, pj[u], pb[u], w[u], ps.t1, ps.L, ps.has_p,
+                                           ps.tp, ps.xp);
+                    a.w[q] = w[u];
+                }
+            }
         }
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u) {
@@ -1192,9 +1377,10 @@ __global__ __launch_bounds__(kBlock) void k_init_state(const UpdateArgs a, uint3
     a.V[j] = v;
     a.ref[j] = 0;
     a.last[j] = -1;
-    if (a.xh) {
-        a.xh[j] = 0.0f;
-        a.yh[j] = 0.0f;
+    if (a.sd.lat) {
+        a.sd.lat[j] = make_uint4(0u, 0u, 0u, 0u);
+        a.sd.pst[2 * (size_t)j] = make_uint4(kNoSpike, 0u, kNoSpike, 0u);
+        a.sd.pst[2 * (size_t)j + 1] = make_uint4(kNoSpike, kNoSpike, 0u, 0u);
     }
     a.pnext[j] = 0;
     a.pcnt[j] = 0;
@@ -1279,7 +1465,9 @@ hipError_t launch_update(const UpdateArgs& a, bool det, bool plastic, bool bucke
 {
     if (a.tw_shift < 8 || a.tw_shift > 10)
         return hipErrorInvalidValue;
-    if (plastic)  // plastic nets deliver through the ring only
+    if (plastic && bucket)
+        return det ? launch_update_tw<true, true, true>(a, pois, s) : launch_update_tw<false, true, true>(a, pois, s);
+    if (plastic)
         return det ? launch_update_tw<true, true, false>(a, pois, s) : launch_update_tw<false, true, false>(a, pois, s);
     if (bucket)
         return det ? launch_update_tw<true, false, true>(a, pois, s) : launch_update_tw<false, false, true>(a, pois, s);
@@ -1315,6 +1503,15 @@ static hipError_t bucket_lds_attributes(const BucketArgs& a)
     return hipSuccess;
 }
 
+template <bool DET, uint32_t U, bool AGG>
+static void launch_place(const BucketArgs& a, bool plastic, size_t lds_c, hipStream_t s)
+{
+    if (plastic)
+        hipLaunchKernelGGL((k_bucket_place<DET, U, AGG, true>), dim3(a.NB), dim3(kBucketThreads), lds_c, s, a);
+    else
+        hipLaunchKernelGGL((k_bucket_place<DET, U, AGG, false>), dim3(a.NB), dim3(kBucketThreads), lds_c, s, a);
+}
+
 template <bool AGG>
 static void launch_bucket_pass(const BucketArgs& a, int step, bool det, size_t lds, size_t lds_c, hipStream_t s)
 {
@@ -1328,16 +1525,18 @@ static void launch_bucket_pass(const BucketArgs& a, int step, bool det, size_t l
         else
             hipLaunchKernelGGL((k_bucket_hist<8, AGG>), grid, block, lds, s, a);
         break;
-    case 2:
+    case 2: {
+        const bool pl = a.sd.lat != nullptr;
         if (det && a.U == 4)
-            hipLaunchKernelGGL((k_bucket_place<true, 4, AGG>), grid, block, lds_c, s, a);
+            launch_place<true, 4, AGG>(a, pl, lds_c, s);
         else if (det)
-            hipLaunchKernelGGL((k_bucket_place<true, 8, AGG>), grid, block, lds_c, s, a);
+            launch_place<true, 8, AGG>(a, pl, lds_c, s);
         else if (a.U == 4)
-            hipLaunchKernelGGL((k_bucket_place<false, 4, AGG>), grid, block, lds_c, s, a);
+            launch_place<false, 4, AGG>(a, pl, lds_c, s);
         else
-            hipLaunchKernelGGL((k_bucket_place<false, 8, AGG>), grid, block, lds_c, s, a);
+            launch_place<false, 8, AGG>(a, pl, lds_c, s);
         break;
+    }
     default:
         hipLaunchKernelGGL(k_bucket_refine<AGG>, dim3((a.NBK >> a.gt_shift) * a.RS), dim3(kRefineThreads), 0, s, a);
         break;
@@ -1408,9 +1607,9 @@ hipError_t launch_scatter(const ScatterArgs& a, bool det, bool plastic, uint32_t
     return hipGetLastError();
 }
 
-hipError_t launch_potentiate(const PotentiateArgs& a, uint32_t grid_x, uint32_t chunks, hipStream_t s)
+hipError_t launch_flush(const FlushArgs& a, uint32_t grid, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_potentiate, dim3(grid_x, chunks), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_flush, dim3(grid), dim3(kBlock), 0, s, a);
     return hipGetLastError();
 }
 

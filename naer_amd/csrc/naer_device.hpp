@@ -121,7 +121,33 @@ struct DevState {
     uint32_t batch;             // batches done; the update writes list[batch & 1]
     uint32_t cur;               // list of the batch in flight
     uint32_t overflow;          // spikes lost to a full list segment
-    uint32_t _pad;
+    uint32_t window;            // plastic nets: steps after a flush within which no neuron fires kSpikeHist times
+    unsigned long long t_flush; // plastic nets: potentiation is applied for every post spike before this step
+};
+
+// Pair STDP with potentiation applied lazily (as update_synapse does at the
+// next pre-synaptic input, stdp.cl:19-26): a plastic synapse i -> j is
+// touched only when i fires, and then first takes the potentiations of j's
+// spikes since i's previous spike (or the last flush), in time order, then
+// its depression.  Every neuron keeps its last kSpikeHist spikes; a flush
+// (all plastic synapses) runs before any neuron can have fired kSpikeHist - 1
+// times since the last one.  Traces in closed form: the trace of neuron j at
+// step t (spikes before t) is (a_s + 1) * dec^(t - s) for its last spike s < t,
+// a_s = its trace at s; dec^k from a table (0 once below 2^-40).
+constexpr uint32_t kSpikeHist = 16;
+
+constexpr uint32_t kNoSpike = 0x80000000u;  // step of an absent spike
+
+struct StdpArgs {
+    // per neuron, its newest spike s1 and the one before, s0 (kNoSpike if absent)
+    uint4* lat;          // [N] pre side: {spikes so far, s1, pre trace a_s(s1) bits, pre trace a_s(s0) bits}
+    uint4* pst;          // [N][2] post side: {s1, post trace a_s(s1) bits, s0, post trace a_s(s0) bits},
+                         // {s2, s3 (the two spikes before s0), spikes so far, 0}: one 32-byte line part
+    uint32_t* hist;      // [N][kSpikeHist] step of spike k at k % kSpikeHist
+    const float* tp;     // [n_tp] exp(-k dt / tau_plus)
+    const float* tm;     // [n_tm] exp(-k dt / tau_minus)
+    uint32_t n_tp, n_tm;
+    float a_plus, a_minus, w_min, w_max;
 };
 
 // Spike lists are segmented by update block: block b of the update kernel
@@ -132,8 +158,7 @@ struct UpdateArgs {
     float* V;
     int32_t* ref;
     int32_t* last;
-    float* xh;
-    float* yh;
+    StdpArgs sd;                       // plastic nets
     uint32_t* pnext;
     uint32_t* pcnt;
     void* ring;
@@ -159,7 +184,6 @@ struct UpdateArgs {
     uint32_t B;
     uint32_t dt_us;
     uint32_t n_pops;
-    float dec_p, dec_m;
     uint32_t tw_shift;   // update block = one tile of 1 << tw_shift neurons
     uint32_t Bfull;      // steps of an aligned batch
     // bucketed delivery (see BucketArgs): events of earlier batches sorted by
@@ -198,7 +222,8 @@ struct BucketArgs {
     DevState* st;
     const unsigned long long* row_ptr;
     const uint32_t* col;
-    const float* w;
+    float* w;
+    StdpArgs sd;       // plastic nets: lazy pair STDP in the placement pass
     uint32_t* hist;    // hist_f [NB][NBK]
     uint32_t* hist_c;  // [NB][NBK >> gt_shift]
     uint32_t* totals;  // [NBK]
@@ -234,8 +259,7 @@ struct ScatterArgs {
     const uint32_t* col;
     float* w;
     void* ring;
-    const float* yh;
-    float a_minus, w_min, w_max;
+    StdpArgs sd;
     uint32_t N, D;
     uint32_t B;
     uint32_t is_last;
@@ -243,19 +267,17 @@ struct ScatterArgs {
     uint32_t R, Bfull;
 };
 
-struct PotentiateArgs {
-    const naer_spike* list0;
-    const naer_spike* list1;
-    const uint32_t* seg_count;
-    uint32_t seg_cap, n_seg;
+// Flush of pending potentiation over every plastic synapse (rows of the
+// sources [0, N)); `force` flushes regardless of the window.
+struct FlushArgs {
     DevState* st;
-    const unsigned long long* rev_ptr;
-    const uint32_t* rev_pos;
-    const uint32_t* rev_src;
+    const unsigned long long* row_ptr;
+    const uint32_t* col;
     float* w;
-    const float* xh;
-    float a_plus, w_min, w_max;
-    uint32_t B;
+    StdpArgs sd;
+    uint32_t N;
+    uint32_t B;        // steps of the batch that follows
+    uint32_t force;
 };
 
 // One connection of the device connectivity build.

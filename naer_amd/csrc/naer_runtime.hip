@@ -162,7 +162,13 @@ struct naer_net {
     std::vector<void*> allocs;
     uint64_t dev_bytes = 0;
 
-    float *V = nullptr, *xh = nullptr, *yh = nullptr;
+    float* V = nullptr;
+    // lazy pair STDP (plastic nets): spike history and the traces' dec^k tables
+    uint4 *lat = nullptr, *pst = nullptr;
+    uint32_t* shist = nullptr;
+    float *d_tp = nullptr, *d_tm = nullptr;
+    std::vector<float> h_tp, h_tm;
+    uint32_t nref_min = 0;  // fewest refractory steps of any LIF population
     int32_t *ref = nullptr, *last = nullptr;
     uint32_t *pnext = nullptr, *pcnt = nullptr;
     void* ring = nullptr;
@@ -502,6 +508,50 @@ int upload_table(naer_net* n)
     return n->drop_graph();
 }
 
+// dec^k = exp(-k dt / tau) for k = 0, 1, ... while it is >= 2^-40 (at most
+// kMaxTraceSteps entries), each rounded to fp32; beyond the table the
+// traces take dec^k = 0.  The oracle builds the same table.
+constexpr uint32_t kMaxTraceSteps = 1u << 22;
+
+std::vector<float> trace_table(float dt_ms, float tau_ms)
+{
+    std::vector<float> t;
+    for (uint32_t k = 0; k < kMaxTraceSteps; ++k) {
+        const double v = std::exp(-(double)k * (double)dt_ms / (double)tau_ms);
+        if (v < 0x1p-40)
+            break;
+        t.push_back((float)v);
+    }
+    return t;
+}
+
+StdpArgs stdp_args(const naer_net* n)
+{
+    StdpArgs a;
+    std::memset(&a, 0, sizeof a);
+    if (!n->plastic)
+        return a;
+    a.lat = n->lat;
+    a.pst = n->pst;
+    a.hist = n->shist;
+    a.tp = n->d_tp;
+    a.tm = n->d_tm;
+    a.n_tp = (uint32_t)n->h_tp.size();
+    a.n_tm = (uint32_t)n->h_tm.size();
+    a.a_plus = n->stdp.a_plus;
+    a.a_minus = n->stdp.a_minus;
+    a.w_min = n->stdp.w_min;
+    a.w_max = n->stdp.w_max;
+    return a;
+}
+
+// Steps after a flush of pending potentiation within which no neuron can
+// fire kSpikeHist times (its spikes are >= nref_min + 1 steps apart).
+uint32_t flush_window(const naer_net* n)
+{
+    return (kSpikeHist - 1) * (n->nref_min + 1);
+}
+
 UpdateArgs make_update_args(naer_net* n)
 {
     UpdateArgs a;
@@ -509,8 +559,7 @@ UpdateArgs make_update_args(naer_net* n)
     a.V = n->V;
     a.ref = n->ref;
     a.last = n->last;
-    a.xh = n->xh;
-    a.yh = n->yh;
+    a.sd = stdp_args(n);
     a.pnext = n->pnext;
     a.pcnt = n->pcnt;
     a.ring = n->ring;
@@ -537,10 +586,6 @@ UpdateArgs make_update_args(naer_net* n)
     a.B = n->B;
     a.dt_us = n->dt_us;
     a.n_pops = (uint32_t)n->pp.size();
-    if (n->plastic) {
-        a.dec_p = (float)std::exp(-(double)n->dt_ms / (double)n->stdp.tau_plus_ms);
-        a.dec_m = (float)std::exp(-(double)n->dt_ms / (double)n->stdp.tau_minus_ms);
-    }
     a.tw_shift = n->tw_shift;
     a.Bfull = n->B;
     a.ev = n->ev;
@@ -679,8 +724,9 @@ int init_net(naer_net* n, const NetInfo& net)
 }
 
 // Steps per update launch and update tile width.  The inputs of a batch
-// were all delivered before it starts when B <= the minimum delay; plastic
-// nets interleave delivery and potentiation every step.  Large nets (N >=
+// were all delivered before it starts when B <= the minimum delay; in plastic
+// nets B <= the fewest refractory steps + 1, so that a neuron fires at most
+// once per batch (lazy STDP).  Large nets (N >=
 // 2^18) use 1024-neuron tiles; when the minimum delay allows fewer than
 // kLongBatch steps per batch, B <= kMaxBatchLargeTile keeps the update's
 // [B][1024] 64-bit LDS accumulators of two tiles within one CU (measured:
@@ -688,11 +734,11 @@ int init_net(naer_net* n, const NetInfo& net)
 // while longer batches keep their length (fewer batches, less fixed cost).
 constexpr uint32_t kLongBatch = 12;
 
-uint32_t batch_steps(bool plastic, uint32_t dmin, uint64_t N)
+uint32_t batch_steps(bool plastic, uint32_t dmin, uint64_t N, uint32_t nref_min)
 {
-    if (plastic)
-        return 1;
     uint32_t b = std::min<uint32_t>(kMaxBatch, dmin == 0xFFFFFFFFu ? kMaxBatch : dmin);
+    if (plastic)  // lazy STDP: every neuron fires at most once per batch
+        b = std::min(b, nref_min + 1);
     if (N >= (1u << 18) && b < kLongBatch)
         b = std::min(b, kMaxBatchLargeTile);
     return std::max(b, 1u);
@@ -714,9 +760,19 @@ int alloc_neurons(naer_net* n, bool any_poisson)
         return rc;
     HIP_TRY(hipMemsetAsync(n->row_ptr, 0, ((size_t)N + 1) * sizeof(unsigned long long), n->stream));
     if (n->plastic) {
-        if ((rc = n->alloc(&n->rev_ptr, (size_t)N + 1)) || (rc = n->alloc(&n->xh, N)) || (rc = n->alloc(&n->yh, N)))
+        n->h_tp = trace_table(n->dt_ms, n->stdp.tau_plus_ms);
+        n->h_tm = trace_table(n->dt_ms, n->stdp.tau_minus_ms);
+        if ((rc = n->alloc(&n->rev_ptr, (size_t)N + 1)) || (rc = n->alloc(&n->lat, N)) ||
+            (rc = n->alloc(&n->pst, 2 * (size_t)N)) || (rc = n->alloc(&n->shist, (size_t)N * kSpikeHist)) ||
+            (rc = n->alloc(&n->d_tp, std::max<size_t>(1, n->h_tp.size()))) ||
+            (rc = n->alloc(&n->d_tm, std::max<size_t>(1, n->h_tm.size()))))
             return rc;
         HIP_TRY(hipMemsetAsync(n->rev_ptr, 0, ((size_t)N + 1) * sizeof(unsigned long long), n->stream));
+        HIP_TRY(hipMemsetAsync(n->lat, 0, (size_t)N * sizeof(uint4), n->stream));
+        // every word kNoSpike: steps absent (the trace words are never read then)
+        HIP_TRY(hipMemsetD32Async((hipDeviceptr_t)n->pst, (int)kNoSpike, (size_t)N * 8, n->stream));
+        HIP_TRY(hipMemcpy(n->d_tp, n->h_tp.data(), n->h_tp.size() * sizeof(float), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(n->d_tm, n->h_tm.data(), n->h_tm.size() * sizeof(float), hipMemcpyHostToDevice));
     }
     if ((rc = n->alloc(&n->V, N)) || (rc = n->alloc(&n->ref, N)) || (rc = n->alloc(&n->last, N)) ||
         (rc = n->alloc(&n->pnext, N)) || (rc = n->alloc(&n->pcnt, N)))
@@ -746,7 +802,10 @@ int alloc_neurons(naer_net* n, bool any_poisson)
     HIP_TRY(hipMemsetAsync(n->ev_part, 0, n->n_seg * sizeof(unsigned long long), n->stream));
     if ((rc = n->alloc(&n->rec, n->rec_cap ? n->rec_cap : 1)) || (rc = n->alloc(&n->st, 1)))
         return rc;
-    HIP_TRY(hipMemsetAsync(n->st, 0, sizeof(DevState), n->stream));
+    DevState st0;
+    std::memset(&st0, 0, sizeof st0);
+    st0.window = flush_window(n);
+    HIP_TRY(hipMemcpy(n->st, &st0, sizeof st0, hipMemcpyHostToDevice));
     return NAER_OK;
 }
 
@@ -794,7 +853,7 @@ int set_block_ranges(naer_net* n)
 int alloc_buckets(naer_net* n)
 {
     n->bucketed = false;
-    if (n->plastic || n->B < 2 || n->S_cap == 0)
+    if (n->B < 2 || n->S_cap == 0)
         return NAER_OK;
     n->M = (n->B - 1 + n->dmax) / n->B;
     n->gt_shift = kGroupNeuronShift - n->tw_shift;
@@ -907,6 +966,34 @@ int naer_ctx_destroy(naer_ctx* ctx)
     return NAER_OK;
 }
 
+// Fewest refractory steps of the LIF populations of the active networks.
+uint32_t min_refractory(const naer_net* n)
+{
+    int64_t m = -1;
+    for (const auto& net : n->nets) {
+        if (!net.active)
+            continue;
+        for (const auto& q : net.pops) {
+            if (q.kind != NAER_POP_LIF)
+                continue;
+            const int64_t r = std::max<int64_t>(0, std::llround((double)q.t_ref_ms / (double)n->dt_ms));
+            m = m < 0 ? r : std::min(m, r);
+        }
+    }
+    return m < 0 ? 0u : (uint32_t)std::min<int64_t>(m, 1 << 20);
+}
+
+// An arena's flush window follows its active jobs (called between steps).
+int set_flush_window(naer_net* n)
+{
+    if (!n->plastic)
+        return NAER_OK;
+    n->nref_min = min_refractory(n);
+    const uint32_t w = flush_window(n);
+    HIP_TRY(hipMemcpy((char*)n->st + offsetof(DevState, window), &w, sizeof w, hipMemcpyHostToDevice));
+    return NAER_OK;
+}
+
 int naer_pack_create(naer_ctx* ctx, const naer_net_desc* descs, uint32_t n_nets, naer_net** out)
 {
     if (!ctx || !descs || !out || n_nets == 0)
@@ -943,7 +1030,8 @@ int naer_pack_create(naer_ctx* ctx, const naer_net_desc* descs, uint32_t n_nets,
     if (rc == NAER_OK) {
         n->N = (uint32_t)N;
         n->D = n->dmax + 1;
-        n->B = batch_steps(n->plastic, n->dmin, n->N);
+        n->nref_min = min_refractory(n);
+        n->B = batch_steps(n->plastic, n->dmin, n->N, n->nref_min);
         n->rec_cap = rec;
         rc = make_stream(n);
     }
@@ -1083,7 +1171,10 @@ int naer_arena_create(naer_ctx* ctx, const naer_arena_desc* a, naer_net** out)
         n->dmin = a->delay_min;
         n->dmax = a->delay_max;
         n->D = n->dmax + 1;
-        n->B = batch_steps(n->plastic, n->dmin, n->N);
+        // jobs arrive later: one step per batch; the flush window follows
+        // the active jobs' refractory periods (set at admission)
+        n->nref_min = 0;
+        n->B = batch_steps(n->plastic, n->dmin, n->N, 0);
         n->max_row = a->max_row;
         n->rec_cap = a->record_capacity;
         n->neuron_alloc.reset(n->N);
@@ -1204,6 +1295,8 @@ int naer_arena_admit(naer_net* n, const naer_net_desc* desc, uint32_t* slot)
         return undo(rc);
     }
     HIP_TRY(hipStreamSynchronize(n->stream));
+    if ((rc = set_flush_window(n)))
+        return rc;
     *slot = s;
     return NAER_OK;
 }
@@ -1226,7 +1319,9 @@ int naer_arena_retire(naer_net* n, uint32_t slot)
     if (n->plastic)
         n->rev_alloc.give(net.rev_begin, net.n_rev);
     net.n_syn = net.n_rev = 0;
-    return upload_table(n);
+    if ((rc = upload_table(n)))
+        return rc;
+    return set_flush_window(n);
 }
 
 static hipError_t do_update(naer_net* n, void* arg)
@@ -1241,10 +1336,49 @@ static hipError_t do_scatter(naer_net* n, void* arg)
     return launch_scatter(*a, n->det, n->plastic, n->n_seg, n->scatter_chunks, n->stream);
 }
 
-static hipError_t do_potentiate(naer_net* n, void* arg)
+FlushArgs flush_args(const naer_net* n, bool force)
 {
-    PotentiateArgs* a = (PotentiateArgs*)arg;
-    return launch_potentiate(*a, n->n_seg, std::max<uint32_t>(1, n->rev_chunks), n->stream);
+    FlushArgs f;
+    std::memset(&f, 0, sizeof f);
+    f.st = n->st;
+    f.row_ptr = n->row_ptr;
+    f.col = n->col;
+    f.w = n->w;
+    f.sd = stdp_args(n);
+    f.N = n->N;
+    f.B = n->B;
+    f.force = force ? 1u : 0u;
+    return f;
+}
+
+// grid-stride: about 8 rows per wave, at most 4096 blocks
+static uint32_t flush_grid(const naer_net* n)
+{
+    const uint32_t wpb = kBlock / kWave;
+    return std::max<uint32_t>(1, std::min<uint64_t>(4096, ((uint64_t)n->N + 8 * wpb - 1) / (8 * wpb)));
+}
+
+static hipError_t do_flush(naer_net* n, void* arg)
+{
+    return launch_flush(*(const FlushArgs*)arg, flush_grid(n), n->stream);
+}
+
+// Applies all pending potentiation now (before weights are read or saved)
+// and records the flush step on the device.
+int flush_now(naer_net* n)
+{
+    if (!n->plastic || !n->built)
+        return NAER_OK;
+    int rc = naer_net_sync(n);
+    if (rc)
+        return rc;
+    const FlushArgs fa = flush_args(n, true);
+    HIP_TRY(launch_flush(fa, flush_grid(n), n->stream));
+    HIP_TRY(hipStreamSynchronize(n->stream));
+    DevState st;
+    HIP_TRY(hipMemcpy(&st, n->st, sizeof st, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy((char*)n->st + offsetof(DevState, t_flush), &st.clock, sizeof st.clock, hipMemcpyHostToDevice));
+    return NAER_OK;
 }
 
 struct BucketStep {
@@ -1276,13 +1410,10 @@ static int enqueue_steps(naer_net* n, uint32_t n_steps, bool timed)
     sa.col = n->col;
     sa.w = n->w;
     sa.ring = n->ring;
-    sa.yh = n->yh;
-    sa.a_minus = n->stdp.a_minus;
-    sa.w_min = n->stdp.w_min;
-    sa.w_max = n->stdp.w_max;
+    sa.sd = stdp_args(n);
     sa.N = n->N;
     sa.D = n->D;
-    sa.is_last = n->plastic ? 0u : 1u;
+    sa.is_last = 1;
     sa.flags = n->bucketed ? n->flags : nullptr;
     sa.R = n->R;
     sa.Bfull = n->B;
@@ -1299,6 +1430,7 @@ static int enqueue_steps(naer_net* n, uint32_t n_steps, bool timed)
     ba.row_ptr = n->row_ptr;
     ba.col = n->col;
     ba.w = n->w;
+    ba.sd = stdp_args(n);
     ba.hist = n->hist;
     ba.hist_c = n->hist_c;
     ba.stage = n->stage;
@@ -1327,30 +1459,19 @@ static int enqueue_steps(naer_net* n, uint32_t n_steps, bool timed)
     ba.AG = n->AG;
     ba.balance = n->balance ? 1u : 0u;
     ba.is_last = 1;
-    PotentiateArgs pa;
-    std::memset(&pa, 0, sizeof pa);
-    pa.list0 = n->list0;
-    pa.list1 = n->list1;
-    pa.seg_count = n->seg_count;
-    pa.seg_cap = n->seg_cap;
-    pa.n_seg = n->n_seg;
-    pa.st = n->st;
-    pa.rev_ptr = n->rev_ptr;
-    pa.rev_pos = n->rev_pos;
-    pa.rev_src = n->rev_src;
-    pa.w = n->w;
-    pa.xh = n->xh;
-    pa.a_plus = n->stdp.a_plus;
-    pa.w_min = n->stdp.w_min;
-    pa.w_max = n->stdp.w_max;
+    FlushArgs fa = flush_args(n, false);
     int rc;
     while (n_steps > 0) {
         const uint32_t off = (uint32_t)(n->steps_enqueued % n->B);
         const uint32_t b = std::min(n_steps, n->B - off);
         ua.B = b;
         sa.B = b;
-        pa.B = b;
         ba.B = b;
+        fa.B = b;
+        // plastic nets: pending potentiation is flushed (on the device's
+        // condition) before a batch that would end outside the window
+        if (n->plastic && (rc = launch_timed(n, NAER_K_POTENTIATE, do_flush, &fa, timed)))
+            return rc;
         if ((rc = launch_timed(n, NAER_K_UPDATE, do_update, &ua, timed)))
             return rc;
         if (n->bucketed && off == 0 && b == n->B) {
@@ -1364,8 +1485,6 @@ static int enqueue_steps(naer_net* n, uint32_t n_steps, bool timed)
         } else if ((rc = launch_timed(n, NAER_K_SCATTER, do_scatter, &sa, timed))) {
             return rc;
         }
-        if (n->plastic && (rc = launch_timed(n, NAER_K_POTENTIATE, do_potentiate, &pa, timed)))
-            return rc;
         n_steps -= b;
         n->steps_enqueued += b;
     }
@@ -1737,7 +1856,7 @@ int naer_net_raster_close(naer_net* n, uint64_t* n_total)
 // --- checkpoint / resume ---------------------------------------------------
 namespace {
 
-constexpr char kCheckpointMagic[8] = { 'N', 'A', 'E', 'R', 'C', 'K', 'P', '1' };
+constexpr char kCheckpointMagic[8] = { 'N', 'A', 'E', 'R', 'C', 'K', 'P', '2' };
 
 // Structure of a net that a checkpoint must match to be loaded.
 struct CheckpointShape {
@@ -1802,7 +1921,8 @@ int checkpoint_body(naer_net* n, CheckpointIO& io)
     int rc;
     if ((rc = io.copy(n->V, N * 4)) || (rc = io.copy(n->ref, N * 4)) || (rc = io.copy(n->last, N * 4)))
         return rc;
-    if (n->plastic && ((rc = io.copy(n->xh, N * 4)) || (rc = io.copy(n->yh, N * 4)) ||
+    if (n->plastic && ((rc = io.copy(n->lat, N * sizeof(uint4))) || (rc = io.copy(n->pst, 2 * N * sizeof(uint4))) ||
+                       (rc = io.copy(n->shist, N * kSpikeHist * sizeof(uint32_t))) ||
                        (rc = io.copy(n->w, n->S_cap * 4))))
         return rc;
     if (n->pnext && ((rc = io.copy(n->pnext, N * 4)) || (rc = io.copy(n->pcnt, N * 4))))
@@ -1906,17 +2026,34 @@ int naer_net_read_state(naer_net* n, float* v, int32_t* refr, float* pre, float*
         HIP_TRY(hipMemcpy(v, n->V, N * sizeof(float), hipMemcpyDeviceToHost));
     if (refr)
         HIP_TRY(hipMemcpy(refr, n->ref, N * sizeof(int32_t), hipMemcpyDeviceToHost));
-    if (pre) {
-        if (n->xh)
-            HIP_TRY(hipMemcpy(pre, n->xh, N * sizeof(float), hipMemcpyDeviceToHost));
-        else
-            std::memset(pre, 0, N * sizeof(float));
-    }
-    if (post) {
-        if (n->yh)
-            HIP_TRY(hipMemcpy(post, n->yh, N * sizeof(float), hipMemcpyDeviceToHost));
-        else
-            std::memset(post, 0, N * sizeof(float));
+    if (pre)
+        std::memset(pre, 0, N * sizeof(float));
+    if (post)
+        std::memset(post, 0, N * sizeof(float));
+    if ((pre || post) && n->plastic) {
+        // the traces the next step sees: (a_s + 1) dec^(clock - s) from the last spike s
+        std::vector<uint4> lat(N), pst(2 * N);
+        DevState st;
+        HIP_TRY(hipMemcpy(lat.data(), n->lat, N * sizeof(uint4), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(pst.data(), n->pst, 2 * N * sizeof(uint4), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(&st, n->st, sizeof st, hipMemcpyDeviceToHost));
+        auto dec = [](const std::vector<float>& t, uint64_t k) { return k < t.size() ? t[k] : 0.0f; };
+        for (size_t j = 0; j < N; ++j) {
+            if (!lat[j].x)
+                continue;
+            const uint64_t k = st.clock - lat[j].y;
+            float a;
+            if (pre) {
+                std::memcpy(&a, &lat[j].z, 4);
+                const float b = a + 1.0f;
+                pre[j] = b * dec(n->h_tp, k);
+            }
+            if (post) {
+                std::memcpy(&a, &pst[2 * j].y, 4);
+                const float b = a + 1.0f;
+                post[j] = b * dec(n->h_tm, k);
+            }
+        }
     }
     return NAER_OK;
 }
@@ -1934,6 +2071,9 @@ int naer_net_read_csr(naer_net* n, uint64_t* row_ptr, uint32_t* col, float* w)
         HIP_TRY(hipMemcpy(row_ptr, n->row_ptr, ((size_t)n->N + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost));
     if (col && n->S_cap)
         HIP_TRY(hipMemcpy(col, n->col, n->S_cap * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    // plastic weights take their pending potentiation first
+    if (w && n->S_cap && (rc = flush_now(n)))
+        return rc;
     if (w && n->S_cap)
         HIP_TRY(hipMemcpy(w, n->w, n->S_cap * sizeof(float), hipMemcpyDeviceToHost));
     return NAER_OK;

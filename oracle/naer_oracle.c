@@ -197,13 +197,17 @@ struct orc_net {
     double* Vd;
     int32_t* ref;
     int32_t* last;
+    /* pair STDP: a neuron's traces at step t count its spikes before t,
+     * x(t) = (a_s + 1) dec^(t - s) for its last spike s < t, where a_s is the
+     * trace at s; xh/yh (xhd/yhd) hold a_s of the last spike (pre / post). */
     float *xh, *yh;
     double *xhd, *yhd;
     uint32_t* pnext;
     uint32_t* pcnt;
     int has_plastic;
-    float dec_p, dec_m;
-    double dec_pd, dec_md;
+    float *tp, *tm; /* dec^k in fp32 while exp(-k dt / tau) >= 2^-40 */
+    uint32_t n_tp, n_tm;
+    double dt_d, tau_pd, tau_md;
 
     uint32_t D;
     int64_t* ring_fx;
@@ -233,6 +237,10 @@ struct orc_net {
     uint32_t** tsp_time;
     uint64_t* tsp_n;
 };
+
+static float* dec_table(float dt_ms, float tau_ms, uint32_t* len);
+static float trace_f(const orc_net* n, int pre, uint32_t j, uint64_t t);
+static double trace_d(const orc_net* n, int pre, uint32_t j, uint64_t t);
 
 static void* xcalloc(size_t n, size_t sz)
 {
@@ -267,6 +275,8 @@ void orc_net_destroy(orc_net* n)
     free(n->last);
     free(n->xh);
     free(n->yh);
+    free(n->tp);
+    free(n->tm);
     free(n->xhd);
     free(n->yhd);
     free(n->pnext);
@@ -440,12 +450,34 @@ orc_net* orc_net_create(const naer_net_desc* d, int precision, int threads)
     n->D = dmax + 1;
     if (n->has_plastic) {
         const naer_stdp_desc* s = &d->stdp;
-        n->dec_pd = exp(-(double)d->dt_ms / (double)s->tau_plus_ms);
-        n->dec_md = exp(-(double)d->dt_ms / (double)s->tau_minus_ms);
-        n->dec_p = (float)n->dec_pd;
-        n->dec_m = (float)n->dec_md;
+        n->dt_d = (double)d->dt_ms;
+        n->tau_pd = (double)s->tau_plus_ms;
+        n->tau_md = (double)s->tau_minus_ms;
+        n->tp = dec_table(d->dt_ms, s->tau_plus_ms, &n->n_tp);
+        n->tm = dec_table(d->dt_ms, s->tau_minus_ms, &n->n_tm);
     }
     return n;
+}
+
+/* dec^k = exp(-k dt / tau) rounded to fp32, for k = 0, 1, ... while the
+ * exact value is >= 2^-40 (at most 2^22 entries); past the table the fp32
+ * traces take dec^k = 0. */
+static float* dec_table(float dt_ms, float tau_ms, uint32_t* len)
+{
+    uint32_t cap = 1024, n = 0;
+    float* t = (float*)malloc(cap * sizeof(float));
+    for (uint32_t k = 0; k < (1u << 22); ++k) {
+        double v = exp(-(double)k * (double)dt_ms / (double)tau_ms);
+        if (v < 0x1p-40)
+            break;
+        if (n == cap) {
+            cap *= 2;
+            t = (float*)realloc(t, cap * sizeof(float));
+        }
+        t[n++] = (float)v;
+    }
+    *len = n;
+    return t;
 }
 
 /* ---------------- connectivity ------------------------------------------ */
@@ -733,19 +765,6 @@ static void update_neuron(orc_net* n, uint32_t j, uint64_t t, uint32_t* list, ui
         n->ring_ex[slot] = 0.0;
     n->ring_abs[slot] = 0.0;
 
-    if (n->has_plastic) {
-        int prev = n->last[j] >= 0 && (int64_t)n->last[j] == (int64_t)t - 1;  /* last < 0: never fired */
-        if (n->prec == ORC_DOUBLE) {
-            n->xhd[j] = (n->xhd[j] + (prev ? 1.0 : 0.0)) * n->dec_pd;
-            n->yhd[j] = (n->yhd[j] + (prev ? 1.0 : 0.0)) * n->dec_md;
-        } else {
-            float s = prev ? 1.0f : 0.0f;
-            float a = n->xh[j] + s;
-            n->xh[j] = a * n->dec_p;
-            float b = n->yh[j] + s;
-            n->yh[j] = b * n->dec_m;
-        }
-    }
 
     if (n->ref[j] > 0) {
         n->ref[j] -= 1;
@@ -773,12 +792,55 @@ static void update_neuron(orc_net* n, uint32_t j, uint64_t t, uint32_t* list, ui
         n->V[j] = fired ? q->v_reset : v;
     }
     if (fired) {
+        if (n->has_plastic) {
+            /* a_s of this spike: the traces at t from the previous spike */
+            if (n->prec == ORC_DOUBLE) {
+                double x = trace_d(n, 1, j, t), y = trace_d(n, 0, j, t);
+                n->xhd[j] = x;
+                n->yhd[j] = y;
+            } else {
+                float x = trace_f(n, 1, j, t), y = trace_f(n, 0, j, t);
+                n->xh[j] = x;
+                n->yh[j] = y;
+            }
+        }
         n->ref[j] = n->nref[p];
         n->last[j] = (int32_t)t;
         list[*cnt] = j;
         times[*cnt] = T;
         ++*cnt;
     }
+}
+
+/* Trace of neuron j at step t (its spikes before t; a spike at t itself not
+ * counted): (a_s + 1) dec^(t - s) for the last spike s < t, or a_s when s =
+ * t; 0 before the first spike.  pre = 1: tau_plus (x), 0: tau_minus (y).
+ * fp32 with dec^k from the table (0 past it); fp64 with exp directly. */
+static float trace_f(const orc_net* n, int pre, uint32_t j, uint64_t t)
+{
+    int32_t s = n->last[j];
+    if (s < 0)
+        return 0.0f;
+    float a = pre ? n->xh[j] : n->yh[j];
+    if ((uint64_t)s == t)
+        return a;
+    uint64_t k = t - (uint64_t)s;
+    const float* tab = pre ? n->tp : n->tm;
+    uint32_t len = pre ? n->n_tp : n->n_tm;
+    float b = a + 1.0f;
+    return b * (k < len ? tab[k] : 0.0f);
+}
+
+static double trace_d(const orc_net* n, int pre, uint32_t j, uint64_t t)
+{
+    int32_t s = n->last[j];
+    if (s < 0)
+        return 0.0;
+    double a = pre ? n->xhd[j] : n->yhd[j];
+    if ((uint64_t)s == t)
+        return a;
+    double k = (double)(t - (uint64_t)s);
+    return (a + 1.0) * exp(-k * n->dt_d / (pre ? n->tau_pd : n->tau_md));
 }
 
 static inline float clipf(float w, float lo, float hi) { return w < lo ? lo : (w > hi ? hi : w); }
@@ -801,7 +863,7 @@ static void scatter_row(orc_net* n, uint32_t s, uint64_t t)
         if (n->prec == ORC_DOUBLE) {
             double w = n->wd[i];
             if (word >> 31) {
-                w = clipd(w - (double)st->a_minus * n->yhd[tgt], st->w_min, st->w_max);
+                w = clipd(w - (double)st->a_minus * trace_d(n, 0, tgt, t), st->w_min, st->w_max);
                 n->wd[i] = w;
             }
             n->ring_d[o] += w;
@@ -809,7 +871,7 @@ static void scatter_row(orc_net* n, uint32_t s, uint64_t t)
         } else {
             float w = n->w[i];
             if (word >> 31) {
-                float dw = st->a_minus * n->yh[tgt];
+                float dw = st->a_minus * trace_f(n, 0, tgt, t);
                 w = clipf(w - dw, st->w_min, st->w_max);
                 n->w[i] = w;
             }
@@ -828,16 +890,16 @@ static void scatter_row(orc_net* n, uint32_t s, uint64_t t)
     }
 }
 
-static void potentiate(orc_net* n, uint32_t post)
+static void potentiate(orc_net* n, uint32_t post, uint64_t t)
 {
     const naer_stdp_desc* st = &n->desc.stdp;
     for (uint64_t r = n->rev_ptr[post]; r < n->rev_ptr[post + 1]; ++r) {
         uint64_t i = n->rev_pos[r];
         uint32_t src = n->rev_src[r];
         if (n->prec == ORC_DOUBLE) {
-            n->wd[i] = clipd(n->wd[i] + (double)st->a_plus * n->xhd[src], st->w_min, st->w_max);
+            n->wd[i] = clipd(n->wd[i] + (double)st->a_plus * trace_d(n, 1, src, t), st->w_min, st->w_max);
         } else {
-            float dw = st->a_plus * n->xh[src];
+            float dw = st->a_plus * trace_f(n, 1, src, t);
             n->w[i] = clipf(n->w[i] + dw, st->w_min, st->w_max);
         }
     }
@@ -900,7 +962,7 @@ static void one_step(orc_net* n)
 #pragma omp parallel for schedule(dynamic, 4) num_threads(nt)
         for (int64_t i = 0; i < (int64_t)n->sp_n; ++i)
             if (n->pops[n->pop_of[n->sp[i]]].kind == NAER_POP_LIF)
-                potentiate(n, n->sp[i]);
+                potentiate(n, n->sp[i], t);
     }
     n->t = t + 1;
 }
@@ -945,10 +1007,11 @@ void orc_net_read_state(const orc_net* n, float* v, int32_t* ref, float* pre, fl
             v[j] = n->prec == ORC_DOUBLE ? (float)n->Vd[j] : n->V[j];
         if (ref)
             ref[j] = n->ref[j];
+        /* the traces the next step sees */
         if (pre)
-            pre[j] = n->prec == ORC_DOUBLE ? (float)n->xhd[j] : n->xh[j];
+            pre[j] = !n->has_plastic ? 0.0f : n->prec == ORC_DOUBLE ? (float)trace_d(n, 1, j, n->t) : trace_f(n, 1, j, n->t);
         if (post)
-            post[j] = n->prec == ORC_DOUBLE ? (float)n->yhd[j] : n->yh[j];
+            post[j] = !n->has_plastic ? 0.0f : n->prec == ORC_DOUBLE ? (float)trace_d(n, 0, j, n->t) : trace_f(n, 0, j, n->t);
     }
 }
 

@@ -212,3 +212,45 @@ def test_pair_stdp_weight_bounds():
     rp, col, w = o.csr()
     assert float(w[rp[0]]) == STDP["w_max"]
     o.close()
+
+
+def two_spike_pair(pre_twice, k1, k2):
+    """The neuron that fires twice (pre if pre_twice, else post) fires at step 0
+    (V0 above threshold) and at k1 (a strong synapse from a forced NOP driver);
+    the other neuron is made to fire at k2 by a second driver.  Refractory
+    5 steps for the twice-firing neuron, long for the others."""
+    net = Network(dt_ms=DT_MS, mode=_abi.MODE_DETERMINISTIC, stdp=STDP, record_capacity=64)
+    twice = dict(t_ref_ms=0.5, v0_min=25.0, v0_max=25.0)
+    once = dict(t_ref_ms=1000.0, v0_min=0.0, v0_max=0.0)
+    pre = net.addPopulation(NeuronType.STDP, 1, **lif(**(twice if pre_twice else once)))
+    post = net.addPopulation(NeuronType.STDP, 1, **lif(**(once if pre_twice else twice)))
+    d1 = net.addPopulation(NeuronType.NOP, 1, **lif(v0_min=25.0, v0_max=25.0, t_ref_ms=1000.0))
+    d2 = net.addPopulation(NeuronType.NOP, 1, **lif(v0_min=25.0, v0_max=25.0, t_ref_ms=1000.0))
+    net.addConnectionFixedIndegree(d1, pre if pre_twice else post, 1, 50.0, seed=3, delay_ms=k1 * DT_MS)
+    net.addConnectionFixedIndegree(d2, post if pre_twice else pre, 1, 50.0, seed=4, delay_ms=k2 * DT_MS)
+    net.addConnectionFixedIndegree(pre, post, 1, 1.0, seed=5, delay_ms=DT_MS)
+    return net
+
+
+@pytest.mark.parametrize("precision", [O.DOUBLE, O.FLOAT, O.FIXED])
+@pytest.mark.parametrize("pre_twice", [True, False])
+def test_pair_stdp_traces_sum_over_spikes(precision, pre_twice):
+    # the trace at the second neuron's spike sums both earlier spikes:
+    # x = dec^k2 + dec^(k2 - k1); a dropped accumulation or a trace reset at
+    # each spike would give one term only
+    k1, k2 = 30, 70
+    net = two_spike_pair(pre_twice, k1, k2)
+    o = oracle(net, precision)
+    o.run(k2 + 20)
+    sp = o.spikes()
+    fired = {}
+    for s, nrn in zip(sp["step"], sp["neuron"]):
+        fired.setdefault(int(nrn), []).append(int(s))
+    first, second = (0, 1) if pre_twice else (1, 0)
+    assert fired[first] == [0, k1] and fired[second] == [k2]
+    rp, col, w = o.csr()
+    tau = STDP["tau_plus_ms"] if pre_twice else STDP["tau_minus_ms"]
+    x = math.exp(-k2 * DT_MS / tau) + math.exp(-(k2 - k1) * DT_MS / tau)
+    expect = 1.0 + STDP["a_plus"] * x if pre_twice else 1.0 - STDP["a_minus"] * x
+    assert float(w[rp[0]]) == pytest.approx(expect, rel=2e-6)
+    o.close()
