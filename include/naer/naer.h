@@ -250,17 +250,20 @@ int naer_pack_layout(naer_net* pack, uint32_t cap, uint32_t* neuron_begin, uint3
  * results as when simulated alone (bit for bit in NAER_MODE_DETERMINISTIC).
  * Jobs must use dt and mode of the arena, delays within [delay_min,
  * delay_max] steps, rows of at most max_row synapses and, when plastic, the
- * arena's STDP parameters.  No counterpart in the reference. */
+ * arena's STDP parameters and LIF refractory periods of at least
+ * refractory_min steps.  No counterpart in the reference. */
 typedef struct naer_arena_desc {
     uint64_t max_neurons;  /* neuron capacity (each job is rounded up to 256) */
     uint64_t max_synapses; /* synapse capacity                               */
-    uint64_t max_plastic;  /* plastic-synapse capacity (reverse index)       */
+    uint32_t refractory_min; /* plastic arenas: fewest refractory steps of a job's LIF
+                                populations; batches take min(delay_min, refractory_min + 1,
+                                16) steps */
     float dt_ms;
     uint32_t mode;
     uint32_t delay_min; /* steps */
     uint32_t delay_max;
     uint32_t max_row;   /* longest CSR row a job may have (<= 8192)        */
-    uint32_t plastic;   /* 1: jobs may carry STDP; the arena then steps one dt per launch */
+    uint32_t plastic;   /* 1: jobs may carry STDP */
     naer_stdp_desc stdp;
     uint32_t record_capacity;
     uint32_t poisson;   /* 1: jobs may contain POISSON sources (larger spike lists) */

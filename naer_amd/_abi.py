@@ -74,7 +74,7 @@ class NetDesc(ctypes.Structure):
 
 class ArenaDesc(ctypes.Structure):
     _fields_ = [
-        ("max_neurons", u64), ("max_synapses", u64), ("max_plastic", u64),
+        ("max_neurons", u64), ("max_synapses", u64), ("refractory_min", u32),
         ("dt_ms", f32), ("mode", u32), ("delay_min", u32), ("delay_max", u32), ("max_row", u32),
         ("plastic", u32), ("stdp", StdpDesc), ("record_capacity", u32), ("poisson", u32),
     ]

@@ -12,20 +12,22 @@ from .network import Context
 class Arena:
     """Continuous-batching container.
 
-    max_neurons / max_synapses / max_plastic   capacities (device memory is
+    max_neurons / max_synapses                 capacities (device memory is
                                                allocated up front)
     delay_steps                                (min, max) delay every job must use
-    plastic                                    jobs may carry STDP (one step per launch)
+    plastic                                    jobs may carry STDP
+    refractory_min                             plastic: fewest refractory steps a
+                                               job's LIF populations may have
     """
 
-    def __init__(self, max_neurons, max_synapses, max_plastic=0, dt_ms=0.1, mode=_abi.MODE_DETERMINISTIC,
-                 delay_steps=(15, 15), max_row=8192, plastic=False, stdp=None, record_capacity=1 << 20,
-                 poisson=False, device=0):
+    def __init__(self, max_neurons, max_synapses, dt_ms=0.1, mode=_abi.MODE_DETERMINISTIC,
+                 delay_steps=(15, 15), max_row=8192, plastic=False, stdp=None, refractory_min=0,
+                 record_capacity=1 << 20, poisson=False, device=0):
         self._ctx = Context.get(device)
         d = _abi.ArenaDesc()
         d.max_neurons = int(max_neurons)
         d.max_synapses = int(max_synapses)
-        d.max_plastic = int(max_plastic)
+        d.refractory_min = int(refractory_min)
         d.dt_ms = dt_ms
         d.mode = mode
         d.delay_min, d.delay_max = delay_steps

@@ -16,6 +16,8 @@
 //                   the post-synaptic spike.
 //   k_conn_*     <- count_connections / integrate_connections /
 //                   generate_connections (network.cl:14-107).
+#include <mutex>
+
 #include "naer_device.hpp"
 #include "naer_kernels.hpp"
 
@@ -682,6 +684,11 @@ __global__ __launch_bounds__(kBlock) void k_flush(const FlushArgs a)
     const uint32_t wpb = kBlock / kWave;
     const uint32_t waves = gridDim.x * wpb;
     for (uint32_t i = blockIdx.x * wpb + rfl(threadIdx.x / kWave); i < a.N; i += waves) {
+        if (!a.n_pops)
+            break;
+        const uint32_t p = find_pop(a.pops, a.n_pops, a.blk_pop[i >> a.tw_shift], i);
+        if (a.pops[p].kind != NAER_POP_LIF || i < a.pops[p].begin || i >= a.pops[p].end)
+            continue;  // plastic synapses leave LIF neurons of active networks only
         const unsigned long long rb = a.row_ptr[i], re = a.row_ptr[i + 1];
         if (rb == re)
             continue;
@@ -1330,29 +1337,18 @@ __global__ __launch_bounds__(512) void k_sort_rows(const unsigned long long* row
     }
 }
 
-template <bool PLACE>
-__global__ __launch_bounds__(kBlock) void k_reverse(const unsigned long long* row_ptr,
-                                                    const uint32_t* col, uint32_t row_begin,
-                                                    uint32_t row_end, uint32_t* rdeg,
-                                                    unsigned long long* rcursor, uint32_t* rev_pos,
-                                                    uint32_t* rev_src)
+// Plastic synapses of the synapse range [begin, end), added to *count.
+__global__ __launch_bounds__(kBlock) void k_count_plastic(const uint32_t* col, unsigned long long begin,
+                                                          unsigned long long end, unsigned long long* count)
 {
-    for (uint32_t s = row_begin + blockIdx.x; s < row_end; s += gridDim.x) {
-        const unsigned long long re = row_ptr[s + 1];
-        for (unsigned long long i = row_ptr[s] + threadIdx.x; i < re; i += kBlock) {
-            const uint32_t c = col[i];
-            if (!(c & kPlasticBit))
-                continue;
-            const uint32_t tgt = c & kTargetMask;
-            if (!PLACE) {
-                atomicAdd(&rdeg[tgt], 1u);
-            } else {
-                const unsigned long long r = atomicAdd(&rcursor[tgt], 1ull);
-                rev_pos[r] = (uint32_t)i;
-                rev_src[r] = s;
-            }
-        }
-    }
+    uint32_t k = 0;
+    for (unsigned long long i = begin + (unsigned long long)blockIdx.x * kBlock + threadIdx.x; i < end;
+         i += (unsigned long long)gridDim.x * kBlock)
+        k += (col[i] & kPlasticBit) ? 1u : 0u;
+    for (uint32_t off = kWave / 2; off > 0; off >>= 1)
+        k += __shfl_down(k, off);
+    if (lane_id() == 0 && k)
+        atomicAdd(count, (unsigned long long)k);
 }
 
 // Initial state of neurons [j_begin, j_end): potentials from the V0 stream,
@@ -1432,17 +1428,24 @@ __global__ __launch_bounds__(kBlock) void k_ref_poisson_epoch(uint32_t n, uint32
 // ---------------------------------------------------------------------------
 static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
+// Raising a kernel's dynamic-LDS limit: only ever raised, under a lock, so
+// nets stepped from several host threads never lower it under each other.
+static std::mutex g_lds_mu;
+
 template <bool DET, bool PLASTIC, bool BUCKET, bool POIS, uint32_t TW>
 static hipError_t launch_update_t(const UpdateArgs& a, hipStream_t s)
 {
     const size_t lds = BUCKET ? (size_t)a.B * TW * sizeof(unsigned long long) : 0;
     static size_t lds_set = 0;  // per instantiation: largest size allowed so far
-    if (lds > 32768 && lds > lds_set) {
-        hipError_t e = hipFuncSetAttribute((const void*)k_update<DET, PLASTIC, BUCKET, POIS, TW>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess)
-            return e;
-        lds_set = lds;
+    if (lds > 32768) {
+        std::lock_guard<std::mutex> lock(g_lds_mu);
+        if (lds > lds_set) {
+            hipError_t e = hipFuncSetAttribute((const void*)k_update<DET, PLASTIC, BUCKET, POIS, TW>,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e != hipSuccess)
+                return e;
+            lds_set = lds;
+        }
     }
     hipLaunchKernelGGL((k_update<DET, PLASTIC, BUCKET, POIS, TW>), dim3(a.n_seg), dim3(TW), lds, s, a);
     return hipGetLastError();
@@ -1489,6 +1492,7 @@ static hipError_t bucket_lds_attributes(const BucketArgs& a)
 {
     const size_t lds = (size_t)a.NBK * sizeof(uint32_t);
     static size_t lds_set = 0;
+    std::lock_guard<std::mutex> lock(g_lds_mu);
     if (lds > 32768 && lds > lds_set) {
         hipError_t e = hist_lds_attributes<false>(lds);
         if (e == hipSuccess)
@@ -1653,20 +1657,11 @@ hipError_t launch_sort_rows(const unsigned long long* row_ptr, uint32_t* col, fl
     return hipGetLastError();
 }
 
-hipError_t launch_reverse(bool place, const unsigned long long* row_ptr, const uint32_t* col,
-                          uint32_t row_begin, uint32_t row_end, uint32_t* rdeg, unsigned long long* rcursor,
-                          uint32_t* rev_pos, uint32_t* rev_src, hipStream_t s)
+hipError_t launch_count_plastic(const uint32_t* col, unsigned long long begin, unsigned long long end,
+                                unsigned long long* count, hipStream_t s)
 {
-    if (row_end <= row_begin)
-        return hipSuccess;
-    const uint32_t rows = row_end - row_begin;
-    const dim3 grid(rows < 65536u ? rows : 65536u);
-    if (place)
-        hipLaunchKernelGGL((k_reverse<true>), grid, dim3(kBlock), 0, s, row_ptr, col, row_begin, row_end,
-                           rdeg, rcursor, rev_pos, rev_src);
-    else
-        hipLaunchKernelGGL((k_reverse<false>), grid, dim3(kBlock), 0, s, row_ptr, col, row_begin, row_end,
-                           rdeg, rcursor, rev_pos, rev_src);
+    const uint32_t grid = std::max<uint32_t>(1, std::min<uint64_t>(4096, cdiv(end - begin, kBlock)));
+    hipLaunchKernelGGL(k_count_plastic, dim3(grid), dim3(kBlock), 0, s, col, begin, end, count);
     return hipGetLastError();
 }
 

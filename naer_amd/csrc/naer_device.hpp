@@ -267,14 +267,19 @@ struct ScatterArgs {
     uint32_t R, Bfull;
 };
 
-// Flush of pending potentiation over every plastic synapse (rows of the
-// sources [0, N)); `force` flushes regardless of the window.
+// Flush of pending potentiation over every plastic synapse (rows of the LIF
+// neurons of the active networks; the row pointers of neurons outside them,
+// such as an arena's free ranges, are not walked); `force` flushes
+// regardless of the window.
 struct FlushArgs {
     DevState* st;
     const unsigned long long* row_ptr;
     const uint32_t* col;
     float* w;
     StdpArgs sd;
+    const PopParams* pops;   // [n_pops]
+    const uint32_t* blk_pop; // first population of each update tile
+    uint32_t n_pops, tw_shift;
     uint32_t N;
     uint32_t B;        // steps of the batch that follows
     uint32_t force;

@@ -21,9 +21,8 @@ hipError_t launch_conn(const ConnArgs& c, uint32_t rule, bool place, uint32_t* d
                        unsigned long long* cursor, uint32_t* col, float* w, hipStream_t s);
 hipError_t launch_sort_rows(const unsigned long long* row_ptr, uint32_t* col, float* w, uint32_t row_begin,
                             uint32_t row_end, uint32_t max_row, hipStream_t s);
-hipError_t launch_reverse(bool place, const unsigned long long* row_ptr, const uint32_t* col,
-                          uint32_t row_begin, uint32_t row_end, uint32_t* rdeg, unsigned long long* rcursor,
-                          uint32_t* rev_pos, uint32_t* rev_src, hipStream_t s);
+hipError_t launch_count_plastic(const uint32_t* col, unsigned long long begin, unsigned long long end,
+                                unsigned long long* count, hipStream_t s);
 hipError_t launch_init_state(const UpdateArgs& a, bool det, uint32_t j_begin, uint32_t j_end, hipStream_t s);
 hipError_t launch_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out,
                          unsigned long long n, hipStream_t s);

@@ -121,9 +121,8 @@ struct NetInfo {
     std::vector<naer_conn_desc> conns;  // population indices local to the network
     uint32_t neuron_begin = 0, n_neurons = 0;
     uint64_t syn_begin = 0, n_syn = 0;
-    uint64_t rev_begin = 0, n_rev = 0;
+    uint64_t n_plastic = 0;  // plastic synapses
     uint32_t max_row = 0;
-    uint32_t max_rev = 0;  // longest reverse (plastic in-synapse) row
     uint32_t dmin = 0xFFFFFFFFu, dmax = 0;
     bool plastic = false;
     bool active = false;
@@ -146,11 +145,10 @@ struct naer_net {
     uint32_t D = 0, B = 0, dmin = 0, dmax = 0;
     uint32_t N = 0;      // neuron capacity
     uint64_t S_cap = 0;  // synapse capacity
-    uint64_t R_cap = 0;  // plastic-synapse capacity
     uint32_t max_row = 0;
 
     std::vector<NetInfo> nets;
-    RangeAlloc neuron_alloc, syn_alloc, rev_alloc;
+    RangeAlloc neuron_alloc, syn_alloc;
 
     // population table of the active networks
     std::vector<PopParams> pp;
@@ -175,8 +173,7 @@ struct naer_net {
     unsigned long long* row_ptr = nullptr;
     uint32_t* col = nullptr;
     float* w = nullptr;
-    unsigned long long* rev_ptr = nullptr;
-    uint32_t *rev_pos = nullptr, *rev_src = nullptr;
+    unsigned long long* count = nullptr;   // build scratch, 1 entry
     uint32_t* deg = nullptr;               // build scratch, N entries
     unsigned long long* cursor = nullptr;  // build scratch, N entries
     naer_spike *list0 = nullptr, *list1 = nullptr, *rec = nullptr;
@@ -195,7 +192,6 @@ struct naer_net {
     uint32_t probe_n = 0;
 
     uint32_t scatter_chunks = 1, scatter_grid = 1;
-    uint32_t rev_chunks = 1;  // potentiation grid rows: longest reverse row / kChunk
 
     // spike raster file (naer_net_raster_*)
     FILE* raster = nullptr;
@@ -689,28 +685,15 @@ int place_net(naer_net* n, const NetInfo& net)
     return NAER_OK;
 }
 
-// Counts the plastic synapses onto every target of a network.
-int reverse_count(naer_net* n, NetInfo& net, std::vector<unsigned long long>* prefix)
+// Counts the plastic synapses of a network (statistics).
+int count_plastic(naer_net* n, NetInfo& net)
 {
-    HIP_TRY(hipMemsetAsync(n->deg + net.neuron_begin, 0, (size_t)net.n_neurons * sizeof(uint32_t), n->stream));
-    HIP_TRY(launch_reverse(false, n->row_ptr, n->col, net.neuron_begin, net.neuron_begin + net.n_neurons, n->deg,
-                           nullptr, nullptr, nullptr, n->stream));
-    int rc = read_prefix(n, net.neuron_begin, net.n_neurons, prefix, &net.max_rev);
-    // reverse rows are walked in kChunk-synapse chunks (one grid row each)
-    n->rev_chunks = std::max<uint32_t>(n->rev_chunks, (net.max_rev + kChunk - 1) / kChunk);
-    return rc;
-}
-
-// Writes the reverse index of a network at net.rev_begin.
-int reverse_place(naer_net* n, const NetInfo& net, std::vector<unsigned long long>& prefix)
-{
-    int rc = write_ptr(n, n->rev_ptr, net.neuron_begin, net.rev_begin, prefix);
-    if (rc)
-        return rc;
-    HIP_TRY(hipMemcpyAsync(n->cursor + net.neuron_begin, n->rev_ptr + net.neuron_begin,
-                           (size_t)net.n_neurons * sizeof(unsigned long long), hipMemcpyDeviceToDevice, n->stream));
-    HIP_TRY(launch_reverse(true, n->row_ptr, n->col, net.neuron_begin, net.neuron_begin + net.n_neurons, nullptr,
-                           n->cursor, n->rev_pos, n->rev_src, n->stream));
+    HIP_TRY(hipMemsetAsync(n->count, 0, sizeof(unsigned long long), n->stream));
+    HIP_TRY(launch_count_plastic(n->col, net.syn_begin, net.syn_begin + net.n_syn, n->count, n->stream));
+    unsigned long long k = 0;
+    HIP_TRY(hipMemcpyAsync(&k, n->count, sizeof k, hipMemcpyDeviceToHost, n->stream));
+    HIP_TRY(hipStreamSynchronize(n->stream));
+    net.n_plastic = k;
     return NAER_OK;
 }
 
@@ -755,19 +738,18 @@ int alloc_neurons(naer_net* n, bool any_poisson)
 {
     const uint32_t N = n->N;
     int rc;
-    if ((rc = n->alloc(&n->deg, N)) || (rc = n->alloc(&n->cursor, N)) ||
+    if ((rc = n->alloc(&n->deg, N)) || (rc = n->alloc(&n->cursor, N)) || (rc = n->alloc(&n->count, 1)) ||
         (rc = n->alloc(&n->row_ptr, (size_t)N + 1)))
         return rc;
     HIP_TRY(hipMemsetAsync(n->row_ptr, 0, ((size_t)N + 1) * sizeof(unsigned long long), n->stream));
     if (n->plastic) {
         n->h_tp = trace_table(n->dt_ms, n->stdp.tau_plus_ms);
         n->h_tm = trace_table(n->dt_ms, n->stdp.tau_minus_ms);
-        if ((rc = n->alloc(&n->rev_ptr, (size_t)N + 1)) || (rc = n->alloc(&n->lat, N)) ||
+        if ((rc = n->alloc(&n->lat, N)) ||
             (rc = n->alloc(&n->pst, 2 * (size_t)N)) || (rc = n->alloc(&n->shist, (size_t)N * kSpikeHist)) ||
             (rc = n->alloc(&n->d_tp, std::max<size_t>(1, n->h_tp.size()))) ||
             (rc = n->alloc(&n->d_tm, std::max<size_t>(1, n->h_tm.size()))))
             return rc;
-        HIP_TRY(hipMemsetAsync(n->rev_ptr, 0, ((size_t)N + 1) * sizeof(unsigned long long), n->stream));
         HIP_TRY(hipMemsetAsync(n->lat, 0, (size_t)N * sizeof(uint4), n->stream));
         // every word kNoSpike: steps absent (the trace words are never read then)
         HIP_TRY(hipMemsetD32Async((hipDeviceptr_t)n->pst, (int)kNoSpike, (size_t)N * 8, n->stream));
@@ -983,17 +965,6 @@ uint32_t min_refractory(const naer_net* n)
     return m < 0 ? 0u : (uint32_t)std::min<int64_t>(m, 1 << 20);
 }
 
-// An arena's flush window follows its active jobs (called between steps).
-int set_flush_window(naer_net* n)
-{
-    if (!n->plastic)
-        return NAER_OK;
-    n->nref_min = min_refractory(n);
-    const uint32_t w = flush_window(n);
-    HIP_TRY(hipMemcpy((char*)n->st + offsetof(DevState, window), &w, sizeof w, hipMemcpyHostToDevice));
-    return NAER_OK;
-}
-
 int naer_pack_create(naer_ctx* ctx, const naer_net_desc* descs, uint32_t n_nets, naer_net** out)
 {
     if (!ctx || !descs || !out || n_nets == 0)
@@ -1099,8 +1070,6 @@ int naer_net_build(naer_net* n)
         S += net.n_syn;
         n->max_row = std::max(n->max_row, net.max_row);
     }
-    if (n->plastic && S >= 0xFFFFFFFFull)
-        return fail(NAER_ERR_CAPACITY, "plastic nets are limited to 2^32-1 synapses");
     n->S_cap = S;
     if ((rc = n->alloc(&n->col, S)) || (rc = n->alloc(&n->w, S)))
         return rc;
@@ -1111,25 +1080,11 @@ int naer_net_build(naer_net* n)
             (rc = place_net(n, net)))
             return rc;
     }
-    // 3. reverse index of plastic synapses, networks back to back
-    if (n->plastic) {
-        std::vector<std::vector<unsigned long long>> rp(n->nets.size());
-        uint64_t R = 0;
-        for (size_t i = 0; i < n->nets.size(); ++i) {
-            NetInfo& net = n->nets[i];
-            if ((rc = reverse_count(n, net, &rp[i])))
+    // 3. plastic synapse counts
+    if (n->plastic)
+        for (auto& net : n->nets)
+            if (net.plastic && (rc = count_plastic(n, net)))
                 return rc;
-            net.rev_begin = R;
-            net.n_rev = rp[i].back();
-            R += net.n_rev;
-        }
-        n->R_cap = R;
-        if ((rc = n->alloc(&n->rev_pos, R)) || (rc = n->alloc(&n->rev_src, R)))
-            return rc;
-        for (size_t i = 0; i < n->nets.size(); ++i)
-            if ((rc = reverse_place(n, n->nets[i], rp[i])))
-                return rc;
-    }
     // 4. delivery buffers, neuron state
     if ((rc = alloc_buckets(n)) || (rc = upload_table(n)))
         return rc;
@@ -1156,30 +1111,26 @@ int naer_arena_create(naer_ctx* ctx, const naer_arena_desc* a, naer_net** out)
         rc = fail(NAER_ERR_INVALID, "arena delays must satisfy 1 <= min <= max <= 127");
     if (rc == NAER_OK && (a->max_row == 0 || a->max_row > kMaxSortedRow))
         rc = fail(NAER_ERR_CAPACITY, "arena max_row must be in [1, %u]", kMaxSortedRow);
-    if (rc == NAER_OK && a->plastic && a->max_synapses >= 0xFFFFFFFFull)
-        rc = fail(NAER_ERR_CAPACITY, "plastic arenas are limited to 2^32-1 synapses");
     if (rc == NAER_OK && a->plastic &&
         (!(a->stdp.tau_plus_ms > 0.0f) || !(a->stdp.tau_minus_ms > 0.0f) || !(a->stdp.w_max >= a->stdp.w_min)))
         rc = fail(NAER_ERR_INVALID, "STDP needs tau_plus, tau_minus > 0 and w_max >= w_min");
     if (rc == NAER_OK) {
         n->N = (uint32_t)((a->max_neurons + kBlock - 1) / kBlock * kBlock);
         n->S_cap = a->max_synapses;
-        n->R_cap = a->plastic ? a->max_plastic : 0;
         n->plastic = a->plastic != 0;
         n->stdp = a->stdp;
         n->stdp_set = n->plastic;
         n->dmin = a->delay_min;
         n->dmax = a->delay_max;
         n->D = n->dmax + 1;
-        // jobs arrive later: one step per batch; the flush window follows
-        // the active jobs' refractory periods (set at admission)
-        n->nref_min = 0;
-        n->B = batch_steps(n->plastic, n->dmin, n->N, 0);
+        // jobs arrive later: plastic arenas batch (and flush) for the
+        // fewest refractory steps a job may have
+        n->nref_min = n->plastic ? a->refractory_min : 0;
+        n->B = batch_steps(n->plastic, n->dmin, n->N, n->nref_min);
         n->max_row = a->max_row;
         n->rec_cap = a->record_capacity;
         n->neuron_alloc.reset(n->N);
         n->syn_alloc.reset(n->S_cap);
-        n->rev_alloc.reset(n->R_cap);
         rc = make_stream(n);
     }
     n->poisson_ok = a->poisson != 0;
@@ -1189,10 +1140,6 @@ int naer_arena_create(naer_ctx* ctx, const naer_arena_desc* a, naer_net** out)
         rc = n->alloc(&n->col, n->S_cap);
     if (rc == NAER_OK)
         rc = n->alloc(&n->w, n->S_cap);
-    if (rc == NAER_OK && n->plastic)
-        rc = n->alloc(&n->rev_pos, n->R_cap);
-    if (rc == NAER_OK && n->plastic)
-        rc = n->alloc(&n->rev_src, n->R_cap);
     if (rc == NAER_OK)
         rc = alloc_buckets(n);
     if (rc == NAER_OK)
@@ -1227,12 +1174,16 @@ int naer_arena_admit(naer_net* n, const naer_net_desc* desc, uint32_t* slot)
                     n->dmin, n->dmax);
     if (net.plastic && (!n->plastic || std::memcmp(&n->stdp, &desc->stdp, sizeof n->stdp) != 0))
         return fail(NAER_ERR_INVALID, "job needs STDP parameters the arena was not created with");
-    for (auto& q : net.pops)
+    for (auto& q : net.pops) {
         if (q.kind == NAER_POP_POISSON && !n->poisson_ok)
             return fail(NAER_ERR_INVALID, "the arena was created without POISSON sources");
+        if (q.kind == NAER_POP_LIF && n->plastic &&
+            std::llround((double)q.t_ref_ms / (double)n->dt_ms) < (long long)n->nref_min)
+            return fail(NAER_ERR_INVALID, "job refractory period below the arena's %u steps", n->nref_min);
+    }
     if ((rc = naer_net_sync(n)))
         return rc;
-    // one extra entry: row_ptr/rev_ptr[begin + n] (the end of the job's last
+    // one extra entry: row_ptr[begin + n] (the end of the job's last
     // row) must not be shared with a job placed right behind it
     uint64_t nb = 0;
     if (!n->neuron_alloc.take((uint64_t)net.n_neurons + 1, kBlock, &nb))
@@ -1241,13 +1192,11 @@ int naer_arena_admit(naer_net* n, const naer_net_desc* desc, uint32_t* slot)
     // a job starts at the next step of the arena clock
     net.t_start = n->steps_enqueued;
     net.active = true;
-    bool syn_taken = false, rev_taken = false;
+    bool syn_taken = false;
     auto undo = [&](int code) {
         n->neuron_alloc.give(net.neuron_begin, (uint64_t)net.n_neurons + 1);
         if (syn_taken)
             n->syn_alloc.give(net.syn_begin, net.n_syn);
-        if (rev_taken)
-            n->rev_alloc.give(net.rev_begin, net.n_rev);
         return code;
     };
     std::vector<unsigned long long> prefix;
@@ -1262,22 +1211,8 @@ int naer_arena_admit(naer_net* n, const naer_net_desc* desc, uint32_t* slot)
     syn_taken = true;
     if ((rc = write_ptr(n, n->row_ptr, net.neuron_begin, net.syn_begin, prefix)) || (rc = place_net(n, net)))
         return undo(rc);
-    if (n->plastic) {
-        // static jobs get an empty reverse index (a flat rev_ptr over their range)
-        std::vector<unsigned long long> rp((size_t)net.n_neurons + 1, 0);
-        if (net.plastic && (rc = reverse_count(n, net, &rp)))
-            return undo(rc);
-        net.n_rev = rp.back();
-        if (!n->rev_alloc.take(net.n_rev, 64, &net.rev_begin))
-            return undo(fail(NAER_ERR_CAPACITY, "no free range of %llu plastic synapses", (unsigned long long)net.n_rev));
-        rev_taken = true;
-        if (net.plastic) {
-            if ((rc = reverse_place(n, net, rp)))
-                return undo(rc);
-        } else if ((rc = write_ptr(n, n->rev_ptr, net.neuron_begin, net.rev_begin, rp))) {
-            return undo(rc);
-        }
-    }
+    if (net.plastic && (rc = count_plastic(n, net)))
+        return undo(rc);
     // slot: reuse a retired one
     uint32_t s = (uint32_t)n->nets.size();
     for (uint32_t i = 0; i < n->nets.size(); ++i)
@@ -1295,8 +1230,6 @@ int naer_arena_admit(naer_net* n, const naer_net_desc* desc, uint32_t* slot)
         return undo(rc);
     }
     HIP_TRY(hipStreamSynchronize(n->stream));
-    if ((rc = set_flush_window(n)))
-        return rc;
     *slot = s;
     return NAER_OK;
 }
@@ -1316,12 +1249,8 @@ int naer_arena_retire(naer_net* n, uint32_t slot)
     net.active = false;
     n->neuron_alloc.give(net.neuron_begin, (uint64_t)net.n_neurons + 1);
     n->syn_alloc.give(net.syn_begin, net.n_syn);
-    if (n->plastic)
-        n->rev_alloc.give(net.rev_begin, net.n_rev);
-    net.n_syn = net.n_rev = 0;
-    if ((rc = upload_table(n)))
-        return rc;
-    return set_flush_window(n);
+    net.n_syn = net.n_plastic = 0;
+    return upload_table(n);
 }
 
 static hipError_t do_update(naer_net* n, void* arg)
@@ -1345,6 +1274,10 @@ FlushArgs flush_args(const naer_net* n, bool force)
     f.col = n->col;
     f.w = n->w;
     f.sd = stdp_args(n);
+    f.pops = n->d_pops;
+    f.blk_pop = n->d_blk_pop;
+    f.n_pops = (uint32_t)n->pp.size();
+    f.tw_shift = n->tw_shift;
     f.N = n->N;
     f.B = n->B;
     f.force = force ? 1u : 0u;
@@ -1718,7 +1651,7 @@ int naer_net_query(naer_net* n, naer_stats* out)
         if (net.active) {
             out->n_neurons += net.n_neurons;
             out->n_synapses += net.n_syn;
-            out->n_plastic += net.n_rev;
+            out->n_plastic += net.n_plastic;
         }
     out->ring_depth = n->D;
     out->batch = n->B;
@@ -1860,7 +1793,7 @@ constexpr char kCheckpointMagic[8] = { 'N', 'A', 'E', 'R', 'C', 'K', 'P', '2' };
 
 // Structure of a net that a checkpoint must match to be loaded.
 struct CheckpointShape {
-    uint64_t N, S_cap, R_cap, D, B, R, NBK, e_cap, n_seg, rec_cap;
+    uint64_t N, S_cap, D, B, R, NBK, e_cap, n_seg, rec_cap;
     uint32_t det, plastic, poisson, bucketed;
 };
 
@@ -1870,7 +1803,6 @@ CheckpointShape shape_of(const naer_net* n)
     std::memset(&c, 0, sizeof c);
     c.N = n->N;
     c.S_cap = n->S_cap;
-    c.R_cap = n->R_cap;
     c.D = n->D;
     c.B = n->B;
     c.R = n->R;

@@ -30,8 +30,7 @@ from . import _abi, models
 DELAY_STEPS = 15       # 1.5 ms at dt = 0.1 ms
 MAX_ROW = 8192
 BYTES_PER_SYNAPSE = 8  # uint32 target/delay word + fp32 weight
-BYTES_PER_PLASTIC = 8  # reverse index: position + source
-MAX_PLASTIC_ARENA_SYNAPSES = (1 << 32) - (1 << 20)
+REFRACTORY_STEPS = int(round(models.BRUNEL_LIF["t_ref_ms"] / 0.1))  # of every job's LIF populations
 
 
 @dataclass
@@ -49,12 +48,6 @@ class JobSpec:
 
     def n_synapses(self):
         return self.n * self.indegree
-
-    def n_plastic(self):
-        if not self.plastic:
-            return 0
-        n_e = int(round(self.n * 0.8))
-        return n_e * int(round(self.indegree * 0.8))
 
     def network(self, mode=_abi.MODE_DETERMINISTIC, record_capacity=0):
         return models.balanced(self.n, self.indegree, 0.1, 5.0, 20_000.0, 0.1, delay_ms=1.5,
@@ -81,11 +74,10 @@ def neuron_bytes(mode, plastic):
     """Device bytes per neuron of arena capacity: ring (delay+1 slots), state,
     row pointers, build scratch and two spike-list segments (B entries each)."""
     ring = (DELAY_STEPS + 1) * (8 if mode == _abi.MODE_DETERMINISTIC else 4)
-    b = 1 if plastic else DELAY_STEPS
-    lists = 2 * b * 16
+    lists = 2 * DELAY_STEPS * 16  # B = 15 steps in both arenas
     state = 4 * 5 + 8 + 4 + 8 + 16  # V ref last pnext pcnt, row_ptr, deg, cursor, counters
     if plastic:
-        state += 8 + 8  # traces, reverse row pointers
+        state += 16 + 32 + 4 * 16  # STDP: pre/post records of the newest spikes, ring of 16 spike steps
     return ring + lists + state
 
 
@@ -93,32 +85,24 @@ def footprint(job, mode=_abi.MODE_DETERMINISTIC):
     """Device bytes a job occupies in its arena."""
     nb = neuron_bytes(mode, job.plastic)
     n = (job.n + 1 + 255) // 256 * 256
-    return n * nb + job.n_synapses() * BYTES_PER_SYNAPSE + job.n_plastic() * BYTES_PER_PLASTIC
+    return n * nb + job.n_synapses() * BYTES_PER_SYNAPSE
 
 
 @dataclass
 class Capacity:
     neurons: int
     synapses: int
-    plastic: int
 
 
 def split_budget(budget_bytes, mode=_abi.MODE_DETERMINISTIC, stdp_fraction=0.3, synapses_per_neuron=700):
     """Capacities of the static and the plastic arena for a total budget,
     shared in proportion to the expected footprint of the job mix."""
-    sb_static = BYTES_PER_SYNAPSE
-    sb_plastic = BYTES_PER_SYNAPSE + 0.64 * BYTES_PER_PLASTIC
-    w_s = (1 - stdp_fraction) * sb_static
-    w_p = stdp_fraction * sb_plastic
+    per_syn = [BYTES_PER_SYNAPSE + neuron_bytes(mode, plastic) / synapses_per_neuron for plastic in (False, True)]
+    w = [(1.0 - stdp_fraction) * per_syn[0], stdp_fraction * per_syn[1]]
     caps = []
-    for frac, plastic, sb in ((w_s / (w_s + w_p), False, sb_static), (w_p / (w_s + w_p), True, sb_plastic)):
-        b = budget_bytes * frac
-        per_syn = sb + neuron_bytes(mode, plastic) / synapses_per_neuron
-        syn = int(b / per_syn)
-        if plastic:  # the reverse index of a plastic arena holds 32-bit synapse positions
-            syn = min(syn, MAX_PLASTIC_ARENA_SYNAPSES)
-        caps.append(Capacity(neurons=max(256, syn // synapses_per_neuron), synapses=syn,
-                             plastic=int(0.64 * syn) if plastic else 0))
+    for k in (0, 1):
+        syn = int(budget_bytes * w[k] / sum(w) / per_syn[k])
+        caps.append(Capacity(neurons=max(256, syn // synapses_per_neuron), synapses=syn))
     return caps[0], caps[1]
 
 
@@ -147,19 +131,23 @@ class Server:
 
     def __init__(self, budget_bytes=200e9, mode=_abi.MODE_DETERMINISTIC, chunk=100, graphs=True,
                  keep_rasters=False, record_per_chunk=1 << 22, arena_factory=None, dt_ms=0.1,
-                 synapses_per_neuron=700):
+                 synapses_per_neuron=700, stdp_fraction=0.3):
         self.mode = mode
         self.chunk = chunk
         self.dt_ms = dt_ms
         self.keep = keep_rasters
-        static_cap, plastic_cap = split_budget(budget_bytes, mode, synapses_per_neuron=synapses_per_neuron)
+        # the two arenas share the budget as the expected job mix does
+        static_cap, plastic_cap = split_budget(budget_bytes, mode, stdp_fraction=stdp_fraction,
+                                               synapses_per_neuron=synapses_per_neuron)
         if arena_factory is None:
             from .arena import Arena
 
             def arena_factory(cap, plastic):
-                return Arena(cap.neurons, cap.synapses, cap.plastic, dt_ms=dt_ms, mode=mode,
+                return Arena(cap.neurons, cap.synapses, dt_ms=dt_ms, mode=mode,
                              delay_steps=(DELAY_STEPS, DELAY_STEPS), max_row=MAX_ROW, plastic=plastic,
-                             stdp=models.PAIR_STDP if plastic else None, record_capacity=record_per_chunk)
+                             stdp=models.PAIR_STDP if plastic else None,
+                             refractory_min=REFRACTORY_STEPS if plastic else 0,
+                             record_capacity=record_per_chunk)
         self.arenas = [arena_factory(static_cap, False), arena_factory(plastic_cap, True)]
         self.capacity = (static_cap, plastic_cap)
         if graphs:
@@ -191,8 +179,7 @@ class Server:
             for job in waiting:
                 ai = 1 if job.plastic else 0
                 cap = self.capacity[ai]
-                if (job.n > cap.neurons or job.n_synapses() > cap.synapses or job.n_plastic() > cap.plastic
-                        or job.indegree * 4 > MAX_ROW):
+                if job.n > cap.neurons or job.n_synapses() > cap.synapses or job.indegree * 4 > MAX_ROW:
                     rejected.append(job.index)  # can never fit
                     continue
                 slot = self.arenas[ai].admit(job.network(self.mode))

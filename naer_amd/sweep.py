@@ -21,6 +21,7 @@ import json
 import math
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -58,8 +59,14 @@ def _progress(msg):
 
 
 def run_packed(specs, steps, mode=_abi.MODE_DETERMINISTIC, chunk=100, record=True, graphs=True, verbose=False,
-               autotune=True):
-    groups = [[s for s in specs if not s["plastic"]], [s for s in specs if s["plastic"]]]
+               autotune=True, packs_per_kind=1):
+    groups = []
+    for plastic in (False, True):
+        kind = [s for s in specs if s["plastic"] == plastic]
+        # packs of a kind take every packs_per_kind-th network by size, so
+        # they carry similar work; each pack runs on its own stream
+        kind.sort(key=lambda s: s["n"])
+        groups += [kind[i::packs_per_kind] for i in range(packs_per_kind)]
     packs = []
     t_build = time.perf_counter()
     for g in groups:
@@ -81,21 +88,27 @@ def run_packed(specs, steps, mode=_abi.MODE_DETERMINISTIC, chunk=100, record=Tru
     # depend on the schedule); each pack then runs the rest of its steps
     done = [0] * len(packs)
     schedules = []
-    # the tuning trials take up to 2 passes x 18 schedules x 2 batches x 16 steps
+    # the tuning trials take up to 2 passes x 19 schedules x 2 batches x 16 steps
     if autotune and steps >= 2000:
         # packs the tuner does not apply to (ring delivery) get all their
-        # steps enqueued first: they keep the device busy while the others tune
+        # steps enqueued first; the others tune side by side, one host thread
+        # each (the library's calls release the interpreter), so the device
+        # stays busy with both
         tunable = [p.schedule()["bucketed"] == 1 for _, p in packs]
         for k, (_, p) in enumerate(packs):
             if not tunable[k]:
                 p.run(steps)
                 done[k] = steps
-        for k, (_, p) in enumerate(packs):
-            if tunable[k]:
-                p.run(min(steps // 10, 500))  # past the initial transient
-                sched = p.autotune(2)
+
+        def tune(p):
+            p.run(min(steps // 10, 500))  # past the initial transient
+            return p.autotune(2), int(p.stats().step)
+
+        picked = [(k, p) for k, (_, p) in enumerate(packs) if tunable[k]]
+        with ThreadPoolExecutor(max_workers=max(1, len(picked))) as ex:
+            for (k, _), (sched, step) in zip(picked, ex.map(lambda kp: tune(kp[1]), picked)):
                 schedules.append(sched)
-                done[k] = int(p.stats().step)
+                done[k] = step
                 if verbose:
                     _progress(f"packed: schedule {sched}")
     last = t0
@@ -158,10 +171,12 @@ def main():
     ap.add_argument("--no-sequential", action="store_true")
     ap.add_argument("--no-graphs", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--packs-per-kind", type=int, default=1)
     a = ap.parse_args()
     mode = _abi.MODE_DETERMINISTIC if a.mode == "det" else _abi.MODE_FAST
     specs = sweep_specs(a.nets, n_max=a.n_max)
-    packed = run_packed(specs, a.steps, mode, graphs=not a.no_graphs, verbose=True, autotune=not a.no_autotune)
+    packed = run_packed(specs, a.steps, mode, graphs=not a.no_graphs, verbose=True, autotune=not a.no_autotune,
+                        packs_per_kind=a.packs_per_kind)
     out = dict(nets=a.nets, steps=a.steps, mode=a.mode, packed_run_s=packed["run_s"],
                packed_build_s=packed["build_s"], packed_jobs_per_s=a.nets / packed["run_s"],
                packs=packed["packs"], schedules=packed["schedules"])

@@ -20,7 +20,7 @@ def test_mix_equals_alone():
                                dur_range=(0.02, 0.05), stdp_fraction=0.4)
     budget = 6 * max(serving.footprint(j) for j in jobs)
     srv = serving.Server(budget, mode=_abi.MODE_DETERMINISTIC, chunk=50, keep_rasters=True,
-                         record_per_chunk=1 << 20, synapses_per_neuron=40)
+                         record_per_chunk=1 << 20, synapses_per_neuron=40, stdp_fraction=0.4)
     out = srv.serve(jobs)
     srv.close()
     assert out["completed"] == len(jobs), out["rejected"]
@@ -54,4 +54,71 @@ def test_arena_slot_reuse_and_capacity_errors():
     alone = serving.run_alone(serving.JobSpec(1, 0, jobs[1].n, jobs[1].indegree, 0.003, False, jobs[1].seed))
     assert first_divergence(np.sort(sp[s1][sp[s1]["step"] < 30], order=["step", "neuron", "time_us"]),
                             alone) is None
+    a.close()
+
+
+def test_plastic_arena_batches_and_rejects_short_refractory():
+    from naer_amd import models
+    from naer_amd.arena import Arena
+    from naer_amd.network import NaerError
+    a = Arena(4096, 4096 * 300, dt_ms=0.1, mode=_abi.MODE_DETERMINISTIC, delay_steps=(15, 15), plastic=True,
+              stdp=models.PAIR_STDP, refractory_min=20)
+    assert a.stats().batch == 15  # min(delay 15, refractory 20 + 1)
+    ok = models.balanced(1000, 100, 0.1, 5.0, 20_000.0, 0.1, delay_ms=1.5, plastic=True, seed=3,
+                         mode=_abi.MODE_DETERMINISTIC)
+    short = models.balanced(1000, 100, 0.1, 5.0, 20_000.0, 0.1, delay_ms=1.5, plastic=True, seed=4,
+                            mode=_abi.MODE_DETERMINISTIC)
+    for p in short._pops:
+        p.params["t_ref_ms"] = 1.0  # 10 steps: a neuron could fire twice in a 15-step batch
+    assert a.admit(ok) is not None
+    with pytest.raises(NaerError):
+        a.admit(short)
+    a.close()
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+@pytest.mark.parametrize("case", ["at_start", "mid_batch", "reused_range", "neighbour_retires", "two_in_retired_range"])
+def test_plastic_job_in_arena_equals_alone(case, graphs):
+    """A plastic job in a batching plastic arena (chunks of 50 steps against
+    15-step batches) gives its alone raster: admitted at step 0, in the
+    middle of a batch, into the ranges of a retired job, or beside a job that
+    retires while it runs."""
+    from naer_amd import models
+    from naer_amd.arena import Arena
+    job = serving.JobSpec(0, 0.0, 700, 150, 0.06, True, 1234)
+    other = serving.JobSpec(1, 0.0, 900, 200, 0.06, True, 999)
+    a = Arena(4096, 4096 * 250, dt_ms=0.1, mode=_abi.MODE_DETERMINISTIC, delay_steps=(15, 15), plastic=True,
+              stdp=models.PAIR_STDP, refractory_min=20, record_capacity=1 << 20)
+    if graphs:
+        a.set_graph(50)
+    so = None
+    if case in ("mid_batch", "neighbour_retires"):
+        so = a.admit(other.network())
+        a.run(7)
+    elif case in ("reused_range", "two_in_retired_range"):
+        big = serving.JobSpec(2, 0.0, 2500, 200, 0.06, True, 777) if case == "two_in_retired_range" else other
+        so = a.admit(big.network())
+        a.run(120)
+        a.drain()
+        a.retire(so)
+        so = None
+        if case == "two_in_retired_range":
+            # a smaller job first: `job` lands behind it, with free neurons
+            # between their ranges, inside the retired job's ranges
+            a.admit(serving.JobSpec(3, 0.0, 300, 100, 0.06, True, 555).network())
+    s = a.admit(job.network())
+    assert s is not None
+    got = []
+    for k in range(job.steps() // 50 + 1):
+        a.run(50)
+        sp = a.drain()
+        if s in sp:
+            got.append(sp[s])
+        if case == "neighbour_retires" and k == 3:
+            a.retire(so)
+    mixed = np.sort(np.concatenate(got), order=["step", "neuron", "time_us"])
+    mixed = mixed[mixed["step"] < job.steps()]
+    alone = serving.run_alone(job, _abi.MODE_DETERMINISTIC)
+    assert len(alone) > 300
+    assert first_divergence(mixed, alone) is None, (case, len(mixed), len(alone))
     a.close()

@@ -15,7 +15,7 @@ class FakeArena:
     def __init__(self, cap, plastic):
         self.cap = cap
         self.plastic = plastic
-        self.used_n = self.used_s = self.used_p = 0
+        self.used_n = self.used_s = 0
         self.slots = {}
         self.steps = 0
         self.admitted = []
@@ -23,22 +23,19 @@ class FakeArena:
     def admit(self, net):
         job = net._job
         n = (job.n + 1 + 255) // 256 * 256
-        if (self.used_n + n > self.cap.neurons or self.used_s + job.n_synapses() > self.cap.synapses
-                or self.used_p + job.n_plastic() > self.cap.plastic):
+        if self.used_n + n > self.cap.neurons or self.used_s + job.n_synapses() > self.cap.synapses:
             return None
         slot = max(self.slots, default=-1) + 1
-        self.slots[slot] = (n, job.n_synapses(), job.n_plastic())
+        self.slots[slot] = (n, job.n_synapses())
         self.used_n += n
         self.used_s += job.n_synapses()
-        self.used_p += job.n_plastic()
         self.admitted.append(job.index)
         return slot
 
     def retire(self, slot):
-        n, s, p = self.slots.pop(slot)
+        n, s = self.slots.pop(slot)
         self.used_n -= n
         self.used_s -= s
-        self.used_p -= p
 
     def run(self, steps):
         self.steps += steps
@@ -82,16 +79,18 @@ def test_budget_split_respects_budget():
     for budget in (1e9, 50e9, 200e9):
         s, p = serving.split_budget(budget)
         total = (s.neurons * serving.neuron_bytes(_abi.MODE_DETERMINISTIC, False) + s.synapses * 8 +
-                 p.neurons * serving.neuron_bytes(_abi.MODE_DETERMINISTIC, True) + p.synapses * 8 + p.plastic * 8)
+                 p.neurons * serving.neuron_bytes(_abi.MODE_DETERMINISTIC, True) + p.synapses * 8)
         assert total <= budget * 1.001
-        assert s.synapses > 0 and p.plastic > 0
+        assert s.synapses > 0 and p.synapses > 0
+        assert p.synapses < s.synapses  # 30 % of the jobs are plastic
 
 
-def test_footprint_counts_synapses_and_plastic_index():
+def test_footprint_counts_synapses_and_stdp_state():
     a = serving.JobSpec(0, 0.0, 10_000, 1000, 1.0, False, 1)
     b = serving.JobSpec(1, 0.0, 10_000, 1000, 1.0, True, 1)
     assert serving.footprint(a) >= 10_000 * 1000 * 8
-    assert serving.footprint(b) - serving.footprint(a) >= 0.9 * 8000 * 800 * 8
+    # STDP keeps per-neuron spike records (112 bytes), no per-synapse index
+    assert serving.footprint(b) - serving.footprint(a) == (10_001 + 255) // 256 * 256 * 112
 
 
 def test_jobs_wait_for_capacity_and_all_complete():
