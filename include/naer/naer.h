@@ -34,9 +34,12 @@
  *        int64 values (w rounded once to a multiple of 2^-32 mV), whose sum is
  *        exact and therefore independent of event order.
  *     3. STDP (plastic synapses only): pair-based additive STDP with per-neuron
- *        pre/post traces; depression at pre-synaptic spikes (fused into the
- *        scatter, the depressed weight is the one delivered), potentiation at
- *        post-synaptic spikes through a reverse (incoming) index.
+ *        pre/post traces; depression at pre-synaptic spikes (the depressed
+ *        weight is the one delivered), potentiation at post-synaptic spikes.
+ *        The device applies a synapse's potentiations when its pre-synaptic
+ *        neuron next fires (update_synapse, stdp.cl:19-26), in the same order
+ *        as the eager statement, or at a flush of all plastic synapses; the
+ *        results are those of eager pair STDP.
  *
  * Synapse storage: outgoing CSR.  row_ptr is uint64[N+1]; each synapse has a
  * uint32 word  target | delay_steps << 24 | plastic << 31  and an fp32 weight.
@@ -205,7 +208,7 @@ typedef struct naer_stats {
 
 #define NAER_K_UPDATE 0
 #define NAER_K_SCATTER 1       /* spike delivery of a batch, all its kernels */
-#define NAER_K_POTENTIATE 2
+#define NAER_K_POTENTIATE 2     /* plastic nets: flush of pending potentiation */
 /* the parts of NAER_K_SCATTER when a batch is delivered through buckets */
 #define NAER_K_BUCKET_HIST 3
 #define NAER_K_BUCKET_SCAN 4
@@ -358,12 +361,14 @@ int naer_net_raster_close(naer_net* net, uint64_t* n_total);
 
 /* Copy neuron state.  Any pointer may be NULL.  Arrays have n_neurons
  * entries; POISSON neurons report v = 0 and refractory = 0.  pre/post are the
- * STDP traces (zeros when the net has no plastic connection). */
+ * STDP traces the next step sees (zeros when the net has no plastic
+ * connection). */
 int naer_net_read_state(naer_net* net, float* v, int32_t* refractory, float* pre_trace,
                         float* post_trace);
 
 /* Copy the CSR (after build).  row_ptr has n_neurons+1 entries, col and
- * weight n_synapses entries; any pointer may be NULL. */
+ * weight n_synapses entries; any pointer may be NULL.  Plastic weights are
+ * read after all pending potentiation has been applied (a flush). */
 int naer_net_read_csr(naer_net* net, uint64_t* row_ptr, uint32_t* col, float* weight);
 
 /* Record the synaptic input I_syn (mV, as used by the update) of every neuron
