@@ -793,8 +793,13 @@ int alloc_neurons(naer_net* n, bool any_poisson)
 
 // Buffers of the bucketed delivery: R = M + 1 batches of events in flight
 // (a batch's events are read during the M batches that follow it), e_cap
-// events each, sized for a mean rate of kBucketRateHz over all synapses.
-constexpr double kBucketRateHz = 200.0;
+// events each, sized for a mean rate over all synapses of the refractory
+// limit 1 / ((n_ref + 1) dt), kept within [kBucketRateMinHz, kBucketRateMaxHz]
+// (and within a quarter of the free memory): a batch that does not fit goes
+// through the ring's device atomics, several times slower (measured: the
+// sweep's plastic networks at 155 Hz overflowed a 200 Hz sizing in bursts).
+constexpr double kBucketRateMinHz = 200.0;
+constexpr double kBucketRateMaxHz = 500.0;
 constexpr uint32_t kMaxBuckets = 16384;
 
 // Default of the run-aggregated LDS counters: on when a wave of 64 row
@@ -844,7 +849,9 @@ int alloc_buckets(naer_net* n)
     if (n->NBK > kMaxBuckets)
         return NAER_OK;
     n->R = n->M + 1;
-    const double expect = (double)n->S_cap * n->B * n->dt_ms * 1e-3 * kBucketRateHz;
+    const double rate = std::min(kBucketRateMaxHz,
+                                 std::max(kBucketRateMinHz, 1e3 / ((n->nref_min + 1.0) * (double)n->dt_ms)));
+    const double expect = (double)n->S_cap * n->B * n->dt_ms * 1e-3 * rate;
     // the R + 1 event buffers take at most a quarter of the free device memory
     size_t free_b = 0, total_b = 0;
     HIP_TRY(hipMemGetInfo(&free_b, &total_b));
