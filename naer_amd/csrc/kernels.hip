@@ -1574,16 +1574,6 @@ hipError_t launch_bucket_step(const BucketArgs& a, int step, bool det, hipStream
     return hipGetLastError();
 }
 
-hipError_t launch_bucketed(const BucketArgs& a, bool det, hipStream_t s)
-{
-    for (int step = 0; step < kBucketSteps; ++step) {
-        hipError_t e = launch_bucket_step(a, step, det, s);
-        if (e != hipSuccess)
-            return e;
-    }
-    return hipSuccess;
-}
-
 hipError_t launch_bucket_scrub(const UpdateArgs& a, uint32_t j_begin, uint32_t j_end, hipStream_t s)
 {
     if (j_end <= j_begin)

@@ -134,7 +134,7 @@ struct DevState {
 // times since the last one.  Traces in closed form: the trace of neuron j at
 // step t (spikes before t) is (a_s + 1) * dec^(t - s) for its last spike s < t,
 // a_s = its trace at s; dec^k from a table (0 once below 2^-40).
-constexpr uint32_t kSpikeHist = 16;
+constexpr uint32_t kSpikeHist = 32;
 
 constexpr uint32_t kNoSpike = 0x80000000u;  // step of an absent spike
 

@@ -7,12 +7,10 @@ namespace naer {
 
 // pois: the net may contain POISSON sources (otherwise their code is left out)
 hipError_t launch_update(const UpdateArgs& a, bool det, bool plastic, bool bucket, bool pois, hipStream_t s);
-// histogram, column scan, bucket starts and placement of one aligned batch
-// the bucketed delivery of one batch: steps 0 histogram, 1 scans, 2 place,
-// 3 refine (launch_bucketed runs all four)
+// the bucketed delivery of one aligned batch: steps 0 split + histogram,
+// 1 scans, 2 place, 3 refine
 constexpr int kBucketSteps = 4;
 hipError_t launch_bucket_step(const BucketArgs& a, int step, bool det, hipStream_t s);
-hipError_t launch_bucketed(const BucketArgs& a, bool det, hipStream_t s);
 hipError_t launch_bucket_scrub(const UpdateArgs& a, uint32_t j_begin, uint32_t j_end, hipStream_t s);
 hipError_t launch_scatter(const ScatterArgs& a, bool det, bool plastic, uint32_t grid_x,
                           uint32_t chunks, hipStream_t s);

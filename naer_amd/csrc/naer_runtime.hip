@@ -191,7 +191,7 @@ struct naer_net {
     uint64_t probe_t0 = 0;
     uint32_t probe_n = 0;
 
-    uint32_t scatter_chunks = 1, scatter_grid = 1;
+    uint32_t scatter_chunks = 1;
 
     // spike raster file (naer_net_raster_*)
     FILE* raster = nullptr;
@@ -891,7 +891,6 @@ int alloc_buckets(naer_net* n)
 void set_scatter_shape(naer_net* n)
 {
     n->scatter_chunks = std::max<uint32_t>(1, (n->max_row + kChunk - 1) / kChunk);
-    n->scatter_grid = std::max<uint32_t>(4, std::min<uint32_t>(1024, 2048 / n->scatter_chunks));
 }
 
 int launch_timed(naer_net* n, int kind, hipError_t (*fn)(naer_net*, void*), void* arg, bool timed)

@@ -77,7 +77,7 @@ def neuron_bytes(mode, plastic):
     lists = 2 * DELAY_STEPS * 16  # B = 15 steps in both arenas
     state = 4 * 5 + 8 + 4 + 8 + 16  # V ref last pnext pcnt, row_ptr, deg, cursor, counters
     if plastic:
-        state += 16 + 32 + 4 * 16  # STDP: pre/post records of the newest spikes, ring of 16 spike steps
+        state += 16 + 32 + 4 * 32  # STDP: pre/post records of the newest spikes, ring of 32 spike steps
     return ring + lists + state
 
 

@@ -1,8 +1,8 @@
 """Lazy pair STDP on the device against the oracle's eager statement.
 
 The device applies potentiation when the pre-synaptic neuron next fires (or
-at a flush of all plastic synapses, before any neuron can have fired 16 times
-since the last one, and before weights are read), from each neuron's last 16
+at a flush of all plastic synapses, before any neuron can have fired 32 times
+since the last one, and before weights are read), from each neuron's last 32
 spikes; plastic nets run batches of B <= the fewest refractory steps + 1
 steps.  The oracle potentiates at every post-synaptic spike through a reverse
 index.  These cases drive the device through what the lazy form depends on:
@@ -74,14 +74,14 @@ def test_irregular_runs_graphs_and_weight_reads():
         net.run(c)
         o.run(c)
         done += c
-    assert done > 5 * 15 * 21  # several flush windows of 315 steps
+    assert done > 2 * 31 * 21  # several flush windows of 651 steps
     assert_same(net, o)
     net.close()
 
 
 @pytest.mark.parametrize("t_ref_ms, batch", [(0.3, 4), (0.0, 1)])
 def test_short_refractory_short_batches(t_ref_ms, batch):
-    # B = refractory steps + 1 (4 or 1) and a flush every 15 * B steps
+    # B = refractory steps + 1 (4 or 1) and a flush every 31 * B steps
     net = plastic_net(n=2000, indegree=200, t_ref_ms=t_ref_ms, drive_hz=18_000.0)
     net.compile()
     assert net.stats().batch == batch

@@ -89,8 +89,8 @@ def test_footprint_counts_synapses_and_stdp_state():
     a = serving.JobSpec(0, 0.0, 10_000, 1000, 1.0, False, 1)
     b = serving.JobSpec(1, 0.0, 10_000, 1000, 1.0, True, 1)
     assert serving.footprint(a) >= 10_000 * 1000 * 8
-    # STDP keeps per-neuron spike records (112 bytes), no per-synapse index
-    assert serving.footprint(b) - serving.footprint(a) == (10_001 + 255) // 256 * 256 * 112
+    # STDP keeps per-neuron spike records (176 bytes), no per-synapse index
+    assert serving.footprint(b) - serving.footprint(a) == (10_001 + 255) // 256 * 256 * 176
 
 
 def test_jobs_wait_for_capacity_and_all_complete():
