@@ -1,8 +1,8 @@
 """Benchmark of the simulation step on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--n N] [--mode fast|det]
+    python bench.py [--steps K] [--warmup W] [--n N] [--mode fast|det] [--workload W]
 
-Workload: the 1M-neuron balanced LIF network (800k E / 200k I, in-degree 2,000,
+Workload (default): the 1M-neuron balanced LIF network (800k E / 200k I, in-degree 2,000,
 2e9 synapses, delays uniform over whole ms in 1-8 ms, J_E = 0.05 mV, g = 5,
 Poisson drive for ~5 Hz), fp32 state and weights, no plasticity.  The
 connectivity is built on the device before the timed region; inputs are
@@ -16,6 +16,9 @@ the CPU oracle timed on a bounded sample of the same network on the host cores.
 
     python bench.py --copy-probe   # a plain device copy of known size, the
                                    # calibration run for rocprofv3 --pmc counters
+    python bench.py --workload plastic_100k   # the plasticity network: N = 1e5,
+                                   # in-degree 1000 (1e8 synapses), pair STDP on E->E
+    python bench.py --workload balanced_10k   # the small network: N = 1e4, in-degree 1000
 """
 import argparse
 import json
@@ -60,15 +63,37 @@ def copy_probe():
     print(json.dumps({"copy_probe_bytes": COPY_PROBE_BYTES, "launches": 5}), flush=True)
 
 
+WORKLOADS = {
+    "large_1m": ("balanced LIF, N=1,000,000 (80/20 E/I), in-degree 2000, 2e9 synapses, delays 1-8 ms, "
+                 "J_E=0.05 mV, g=5"),
+    "plastic_100k": ("balanced LIF, N=100,000, in-degree 1000 (1e8 synapses), delays 1-5 ms, pair STDP on E->E "
+                     "(w0 uniform in [0, 0.3 mV])"),
+    "balanced_10k": "balanced LIF, N=10,000, in-degree 1000, J_E=0.1 mV, J_I=-0.5 mV, delay 1.5 ms",
+}
+
+
+WORKLOAD_N = {"large_1m": 1_000_000, "plastic_100k": 100_000, "balanced_10k": 10_000}
+
+
+def make_workload(name, n=None, indegree=None, **kw):
+    from naer_amd import models
+    size = {} if n is None else {"n": n}
+    if name == "large_1m":
+        return models.large_1m(**size, indegree=indegree or 2000, **kw)
+    if name == "plastic_100k":
+        return models.plastic_100k(**size, indegree=indegree or 1000, **kw)
+    return models.balanced_10k(**size, **kw)
+
+
 def cpu_baseline(args):
     """The oracle (clean-room C restatement, OpenMP) on a bounded sample: the
     same network parameters at N = args.cpu_n, args.cpu_steps steps."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    from naer_amd import models
 
     threads = min(16, os.cpu_count() or 1)
-    net = models.large_1m(n=args.cpu_n)
+    net = make_workload(args.workload, n=min(args.cpu_n, args.n or WORKLOAD_N[args.workload]),
+                        indegree=args.indegree if args.workload == "large_1m" else None)
     o = O.OracleNet(net.description(), O.FIXED, threads=threads)
     o.run(200)  # leave the initial transient
     e0 = o.event_count
@@ -79,8 +104,7 @@ def cpu_baseline(args):
     o.close()
     return {"value": ev / dt, "unit": "synaptic events/s", "cores": threads, "kind": "port",
             "sample": f"oracle (fp32 + fixed-point ring, OpenMP) on the same network at "
-                      f"N={args.cpu_n} (in-degree 2000), {args.cpu_steps} steps after 200 "
-                      f"warm-up steps, {dt:.1f} s",
+                      f"N={net.getNeurons()}, {args.cpu_steps} steps after 200 warm-up steps, {dt:.1f} s",
             "rtf": args.cpu_steps * 1e-4 / dt}
 
 
@@ -89,8 +113,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10_000)
     ap.add_argument("--warmup", type=int, default=1_000)
-    ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--indegree", type=int, default=2000)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="large_1m")
+    ap.add_argument("--n", type=int, default=None)
+    ap.add_argument("--indegree", type=int, default=None)
     ap.add_argument("--mode", choices=["fast", "det"], default="fast")
     ap.add_argument("--cpu-n", type=int, default=50_000)
     ap.add_argument("--cpu-steps", type=int, default=30_000)  # ~10-15 s of oracle work
@@ -112,7 +137,7 @@ def main():
 
     mode = _abi.MODE_FAST if args.mode == "fast" else _abi.MODE_DETERMINISTIC
     t = time.perf_counter()
-    net = models.large_1m(n=args.n, indegree=args.indegree, mode=mode)
+    net = make_workload(args.workload, n=args.n, indegree=args.indegree, mode=mode)
     net.compile()
     build_s = time.perf_counter() - t
     print(f"# built N={net.getNeurons()} S={net.getSynapses()} in {build_s:.1f} s", file=sys.stderr,
@@ -181,7 +206,8 @@ def main():
         dl_us = dl_ms * 1e3 / max(dl_n, 1)
         ach = dl_bytes / (dl_us * 1e-6) / 1e9 if dl_us > 0 else 0.0
         roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
+                    "frac": round(ach / HBM_PEAK_GBS, 4),
+                    "traffic": measured_traffic() if args.workload == "large_1m" and args.n is None else None,
                     "kernel": ("spike delivery: k_bucket_hist + k_bucket_colscan + k_bucket_starts + "
                                "k_bucket_place + k_bucket_refine") if bucketed else "k_scatter",
                     "avg_us": round(dl_us, 2), "bytes_per_launch": round(dl_bytes),
@@ -195,7 +221,8 @@ def main():
             cpu = {"error": str(e)}
 
     out = {
-        "metric": "synaptic events/s & realtime factor, 1M LIF net",
+        "metric": "synaptic events/s & realtime factor, 1M LIF net" if args.workload == "large_1m"
+                  else f"synaptic events/s & realtime factor, {args.workload}",
         "value": round(value, 1),
         "unit": "synaptic events/s",
         "n_gpus": 1,
@@ -207,8 +234,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (device-built connectivity and Poisson drive from fixed seeds)",
-        "config": {"workload": "large_1m: balanced LIF, N=1,000,000 (80/20 E/I), in-degree 2000, "
-                               "2e9 synapses, delays 1-8 ms, J_E=0.05 mV, g=5, 1 s at dt=0.1 ms",
+        "config": {"workload": f"{args.workload}: {WORKLOADS[args.workload]}, {steps * dt_ms * 1e-3:g} s at "
+                               f"dt={dt_ms:g} ms",
                    "n_neurons": N, "n_synapses": int(net.getSynapses()), "mode": args.mode,
                    "dt_ms": dt_ms},
         "realtime_factor": round(rtf, 3),
