@@ -709,12 +709,15 @@ int init_net(naer_net* n, const NetInfo& net)
 // Steps per update launch and update tile width.  The inputs of a batch
 // were all delivered before it starts when B <= the minimum delay; in plastic
 // nets B <= the fewest refractory steps + 1, so that a neuron fires at most
-// once per batch (lazy STDP).  Large nets (N >=
-// 2^18) use 1024-neuron tiles; when the minimum delay allows fewer than
-// kLongBatch steps per batch, B <= kMaxBatchLargeTile keeps the update's
-// [B][1024] 64-bit LDS accumulators of two tiles within one CU (measured:
-// 1M neurons, d_min 10: 9 steps and two tiles per CU beat 10 steps and one),
-// while longer batches keep their length (fewer batches, less fixed cost).
+// once per batch (lazy STDP).  Large nets (N >= 2^18) whose minimum delay
+// allows fewer than kLongBatch steps per batch use 1024-neuron tiles with B
+// <= kMaxBatchLargeTile, which keeps the update's [B][1024] 64-bit LDS
+// accumulators of two tiles within one CU (measured: 1M neurons, d_min 10:
+// 9 steps and two tiles per CU beat 10 steps and one).  Longer batches keep
+// their length (fewer batches, less fixed cost) on 256-neuron tiles
+// (measured on the sweep's packs of 1.4-1.6M neurons, B = 15: 256-neuron
+// tiles 0.63 / 1.02 ms per step (static / plastic pack), 512: 0.63 / 1.05,
+// 1024: 0.69 / 1.22).
 constexpr uint32_t kLongBatch = 12;
 
 uint32_t batch_steps(bool plastic, uint32_t dmin, uint64_t N, uint32_t nref_min)
@@ -727,9 +730,9 @@ uint32_t batch_steps(bool plastic, uint32_t dmin, uint64_t N, uint32_t nref_min)
     return std::max(b, 1u);
 }
 
-uint32_t tile_shift(uint64_t N, uint32_t)
+uint32_t tile_shift(uint64_t N, uint32_t B)
 {
-    return N < (1u << 18) ? 8u : 10u;
+    return (N < (1u << 18) || B >= kLongBatch) ? 8u : 10u;
 }
 
 // Allocation of everything sized by the neuron capacity.
