@@ -730,9 +730,21 @@ uint32_t batch_steps(bool plastic, uint32_t dmin, uint64_t N, uint32_t nref_min)
     return std::max(b, 1u);
 }
 
-uint32_t tile_shift(uint64_t N, uint32_t B)
+// Histogram LDS holds 4 bytes per bucket: at most kMaxBuckets (128 KB).
+constexpr uint32_t kMaxBuckets = 32768;
+
+uint32_t tile_shift(uint64_t N, uint32_t B, uint32_t dmax)
 {
-    return (N < (1u << 18) || B >= kLongBatch) ? 8u : 10u;
+    if (N >= (1u << 18) && B < kLongBatch)
+        return 10u;
+    // the smallest tile whose buckets (arrival batches x tiles, whole groups
+    // of 2^14 neurons) fit the histogram, e.g. for an arena of 2e7 neurons
+    const uint64_t M = (B - 1 + dmax) / std::max(B, 1u);
+    const uint64_t groups = (N + (1u << kGroupNeuronShift) - 1) >> kGroupNeuronShift;
+    for (uint32_t shift = 8; shift < 10; ++shift)
+        if (M * (groups << (kGroupNeuronShift - shift)) <= kMaxBuckets)
+            return shift;
+    return 10u;
 }
 
 // Allocation of everything sized by the neuron capacity.
@@ -771,7 +783,7 @@ int alloc_neurons(naer_net* n, bool any_poisson)
     const uint64_t epochs = ((uint64_t)n->B * n->dt_us + kEpochUs - 1) / kEpochUs + 1;
     const uint64_t per_neuron = any_poisson ? std::max<uint64_t>(n->B, kMaxPoissonPerEpoch * epochs) : n->B;
     // update tiles of 256, 512 or 1024 neurons (tile_shift)
-    n->tw_shift = tile_shift(N, n->B);
+    n->tw_shift = tile_shift(N, n->B, n->dmax);
     const uint32_t TW = 1u << n->tw_shift;
     n->seg_cap = (uint32_t)(TW * per_neuron);
     n->n_seg = (N + TW - 1) / TW;
@@ -803,7 +815,6 @@ int alloc_neurons(naer_net* n, bool any_poisson)
 // sweep's plastic networks at 155 Hz overflowed a 200 Hz sizing in bursts).
 constexpr double kBucketRateMinHz = 200.0;
 constexpr double kBucketRateMaxHz = 500.0;
-constexpr uint32_t kMaxBuckets = 16384;
 
 // Default of the run-aggregated LDS counters: on when a wave of 64 row
 // synapses is expected to hold runs of >= 4 equal bucket keys, i.e. a row's
@@ -874,7 +885,8 @@ int alloc_buckets(naer_net* n)
     n->U = kBucketUnroll;
     n->RS = std::max<uint32_t>(1, std::min<uint32_t>(64, 2048 / std::max<uint32_t>(1, n->NBK >> n->gt_shift)));
     n->AG = run_aggregation(n);
-    if (n->M > kMaxArrivalBatches || n->G > kMaxSegPerBlock || (n->NBK >> n->gt_shift) > 1024)
+    // placement counters: one per group bucket in LDS (<= 32 KB without raising the limit)
+    if (n->M > kMaxArrivalBatches || n->G > kMaxSegPerBlock || (n->NBK >> n->gt_shift) > 8192)
         return NAER_OK;
     int rc;
     if ((rc = n->alloc(&n->ev, (size_t)n->R * n->e_cap)) || (rc = n->alloc(&n->stage, n->e_cap)) ||
