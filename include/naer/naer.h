@@ -335,7 +335,7 @@ int naer_net_read_spikes(naer_net* net, naer_spike* out, uint64_t cap, uint64_t*
 typedef struct naer_schedule {
     uint32_t bucketed;           /* out: 1 if the net delivers through buckets */
     uint32_t task_synapses;      /* synapses of a row per wave task: 256 or 512 */
-    uint32_t segments_per_block; /* spike-list segments per histogram/placement block, 1..64 */
+    uint32_t segments_per_block; /* mean spike-list segments per histogram/placement block, 1..4096 */
     uint32_t refine_split;       /* refine blocks per group bucket, 1..64 */
     uint32_t run_aggregate;      /* 1: one LDS counter atomic per run of equal bucket keys in a
                                     wave (rows concentrated on few tiles), 0: one per synapse */

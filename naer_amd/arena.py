@@ -84,6 +84,10 @@ class Arena:
     def sync(self):
         check(_abi.lib().naer_net_sync(self._handle))
 
+    def schedule(self):
+        """Delivery schedule of the arena (naer_net_get_schedule)."""
+        return _abi.get_schedule(self._handle)
+
     def stats(self):
         s = _abi.Stats()
         check(_abi.lib().naer_net_query(self._handle, ctypes.byref(s)))

@@ -259,11 +259,31 @@ __global__ __launch_bounds__(TW, (BUCKET && !PLASTIC && !POIS && TW == 1024) ? 8
     __shared__ uint32_t s_n;
     __shared__ unsigned long long s_events;
     __shared__ unsigned long long s_base;
+    const uint32_t bp = a.n_pops ? a.blk_pop[blockIdx.x] : kTileIdle;
+    if (bp & kTileIdle) {
+        // no population on this tile (an arena's free range): no neuron to
+        // update and no spike this batch; block 0 still does the batch's
+        // bookkeeping (as below)
+        if (threadIdx.x == 0) {
+            DevState* st = a.st;
+            const uint32_t pl = st->batch & 1u;
+            a.seg_count[pl * a.n_seg + blockIdx.x] = 0;
+            a.seg_ev[pl * a.n_seg + blockIdx.x] = 0;
+            if (blockIdx.x == 0) {
+                const unsigned long long t0 = st->clock;
+                st->cur = pl;
+                st->t0_cur = t0;
+                if (PLASTIC && t0 + a.B > st->t_flush + st->window)
+                    st->t_flush = t0;
+            }
+        }
+        return;
+    }
     if (threadIdx.x == 0) {
         s_n = 0;
         s_events = 0;
     }
-    const uint32_t p_first = a.n_pops ? a.blk_pop[blockIdx.x] : 0u;
+    const uint32_t p_first = bp;
     {
         const uint32_t np = a.n_pops ? min(kStagePops, a.n_pops - p_first) : 0u;
         for (uint32_t i = threadIdx.x; i < np * NAER_DRIVE_TABLE; i += TW)
@@ -686,7 +706,7 @@ __global__ __launch_bounds__(kBlock) void k_flush(const FlushArgs a)
     for (uint32_t i = blockIdx.x * wpb + rfl(threadIdx.x / kWave); i < a.N; i += waves) {
         if (!a.n_pops)
             break;
-        const uint32_t p = find_pop(a.pops, a.n_pops, a.blk_pop[i >> a.tw_shift], i);
+        const uint32_t p = find_pop(a.pops, a.n_pops, a.blk_pop[i >> a.tw_shift] & ~kTileIdle, i);
         if (a.pops[p].kind != NAER_POP_LIF || i < a.pops[p].begin || i >= a.pops[p].end)
             continue;  // plastic synapses leave LIF neurons of active networks only
         const unsigned long long rb = a.row_ptr[i], re = a.row_ptr[i + 1];
@@ -1361,7 +1381,7 @@ __global__ __launch_bounds__(kBlock) void k_init_state(const UpdateArgs a, uint3
         return;
     float v = 0.0f;
     if (a.n_pops) {
-        const uint32_t p = find_pop(a.pops, a.n_pops, a.blk_pop[j >> a.tw_shift], j);
+        const uint32_t p = find_pop(a.pops, a.n_pops, a.blk_pop[j >> a.tw_shift] & ~kTileIdle, j);
         const PopParams pp = a.pops[p];
         if (pp.kind == NAER_POP_LIF && j >= pp.begin && j < pp.end) {
             const U4 r = philox(U4{ j - pp.net_base, 0u, NAER_STREAM_V0, 0u }, pp.seed_lo, pp.seed_hi);

@@ -136,6 +136,9 @@ struct DevState {
 // a_s = its trace at s; dec^k from a table (0 once below 2^-40).
 constexpr uint32_t kSpikeHist = 32;
 
+// Flag in the per-tile first-population table: no population overlaps the tile.
+constexpr uint32_t kTileIdle = 0x80000000u;
+
 constexpr uint32_t kNoSpike = 0x80000000u;  // step of an absent spike
 
 struct StdpArgs {
@@ -173,7 +176,7 @@ struct UpdateArgs {
     DevState* st;
     const float* cdf;                  // NAER_DRIVE_TABLE floats per population
     const PopParams* pops;             // [n_pops]
-    const uint32_t* blk_pop;           // first population of each update block
+    const uint32_t* blk_pop;           // first population of each update block (| kTileIdle)
     float* probe;
     unsigned long long rec_cap;
     unsigned long long probe_t0;
@@ -246,7 +249,8 @@ struct BucketArgs {
     uint32_t is_last;
 };
 
-constexpr uint32_t kMaxSegPerBlock = 64;  // G bound, and the segments a block stages at once
+constexpr uint32_t kMaxSegPerBlock = 64;  // segments a histogram/placement block stages at once
+constexpr uint32_t kMaxSegSchedule = 4096;  // bound of the schedule's mean segments per block
 constexpr uint32_t kMaxArrivalBatches = 64;  // M bound: (B - 1 + 127) / B for B >= 2
 
 struct ScatterArgs {

@@ -483,13 +483,19 @@ int upload_table(naer_net* n)
     }
     // first population of every update block: the last one starting at or
     // before the block's first neuron
+    // (kTileIdle: no population overlaps the tile, e.g. an arena's free
+    // ranges; the update skips it)
     n->blk_pop_h.assign(n->n_seg, 0);
     uint32_t p = 0;
+    const uint32_t TW = 1u << n->tw_shift;
     for (uint32_t b = 0; b < n->n_seg; ++b) {
         const uint32_t j = b << n->tw_shift;
         while (p + 1 < P && n->pp[p + 1].begin <= j)
             ++p;
-        n->blk_pop_h[b] = p;
+        bool active = false;
+        for (uint32_t q = p; q < P && n->pp[q].begin < j + TW && !active; ++q)
+            active = n->pp[q].end > j;
+        n->blk_pop_h[b] = p | (active ? 0u : kTileIdle);
     }
     if (P) {
         HIP_TRY(hipMemcpyAsync(n->d_pops, n->pp.data(), P * sizeof(PopParams), hipMemcpyHostToDevice, n->stream));
@@ -886,7 +892,7 @@ int alloc_buckets(naer_net* n)
     n->RS = std::max<uint32_t>(1, std::min<uint32_t>(64, 2048 / std::max<uint32_t>(1, n->NBK >> n->gt_shift)));
     n->AG = run_aggregation(n);
     // placement counters: one per group bucket in LDS (<= 32 KB without raising the limit)
-    if (n->M > kMaxArrivalBatches || n->G > kMaxSegPerBlock || (n->NBK >> n->gt_shift) > 8192)
+    if (n->M > kMaxArrivalBatches || (n->NBK >> n->gt_shift) > 8192)
         return NAER_OK;
     int rc;
     if ((rc = n->alloc(&n->ev, (size_t)n->R * n->e_cap)) || (rc = n->alloc(&n->stage, n->e_cap)) ||
@@ -1524,8 +1530,8 @@ int naer_net_set_schedule(naer_net* n, const naer_schedule* s)
         return fail(NAER_ERR_INVALID, "NULL argument");
     if (s->task_synapses != 4 * kWave && s->task_synapses != 8 * kWave)
         return fail(NAER_ERR_INVALID, "task_synapses must be %u or %u", 4 * kWave, 8 * kWave);
-    if (s->segments_per_block == 0 || s->segments_per_block > kMaxSegPerBlock)
-        return fail(NAER_ERR_INVALID, "segments_per_block must be in [1, %u]", kMaxSegPerBlock);
+    if (s->segments_per_block == 0 || s->segments_per_block > kMaxSegSchedule)
+        return fail(NAER_ERR_INVALID, "segments_per_block must be in [1, %u]", kMaxSegSchedule);
     if (s->refine_split == 0 || s->refine_split > 64)
         return fail(NAER_ERR_INVALID, "refine_split must be in [1, 64]");
     if (s->run_aggregate > 1)
@@ -1569,7 +1575,7 @@ int naer_net_autotune(naer_net* n, uint32_t batches, naer_schedule* chosen)
     std::vector<naer_schedule> cand;
     for (uint32_t task : { 4 * kWave, 8 * kWave })
         for (uint32_t g : { std::max<uint32_t>(1, base.segments_per_block / 2), base.segments_per_block,
-                            std::min<uint32_t>(kMaxSegPerBlock, base.segments_per_block * 2) })
+                            std::min<uint32_t>(kMaxSegSchedule, base.segments_per_block * 2) })
             for (uint32_t r : { std::max<uint32_t>(1, base.refine_split / 2), base.refine_split,
                                 std::min<uint32_t>(64, base.refine_split * 2) }) {
                 naer_schedule c = base;

@@ -122,3 +122,36 @@ def test_plastic_job_in_arena_equals_alone(case, graphs):
     assert len(alone) > 300
     assert first_divergence(mixed, alone) is None, (case, len(mixed), len(alone))
     a.close()
+
+
+@pytest.mark.parametrize("plastic", [False, True])
+def test_job_in_large_arena_equals_alone(plastic):
+    """An arena with the neuron capacity of the served benchmark (2e7): its
+    buckets only fit the histogram with 1024-neuron tiles, and a
+    histogram/placement block spans ~80 spike-list segments (staged 64 at a
+    time).  A job in it gives its alone raster."""
+    from naer_amd import models
+    from naer_amd.arena import Arena
+    a = Arena(20_000_000, 50_000_000, dt_ms=0.1, mode=_abi.MODE_DETERMINISTIC, delay_steps=(15, 15),
+              plastic=plastic, stdp=models.PAIR_STDP if plastic else None, refractory_min=20 if plastic else 0,
+              record_capacity=1 << 20)
+    st = a.stats()
+    assert st.batch == 15
+    assert a.schedule()["bucketed"] == 1 and a.schedule()["segments_per_block"] > 64
+    job = serving.JobSpec(0, 0.0, 3000, 300, 0.03, plastic, 4321)
+    first = serving.JobSpec(1, 0.0, 2000, 200, 0.03, plastic, 99)
+    a.admit(first.network())  # the job lands behind another one
+    a.run(11)
+    s = a.admit(job.network())
+    got = []
+    for _ in range(job.steps() // 50 + 1):
+        a.run(50)
+        sp = a.drain()
+        if s in sp:
+            got.append(sp[s])
+    mixed = np.sort(np.concatenate(got), order=["step", "neuron", "time_us"])
+    mixed = mixed[mixed["step"] < job.steps()]
+    alone = serving.run_alone(job, _abi.MODE_DETERMINISTIC)
+    assert len(alone) > 300
+    assert first_divergence(mixed, alone) is None, (len(mixed), len(alone))
+    a.close()
