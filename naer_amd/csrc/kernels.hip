@@ -12,8 +12,12 @@
 //                   stdp.cl:86-105 plus distribute_offsets
 //                   (spike_aggregation.cl:20-32): the CSR row of a source is
 //                   row_ptr[s]..row_ptr[s+1], no per-group offset table.
-//   k_potentiate <- lazy LTP of update_synapse (stdp.cl:19-23), here eager at
-//                   the post-synaptic spike.
+//   lazy_stdp / k_flush <- the lazy LTP of update_synapse (stdp.cl:19-26):
+//                   potentiation is applied when the pre-synaptic neuron next
+//                   fires, from each post neuron's last spikes (bounded by a
+//                   flush instead of a truncated history).
+//   k_bucket_*   <- the delivery of a batch of steps: events sorted by arrival
+//                   batch and target tile, summed in the update's LDS.
 //   k_conn_*     <- count_connections / integrate_connections /
 //                   generate_connections (network.cl:14-107).
 #include <mutex>

@@ -8,8 +8,9 @@ base+i, every even-numbered network with pair STDP on E->E; each runs
 
 Two ways to run it:
   packed      all static networks in one pack and all plastic networks in a
-              second pack (plasticity forces one step per launch), the two packs
-              on their own streams, steps enqueued in graph-captured chunks;
+              second pack (a flush of pending potentiation runs over the plastic
+              pack only), the two packs on their own streams, steps enqueued in
+              graph-captured chunks, delivery schedules from the autotuner;
   sequential  the plain loop: one network per launch sequence, one after the
               other.
 In deterministic mode every network's raster is bit-identical either way.

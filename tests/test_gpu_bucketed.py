@@ -24,7 +24,8 @@ pytestmark = pytest.mark.gpu
 def multi_delay_net(record=2_000_000, plastic=False):
     """8,000 neurons, in-degree 800, delays 1.5-6 ms on a 0.5 ms grid: B = 15
     steps per batch, events spread over M = 4 later batches.  With pair STDP
-    on E->E (plastic=True) every step is a batch and delivery uses the ring."""
+    on E->E (plastic=True) the batches are as long (refractory 20 steps) and the
+    placement applies lazy STDP."""
     net = models.balanced(8000, 800, 0.1, 5.0, 20_000.0, 0.1, delay_ms=(1.5, 6.0, 0.5), plastic=plastic,
                           mode=_abi.MODE_DETERMINISTIC, seed=0xB0C4E7)
     net.record_capacity = record
