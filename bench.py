@@ -124,6 +124,9 @@ def main():
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=2_000)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--graph-steps", type=int, default=0,
+                    help="steps per HIP-graph-captured chunk of the timed run (0: plain launches; measured "
+                         "on large_1m: no difference, the launches already overlap)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the simulation step of one network is not "
@@ -151,6 +154,12 @@ def main():
     if args.warmup > done:
         net.run(args.warmup - done)
     net.finish()
+    if args.graph_steps:
+        # the timed steps replay captured chunks (whole batches); the capture
+        # itself happens on the first chunk, so run one before timing
+        net.set_graph(args.graph_steps)
+        net.run(net.stats().batch * ((args.graph_steps + net.stats().batch - 1) // net.stats().batch))
+        net.finish()
     s0 = net.stats()
     warmup_run = int(s0.step)
     t0 = time.perf_counter()
@@ -174,6 +183,7 @@ def main():
         # kernel times: a further profiled stretch (HIP events around every
         # launch on the net's stream), kept out of the timed region above
         prof_steps = min(steps, args.profile_steps)
+        net.set_graph(0)  # per-launch HIP events need plain launches
         net.set_profiling(True)
         s0 = net.stats()
         net.run(prof_steps)
@@ -244,6 +254,7 @@ def main():
         "build_s": round(build_s, 2),
         "schedule": schedule,
         "warmup_steps_run": warmup_run,
+        "graph_steps": args.graph_steps,
         "roofline": roofline,
         "kernels": kernels,
         "cpu_baseline": cpu,
