@@ -7,10 +7,12 @@ base+i, every even-numbered network with pair STDP on E->E; each runs
 `steps` steps of 0.1 ms.
 
 Two ways to run it:
-  packed      all static networks in one pack and all plastic networks in a
-              second pack (a flush of pending potentiation runs over the plastic
-              pack only), the two packs on their own streams, steps enqueued in
-              graph-captured chunks, delivery schedules from the autotuner;
+  packed      all static networks in one pack and the plastic networks in two
+              more (a flush of pending potentiation runs over plastic packs
+              only; a plastic network steps about 1.6x slower, so its work is
+              split to end with the static pack's), every pack on its own
+              stream, steps enqueued in graph-captured chunks, delivery
+              schedules from the autotuner;
   sequential  the plain loop: one network per launch sequence, one after the
               other.
 In deterministic mode every network's raster is bit-identical either way.
@@ -60,14 +62,15 @@ def _progress(msg):
 
 
 def run_packed(specs, steps, mode=_abi.MODE_DETERMINISTIC, chunk=100, record=True, graphs=True, verbose=False,
-               autotune=True, packs_per_kind=1):
+               autotune=True, packs_per_kind=1, plastic_packs=None):
     groups = []
     for plastic in (False, True):
         kind = [s for s in specs if s["plastic"] == plastic]
-        # packs of a kind take every packs_per_kind-th network by size, so
-        # they carry similar work; each pack runs on its own stream
+        # packs of a kind take every k-th network by size, so they carry
+        # similar work; each pack runs on its own stream
+        k = plastic_packs if (plastic and plastic_packs) else packs_per_kind
         kind.sort(key=lambda s: s["n"])
-        groups += [kind[i::packs_per_kind] for i in range(packs_per_kind)]
+        groups += [kind[i::k] for i in range(k)]
     packs = []
     t_build = time.perf_counter()
     for g in groups:
@@ -173,11 +176,12 @@ def main():
     ap.add_argument("--no-graphs", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--packs-per-kind", type=int, default=1)
+    ap.add_argument("--plastic-packs", type=int, default=2)
     a = ap.parse_args()
     mode = _abi.MODE_DETERMINISTIC if a.mode == "det" else _abi.MODE_FAST
     specs = sweep_specs(a.nets, n_max=a.n_max)
     packed = run_packed(specs, a.steps, mode, graphs=not a.no_graphs, verbose=True, autotune=not a.no_autotune,
-                        packs_per_kind=a.packs_per_kind)
+                        packs_per_kind=a.packs_per_kind, plastic_packs=a.plastic_packs)
     out = dict(nets=a.nets, steps=a.steps, mode=a.mode, packed_run_s=packed["run_s"],
                packed_build_s=packed["build_s"], packed_jobs_per_s=a.nets / packed["run_s"],
                packs=packed["packs"], schedules=packed["schedules"])

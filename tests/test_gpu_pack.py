@@ -51,9 +51,11 @@ def test_pack_equals_alone_and_oracle():
         assert first_divergence(alone[i], o.spikes()) is None, i
 
 
-def test_split_packs_with_graphs_equal_alone():
+@pytest.mark.parametrize("plastic_packs", [1, 2])
+def test_split_packs_with_graphs_equal_alone(plastic_packs):
     specs = _specs()
-    res = sweep.run_packed(specs, STEPS, _abi.MODE_DETERMINISTIC, chunk=100, record=True, graphs=True)
+    res = sweep.run_packed(specs, STEPS, _abi.MODE_DETERMINISTIC, chunk=100, record=True, graphs=True,
+                           plastic_packs=plastic_packs)
     for s in specs:
         assert first_divergence(res["rasters"][s["index"]], _alone(s)) is None, s
 
