@@ -99,6 +99,17 @@ extern "C" {
  * property of connections here.  NeuronType::POISSON is a spike source. */
 #define NAER_POP_LIF 0u
 #define NAER_POP_POISSON 1u
+/* A neuron model given as source (naer_model_create): the reference's
+ * NeuronType plugin path, where a model's dynamics come as source strings
+ * compiled at run time (NeuronType.h:14, processing_neuron_group.cl:8-82). */
+#define NAER_POP_MODEL 2u
+
+/* Most state floats per neuron and parameter floats per population of a
+ * source-defined model. */
+#define NAER_MODEL_STATE 8
+#define NAER_MODEL_PARAMS 16
+
+typedef struct naer_model naer_model;
 
 /* Connection rules. */
 #define NAER_RULE_BERNOULLI 0u      /* pairwise Bernoulli(p), network.cl:14-107 */
@@ -125,6 +136,11 @@ typedef struct naer_pop_desc {
     /* POISSON source parameter: rate of the exponential-ISI train (Hz);
      * the reference hard-codes 20 Hz (input_neuron_group.cl:4). */
     float src_rate_hz;
+    /* NAER_POP_MODEL: the compiled model (it must live until naer_net_build
+     * returns) and the population's parameters, passed to every step.
+     * v0_min/v0_max, ext_rate_hz and ext_weight keep their LIF meaning. */
+    const naer_model* model;
+    float params[NAER_MODEL_PARAMS];
 } naer_pop_desc;
 
 typedef struct naer_conn_desc {
@@ -214,6 +230,7 @@ typedef struct naer_stats {
 #define NAER_K_BUCKET_SCAN 4
 #define NAER_K_BUCKET_PLACE 5
 #define NAER_K_BUCKET_REFINE 6
+#define NAER_K_MODEL 7          /* kernels of source-defined neuron models */
 
 /* ---- context ---------------------------------------------------------------
  * Replaces the OpenCL platform/device/context/queue set-up of
@@ -221,6 +238,38 @@ typedef struct naer_stats {
 typedef struct naer_ctx naer_ctx;
 int naer_ctx_create(int device, naer_ctx** out);
 int naer_ctx_destroy(naer_ctx* ctx);
+
+/* ---- neuron models from source ----------------------------------------------
+ * Replaces the reference's NeuronType source strings (NeuronType.h:14), whose
+ * neuron_dynamics function a processing kernel calls per input
+ * (processing_neuron_group.cl:8-82).  `source` is HIP device code defining
+ *
+ *   __device__ bool naer_model_step(float* s, float I, float ext,
+ *                                   const float* p, uint32_t t);
+ *
+ * called once per neuron and step, in step order, with
+ *   s    the neuron's n_state floats (1..NAER_MODEL_STATE).  s[0] is the
+ *        membrane potential: drawn uniformly in [v0_min, v0_max) at build
+ *        exactly as for a LIF neuron, and returned by naer_net_read_state;
+ *        the other entries start at 0;
+ *   I    the step's synaptic input (the weights arriving at that step, summed
+ *        as for LIF neurons: in 32.32 fixed point in deterministic mode);
+ *   ext  the step's Poisson drive k * ext_weight, drawn every step from the
+ *        same counter-based stream a LIF neuron uses (a LIF neuron skips the
+ *        draws of its refractory steps);
+ *   p    the population's params (NAER_MODEL_PARAMS floats);
+ *   t    the network-local step.
+ * It returns true when the neuron fires at step t; the spike is delivered,
+ * recorded and counted like a LIF spike.  Refractoriness, if any, is part of
+ * the model's state.  The source is compiled with hipRTC for gfx950 with the
+ * library's floating-point flags (no contraction into FMAs, no fast math).
+ * naer_model_create needs no device; a source that does not compile gives
+ * NAER_ERR_INVALID with the compiler's messages in naer_last_error().
+ * Model populations may be targets and sources of static connections; they
+ * are not plastic, are not admitted into arenas, and nets holding them are
+ * not checkpointed (naer_net_save/naer_net_load give NAER_ERR_INVALID). */
+int naer_model_create(const char* source, uint32_t n_state, naer_model** out);
+int naer_model_destroy(naer_model* m);
 
 /* ---- network ---------------------------------------------------------------
  * naer_net_create copies the description (pops/conns arrays may be freed

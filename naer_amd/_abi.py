@@ -21,6 +21,9 @@ NAER_ERR_NODEVICE = -6
 
 POP_LIF = 0
 POP_POISSON = 1
+POP_MODEL = 2
+MODEL_STATE = 8
+MODEL_PARAMS = 16
 RULE_BERNOULLI = 0
 RULE_FIXED_INDEGREE = 1
 MODE_FAST = 0
@@ -33,6 +36,7 @@ DRIVE_TABLE = 64
 
 K_UPDATE, K_SCATTER, K_POTENTIATE = 0, 1, 2
 K_BUCKET_HIST, K_BUCKET_SCAN, K_BUCKET_PLACE, K_BUCKET_REFINE = 3, 4, 5, 6
+K_MODEL = 7
 
 u32, u64, f32, f64 = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_float, ctypes.c_double
 
@@ -43,6 +47,7 @@ class PopDesc(ctypes.Structure):
         ("tau_m_ms", f32), ("v_th", f32), ("v_reset", f32), ("v_rest", f32), ("t_ref_ms", f32),
         ("v0_min", f32), ("v0_max", f32), ("ext_rate_hz", f32), ("ext_weight", f32),
         ("src_rate_hz", f32),
+        ("model", ctypes.c_void_p), ("params", f32 * 16),
     ]
 
 
@@ -149,6 +154,8 @@ _SIGS = {
     "naer_philox": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, u64],
     "naer_ref_poisson_epoch": [ctypes.c_void_p, u32, u32, u32, f32, u32, u32, ctypes.c_void_p,
                                ctypes.c_void_p],
+    "naer_model_create": [ctypes.c_char_p, u32, ctypes.POINTER(ctypes.c_void_p)],
+    "naer_model_destroy": [ctypes.c_void_p],
 }
 
 # every entry point include/naer/naer.h declares

@@ -35,7 +35,7 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-result",
 ]
 
-HIP_SOURCES = ["kernels.hip", "naer_runtime.hip"]
+HIP_SOURCES = ["kernels.hip", "naer_runtime.hip", "naer_model.hip"]
 CXX_DRIVER = ["Network.cpp", "main.cpp"]
 
 
@@ -60,7 +60,7 @@ def build_lib():
         objs = list(ex.map(_compile_hip, HIP_SOURCES))
     out = os.path.join(LIB_DIR, "libnaer.so")
     tmp = out + ".tmp"
-    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs])
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs, "-lhiprtc"])
     os.replace(tmp, out)
     return out
 

@@ -249,8 +249,10 @@ __device__ __forceinline__ void emit(Stage& sg, bool fire, uint32_t neuron, uint
 // slots and buckets of all B steps are read up front: nothing can arrive
 // during the batch because B <= the minimum delay.
 // ---------------------------------------------------------------------------
-template <bool DET, bool PLASTIC, bool BUCKET, bool POIS, uint32_t TW>
-__global__ __launch_bounds__(TW, (BUCKET && !PLASTIC && !POIS && TW == 1024) ? 8 : 1) void k_update(const UpdateArgs a)
+// EXTRA: the rarely used population kinds (POISSON sources, source-defined
+// models) are compiled in; nets of LIF populations only run without them.
+template <bool DET, bool PLASTIC, bool BUCKET, bool EXTRA, uint32_t TW>
+__global__ __launch_bounds__(TW, (BUCKET && !PLASTIC && !EXTRA && TW == 1024) ? 8 : 1) void k_update(const UpdateArgs a)
 {
     using RingT = typename std::conditional<DET, long long, float>::type;
     // bucketed input accumulates in 32.32 fixed point in both modes: integer
@@ -415,7 +417,11 @@ __global__ __launch_bounds__(TW, (BUCKET && !PLASTIC && !POIS && TW == 1024) ? 8
         const PopParams& pp = a.pops[pu];
         const uint32_t jl = j - rfl(pp.net_base);                          // neuron id within its net
         const uint32_t t0l = (uint32_t)(t0 - ((unsigned long long)rfl(pp.t_start_hi) << 32 | rfl(pp.t_start_lo)));
-        if (rfl(pp.kind) == NAER_POP_LIF) {
+        const uint32_t kind = rfl(pp.kind);
+        // a model neuron gets its input exactly as a LIF neuron does and
+        // leaves its dynamics to the model's kernel
+        const bool model = EXTRA && kind == NAER_POP_MODEL;
+        if (kind == NAER_POP_LIF || model) {
             RingT* ring = (RingT*)a.ring;
             uint32_t slot0 = (uint32_t)(t0 % a.D);
             // ring slots of the batch: into registers, or (bucketed) added to
@@ -472,9 +478,28 @@ __global__ __launch_bounds__(TW, (BUCKET && !PLASTIC && !POIS && TW == 1024) ? 8
             const uint32_t s_lo = rfl(pp.seed_lo), s_hi = rfl(pp.seed_hi);
             U4 rnd{ 0, 0, 0, 0 };
             uint32_t rnd_blk = 0xFFFFFFFFu;
-#pragma nounroll
-            for (uint32_t s = 0; s < a.B; ++s) {
-                const unsigned long long t = t0 + s;
+            // the drive of step s: k * w_ext, k drawn by inverting the Poisson
+            // CDF at the step's counter-based uniform
+            auto drive = [&](uint32_t s) -> float {
+                const uint32_t tl = t0l + s;
+                const uint32_t blk = (uint32_t)(tl >> 2);
+                if (blk != rnd_blk) {
+                    rnd = philox(U4{ jl, blk, NAER_STREAM_DRIVE, 0u }, s_lo, s_hi);
+                    rnd_blk = blk;
+                }
+                const float u = unit24(word_of(rnd, (uint32_t)tl & 3u));
+                uint32_t k = 0;
+#pragma unroll
+                for (uint32_t i = 0; i < kCdfHead; ++i)
+                    k += u >= head[i] ? 1u : 0u;
+                if (k == kCdfHead) {
+                    while (u >= cdf[k])
+                        ++k;
+                }
+                return (float)k * w_ext;
+            };
+            // synaptic input of the next step (steps are taken in order)
+            auto input = [&](uint32_t s) -> float {
                 float I;
                 if (BUCKET) {
                     I = from_fixed((long long)acc[s * TW + threadIdx.x]);
@@ -485,72 +510,74 @@ __global__ __launch_bounds__(TW, (BUCKET && !PLASTIC && !POIS && TW == 1024) ? 8
                         in[q] = in[q + 1];
                     I = DET ? from_fixed((long long)cur) : (float)cur;
                 }
+                const unsigned long long t = t0 + s;
                 if (a.probe && t - a.probe_t0 < a.probe_n)
                     a.probe[(size_t)(t - a.probe_t0) * a.N + j] = I;
-                bool fired = false;
-                if (ref > 0) {
-                    --ref;
-                } else {
-                    const float x = V - v_rest;
-                    const float xp = x * P;
-                    float v = xp + v_rest;
-                    v = v + I;
-                    if (has_drive) {
-                        const uint32_t tl = t0l + s;
-                        const uint32_t blk = (uint32_t)(tl >> 2);
-                        if (blk != rnd_blk) {
-                            rnd = philox(U4{ jl, blk, NAER_STREAM_DRIVE, 0u }, s_lo, s_hi);
-                            rnd_blk = blk;
-                        }
-                        const float u = unit24(word_of(rnd, (uint32_t)tl & 3u));
-                        uint32_t k = 0;
-#pragma unroll
-                        for (uint32_t i = 0; i < kCdfHead; ++i)
-                            k += u >= head[i] ? 1u : 0u;
-                        if (k == kCdfHead) {
-                            while (u >= cdf[k])
-                                ++k;
-                        }
-                        const float e = (float)k * w_ext;
-                        v = v + e;
-                    }
-                    fired = v > v_th;
-                    V = fired ? v_reset : v;
-                    if (fired) {
-                        ref = n_ref;
-                        last = (int32_t)t;
-                        my_events += (uint32_t)(a.row_ptr[j + 1] - a.row_ptr[j]);
-                        if (PLASTIC) {
-                            // traces at t (spikes before t), from the previous spike
-                            const uint32_t ns = lt.x;
-                            float xa = 0.0f, ya = 0.0f;
-                            if (ns) {
-                                const uint32_t k = (uint32_t)t - lt.y;
-                                const float ax = __uint_as_float(lt.z) + 1.0f;
-                                xa = ax * dec_pow(a.sd.tp, a.sd.n_tp, k);
-                                const float ay = __uint_as_float(ps.y) + 1.0f;
-                                ya = ay * dec_pow(a.sd.tm, a.sd.n_tm, k);
-                            }
-                            a.sd.hist[(size_t)j * kSpikeHist + (ns & (kSpikeHist - 1))] = (uint32_t)t;
-                            uint4* pb = a.sd.pst + 2 * (size_t)j + 1;
-                            const uint4 b = *pb;
-                            *pb = make_uint4(ps.z, b.x, ns + 1u, 0u);  // s0 -> s2, s2 -> s3
-                            lt = make_uint4(ns + 1u, (uint32_t)t, __float_as_uint(xa), lt.z);
-                            ps = make_uint4((uint32_t)t, __float_as_uint(ya), ps.x, ps.y);
-                            lt_new = true;
-                        }
-                    }
+                return I;
+            };
+            if (model) {
+                // the drive is drawn every step (counter-based: the same values
+                // a LIF neuron draws when it is not refractory)
+#pragma nounroll
+                for (uint32_t s = 0; s < a.B; ++s) {
+                    const float I = input(s);
+                    a.cin[(size_t)s * a.N + j] = make_float2(I, has_drive ? drive(s) : 0.0f);
                 }
-                emit(sg, fired, j, (uint32_t)t, (uint32_t)(t0l + s) * a.dt_us);
+            } else {
+#pragma nounroll
+                for (uint32_t s = 0; s < a.B; ++s) {
+                    const unsigned long long t = t0 + s;
+                    const float I = input(s);
+                    bool fired = false;
+                    if (ref > 0) {
+                        --ref;
+                    } else {
+                        const float x = V - v_rest;
+                        const float xp = x * P;
+                        float v = xp + v_rest;
+                        v = v + I;
+                        if (has_drive) {
+                            const float e = drive(s);
+                            v = v + e;
+                        }
+                        fired = v > v_th;
+                        V = fired ? v_reset : v;
+                        if (fired) {
+                            ref = n_ref;
+                            last = (int32_t)t;
+                            my_events += (uint32_t)(a.row_ptr[j + 1] - a.row_ptr[j]);
+                            if (PLASTIC) {
+                                // traces at t (spikes before t), from the previous spike
+                                const uint32_t ns = lt.x;
+                                float xa = 0.0f, ya = 0.0f;
+                                if (ns) {
+                                    const uint32_t k = (uint32_t)t - lt.y;
+                                    const float ax = __uint_as_float(lt.z) + 1.0f;
+                                    xa = ax * dec_pow(a.sd.tp, a.sd.n_tp, k);
+                                    const float ay = __uint_as_float(ps.y) + 1.0f;
+                                    ya = ay * dec_pow(a.sd.tm, a.sd.n_tm, k);
+                                }
+                                a.sd.hist[(size_t)j * kSpikeHist + (ns & (kSpikeHist - 1))] = (uint32_t)t;
+                                uint4* pb = a.sd.pst + 2 * (size_t)j + 1;
+                                const uint4 b = *pb;
+                                *pb = make_uint4(ps.z, b.x, ns + 1u, 0u);  // s0 -> s2, s2 -> s3
+                                lt = make_uint4(ns + 1u, (uint32_t)t, __float_as_uint(xa), lt.z);
+                                ps = make_uint4((uint32_t)t, __float_as_uint(ya), ps.x, ps.y);
+                                lt_new = true;
+                            }
+                        }
+                    }
+                    emit(sg, fired, j, (uint32_t)t, (uint32_t)(t0l + s) * a.dt_us);
+                }
+                a.V[j] = V;
+                a.ref[j] = ref;
+                a.last[j] = last;
             }
-            a.V[j] = V;
-            a.ref[j] = ref;
-            a.last[j] = last;
             if (PLASTIC && lt_new) {
                 a.sd.lat[j] = lt;
                 a.sd.pst[2 * (size_t)j] = ps;
             }
-        } else if (POIS) {
+        } else if (EXTRA) {
             // POISSON source: the reference train (input_neuron_group.cl:26-39)
             // evaluated window by window; state = next spike time and the
             // number already emitted in the current 1500 us epoch.
@@ -1387,7 +1414,7 @@ __global__ __launch_bounds__(kBlock) void k_init_state(const UpdateArgs a, uint3
     if (a.n_pops) {
         const uint32_t p = find_pop(a.pops, a.n_pops, a.blk_pop[j >> a.tw_shift] & ~kTileIdle, j);
         const PopParams pp = a.pops[p];
-        if (pp.kind == NAER_POP_LIF && j >= pp.begin && j < pp.end) {
+        if ((pp.kind == NAER_POP_LIF || pp.kind == NAER_POP_MODEL) && j >= pp.begin && j < pp.end) {
             const U4 r = philox(U4{ j - pp.net_base, 0u, NAER_STREAM_V0, 0u }, pp.seed_lo, pp.seed_hi);
             const float span = pp.v0_max - pp.v0_min;
             const float x = span * unit24(r.x);
@@ -1456,7 +1483,7 @@ static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b -
 // nets stepped from several host threads never lower it under each other.
 static std::mutex g_lds_mu;
 
-template <bool DET, bool PLASTIC, bool BUCKET, bool POIS, uint32_t TW>
+template <bool DET, bool PLASTIC, bool BUCKET, bool EXTRA, uint32_t TW>
 static hipError_t launch_update_t(const UpdateArgs& a, hipStream_t s)
 {
     const size_t lds = BUCKET ? (size_t)a.B * TW * sizeof(unsigned long long) : 0;
@@ -1464,41 +1491,41 @@ static hipError_t launch_update_t(const UpdateArgs& a, hipStream_t s)
     if (lds > 32768) {
         std::lock_guard<std::mutex> lock(g_lds_mu);
         if (lds > lds_set) {
-            hipError_t e = hipFuncSetAttribute((const void*)k_update<DET, PLASTIC, BUCKET, POIS, TW>,
+            hipError_t e = hipFuncSetAttribute((const void*)k_update<DET, PLASTIC, BUCKET, EXTRA, TW>,
                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
             if (e != hipSuccess)
                 return e;
             lds_set = lds;
         }
     }
-    hipLaunchKernelGGL((k_update<DET, PLASTIC, BUCKET, POIS, TW>), dim3(a.n_seg), dim3(TW), lds, s, a);
+    hipLaunchKernelGGL((k_update<DET, PLASTIC, BUCKET, EXTRA, TW>), dim3(a.n_seg), dim3(TW), lds, s, a);
     return hipGetLastError();
 }
 
 template <bool DET, bool PLASTIC, bool BUCKET>
-static hipError_t launch_update_tw(const UpdateArgs& a, bool pois, hipStream_t s)
+static hipError_t launch_update_tw(const UpdateArgs& a, bool extra, hipStream_t s)
 {
     if (a.tw_shift == 10)
-        return pois ? launch_update_t<DET, PLASTIC, BUCKET, true, 1024>(a, s)
+        return extra ? launch_update_t<DET, PLASTIC, BUCKET, true, 1024>(a, s)
                     : launch_update_t<DET, PLASTIC, BUCKET, false, 1024>(a, s);
     if (a.tw_shift == 9)
-        return pois ? launch_update_t<DET, PLASTIC, BUCKET, true, 512>(a, s)
+        return extra ? launch_update_t<DET, PLASTIC, BUCKET, true, 512>(a, s)
                     : launch_update_t<DET, PLASTIC, BUCKET, false, 512>(a, s);
-    return pois ? launch_update_t<DET, PLASTIC, BUCKET, true, 256>(a, s)
+    return extra ? launch_update_t<DET, PLASTIC, BUCKET, true, 256>(a, s)
                 : launch_update_t<DET, PLASTIC, BUCKET, false, 256>(a, s);
 }
 
-hipError_t launch_update(const UpdateArgs& a, bool det, bool plastic, bool bucket, bool pois, hipStream_t s)
+hipError_t launch_update(const UpdateArgs& a, bool det, bool plastic, bool bucket, bool extra, hipStream_t s)
 {
     if (a.tw_shift < 8 || a.tw_shift > 10)
         return hipErrorInvalidValue;
     if (plastic && bucket)
-        return det ? launch_update_tw<true, true, true>(a, pois, s) : launch_update_tw<false, true, true>(a, pois, s);
+        return det ? launch_update_tw<true, true, true>(a, extra, s) : launch_update_tw<false, true, true>(a, extra, s);
     if (plastic)
-        return det ? launch_update_tw<true, true, false>(a, pois, s) : launch_update_tw<false, true, false>(a, pois, s);
+        return det ? launch_update_tw<true, true, false>(a, extra, s) : launch_update_tw<false, true, false>(a, extra, s);
     if (bucket)
-        return det ? launch_update_tw<true, false, true>(a, pois, s) : launch_update_tw<false, false, true>(a, pois, s);
-    return det ? launch_update_tw<true, false, false>(a, pois, s) : launch_update_tw<false, false, false>(a, pois, s);
+        return det ? launch_update_tw<true, false, true>(a, extra, s) : launch_update_tw<false, false, true>(a, extra, s);
+    return det ? launch_update_tw<true, false, false>(a, extra, s) : launch_update_tw<false, false, false>(a, extra, s);
 }
 
 template <bool AGG>

@@ -196,6 +196,9 @@ struct UpdateArgs {
     const uint32_t* flags;    // [R] 1: that batch's events are in the buckets
     unsigned long long e_cap;
     uint32_t R, M, NT, NBK;
+    // neurons of source-defined models (NAER_POP_MODEL): their synaptic input
+    // and drive per step, [B][N], read by the model's own kernel
+    float2* cin;
 };
 
 // Bucketed delivery of one aligned batch E (steps [E*Bfull, (E+1)*Bfull)):

@@ -1,12 +1,16 @@
 // naer_kernels.hpp — host launch wrappers of the kernels in kernels.hip.
 #pragma once
 
+#include <string>
+#include <vector>
+
 #include "naer_device.hpp"
 
 namespace naer {
 
-// pois: the net may contain POISSON sources (otherwise their code is left out)
-hipError_t launch_update(const UpdateArgs& a, bool det, bool plastic, bool bucket, bool pois, hipStream_t s);
+// extra: the net may contain POISSON sources or source-defined models
+// (otherwise their code is left out)
+hipError_t launch_update(const UpdateArgs& a, bool det, bool plastic, bool bucket, bool extra, hipStream_t s);
 // the bucketed delivery of one aligned batch: steps 0 split + histogram,
 // 1 scans, 2 place, 3 refine
 constexpr int kBucketSteps = 4;
@@ -27,6 +31,11 @@ hipError_t launch_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out
 hipError_t launch_ref_poisson_epoch(uint32_t n, uint32_t n_pois, uint32_t seed, double neg_rate,
                                     uint32_t t0, uint32_t t1, uint32_t* times, uint32_t* count,
                                     hipStream_t s);
+
+// Compiles a source-defined neuron model (naer_model.hip) into a gfx950 code
+// object whose kernel naer_model_kernel takes one ModelArgs; on failure `log`
+// holds the compiler's messages.
+int compile_model(const char* source, uint32_t n_state, std::vector<char>* code, std::string* log);
 
 // Longest CSR row the LDS row sort accepts.
 constexpr uint32_t kMaxSortedRow = 8192;

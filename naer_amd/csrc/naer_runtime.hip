@@ -23,8 +23,13 @@
 
 #include "naer_device.hpp"
 #include "naer_kernels.hpp"
+#include "naer_model_abi.hpp"
 
 using namespace naer;
+
+namespace naer {
+NAER_MODEL_ARGS;
+}
 
 namespace {
 
@@ -109,6 +114,24 @@ struct naer_ctx {
     int device;
 };
 
+// A neuron model compiled from source (naer_model_create).
+struct naer_model {
+    std::vector<char> code;  // gfx950 code object
+    uint32_t n_state = 0;
+};
+
+// A population of a source-defined model in a built net: the module loaded
+// from the model's code object (owned by the net, so the model object may be
+// destroyed once the net is built) and the population's device state.
+struct ModelPop {
+    hipModule_t mod = nullptr;
+    hipFunction_t fn = nullptr;
+    uint32_t begin = 0, end = 0, n_state = 0;
+    uint64_t t_start = 0;
+    float* state = nullptr;   // [n_state][end - begin]
+    float* params = nullptr;  // [NAER_MODEL_PARAMS]
+};
+
 struct PendingTimer {
     int kind;
     hipEvent_t a, b;
@@ -191,6 +214,10 @@ struct naer_net {
     uint64_t probe_t0 = 0;
     uint32_t probe_n = 0;
 
+    // source-defined neuron models: staged input [B][N] and their populations
+    float2* cin = nullptr;
+    std::vector<ModelPop> mpops;
+
     uint32_t scatter_chunks = 1;
 
     // spike raster file (naer_net_raster_*)
@@ -237,6 +264,9 @@ struct naer_net {
             (void)hipEventDestroy(e);
         for (void* p : allocs)
             (void)hipFree(p);
+        for (auto& m : mpops)
+            if (m.mod)
+                (void)hipModuleUnload(m.mod);
         if (raster)
             std::fclose(raster);  // unfinished: the header keeps 0 records
         if (raster_stage)
@@ -349,9 +379,11 @@ int check_net(naer_net* n, const naer_net_desc& d, NetInfo* info)
         if (q.kind == NAER_POP_POISSON) {
             if (kEpochUs % n->dt_us != 0)
                 return fail(NAER_ERR_INVALID, "POISSON sources need dt dividing the 1500 us epoch");
-        } else if (q.kind == NAER_POP_LIF) {
-            if (!(q.tau_m_ms > 0.0f))
+        } else if (q.kind == NAER_POP_LIF || q.kind == NAER_POP_MODEL) {
+            if (q.kind == NAER_POP_LIF && !(q.tau_m_ms > 0.0f))
                 return fail(NAER_ERR_INVALID, "population %u: tau_m must be > 0", p);
+            if (q.kind == NAER_POP_MODEL && !q.model)
+                return fail(NAER_ERR_INVALID, "population %u: a MODEL population needs a model", p);
             const double lam = (double)q.ext_rate_hz * (double)d.dt_ms * 1e-3;
             if (!(lam >= 0.0 && lam <= 20.0))
                 return fail(NAER_ERR_INVALID, "population %u: drive rate*dt must be in [0, 20]", p);
@@ -366,14 +398,14 @@ int check_net(naer_net* n, const naer_net_desc& d, NetInfo* info)
         const naer_conn_desc& k = d.conns[c];
         if (k.src_pop >= d.n_pops || k.dst_pop >= d.n_pops)
             return fail(NAER_ERR_INVALID, "connection %u: population index out of range", c);
-        if (d.pops[k.dst_pop].kind != NAER_POP_LIF)
-            return fail(NAER_ERR_INVALID, "connection %u: target population must be LIF", c);
+        if (d.pops[k.dst_pop].kind == NAER_POP_POISSON)
+            return fail(NAER_ERR_INVALID, "connection %u: target population must be LIF or MODEL", c);
         const uint32_t q = k.delay_quantum ? k.delay_quantum : 1;
         if (k.delay_min < 1 || k.delay_max < k.delay_min || k.delay_max > 127 || (k.delay_max - k.delay_min) % q)
             return fail(NAER_ERR_INVALID,
                         "connection %u: delays must satisfy 1 <= min <= max <= 127 on the quantum grid", c);
-        if (k.plastic && d.pops[k.src_pop].kind != NAER_POP_LIF)
-            return fail(NAER_ERR_INVALID, "connection %u: plastic connections need a LIF source", c);
+        if (k.plastic && (d.pops[k.src_pop].kind != NAER_POP_LIF || d.pops[k.dst_pop].kind != NAER_POP_LIF))
+            return fail(NAER_ERR_INVALID, "connection %u: plastic connections need LIF source and target", c);
         if (k.rule != NAER_RULE_BERNOULLI && k.rule != NAER_RULE_FIXED_INDEGREE)
             return fail(NAER_ERR_INVALID, "connection %u: unknown rule %u", c, k.rule);
         if (k.rule == NAER_RULE_BERNOULLI && !(k.probability >= 0.0 && k.probability <= 1.0))
@@ -598,6 +630,7 @@ UpdateArgs make_update_args(naer_net* n)
     a.M = n->M;
     a.NT = n->NT;
     a.NBK = n->NBK;
+    a.cin = n->cin;
     return a;
 }
 
@@ -1066,6 +1099,45 @@ int naer_pack_layout(naer_net* n, uint32_t cap, uint32_t* neuron_begin, uint32_t
     return NAER_OK;
 }
 
+// Populations of source-defined models: staged-input buffer, per-population
+// state (s[0] = the initial potential init_state drew) and loaded modules.
+static int setup_models(naer_net* n)
+{
+    bool any = false;
+    for (auto& net : n->nets)
+        for (auto& q : net.pops)
+            any = any || (q.kind == NAER_POP_MODEL && q.size);
+    if (!any)
+        return NAER_OK;
+    int rc;
+    if ((rc = n->alloc(&n->cin, (size_t)n->B * n->N)))
+        return rc;
+    for (auto& net : n->nets) {
+        uint32_t begin = net.neuron_begin;
+        for (auto& q : net.pops) {
+            if (q.kind == NAER_POP_MODEL && q.size) {
+                ModelPop m;
+                m.begin = begin;
+                m.end = begin + q.size;
+                m.n_state = q.model->n_state;
+                m.t_start = net.t_start;
+                if ((rc = n->alloc(&m.state, (size_t)m.n_state * q.size)) ||
+                    (rc = n->alloc(&m.params, NAER_MODEL_PARAMS)))
+                    return rc;
+                HIP_TRY(hipMemcpyAsync(m.params, q.params, sizeof q.params, hipMemcpyHostToDevice, n->stream));
+                HIP_TRY(hipMemsetAsync(m.state, 0, sizeof(float) * m.n_state * q.size, n->stream));
+                HIP_TRY(hipMemcpyAsync(m.state, n->V + begin, sizeof(float) * q.size, hipMemcpyDeviceToDevice,
+                                       n->stream));
+                HIP_TRY(hipModuleLoadData(&m.mod, q.model->code.data()));
+                n->mpops.push_back(m);  // the destructor unloads the module
+                HIP_TRY(hipModuleGetFunction(&n->mpops.back().fn, n->mpops.back().mod, "naer_model_kernel"));
+            }
+            begin += q.size;
+        }
+    }
+    return NAER_OK;
+}
+
 int naer_net_build(naer_net* n)
 {
     if (!n)
@@ -1118,6 +1190,8 @@ int naer_net_build(naer_net* n)
     for (auto& net : n->nets)
         if ((rc = init_net(n, net)))
             return rc;
+    if ((rc = setup_models(n)))
+        return rc;
     HIP_TRY(hipStreamSynchronize(n->stream));
     set_scatter_shape(n);
     n->built = true;
@@ -1196,6 +1270,9 @@ int naer_arena_admit(naer_net* n, const naer_net_desc* desc, uint32_t* slot)
     int rc;
     if ((rc = check_net(n, *desc, &net)))
         return rc;
+    for (auto& q : net.pops)
+        if (q.kind == NAER_POP_MODEL)
+            return fail(NAER_ERR_INVALID, "arenas do not admit populations of source-defined models");
     if (net.dmin < n->dmin || net.dmax > n->dmax)
         return fail(NAER_ERR_INVALID, "job delays [%u, %u] outside the arena's [%u, %u]", net.dmin, net.dmax,
                     n->dmin, n->dmax);
@@ -1283,7 +1360,7 @@ int naer_arena_retire(naer_net* n, uint32_t slot)
 static hipError_t do_update(naer_net* n, void* arg)
 {
     UpdateArgs* a = (UpdateArgs*)arg;
-    return launch_update(*a, n->det, n->plastic, n->bucketed, n->poisson_ok, n->stream);
+    return launch_update(*a, n->det, n->plastic, n->bucketed, n->poisson_ok || !n->mpops.empty(), n->stream);
 }
 
 static hipError_t do_scatter(naer_net* n, void* arg)
@@ -1339,6 +1416,55 @@ int flush_now(naer_net* n)
     HIP_TRY(hipMemcpy(&st, n->st, sizeof st, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy((char*)n->st + offsetof(DevState, t_flush), &st.clock, sizeof st.clock, hipMemcpyHostToDevice));
     return NAER_OK;
+}
+
+struct ModelStep {
+    const ModelPop* m;
+    const ModelArgs* a;
+};
+
+// one block of one tile's width per tile the population overlaps
+static hipError_t do_model(naer_net* n, void* arg)
+{
+    const ModelStep* ms = (const ModelStep*)arg;
+    void* params[] = { (void*)ms->a };
+    const uint32_t tiles = ((ms->m->end - 1) >> n->tw_shift) - (ms->m->begin >> n->tw_shift) + 1;
+    return hipModuleLaunchKernel(ms->m->fn, tiles, 1, 1, 1u << n->tw_shift, 1, 1, 0, n->stream, params, nullptr);
+}
+
+static ModelArgs model_args(const naer_net* n, const ModelPop& m)
+{
+    ModelArgs a;
+    std::memset(&a, 0, sizeof a);
+    char* st = (char*)n->st;
+    a.cin = n->cin;
+    a.state = m.state;
+    a.V = n->V;
+    a.row_ptr = n->row_ptr;
+    a.list0 = (uint4*)n->list0;
+    a.list1 = (uint4*)n->list1;
+    a.seg_count = n->seg_count;
+    a.seg_ev = n->seg_ev;
+    a.spk_part = n->spk_part;
+    a.ev_part = n->ev_part;
+    a.rec = (uint4*)n->rec;
+    a.rec_n = (unsigned long long*)(st + offsetof(DevState, rec_n));
+    a.overflow = (unsigned int*)(st + offsetof(DevState, overflow));
+    a.cur = (const unsigned int*)(st + offsetof(DevState, cur));
+    a.t0_cur = (const unsigned long long*)(st + offsetof(DevState, t0_cur));
+    a.params = m.params;
+    a.rec_cap = n->rec_cap;
+    a.t_start = m.t_start;
+    a.begin = m.begin;
+    a.end = m.end;
+    a.tile0 = m.begin >> n->tw_shift;
+    a.seg_cap = n->seg_cap;
+    a.n_seg = n->n_seg;
+    a.tw_shift = n->tw_shift;
+    a.N = n->N;
+    a.B = n->B;
+    a.dt_us = n->dt_us;
+    return a;
 }
 
 struct BucketStep {
@@ -1420,6 +1546,9 @@ static int enqueue_steps(naer_net* n, uint32_t n_steps, bool timed)
     ba.balance = n->balance ? 1u : 0u;
     ba.is_last = 1;
     FlushArgs fa = flush_args(n, false);
+    std::vector<ModelArgs> ma;
+    for (const auto& m : n->mpops)
+        ma.push_back(model_args(n, m));
     int rc;
     while (n_steps > 0) {
         const uint32_t off = (uint32_t)(n->steps_enqueued % n->B);
@@ -1434,6 +1563,12 @@ static int enqueue_steps(naer_net* n, uint32_t n_steps, bool timed)
             return rc;
         if ((rc = launch_timed(n, NAER_K_UPDATE, do_update, &ua, timed)))
             return rc;
+        for (size_t i = 0; i < ma.size(); ++i) {
+            ma[i].B = b;
+            ModelStep ms{ &n->mpops[i], &ma[i] };
+            if ((rc = launch_timed(n, NAER_K_MODEL, do_model, &ms, timed)))
+                return rc;
+        }
         if (n->bucketed && off == 0 && b == n->B) {
             // timed per part; the parts add up to NAER_K_SCATTER
             for (int k = 0; k < kBucketSteps; ++k) {
@@ -1684,7 +1819,7 @@ int naer_net_query(naer_net* n, naer_stats* out)
     out->batch = n->B;
     out->max_row = n->max_row;
     out->device_bytes = n->dev_bytes;
-    out->kernels = 7;
+    out->kernels = 8;
     for (int k = 0; k < 8; ++k) {
         out->kernel_ms[k] = n->kernel_ms[k];
         out->kernel_calls[k] = n->kernel_calls[k];
@@ -1923,6 +2058,8 @@ int naer_net_save(naer_net* n, const char* path)
         return fail(NAER_ERR_STATE, "net not built");
     if (n->arena)
         return fail(NAER_ERR_STATE, "arenas are not checkpointed");
+    if (!n->mpops.empty())
+        return fail(NAER_ERR_INVALID, "nets with populations of source-defined models are not checkpointed");
     int rc = naer_net_sync(n);
     if (rc)
         return rc;
@@ -1949,6 +2086,8 @@ int naer_net_load(naer_net* n, const char* path)
         return fail(NAER_ERR_STATE, "net not built");
     if (n->arena)
         return fail(NAER_ERR_STATE, "arenas are not checkpointed");
+    if (!n->mpops.empty())
+        return fail(NAER_ERR_INVALID, "nets with populations of source-defined models are not checkpointed");
     int rc = naer_net_sync(n);
     if (rc)
         return rc;
@@ -2080,6 +2219,29 @@ int naer_net_read_probe(naer_net* n, float* out)
 int naer_net_destroy(naer_net* n)
 {
     delete n;
+    return NAER_OK;
+}
+
+int naer_model_create(const char* source, uint32_t n_state, naer_model** out)
+{
+    if (!source || !out)
+        return fail(NAER_ERR_INVALID, "NULL argument");
+    if (n_state < 1 || n_state > NAER_MODEL_STATE)
+        return fail(NAER_ERR_INVALID, "n_state must be in [1, %d]", NAER_MODEL_STATE);
+    naer_model* m = new naer_model;
+    m->n_state = n_state;
+    std::string log;
+    if (compile_model(source, n_state, &m->code, &log)) {
+        delete m;
+        return fail(NAER_ERR_INVALID, "model source does not compile:\n%s", log.c_str());
+    }
+    *out = m;
+    return NAER_OK;
+}
+
+int naer_model_destroy(naer_model* m)
+{
+    delete m;
     return NAER_OK;
 }
 

@@ -37,6 +37,47 @@ class NeuronType:
     ORDER = (STDP, NOP, POISSON)
 
 
+class NeuronModel:
+    """A neuron model given as HIP source, compiled with hipRTC
+    (naer_model_create): the reference's NeuronType source strings
+    (NeuronType.h:14), whose dynamics function its processing kernel calls
+    (processing_neuron_group.cl:8-82).
+
+    `source` defines
+
+        __device__ bool naer_model_step(float* s, float I, float ext,
+                                        const float* p, uint32_t t);
+
+    called per neuron and step with the neuron's `n_state` floats (s[0] = the
+    membrane potential, drawn in [v0_min, v0_max) at build), the step's
+    synaptic input I, its Poisson drive ext, the population's params p and the
+    network-local step t; it returns true when the neuron fires.  Pass the
+    model as the type of addPopulation, with params=[...] (up to 16 floats).
+    """
+
+    def __init__(self, source, n_state, name="model"):
+        h = ctypes.c_void_p()
+        check(_abi.lib().naer_model_create(source.encode(), int(n_state), ctypes.byref(h)))
+        self._handle = h
+        self.source = source
+        self.n_state = int(n_state)
+        self.name = name
+
+    def __repr__(self):
+        return f"NeuronModel({self.name!r}, n_state={self.n_state})"
+
+    def close(self):
+        if self._handle is not None:
+            _abi.lib().naer_model_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # Reference LIF constants (lif.cl:3-6), in volts and microseconds there.
 REFERENCE_LIF = dict(tau_m_ms=20.0, v_th=0.020, v_reset=0.0, v_rest=0.0, t_ref_ms=2.0,
                      v0_min=0.0, v0_max=0.0, ext_rate_hz=0.0, ext_weight=0.0)
@@ -124,20 +165,36 @@ class Network:
 
     # --- description (Network.cpp:76-104) ---------------------------------
     def addPopulation(self, type_, size, **params):
-        if type_ not in NeuronType.ORDER:
+        """type_: a NeuronType, or a NeuronModel (then params=[...] holds the
+        model's parameters; v0_min/v0_max/ext_rate_hz/ext_weight apply)."""
+        model = isinstance(type_, NeuronModel)
+        if not model and type_ not in NeuronType.ORDER:
             raise ValueError(f"unknown neuron type {type_!r}")
         if self._handle is not None:
             raise NaerError(_abi.NAER_ERR_STATE, "network already compiled")
         p = dict(REFERENCE_LIF)
         p["src_rate_hz"] = 20.0  # NAER_F_POISSON, input_neuron_group.cl:4
+        if model:
+            p["params"] = ()
         unknown = set(params) - set(p)
         if unknown:
             raise TypeError(f"unknown population parameters {sorted(unknown)}")
         p.update(params)
+        if model:
+            if len(p["params"]) > _abi.MODEL_PARAMS:
+                raise ValueError(f"a model takes at most {_abi.MODEL_PARAMS} parameters")
+            self._type_counts.setdefault(type_, 0)
         pop = Population(type_, self._type_counts[type_], int(size), p)
         self._type_counts[type_] += int(size)
         self._pops.append(pop)
         return pop
+
+    def _types(self):
+        """Address order of the population types: STDP, NOP, the models in
+        order of first use, POISSON (Network.cpp:269-285 for the reference's
+        three)."""
+        models = [t for t in self._type_counts if isinstance(t, NeuronModel)]
+        return [NeuronType.STDP, NeuronType.NOP, *models, NeuronType.POISSON]
 
     def addConnection(self, source, destination, defaultWeights, *args, delay_ms=None):
         """addConnection(src, dst, w, p) or addConnection(src, dst, w, wmin, wmax, p, seed)
@@ -174,13 +231,13 @@ class Network:
     def ordered_populations(self):
         """Populations in address order: by type [STDP | NOP | POISSON], then
         insertion order, as Network::compile lays them out (Network.cpp:269-285)."""
-        return [p for t in NeuronType.ORDER for p in self._pops if p.type == t]
+        return [p for t in self._types() for p in self._pops if p.type is t or p.type == t]
 
     def _desc(self):
         order = self.ordered_populations()
         type_base = {}
         acc = 0
-        for t in NeuronType.ORDER:
+        for t in self._types():
             type_base[t] = acc
             acc += self._type_counts[t]
         for p in self._pops:
@@ -189,10 +246,18 @@ class Network:
         pops = (_abi.PopDesc * max(1, len(order)))()
         for i, p in enumerate(order):
             q = p.params
+            if isinstance(p.type, NeuronModel):
+                kind = _abi.POP_MODEL
+            else:
+                kind = _abi.POP_POISSON if p.type == NeuronType.POISSON else _abi.POP_LIF
             pops[i] = _abi.PopDesc(
-                _abi.POP_POISSON if p.type == NeuronType.POISSON else _abi.POP_LIF, p.size,
+                kind, p.size,
                 q["tau_m_ms"], q["v_th"], q["v_reset"], q["v_rest"], q["t_ref_ms"], q["v0_min"],
                 q["v0_max"], q["ext_rate_hz"], q["ext_weight"], q["src_rate_hz"])
+            if kind == _abi.POP_MODEL:
+                pops[i].model = p.type._handle
+                for k, x in enumerate(q["params"]):
+                    pops[i].params[k] = float(x)
         conns = (_abi.ConnDesc * max(1, len(self._conns)))()
         for i, c in enumerate(self._conns):
             dl = c.delay or self.default_delay
