@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "Connection.h"
+#include "NeuronModel.h"
 #include "naer.h"
 
 namespace Naer {
@@ -43,6 +44,10 @@ public:
     std::shared_ptr<const Naer::Population> addPopulation(Naer::NeuronType type, uint32_t size);
     std::shared_ptr<const Naer::Population> addPopulation(Naer::NeuronType type, uint32_t size,
                                                           const LifParams& params);
+    // a population of a source-defined model (the reference's NeuronType
+    // source-string path); the model must live until compile() returns
+    std::shared_ptr<const Naer::Population> addPopulation(std::shared_ptr<const NeuronModel> model, uint32_t size,
+                                                          const ModelParams& params);
 
     void addConnection(std::shared_ptr<const Naer::Population> source,
                        std::shared_ptr<const Naer::Population> destination, float defaultWeights,
@@ -93,6 +98,7 @@ private:
     std::vector<std::shared_ptr<Population>> m_populations;
     std::vector<std::shared_ptr<Connection>> m_connections;
     uint32_t m_typeCounts[3] = { 0, 0, 0 };
+    uint32_t m_modelCount = 0;  // neurons of MODEL populations
     naer_ctx* m_ctx = nullptr;
     naer_net* m_net = nullptr;
 };

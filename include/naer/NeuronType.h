@@ -7,6 +7,8 @@
 // source with the reference's exponential-ISI trains.
 namespace Naer {
 
-enum class NeuronType { STDP = 0, NOP = 1, POISSON = 2 };
+// MODEL: a population of a source-defined neuron model (NeuronModel.h); its
+// neurons follow the NOP populations in the address order.
+enum class NeuronType { STDP = 0, NOP = 1, POISSON = 2, MODEL = 3 };
 
 }  // namespace Naer

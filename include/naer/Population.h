@@ -2,10 +2,14 @@
 // Population descriptor (reference include/naer/Population.h:9-17), plus the
 // LIF parameters the clock-driven step needs.
 #include <cstdint>
+#include <memory>
+#include <vector>
 
 #include "NeuronType.h"
 
 namespace Naer {
+
+class NeuronModel;
 
 // LIF parameters in the units of the model; the defaults are the reference's
 // constants (lif.cl:3-6): tau 20 ms, threshold 0.020 V, reset 0 V, t_ref 2 ms.
@@ -24,9 +28,12 @@ struct LifParams {
 
 struct Population {
     NeuronType type;
-    uint32_t baseAddress;  // address within its type until compile(), absolute after
+    uint32_t baseAddress;  // address within its type (or model) until compile(), absolute after
     uint32_t size;
     LifParams params;
+    // NeuronType::MODEL: the model and its parameters
+    std::shared_ptr<const NeuronModel> model;
+    std::vector<float> modelParams;
     Population(NeuronType _type, uint32_t _baseAddress, uint32_t _size, const LifParams& _params = {})
         : type(_type)
         , baseAddress(_baseAddress)

@@ -271,6 +271,18 @@ int naer_ctx_destroy(naer_ctx* ctx);
 int naer_model_create(const char* source, uint32_t n_state, naer_model** out);
 int naer_model_destroy(naer_model* m);
 
+/* Source text (static storage) of a model the library ships:
+ *   "lif"       the built-in LIF step; n_state 2, s = {V, refractory steps
+ *               left}, p = {P = exp(-dt/tau_m), v_th, v_reset, v_rest, n_ref};
+ *   "cuba_lif"  current-based LIF: a spike of weight w adds w to a synaptic
+ *               current c that decays with tau_s and charges V by w in total;
+ *               n_state 3, s = {V, refractory steps left, c}, p = {exp(-dt/
+ *               tau_m), v_th, v_reset, v_rest, n_ref, exp(-dt/tau_s),
+ *               tau_m (p[5] - p[0]) / (tau_s - tau_m)}.
+ * The reference names a CubaLIF neuron type for its vogels model
+ * (src/main.cpp:52-69) without defining it. */
+int naer_model_source(const char* name, const char** out);
+
 /* ---- network ---------------------------------------------------------------
  * naer_net_create copies the description (pops/conns arrays may be freed
  * afterwards).  It replaces addPopulation/addConnection bookkeeping

@@ -156,6 +156,7 @@ _SIGS = {
                                ctypes.c_void_p],
     "naer_model_create": [ctypes.c_char_p, u32, ctypes.POINTER(ctypes.c_void_p)],
     "naer_model_destroy": [ctypes.c_void_p],
+    "naer_model_source": [ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p)],
 }
 
 # every entry point include/naer/naer.h declares

@@ -47,9 +47,33 @@ std::shared_ptr<const Naer::Population> Naer::Network::addPopulation(Naer::Neuro
 {
     if (m_net)
         throw std::runtime_error("Couldn't add a population: network already compiled");
+    if (type == NeuronType::MODEL)
+        throw std::runtime_error("Couldn't add a population: MODEL populations are added with their model");
     auto pop = std::make_shared<Population>(type, m_typeCounts[(int)type], size, params);
     m_populations.push_back(pop);
     m_typeCounts[(int)type] += size;
+    return pop;
+}
+
+std::shared_ptr<const Naer::Population> Naer::Network::addPopulation(std::shared_ptr<const NeuronModel> model,
+                                                                     uint32_t size, const ModelParams& params)
+{
+    if (m_net)
+        throw std::runtime_error("Couldn't add a population: network already compiled");
+    if (!model)
+        throw std::runtime_error("Couldn't add a population: no model");
+    if (params.params.size() > NAER_MODEL_PARAMS)
+        throw std::runtime_error("Couldn't add a population: too many model parameters");
+    LifParams lif;
+    lif.v0_min = params.v0_min;
+    lif.v0_max = params.v0_max;
+    lif.ext_rate_hz = params.ext_rate_hz;
+    lif.ext_weight = params.ext_weight;
+    auto pop = std::make_shared<Population>(NeuronType::MODEL, 0, size, lif);
+    pop->model = model;
+    pop->modelParams = params.params;
+    m_populations.push_back(pop);
+    m_modelCount += size;
     return pop;
 }
 
@@ -86,23 +110,43 @@ void Naer::Network::compile()
 {
     if (m_net)
         throw std::runtime_error("Couldn't compile: network already compiled");
-    // address layout by type [STDP | NOP | POISSON] (src/Network.cpp:269-285)
-    uint32_t typeBase[3];
-    typeBase[0] = 0;
-    typeBase[1] = m_typeCounts[0];
-    typeBase[2] = m_typeCounts[0] + m_typeCounts[1];
+    // address layout by type [STDP | NOP | POISSON] (src/Network.cpp:269-285),
+    // with the MODEL populations after NOP, grouped by model in order of first
+    // use
     std::vector<std::shared_ptr<Population>> order;
-    for (int t = 0; t < 3; ++t)
+    for (int t : { 0, 1 })
         for (auto& p : m_populations)
             if ((int)p->type == t)
                 order.push_back(p);
+    std::vector<const NeuronModel*> models;
+    for (auto& p : m_populations)
+        if (p->type == NeuronType::MODEL &&
+            std::find(models.begin(), models.end(), p->model.get()) == models.end())
+            models.push_back(p->model.get());
+    for (const NeuronModel* m : models)
+        for (auto& p : m_populations)
+            if (p->type == NeuronType::MODEL && p->model.get() == m)
+                order.push_back(p);
+    for (auto& p : m_populations)
+        if (p->type == NeuronType::POISSON)
+            order.push_back(p);
     std::vector<naer_pop_desc> pops;
+    uint32_t address = 0;
     for (auto& p : order) {
-        p->baseAddress += typeBase[(int)p->type];
+        p->baseAddress = address;
+        address += p->size;
         const LifParams& q = p->params;
-        pops.push_back(naer_pop_desc{ p->type == NeuronType::POISSON ? NAER_POP_POISSON : NAER_POP_LIF, p->size,
-                                      q.tau_m_ms, q.v_th, q.v_reset, q.v_rest, q.t_ref_ms, q.v0_min, q.v0_max,
-                                      q.ext_rate_hz, q.ext_weight, q.src_rate_hz });
+        const uint32_t kind = p->type == NeuronType::POISSON ? NAER_POP_POISSON
+                            : p->type == NeuronType::MODEL   ? NAER_POP_MODEL
+                                                             : NAER_POP_LIF;
+        naer_pop_desc d{ kind, p->size, q.tau_m_ms, q.v_th, q.v_reset, q.v_rest, q.t_ref_ms, q.v0_min, q.v0_max,
+                         q.ext_rate_hz, q.ext_weight, q.src_rate_hz };
+        if (p->type == NeuronType::MODEL) {
+            d.model = p->model->handle();
+            for (size_t k = 0; k < p->modelParams.size(); ++k)
+                d.params[k] = p->modelParams[k];
+        }
+        pops.push_back(d);
     }
     auto index_of = [&](const std::shared_ptr<const Population>& p) -> uint32_t {
         for (size_t i = 0; i < order.size(); ++i)
@@ -211,5 +255,5 @@ uint32_t Naer::Network::getSynapses() { return (uint32_t)stats().n_synapses; }
 
 uint32_t Naer::Network::getNeurons()
 {
-    return m_typeCounts[0] + m_typeCounts[1] + m_typeCounts[2];
+    return m_typeCounts[0] + m_typeCounts[1] + m_typeCounts[2] + m_modelCount;
 }

@@ -51,6 +51,28 @@ std::unique_ptr<Naer::Network> brunel(int num_synapses, const Naer::NetworkOptio
     return net;
 }
 
+// src/main.cpp:52-69, commented out there: its CubaLIF type is not defined.
+// Here E and I are the library's cuba_lif model (current-based LIF) with the
+// reference's LIF constants (lif.cl:3-6) and tau_s = 5 ms.
+std::unique_ptr<Naer::Network> vogels(int num_synapses, const Naer::NetworkOptions& o)
+{
+    double N = std::floor(std::sqrt(num_synapses / 0.05));
+    auto net = std::make_unique<Naer::Network>(o);
+    auto cuba = Naer::NeuronModel::builtin("cuba_lif");
+    Naer::ModelParams mp;
+    mp.params = Naer::cubaLifParams(Naer::LifParams{}, 5.0f, o.dt_ms);
+    auto P = net->addPopulation(Naer::NeuronType::POISSON, N / 2);
+    auto E = net->addPopulation(cuba, N * 4 / 10, mp);
+    auto I = net->addPopulation(cuba, N / 10, mp);
+    net->addConnection(P, E, 2.0 / N, 0.1);
+    net->addConnection(P, I, 2.0 / N, 0.1);
+    net->addConnection(E, E, 2.0 / N, 0.1);
+    net->addConnection(E, I, 2.0 / N, 0.1);
+    net->addConnection(I, E, -10.0 / N, 0.1);
+    net->addConnection(I, I, -10.0 / N, 0.1);
+    return net;
+}
+
 // Balanced fixed in-degree network in mV (tau 20 ms, theta 20, reset 10,
 // t_ref 2 ms, V0 ~ U[0,20)).
 std::unique_ptr<Naer::Network> balanced(uint32_t n, uint32_t k, float je, float g, float drive_hz, float drive_w,
@@ -128,9 +150,7 @@ int main(int argc, char* argv[])
         } else if (model == "brunel+") {
             net = brunel_plus(S, opts);
         } else if (model == "vogels") {
-            // the reference's vogels() is commented out and returns nullptr (src/main.cpp:52-69)
-            std::cout << "vogels is not available (the reference model is not defined)" << std::endl;
-            return 1;
+            net = vogels(S, opts);
         } else {
             std::cout << model << " is not a valid model name" << std::endl;
             std::cout << "you can choose from: [brunel, brunel+, vogels]" << std::endl;

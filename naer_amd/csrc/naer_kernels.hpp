@@ -36,6 +36,8 @@ hipError_t launch_ref_poisson_epoch(uint32_t n, uint32_t n_pois, uint32_t seed, 
 // object whose kernel naer_model_kernel takes one ModelArgs; on failure `log`
 // holds the compiler's messages.
 int compile_model(const char* source, uint32_t n_state, std::vector<char>* code, std::string* log);
+// Source of a model the library ships ("lif", "cuba_lif"), or NULL.
+const char* model_source(const char* name);
 
 // Longest CSR row the LDS row sort accepts.
 constexpr uint32_t kMaxSortedRow = 8192;

@@ -99,7 +99,68 @@ extern "C" __global__ void __launch_bounds__(1024) naer_model_kernel(const Model
 }
 )NAERSRC";
 
+// Models the library ships as source (naer_model_source).
+const char kLifSource[] = R"NAERSRC(// LIF as the built-in update computes it.
+// s = {V, refractory steps left}; p = {P = exp(-dt/tau_m), v_th, v_reset, v_rest, n_ref}
+__device__ bool naer_model_step(float* s, float I, float ext, const float* p, uint32_t t)
+{
+    if (s[1] > 0.0f) {
+        s[1] = s[1] - 1.0f;
+        return false;
+    }
+    const float x = s[0] - p[3];
+    const float xp = x * p[0];
+    float v = xp + p[3];
+    v = v + I;
+    v = v + ext;
+    const bool f = v > p[1];
+    s[0] = f ? p[2] : v;
+    if (f)
+        s[1] = p[4];
+    return f;
+}
+)NAERSRC";
+
+const char kCubaLifSource[] = R"NAERSRC(// Current-based LIF: a spike of weight w adds w to
+// the synaptic current c, which decays with tau_s and charges V by w in total
+// (dV/dt = -(V - v_rest)/tau_m + c/tau_s), integrated exactly over each step.
+// s = {V, refractory steps left, c};
+// p = {P22 = exp(-dt/tau_m), v_th, v_reset, v_rest, n_ref, P11 = exp(-dt/tau_s),
+//      P21 = tau_m (P11 - P22) / (tau_s - tau_m)}
+__device__ bool naer_model_step(float* s, float I, float ext, const float* p, uint32_t t)
+{
+    const float c = s[2];
+    const float cd = c * p[5];
+    s[2] = cd + I;
+    if (s[1] > 0.0f) {
+        s[1] = s[1] - 1.0f;
+        return false;
+    }
+    const float x = s[0] - p[3];
+    const float xp = x * p[0];
+    float v = xp + p[3];
+    const float q = c * p[6];
+    v = v + q;
+    v = v + ext;
+    const bool f = v > p[1];
+    s[0] = f ? p[2] : v;
+    if (f)
+        s[1] = p[4];
+    return f;
+}
+)NAERSRC";
+
 }  // namespace
+
+const char* model_source(const char* name)
+{
+    const std::string n = name;
+    if (n == "lif")
+        return kLifSource;
+    if (n == "cuba_lif")
+        return kCubaLifSource;
+    return nullptr;
+}
 
 // Compiles a model's source into a gfx950 code object; on failure `log`
 // holds the compiler's messages.

@@ -2245,6 +2245,16 @@ int naer_model_destroy(naer_model* m)
     return NAER_OK;
 }
 
+int naer_model_source(const char* name, const char** out)
+{
+    if (!name || !out)
+        return fail(NAER_ERR_INVALID, "NULL argument");
+    *out = model_source(name);
+    if (!*out)
+        return fail(NAER_ERR_INVALID, "no model named '%s' (the library ships lif and cuba_lif)", name);
+    return NAER_OK;
+}
+
 int naer_philox(naer_ctx* ctx, const uint32_t* ctr, const uint32_t* key, uint32_t* out, uint64_t n)
 {
     if (!ctx || (n && (!ctr || !key || !out)))

@@ -1,6 +1,7 @@
 """Model builders.
 
-brunel / brunel_plus follow the reference's driver models (src/main.cpp:12-50).
+brunel / brunel_plus / vogels follow the reference's driver models
+(src/main.cpp:12-69).
 The workload builders describe the benchmark networks in mV and ms:
 
     balanced_10k      N=10,000 (8,000 E / 2,000 I), in-degree 1,000, J_E=+0.1,
@@ -13,7 +14,7 @@ The workload builders describe the benchmark networks in mV and ms:
 import math
 
 from . import _abi
-from .network import Network, NeuronType, REFERENCE_SEED
+from .network import REFERENCE_LIF, REFERENCE_SEED, Network, NeuronType
 
 # Brunel LIF parameters used by every workload (tau_m 20 ms, theta 20 mV,
 # reset 10 mV, t_ref 2 ms, V0 ~ U[0, 20) mV).
@@ -35,6 +36,30 @@ def brunel(num_synapses, **net_kw):
     P = net.addPopulation(NeuronType.POISSON, int(N / 2))
     E = net.addPopulation(NeuronType.NOP, int(N * 4 / 10))
     I = net.addPopulation(NeuronType.NOP, int(N / 10))
+    net.addConnection(P, E, 2.0 / N, 0.1)
+    net.addConnection(P, I, 2.0 / N, 0.1)
+    net.addConnection(E, E, 2.0 / N, 0.1)
+    net.addConnection(E, I, 2.0 / N, 0.1)
+    net.addConnection(I, E, -10.0 / N, 0.1)
+    net.addConnection(I, I, -10.0 / N, 0.1)
+    return net
+
+
+def vogels(num_synapses, tau_s_ms=5.0, **net_kw):
+    """src/main.cpp:52-69 (commented out there: its CubaLIF type is not
+    defined): brunel's populations and connections with E and I current-based
+    LIF neurons, the library's cuba_lif model with the reference's LIF
+    constants (lif.cl:3-6) and a synaptic time constant tau_s."""
+    from . import neuron_models
+    N = math.floor(math.sqrt(num_synapses / 0.05))
+    net = Network(**net_kw)
+    cuba = neuron_models.model("cuba_lif")
+    q = REFERENCE_LIF
+    params = neuron_models.cuba_lif_params(q["tau_m_ms"], tau_s_ms, q["v_th"], q["v_reset"], q["v_rest"],
+                                           q["t_ref_ms"], net.dt_ms)
+    P = net.addPopulation(NeuronType.POISSON, int(N / 2))
+    E = net.addPopulation(cuba, int(N * 4 / 10), params=params)
+    I = net.addPopulation(cuba, int(N / 10), params=params)
     net.addConnection(P, E, 2.0 / N, 0.1)
     net.addConnection(P, I, 2.0 / N, 0.1)
     net.addConnection(E, E, 2.0 / N, 0.1)

@@ -1,5 +1,6 @@
 """The C++ drop-in (include/naer/Network.h over the C ABI) through the CLI
-driver with the reference's interface (src/main.cpp: --synapses, --model):
+driver with the reference's interface (src/main.cpp: --synapses, --model;
+vogels runs the library's current-based LIF model, compiled with hipRTC):
 the driver's network has the size and gives the spike and event counts of the
 same model built through the Python binding, which is bit-exact with the
 oracle (test_gpu_parity.py).  Deterministic mode, so the counts are exact."""
@@ -29,13 +30,15 @@ def run_driver(*args):
     return n, s, t_us, spikes, events
 
 
-@pytest.mark.parametrize("model", ["brunel", "brunel+"])
+@pytest.mark.parametrize("model", ["brunel", "brunel+", "vogels"])
 def test_driver_equals_python_binding(model):
     n, s, t_us, spikes, events = run_driver("--synapses", "1000000", "--model", model, "--mode", "det",
                                             "--seconds", "0.03")
     assert t_us == 30_000  # 20 epochs of 1500 us
     if model == "brunel":
         net = models.brunel(1_000_000, mode=_abi.MODE_DETERMINISTIC)
+    elif model == "vogels":
+        net = models.vogels(1_000_000, mode=_abi.MODE_DETERMINISTIC)
     else:
         net = models.brunel_plus(1_000_000, mode=_abi.MODE_DETERMINISTIC, stdp=DRIVER_STDP)
     net.compile()

@@ -17,7 +17,7 @@ import math
 import numpy as np
 import pytest
 
-from naer_amd import _abi
+from naer_amd import _abi, neuron_models
 from naer_amd.arena import Arena
 from naer_amd.network import Network, NeuronModel, NeuronType
 from naer_amd.pack import Pack
@@ -208,6 +208,75 @@ def test_adaptation_model_matches_numpy_replay():
         V = np.where(refr, V, np.where(fired, v_reset, v)).astype(np.float32)
         R = np.where(refr, R - f(1.0), np.where(fired, n_ref, R)).astype(np.float32)
         a = np.where(fired, (a + jump).astype(np.float32), a)
+        want += [(t, base + int(j)) for j in np.nonzero(fired)[0]]
+    got = [(int(s["step"]), int(s["neuron"])) for s in sp if base <= s["neuron"] < base + 250]
+    assert len(want) > 200
+    assert got == sorted(want)
+    assert np.array_equal(v_end, V)
+
+
+def test_shipped_lif_model_equals_builtin_lif():
+    """neuron_models' lif source and parameter helper give the built-in LIF."""
+    q = LIF
+    params = neuron_models.lif_params(q["tau_m_ms"], q["v_th"], q["v_reset"], q["v_rest"], q["t_ref_ms"], DT)
+    assert params == lif_params(q)
+    _models["lif"] = neuron_models.model("lif")  # ei_net's model: the shipped source
+    try:
+        ref_sp, ref_v = run(ei_net(1000, False, seed=9), (900,))
+        sp, v = run(ei_net(1000, True, seed=9), (900,))
+    finally:
+        del _models["lif"]
+    assert len(ref_sp) > 500
+    assert np.array_equal(sp, ref_sp)
+    assert np.array_equal(v, ref_v)
+
+
+def test_cuba_lif_matches_numpy_replay():
+    """The shipped current-based LIF, replayed in float32 numpy from the
+    probed input: the synaptic current decays by P11 and takes the step's
+    arrivals; V integrates the current of the step's start through P21.  (The
+    drive of a model population is pinned by the LIF tests above.)"""
+    steps = 1500
+    params = neuron_models.cuba_lif_params(tau_m_ms=20.0, tau_s_ms=5.0, v_th=20.0, v_reset=10.0, v_rest=0.0,
+                                           t_ref_ms=2.0, dt_ms=DT)
+    p11, p22 = math.exp(-DT / 5.0), math.exp(-DT / 20.0)
+    assert params[6] == pytest.approx(20.0 * (p11 - p22) / (5.0 - 20.0), rel=1e-6)
+    # for dt << tau_s the step charges V by ~ (dt / tau_s) * current
+    assert params[6] == pytest.approx(DT / 5.0, rel=0.02)
+    net = Network(dt_ms=DT, mode=_abi.MODE_DETERMINISTIC, seed=6, record_capacity=1 << 20)
+    src = net.addPopulation(NeuronType.POISSON, 400, src_rate_hz=400.0)
+    L = net.addPopulation(NeuronType.NOP, 300, **LIF)
+    C = net.addPopulation(neuron_models.model("cuba_lif"), 250, params=params, v0_min=0.0, v0_max=20.0)
+    net.addConnectionFixedIndegree(src, L, 40, 1.0, seed=1)
+    net.addConnectionFixedIndegree(src, C, 60, 1.5, seed=2)
+    net.addConnectionFixedIndegree(L, C, 30, 0.5, seed=3)
+    net.addConnectionFixedIndegree(C, L, 20, -0.5, seed=4)
+    net.compile()
+    base = C.address
+    v0 = net.state()["v"][base: base + 250].copy()
+    net.set_probe(0, steps)
+    net.run(steps)
+    net.finish()
+    sp = net.spikes()
+    I = net.probe()[:, base: base + 250]
+    v_end = net.state()["v"][base: base + 250]
+
+    f = np.float32
+    P22, v_th, v_reset, v_rest, n_ref, P11, P21 = (f(x) for x in params)
+    V = v0.astype(np.float32)
+    R = np.zeros(250, np.float32)
+    c = np.zeros(250, np.float32)
+    want = []
+    for t in range(steps):
+        c_old = c
+        c = ((c_old * P11).astype(np.float32) + I[t]).astype(np.float32)
+        refr = R > 0
+        v = ((V - v_rest) * P22 + v_rest).astype(np.float32)
+        v = (v + (c_old * P21).astype(np.float32)).astype(np.float32)
+        v = (v + f(0.0)).astype(np.float32)
+        fired = (~refr) & (v > v_th)
+        V = np.where(refr, V, np.where(fired, v_reset, v)).astype(np.float32)
+        R = np.where(refr, R - f(1.0), np.where(fired, n_ref, R)).astype(np.float32)
         want += [(t, base + int(j)) for j in np.nonzero(fired)[0]]
     got = [(int(s["step"]), int(s["neuron"])) for s in sp if base <= s["neuron"] < base + 250]
     assert len(want) > 200

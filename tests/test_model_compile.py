@@ -32,6 +32,39 @@ def test_model_compiles_without_device():
     m.close()
 
 
+@pytest.mark.parametrize("name", ["lif", "cuba_lif"])
+def test_shipped_models_compile(name):
+    from naer_amd import neuron_models
+    src = neuron_models.source(name)
+    assert "naer_model_step" in src
+    m = NeuronModel(src, neuron_models.N_STATE[name], name)
+    m.close()
+
+
+def test_unknown_shipped_model():
+    from naer_amd import neuron_models
+    with pytest.raises(_abi.NaerError, match="cuba_lif"):
+        neuron_models.source("hodgkin_huxley")
+
+
+def test_cuba_lif_parameters():
+    """Closed forms of the exact step: P22 = e^(-dt/tau_m), P11 = e^(-dt/tau_s),
+    P21 = tau_m (P11 - P22) / (tau_s - tau_m); as tau_s -> 0 the current model
+    becomes a voltage jump one step later (P21 -> P22)."""
+    import math
+    from naer_amd import neuron_models
+    p = neuron_models.cuba_lif_params(20.0, 5.0, 20.0, 10.0, 0.0, 2.0, 0.1)
+    assert p[4] == 20.0
+    assert p[0] == pytest.approx(math.exp(-0.1 / 20.0), rel=1e-7)
+    assert p[5] == pytest.approx(math.exp(-0.1 / 5.0), rel=1e-7)
+    p11, p22 = math.exp(-0.1 / 5.0), math.exp(-0.1 / 20.0)
+    assert p[6] == pytest.approx(20.0 * (p11 - p22) / (5.0 - 20.0), rel=1e-5)
+    q = neuron_models.cuba_lif_params(20.0, 1e-4, 20.0, 10.0, 0.0, 2.0, 0.1)
+    assert q[6] == pytest.approx(q[0], rel=1e-3)
+    with pytest.raises(ValueError):
+        neuron_models.cuba_lif_params(20.0, 20.0, 20.0, 10.0, 0.0, 2.0, 0.1)
+
+
 def test_compile_error_carries_the_compiler_log():
     bad = STEP.replace("p[2]", "p[2] + undeclared_thing")
     with pytest.raises(_abi.NaerError) as e:
