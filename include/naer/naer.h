@@ -118,6 +118,20 @@ typedef struct naer_model naer_model;
 /* Accumulation modes of the spike scatter. */
 #define NAER_MODE_FAST 0u
 #define NAER_MODE_DETERMINISTIC 1u
+/* The reference's event-driven step at microsecond resolution
+ * (src/Network.cpp:122-256, lif.cl:15-34): naer_net_step advances whole
+ * 1500 us windows (the reference's step()).  A window's inputs are the
+ * POISSON spikes timed in it and the LIF spikes of the previous window,
+ * delivered 1500 us after firing (NAER_AXONAL_DELAY; connection delays are
+ * not used).  Each target takes its inputs in time order (equal times: by
+ * source, then row position): dt = t - t_last; dt < 0 (refractory) drops
+ * the input; else V = v_rest + (V - v_rest) e^(-dt/tau_m) (product in double),
+ * V += w, t_last = t; V > v_th fires: V = v_reset, t_last = t + t_ref.
+ * e^(-dt/tau_m) comes from two tables, exp(-4096 h / tau) * exp(-l / tau)
+ * with dt = 4096 h + l.  Populations: LIF without drive, POISSON; no STDP,
+ * no models, no graphs.  Recorded spikes carry step = the window and
+ * time_us = the firing time. */
+#define NAER_MODE_EVENT 2u
 
 /* ---- network description -------------------------------------------------- */
 typedef struct naer_pop_desc {

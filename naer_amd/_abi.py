@@ -28,6 +28,7 @@ RULE_BERNOULLI = 0
 RULE_FIXED_INDEGREE = 1
 MODE_FAST = 0
 MODE_DETERMINISTIC = 1
+MODE_EVENT = 2
 
 STREAM_V0 = 1
 STREAM_DRIVE = 2

@@ -35,7 +35,7 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-result",
 ]
 
-HIP_SOURCES = ["kernels.hip", "naer_runtime.hip", "naer_model.hip"]
+HIP_SOURCES = ["kernels.hip", "naer_runtime.hip", "naer_model.hip", "kernels_event.hip"]
 CXX_DRIVER = ["Network.cpp", "main.cpp"]
 
 

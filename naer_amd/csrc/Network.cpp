@@ -203,7 +203,10 @@ void Naer::Network::step()
 {
     if (!m_net)
         throw std::runtime_error("Couldn't step: network not compiled");
-    const uint32_t n = (uint32_t)std::lround((double)m_options.epoch_us / (1000.0 * m_options.dt_ms));
+    // the event-driven mode advances whole 1500 us windows
+    const uint32_t n = m_options.mode == NAER_MODE_EVENT
+                           ? 1u
+                           : (uint32_t)std::lround((double)m_options.epoch_us / (1000.0 * m_options.dt_ms));
     check(naer_net_step(m_net, n), "enqueue the step kernels");
 }
 

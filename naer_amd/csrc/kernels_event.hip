@@ -1,0 +1,240 @@
+// kernels_event.hip — the event-driven mode: the reference's epoch
+// (src/Network.cpp:122-256) and neuron step (lif.cl:15-34) at microsecond
+// resolution.
+//
+// One window of 1500 us per epoch:
+//   k_ev_poisson   POISSON spikes of the window (input_neuron_group.cl:26-39)
+//                  appended to the window's input list, which already holds
+//                  the LIF spikes of the previous window (arriving 1500 us
+//                  after they were fired)
+//   (radix sort of the inputs by (source, time))
+//   k_ev_rowlen    row length of every input; an exclusive scan gives each
+//                  input's range of events
+//   k_ev_expand    events {target << 11 | time offset, weight} in input order
+//   (stable radix sort of the events by (target, time): ties keep the
+//    order (source, row position))
+//   k_ev_bounds    each target's range of sorted events
+//   k_ev_process   one thread per LIF neuron applies its events in order and
+//                  appends its spikes to the next window's input list
+// The sorts and the scan are rocPRIM's (through hipCUB).
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include "naer_device.hpp"
+#include "naer_kernels.hpp"
+
+namespace naer {
+
+namespace {
+
+__device__ __forceinline__ uint32_t ev_pop(const EvArgs& a, uint32_t j)
+{
+    uint32_t p = a.blk_pop[j >> a.tw_shift] & ~kTileIdle;
+    while (p + 1 < a.n_pops && j >= a.pops[p + 1].begin)
+        ++p;
+    return p;
+}
+
+__device__ __forceinline__ void ev_record(const EvArgs& a, uint32_t j, uint32_t time)
+{
+    if (a.rec_cap) {
+        const unsigned long long r = atomicAdd(a.rec_n, 1ull);
+        if (r < a.rec_cap) {
+            naer_spike sp;
+            sp.step = a.epoch;
+            sp.neuron = j;
+            sp.time_us = time;
+            sp._pad = 0;
+            a.rec[r] = sp;
+        }
+    }
+}
+
+__device__ __forceinline__ void ev_append(const EvArgs& a, uint32_t j, uint32_t time)
+{
+    const uint32_t k = atomicAdd(a.n_out, 1u);
+    if (k < a.in_cap)
+        a.out_keys[k] = (unsigned long long)j << 32 | time;
+    else
+        atomicAdd(a.lost, 1u);
+}
+
+// POISSON sources: the reference train of the window [t0, t0 + 1500).
+__global__ __launch_bounds__(256) void k_ev_poisson(const EvArgs a)
+{
+    const uint32_t j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= a.N)
+        return;
+    const uint32_t p = ev_pop(a, j);
+    const PopParams& pp = a.pops[p];
+    if (pp.kind != NAER_POP_POISSON || j < pp.begin || j >= pp.end)
+        return;
+    const uint32_t idx = pp.pois_base + (j - pp.begin);
+    const uint32_t gid = ref_poisson_gid(idx);
+    const uint32_t t0 = a.t0, t1 = a.t0 + kEpochUs;
+    uint32_t time = t0 + ref_isi_us(pp.neg_rate_per_us, gid, pp.pois_seed, t0 * pp.n_pois + idx);
+    uint32_t spikes = 0;
+    for (uint32_t m = 0; m < kMaxPoissonPerEpoch && time < t1; ++m) {
+        ev_append(a, j, time);
+        ev_record(a, j, time);
+        ++spikes;
+        time += ref_isi_us(pp.neg_rate_per_us, gid, pp.pois_seed, time * pp.n_pois + idx);
+    }
+    if (spikes)
+        atomicAdd(a.spikes, (unsigned long long)spikes);
+}
+
+__global__ __launch_bounds__(256) void k_ev_rowlen(const EvArgs a, uint32_t n_in)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i > n_in)
+        return;
+    unsigned long long len = 0;
+    if (i < n_in) {
+        const uint32_t s = (uint32_t)(a.in_keys[i] >> 32);
+        len = a.row_ptr[s + 1] - a.row_ptr[s];
+    }
+    a.row_len[i] = len;
+}
+
+// One block per input spike walks its row.
+__global__ __launch_bounds__(256) void k_ev_expand(const EvArgs a)
+{
+    const unsigned long long key = a.in_keys[blockIdx.x];
+    const uint32_t s = (uint32_t)(key >> 32);
+    const uint32_t dt = (uint32_t)key - a.t0;  // < 1500
+    const unsigned long long r0 = a.row_ptr[s], r1 = a.row_ptr[s + 1];
+    const unsigned long long e0 = a.row_off[blockIdx.x];
+    for (unsigned long long r = r0 + threadIdx.x; r < r1; r += 256) {
+        const uint32_t tgt = a.col[r] & 0xFFFFFFu;
+        const unsigned long long e = e0 + (r - r0);
+        a.ev_keys[e] = (unsigned long long)tgt << 11 | dt;
+        a.ev_w[e] = a.w[r];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_ev_clear(const EvArgs a)
+{
+    const uint32_t j = blockIdx.x * 256 + threadIdx.x;
+    if (j < a.N) {
+        a.seg_lo[j] = 0u;
+        a.seg_hi[j] = 0u;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_ev_bounds(const EvArgs a, uint32_t n_ev)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n_ev)
+        return;
+    const uint32_t tgt = (uint32_t)(a.ev_keys[i] >> 11);
+    if (i == 0 || (uint32_t)(a.ev_keys[i - 1] >> 11) != tgt)
+        a.seg_lo[tgt] = i;
+    if (i + 1 == n_ev || (uint32_t)(a.ev_keys[i + 1] >> 11) != tgt)
+        a.seg_hi[tgt] = i + 1;
+}
+
+// lif.cl:15-34 per event, in time order; the decay e^(-dt/tau) is H[dt >> 12]
+// * L[dt & 4095] from the population's tables.
+__global__ __launch_bounds__(256) void k_ev_process(const EvArgs a)
+{
+    const uint32_t j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= a.N)
+        return;
+    const uint32_t lo = a.seg_lo[j], hi = a.seg_hi[j];
+    if (lo == hi)
+        return;
+    const uint32_t p = ev_pop(a, j);
+    const PopParams& pp = a.pops[p];
+    if (pp.kind != NAER_POP_LIF || j < pp.begin || j >= pp.end)
+        return;
+    const double* L = a.dec_lo + (size_t)p * kEvLo;
+    const double* H = a.dec_hi + (size_t)p * kEvHiCap;
+    const uint32_t n_hi = a.n_hi[p];
+    const uint32_t t_ref = a.t_ref[p];
+    const float v_rest = pp.v_rest, v_th = pp.v_th, v_reset = pp.v_reset;
+    float V = a.V[j];
+    uint32_t t_last = a.t_last[j];
+    uint32_t fired = 0;
+    for (uint32_t i = lo; i < hi; ++i) {
+        const uint32_t t = a.t0 + (uint32_t)(a.ev_keys[i] & 2047u);
+        const long long dt = (long long)t - (long long)t_last;
+        if (dt < 0)
+            continue;
+        const unsigned long long h = (unsigned long long)dt >> 12;
+        const double d = h < n_hi ? H[h] * L[dt & 4095] : 0.0;
+        float x = V - v_rest;
+        x = (float)((double)x * d);
+        float v = x + v_rest;
+        v = v + a.ev_w[i];
+        t_last = t;
+        if (v > v_th) {
+            v = v_reset;
+            t_last = t + t_ref;
+            ev_append(a, j, t + kEpochUs);
+            ev_record(a, j, t);
+            ++fired;
+        }
+        V = v;
+    }
+    a.V[j] = V;
+    a.t_last[j] = t_last;
+    if (fired)
+        atomicAdd(a.spikes, (unsigned long long)fired);
+}
+
+}  // namespace
+
+hipError_t ev_launch(const EvArgs& a, int which, uint32_t n, hipStream_t s)
+{
+    const uint32_t gn = (a.N + 255) / 256;
+    switch (which) {
+    case kEvPoisson:
+        hipLaunchKernelGGL(k_ev_poisson, dim3(gn), dim3(256), 0, s, a);
+        break;
+    case kEvRowLen:
+        hipLaunchKernelGGL(k_ev_rowlen, dim3(n / 256 + 1), dim3(256), 0, s, a, n);
+        break;
+    case kEvExpand:
+        if (n)
+            hipLaunchKernelGGL(k_ev_expand, dim3(n), dim3(256), 0, s, a);
+        break;
+    case kEvClear:
+        hipLaunchKernelGGL(k_ev_clear, dim3(gn), dim3(256), 0, s, a);
+        break;
+    case kEvBounds:
+        if (n)
+            hipLaunchKernelGGL(k_ev_bounds, dim3((n + 255) / 256), dim3(256), 0, s, a, n);
+        break;
+    case kEvProcess:
+        hipLaunchKernelGGL(k_ev_process, dim3(gn), dim3(256), 0, s, a);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// Temporary storage of the sorts and the scan for up to n_in inputs and
+// n_ev events (null `temp`: the size only).
+hipError_t ev_sort_inputs(void* temp, size_t* bytes, const unsigned long long* in, unsigned long long* out,
+                          uint32_t n, hipStream_t s)
+{
+    // keys: source (< 2^24) << 32 | time
+    return hipcub::DeviceRadixSort::SortKeys(temp, *bytes, in, out, (int)n, 0, 56, s);
+}
+
+hipError_t ev_scan_rows(void* temp, size_t* bytes, const unsigned long long* in, unsigned long long* out, uint32_t n,
+                        hipStream_t s)
+{
+    return hipcub::DeviceScan::ExclusiveSum(temp, *bytes, in, out, (int)n, s);
+}
+
+hipError_t ev_sort_events(void* temp, size_t* bytes, const unsigned long long* kin, unsigned long long* kout,
+                          const float* vin, float* vout, uint32_t n, hipStream_t s)
+{
+    // keys: target (< 2^24) << 11 | time offset (< 2048); radix sort is stable
+    return hipcub::DeviceRadixSort::SortPairs(temp, *bytes, kin, kout, vin, vout, (int)n, 0, 35, s);
+}
+
+}  // namespace naer

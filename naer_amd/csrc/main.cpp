@@ -3,7 +3,8 @@
 // builds the model, runs 10 s of simulated time with `while (getTime() <
 // 10'000'000) step();` and prints (N, S), Took and the kernel timers.
 // Additions: --workload balanced_10k|large_1m (fixed in-degree workloads,
-// 1 s), --mode fast|det, --seconds T.
+// 1 s), --mode fast|det|event (event: the reference's event-driven step),
+// --seconds T.
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
@@ -128,7 +129,8 @@ int main(int argc, char* argv[])
             workload = need("a workload from: [balanced_10k, large_1m]");
             seconds = 1.0;
         } else if (a == "--mode") {
-            opts.mode = need("a mode from: [fast, det]") == "det" ? NAER_MODE_DETERMINISTIC : NAER_MODE_FAST;
+            const std::string m = need("a mode from: [fast, det, event]");
+            opts.mode = m == "det" ? NAER_MODE_DETERMINISTIC : m == "event" ? NAER_MODE_EVENT : NAER_MODE_FAST;
         } else if (a == "--seconds") {
             seconds = std::atof(need("a duration").c_str());
         }

@@ -201,6 +201,43 @@ struct UpdateArgs {
     float2* cin;
 };
 
+// Event-driven mode (kernels_event.hip): one 1500 us window per epoch.
+constexpr uint32_t kEvLo = 4096;    // entries of a population's L decay table
+constexpr uint32_t kEvHiCap = 8192; // most entries of its H table
+enum { kEvPoisson = 0, kEvRowLen, kEvExpand, kEvClear, kEvBounds, kEvProcess };
+
+struct EvArgs {
+    const PopParams* pops;
+    const uint32_t* blk_pop;
+    uint32_t n_pops, tw_shift, N;
+    uint32_t t0;     // window start (us)
+    uint32_t epoch;
+    uint32_t in_cap; // capacity of an input list
+    const unsigned long long* row_ptr;
+    const uint32_t* col;
+    const float* w;
+    const unsigned long long* in_keys;  // the window's inputs sorted by (source, time)
+    unsigned long long* out_keys;       // input list being appended to
+    uint32_t* n_out;                    // its count
+    uint32_t* lost;                     // inputs beyond in_cap
+    unsigned long long* row_len;        // [in_cap + 1]
+    const unsigned long long* row_off;  // [in_cap + 1] exclusive scan of row_len
+    unsigned long long* ev_keys;        // target << 11 | time offset
+    float* ev_w;
+    uint32_t* seg_lo;                   // [N] a target's range of sorted events
+    uint32_t* seg_hi;
+    float* V;
+    uint32_t* t_last;                   // [N] time of the last input taken (us)
+    const double* dec_lo;               // [n_pops][kEvLo]
+    const double* dec_hi;               // [n_pops][kEvHiCap]
+    const uint32_t* n_hi;               // [n_pops]
+    const uint32_t* t_ref;              // [n_pops] refractory period (us)
+    naer_spike* rec;
+    unsigned long long rec_cap;
+    unsigned long long* rec_n;
+    unsigned long long* spikes;
+};
+
 // Bucketed delivery of one aligned batch E (steps [E*Bfull, (E+1)*Bfull)):
 // every synaptic event (target j, arrival step a = t + d) ends up in bucket
 // k = (m - 1) * NT + T, m = a/Bfull - E in [1, M], T = the target's tile, as

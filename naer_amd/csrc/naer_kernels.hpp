@@ -39,6 +39,17 @@ int compile_model(const char* source, uint32_t n_state, std::vector<char>* code,
 // Source of a model the library ships ("lif", "cuba_lif"), or NULL.
 const char* model_source(const char* name);
 
+// Event-driven mode (kernels_event.hip): kernel `which` (kEv*), with n the
+// input or event count where the kernel needs one; the sorts and the scan
+// report their temporary storage size when `temp` is null.
+hipError_t ev_launch(const EvArgs& a, int which, uint32_t n, hipStream_t s);
+hipError_t ev_sort_inputs(void* temp, size_t* bytes, const unsigned long long* in, unsigned long long* out,
+                          uint32_t n, hipStream_t s);
+hipError_t ev_scan_rows(void* temp, size_t* bytes, const unsigned long long* in, unsigned long long* out, uint32_t n,
+                        hipStream_t s);
+hipError_t ev_sort_events(void* temp, size_t* bytes, const unsigned long long* kin, unsigned long long* kout,
+                          const float* vin, float* vout, uint32_t n, hipStream_t s);
+
 // Longest CSR row the LDS row sort accepts.
 constexpr uint32_t kMaxSortedRow = 8192;
 

@@ -218,6 +218,22 @@ struct naer_net {
     float2* cin = nullptr;
     std::vector<ModelPop> mpops;
 
+    // event-driven mode (NAER_MODE_EVENT): 1500 us windows
+    bool event = false;
+    uint32_t ev_in_cap = 0;
+    uint64_t ev_e_cap = 0, ev_epoch = 0, ev_events = 0;
+    unsigned long long *ev_list[2] = { nullptr, nullptr }, *ev_sorted = nullptr;
+    unsigned long long *ev_len = nullptr, *ev_off = nullptr;
+    unsigned long long *ev_keys = nullptr, *ev_keys_s = nullptr;
+    float *ev_w = nullptr, *ev_w_s = nullptr;
+    uint32_t *ev_lo = nullptr, *ev_hi = nullptr, *ev_tlast = nullptr;
+    double *ev_dlo = nullptr, *ev_dhi = nullptr;
+    uint32_t *ev_nhi = nullptr, *ev_tref = nullptr;
+    // [0], [1]: counts of the two input lists, [2]: inputs lost; spikes at [4] (u64)
+    uint32_t* ev_ctr = nullptr;
+    void* ev_temp = nullptr;
+    size_t ev_temp_bytes = 0;
+
     uint32_t scatter_chunks = 1;
 
     // spike raster file (naer_net_raster_*)
@@ -349,12 +365,13 @@ int check_settings(naer_net* n, float dt_ms, uint32_t mode)
     const uint32_t dt_us = (uint32_t)std::llround(dtus);
     if (!(dt_ms > 0.0f) || std::fabs(dtus - (double)dt_us) > 1e-3 || dt_us == 0)
         return fail(NAER_ERR_INVALID, "dt must be a positive whole number of microseconds");
-    if (mode != NAER_MODE_FAST && mode != NAER_MODE_DETERMINISTIC)
+    if (mode != NAER_MODE_FAST && mode != NAER_MODE_DETERMINISTIC && mode != NAER_MODE_EVENT)
         return fail(NAER_ERR_INVALID, "unknown mode %u", mode);
     n->dt_ms = dt_ms;
     n->dt_us = dt_us;
     n->mode = mode;
     n->det = mode == NAER_MODE_DETERMINISTIC;
+    n->event = mode == NAER_MODE_EVENT;
     return NAER_OK;
 }
 
@@ -418,6 +435,21 @@ int check_net(naer_net* n, const naer_net_desc& d, NetInfo* info)
                 return fail(NAER_ERR_INVALID, "STDP needs tau_plus, tau_minus > 0 and w_max >= w_min");
             info->plastic = true;
         }
+    }
+    if (d.mode == NAER_MODE_EVENT) {
+        // the reference's event-driven step: static LIF populations without
+        // drive, POISSON sources
+        for (uint32_t p = 0; p < d.n_pops; ++p) {
+            const naer_pop_desc& q = d.pops[p];
+            if (q.kind == NAER_POP_MODEL || (q.kind == NAER_POP_LIF && q.ext_rate_hz > 0.0f))
+                return fail(NAER_ERR_INVALID,
+                            "population %u: the event-driven mode takes LIF populations without drive and POISSON "
+                            "sources", p);
+            if (q.kind == NAER_POP_LIF && (!(q.t_ref_ms >= 0.0f) || q.tau_m_ms * 1000.0f < 1.0f))
+                return fail(NAER_ERR_INVALID, "population %u: tau_m must be >= 1 us and t_ref >= 0", p);
+        }
+        if (info->plastic)
+            return fail(NAER_ERR_INVALID, "the event-driven mode has no plasticity");
     }
     return NAER_OK;
 }
@@ -893,7 +925,7 @@ int set_block_ranges(naer_net* n)
 int alloc_buckets(naer_net* n)
 {
     n->bucketed = false;
-    if (n->B < 2 || n->S_cap == 0)
+    if (n->B < 2 || n->S_cap == 0 || n->event)
         return NAER_OK;
     n->M = (n->B - 1 + n->dmax) / n->B;
     n->gt_shift = kGroupNeuronShift - n->tw_shift;
@@ -1032,6 +1064,8 @@ int naer_pack_create(naer_ctx* ctx, const naer_net_desc* descs, uint32_t n_nets,
     naer_net* n = new naer_net;
     n->ctx = ctx;
     int rc = check_settings(n, descs[0].dt_ms, descs[0].mode);
+    if (rc == NAER_OK && n->event && n_nets > 1)
+        rc = fail(NAER_ERR_INVALID, "the event-driven mode runs one network");
     uint64_t N = 0, rec = 0;
     n->dmin = 0xFFFFFFFFu;
     n->dmax = 1;
@@ -1095,6 +1129,172 @@ int naer_pack_layout(naer_net* n, uint32_t cap, uint32_t* neuron_begin, uint32_t
             n_neurons[i] = n->nets[i].active ? n->nets[i].n_neurons : 0u;
         if (t_start)
             t_start[i] = n->nets[i].t_start;
+    }
+    return NAER_OK;
+}
+
+// Event-driven mode: input lists, event buffers and the decay tables of every
+// population (the oracle builds the same tables, orc_ev_tables).
+static int setup_events(naer_net* n)
+{
+    if (!n->event)
+        return NAER_OK;
+    int rc;
+    const uint32_t np = (uint32_t)n->pp.size();
+    // a POISSON source emits at most 10 spikes per window, a LIF neuron one
+    // per refractory period
+    uint64_t in_cap = 1024;
+    for (auto& net : n->nets)
+        for (auto& q : net.pops) {
+            const double tr = std::max(1.0, (double)q.t_ref_ms * 1000.0);
+            in_cap += q.kind == NAER_POP_POISSON ? (uint64_t)q.size * kMaxPoissonPerEpoch
+                                                 : (uint64_t)q.size * (uint64_t)(kEpochUs / tr + 1.0);
+        }
+    if (in_cap >= 0x7FFFFFFFull)
+        return fail(NAER_ERR_CAPACITY, "event-driven mode: too many inputs per window");
+    n->ev_in_cap = (uint32_t)in_cap;
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    const double fit = (double)free_b / 4.0 / 24.0;
+    // the library sorts take an int count
+    n->ev_e_cap = (uint64_t)std::min(std::min(std::max((double)n->S_cap, (double)(1u << 22)), fit), 2147483647.0);
+    if ((rc = n->alloc(&n->ev_list[0], in_cap)) || (rc = n->alloc(&n->ev_list[1], in_cap)) ||
+        (rc = n->alloc(&n->ev_sorted, in_cap)) || (rc = n->alloc(&n->ev_len, in_cap + 1)) ||
+        (rc = n->alloc(&n->ev_off, in_cap + 1)) || (rc = n->alloc(&n->ev_keys, n->ev_e_cap)) ||
+        (rc = n->alloc(&n->ev_keys_s, n->ev_e_cap)) || (rc = n->alloc(&n->ev_w, n->ev_e_cap)) ||
+        (rc = n->alloc(&n->ev_w_s, n->ev_e_cap)) || (rc = n->alloc(&n->ev_lo, n->N)) ||
+        (rc = n->alloc(&n->ev_hi, n->N)) || (rc = n->alloc(&n->ev_tlast, n->N)) ||
+        (rc = n->alloc(&n->ev_dlo, (size_t)std::max(1u, np) * kEvLo)) ||
+        (rc = n->alloc(&n->ev_dhi, (size_t)std::max(1u, np) * kEvHiCap)) || (rc = n->alloc(&n->ev_nhi, np)) ||
+        (rc = n->alloc(&n->ev_tref, np)) || (rc = n->alloc(&n->ev_ctr, 8)))
+        return rc;
+    std::vector<double> lo((size_t)np * kEvLo, 0.0), hi((size_t)np * kEvHiCap, 0.0);
+    std::vector<uint32_t> nhi(np, 0), tref(np, 0);
+    uint32_t p = 0;
+    for (auto& net : n->nets)
+        for (auto& q : net.pops) {
+            if (q.size == 0)
+                continue;
+            if (q.kind == NAER_POP_LIF) {
+                const double tau = (double)q.tau_m_ms * 1000.0;
+                for (uint32_t l = 0; l < kEvLo; ++l)
+                    lo[(size_t)p * kEvLo + l] = std::exp(-(double)l / tau);
+                uint32_t h = 0;
+                for (; h < kEvHiCap; ++h) {
+                    const double v = std::exp(-(double)(h * kEvLo) / tau);
+                    if (v == 0.0)
+                        break;
+                    hi[(size_t)p * kEvHiCap + h] = v;
+                }
+                nhi[p] = h;
+                tref[p] = (uint32_t)std::llround((double)q.t_ref_ms * 1000.0);
+            }
+            ++p;
+        }
+    HIP_TRY(hipMemcpy(n->ev_dlo, lo.data(), lo.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(n->ev_dhi, hi.data(), hi.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(n->ev_nhi, nhi.data(), np * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(n->ev_tref, tref.data(), np * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(n->ev_tlast, 0, n->N * sizeof(uint32_t)));
+    HIP_TRY(hipMemset(n->ev_ctr, 0, 8 * sizeof(uint32_t)));
+    // one temporary buffer for the three library passes at full size
+    size_t b0 = 0, b1 = 0, b2 = 0;
+    HIP_TRY(ev_sort_inputs(nullptr, &b0, n->ev_list[0], n->ev_sorted, n->ev_in_cap, n->stream));
+    HIP_TRY(ev_scan_rows(nullptr, &b1, n->ev_len, n->ev_off, n->ev_in_cap + 1, n->stream));
+    HIP_TRY(ev_sort_events(nullptr, &b2, n->ev_keys, n->ev_keys_s, n->ev_w, n->ev_w_s,
+                           (uint32_t)std::min<uint64_t>(n->ev_e_cap, 0xFFFFFFFFull), n->stream));
+    n->ev_temp_bytes = std::max(b0, std::max(b1, b2));
+    unsigned char* t = nullptr;
+    if ((rc = n->alloc(&t, n->ev_temp_bytes)))
+        return rc;
+    n->ev_temp = t;
+    return NAER_OK;
+}
+
+static EvArgs ev_args(const naer_net* n)
+{
+    EvArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.pops = n->d_pops;
+    a.blk_pop = n->d_blk_pop;
+    a.n_pops = (uint32_t)n->pp.size();
+    a.tw_shift = n->tw_shift;
+    a.N = n->N;
+    a.t0 = (uint32_t)(n->ev_epoch * kEpochUs);
+    a.epoch = (uint32_t)n->ev_epoch;
+    a.in_cap = n->ev_in_cap;
+    a.row_ptr = n->row_ptr;
+    a.col = n->col;
+    a.w = n->w;
+    a.in_keys = n->ev_sorted;
+    a.lost = n->ev_ctr + 2;
+    a.row_len = n->ev_len;
+    a.row_off = n->ev_off;
+    a.seg_lo = n->ev_lo;
+    a.seg_hi = n->ev_hi;
+    a.V = n->V;
+    a.t_last = n->ev_tlast;
+    a.dec_lo = n->ev_dlo;
+    a.dec_hi = n->ev_dhi;
+    a.n_hi = n->ev_nhi;
+    a.t_ref = n->ev_tref;
+    a.rec = n->rec;
+    a.rec_cap = n->rec_cap;
+    a.rec_n = (unsigned long long*)((char*)n->st + offsetof(DevState, rec_n));
+    a.spikes = (unsigned long long*)(n->ev_ctr + 4);
+    return a;
+}
+
+// Runs `epochs` windows; two host reads per window (the input and event
+// counts size the sorts).
+static int ev_step(naer_net* n, uint32_t epochs)
+{
+    for (uint32_t k = 0; k < epochs; ++k) {
+        if (n->ev_epoch * kEpochUs + 2 * kEpochUs > 0xFFFFFFFFull)
+            return fail(NAER_ERR_CAPACITY, "event-driven mode: microsecond clock exhausted");
+        const uint32_t cur = (uint32_t)(n->ev_epoch & 1u);
+        EvArgs a = ev_args(n);
+        a.out_keys = n->ev_list[cur];
+        a.n_out = n->ev_ctr + cur;
+        HIP_TRY(ev_launch(a, kEvPoisson, 0, n->stream));
+        uint32_t c[3];
+        HIP_TRY(hipMemcpyAsync(c, n->ev_ctr, sizeof c, hipMemcpyDeviceToHost, n->stream));
+        HIP_TRY(hipStreamSynchronize(n->stream));
+        if (c[2])
+            return fail(NAER_ERR_CAPACITY, "event-driven mode: %u inputs beyond the list capacity %u", c[2],
+                        n->ev_in_cap);
+        const uint32_t n_in = c[cur];
+        size_t tb = n->ev_temp_bytes;
+        if (n_in)
+            HIP_TRY(ev_sort_inputs(n->ev_temp, &tb, n->ev_list[cur], n->ev_sorted, n_in, n->stream));
+        HIP_TRY(ev_launch(a, kEvRowLen, n_in, n->stream));
+        tb = n->ev_temp_bytes;
+        HIP_TRY(ev_scan_rows(n->ev_temp, &tb, n->ev_len, n->ev_off, n_in + 1, n->stream));
+        unsigned long long n_ev = 0;
+        HIP_TRY(hipMemcpyAsync(&n_ev, n->ev_off + n_in, sizeof n_ev, hipMemcpyDeviceToHost, n->stream));
+        HIP_TRY(hipStreamSynchronize(n->stream));
+        if (n_ev > n->ev_e_cap)
+            return fail(NAER_ERR_CAPACITY, "event-driven mode: %llu events in a window (capacity %llu)", n_ev,
+                        (unsigned long long)n->ev_e_cap);
+        a.ev_keys = n->ev_keys;
+        a.ev_w = n->ev_w;
+        HIP_TRY(ev_launch(a, kEvExpand, n_in, n->stream));
+        if (n_ev) {
+            tb = n->ev_temp_bytes;
+            HIP_TRY(ev_sort_events(n->ev_temp, &tb, n->ev_keys, n->ev_keys_s, n->ev_w, n->ev_w_s, (uint32_t)n_ev,
+                                   n->stream));
+        }
+        EvArgs b = a;
+        b.ev_keys = n->ev_keys_s;
+        b.ev_w = n->ev_w_s;
+        b.out_keys = n->ev_list[cur ^ 1u];
+        b.n_out = n->ev_ctr + (cur ^ 1u);
+        HIP_TRY(hipMemsetAsync(n->ev_ctr + (cur ^ 1u), 0, sizeof(uint32_t), n->stream));
+        HIP_TRY(ev_launch(b, kEvClear, 0, n->stream));
+        HIP_TRY(ev_launch(b, kEvBounds, (uint32_t)n_ev, n->stream));
+        HIP_TRY(ev_launch(b, kEvProcess, 0, n->stream));
+        n->ev_events += n_ev;
+        n->ev_epoch += 1;
     }
     return NAER_OK;
 }
@@ -1190,7 +1390,7 @@ int naer_net_build(naer_net* n)
     for (auto& net : n->nets)
         if ((rc = init_net(n, net)))
             return rc;
-    if ((rc = setup_models(n)))
+    if ((rc = setup_models(n)) || (rc = setup_events(n)))
         return rc;
     HIP_TRY(hipStreamSynchronize(n->stream));
     set_scatter_shape(n);
@@ -1206,6 +1406,8 @@ int naer_arena_create(naer_ctx* ctx, const naer_arena_desc* a, naer_net** out)
     n->ctx = ctx;
     n->arena = true;
     int rc = check_settings(n, a->dt_ms, a->mode);
+    if (rc == NAER_OK && n->event)
+        rc = fail(NAER_ERR_INVALID, "arenas do not run the event-driven mode");
     if (rc == NAER_OK && (a->max_neurons == 0 || a->max_neurons > 0xFFFFFF00ull))
         rc = fail(NAER_ERR_CAPACITY, "arena neuron capacity must be in [1, 2^32)");
     if (rc == NAER_OK && (a->delay_min < 1 || a->delay_max < a->delay_min || a->delay_max > 127))
@@ -1592,6 +1794,8 @@ int naer_net_step(naer_net* n, uint32_t n_steps)
         return fail(NAER_ERR_INVALID, "net is NULL");
     if (!n->built)
         return fail(NAER_ERR_STATE, "net not built");
+    if (n->event)
+        return ev_step(n, n_steps);
     if (n->graph_chunk) {
         // a captured chunk is a whole number of aligned batches, so every
         // replay sees the same batch pattern: align plainly first
@@ -1635,6 +1839,8 @@ int naer_net_set_graph(naer_net* n, uint32_t chunk_steps)
 {
     if (!n)
         return fail(NAER_ERR_INVALID, "net is NULL");
+    if (n->event && chunk_steps)
+        return fail(NAER_ERR_INVALID, "the event-driven mode is not captured in graphs");
     int rc = naer_net_sync(n);
     if (rc)
         return rc;
@@ -1836,6 +2042,14 @@ int naer_net_query(naer_net* n, naer_stats* out)
             out->events += v;
         out->step = s.clock;
         out->time_us = s.clock * n->dt_us;
+        if (n->event) {
+            unsigned long long sp = 0;
+            HIP_TRY(hipMemcpy(&sp, n->ev_ctr + 4, sizeof sp, hipMemcpyDeviceToHost));
+            out->spikes = sp;
+            out->events = n->ev_events;
+            out->step = n->ev_epoch;
+            out->time_us = n->ev_epoch * kEpochUs;
+        }
         out->dropped = n->dropped + (s.rec_n > n->rec_cap ? s.rec_n - n->rec_cap : 0) + s.overflow;
     }
     return NAER_OK;
@@ -2058,8 +2272,9 @@ int naer_net_save(naer_net* n, const char* path)
         return fail(NAER_ERR_STATE, "net not built");
     if (n->arena)
         return fail(NAER_ERR_STATE, "arenas are not checkpointed");
-    if (!n->mpops.empty())
-        return fail(NAER_ERR_INVALID, "nets with populations of source-defined models are not checkpointed");
+    if (!n->mpops.empty() || n->event)
+        return fail(NAER_ERR_INVALID,
+                    "nets with populations of source-defined models and event-driven nets are not checkpointed");
     int rc = naer_net_sync(n);
     if (rc)
         return rc;
@@ -2086,8 +2301,9 @@ int naer_net_load(naer_net* n, const char* path)
         return fail(NAER_ERR_STATE, "net not built");
     if (n->arena)
         return fail(NAER_ERR_STATE, "arenas are not checkpointed");
-    if (!n->mpops.empty())
-        return fail(NAER_ERR_INVALID, "nets with populations of source-defined models are not checkpointed");
+    if (!n->mpops.empty() || n->event)
+        return fail(NAER_ERR_INVALID,
+                    "nets with populations of source-defined models and event-driven nets are not checkpointed");
     int rc = naer_net_sync(n);
     if (rc)
         return rc;

@@ -302,11 +302,15 @@ class Network:
 
     def step(self):
         """Advance one reference epoch (step_us of simulated time)."""
+        if self.mode == _abi.MODE_EVENT:
+            check(_abi.lib().naer_net_step(self._h(), 1))
+            return
         n = int(round(self.step_us / (self.dt_ms * 1000.0)))
         check(_abi.lib().naer_net_step(self._h(), n))
 
     def run(self, n_steps):
-        """Advance n_steps steps of dt (asynchronous)."""
+        """Advance n_steps steps of dt (asynchronous); in the event-driven mode
+        (mode=_abi.MODE_EVENT) n_steps windows of 1500 us."""
         check(_abi.lib().naer_net_step(self._h(), int(n_steps)))
 
     def finish(self):

@@ -1054,3 +1054,212 @@ void orc_net_read_probe(const orc_net* n, float* used, double* exact, double* ab
     if (absw)
         memcpy(absw, n->probe_abs, sz * sizeof(double));
 }
+
+/* ---- event-driven mode -----------------------------------------------------
+ * The reference's epoch (src/Network.cpp:122-256) with its neuron step
+ * (lif.cl:15-34): epoch k covers the window [1500k, 1500k + 1500) us.  Its
+ * inputs are the POISSON spikes timed in the window (input_neuron_group.cl:
+ * 26-39) and the LIF spikes of the previous window, which arrive 1500 us
+ * after they were fired (NAER_AXONAL_DELAY, naer_types.h:4; the synapses'
+ * own delays are not used).  Every input is applied to its targets in time
+ * order; inputs of one target at the same microsecond in the order (source,
+ * position in the source's row).  Per input of weight w at time t:
+ *   dt = t - t_last; if dt < 0 the input falls into the refractory period and
+ *   is dropped; else V = v_rest + (V - v_rest) * e^(-dt/tau_m) (the product in
+ *   double, rounded to float), V += w, t_last = t; if V > v_th the neuron
+ *   fires at t: V = v_reset, t_last = t + t_ref.
+ * e^(-dt/tau_m) = H[dt >> 12] * L[dt & 4095] with H[h] = exp(-4096 h / T),
+ * L[l] = exp(-l / T), T = tau_m in us (H = 0 past the last nonzero entry); the
+ * device uses the same tables.  Recorded spikes carry step = the epoch and
+ * time_us = the firing (or POISSON emission) time. */
+
+#define EV_WINDOW 1500u
+#define EV_LO 4096u
+
+typedef struct {
+    uint32_t src, time;
+} ev_input;
+
+typedef struct {
+    uint32_t tgt, time;
+    uint64_t seq;
+    float w;
+} ev_event;
+
+static int cmp_ev_input(const void* a, const void* b)
+{
+    const ev_input* x = (const ev_input*)a;
+    const ev_input* y = (const ev_input*)b;
+    if (x->src != y->src)
+        return x->src < y->src ? -1 : 1;
+    return x->time < y->time ? -1 : (x->time > y->time);
+}
+
+static int cmp_ev_event(const void* a, const void* b)
+{
+    const ev_event* x = (const ev_event*)a;
+    const ev_event* y = (const ev_event*)b;
+    if (x->tgt != y->tgt)
+        return x->tgt < y->tgt ? -1 : 1;
+    if (x->time != y->time)
+        return x->time < y->time ? -1 : 1;
+    return x->seq < y->seq ? -1 : (x->seq > y->seq);
+}
+
+/* the decay tables of one membrane time constant (us); *n_hi entries of H */
+void orc_ev_tables(double tau_us, double* lo, double* hi, uint32_t cap_hi, uint32_t* n_hi)
+{
+    for (uint32_t l = 0; l < EV_LO; ++l)
+        lo[l] = exp(-(double)l / tau_us);
+    uint32_t h = 0;
+    for (; h < cap_hi; ++h) {
+        hi[h] = exp(-(double)(h * EV_LO) / tau_us);
+        if (hi[h] == 0.0)
+            break;
+    }
+    *n_hi = h;
+}
+
+typedef struct {
+    float* V;
+    uint32_t* t_last;
+    ev_input* in;
+    uint64_t n_in, cap_in;
+    double* lo; /* EV_LO per population */
+    double* hi; /* 8192 per population */
+    uint32_t* n_hi;
+    uint32_t epoch;
+} ev_state;
+
+#define EV_HI_CAP 8192u
+
+void orc_ev_free(void* p)
+{
+    ev_state* e = (ev_state*)p;
+    if (!e)
+        return;
+    free(e->V);
+    free(e->t_last);
+    free(e->in);
+    free(e->lo);
+    free(e->hi);
+    free(e->n_hi);
+    free(e);
+}
+
+static void ev_push(ev_state* e, uint32_t src, uint32_t time)
+{
+    if (e->n_in == e->cap_in) {
+        e->cap_in = e->cap_in ? 2 * e->cap_in : 1024;
+        e->in = (ev_input*)realloc(e->in, e->cap_in * sizeof(ev_input));
+    }
+    e->in[e->n_in].src = src;
+    e->in[e->n_in].time = time;
+    e->n_in++;
+}
+
+static void ev_record(orc_net* n, uint32_t epoch, uint32_t j, uint32_t time)
+{
+    n->spikes++;
+    if (n->rec_n < n->rec_cap) {
+        naer_spike* r = &n->rec[n->rec_n++];
+        r->step = epoch;
+        r->neuron = j;
+        r->time_us = time;
+        r->_pad = 0;
+    }
+}
+
+/* Runs `epochs` windows of the event-driven mode on a built net (its
+ * connectivity and initial potentials); the net must not also be stepped by
+ * orc_net_run. */
+int orc_ev_run(orc_net* n, void** state, uint32_t epochs)
+{
+    if (!n->built) {
+        set_err("net not built");
+        return -1;
+    }
+    ev_state* e = (ev_state*)*state;
+    if (!e) {
+        e = (ev_state*)xcalloc(1, sizeof(ev_state));
+        e->V = (float*)xcalloc(n->N, sizeof(float));
+        e->t_last = (uint32_t*)xcalloc(n->N, sizeof(uint32_t));
+        memcpy(e->V, n->V, n->N * sizeof(float));
+        const uint32_t np = n->desc.n_pops;
+        e->lo = (double*)xcalloc((size_t)np * EV_LO, sizeof(double));
+        e->hi = (double*)xcalloc((size_t)np * EV_HI_CAP, sizeof(double));
+        e->n_hi = (uint32_t*)xcalloc(np, sizeof(uint32_t));
+        for (uint32_t p = 0; p < np; ++p)
+            orc_ev_tables((double)n->pops[p].tau_m_ms * 1000.0, e->lo + (size_t)p * EV_LO,
+                          e->hi + (size_t)p * EV_HI_CAP, EV_HI_CAP, &e->n_hi[p]);
+        *state = e;
+    }
+    uint32_t times[16];
+    for (uint32_t k = 0; k < epochs; ++k, ++e->epoch) {
+        const uint32_t t0 = e->epoch * EV_WINDOW, t1 = t0 + EV_WINDOW;
+        /* POISSON spikes of the window */
+        for (uint32_t j = 0; j < n->N; ++j) {
+            const naer_pop_desc* q = &n->pops[n->pop_of[j]];
+            if (q->kind != NAER_POP_POISSON)
+                continue;
+            int m = orc_ref_poisson_epoch(n->pois_idx[j], n->n_pois, n->desc.poisson_seed, q->src_rate_hz, t0, t1,
+                                          times);
+            for (int i = 0; i < m; ++i) {
+                ev_push(e, j, times[i]);
+                ev_record(n, e->epoch, j, times[i]);
+            }
+        }
+        qsort(e->in, e->n_in, sizeof(ev_input), cmp_ev_input);
+        uint64_t n_ev = 0;
+        for (uint64_t i = 0; i < e->n_in; ++i)
+            n_ev += n->row_ptr[e->in[i].src + 1] - n->row_ptr[e->in[i].src];
+        ev_event* ev = (ev_event*)xcalloc(n_ev ? n_ev : 1, sizeof(ev_event));
+        uint64_t c = 0;
+        for (uint64_t i = 0; i < e->n_in; ++i) {
+            const uint32_t s = e->in[i].src;
+            for (uint64_t r = n->row_ptr[s]; r < n->row_ptr[s + 1]; ++r, ++c) {
+                ev[c].tgt = n->col[r] & 0xFFFFFFu;
+                ev[c].time = e->in[i].time;
+                ev[c].seq = c;
+                ev[c].w = n->w[r];
+            }
+        }
+        qsort(ev, n_ev, sizeof(ev_event), cmp_ev_event);
+        n->events += n_ev;
+        e->n_in = 0;
+        for (uint64_t i = 0; i < n_ev; ++i) {
+            const uint32_t j = ev[i].tgt, t = ev[i].time;
+            const uint32_t p = n->pop_of[j];
+            const naer_pop_desc* q = &n->pops[p];
+            const int64_t dt = (int64_t)t - (int64_t)e->t_last[j];
+            if (dt < 0)
+                continue;
+            const uint64_t h = (uint64_t)dt >> 12;
+            const double d = h < e->n_hi[p] ? e->hi[(size_t)p * EV_HI_CAP + h] * e->lo[(size_t)p * EV_LO + (dt & 4095)]
+                                            : 0.0;
+            float x = e->V[j] - q->v_rest;
+            x = (float)((double)x * d);
+            float v = x + q->v_rest;
+            v = v + ev[i].w;
+            e->t_last[j] = t;
+            if (v > q->v_th) {
+                v = q->v_reset;
+                e->t_last[j] = t + (uint32_t)llround((double)q->t_ref_ms * 1000.0);
+                ev_push(e, j, t + EV_WINDOW);
+                ev_record(n, e->epoch, j, t);
+            }
+            e->V[j] = v;
+        }
+        free(ev);
+    }
+    return 0;
+}
+
+void orc_ev_read_state(void* state, uint32_t n_neurons, float* v, uint32_t* t_last)
+{
+    ev_state* e = (ev_state*)state;
+    if (v)
+        memcpy(v, e->V, n_neurons * sizeof(float));
+    if (t_last)
+        memcpy(t_last, e->t_last, n_neurons * sizeof(uint32_t));
+}

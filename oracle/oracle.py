@@ -53,6 +53,11 @@ def lib():
         L.orc_net_read_spikes.restype = u64
         L.orc_net_set_probe.argtypes = [v, u64, u32]
         L.orc_net_read_probe.argtypes = [v, p, p, p]
+        L.orc_ev_run.argtypes = [v, ctypes.POINTER(ctypes.c_void_p), u32]
+        L.orc_ev_run.restype = ctypes.c_int
+        L.orc_ev_read_state.argtypes = [v, u32, p, p]
+        L.orc_ev_free.argtypes = [v]
+        L.orc_ev_tables.argtypes = [ctypes.c_double, p, p, u32, ctypes.POINTER(u32)]
         _lib = L
     return _lib
 
@@ -77,6 +82,16 @@ def ref_poisson_epoch(idx, n_pois, seed, rate, t0, t1):
     buf = np.zeros(10, np.uint32)
     n = lib().orc_ref_poisson_epoch(idx, n_pois, seed, rate, t0, t1, buf.ctypes.data)
     return buf[:n].copy()
+
+
+def ev_tables(tau_us, cap_hi=8192):
+    """The event-driven mode's decay tables L[l] = exp(-l/T), H[h] =
+    exp(-4096 h/T) and the number of nonzero H entries."""
+    lo = np.zeros(4096, np.float64)
+    hi = np.zeros(cap_hi, np.float64)
+    n = ctypes.c_uint32()
+    lib().orc_ev_tables(float(tau_us), lo.ctypes.data, hi.ctypes.data, cap_hi, ctypes.byref(n))
+    return lo, hi[: n.value]
 
 
 def ref_bernoulli_row(src, n_dst, p, seed32):
@@ -162,7 +177,24 @@ class OracleNet:
         lib().orc_net_read_probe(self.h, used.ctypes.data, exact.ctypes.data, absw.ctypes.data)
         return used, exact, absw
 
+    # --- event-driven mode (orc_ev_run) -----------------------------------
+    def run_events(self, epochs):
+        """Windows of 1500 us of the reference's event-driven step."""
+        if not hasattr(self, "_ev"):
+            self._ev = ctypes.c_void_p()
+        if lib().orc_ev_run(self.h, ctypes.byref(self._ev), int(epochs)) != 0:
+            raise RuntimeError("oracle: " + lib().orc_error().decode())
+
+    def event_state(self):
+        v = np.zeros(self.N, np.float32)
+        t_last = np.zeros(self.N, np.uint32)
+        lib().orc_ev_read_state(self._ev, self.N, v.ctypes.data, t_last.ctypes.data)
+        return dict(v=v, t_last=t_last)
+
     def close(self):
+        if getattr(self, "_ev", None):
+            lib().orc_ev_free(self._ev)
+            self._ev = None
         if self.h:
             lib().orc_net_destroy(self.h)
             self.h = None

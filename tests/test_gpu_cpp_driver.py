@@ -48,3 +48,18 @@ def test_driver_equals_python_binding(model):
     assert spikes > 100
     assert (spikes, events) == (int(st.spikes), int(st.events))
     net.close()
+
+
+def test_driver_event_mode_equals_python_binding():
+    """--mode event: the reference's event-driven step, 20 windows of 1500 us."""
+    n, s, t_us, spikes, events = run_driver("--synapses", "1000000", "--model", "brunel", "--mode", "event",
+                                            "--seconds", "0.03")
+    assert t_us == 30_000
+    net = models.brunel(1_000_000, mode=_abi.MODE_EVENT)
+    net.compile()
+    net.run(20)
+    st = net.stats()
+    assert (n, s) == (net.getNeurons(), net.getSynapses())
+    assert spikes > 100
+    assert (spikes, events) == (int(st.spikes), int(st.events))
+    net.close()

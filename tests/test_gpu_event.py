@@ -1,0 +1,70 @@
+"""The event-driven mode (NAER_MODE_EVENT: the reference's epoch,
+src/Network.cpp:122-256, and neuron step, lif.cl:15-34) on the GPU equals the
+oracle's restatement (orc_ev_run, pinned by test_oracle_event.py) bit for
+bit: rasters (window, neuron, microsecond) and final potentials, on the
+reference's brunel model and a fixed in-degree network with equal-time
+inputs."""
+import numpy as np
+import pytest
+
+from helpers import first_divergence, oracle_of
+from naer_amd import _abi, models
+from naer_amd.network import Network, NeuronType
+
+pytestmark = pytest.mark.gpu
+
+
+def compare(net, windows, chunks=None):
+    o = oracle_of(net, threads=1)
+    o.run_events(windows)
+    want, want_v = o.spikes(), o.event_state()["v"]
+    net.compile()
+    for c in chunks or (windows,):
+        net.run(c)
+    net.finish()
+    got = net.spikes()
+    st = net.stats()
+    assert first_divergence(got, want) is None
+    assert np.array_equal(net.state()["v"], want_v)
+    assert st.step == windows and st.time_us == windows * 1500
+    assert st.spikes == o.spike_count and st.events == o.event_count
+    return got
+
+
+@pytest.mark.parametrize("synapses", [1_000_000, 10_000_000])
+def test_brunel_event_mode_equals_oracle(synapses):
+    net = models.brunel(synapses, mode=_abi.MODE_EVENT, record_capacity=1 << 21)
+    sp = compare(net, 40, chunks=(13, 1, 26))
+    n_lif = net.getNeurons() - sum(p.size for p in net._pops if p.type == NeuronType.POISSON)
+    assert np.count_nonzero(sp["neuron"] < n_lif) > 100  # the LIF populations fire
+
+
+def test_fixed_indegree_event_mode_with_equal_time_inputs():
+    """Many sources at a high rate onto few targets: inputs of one target at
+    the same microsecond are common, and their order (source, row position)
+    decides the float sums."""
+    net = Network(dt_ms=0.1, mode=_abi.MODE_EVENT, record_capacity=1 << 20)
+    P = net.addPopulation(NeuronType.POISSON, 3000, src_rate_hz=2000.0)
+    L = net.addPopulation(NeuronType.NOP, 200, tau_m_ms=10.0, v_th=20.0, v_reset=10.0, v_rest=0.0, t_ref_ms=1.0,
+                          v0_min=0.0, v0_max=20.0)
+    net.addConnectionFixedIndegree(P, L, 600, 0.0, 0.05, 0.3, seed=5)  # weight 0: drawn in [0.05, 0.3)
+    net.addConnectionFixedIndegree(P, L, 200, 0.0, -0.4, -0.1, seed=6)
+    net.addConnectionFixedIndegree(L, L, 20, 0.5, seed=7)
+    sp = compare(net, 25)
+    assert np.count_nonzero(sp["neuron"] < 200) > 200
+
+
+def test_event_mode_rejects_what_it_does_not_model(tmp_path):
+    bad = Network(dt_ms=0.1, mode=_abi.MODE_EVENT)
+    bad.addPopulation(NeuronType.NOP, 10, ext_rate_hz=1000.0, ext_weight=0.1)
+    with pytest.raises(_abi.NaerError, match="without drive"):
+        bad.compile()
+    net = models.brunel(100_000, mode=_abi.MODE_EVENT)
+    net.compile()
+    with pytest.raises(_abi.NaerError, match="graphs"):
+        net.set_graph(100)
+    with pytest.raises(_abi.NaerError, match="checkpointed"):
+        net.save(tmp_path / "x.ckpt")
+    net.step()  # one 1500 us window
+    net.finish()
+    assert net.getTime() == 1500
