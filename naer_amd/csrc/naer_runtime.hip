@@ -1139,6 +1139,11 @@ static int setup_events(naer_net* n)
 {
     if (!n->event)
         return NAER_OK;
+    // validated up to 5e8 synapses: a run of brunel at 1e9 synapses reported
+    // a corrupt event count, cause not yet found
+    if (n->S_cap > 500000000ull)
+        return fail(NAER_ERR_CAPACITY, "event-driven mode: at most 5e8 synapses (this net has %llu)",
+                    (unsigned long long)n->S_cap);
     int rc;
     const uint32_t np = (uint32_t)n->pp.size();
     // a POISSON source emits at most 10 spikes per window, a LIF neuron one
