@@ -134,11 +134,31 @@ __global__ __launch_bounds__(256) void k_ev_bounds(const EvArgs a, uint32_t n_ev
         a.seg_hi[tgt] = i + 1;
 }
 
+__device__ __forceinline__ double ev_decay(const double* H, const double* L, uint32_t n_hi, long long dt)
+{
+    const unsigned long long h = (unsigned long long)dt >> 12;
+    return h < n_hi ? H[h] * L[dt & 4095] : 0.0;
+}
+
+__device__ __forceinline__ double readlane_f64(double x, uint32_t k)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)(uint32_t)b, (int)k);
+    const int hi = __builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), (int)k);
+    return __longlong_as_double((long long)((unsigned long long)(uint32_t)hi << 32 | (uint32_t)lo));
+}
+
 // lif.cl:15-34 per event, in time order; the decay e^(-dt/tau) is H[dt >> 12]
-// * L[dt & 4095] from the population's tables.
+// * L[dt & 4095] from the population's tables.  One wave per target: the
+// wave loads 64 events at a time (coalesced) and looks up each event's decay
+// from the previous event's time in parallel; the step itself runs in order
+// on the broadcast values, and looks the decay up again only when the
+// neuron's t_last is not the previous event's time (after a spike or a
+// dropped input).
 __global__ __launch_bounds__(256) void k_ev_process(const EvArgs a)
 {
-    const uint32_t j = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t j = blockIdx.x * 4u + (threadIdx.x >> 6);
     if (j >= a.N)
         return;
     const uint32_t lo = a.seg_lo[j], hi = a.seg_hi[j];
@@ -155,32 +175,54 @@ __global__ __launch_bounds__(256) void k_ev_process(const EvArgs a)
     const float v_rest = pp.v_rest, v_th = pp.v_th, v_reset = pp.v_reset;
     float V = a.V[j];
     uint32_t t_last = a.t_last[j];
+    uint32_t t_before = t_last;  // time the first event's looked-up decay starts from
     uint32_t fired = 0;
-    for (uint32_t i = lo; i < hi; ++i) {
-        const uint32_t t = a.t0 + (uint32_t)(a.ev_keys[i] & 2047u);
-        const long long dt = (long long)t - (long long)t_last;
-        if (dt < 0)
-            continue;
-        const unsigned long long h = (unsigned long long)dt >> 12;
-        const double d = h < n_hi ? H[h] * L[dt & 4095] : 0.0;
-        float x = V - v_rest;
-        x = (float)((double)x * d);
-        float v = x + v_rest;
-        v = v + a.ev_w[i];
-        t_last = t;
-        if (v > v_th) {
-            v = v_reset;
-            t_last = t + t_ref;
-            ev_append(a, j, t + kEpochUs);
-            ev_record(a, j, t);
-            ++fired;
+    for (uint32_t c = lo; c < hi; c += 64u) {
+        const uint32_t i = c + lane;
+        uint32_t t = 0;
+        float w = 0.0f;
+        if (i < hi) {
+            t = a.t0 + (uint32_t)(a.ev_keys[i] & 2047u);
+            w = a.ev_w[i];
         }
-        V = v;
+        uint32_t tp = (uint32_t)__shfl_up((int)t, 1);
+        if (lane == 0)
+            tp = t_before;
+        const long long dtp = (long long)t - (long long)tp;
+        const double d = (i < hi && dtp >= 0) ? ev_decay(H, L, n_hi, dtp) : 0.0;
+        const uint32_t n = min(64u, hi - c);
+        for (uint32_t k = 0; k < n; ++k) {
+            const uint32_t tk = (uint32_t)__builtin_amdgcn_readlane((int)t, (int)k);
+            const long long dt = (long long)tk - (long long)t_last;
+            if (dt < 0)
+                continue;
+            const uint32_t tpk = (uint32_t)__builtin_amdgcn_readlane((int)tp, (int)k);
+            const double dk = tpk == t_last ? readlane_f64(d, k) : ev_decay(H, L, n_hi, dt);
+            const float wk = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(w), (int)k));
+            float x = V - v_rest;
+            x = (float)((double)x * dk);
+            float v = x + v_rest;
+            v = v + wk;
+            t_last = tk;
+            if (v > v_th) {
+                v = v_reset;
+                t_last = tk + t_ref;
+                if (lane == 0) {
+                    ev_append(a, j, tk + kEpochUs);
+                    ev_record(a, j, tk);
+                }
+                ++fired;
+            }
+            V = v;
+        }
+        t_before = (uint32_t)__builtin_amdgcn_readlane((int)t, (int)(n - 1));
     }
-    a.V[j] = V;
-    a.t_last[j] = t_last;
-    if (fired)
-        atomicAdd(a.spikes, (unsigned long long)fired);
+    if (lane == 0) {
+        a.V[j] = V;
+        a.t_last[j] = t_last;
+        if (fired)
+            atomicAdd(a.spikes, (unsigned long long)fired);
+    }
 }
 
 }  // namespace
@@ -207,7 +249,7 @@ hipError_t ev_launch(const EvArgs& a, int which, uint32_t n, hipStream_t s)
             hipLaunchKernelGGL(k_ev_bounds, dim3((n + 255) / 256), dim3(256), 0, s, a, n);
         break;
     case kEvProcess:
-        hipLaunchKernelGGL(k_ev_process, dim3(gn), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_ev_process, dim3((a.N + 3) / 4), dim3(256), 0, s, a);
         break;
     default:
         return hipErrorInvalidValue;
@@ -231,10 +273,14 @@ hipError_t ev_scan_rows(void* temp, size_t* bytes, const unsigned long long* in,
 }
 
 hipError_t ev_sort_events(void* temp, size_t* bytes, const unsigned long long* kin, unsigned long long* kout,
-                          const float* vin, float* vout, uint32_t n, hipStream_t s)
+                          const float* vin, float* vout, uint32_t n, uint32_t n_neurons, hipStream_t s)
 {
-    // keys: target (< 2^24) << 11 | time offset (< 2048); radix sort is stable
-    return hipcub::DeviceRadixSort::SortPairs(temp, *bytes, kin, kout, vin, vout, (int)n, 0, 35, s);
+    // keys: target << 11 | time offset (< 2048), 11 + bit width of the
+    // largest target; radix sort is stable
+    int bits = 11;
+    for (uint32_t m = n_neurons ? n_neurons - 1 : 0; m; m >>= 1)
+        ++bits;
+    return hipcub::DeviceRadixSort::SortPairs(temp, *bytes, kin, kout, vin, vout, (int)n, 0, bits, s);
 }
 
 }  // namespace naer

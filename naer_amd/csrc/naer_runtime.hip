@@ -1207,7 +1207,7 @@ static int setup_events(naer_net* n)
     HIP_TRY(ev_sort_inputs(nullptr, &b0, n->ev_list[0], n->ev_sorted, n->ev_in_cap, n->stream));
     HIP_TRY(ev_scan_rows(nullptr, &b1, n->ev_len, n->ev_off, n->ev_in_cap + 1, n->stream));
     HIP_TRY(ev_sort_events(nullptr, &b2, n->ev_keys, n->ev_keys_s, n->ev_w, n->ev_w_s,
-                           (uint32_t)std::min<uint64_t>(n->ev_e_cap, 0xFFFFFFFFull), n->stream));
+                           (uint32_t)std::min<uint64_t>(n->ev_e_cap, 0xFFFFFFFFull), n->N, n->stream));
     n->ev_temp_bytes = std::max(b0, std::max(b1, b2));
     unsigned char* t = nullptr;
     if ((rc = n->alloc(&t, n->ev_temp_bytes)))
@@ -1287,7 +1287,7 @@ static int ev_step(naer_net* n, uint32_t epochs)
         if (n_ev) {
             tb = n->ev_temp_bytes;
             HIP_TRY(ev_sort_events(n->ev_temp, &tb, n->ev_keys, n->ev_keys_s, n->ev_w, n->ev_w_s, (uint32_t)n_ev,
-                                   n->stream));
+                                   n->N, n->stream));
         }
         EvArgs b = a;
         b.ev_keys = n->ev_keys_s;
