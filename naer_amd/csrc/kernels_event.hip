@@ -7,16 +7,19 @@
 //                  appended to the window's input list, which already holds
 //                  the LIF spikes of the previous window (arriving 1500 us
 //                  after they were fired)
-//   (radix sort of the inputs by (source, time))
+//   (radix sort of the inputs by (time, source))
 //   k_ev_rowlen    row length of every input; an exclusive scan gives each
 //                  input's range of events
-//   k_ev_expand    events {target << 11 | time offset, weight} in input order
-//   (stable radix sort of the events by (target, time): ties keep the
-//    order (source, row position))
+//   k_ev_expand    events {target << 11 | time offset, weight} in input
+//                  order, i.e. by (time, source, row position)
+//   (stable radix sort of the events by target alone: each target's events
+//    keep the order (time, source, row position))
 //   k_ev_bounds    each target's range of sorted events
 //   k_ev_process   one thread per LIF neuron applies its events in order and
 //                  appends its spikes to the next window's input list
 // The sorts and the scan are rocPRIM's (through hipCUB).
+#include <algorithm>
+
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -54,7 +57,7 @@ __device__ __forceinline__ void ev_append(const EvArgs& a, uint32_t j, uint32_t 
 {
     const uint32_t k = atomicAdd(a.n_out, 1u);
     if (k < a.in_cap)
-        a.out_keys[k] = (unsigned long long)j << 32 | time;
+        a.out_keys[k] = (unsigned long long)time << 32 | j;
     else
         atomicAdd(a.lost, 1u);
 }
@@ -91,7 +94,7 @@ __global__ __launch_bounds__(256) void k_ev_rowlen(const EvArgs a, uint32_t n_in
         return;
     unsigned long long len = 0;
     if (i < n_in) {
-        const uint32_t s = (uint32_t)(a.in_keys[i] >> 32);
+        const uint32_t s = (uint32_t)a.in_keys[i];
         len = a.row_ptr[s + 1] - a.row_ptr[s];
     }
     a.row_len[i] = len;
@@ -101,14 +104,14 @@ __global__ __launch_bounds__(256) void k_ev_rowlen(const EvArgs a, uint32_t n_in
 __global__ __launch_bounds__(256) void k_ev_expand(const EvArgs a)
 {
     const unsigned long long key = a.in_keys[blockIdx.x];
-    const uint32_t s = (uint32_t)(key >> 32);
-    const uint32_t dt = (uint32_t)key - a.t0;  // < 1500
+    const uint32_t s = (uint32_t)key;
+    const uint32_t dt = (uint32_t)(key >> 32) - a.t0;  // < 1500
     const unsigned long long r0 = a.row_ptr[s], r1 = a.row_ptr[s + 1];
     const unsigned long long e0 = a.row_off[blockIdx.x];
     for (unsigned long long r = r0 + threadIdx.x; r < r1; r += 256) {
         const uint32_t tgt = a.col[r] & 0xFFFFFFu;
         const unsigned long long e = e0 + (r - r0);
-        a.ev_keys[e] = (unsigned long long)tgt << 11 | dt;
+        a.ev_keys[e] = tgt << 11 | dt;
         a.ev_w[e] = a.w[r];
     }
 }
@@ -127,10 +130,10 @@ __global__ __launch_bounds__(256) void k_ev_bounds(const EvArgs a, uint32_t n_ev
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n_ev)
         return;
-    const uint32_t tgt = (uint32_t)(a.ev_keys[i] >> 11);
-    if (i == 0 || (uint32_t)(a.ev_keys[i - 1] >> 11) != tgt)
+    const uint32_t tgt = a.ev_keys[i] >> 11;
+    if (i == 0 || (a.ev_keys[i - 1] >> 11) != tgt)
         a.seg_lo[tgt] = i;
-    if (i + 1 == n_ev || (uint32_t)(a.ev_keys[i + 1] >> 11) != tgt)
+    if (i + 1 == n_ev || (a.ev_keys[i + 1] >> 11) != tgt)
         a.seg_hi[tgt] = i + 1;
 }
 
@@ -182,7 +185,7 @@ __global__ __launch_bounds__(256) void k_ev_process(const EvArgs a)
         uint32_t t = 0;
         float w = 0.0f;
         if (i < hi) {
-            t = a.t0 + (uint32_t)(a.ev_keys[i] & 2047u);
+            t = a.t0 + (a.ev_keys[i] & 2047u);
             w = a.ev_w[i];
         }
         uint32_t tp = (uint32_t)__shfl_up((int)t, 1);
@@ -262,8 +265,8 @@ hipError_t ev_launch(const EvArgs& a, int which, uint32_t n, hipStream_t s)
 hipError_t ev_sort_inputs(void* temp, size_t* bytes, const unsigned long long* in, unsigned long long* out,
                           uint32_t n, hipStream_t s)
 {
-    // keys: source (< 2^24) << 32 | time
-    return hipcub::DeviceRadixSort::SortKeys(temp, *bytes, in, out, (int)n, 0, 56, s);
+    // keys: time << 32 | source
+    return hipcub::DeviceRadixSort::SortKeys(temp, *bytes, in, out, (int)n, 0, 64, s);
 }
 
 hipError_t ev_scan_rows(void* temp, size_t* bytes, const unsigned long long* in, unsigned long long* out, uint32_t n,
@@ -272,15 +275,15 @@ hipError_t ev_scan_rows(void* temp, size_t* bytes, const unsigned long long* in,
     return hipcub::DeviceScan::ExclusiveSum(temp, *bytes, in, out, (int)n, s);
 }
 
-hipError_t ev_sort_events(void* temp, size_t* bytes, const unsigned long long* kin, unsigned long long* kout,
-                          const float* vin, float* vout, uint32_t n, uint32_t n_neurons, hipStream_t s)
+hipError_t ev_sort_events(void* temp, size_t* bytes, const uint32_t* kin, uint32_t* kout, const float* vin,
+                          float* vout, uint32_t n, uint32_t n_neurons, hipStream_t s)
 {
-    // keys: target << 11 | time offset (< 2048), 11 + bit width of the
-    // largest target; radix sort is stable
+    // keys: target << 11 | time offset (< 2048), sorted on the target bits
+    // only (radix sort is stable, so the expansion order survives)
     int bits = 11;
     for (uint32_t m = n_neurons ? n_neurons - 1 : 0; m; m >>= 1)
         ++bits;
-    return hipcub::DeviceRadixSort::SortPairs(temp, *bytes, kin, kout, vin, vout, (int)n, 0, bits, s);
+    return hipcub::DeviceRadixSort::SortPairs(temp, *bytes, kin, kout, vin, vout, (int)n, 11, std::max(bits, 12), s);
 }
 
 }  // namespace naer

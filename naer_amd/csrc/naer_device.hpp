@@ -216,13 +216,13 @@ struct EvArgs {
     const unsigned long long* row_ptr;
     const uint32_t* col;
     const float* w;
-    const unsigned long long* in_keys;  // the window's inputs sorted by (source, time)
+    const unsigned long long* in_keys;  // the window's inputs (time << 32 | source), sorted
     unsigned long long* out_keys;       // input list being appended to
     uint32_t* n_out;                    // its count
     uint32_t* lost;                     // inputs beyond in_cap
     unsigned long long* row_len;        // [in_cap + 1]
     const unsigned long long* row_off;  // [in_cap + 1] exclusive scan of row_len
-    unsigned long long* ev_keys;        // target << 11 | time offset
+    uint32_t* ev_keys;                  // target << 11 | time offset (N < 2^21)
     float* ev_w;
     uint32_t* seg_lo;                   // [N] a target's range of sorted events
     uint32_t* seg_hi;

@@ -47,8 +47,8 @@ hipError_t ev_sort_inputs(void* temp, size_t* bytes, const unsigned long long* i
                           uint32_t n, hipStream_t s);
 hipError_t ev_scan_rows(void* temp, size_t* bytes, const unsigned long long* in, unsigned long long* out, uint32_t n,
                         hipStream_t s);
-hipError_t ev_sort_events(void* temp, size_t* bytes, const unsigned long long* kin, unsigned long long* kout,
-                          const float* vin, float* vout, uint32_t n, uint32_t n_neurons, hipStream_t s);
+hipError_t ev_sort_events(void* temp, size_t* bytes, const uint32_t* kin, uint32_t* kout, const float* vin,
+                          float* vout, uint32_t n, uint32_t n_neurons, hipStream_t s);
 
 // Longest CSR row the LDS row sort accepts.
 constexpr uint32_t kMaxSortedRow = 8192;

@@ -224,7 +224,7 @@ struct naer_net {
     uint64_t ev_e_cap = 0, ev_epoch = 0, ev_events = 0;
     unsigned long long *ev_list[2] = { nullptr, nullptr }, *ev_sorted = nullptr;
     unsigned long long *ev_len = nullptr, *ev_off = nullptr;
-    unsigned long long *ev_keys = nullptr, *ev_keys_s = nullptr;
+    uint32_t *ev_keys = nullptr, *ev_keys_s = nullptr;
     float *ev_w = nullptr, *ev_w_s = nullptr;
     uint32_t *ev_lo = nullptr, *ev_hi = nullptr, *ev_tlast = nullptr;
     double *ev_dlo = nullptr, *ev_dhi = nullptr;
@@ -1144,6 +1144,9 @@ static int setup_events(naer_net* n)
     if (n->S_cap > 500000000ull)
         return fail(NAER_ERR_CAPACITY, "event-driven mode: at most 5e8 synapses (this net has %llu)",
                     (unsigned long long)n->S_cap);
+    // event keys hold target << 11 in 32 bits
+    if (n->N >= (1u << 21))
+        return fail(NAER_ERR_CAPACITY, "event-driven mode: fewer than 2^21 neurons");
     int rc;
     const uint32_t np = (uint32_t)n->pp.size();
     // a POISSON source emits at most 10 spikes per window, a LIF neuron one
@@ -1160,7 +1163,7 @@ static int setup_events(naer_net* n)
     n->ev_in_cap = (uint32_t)in_cap;
     size_t free_b = 0, total_b = 0;
     HIP_TRY(hipMemGetInfo(&free_b, &total_b));
-    const double fit = (double)free_b / 4.0 / 24.0;
+    const double fit = (double)free_b / 4.0 / 16.0;
     // the library sorts take an int count
     n->ev_e_cap = (uint64_t)std::min(std::min(std::max((double)n->S_cap, (double)(1u << 22)), fit), 2147483647.0);
     if ((rc = n->alloc(&n->ev_list[0], in_cap)) || (rc = n->alloc(&n->ev_list[1], in_cap)) ||
