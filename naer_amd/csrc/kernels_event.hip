@@ -143,84 +143,97 @@ __device__ __forceinline__ double ev_decay(const double* H, const double* L, uin
     return h < n_hi ? H[h] * L[dt & 4095] : 0.0;
 }
 
-__device__ __forceinline__ double readlane_f64(double x, uint32_t k)
-{
-    const long long b = __double_as_longlong(x);
-    const int lo = __builtin_amdgcn_readlane((int)(uint32_t)b, (int)k);
-    const int hi = __builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), (int)k);
-    return __longlong_as_double((long long)((unsigned long long)(uint32_t)hi << 32 | (uint32_t)lo));
-}
-
 // lif.cl:15-34 per event, in time order; the decay e^(-dt/tau) is H[dt >> 12]
-// * L[dt & 4095] from the population's tables.  One wave per target: the
-// wave loads 64 events at a time (coalesced) and looks up each event's decay
-// from the previous event's time in parallel; the step itself runs in order
-// on the broadcast values, and looks the decay up again only when the
-// neuron's t_last is not the previous event's time (after a spike or a
-// dropped input).
+// * L[dt & 4095] from the population's tables.  A target per group of 16
+// lanes (4 per wave): the group loads 16 events at a time (coalesced) and
+// looks up each event's decay from the previous event's time in parallel; the
+// step itself runs in order on values shuffled within the group, and looks
+// the decay up again only when the neuron's t_last is not the previous
+// event's time (after a spike or a dropped input).
+constexpr uint32_t kEvGroup = 16;
+
 __global__ __launch_bounds__(256) void k_ev_process(const EvArgs a)
 {
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t j = blockIdx.x * 4u + (threadIdx.x >> 6);
-    if (j >= a.N)
-        return;
-    const uint32_t lo = a.seg_lo[j], hi = a.seg_hi[j];
-    if (lo == hi)
-        return;
-    const uint32_t p = ev_pop(a, j);
-    const PopParams& pp = a.pops[p];
-    if (pp.kind != NAER_POP_LIF || j < pp.begin || j >= pp.end)
+    const uint32_t gl = threadIdx.x & (kEvGroup - 1);
+    const uint32_t j = blockIdx.x * (256 / kEvGroup) + threadIdx.x / kEvGroup;
+    uint32_t lo = 0, hi = 0, p = 0;
+    if (j < a.N) {
+        lo = a.seg_lo[j];
+        hi = a.seg_hi[j];
+        if (lo != hi) {
+            p = ev_pop(a, j);
+            const PopParams& pp = a.pops[p];
+            if (pp.kind != NAER_POP_LIF || j < pp.begin || j >= pp.end)
+                hi = lo;
+        }
+    }
+    if (!__any(lo != hi))
         return;
     const double* L = a.dec_lo + (size_t)p * kEvLo;
     const double* H = a.dec_hi + (size_t)p * kEvHiCap;
-    const uint32_t n_hi = a.n_hi[p];
-    const uint32_t t_ref = a.t_ref[p];
-    const float v_rest = pp.v_rest, v_th = pp.v_th, v_reset = pp.v_reset;
-    float V = a.V[j];
-    uint32_t t_last = a.t_last[j];
+    uint32_t n_hi = 0, t_ref = 0, t_last = 0;
+    float v_rest = 0.0f, v_th = 0.0f, v_reset = 0.0f, V = 0.0f;
+    if (lo != hi) {
+        n_hi = a.n_hi[p];
+        t_ref = a.t_ref[p];
+        v_rest = a.pops[p].v_rest;
+        v_th = a.pops[p].v_th;
+        v_reset = a.pops[p].v_reset;
+        V = a.V[j];
+        t_last = a.t_last[j];
+    }
     uint32_t t_before = t_last;  // time the first event's looked-up decay starts from
     uint32_t fired = 0;
-    for (uint32_t c = lo; c < hi; c += 64u) {
-        const uint32_t i = c + lane;
+    for (uint32_t c = lo;; c += kEvGroup) {
+        const bool more = c < hi;
+        if (!__any(more))
+            break;
+        const uint32_t i = c + gl;
         uint32_t t = 0;
         float w = 0.0f;
         if (i < hi) {
             t = a.t0 + (a.ev_keys[i] & 2047u);
             w = a.ev_w[i];
         }
-        uint32_t tp = (uint32_t)__shfl_up((int)t, 1);
-        if (lane == 0)
+        uint32_t tp = (uint32_t)__shfl_up((int)t, 1, kEvGroup);
+        if (gl == 0)
             tp = t_before;
         const long long dtp = (long long)t - (long long)tp;
         const double d = (i < hi && dtp >= 0) ? ev_decay(H, L, n_hi, dtp) : 0.0;
-        const uint32_t n = min(64u, hi - c);
-        for (uint32_t k = 0; k < n; ++k) {
-            const uint32_t tk = (uint32_t)__builtin_amdgcn_readlane((int)t, (int)k);
-            const long long dt = (long long)tk - (long long)t_last;
-            if (dt < 0)
-                continue;
-            const uint32_t tpk = (uint32_t)__builtin_amdgcn_readlane((int)tp, (int)k);
-            const double dk = tpk == t_last ? readlane_f64(d, k) : ev_decay(H, L, n_hi, dt);
-            const float wk = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(w), (int)k));
-            float x = V - v_rest;
-            x = (float)((double)x * dk);
-            float v = x + v_rest;
-            v = v + wk;
-            t_last = tk;
-            if (v > v_th) {
-                v = v_reset;
-                t_last = tk + t_ref;
-                if (lane == 0) {
-                    ev_append(a, j, tk + kEpochUs);
-                    ev_record(a, j, tk);
+        const uint32_t n = more ? min(kEvGroup, hi - c) : 0u;
+        for (uint32_t k = 0; k < kEvGroup; ++k) {
+            // every lane takes part in the shuffles; the step is predicated
+            const uint32_t tk = (uint32_t)__shfl((int)t, (int)k, kEvGroup);
+            const uint32_t tpk = (uint32_t)__shfl((int)tp, (int)k, kEvGroup);
+            const float wk = __shfl(w, (int)k, kEvGroup);
+            const double dpk = __shfl(d, (int)k, kEvGroup);
+            if (k < n) {
+                const long long dt = (long long)tk - (long long)t_last;
+                if (dt >= 0) {
+                    const double dk = tpk == t_last ? dpk : ev_decay(H, L, n_hi, dt);
+                    float x = V - v_rest;
+                    x = (float)((double)x * dk);
+                    float v = x + v_rest;
+                    v = v + wk;
+                    t_last = tk;
+                    if (v > v_th) {
+                        v = v_reset;
+                        t_last = tk + t_ref;
+                        if (gl == 0) {
+                            ev_append(a, j, tk + kEpochUs);
+                            ev_record(a, j, tk);
+                        }
+                        ++fired;
+                    }
+                    V = v;
                 }
-                ++fired;
             }
-            V = v;
         }
-        t_before = (uint32_t)__builtin_amdgcn_readlane((int)t, (int)(n - 1));
+        const uint32_t tl = (uint32_t)__shfl((int)t, (int)(n ? n - 1 : 0), kEvGroup);
+        if (n)
+            t_before = tl;
     }
-    if (lane == 0) {
+    if (gl == 0 && lo != hi) {
         a.V[j] = V;
         a.t_last[j] = t_last;
         if (fired)
@@ -252,7 +265,7 @@ hipError_t ev_launch(const EvArgs& a, int which, uint32_t n, hipStream_t s)
             hipLaunchKernelGGL(k_ev_bounds, dim3((n + 255) / 256), dim3(256), 0, s, a, n);
         break;
     case kEvProcess:
-        hipLaunchKernelGGL(k_ev_process, dim3((a.N + 3) / 4), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_ev_process, dim3((a.N + 255 / kEvGroup) / (256 / kEvGroup)), dim3(256), 0, s, a);
         break;
     default:
         return hipErrorInvalidValue;
