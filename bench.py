@@ -69,10 +69,68 @@ WORKLOADS = {
     "plastic_100k": ("balanced LIF, N=100,000, in-degree 1000 (1e8 synapses), delays 1-5 ms, pair STDP on E->E "
                      "(w0 uniform in [0, 0.3 mV])"),
     "balanced_10k": "balanced LIF, N=10,000, in-degree 1000, J_E=0.1 mV, J_I=-0.5 mV, delay 1.5 ms",
+    "brunel_event": ("the reference's brunel model (src/main.cpp:32-50) at 1e8 synapses (N=44,720) in the "
+                     "event-driven mode: the reference's 1500 us epochs and LIF step"),
 }
 
 
-WORKLOAD_N = {"large_1m": 1_000_000, "plastic_100k": 100_000, "balanced_10k": 10_000}
+WORKLOAD_N = {"large_1m": 1_000_000, "plastic_100k": 100_000, "balanced_10k": 10_000, "brunel_event": 44_720}
+EVENT_WINDOW_US = 1500
+
+
+def bench_event(args):
+    """brunel in the event-driven mode: --steps steps of 0.1 ms become whole
+    1500 us windows; the oracle's event-driven restatement on brunel at 1e6
+    synapses is the CPU baseline."""
+    from naer_amd import _abi, models
+    synapses = args.n or 100_000_000
+    t = time.perf_counter()
+    net = models.brunel(synapses, mode=_abi.MODE_EVENT)
+    net.compile()
+    build_s = time.perf_counter() - t
+    windows = max(1, round(args.steps * 100 / EVENT_WINDOW_US))
+    net.run(max(1, round(args.warmup * 100 / EVENT_WINDOW_US)))
+    net.finish()
+    s0 = net.stats()
+    t0 = time.perf_counter()
+    net.run(windows)
+    net.finish()
+    wall = time.perf_counter() - t0
+    s1 = net.stats()
+    events = s1.events - s0.events
+    bio_s = windows * EVENT_WINDOW_US * 1e-6
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        small = models.brunel(1_000_000, mode=_abi.MODE_EVENT)
+        o = O.OracleNet(small.description(), O.FIXED, threads=1)
+        o.run_events(20)
+        e0 = o.event_count
+        tc = time.perf_counter()
+        o.run_events(200)
+        dtc = time.perf_counter() - tc
+        cpu = {"value": (o.event_count - e0) / dtc, "unit": "synaptic events/s", "cores": 1, "kind": "port",
+               "sample": f"oracle event-driven restatement (single thread) on brunel at 1e6 synapses "
+                         f"(N={o.N}), 200 windows after 20, {dtc:.1f} s"}
+        o.close()
+    out = {
+        "metric": "synaptic events/s & realtime factor, brunel_event",
+        "value": round(events / wall, 1), "unit": "synaptic events/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / (windows * EVENT_WINDOW_US / 100), 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 (f64 decay products)",
+        "data": "synthetic (device-built Bernoulli connectivity and the reference's Poisson trains)",
+        "config": {"workload": f"brunel_event: {WORKLOADS['brunel_event']}, {windows} windows = {bio_s:g} s",
+                   "n_neurons": net.getNeurons(), "n_synapses": int(net.getSynapses()), "mode": "event"},
+        "realtime_factor": round(bio_s / wall, 3),
+        "rate_hz": round((s1.spikes - s0.spikes) / net.getNeurons() / bio_s, 3),
+        "events": int(events), "build_s": round(build_s, 2),
+        "roofline": None,
+        "roofline_note": "the event-driven step kernel is bound by its in-order per-target arithmetic; see "
+                         "profiles/kernel_stats_event_brunel_1e8.md",
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
 
 
 def make_workload(name, n=None, indegree=None, **kw):
@@ -134,6 +192,9 @@ def main():
         sys.exit(2)
     if args.copy_probe:
         copy_probe()
+        return
+    if args.workload == "brunel_event":
+        bench_event(args)
         return
 
     from naer_amd import _abi, models

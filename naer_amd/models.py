@@ -87,11 +87,13 @@ def brunel_plus(num_synapses, **net_kw):
 
 
 def balanced(n, indegree, j_e, g, drive_hz, drive_w, delay_ms=1.5, plastic=False, seed=REFERENCE_SEED,
-             exc_frac=0.8, stdp=None, **net_kw):
+             exc_frac=0.8, stdp=None, i_model=None, **net_kw):
     """Brunel-style E/I network with fixed in-degree: every neuron receives
     exc_frac*indegree synapses from E (weight j_e) and the rest from I (weight
     -g*j_e).  With plastic=True, E->E synapses start uniform in [0, wmax] and
-    follow additive pair STDP."""
+    follow additive pair STDP.  i_model: the I population as a source-defined
+    model with the LIF step (a NeuronModel compiled from the shipped "lif"
+    source), the same neurons as the built-in LIF."""
     n_e = int(round(n * exc_frac))
     n_i = n - n_e
     k_e = int(round(indegree * exc_frac))
@@ -101,7 +103,14 @@ def balanced(n, indegree, j_e, g, drive_hz, drive_w, delay_ms=1.5, plastic=False
     net = Network(seed=seed, **net_kw)
     lif = dict(BRUNEL_LIF, ext_rate_hz=drive_hz, ext_weight=drive_w)
     E = net.addPopulation(NeuronType.STDP if plastic else NeuronType.NOP, n_e, **lif)
-    I = net.addPopulation(NeuronType.NOP, n_i, **lif)
+    if i_model is not None:
+        from . import neuron_models
+        q = BRUNEL_LIF
+        mp = neuron_models.lif_params(q["tau_m_ms"], q["v_th"], q["v_reset"], q["v_rest"], q["t_ref_ms"], net.dt_ms)
+        I = net.addPopulation(i_model, n_i, params=mp, v0_min=q["v0_min"], v0_max=q["v0_max"],
+                              ext_rate_hz=drive_hz, ext_weight=drive_w)
+    else:
+        I = net.addPopulation(NeuronType.NOP, n_i, **lif)
     s = seed
     if plastic:
         wmax = net.stdp["w_max"]
