@@ -144,13 +144,13 @@ __device__ __forceinline__ double ev_decay(const double* H, const double* L, uin
 }
 
 // lif.cl:15-34 per event, in time order; the decay e^(-dt/tau) is H[dt >> 12]
-// * L[dt & 4095] from the population's tables.  A target per group of 16
-// lanes (4 per wave): the group loads 16 events at a time (coalesced) and
+// * L[dt & 4095] from the population's tables.  A target per group of 8
+// lanes (8 per wave): the group loads 8 events at a time (coalesced) and
 // looks up each event's decay from the previous event's time in parallel; the
 // step itself runs in order on values shuffled within the group, and looks
 // the decay up again only when the neuron's t_last is not the previous
 // event's time (after a spike or a dropped input).
-constexpr uint32_t kEvGroup = 16;
+constexpr uint32_t kEvGroup = 8;
 
 __global__ __launch_bounds__(256) void k_ev_process(const EvArgs a)
 {
