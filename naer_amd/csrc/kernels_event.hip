@@ -15,8 +15,9 @@
 //   (stable radix sort of the events by target alone: each target's events
 //    keep the order (time, source, row position))
 //   k_ev_bounds    each target's range of sorted events
-//   k_ev_process   one thread per LIF neuron applies its events in order and
-//                  appends its spikes to the next window's input list
+//   k_ev_process   a group of 8 lanes per LIF neuron applies its events in
+//                  order and appends its spikes to the next window's input
+//                  list
 // The sorts and the scan are rocPRIM's (through hipCUB).
 #include <algorithm>
 
