@@ -54,6 +54,24 @@ def test_fixed_indegree_event_mode_with_equal_time_inputs():
     assert np.count_nonzero(sp["neuron"] < 200) > 200
 
 
+@pytest.mark.parametrize("t_ref_ms,v_rest", [(0.0, 0.0), (0.5, -5.0), (3.0, 2.0)])
+def test_event_mode_refractory_and_rest_edges(t_ref_ms, v_rest):
+    """t_ref = 0 lets a neuron fire several times per window (and at the same
+    microsecond as its input); v_rest != 0 decays towards it."""
+    net = Network(dt_ms=0.1, mode=_abi.MODE_EVENT, record_capacity=1 << 20)
+    P = net.addPopulation(NeuronType.POISSON, 1000, src_rate_hz=1500.0)
+    L = net.addPopulation(NeuronType.NOP, 300, tau_m_ms=15.0, v_th=12.0, v_reset=v_rest + 1.0, v_rest=v_rest,
+                          t_ref_ms=t_ref_ms, v0_min=v_rest, v0_max=12.0)
+    net.addConnectionFixedIndegree(P, L, 200, 0.0, 0.1, 0.6, seed=8)
+    net.addConnectionFixedIndegree(L, L, 30, -0.7, seed=9)
+    sp = compare(net, 30)
+    lif = sp[sp["neuron"] < 300]
+    assert len(lif) > 300
+    if t_ref_ms == 0.0:
+        per = np.unique(np.stack([lif["neuron"], lif["step"]], 1), axis=0, return_counts=True)[1]
+        assert per.max() > 1  # several spikes of one neuron in one window
+
+
 def test_event_mode_rejects_what_it_does_not_model(tmp_path):
     bad = Network(dt_ms=0.1, mode=_abi.MODE_EVENT)
     bad.addPopulation(NeuronType.NOP, 10, ext_rate_hz=1000.0, ext_weight=0.1)
