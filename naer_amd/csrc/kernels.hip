@@ -169,23 +169,24 @@ __device__ __forceinline__ float lazy_stdp(const StdpArgs& s, const TraceTabs& t
 
 // The pre-synaptic side of a spike of i at step t1, right after its batch's
 // update (lat[i] is that spike): potentiation is pending for post spikes from
-// L = max(i's previous spike, the last flush) on.
+// L = max(i's previous spike, the last flush of row i) on.
 struct PreSide {
     int32_t t1, L, tp;
     float xp;
     bool has_p;
 };
 
-__device__ __forceinline__ PreSide pre_side(const StdpArgs& s, uint32_t i, uint32_t t1, unsigned long long t_flush)
+__device__ __forceinline__ PreSide pre_side(const StdpArgs& s, uint32_t i, uint32_t t1)
 {
     PreSide p;
     const uint4 li = s.lat[i];
     const uint32_t s0 = s.pst[2 * (size_t)i].z;
+    const int32_t f = (int32_t)s.flushed[i];
     p.t1 = (int32_t)t1;
     p.has_p = li.x >= 2;
     p.tp = (int32_t)s0;
     p.xp = __uint_as_float(li.w);
-    p.L = p.has_p ? max(p.tp, (int32_t)t_flush) : (int32_t)t_flush;
+    p.L = p.has_p ? max(p.tp, f) : f;
     return p;
 }
 
@@ -279,7 +280,7 @@ __global__ __launch_bounds__(TW, (BUCKET && !PLASTIC && !EXTRA && TW == 1024) ? 
                 const unsigned long long t0 = st->clock;
                 st->cur = pl;
                 st->t0_cur = t0;
-                if (PLASTIC && t0 + a.B > st->t_flush + st->window)
+                if (PLASTIC && t0 + a.B > st->t_flush + (st->window >> 1))
                     st->t_flush = t0;
             }
         }
@@ -394,7 +395,7 @@ __global__ __launch_bounds__(TW, (BUCKET && !PLASTIC && !EXTRA && TW == 1024) ? 
         st->t0_cur = t0;
         // the flush kernel before this update applied the pending potentiation
         // under the same condition
-        if (PLASTIC && t0 + a.B > st->t_flush + st->window)
+        if (PLASTIC && t0 + a.B > st->t_flush + (st->window >> 1))
             st->t_flush = t0;
     }
     Stage sg{ s_list, &s_n, seg, a.seg_cap };
@@ -669,7 +670,6 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const ScatterArgs a)
     RingT* ring = (RingT*)a.ring;
     const uint32_t lane = lane_id();
     const unsigned long long chunk_off = (unsigned long long)blockIdx.y * kChunk;
-    const unsigned long long t_flush = PLASTIC ? st->t_flush : 0ull;
     {
         const uint32_t g = blockIdx.x;  // one block per list segment, its waves over the spikes
         const uint32_t cnt = counts[g];
@@ -684,7 +684,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const ScatterArgs a)
             const uint32_t tD = sp.step % a.D;
             PreSide ps{};
             if (PLASTIC)
-                ps = pre_side(a.sd, s, sp.step, t_flush);
+                ps = pre_side(a.sd, s, sp.step);
             uint32_t c[kChunkUnroll];
             float wv[kChunkUnroll];
 #pragma unroll
@@ -729,8 +729,13 @@ __global__ __launch_bounds__(kBlock) void k_flush(const FlushArgs a)
 {
     const DevState* st = a.st;
     const unsigned long long clock = st->clock, t_flush = st->t_flush;
-    if (!a.force && clock + a.B <= t_flush + st->window)
+    const uint32_t period = st->window >> 1;
+    if (!a.force && clock + a.B <= t_flush + period)
         return;
+    // a row is flushed now unless its pending range [L, ...) stays within
+    // the window until after the next flush (at most period + B steps away):
+    // rows of neurons that fired recently are left to their next spike
+    const long long keep = (long long)clock + period + 2 * (long long)a.B - (long long)st->window;
     const uint32_t lane = lane_id();
     const uint32_t wpb = kBlock / kWave;
     const uint32_t waves = gridDim.x * wpb;
@@ -745,9 +750,13 @@ __global__ __launch_bounds__(kBlock) void k_flush(const FlushArgs a)
             continue;
         const uint4 li = a.sd.lat[i];
         const bool has_p = li.x != 0;
+        if (!has_p)
+            continue;  // no spike of i yet: nothing to potentiate
         const int32_t tp = (int32_t)li.y;
         const float xp = __uint_as_float(li.z);
-        const int32_t L = has_p ? max(tp, (int32_t)t_flush) : (int32_t)t_flush;
+        const int32_t L = max(tp, (int32_t)a.sd.flushed[i]);
+        if (!a.force && (long long)L >= keep)
+            continue;
         constexpr uint32_t U = 4;  // synapses per lane in flight
         for (unsigned long long q0 = rb; q0 < re; q0 += U * kWave) {
             uint32_t c[U];
@@ -775,6 +784,8 @@ __global__ __launch_bounds__(kBlock) void k_flush(const FlushArgs a)
                                               pb[u], w[u], 0x7FFFFFFF, L, has_p, tp, xp);
             }
         }
+        if (lane == 0)
+            a.sd.flushed[i] = (uint32_t)clock;
     }
 }
 
@@ -1119,7 +1130,6 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_place(const BucketArg
     uint2* stage = a.stage;
     RingT* ring = (RingT*)a.ring;
     const uint32_t lane = lane_id();
-    const unsigned long long t_flush = PLASTIC ? st->t_flush : 0ull;
     for_block_chunks<U>(a, cur, s_off, t0, rs, [&](uint32_t r, uint32_t nrn, unsigned long long beg,
                                                    unsigned long long end) {
         uint32_t c[U];
@@ -1132,7 +1142,7 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_place(const BucketArg
         }
         if (PLASTIC) {
             // lazy pair STDP of the row's plastic synapses before they are placed
-            const PreSide ps = pre_side(a.sd, nrn, (uint32_t)(t0 + r), t_flush);
+            const PreSide ps = pre_side(a.sd, nrn, (uint32_t)(t0 + r));
             // the targets' newest two spikes for all U synapses first, then the walks
             uint4 pj[U], pb[U];
 #pragma unroll
@@ -1425,6 +1435,7 @@ __global__ __launch_bounds__(kBlock) void k_init_state(const UpdateArgs a, uint3
     a.ref[j] = 0;
     a.last[j] = -1;
     if (a.sd.lat) {
+        a.sd.flushed[j] = 0u;
         a.sd.lat[j] = make_uint4(0u, 0u, 0u, 0u);
         a.sd.pst[2 * (size_t)j] = make_uint4(kNoSpike, 0u, kNoSpike, 0u);
         a.sd.pst[2 * (size_t)j + 1] = make_uint4(kNoSpike, kNoSpike, 0u, 0u);

@@ -121,8 +121,8 @@ struct DevState {
     uint32_t batch;             // batches done; the update writes list[batch & 1]
     uint32_t cur;               // list of the batch in flight
     uint32_t overflow;          // spikes lost to a full list segment
-    uint32_t window;            // plastic nets: steps after a flush within which no neuron fires kSpikeHist times
-    unsigned long long t_flush; // plastic nets: potentiation is applied for every post spike before this step
+    uint32_t window;            // plastic nets: steps within which no neuron fires kSpikeHist times
+    unsigned long long t_flush; // plastic nets: step of the last flush (flushes come every window / 2 steps)
 };
 
 // Pair STDP with potentiation applied lazily (as update_synapse does at the
@@ -147,6 +147,7 @@ struct StdpArgs {
     uint4* pst;          // [N][2] post side: {s1, post trace a_s(s1) bits, s0, post trace a_s(s0) bits},
                          // {s2, s3 (the two spikes before s0), spikes so far, 0}: one 32-byte line part
     uint32_t* hist;      // [N][kSpikeHist] step of spike k at k % kSpikeHist
+    uint32_t* flushed;   // [N] row i's potentiation is applied for every post spike before flushed[i]
     const float* tp;     // [n_tp] exp(-k dt / tau_plus)
     const float* tm;     // [n_tm] exp(-k dt / tau_minus)
     uint32_t n_tp, n_tm;

@@ -187,6 +187,7 @@ struct naer_net {
     // lazy pair STDP (plastic nets): spike history and the traces' dec^k tables
     uint4 *lat = nullptr, *pst = nullptr;
     uint32_t* shist = nullptr;
+    uint32_t* flushed = nullptr;  // per row: potentiation applied for every post spike before this step
     float *d_tp = nullptr, *d_tm = nullptr;
     std::vector<float> h_tp, h_tm;
     uint32_t nref_min = 0;  // fewest refractory steps of any LIF population
@@ -600,6 +601,7 @@ StdpArgs stdp_args(const naer_net* n)
     a.lat = n->lat;
     a.pst = n->pst;
     a.hist = n->shist;
+    a.flushed = n->flushed;
     a.tp = n->d_tp;
     a.tm = n->d_tm;
     a.n_tp = (uint32_t)n->h_tp.size();
@@ -611,8 +613,11 @@ StdpArgs stdp_args(const naer_net* n)
     return a;
 }
 
-// Steps after a flush of pending potentiation within which no neuron can
-// fire kSpikeHist times (its spikes are >= nref_min + 1 steps apart).
+// Steps within which no neuron can fire kSpikeHist times (its spikes are >=
+// nref_min + 1 steps apart): a row's pending potentiation must be applied
+// (at the pre neuron's spike or by a flush) within this many steps of its
+// start.  Flushes run every half window and take only the rows whose pending
+// range would otherwise outgrow the window before the next one.
 uint32_t flush_window(const naer_net* n)
 {
     return (kSpikeHist - 1) * (n->nref_min + 1);
@@ -833,10 +838,12 @@ int alloc_neurons(naer_net* n, bool any_poisson)
         n->h_tm = trace_table(n->dt_ms, n->stdp.tau_minus_ms);
         if ((rc = n->alloc(&n->lat, N)) ||
             (rc = n->alloc(&n->pst, 2 * (size_t)N)) || (rc = n->alloc(&n->shist, (size_t)N * kSpikeHist)) ||
+            (rc = n->alloc(&n->flushed, N)) ||
             (rc = n->alloc(&n->d_tp, std::max<size_t>(1, n->h_tp.size()))) ||
             (rc = n->alloc(&n->d_tm, std::max<size_t>(1, n->h_tm.size()))))
             return rc;
         HIP_TRY(hipMemsetAsync(n->lat, 0, (size_t)N * sizeof(uint4), n->stream));
+        HIP_TRY(hipMemsetAsync(n->flushed, 0, (size_t)N * sizeof(uint32_t), n->stream));
         // every word kNoSpike: steps absent (the trace words are never read then)
         HIP_TRY(hipMemsetD32Async((hipDeviceptr_t)n->pst, (int)kNoSpike, (size_t)N * 8, n->stream));
         HIP_TRY(hipMemcpy(n->d_tp, n->h_tp.data(), n->h_tp.size() * sizeof(float), hipMemcpyHostToDevice));
@@ -1767,8 +1774,8 @@ static int enqueue_steps(naer_net* n, uint32_t n_steps, bool timed)
         sa.B = b;
         ba.B = b;
         fa.B = b;
-        // plastic nets: pending potentiation is flushed (on the device's
-        // condition) before a batch that would end outside the window
+        // plastic nets: every half window (the device's condition) the rows
+        // whose pending potentiation would outgrow the window are flushed
         if (n->plastic && (rc = launch_timed(n, NAER_K_POTENTIATE, do_flush, &fa, timed)))
             return rc;
         if ((rc = launch_timed(n, NAER_K_UPDATE, do_update, &ua, timed)))
@@ -2173,7 +2180,7 @@ int naer_net_raster_close(naer_net* n, uint64_t* n_total)
 // --- checkpoint / resume ---------------------------------------------------
 namespace {
 
-constexpr char kCheckpointMagic[8] = { 'N', 'A', 'E', 'R', 'C', 'K', 'P', '2' };
+constexpr char kCheckpointMagic[8] = { 'N', 'A', 'E', 'R', 'C', 'K', 'P', '3' };
 
 // Structure of a net that a checkpoint must match to be loaded.
 struct CheckpointShape {
@@ -2239,6 +2246,7 @@ int checkpoint_body(naer_net* n, CheckpointIO& io)
         return rc;
     if (n->plastic && ((rc = io.copy(n->lat, N * sizeof(uint4))) || (rc = io.copy(n->pst, 2 * N * sizeof(uint4))) ||
                        (rc = io.copy(n->shist, N * kSpikeHist * sizeof(uint32_t))) ||
+                       (rc = io.copy(n->flushed, N * sizeof(uint32_t))) ||
                        (rc = io.copy(n->w, n->S_cap * 4))))
         return rc;
     if (n->pnext && ((rc = io.copy(n->pnext, N * 4)) || (rc = io.copy(n->pcnt, N * 4))))
