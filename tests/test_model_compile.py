@@ -98,3 +98,25 @@ def test_pop_desc_layout_matches_header(tmp_path):
     P = _abi.PopDesc
     assert got == [ctypes.sizeof(P), P.src_rate_hz.offset, P.model.offset, P.params.offset,
                    ctypes.sizeof(_abi.NetDesc)]
+
+
+def test_model_populations_sit_between_nop_and_poisson():
+    """Address order STDP | NOP | models (grouped per model, in order of
+    first use) | POISSON, and the descriptor carries the model and params."""
+    from naer_amd.network import Network, NeuronType
+    a, b = NeuronModel(STEP, 1, "a"), NeuronModel(STEP, 1, "b")
+    net = Network(mode=_abi.MODE_DETERMINISTIC)
+    P = net.addPopulation(NeuronType.POISSON, 5)
+    A1 = net.addPopulation(a, 3, params=[0.9, 1.0, 0.0])
+    N1 = net.addPopulation(NeuronType.NOP, 4)
+    B1 = net.addPopulation(b, 2, params=[0.5])
+    S1 = net.addPopulation(NeuronType.STDP, 6)
+    A2 = net.addPopulation(a, 7, params=[0.8])
+    d = net.description()
+    assert [S1.address, N1.address, A1.address, A2.address, B1.address, P.address] == [0, 6, 10, 13, 20, 22]
+    kinds = [d.pops[i].kind for i in range(d.n_pops)]
+    assert kinds == [_abi.POP_LIF, _abi.POP_LIF, _abi.POP_MODEL, _abi.POP_MODEL, _abi.POP_MODEL, _abi.POP_POISSON]
+    assert d.pops[2].model == a._handle.value and d.pops[4].model == b._handle.value
+    assert list(d.pops[2].params)[:3] == pytest.approx([0.9, 1.0, 0.0])
+    with pytest.raises(ValueError):
+        net.addPopulation(a, 1, params=[0.0] * (_abi.MODEL_PARAMS + 1))
