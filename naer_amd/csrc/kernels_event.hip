@@ -14,10 +14,9 @@
 //                  order, i.e. by (time, source, row position)
 //   (stable radix sort of the events by target alone: each target's events
 //    keep the order (time, source, row position))
-//   k_ev_bounds    each target's range of sorted events
-//   k_ev_process   a group of 8 lanes per LIF neuron applies its events in
-//                  order and appends its spikes to the next window's input
-//                  list
+//   k_ev_process   a group of 8 lanes per LIF neuron finds its range of sorted
+//                  events (8-ary search), applies them in order and appends
+//                  its spikes to the next window's input list
 // The sorts and the scan are rocPRIM's (through hipCUB).
 #include <algorithm>
 
@@ -117,27 +116,6 @@ __global__ __launch_bounds__(256) void k_ev_expand(const EvArgs a)
     }
 }
 
-__global__ __launch_bounds__(256) void k_ev_clear(const EvArgs a)
-{
-    const uint32_t j = blockIdx.x * 256 + threadIdx.x;
-    if (j < a.N) {
-        a.seg_lo[j] = 0u;
-        a.seg_hi[j] = 0u;
-    }
-}
-
-__global__ __launch_bounds__(256) void k_ev_bounds(const EvArgs a, uint32_t n_ev)
-{
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n_ev)
-        return;
-    const uint32_t tgt = a.ev_keys[i] >> 11;
-    if (i == 0 || (a.ev_keys[i - 1] >> 11) != tgt)
-        a.seg_lo[tgt] = i;
-    if (i + 1 == n_ev || (a.ev_keys[i + 1] >> 11) != tgt)
-        a.seg_hi[tgt] = i + 1;
-}
-
 __device__ __forceinline__ double ev_decay(const double* H, const double* L, uint32_t n_hi, long long dt)
 {
     const unsigned long long h = (unsigned long long)dt >> 12;
@@ -153,14 +131,49 @@ __device__ __forceinline__ double ev_decay(const double* H, const double* L, uin
 // event's time (after a spike or a dropped input).
 constexpr uint32_t kEvGroup = 8;
 
+// First index of keys[0, n) whose target (key >> 11) is >= tgt (n if none),
+// found by the kEvGroup lanes of a group together: each round probes 8 points
+// of the remaining range.  Every lane of the wave takes part (ballots).
+__device__ __forceinline__ uint32_t group_lower_bound(const uint32_t* keys, uint32_t n, uint32_t tgt, uint32_t gl)
+{
+    const uint32_t base = (threadIdx.x & 63u) & ~(kEvGroup - 1);
+    uint32_t lo = 0, hi = n;  // keys[< lo] are below tgt; hi == n or keys[hi] is not
+    for (;;) {
+        const bool busy = hi - lo > kEvGroup;
+        if (!__any(busy))
+            break;
+        const unsigned long long span = hi - lo;
+        // probes q(g) = lo + (g + 1) span / 9, strictly increasing and < hi
+        auto q = [&](uint32_t g) { return lo + (uint32_t)((g + 1) * span / (kEvGroup + 1)); };
+        const bool ge = busy && (keys[q(gl)] >> 11) >= tgt;
+        const uint32_t bits = (uint32_t)(__ballot(ge) >> base) & ((1u << kEvGroup) - 1);
+        if (busy) {
+            if (bits == 0) {
+                lo = q(kEvGroup - 1) + 1;
+            } else {
+                const uint32_t f = (uint32_t)__ffs(bits) - 1;
+                const uint32_t nlo = f ? q(f - 1) + 1 : lo;
+                hi = q(f);
+                lo = nlo;
+            }
+        }
+    }
+    const uint32_t i = lo + gl;
+    const bool ge = i < hi && (keys[i] >> 11) >= tgt;
+    const uint32_t bits = (uint32_t)(__ballot(ge) >> base) & ((1u << kEvGroup) - 1);
+    return bits ? lo + (uint32_t)__ffs(bits) - 1 : hi;
+}
+
 __global__ __launch_bounds__(256) void k_ev_process(const EvArgs a)
 {
     const uint32_t gl = threadIdx.x & (kEvGroup - 1);
     const uint32_t j = blockIdx.x * (256 / kEvGroup) + threadIdx.x / kEvGroup;
-    uint32_t lo = 0, hi = 0, p = 0;
+    uint32_t p = 0;
+    // the target's range of sorted events (groups past N search nothing)
+    const uint32_t n_ev = j < a.N ? a.n_ev : 0u;
+    uint32_t lo = group_lower_bound(a.ev_keys, n_ev, j, gl);
+    uint32_t hi = group_lower_bound(a.ev_keys, n_ev, j + 1, gl);
     if (j < a.N) {
-        lo = a.seg_lo[j];
-        hi = a.seg_hi[j];
         if (lo != hi) {
             p = ev_pop(a, j);
             const PopParams& pp = a.pops[p];
@@ -257,13 +270,6 @@ hipError_t ev_launch(const EvArgs& a, int which, uint32_t n, hipStream_t s)
     case kEvExpand:
         if (n)
             hipLaunchKernelGGL(k_ev_expand, dim3(n), dim3(256), 0, s, a);
-        break;
-    case kEvClear:
-        hipLaunchKernelGGL(k_ev_clear, dim3(gn), dim3(256), 0, s, a);
-        break;
-    case kEvBounds:
-        if (n)
-            hipLaunchKernelGGL(k_ev_bounds, dim3((n + 255) / 256), dim3(256), 0, s, a, n);
         break;
     case kEvProcess:
         hipLaunchKernelGGL(k_ev_process, dim3((a.N + 255 / kEvGroup) / (256 / kEvGroup)), dim3(256), 0, s, a);

@@ -205,7 +205,7 @@ struct UpdateArgs {
 // Event-driven mode (kernels_event.hip): one 1500 us window per epoch.
 constexpr uint32_t kEvLo = 4096;    // entries of a population's L decay table
 constexpr uint32_t kEvHiCap = 8192; // most entries of its H table
-enum { kEvPoisson = 0, kEvRowLen, kEvExpand, kEvClear, kEvBounds, kEvProcess };
+enum { kEvPoisson = 0, kEvRowLen, kEvExpand, kEvProcess };
 
 struct EvArgs {
     const PopParams* pops;
@@ -225,8 +225,7 @@ struct EvArgs {
     const unsigned long long* row_off;  // [in_cap + 1] exclusive scan of row_len
     uint32_t* ev_keys;                  // target << 11 | time offset (N < 2^21)
     float* ev_w;
-    uint32_t* seg_lo;                   // [N] a target's range of sorted events
-    uint32_t* seg_hi;
+    uint32_t n_ev;                      // events of the window
     float* V;
     uint32_t* t_last;                   // [N] time of the last input taken (us)
     const double* dec_lo;               // [n_pops][kEvLo]

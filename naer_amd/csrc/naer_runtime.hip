@@ -227,7 +227,7 @@ struct naer_net {
     unsigned long long *ev_len = nullptr, *ev_off = nullptr;
     uint32_t *ev_keys = nullptr, *ev_keys_s = nullptr;
     float *ev_w = nullptr, *ev_w_s = nullptr;
-    uint32_t *ev_lo = nullptr, *ev_hi = nullptr, *ev_tlast = nullptr;
+    uint32_t* ev_tlast = nullptr;
     double *ev_dlo = nullptr, *ev_dhi = nullptr;
     uint32_t *ev_nhi = nullptr, *ev_tref = nullptr;
     // [0], [1]: counts of the two input lists, [2]: inputs lost; spikes at [4] (u64)
@@ -1177,8 +1177,7 @@ static int setup_events(naer_net* n)
         (rc = n->alloc(&n->ev_sorted, in_cap)) || (rc = n->alloc(&n->ev_len, in_cap + 1)) ||
         (rc = n->alloc(&n->ev_off, in_cap + 1)) || (rc = n->alloc(&n->ev_keys, n->ev_e_cap)) ||
         (rc = n->alloc(&n->ev_keys_s, n->ev_e_cap)) || (rc = n->alloc(&n->ev_w, n->ev_e_cap)) ||
-        (rc = n->alloc(&n->ev_w_s, n->ev_e_cap)) || (rc = n->alloc(&n->ev_lo, n->N)) ||
-        (rc = n->alloc(&n->ev_hi, n->N)) || (rc = n->alloc(&n->ev_tlast, n->N)) ||
+        (rc = n->alloc(&n->ev_w_s, n->ev_e_cap)) || (rc = n->alloc(&n->ev_tlast, n->N)) ||
         (rc = n->alloc(&n->ev_dlo, (size_t)std::max(1u, np) * kEvLo)) ||
         (rc = n->alloc(&n->ev_dhi, (size_t)std::max(1u, np) * kEvHiCap)) || (rc = n->alloc(&n->ev_nhi, np)) ||
         (rc = n->alloc(&n->ev_tref, np)) || (rc = n->alloc(&n->ev_ctr, 8)))
@@ -1245,8 +1244,6 @@ static EvArgs ev_args(const naer_net* n)
     a.lost = n->ev_ctr + 2;
     a.row_len = n->ev_len;
     a.row_off = n->ev_off;
-    a.seg_lo = n->ev_lo;
-    a.seg_hi = n->ev_hi;
     a.V = n->V;
     a.t_last = n->ev_tlast;
     a.dec_lo = n->ev_dlo;
@@ -1304,9 +1301,8 @@ static int ev_step(naer_net* n, uint32_t epochs)
         b.ev_w = n->ev_w_s;
         b.out_keys = n->ev_list[cur ^ 1u];
         b.n_out = n->ev_ctr + (cur ^ 1u);
+        b.n_ev = (uint32_t)n_ev;
         HIP_TRY(hipMemsetAsync(n->ev_ctr + (cur ^ 1u), 0, sizeof(uint32_t), n->stream));
-        HIP_TRY(ev_launch(b, kEvClear, 0, n->stream));
-        HIP_TRY(ev_launch(b, kEvBounds, (uint32_t)n_ev, n->stream));
         HIP_TRY(ev_launch(b, kEvProcess, 0, n->stream));
         n->ev_events += n_ev;
         n->ev_epoch += 1;
