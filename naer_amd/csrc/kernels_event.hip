@@ -196,31 +196,53 @@ __global__ __launch_bounds__(256) void k_ev_process(const EvArgs a)
         V = a.V[j];
         t_last = a.t_last[j];
     }
-    uint32_t t_before = t_last;  // time the first event's looked-up decay starts from
+    // Software pipeline over chunks of kEvGroup events: while chunk c is
+    // stepped, chunk c + 1's decays are looked up and chunk c + 2's events
+    // are loaded.
+    auto load = [&](uint32_t c, uint32_t& t, float& w) {
+        const uint32_t i = c + gl;
+        t = 0;
+        w = 0.0f;
+        if (i < hi) {
+            t = a.t0 + (a.ev_keys[i] & 2047u);
+            w = a.ev_w[i];
+        }
+    };
+    // the decay of each event of chunk c from the previous event's time
+    // (t_before: the last event of the chunk before)
+    auto lookup = [&](uint32_t c, uint32_t t, uint32_t t_before, uint32_t& tp, double& d) {
+        tp = (uint32_t)__shfl_up((int)t, 1, kEvGroup);
+        if (gl == 0)
+            tp = t_before;
+        const long long dtp = (long long)t - (long long)tp;
+        d = (c + gl < hi && dtp >= 0) ? ev_decay(H, L, n_hi, dtp) : 0.0;
+    };
+    uint32_t t0c, t1c, tp0;
+    float w0c, w1c;
+    double d0;
+    load(lo, t0c, w0c);
+    lookup(lo, t0c, t_last, tp0, d0);
+    load(lo + kEvGroup, t1c, w1c);
     uint32_t fired = 0;
     for (uint32_t c = lo;; c += kEvGroup) {
         const bool more = c < hi;
         if (!__any(more))
             break;
-        const uint32_t i = c + gl;
-        uint32_t t = 0;
-        float w = 0.0f;
-        if (i < hi) {
-            t = a.t0 + (a.ev_keys[i] & 2047u);
-            w = a.ev_w[i];
-        }
-        uint32_t tp = (uint32_t)__shfl_up((int)t, 1, kEvGroup);
-        if (gl == 0)
-            tp = t_before;
-        const long long dtp = (long long)t - (long long)tp;
-        const double d = (i < hi && dtp >= 0) ? ev_decay(H, L, n_hi, dtp) : 0.0;
         const uint32_t n = more ? min(kEvGroup, hi - c) : 0u;
+        // next chunk's decays start from this chunk's last event
+        const uint32_t t_end = (uint32_t)__shfl((int)t0c, (int)(n ? n - 1 : 0), kEvGroup);
+        uint32_t tp1;
+        double d1;
+        lookup(c + kEvGroup, t1c, t_end, tp1, d1);
+        uint32_t t2c;
+        float w2c;
+        load(c + 2 * kEvGroup, t2c, w2c);
         for (uint32_t k = 0; k < kEvGroup; ++k) {
             // every lane takes part in the shuffles; the step is predicated
-            const uint32_t tk = (uint32_t)__shfl((int)t, (int)k, kEvGroup);
-            const uint32_t tpk = (uint32_t)__shfl((int)tp, (int)k, kEvGroup);
-            const float wk = __shfl(w, (int)k, kEvGroup);
-            const double dpk = __shfl(d, (int)k, kEvGroup);
+            const uint32_t tk = (uint32_t)__shfl((int)t0c, (int)k, kEvGroup);
+            const uint32_t tpk = (uint32_t)__shfl((int)tp0, (int)k, kEvGroup);
+            const float wk = __shfl(w0c, (int)k, kEvGroup);
+            const double dpk = __shfl(d0, (int)k, kEvGroup);
             if (k < n) {
                 const long long dt = (long long)tk - (long long)t_last;
                 if (dt >= 0) {
@@ -243,9 +265,12 @@ __global__ __launch_bounds__(256) void k_ev_process(const EvArgs a)
                 }
             }
         }
-        const uint32_t tl = (uint32_t)__shfl((int)t, (int)(n ? n - 1 : 0), kEvGroup);
-        if (n)
-            t_before = tl;
+        t0c = t1c;
+        w0c = w1c;
+        tp0 = tp1;
+        d0 = d1;
+        t1c = t2c;
+        w1c = w2c;
     }
     if (gl == 0 && lo != hi) {
         a.V[j] = V;
