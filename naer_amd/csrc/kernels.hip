@@ -20,6 +20,9 @@
 //                   batch and target tile, summed in the update's LDS.
 //   k_conn_*     <- count_connections / integrate_connections /
 //                   generate_connections (network.cl:14-107).
+// k_update also stages the input of populations of source-defined models
+// (NeuronType.h:14), whose own kernels naer_model.hip compiles; the
+// reference's event-driven step itself is in kernels_event.hip.
 #include <mutex>
 
 #include "naer_device.hpp"
