@@ -61,21 +61,26 @@ def _progress(msg):
     print("# " + msg, file=sys.stderr, flush=True)
 
 
-def run_packed(specs, steps, mode=_abi.MODE_DETERMINISTIC, chunk=100, record=True, graphs=True, verbose=False,
-               autotune=True, packs_per_kind=1, plastic_packs=None):
+def pack_groups(specs, packs_per_kind=1, plastic_packs=None):
+    """Networks per pack: static and plastic networks never share a pack (a
+    flush of pending potentiation then runs over plastic packs only); packs of
+    a kind take every k-th network by size, so they carry similar work.  Empty
+    groups are dropped."""
     groups = []
     for plastic in (False, True):
-        kind = [s for s in specs if s["plastic"] == plastic]
-        # packs of a kind take every k-th network by size, so they carry
-        # similar work; each pack runs on its own stream
+        kind = sorted((s for s in specs if s["plastic"] == plastic), key=lambda s: s["n"])
         k = plastic_packs if (plastic and plastic_packs) else packs_per_kind
-        kind.sort(key=lambda s: s["n"])
         groups += [kind[i::k] for i in range(k)]
+    return [g for g in groups if g]
+
+
+def run_packed(specs, steps, mode=_abi.MODE_DETERMINISTIC, chunk=100, record=True, graphs=True, verbose=False,
+               autotune=True, packs_per_kind=1, plastic_packs=None):
+    # each pack runs on its own stream
+    groups = pack_groups(specs, packs_per_kind, plastic_packs)
     packs = []
     t_build = time.perf_counter()
     for g in groups:
-        if not g:
-            continue
         nets = [make_network(s, mode, expected_record(s, steps) if record else 0) for s in g]
         p = Pack(nets)
         p.build()
