@@ -130,6 +130,8 @@ __device__ __forceinline__ double ev_decay(const double* H, const double* L, uin
 // the decay up again only when the neuron's t_last is not the previous
 // event's time (after a spike or a dropped input).
 constexpr uint32_t kEvGroup = 8;
+static_assert(kEvGroup >= 2 && kEvGroup < 32 && (kEvGroup & (kEvGroup - 1)) == 0,
+              "a step group is a power of two of lanes below 32 (its ballot mask is 32-bit)");
 
 // First index of keys[0, n) whose target (key >> 11) is >= tgt (n if none),
 // found by the kEvGroup lanes of a group together: each round probes 8 points
