@@ -817,16 +817,23 @@ __device__ __forceinline__ void bucket_setup(const BucketArgs& a, uint8_t* lut_m
 // a nondecreasing map, so every block gets a contiguous (possibly empty)
 // range; blk_seg[b] = first segment of block b, blk_seg[NB] = n_seg.  One
 // block of 1024 threads, each owning a contiguous run of segments.
+// Walking a segment costs something even without events (its count is read
+// in a window of kMaxSegPerBlock, under a barrier): each segment weighs its
+// events plus this many, so blocks of a sparse arena do not get thousands of
+// empty segments.
+constexpr unsigned long long kSegWeight = 64;
+
 __global__ __launch_bounds__(1024) void k_bucket_split(const BucketArgs a)
 {
     __shared__ unsigned long long s_part[1024];
     const uint32_t cur = a.st->cur;
-    const unsigned long long* ev = a.seg_ev + (size_t)cur * a.n_seg;
+    const unsigned long long* seg_ev = a.seg_ev + (size_t)cur * a.n_seg;
+    auto weight = [&](uint32_t k) { return seg_ev[k] + kSegWeight; };
     const uint32_t per = (a.n_seg + 1023) / 1024;
     const uint32_t lo = min(threadIdx.x * per, a.n_seg), hi = min(lo + per, a.n_seg);
     unsigned long long sum = 0;
     for (uint32_t k = lo; k < hi; ++k)
-        sum += ev[k];
+        sum += weight(k);
     s_part[threadIdx.x] = sum;
     __syncthreads();
     for (uint32_t off = 1; off < 1024; off <<= 1) {
@@ -844,11 +851,11 @@ __global__ __launch_bounds__(1024) void k_bucket_split(const BucketArgs a)
     };
     uint32_t prev = 0;
     if (lo > 0 && lo < hi) {
-        const unsigned long long e = ev[lo - 1];
+        const unsigned long long e = weight(lo - 1);
         prev = block_of(lo - 1, run - e, e);
     }
     for (uint32_t k = lo; k < hi; ++k) {
-        const unsigned long long e = ev[k];
+        const unsigned long long e = weight(k);
         const uint32_t b = block_of(k, run, e);
         // blocks (prev, b] start at k (blocks [0, b] for the first segment)
         for (uint32_t q = k == 0 ? 0u : prev + 1u; q <= b; ++q)
