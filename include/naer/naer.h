@@ -74,7 +74,9 @@ extern "C" {
  *   initial potential of neuron j:        ctr = {j, 0, NAER_STREAM_V0, 0}, word 0
  *   Poisson drive of neuron j at step t:   ctr = {j, t >> 2, NAER_STREAM_DRIVE, 0},
  *                                          word t & 3
- *   draw k of target j, FIXED_INDEGREE connection c (connection seed):
+ *   draw k of target j, FIXED_INDEGREE connection c (connection seed; here
+ *   j is the target's index within the target population and c the
+ *   connection's position in the description):
  *                                          ctr = {j, k, NAER_STREAM_CONN, c};
  *                                          word 0 -> source, 1 -> delay, 2 -> weight
  * A uniform u in [0,1) is (word >> 8) * 2^-24 in fp32; an index in [0,n) is

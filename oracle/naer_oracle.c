@@ -496,7 +496,17 @@ static inline uint32_t delay_of(const naer_conn_desc* k, uint32_t word)
     return k->delay_min + q * bounded(word, choices);
 }
 
-/* Enumerates the synapses of connection c, calling cb for each. */
+/* Test hook: plants one known mistake in the fixed in-degree rule, so the
+ * connectivity tests can show that their checks catch it.  0 (the default)
+ * is the rule as specified; the other values are listed in naer_oracle.h. */
+static uint32_t g_conn_mutation = ORC_MUT_NONE;
+void orc_set_conn_mutation(uint32_t m) { g_conn_mutation = m; }
+
+/* Enumerates the synapses of connection c, calling cb for each.
+ * FIXED_INDEGREE: target j (index within the target population) takes
+ * indegree draws k; draw k is Philox block {j, k, NAER_STREAM_CONN, c} under
+ * the connection's 64-bit seed: word 0 -> source (uniform over the source
+ * population), word 1 -> delay class, word 2 -> weight when weight == 0. */
 static void gen_conn(const orc_net* n, uint32_t c, syn_cb cb, void* ctx)
 {
     const naer_conn_desc* k = &n->conns[c];
@@ -504,16 +514,30 @@ static void gen_conn(const orc_net* n, uint32_t c, syn_cb cb, void* ctx)
     uint32_t db = n->pop_base[k->dst_pop], nd = n->pops[k->dst_pop].size;
     uint32_t pl = k->plastic ? 0x80000000u : 0u;
     if (k->rule == NAER_RULE_FIXED_INDEGREE) {
+        const uint32_t mut = g_conn_mutation;
+        const uint32_t s_base = mut == ORC_MUT_SRC_BASE ? db : sb;
+        const uint32_t s_range = mut == ORC_MUT_SRC_RANGE ? nd : ns;
         for (uint32_t j = 0; j < nd; ++j) {
             for (uint32_t kk = 0; kk < k->indegree; ++kk) {
                 uint32_t r[4];
                 philox_seeded(j, kk, NAER_STREAM_CONN, c, k->seed, r);
-                uint32_t src = sb + bounded(r[0], ns);
-                uint32_t del = delay_of(k, r[1]);
+                uint32_t w_src = r[0], w_del = r[1], w_wt = r[2];
+                if (mut == ORC_MUT_WORDS_SWAPPED) {
+                    w_src = r[1];
+                    w_del = r[0];
+                } else if (mut == ORC_MUT_DELAY_REUSE) {
+                    w_del = r[0];
+                } else if (mut == ORC_MUT_WEIGHT_REUSE) {
+                    w_wt = r[0];
+                }
+                uint32_t src = s_base + bounded(w_src, s_range);
+                if (mut != ORC_MUT_NONE)
+                    src %= n->N;  /* a planted mistake still names a neuron of the net */
+                uint32_t del = delay_of(k, w_del);
                 float w = k->weight;
                 if (k->weight == 0.0f) {
-                    float span = k->wmax - k->wmin;
-                    float x = span * u24(r[2]);
+                    float span = mut == ORC_MUT_WEIGHT_SPAN ? k->wmax : k->wmax - k->wmin;
+                    float x = span * u24(w_wt);
                     w = k->wmin + x;
                 }
                 cb(ctx, src, (db + j) | (del << 24) | pl, w);

@@ -48,6 +48,22 @@ int orc_ref_poisson_epoch(uint32_t idx, uint32_t n_pois, uint32_t seed, float ra
 uint32_t orc_ref_bernoulli_row(uint32_t src, uint32_t n_dst, double p, uint32_t seed32,
                                uint32_t* out, uint32_t cap);
 
+/* Planted mistakes of the fixed in-degree rule (orc_set_conn_mutation), for
+ * tests that must fail on them: sources drawn over the target population's
+ * size, sources offset by the target population's base, Philox words 0 and 1
+ * swapped, the delay or the weight drawn from the source's word, random
+ * weights spanning [wmin, wmin + wmax). */
+enum {
+    ORC_MUT_NONE = 0,
+    ORC_MUT_SRC_RANGE = 1,
+    ORC_MUT_SRC_BASE = 2,
+    ORC_MUT_WORDS_SWAPPED = 3,
+    ORC_MUT_DELAY_REUSE = 4,
+    ORC_MUT_WEIGHT_REUSE = 5,
+    ORC_MUT_WEIGHT_SPAN = 6,
+};
+void orc_set_conn_mutation(uint32_t m);
+
 typedef struct orc_net orc_net;
 orc_net* orc_net_create(const naer_net_desc* desc, int precision, int threads);
 const char* orc_error(void);

@@ -58,8 +58,20 @@ def lib():
         L.orc_ev_read_state.argtypes = [v, u32, p, p]
         L.orc_ev_free.argtypes = [v]
         L.orc_ev_tables.argtypes = [ctypes.c_double, p, p, u32, ctypes.POINTER(u32)]
+        L.orc_set_conn_mutation.argtypes = [u32]
         _lib = L
     return _lib
+
+
+# Planted mistakes of the fixed in-degree rule (naer_oracle.h), for tests
+# that must fail on them.
+MUT_NONE, MUT_SRC_RANGE, MUT_SRC_BASE, MUT_WORDS_SWAPPED, MUT_DELAY_REUSE, MUT_WEIGHT_REUSE, MUT_WEIGHT_SPAN = range(7)
+
+
+def set_conn_mutation(m):
+    """Plant mistake m in the oracle's fixed in-degree rule (MUT_NONE: the
+    rule as specified) for the networks built afterwards."""
+    lib().orc_set_conn_mutation(int(m))
 
 
 def philox(ctr, key):
