@@ -43,8 +43,9 @@
  *
  * Synapse storage: outgoing CSR.  row_ptr is uint64[N+1]; each synapse has a
  * uint32 word  target | delay_steps << 24 | plastic << 31  and an fp32 weight.
- * Rows are sorted ascending by (word, weight bits), so the layout is a pure
- * function of the network description and seeds.
+ * Rows are sorted ascending by (g, word, weight bits), g = the target's index
+ * within its network divided by 2^14, so the layout is a pure function of the
+ * network description and seeds (not of its place in a pack or arena).
  * Limits: N <= 2^24 neurons per net, delays 1..127 steps.
  */
 #ifndef NAER_H
@@ -414,8 +415,14 @@ typedef struct naer_schedule {
     uint32_t task_synapses;      /* synapses of a row per wave task: 256 or 512 */
     uint32_t segments_per_block; /* mean spike-list segments per histogram/placement block, 1..4096 */
     uint32_t refine_split;       /* refine blocks per group bucket, 1..64 */
-    uint32_t run_aggregate;      /* 1: one LDS counter atomic per run of equal bucket keys in a
-                                    wave (rows concentrated on few tiles), 0: one per synapse */
+    uint32_t run_aggregate;      /* bit 0: the histogram and refine counters (keyed by target
+                                    tile), bit 1: the placement counters (keyed by target group):
+                                    one LDS counter atomic per run of equal keys in a wave instead
+                                    of one per synapse */
+    uint32_t record_bytes;       /* out: bytes per bucketed event: 4 when every population's
+                                    outgoing synapses share one constant weight (the record
+                                    names the source's population), else 8 (the record carries
+                                    the weight) */
     double delivery_us;          /* out: delivery time per batch measured by the autotuner */
 } naer_schedule;
 int naer_net_get_schedule(naer_net* net, naer_schedule* out);

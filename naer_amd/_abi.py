@@ -93,7 +93,8 @@ class Spike(ctypes.Structure):
 class Schedule(ctypes.Structure):
     _fields_ = [("bucketed", ctypes.c_uint32), ("task_synapses", ctypes.c_uint32),
                 ("segments_per_block", ctypes.c_uint32), ("refine_split", ctypes.c_uint32),
-                ("run_aggregate", ctypes.c_uint32), ("delivery_us", ctypes.c_double)]
+                ("run_aggregate", ctypes.c_uint32), ("record_bytes", ctypes.c_uint32),
+                ("delivery_us", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -194,7 +195,7 @@ def set_schedule(handle, task_synapses, segments_per_block, refine_split, run_ag
     """run_aggregate None keeps the current counter mode."""
     if run_aggregate is None:
         run_aggregate = get_schedule(handle)["run_aggregate"]
-    s = Schedule(0, int(task_synapses), int(segments_per_block), int(refine_split), int(run_aggregate), 0.0)
+    s = Schedule(0, int(task_synapses), int(segments_per_block), int(refine_split), int(run_aggregate), 0, 0.0)
     check(lib().naer_net_set_schedule(handle, ctypes.byref(s)))
 
 

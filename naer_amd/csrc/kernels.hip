@@ -59,9 +59,14 @@ __device__ __forceinline__ float clip(float w, float lo, float hi)
 // whole waves on one tile, and lane-by-lane atomics on one LDS word
 // serialise.  Without AGG (rows spread over many tiles) a plain atomic per
 // lane keeps the unrolled atomics of a task in flight together.
-constexpr uint32_t kRunHeads = 16;  // more run heads per wave: lane-by-lane atomics
+// More run heads per wave than HEADS: lane-by-lane atomics.  The histogram
+// and refine counters (tile keys) aggregate only long runs; the placement's
+// group keys come in runs of a few synapses (rows are ordered by group and
+// delay), where one atomic per run still beats one per lane.
+constexpr uint32_t kRunHeads = 16;
+constexpr uint32_t kPlaceRunHeads = 40;
 
-template <bool POS, bool AGG>
+template <bool POS, bool AGG, uint32_t HEADS = kRunHeads>
 __device__ __forceinline__ uint32_t run_add(uint32_t* cnt, uint32_t key)
 {
     if (!AGG)
@@ -70,7 +75,7 @@ __device__ __forceinline__ uint32_t run_add(uint32_t* cnt, uint32_t key)
     // previous lane's key (DPP wave_shr:1; lane 0 is a head regardless)
     const uint32_t prev = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)key, 0x138, 0xF, 0xF, false);
     const unsigned long long heads = __ballot(lane == 0 || prev != key);
-    if (__popcll(heads) > kRunHeads)
+    if (__popcll(heads) > HEADS)
         return atomicAdd(&cnt[key], 1u);
     const unsigned long long act = __ballot(1);
     const unsigned long long upto = (2ull << lane) - 1ull;  // lanes <= lane (all for lane 63)
@@ -255,9 +260,11 @@ __device__ __forceinline__ void emit(Stage& sg, bool fire, uint32_t neuron, uint
 // ---------------------------------------------------------------------------
 // EXTRA: the rarely used population kinds (POISSON sources, source-defined
 // models) are compiled in; nets of LIF populations only run without them.
-template <bool DET, bool PLASTIC, bool BUCKET, bool EXTRA, uint32_t TW>
+// REC4: the bucketed events are 4-byte class records (kRec4*).
+template <bool DET, bool PLASTIC, bool BUCKET, bool EXTRA, uint32_t TW, bool REC4>
 __global__ __launch_bounds__(TW, (BUCKET && !PLASTIC && !EXTRA && TW == 1024) ? 8 : 1) void k_update(const UpdateArgs a)
 {
+    static_assert(!REC4 || (BUCKET && !PLASTIC), "class records: bucketed static delivery only");
     using RingT = typename std::conditional<DET, long long, float>::type;
     // bucketed input accumulates in 32.32 fixed point in both modes: integer
     // LDS atomics run at several times the rate of fp32 ones on gfx950
@@ -320,6 +327,11 @@ __global__ __launch_bounds__(TW, (BUCKET && !PLASTIC && !EXTRA && TW == 1024) ? 
         ref_in = a.ref[j];
         last_in = a.last[j];
     }
+    __shared__ unsigned long long s_cls[REC4 ? kRec4StagedClasses : 1];
+    if (REC4) {
+        for (uint32_t i = threadIdx.x; i < min(a.n_cls, kRec4StagedClasses); i += TW)
+            s_cls[i] = a.cls_fx[i];
+    }
     if (BUCKET) {
         for (uint32_t i = threadIdx.x; i < a.B * TW; i += TW)
             acc[i] = (AccT)0;
@@ -362,7 +374,32 @@ __global__ __launch_bounds__(TW, (BUCKET && !PLASTIC && !EXTRA && TW == 1024) ? 
         const uint32_t total = s_pre[a.M];
         constexpr uint32_t U = 8;
         uint32_t m = 0;  // bucket of the thread's current event (its indexes only grow)
-        for (uint32_t i0 = threadIdx.x; i0 < total; i0 += TW * U) {
+        if constexpr (REC4) {
+            const uint32_t* ev4 = (const uint32_t*)a.ev;
+            for (uint32_t i0 = threadIdx.x; i0 < total; i0 += TW * U) {
+                uint32_t e[U];
+#pragma unroll
+                for (uint32_t u = 0; u < U; ++u) {
+                    const uint32_t i = i0 + u * TW;
+                    e[u] = 0u;
+                    if (i < total) {
+                        while (i >= s_pre[m + 1])
+                            ++m;
+                        e[u] = ev4[s_evbase[m] + i];
+                    }
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < U; ++u) {
+                    const uint32_t o = ((e[u] >> kRec4SlotShift) & 0xFu) - o0;
+                    if (i0 + u * TW < total && o < a.B) {
+                        const uint32_t c = e[u] >> kRec4ClassShift;
+                        const unsigned long long x = c < kRec4StagedClasses ? s_cls[c] : a.cls_fx[c];
+                        atomicAdd(&acc[o * TW + (e[u] & kRec4OffsetMask)], x);
+                    }
+                }
+            }
+        }
+        for (uint32_t i0 = threadIdx.x; !REC4 && i0 < total; i0 += TW * U) {
             uint2 e[U];
 #pragma unroll
             for (uint32_t u = 0; u < U; ++u) {
@@ -880,8 +917,15 @@ struct RowStage {
     uint32_t nrn[kRowRound];  // the spiking neuron
 };
 
-// Calls f(r, neuron, q_begin, q_end) for every non-empty (64 U)-synapse chunk of the
-// rows of the block's spikes; f runs on a whole wave (q range wave-uniform).
+// ... and the class of each spike's neuron (class-weight delivery)
+struct RowStageCls : RowStage {
+    uint32_t cls[kRowRound];
+};
+
+// Calls f(r, neuron, q_begin, q_end, i) for every non-empty (64 U)-synapse chunk of the
+// rows of the block's spikes (i: the spike's entry in the row stage); f runs
+// on a whole wave (q range wave-uniform).  CLS: the stage also holds the
+// class of each spike's neuron.
 // The block's segments are staged kMaxSegPerBlock at a time.
 // Workgroups are dealt to the 8 XCDs round-robin (block b to XCD b % 8);
 // xcd_block gives XCD x the x-th contiguous share of the logical blocks, so
@@ -896,9 +940,9 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb)
     return x * q + min(x, r) + i;
 }
 
-template <uint32_t U, typename F>
+template <uint32_t U, bool CLS, typename RS, typename F>
 __device__ __forceinline__ void for_block_chunks(const BucketArgs& a, uint32_t cur, uint32_t* s_off,
-                                                 unsigned long long t0, RowStage& rs, F f)
+                                                 unsigned long long t0, RS& rs, F f)
 {
     constexpr uint32_t kTask = kWave * U;  // synapses of a row per task
     const naer_spike* list = cur ? a.list1 : a.list0;
@@ -935,6 +979,8 @@ __device__ __forceinline__ void for_block_chunks(const BucketArgs& a, uint32_t c
                 rs.len[i] = (uint32_t)(a.row_ptr[sp.neuron + 1] - rb);
                 rs.r[i] = (uint32_t)(sp.step - t0);
                 rs.nrn[i] = sp.neuron;
+                if constexpr (CLS)
+                    rs.cls[i] = a.ncls[sp.neuron];
             }
             __syncthreads();
             const uint32_t n_tasks = cnt * a.C;
@@ -942,7 +988,7 @@ __device__ __forceinline__ void for_block_chunks(const BucketArgs& a, uint32_t c
                 const uint32_t i = t / a.C, c = t - i * a.C;
                 const uint32_t off = c * kTask, len = rs.len[i];
                 if (off < len)
-                    f(rs.r[i], rs.nrn[i], rs.beg[i] + off, rs.beg[i] + min(len, off + kTask));
+                    f(rs.r[i], rs.nrn[i], rs.beg[i] + off, rs.beg[i] + min(len, off + kTask), i);
             }
             __syncthreads();
         }
@@ -965,8 +1011,8 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_hist(const BucketArgs
     bucket_setup(a, lut_m, lut_o);
     __syncthreads();
     const uint32_t lane = lane_id();
-    for_block_chunks<U>(a, cur, s_off, t0, rs, [&](uint32_t r, uint32_t, unsigned long long beg,
-                                                   unsigned long long end) {
+    for_block_chunks<U, false>(a, cur, s_off, t0, rs, [&](uint32_t r, uint32_t, unsigned long long beg,
+                                                          unsigned long long end, uint32_t) {
         uint32_t c[U];
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u) {
@@ -1101,14 +1147,18 @@ __global__ __launch_bounds__(1024) void k_bucket_starts(const BucketArgs a)
     }
 }
 
-template <bool DET, uint32_t U, bool AGG, bool PLASTIC>
+// REC4: class-weight delivery (4-byte records, no weight reads); static
+// networks only.
+template <bool DET, uint32_t U, bool AGG, bool PLASTIC, bool REC4>
 __global__ __launch_bounds__(kBucketThreads) void k_bucket_place(const BucketArgs a)
 {
+    static_assert(!(PLASTIC && REC4), "plastic synapses carry their own weights");
     using RingT = typename std::conditional<DET, unsigned long long, float>::type;
+    using Stage = typename std::conditional<REC4, RowStageCls, RowStage>::type;
     extern __shared__ uint32_t s_cur[];
     __shared__ uint8_t lut_m[kArrivalLut], lut_o[kArrivalLut];
     __shared__ uint32_t s_off[kMaxSegPerBlock + 1];
-    __shared__ RowStage rs;
+    __shared__ Stage rs;
     DevState* st = a.st;
     const uint32_t cur = st->cur;
     const unsigned long long t0 = st->t0_cur;
@@ -1138,17 +1188,30 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_place(const BucketArg
     if (a.is_last && blockIdx.x == 0 && threadIdx.x == 0)
         batch_epilogue(st, a.B);
     uint2* stage = a.stage;
+    uint32_t* stage4 = (uint32_t*)a.stage;
     RingT* ring = (RingT*)a.ring;
     const uint32_t lane = lane_id();
-    for_block_chunks<U>(a, cur, s_off, t0, rs, [&](uint32_t r, uint32_t nrn, unsigned long long beg,
-                                                   unsigned long long end) {
+    for_block_chunks<U, REC4>(a, cur, s_off, t0, rs, [&](uint32_t r, uint32_t nrn, unsigned long long beg,
+                                                         unsigned long long end, uint32_t si) {
         uint32_t c[U];
         float w[U];
+        uint32_t cls = 0;
+        if constexpr (REC4) {
+            cls = rfl(rs.cls[si]);
+            const float wc = a.cls_w[cls];  // the ring fallback's weight
 #pragma unroll
-        for (uint32_t u = 0; u < U; ++u) {
-            const unsigned long long q = beg + u * kWave + lane;
-            c[u] = q < end ? a.col[q] : 0u;
-            w[u] = q < end ? a.w[q] : 0.0f;
+            for (uint32_t u = 0; u < U; ++u) {
+                const unsigned long long q = beg + u * kWave + lane;
+                c[u] = q < end ? a.col[q] : 0u;
+                w[u] = wc;
+            }
+        } else {
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u) {
+                const unsigned long long q = beg + u * kWave + lane;
+                c[u] = q < end ? a.col[q] : 0u;
+                w[u] = q < end ? a.w[q] : 0.0f;
+            }
         }
         if (PLASTIC) {
             // lazy pair STDP of the row's plastic synapses before they are placed
@@ -1182,9 +1245,13 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_place(const BucketArg
                 if (ok) {
                     const uint32_t v = r + d;
                     const uint32_t g = tgt >> g_shift;
-                    const uint32_t pos = run_add<true, AGG>(s_cur, (lut_m[v] - 1u) * ng + g);
-                    stage[pos] = make_uint2((tgt - (g << g_shift)) | ((uint32_t)lut_o[v] << 16),
-                                            __float_as_uint(w[u]));
+                    const uint32_t pos = run_add<true, AGG, kPlaceRunHeads>(s_cur, (lut_m[v] - 1u) * ng + g);
+                    if constexpr (REC4)
+                        stage4[pos] = (tgt - (g << g_shift)) | ((uint32_t)lut_o[v] << kRec4SlotShift) |
+                                      (cls << kRec4ClassShift);
+                    else
+                        stage[pos] = make_uint2((tgt - (g << g_shift)) | ((uint32_t)lut_o[v] << 16),
+                                                __float_as_uint(w[u]));
                 } else {
                     // the batch does not fit the event buffer: deliver through the ring
                     uint32_t sl = (uint32_t)((t0 + r) % a.D) + d;
@@ -1211,10 +1278,11 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_place(const BucketArg
 constexpr uint32_t kRefineThreads = 512;
 constexpr uint32_t kRefineChunk = 4096;
 
-template <bool AGG>
+template <bool AGG, bool REC4>
 __global__ __launch_bounds__(kRefineThreads) void k_bucket_refine(const BucketArgs a)
 {
-    __shared__ uint2 s_ev[kRefineChunk];
+    using Rec = typename std::conditional<REC4, uint32_t, uint2>::type;
+    __shared__ Rec s_ev[kRefineChunk];
     __shared__ uint16_t s_rank[kRefineChunk];
     __shared__ uint32_t s_cnt[kMaxGroupTiles];
     __shared__ uint32_t s_base[kMaxGroupTiles];
@@ -1252,8 +1320,23 @@ __global__ __launch_bounds__(kRefineThreads) void k_bucket_refine(const BucketAr
         if (a0 >= a1)
             return;
     }
-    uint2* ev = a.ev + (size_t)buf * a.e_cap;
+    Rec* ev = (Rec*)a.ev + (size_t)buf * a.e_cap;
+    const Rec* stage = (const Rec*)a.stage;
     const uint32_t tmask = (1u << a.tw_shift) - 1u;
+    // the record's target offset within its group, and the record with the
+    // offset within its tile
+    auto offset = [](const Rec& e) -> uint32_t {
+        if constexpr (REC4)
+            return e & kRec4OffsetMask;
+        else
+            return e.x & 0xFFFFu;
+    };
+    auto retile = [&](const Rec& e) -> Rec {
+        if constexpr (REC4)
+            return (e & tmask) | (e & ~kRec4OffsetMask);
+        else
+            return make_uint2((e.x & tmask) | (e.x & 0xFFFF0000u), e.y);
+    };
     for (uint32_t kc = lo_k; kc < ngb && bs[kc << a.gt_shift] < a1; ++kc) {
         const uint32_t k0 = kc << a.gt_shift;
         const uint32_t g0 = max(bs[k0], a0), g1 = min(bs[k0 + gt], a1);
@@ -1263,18 +1346,18 @@ __global__ __launch_bounds__(kRefineThreads) void k_bucket_refine(const BucketAr
                 s_cnt[threadIdx.x] = 0;
             __syncthreads();
             for (uint32_t i = threadIdx.x; i < n; i += kRefineThreads) {
-                const uint2 e = a.stage[r0 + i];
+                const Rec e = stage[r0 + i];
                 s_ev[i] = e;
-                s_rank[i] = (uint16_t)run_add<true, AGG>(s_cnt, (e.x & 0xFFFFu) >> a.tw_shift);
+                s_rank[i] = (uint16_t)run_add<true, AGG>(s_cnt, offset(e) >> a.tw_shift);
             }
             __syncthreads();
             if (threadIdx.x < gt && s_cnt[threadIdx.x])
                 s_base[threadIdx.x] = atomicAdd(&a.gcur[k0 + threadIdx.x], s_cnt[threadIdx.x]);
             __syncthreads();
             for (uint32_t i = threadIdx.x; i < n; i += kRefineThreads) {
-                const uint2 e = s_ev[i];
-                const uint32_t t = (e.x & 0xFFFFu) >> a.tw_shift;
-                ev[s_base[t] + s_rank[i]] = make_uint2((e.x & tmask) | (e.x & 0xFFFF0000u), e.y);
+                const Rec e = s_ev[i];
+                const uint32_t t = offset(e) >> a.tw_shift;
+                ev[s_base[t] + s_rank[i]] = retile(e);
             }
             __syncthreads();
         }
@@ -1365,10 +1448,30 @@ __global__ __launch_bounds__(kBlock) void k_conn_bernoulli(const ConnArgs c, uin
     }
 }
 
-// Canonical row order: ascending (word << 32 | weight bits), bitonic sort of
-// one row per block in LDS.
+// Canonical row order: ascending (row_key(word) << 32 | weight bits), bitonic
+// sort of one row per block in LDS.  row_key orders a row by the target's
+// 2^14-neuron group within its network first, then by the word (plastic bit,
+// delay, target): the bucketed placement keys events by (arrival batch,
+// group), so a wave's 64 consecutive synapses form runs of equal keys (one
+// LDS atomic and one contiguous store run each) instead of 64 scattered ones.
+// net_base: the network's first neuron, so the order is a function of the
+// network alone (not of where a pack or an arena places it).
+__device__ __forceinline__ uint32_t row_key(uint32_t word, uint32_t net_base)
+{
+    const uint32_t local = (word & kTargetMask) - net_base;
+    return ((local >> kGroupNeuronShift) << 22) | ((word >> 31) << 21) | (((word >> 24) & 0x7Fu) << 14) |
+           (local & ((1u << kGroupNeuronShift) - 1u));
+}
+
+__device__ __forceinline__ uint32_t row_word(uint32_t key, uint32_t net_base)
+{
+    const uint32_t local = ((key >> 22) << kGroupNeuronShift) | (key & ((1u << kGroupNeuronShift) - 1u));
+    return (local + net_base) | (((key >> 14) & 0x7Fu) << 24) | (((key >> 21) & 1u) << 31);
+}
+
 __global__ __launch_bounds__(512) void k_sort_rows(const unsigned long long* row_ptr, uint32_t* col,
-                                                   float* w, uint32_t row_begin, uint32_t row_end)
+                                                   float* w, uint32_t row_begin, uint32_t row_end,
+                                                   uint32_t net_base)
 {
     extern __shared__ unsigned long long s_keys[];
     for (uint32_t r = row_begin + blockIdx.x; r < row_end; r += gridDim.x) {
@@ -1380,7 +1483,7 @@ __global__ __launch_bounds__(512) void k_sort_rows(const unsigned long long* row
         while (L < len)
             L <<= 1;
         for (uint32_t i = threadIdx.x; i < L; i += blockDim.x)
-            s_keys[i] = i < len ? ((unsigned long long)col[beg + i] << 32) |
+            s_keys[i] = i < len ? ((unsigned long long)row_key(col[beg + i], net_base) << 32) |
                                       (unsigned long long)__float_as_uint(w[beg + i])
                                 : ~0ull;
         __syncthreads();
@@ -1401,7 +1504,7 @@ __global__ __launch_bounds__(512) void k_sort_rows(const unsigned long long* row
             }
         }
         for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) {
-            col[beg + i] = (uint32_t)(s_keys[i] >> 32);
+            col[beg + i] = row_word((uint32_t)(s_keys[i] >> 32), net_base);
             w[beg + i] = __uint_as_float((uint32_t)s_keys[i]);
         }
         __syncthreads();
@@ -1504,7 +1607,7 @@ static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b -
 // nets stepped from several host threads never lower it under each other.
 static std::mutex g_lds_mu;
 
-template <bool DET, bool PLASTIC, bool BUCKET, bool EXTRA, uint32_t TW>
+template <bool DET, bool PLASTIC, bool BUCKET, bool EXTRA, uint32_t TW, bool REC4>
 static hipError_t launch_update_t(const UpdateArgs& a, hipStream_t s)
 {
     const size_t lds = BUCKET ? (size_t)a.B * TW * sizeof(unsigned long long) : 0;
@@ -1512,41 +1615,49 @@ static hipError_t launch_update_t(const UpdateArgs& a, hipStream_t s)
     if (lds > 32768) {
         std::lock_guard<std::mutex> lock(g_lds_mu);
         if (lds > lds_set) {
-            hipError_t e = hipFuncSetAttribute((const void*)k_update<DET, PLASTIC, BUCKET, EXTRA, TW>,
+            hipError_t e = hipFuncSetAttribute((const void*)k_update<DET, PLASTIC, BUCKET, EXTRA, TW, REC4>,
                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
             if (e != hipSuccess)
                 return e;
             lds_set = lds;
         }
     }
-    hipLaunchKernelGGL((k_update<DET, PLASTIC, BUCKET, EXTRA, TW>), dim3(a.n_seg), dim3(TW), lds, s, a);
+    hipLaunchKernelGGL((k_update<DET, PLASTIC, BUCKET, EXTRA, TW, REC4>), dim3(a.n_seg), dim3(TW), lds, s, a);
     return hipGetLastError();
 }
 
-template <bool DET, bool PLASTIC, bool BUCKET>
+template <bool DET, bool PLASTIC, bool BUCKET, bool REC4>
 static hipError_t launch_update_tw(const UpdateArgs& a, bool extra, hipStream_t s)
 {
     if (a.tw_shift == 10)
-        return extra ? launch_update_t<DET, PLASTIC, BUCKET, true, 1024>(a, s)
-                    : launch_update_t<DET, PLASTIC, BUCKET, false, 1024>(a, s);
+        return extra ? launch_update_t<DET, PLASTIC, BUCKET, true, 1024, REC4>(a, s)
+                    : launch_update_t<DET, PLASTIC, BUCKET, false, 1024, REC4>(a, s);
     if (a.tw_shift == 9)
-        return extra ? launch_update_t<DET, PLASTIC, BUCKET, true, 512>(a, s)
-                    : launch_update_t<DET, PLASTIC, BUCKET, false, 512>(a, s);
-    return extra ? launch_update_t<DET, PLASTIC, BUCKET, true, 256>(a, s)
-                : launch_update_t<DET, PLASTIC, BUCKET, false, 256>(a, s);
+        return extra ? launch_update_t<DET, PLASTIC, BUCKET, true, 512, REC4>(a, s)
+                    : launch_update_t<DET, PLASTIC, BUCKET, false, 512, REC4>(a, s);
+    return extra ? launch_update_t<DET, PLASTIC, BUCKET, true, 256, REC4>(a, s)
+                : launch_update_t<DET, PLASTIC, BUCKET, false, 256, REC4>(a, s);
 }
 
-hipError_t launch_update(const UpdateArgs& a, bool det, bool plastic, bool bucket, bool extra, hipStream_t s)
+hipError_t launch_update(const UpdateArgs& a, bool det, bool plastic, bool bucket, bool extra, bool rec4,
+                         hipStream_t s)
 {
-    if (a.tw_shift < 8 || a.tw_shift > 10)
+    if (a.tw_shift < 8 || a.tw_shift > 10 || (rec4 && (plastic || !bucket)))
         return hipErrorInvalidValue;
+    if (rec4)
+        return det ? launch_update_tw<true, false, true, true>(a, extra, s)
+                   : launch_update_tw<false, false, true, true>(a, extra, s);
     if (plastic && bucket)
-        return det ? launch_update_tw<true, true, true>(a, extra, s) : launch_update_tw<false, true, true>(a, extra, s);
+        return det ? launch_update_tw<true, true, true, false>(a, extra, s)
+                   : launch_update_tw<false, true, true, false>(a, extra, s);
     if (plastic)
-        return det ? launch_update_tw<true, true, false>(a, extra, s) : launch_update_tw<false, true, false>(a, extra, s);
+        return det ? launch_update_tw<true, true, false, false>(a, extra, s)
+                   : launch_update_tw<false, true, false, false>(a, extra, s);
     if (bucket)
-        return det ? launch_update_tw<true, false, true>(a, extra, s) : launch_update_tw<false, false, true>(a, extra, s);
-    return det ? launch_update_tw<true, false, false>(a, extra, s) : launch_update_tw<false, false, false>(a, extra, s);
+        return det ? launch_update_tw<true, false, true, false>(a, extra, s)
+                   : launch_update_tw<false, false, true, false>(a, extra, s);
+    return det ? launch_update_tw<true, false, false, false>(a, extra, s)
+               : launch_update_tw<false, false, false, false>(a, extra, s);
 }
 
 template <bool AGG>
@@ -1583,9 +1694,14 @@ template <bool DET, uint32_t U, bool AGG>
 static void launch_place(const BucketArgs& a, bool plastic, size_t lds_c, hipStream_t s)
 {
     if (plastic)
-        hipLaunchKernelGGL((k_bucket_place<DET, U, AGG, true>), dim3(a.NB), dim3(kBucketThreads), lds_c, s, a);
+        hipLaunchKernelGGL((k_bucket_place<DET, U, AGG, true, false>), dim3(a.NB), dim3(kBucketThreads), lds_c, s,
+                           a);
+    else if (a.ncls)
+        hipLaunchKernelGGL((k_bucket_place<DET, U, AGG, false, true>), dim3(a.NB), dim3(kBucketThreads), lds_c, s,
+                           a);
     else
-        hipLaunchKernelGGL((k_bucket_place<DET, U, AGG, false>), dim3(a.NB), dim3(kBucketThreads), lds_c, s, a);
+        hipLaunchKernelGGL((k_bucket_place<DET, U, AGG, false, false>), dim3(a.NB), dim3(kBucketThreads), lds_c, s,
+                           a);
 }
 
 template <bool AGG>
@@ -1602,19 +1718,28 @@ static void launch_bucket_pass(const BucketArgs& a, int step, bool det, size_t l
             hipLaunchKernelGGL((k_bucket_hist<8, AGG>), grid, block, lds, s, a);
         break;
     case 2: {
+        // placement counters aggregate by their own flag
         const bool pl = a.sd.lat != nullptr;
-        if (det && a.U == 4)
-            launch_place<true, 4, AGG>(a, pl, lds_c, s);
-        else if (det)
-            launch_place<true, 8, AGG>(a, pl, lds_c, s);
-        else if (a.U == 4)
-            launch_place<false, 4, AGG>(a, pl, lds_c, s);
-        else
-            launch_place<false, 8, AGG>(a, pl, lds_c, s);
+        if (a.AGP) {
+            if (det)
+                a.U == 4 ? launch_place<true, 4, true>(a, pl, lds_c, s) : launch_place<true, 8, true>(a, pl, lds_c, s);
+            else
+                a.U == 4 ? launch_place<false, 4, true>(a, pl, lds_c, s) : launch_place<false, 8, true>(a, pl, lds_c, s);
+        } else {
+            if (det)
+                a.U == 4 ? launch_place<true, 4, false>(a, pl, lds_c, s) : launch_place<true, 8, false>(a, pl, lds_c, s);
+            else
+                a.U == 4 ? launch_place<false, 4, false>(a, pl, lds_c, s) : launch_place<false, 8, false>(a, pl, lds_c, s);
+        }
         break;
     }
     default:
-        hipLaunchKernelGGL(k_bucket_refine<AGG>, dim3((a.NBK >> a.gt_shift) * a.RS), dim3(kRefineThreads), 0, s, a);
+        if (a.ncls)
+            hipLaunchKernelGGL((k_bucket_refine<AGG, true>), dim3((a.NBK >> a.gt_shift) * a.RS), dim3(kRefineThreads),
+                               0, s, a);
+        else
+            hipLaunchKernelGGL((k_bucket_refine<AGG, false>), dim3((a.NBK >> a.gt_shift) * a.RS),
+                               dim3(kRefineThreads), 0, s, a);
         break;
     }
 }
@@ -1698,6 +1823,7 @@ hipError_t launch_conn(const ConnArgs& c, uint32_t rule, bool place, uint32_t* d
     return hipGetLastError();
 }
 
+// Rows [row_begin, row_end) are one network's neurons: row_begin is its base.
 hipError_t launch_sort_rows(const unsigned long long* row_ptr, uint32_t* col, float* w, uint32_t row_begin,
                             uint32_t row_end, uint32_t max_row, hipStream_t s)
 {
@@ -1715,7 +1841,7 @@ hipError_t launch_sort_rows(const unsigned long long* row_ptr, uint32_t* col, fl
     }
     const uint32_t rows = row_end - row_begin;
     const dim3 grid(rows < 65536u ? rows : 65536u);
-    hipLaunchKernelGGL(k_sort_rows, grid, dim3(512), lds, s, row_ptr, col, w, row_begin, row_end);
+    hipLaunchKernelGGL(k_sort_rows, grid, dim3(512), lds, s, row_ptr, col, w, row_begin, row_end, row_begin);
     return hipGetLastError();
 }
 

@@ -200,7 +200,26 @@ struct UpdateArgs {
     // neurons of source-defined models (NAER_POP_MODEL): their synaptic input
     // and drive per step, [B][N], read by the model's own kernel
     float2* cin;
+    // class-weight delivery (4-byte records, see kRec4*): the weight of class
+    // c (the source's population) in 32.32 fixed point
+    const unsigned long long* cls_fx;
+    uint32_t n_cls;
 };
+
+// Class-weight delivery.  When every population's outgoing synapses carry
+// one constant weight (all static networks of the benchmark workloads), a
+// bucketed event needs no weight bits: a 4-byte record
+//   target offset (bits 0-13) | arrival slot (14-17) | class (18-31)
+// names the source's population, whose weight the update looks up.  The
+// target offset is within the 2^14-neuron group in the staging buffer and
+// within the tile in the event buffers.  The placement then reads no
+// weights, and every pass moves half the bytes of the 8-byte records
+// {offset | slot << 16, weight bits}.
+constexpr uint32_t kRec4SlotShift = 14;
+constexpr uint32_t kRec4ClassShift = 18;
+constexpr uint32_t kRec4OffsetMask = (1u << kRec4SlotShift) - 1u;
+constexpr uint32_t kRec4MaxClasses = 1u << (32 - kRec4ClassShift);
+constexpr uint32_t kRec4StagedClasses = 64;  // class weights the update keeps in LDS
 
 // Event-driven mode (kernels_event.hip): one 1500 us window per epoch.
 constexpr uint32_t kEvLo = 4096;    // entries of a population's L decay table
@@ -285,8 +304,12 @@ struct BucketArgs {
     uint32_t U;        // synapses per lane per task: 4 or 8
     uint32_t RS;       // refine blocks per group bucket (on average: they split the batch evenly)
     uint32_t balance;  // 1: k_bucket_split sets blk_seg per batch; 0: fixed ranges of G segments
-    uint32_t AG;       // 1: LDS counters take one atomic per run of equal keys in a wave
+    uint32_t AG;       // 1: histogram/refine LDS counters take one atomic per run of equal keys in a wave
+    uint32_t AGP;      // 1: the same for the placement counters
     uint32_t is_last;
+    // class-weight delivery (4-byte records): class and weight of each neuron's population
+    const uint32_t* ncls;   // [N]
+    const float* cls_w;     // [classes]
 };
 
 constexpr uint32_t kMaxSegPerBlock = 64;  // segments a histogram/placement block stages at once

@@ -10,7 +10,8 @@ namespace naer {
 
 // extra: the net may contain POISSON sources or source-defined models
 // (otherwise their code is left out)
-hipError_t launch_update(const UpdateArgs& a, bool det, bool plastic, bool bucket, bool extra, hipStream_t s);
+hipError_t launch_update(const UpdateArgs& a, bool det, bool plastic, bool bucket, bool extra, bool rec4,
+                         hipStream_t s);
 // the bucketed delivery of one aligned batch: steps 0 split + histogram,
 // 1 scans, 2 place, 3 refine
 constexpr int kBucketSteps = 4;

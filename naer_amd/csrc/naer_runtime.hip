@@ -253,9 +253,15 @@ struct naer_net {
     uint32_t RS = 1;             // schedule: refine blocks per group bucket
     uint32_t AG = 1;             // schedule: run-aggregated LDS counters
     bool balance = false;        // histogram/placement blocks split each batch by events
-    uint64_t e_cap = 0;
-    uint2* ev = nullptr;
-    uint2* stage = nullptr;
+    uint64_t e_cap = 0;       // records per event buffer
+    uint32_t rec_bytes = 8;   // 4: class-weight delivery (kRec4* records)
+    uint2* ev = nullptr;      // [R][e_cap] records of rec_bytes
+    uint2* stage = nullptr;   // [e_cap] records of rec_bytes
+    // class-weight delivery: class (population) of every neuron, weight of every class
+    uint32_t* ncls = nullptr;
+    float* cls_w = nullptr;
+    unsigned long long* cls_fx = nullptr;
+    uint32_t cls_cap = 0;
     uint32_t *bs = nullptr, *flags = nullptr, *hist = nullptr, *hist_c = nullptr, *totals = nullptr;
     uint32_t* gcur = nullptr;
 
@@ -562,6 +568,48 @@ int upload_table(naer_net* n)
             active = n->pp[q].end > j;
         n->blk_pop_h[b] = p | (active ? 0u : kTileIdle);
     }
+    std::vector<float> cls_w;
+    std::vector<unsigned long long> cls_fx;
+    std::vector<uint32_t> ncls;
+    if (n->rec_bytes == 4) {
+        // class c = population c of the table: its neurons and its outgoing weight
+        ncls.assign(n->N, 0u);
+        cls_w.assign(P, 0.0f);
+        for (uint32_t i : order) {
+            const NetInfo& net = n->nets[i];
+            uint32_t q = 0;  // index of the net's first population in the table
+            while (q < P && n->pp[q].begin < net.neuron_begin)
+                ++q;
+            for (uint32_t p = 0; p < net.pops.size(); ++p) {
+                if (net.pops[p].size == 0)
+                    continue;
+                for (const auto& c : net.conns)
+                    if (c.src_pop == p)
+                        cls_w[q] = c.weight;
+                for (uint32_t j = n->pp[q].begin; j < n->pp[q].end; ++j)
+                    ncls[j] = q;
+                ++q;
+            }
+        }
+        cls_fx.resize(P);
+        for (uint32_t c = 0; c < P; ++c)
+            cls_fx[c] = (unsigned long long)std::llrint((double)cls_w[c] * 4294967296.0);
+        if (P > n->cls_cap) {
+            if (n->cls_w)
+                n->release(n->cls_w, (size_t)n->cls_cap * sizeof(float));
+            if (n->cls_fx)
+                n->release(n->cls_fx, (size_t)n->cls_cap * sizeof(unsigned long long));
+            n->cls_cap = P;
+            int rc;
+            if ((rc = n->alloc(&n->cls_w, P)) || (rc = n->alloc(&n->cls_fx, P)))
+                return rc;
+        }
+        HIP_TRY(hipMemcpyAsync(n->cls_w, cls_w.data(), P * sizeof(float), hipMemcpyHostToDevice, n->stream));
+        HIP_TRY(hipMemcpyAsync(n->cls_fx, cls_fx.data(), P * sizeof(unsigned long long), hipMemcpyHostToDevice,
+                               n->stream));
+        HIP_TRY(hipMemcpyAsync(n->ncls, ncls.data(), (size_t)n->N * sizeof(uint32_t), hipMemcpyHostToDevice,
+                               n->stream));
+    }
     if (P) {
         HIP_TRY(hipMemcpyAsync(n->d_pops, n->pp.data(), P * sizeof(PopParams), hipMemcpyHostToDevice, n->stream));
         HIP_TRY(hipMemcpyAsync(n->d_cdf, n->cdf.data(), n->cdf.size() * sizeof(float), hipMemcpyHostToDevice,
@@ -668,6 +716,8 @@ UpdateArgs make_update_args(naer_net* n)
     a.NT = n->NT;
     a.NBK = n->NBK;
     a.cin = n->cin;
+    a.cls_fx = n->cls_fx;
+    a.n_cls = n->rec_bytes == 4 ? (uint32_t)n->pp.size() : 0u;
     return a;
 }
 
@@ -894,24 +944,59 @@ int alloc_neurons(naer_net* n, bool any_poisson)
 constexpr double kBucketRateMinHz = 200.0;
 constexpr double kBucketRateMaxHz = 500.0;
 
-// Default of the run-aggregated LDS counters: on when a wave of 64 row
-// synapses is expected to hold runs of >= 4 equal bucket keys, i.e. a row's
-// synapses per (tile, arrival batch) of its own network; weighted by synapses
-// over the networks of a pack.  Arenas (networks unknown at build) default on;
-// the kernels fall back to lane-by-lane atomics on waves of short runs.
+// Default of the run-aggregated LDS counters (naer_schedule.run_aggregate):
+// bit 0 (histogram and refine counters, keyed by tile) is on when a wave of
+// 64 row synapses is expected to hold runs of >= 4 equal keys, i.e. a row's
+// synapses per (tile, arrival batch) of its own network; bit 1 (placement
+// counters, keyed by 2^14-neuron group; rows are ordered by group, then
+// delay) when runs per (group, arrival batch) reach 8 (on the 1M network's
+// runs of ~4 the aggregated placement measured 20 % slower).  Weighted by synapses
+// over the networks of a pack.  Arenas (networks unknown at build) default
+// on; the kernels fall back to lane-by-lane atomics on waves of short runs.
 uint32_t run_aggregation(const naer_net* n)
 {
-    double w = 0.0, sum = 0.0;
+    double wt = 0.0, wg = 0.0, sum = 0.0;
     for (const auto& net : n->nets) {
         if (net.n_neurons == 0 || net.n_syn == 0)
             continue;
         const double tiles = std::ceil((double)net.n_neurons / (double)(1u << n->tw_shift));
+        const double groups = std::ceil((double)net.n_neurons / (double)(1u << kGroupNeuronShift));
         const double batches = (double)(net.dmax - std::min(net.dmin, net.dmax)) / n->B + 1.0;
-        const double run = (double)net.n_syn / net.n_neurons / (tiles * batches);
-        w += (double)net.n_syn * run;
+        const double row = (double)net.n_syn / net.n_neurons;
+        wt += (double)net.n_syn * row / (tiles * batches);
+        wg += (double)net.n_syn * row / (groups * batches);
         sum += (double)net.n_syn;
     }
-    return sum == 0.0 || w / sum >= 4.0 ? 1u : 0u;
+    if (sum == 0.0)
+        return 3u;
+    return (wt / sum >= 4.0 ? 1u : 0u) | (wg / sum >= 8.0 ? 2u : 0u);
+}
+
+// Class-weight delivery (kRec4*): possible when no synapse is plastic,
+// the net is no arena (its jobs are not known at build), every population's
+// outgoing connections share one constant weight, and the populations fit
+// the record's class bits.
+bool class_weights(const naer_net* n)
+{
+    if (n->arena || n->plastic)
+        return false;
+    uint64_t pops = 0;
+    for (const auto& net : n->nets) {
+        pops += net.pops.size();
+        for (uint32_t p = 0; p < net.pops.size(); ++p) {
+            bool have = false;
+            float w = 0.0f;
+            for (const auto& c : net.conns) {
+                if (c.src_pop != p)
+                    continue;
+                if (c.weight == 0.0f || c.plastic || (have && c.weight != w))
+                    return false;  // random, plastic or unequal weights
+                have = true;
+                w = c.weight;
+            }
+        }
+    }
+    return pops < kRec4MaxClasses;
 }
 
 // Segment ranges of the histogram/placement blocks.  Packs and arenas
@@ -967,7 +1052,12 @@ int alloc_buckets(naer_net* n)
     if (n->M > kMaxArrivalBatches || (n->NBK >> n->gt_shift) > 8192)
         return NAER_OK;
     int rc;
-    if ((rc = n->alloc(&n->ev, (size_t)n->R * n->e_cap)) || (rc = n->alloc(&n->stage, n->e_cap)) ||
+    n->rec_bytes = class_weights(n) ? 4u : 8u;
+    // 4-byte records: buffers of uint2 holding twice as many records
+    const size_t rec_pairs = n->rec_bytes == 4 ? (size_t)(n->e_cap + 1) / 2 : (size_t)n->e_cap;
+    if (n->rec_bytes == 4 && (rc = n->alloc(&n->ncls, n->N)))
+        return rc;
+    if ((rc = n->alloc(&n->ev, (size_t)n->R * rec_pairs)) || (rc = n->alloc(&n->stage, rec_pairs)) ||
         (rc = n->alloc(&n->bs, (size_t)n->R * (n->NBK + 1))) || (rc = n->alloc(&n->flags, n->R)) ||
         (rc = n->alloc(&n->hist, (size_t)n->n_seg * n->NBK)) ||  // rows for any G (G = 1: n_seg blocks)
         (rc = n->alloc(&n->hist_c, (size_t)n->n_seg * (n->NBK >> n->gt_shift))) || (rc = n->alloc(&n->totals, n->NBK)) ||
@@ -1573,7 +1663,8 @@ int naer_arena_retire(naer_net* n, uint32_t slot)
 static hipError_t do_update(naer_net* n, void* arg)
 {
     UpdateArgs* a = (UpdateArgs*)arg;
-    return launch_update(*a, n->det, n->plastic, n->bucketed, n->poisson_ok || !n->mpops.empty(), n->stream);
+    return launch_update(*a, n->det, n->plastic, n->bucketed, n->poisson_ok || !n->mpops.empty(),
+                         n->rec_bytes == 4, n->stream);
 }
 
 static hipError_t do_scatter(naer_net* n, void* arg)
@@ -1755,9 +1846,12 @@ static int enqueue_steps(naer_net* n, uint32_t n_steps, bool timed)
     ba.gcur = n->gcur;
     // about 2048 refine blocks over the group buckets
     ba.RS = n->RS;
-    ba.AG = n->AG;
+    ba.AG = n->AG & 1u;
+    ba.AGP = (n->AG >> 1) & 1u;
     ba.balance = n->balance ? 1u : 0u;
     ba.is_last = 1;
+    ba.ncls = n->rec_bytes == 4 ? n->ncls : nullptr;
+    ba.cls_w = n->cls_w;
     FlushArgs fa = flush_args(n, false);
     std::vector<ModelArgs> ma;
     for (const auto& m : n->mpops)
@@ -1872,6 +1966,7 @@ int naer_net_get_schedule(naer_net* n, naer_schedule* out)
     out->segments_per_block = n->G;
     out->refine_split = n->RS;
     out->run_aggregate = n->AG;
+    out->record_bytes = n->bucketed ? n->rec_bytes : 0u;
     out->delivery_us = n->tuned_us;
     return NAER_OK;
 }
@@ -1886,8 +1981,8 @@ int naer_net_set_schedule(naer_net* n, const naer_schedule* s)
         return fail(NAER_ERR_INVALID, "segments_per_block must be in [1, %u]", kMaxSegSchedule);
     if (s->refine_split == 0 || s->refine_split > 64)
         return fail(NAER_ERR_INVALID, "refine_split must be in [1, 64]");
-    if (s->run_aggregate > 1)
-        return fail(NAER_ERR_INVALID, "run_aggregate must be 0 or 1");
+    if (s->run_aggregate > 3)
+        return fail(NAER_ERR_INVALID, "run_aggregate must be in [0, 3]");
     int rc = naer_net_sync(n);
     if (rc || (rc = n->drop_graph()))  // captured launches carry the old shapes
         return rc;
@@ -1940,12 +2035,12 @@ int naer_net_autotune(naer_net* n, uint32_t batches, naer_schedule* chosen)
                 if (!dup)
                     cand.push_back(c);
             }
-    // the base schedule with the other counter mode; the winner takes the
-    // mode that did better on the base
+    // the base schedule with either counter mode flipped; the winner takes
+    // each mode that did better on the base
     const size_t n_main = cand.size();
-    {
+    for (uint32_t bit : { 1u, 2u }) {
         naer_schedule c = base;
-        c.run_aggregate = base.run_aggregate ? 0u : 1u;
+        c.run_aggregate = base.run_aggregate ^ bit;
         cand.push_back(c);
     }
     size_t base_i = 0;
@@ -1985,10 +2080,13 @@ int naer_net_autotune(naer_net* n, uint32_t batches, naer_schedule* chosen)
             bi = i;
     naer_schedule best_s = cand[bi];
     best_s.delivery_us = per_batch[bi];
-    if (score[n_main] < score[base_i]) {
-        best_s.run_aggregate = cand[n_main].run_aggregate;
-        if (bi == base_i)
-            best_s.delivery_us = per_batch[n_main];
+    for (size_t f = 0; f < 2; ++f) {
+        const size_t i = n_main + f;
+        if (score[i] < score[base_i]) {
+            best_s.run_aggregate ^= 1u << f;
+            if (bi == base_i)
+                best_s.delivery_us = std::min(best_s.delivery_us, per_batch[i]);
+        }
     }
     n->profiling = prof;
     n->graph_chunk = graph;
@@ -2267,7 +2365,7 @@ int checkpoint_body(naer_net* n, CheckpointIO& io)
             used[r] = 0;
             if (flags[r])
                 HIP_TRY(hipMemcpy(&used[r], n->bs + (size_t)r * (n->NBK + 1) + n->NBK, 4, hipMemcpyDeviceToHost));
-            if ((rc = io.copy(n->ev + (size_t)r * n->e_cap, (uint64_t)used[r] * sizeof(uint2))))
+            if ((rc = io.copy((char*)n->ev + (size_t)r * n->e_cap * n->rec_bytes, (uint64_t)used[r] * n->rec_bytes)))
                 return rc;
         }
     }

@@ -578,12 +578,29 @@ typedef struct {
     uint64_t* keys;
 } place_ctx;
 
+/* Canonical row order: ascending (row_key(word), weight bits).  row_key puts
+ * the target's 2^14-neuron group first, then the word itself (plastic bit,
+ * delay, target); a bijection of the 32-bit word for targets below 2^24. */
+#define ORC_GROUP_SHIFT 14u
+static inline uint32_t row_key(uint32_t word)
+{
+    uint32_t t = word & 0x00FFFFFFu;
+    return ((t >> ORC_GROUP_SHIFT) << 22) | ((word >> 31) << 21) | (((word >> 24) & 0x7Fu) << 14) |
+           (t & ((1u << ORC_GROUP_SHIFT) - 1u));
+}
+
+static inline uint32_t row_word(uint32_t key)
+{
+    uint32_t t = ((key >> 22) << ORC_GROUP_SHIFT) | (key & ((1u << ORC_GROUP_SHIFT) - 1u));
+    return t | (((key >> 14) & 0x7Fu) << 24) | (((key >> 21) & 1u) << 31);
+}
+
 static void cb_place(void* ctx, uint32_t src, uint32_t word, float w)
 {
     place_ctx* p = (place_ctx*)ctx;
     uint32_t wb;
     memcpy(&wb, &w, 4);
-    p->keys[p->cursor[src]++] = ((uint64_t)word << 32) | wb;
+    p->keys[p->cursor[src]++] = ((uint64_t)row_key(word) << 32) | wb;
 }
 
 static int cmp_u64(const void* a, const void* b)
@@ -624,7 +641,7 @@ int orc_net_build(orc_net* n)
     uint64_t np = 0;
     for (uint64_t i = 0; i < S; ++i) {
         uint32_t wb = (uint32_t)keys[i];
-        n->col[i] = (uint32_t)(keys[i] >> 32);
+        n->col[i] = row_word((uint32_t)(keys[i] >> 32));
         memcpy(&n->w[i], &wb, 4);
         np += n->col[i] >> 31;
     }

@@ -17,17 +17,31 @@ import pytest
 import oracle as O
 from helpers import first_divergence, oracle_of
 from naer_amd import _abi, models
+from naer_amd.network import Network, NeuronType
 
 pytestmark = pytest.mark.gpu
 
 
-def multi_delay_net(record=2_000_000, plastic=False):
+def multi_delay_net(record=2_000_000, plastic=False, random_weights=False):
     """8,000 neurons, in-degree 800, delays 1.5-6 ms on a 0.5 ms grid: B = 15
     steps per batch, events spread over M = 4 later batches.  With pair STDP
     on E->E (plastic=True) the batches are as long (refractory 20 steps) and the
-    placement applies lazy STDP."""
-    net = models.balanced(8000, 800, 0.1, 5.0, 20_000.0, 0.1, delay_ms=(1.5, 6.0, 0.5), plastic=plastic,
-                          mode=_abi.MODE_DETERMINISTIC, seed=0xB0C4E7)
+    placement applies lazy STDP.  Constant weights per source population are
+    delivered as 4-byte class records; random_weights=True (E->E weights
+    uniform in [0.05, 0.15) mV) keeps the 8-byte records with weights."""
+    if random_weights:
+        net = Network(mode=_abi.MODE_DETERMINISTIC, seed=0xB0C4E7)
+        lif = dict(models.BRUNEL_LIF, ext_rate_hz=20_000.0, ext_weight=0.1)
+        E = net.addPopulation(NeuronType.NOP, 6400, **lif)
+        I = net.addPopulation(NeuronType.NOP, 1600, **lif)
+        d = (1.5, 6.0, 0.5)
+        net.addConnectionFixedIndegree(E, E, 640, 0.0, 0.05, 0.15, seed=11, delay_ms=d)
+        net.addConnectionFixedIndegree(E, I, 640, 0.1, seed=12, delay_ms=d)
+        net.addConnectionFixedIndegree(I, E, 160, -0.5, seed=13, delay_ms=d)
+        net.addConnectionFixedIndegree(I, I, 160, -0.5, seed=14, delay_ms=d)
+    else:
+        net = models.balanced(8000, 800, 0.1, 5.0, 20_000.0, 0.1, delay_ms=(1.5, 6.0, 0.5), plastic=plastic,
+                              mode=_abi.MODE_DETERMINISTIC, seed=0xB0C4E7)
     net.record_capacity = record
     return net
 
@@ -48,11 +62,18 @@ def test_batch_shape():
     net.compile()
     s = net.stats()
     assert s.batch == 15 and s.ring_depth == 61
+    assert net.schedule()["record_bytes"] == 4
     net.close()
+    for kw in (dict(random_weights=True), dict(plastic=True)):
+        net = multi_delay_net(**kw)
+        net.compile()
+        assert net.schedule()["record_bytes"] == 8, kw
+        net.close()
 
 
-def test_irregular_run_lengths_bit_exact():
-    net = multi_delay_net()
+@pytest.mark.parametrize("random_weights", [False, True])
+def test_irregular_run_lengths_bit_exact(random_weights):
+    net = multi_delay_net(random_weights=random_weights)
     net.compile()
     o = oracle_of(net, O.FIXED)
     chunks = [1, 14, 15, 7, 30, 23, 45, 3, 12, 100, 50, 15, 15, 29, 1, 140]
@@ -63,13 +84,13 @@ def test_irregular_run_lengths_bit_exact():
     net.close()
 
 
-@pytest.mark.parametrize("cap", [2_000, 150_000])
-def test_event_buffer_overflow_falls_back_to_ring(cap):
+@pytest.mark.parametrize("cap,random_weights", [(2_000, False), (150_000, False), (150_000, True)])
+def test_event_buffer_overflow_falls_back_to_ring(cap, random_weights):
     # a small event buffer: every batch (2_000) or the busier ones (150_000)
     # go through the ring from the placement kernel
     os.environ["NAER_BUCKET_EVENTS"] = str(cap)
     try:
-        net = multi_delay_net()
+        net = multi_delay_net(random_weights=random_weights)
         net.compile()
     finally:
         del os.environ["NAER_BUCKET_EVENTS"]
@@ -138,12 +159,16 @@ def test_autotune_and_schedules_leave_results_unchanged():
     assert steps > 100
     net.set_schedule(256, 1, 3, run_aggregate=1)
     net.run(60)
+    net.set_schedule(512, 2, 2, run_aggregate=2)
+    net.run(40)
+    net.set_schedule(512, 4, 1, run_aggregate=3)
+    net.run(40)
     net.set_schedule(512, 4, 1, run_aggregate=0)
-    net.run(700 - steps - 60)
+    net.run(700 - steps - 140)
     o.run(700)
     assert_same(net, o)
     with pytest.raises(Exception):
         net.set_schedule(128, 1, 1)
     with pytest.raises(Exception):
-        net.set_schedule(256, 1, 1, run_aggregate=2)
+        net.set_schedule(256, 1, 1, run_aggregate=4)
     net.close()

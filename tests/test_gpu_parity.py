@@ -84,6 +84,30 @@ def test_fixed_indegree_connectivity_random_weights_and_delays():
     assert set(np.unique(delays)) == {10, 20, 30, 40, 50}
 
 
+def test_multi_group_network_csr_and_deterministic_step():
+    """50,000 neurons span four 2^14-neuron groups: the canonical row order
+    (group, word, weight) and the bucketed delivery over several groups and
+    arrival batches, bit-exact with the oracle in deterministic mode."""
+    net = models.large_1m(n=50_000, indegree=200, mode=_abi.MODE_DETERMINISTIC, record_capacity=4_000_000)
+    net.compile()
+    rp, col, w = net.csr()
+    o = oracle_of(net, O.FIXED)
+    orp, ocol, ow = o.csr()
+    np.testing.assert_array_equal(rp, orp)
+    np.testing.assert_array_equal(col, ocol)
+    np.testing.assert_array_equal(w.view(np.uint32), ow.view(np.uint32))
+    assert int(np.max(col & 0xFFFFFF)) >= 3 << 14
+    net.run(600)
+    o.run(600)
+    gs, os_ = net.spikes(), o.spikes()
+    assert len(gs) > 1000
+    assert first_divergence(gs, os_) is None
+    np.testing.assert_array_equal(net.state()["v"].view(np.uint32), o.state()["v"].view(np.uint32))
+    assert net.stats().events == o.event_count
+    net.close()
+    o.close()
+
+
 def _run_pair(net, steps, precision, probe=None):
     net.record_capacity = 4_000_000
     net.compile()

@@ -234,3 +234,21 @@ def test_configs_networks_use_the_pinned_rule():
         assert [c.indegree for c in cs] == [int(round(0.8 * k))] * 2 + [k - int(round(0.8 * k))] * 2
         assert np.float32(cs[1].weight) == np.float32(je)
         assert np.float32(cs[2].weight) == np.float32(-5.0 * je)
+
+
+def test_canonical_row_order():
+    """Rows are sorted by (target group within the net, word, weight bits),
+    group = target >> 14: a net of 40,000 neurons spans three groups."""
+    net = models.large_1m(n=40_000, indegree=20)
+    o = O.OracleNet(net.description(), O.FIXED, threads=4)
+    rp, col, w = o.csr()
+    o.close()
+    assert int(np.max(col & 0xFFFFFF)) >= 2 << 14
+    key = (((col & 0xFFFFFF).astype(np.uint64) >> np.uint64(14)) << np.uint64(33)) | \
+        (col.astype(np.uint64) << np.uint64(1))
+    src = np.repeat(np.arange(len(rp) - 1), np.diff(rp).astype(np.int64))
+    same_row = src[1:] == src[:-1]
+    k0, k1 = key[:-1], key[1:]
+    w0, w1 = w[:-1].view(np.uint32), w[1:].view(np.uint32)
+    ordered = (k0 < k1) | ((k0 == k1) & (w0 <= w1))
+    assert np.all(ordered[same_row])
