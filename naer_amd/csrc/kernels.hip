@@ -283,11 +283,13 @@ __global__ __launch_bounds__(TW, (BUCKET && !PLASTIC && !EXTRA && TW == 1024) ? 
         // bookkeeping (as below)
         if (threadIdx.x == 0) {
             DevState* st = a.st;
-            const uint32_t pl = st->batch & 1u;
-            a.seg_count[pl * a.n_seg + blockIdx.x] = 0;
-            a.seg_ev[pl * a.n_seg + blockIdx.x] = 0;
+            const unsigned long long t0 = st->clock;
+            const uint32_t pl = (uint32_t)((t0 / a.Bfull) & 1u);
+            if (!a.defer || t0 % a.Bfull == 0) {
+                a.seg_count[pl * a.n_seg + blockIdx.x] = 0;
+                a.seg_ev[pl * a.n_seg + blockIdx.x] = 0;
+            }
             if (blockIdx.x == 0) {
-                const unsigned long long t0 = st->clock;
                 st->cur = pl;
                 st->t0_cur = t0;
                 if (PLASTIC && t0 + a.B > st->t_flush + (st->window >> 1))
@@ -428,8 +430,12 @@ __global__ __launch_bounds__(TW, (BUCKET && !PLASTIC && !EXTRA && TW == 1024) ? 
     // from went through the ring
     const bool ring_live = !BUCKET || s_ring_live != 0;
 
-    const uint32_t pl = st->batch & 1u;
+    // the list segments of aligned batch A; a launch that continues the
+    // batch (deferred delivery) appends to what the earlier launches wrote
+    const uint32_t pl = (uint32_t)(A & 1u);
     naer_spike* seg = (pl ? a.list1 : a.list0) + (size_t)blockIdx.x * a.seg_cap;
+    const bool append = a.defer && o0 > 0;
+    const uint32_t base = append ? min(a.seg_count[pl * a.n_seg + blockIdx.x], a.seg_cap) : 0u;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         st->cur = pl;
         st->t0_cur = t0;
@@ -438,7 +444,7 @@ __global__ __launch_bounds__(TW, (BUCKET && !PLASTIC && !EXTRA && TW == 1024) ? 
         if (PLASTIC && t0 + a.B > st->t_flush + (st->window >> 1))
             st->t_flush = t0;
     }
-    Stage sg{ s_list, &s_n, seg, a.seg_cap };
+    Stage sg{ s_list, &s_n, seg + base, a.seg_cap - base };
 
     uint32_t my_events = 0;  // <= B * longest row
     // neurons outside every population (free ranges of an arena) stay idle
@@ -655,13 +661,15 @@ __global__ __launch_bounds__(TW, (BUCKET && !PLASTIC && !EXTRA && TW == 1024) ? 
         atomicAdd(&s_events, (unsigned long long)my_events);
     __syncthreads();
     const uint32_t total = s_n;
-    const uint32_t n = total < a.seg_cap ? total : a.seg_cap;
+    const uint32_t room = a.seg_cap - base;
+    const uint32_t n = total < room ? total : room;
     const uint32_t staged = n < kLdsSpikes ? n : kLdsSpikes;
     for (uint32_t i = threadIdx.x; i < staged; i += TW)
-        seg[i] = s_list[i];
+        seg[base + i] = s_list[i];
     if (threadIdx.x == 0) {
-        a.seg_count[pl * a.n_seg + blockIdx.x] = n;
-        a.seg_ev[pl * a.n_seg + blockIdx.x] = s_events;
+        a.seg_count[pl * a.n_seg + blockIdx.x] = base + n;
+        unsigned long long* sev = a.seg_ev + pl * a.n_seg + blockIdx.x;
+        *sev = (append ? *sev : 0ull) + s_events;
         a.spk_part[blockIdx.x] += total;
         a.ev_part[blockIdx.x] += s_events;
         if (total > n)
@@ -674,7 +682,7 @@ __global__ __launch_bounds__(TW, (BUCKET && !PLASTIC && !EXTRA && TW == 1024) ? 
         for (uint32_t i = threadIdx.x; i < n; i += TW) {
             const unsigned long long r = s_base + i;
             if (r < a.rec_cap)
-                a.rec[r] = i < kLdsSpikes ? s_list[i] : seg[i];
+                a.rec[r] = i < kLdsSpikes ? s_list[i] : seg[base + i];
         }
     }
 }
@@ -691,6 +699,14 @@ __device__ __forceinline__ void batch_epilogue(DevState* st, uint32_t B)
     st->batch += 1;
 }
 
+
+// A launch of `b` steps whose spikes stay in the lists until their aligned
+// batch is complete (deferred delivery): only the clock advances.
+__global__ __launch_bounds__(kWave) void k_clock_advance(DevState* st, uint32_t b)
+{
+    if (threadIdx.x == 0)
+        batch_epilogue(st, b);
+}
 
 template <bool DET, bool PLASTIC>
 __global__ __launch_bounds__(kBlock) void k_scatter(const ScatterArgs a)
@@ -1005,7 +1021,7 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_hist(const BucketArgs
     __shared__ RowStage rs;
     const DevState* st = a.st;
     const uint32_t cur = st->cur;
-    const unsigned long long t0 = st->t0_cur;
+    const unsigned long long t0 = st->t0_cur - st->t0_cur % a.Bfull;  // the aligned batch (deferred launches)
     for (uint32_t i = threadIdx.x; i < a.NBK; i += blockDim.x)
         s_hist[i] = 0;
     bucket_setup(a, lut_m, lut_o);
@@ -1161,7 +1177,7 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_place(const BucketArg
     __shared__ Stage rs;
     DevState* st = a.st;
     const uint32_t cur = st->cur;
-    const unsigned long long t0 = st->t0_cur;
+    const unsigned long long t0 = st->t0_cur - st->t0_cur % a.Bfull;  // the aligned batch (deferred launches)
     const uint32_t buf = (uint32_t)((t0 / a.Bfull) % a.R);
     const bool ok = a.flags[buf] != 0;
     const uint32_t nbc = a.NBK >> a.gt_shift;
@@ -1795,6 +1811,12 @@ hipError_t launch_scatter(const ScatterArgs& a, bool det, bool plastic, uint32_t
         else
             hipLaunchKernelGGL((k_scatter<false, false>), grid, dim3(kBlock), 0, s, a);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_clock_advance(DevState* st, uint32_t b, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_clock_advance, dim3(1), dim3(kWave), 0, s, st, b);
     return hipGetLastError();
 }
 

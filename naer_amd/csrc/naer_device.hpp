@@ -118,7 +118,7 @@ struct DevState {
     unsigned long long clock;   // first step of the next batch
     unsigned long long t0_cur;  // first step of the batch in flight (set by the update)
     unsigned long long rec_n;   // spikes offered to the record buffer since the last drain
-    uint32_t batch;             // batches done; the update writes list[batch & 1]
+    uint32_t batch;             // launches done (the update writes the lists of aligned batch A to list[A & 1])
     uint32_t cur;               // list of the batch in flight
     uint32_t overflow;          // spikes lost to a full list segment
     uint32_t window;            // plastic nets: steps within which no neuron fires kSpikeHist times
@@ -190,6 +190,8 @@ struct UpdateArgs {
     uint32_t n_pops;
     uint32_t tw_shift;   // update block = one tile of 1 << tw_shift neurons
     uint32_t Bfull;      // steps of an aligned batch
+    uint32_t defer;      // 1: launches within an aligned batch append to its spike lists, which are
+                         // delivered once the batch is complete (bucketed static nets)
     // bucketed delivery (see BucketArgs): events of earlier batches sorted by
     // (arrival batch offset m, target tile)
     const uint2* ev;          // [R][e_cap]

@@ -1660,6 +1660,14 @@ int naer_arena_retire(naer_net* n, uint32_t slot)
     return upload_table(n);
 }
 
+// Deferred delivery (enqueue_steps): bucketed static nets other than arenas
+// (an arena admits and retires jobs between any two steps, and plastic
+// synapses take their STDP when their spikes are delivered).
+static bool deferred_delivery(const naer_net* n)
+{
+    return n->bucketed && !n->plastic && !n->arena;
+}
+
 static hipError_t do_update(naer_net* n, void* arg)
 {
     UpdateArgs* a = (UpdateArgs*)arg;
@@ -1782,12 +1790,22 @@ static hipError_t do_bucket_step(naer_net* n, void* arg)
     return launch_bucket_step(*b->a, b->step, n->det, n->stream);
 }
 
+static hipError_t do_clock_advance(naer_net* n, void* arg)
+{
+    return launch_clock_advance(n->st, *(const uint32_t*)arg, n->stream);
+}
+
 // Enqueues steps batch by batch.  Batches are aligned to multiples of B
-// steps; a full aligned batch of a bucketed net delivers its spikes through
-// the buckets, any other batch (partial, plastic) through the ring.
+// steps.  A bucketed static net (no arena) defers the delivery of a launch
+// that ends inside its aligned batch: its spikes stay in the batch's lists,
+// and the launch that completes the batch delivers all of them through the
+// buckets, so any run length costs one bucketed delivery per aligned batch.
+// Other nets deliver every launch: full aligned batches of bucketed nets
+// through the buckets, partial ones through the ring.
 static int enqueue_steps(naer_net* n, uint32_t n_steps, bool timed)
 {
     UpdateArgs ua = make_update_args(n);
+    ua.defer = deferred_delivery(n) ? 1u : 0u;
     ScatterArgs sa;
     std::memset(&sa, 0, sizeof sa);
     sa.list0 = n->list0;
@@ -1876,7 +1894,8 @@ static int enqueue_steps(naer_net* n, uint32_t n_steps, bool timed)
             if ((rc = launch_timed(n, NAER_K_MODEL, do_model, &ms, timed)))
                 return rc;
         }
-        if (n->bucketed && off == 0 && b == n->B) {
+        const bool completes = off + b == n->B;
+        if (n->bucketed && completes && (ua.defer || off == 0)) {
             // timed per part; the parts add up to NAER_K_SCATTER
             for (int k = 0; k < kBucketSteps; ++k) {
                 BucketStep bst{ &ba, k };
@@ -1884,6 +1903,11 @@ static int enqueue_steps(naer_net* n, uint32_t n_steps, bool timed)
                     return rc;
             }
             n->kernel_calls[NAER_K_SCATTER] += 1;
+        } else if (ua.defer) {
+            uint32_t steps = b;
+            if ((rc = launch_timed(n, NAER_K_SCATTER, do_clock_advance, &steps, false)))
+                return rc;
+            n->kernel_calls[NAER_K_SCATTER] -= 1;  // not a delivery
         } else if ((rc = launch_timed(n, NAER_K_SCATTER, do_scatter, &sa, timed))) {
             return rc;
         }
@@ -2272,14 +2296,46 @@ int naer_net_raster_close(naer_net* n, uint64_t* n_total)
 }
 
 // --- checkpoint / resume ---------------------------------------------------
+extern "C++" {
 namespace {
 
-constexpr char kCheckpointMagic[8] = { 'N', 'A', 'E', 'R', 'C', 'K', 'P', '3' };
+constexpr char kCheckpointMagic[8] = { 'N', 'A', 'E', 'R', 'C', 'K', 'P', '4' };
+
+// FNV-1a over the description of every network (populations, connections,
+// STDP parameters, step, mode, seeds): a checkpoint loads only into a net
+// built from the same description.
+uint64_t description_hash(const naer_net* n)
+{
+    uint64_t h = 0xcbf29ce484222325ull;
+    auto mix = [&](const void* p, size_t bytes) {
+        const unsigned char* c = (const unsigned char*)p;
+        for (size_t i = 0; i < bytes; ++i)
+            h = (h ^ c[i]) * 0x100000001b3ull;
+    };
+    for (const auto& net : n->nets) {
+        const uint32_t np = (uint32_t)net.pops.size(), nc = (uint32_t)net.conns.size();
+        mix(&np, 4);
+        for (naer_pop_desc q : net.pops) {
+            q.model = nullptr;  // an address, not part of the description
+            mix(&q, sizeof q);
+        }
+        mix(&nc, 4);
+        for (const auto& c : net.conns)
+            mix(&c, sizeof c);
+        mix(&net.desc.stdp, sizeof net.desc.stdp);
+        mix(&net.desc.dt_ms, sizeof net.desc.dt_ms);
+        mix(&net.desc.mode, sizeof net.desc.mode);
+        mix(&net.desc.seed, sizeof net.desc.seed);
+        mix(&net.desc.poisson_seed, sizeof net.desc.poisson_seed);
+        mix(&net.neuron_begin, sizeof net.neuron_begin);
+    }
+    return h;
+}
 
 // Structure of a net that a checkpoint must match to be loaded.
 struct CheckpointShape {
-    uint64_t N, S_cap, D, B, R, NBK, e_cap, n_seg, rec_cap;
-    uint32_t det, plastic, poisson, bucketed;
+    uint64_t N, S_cap, D, B, R, NBK, e_cap, n_seg, rec_cap, seg_cap, desc_hash;
+    uint32_t det, plastic, poisson, bucketed, rec_bytes, defer;
 };
 
 CheckpointShape shape_of(const naer_net* n)
@@ -2295,10 +2351,14 @@ CheckpointShape shape_of(const naer_net* n)
     c.e_cap = n->e_cap;
     c.n_seg = n->n_seg;
     c.rec_cap = n->rec_cap;
+    c.seg_cap = n->seg_cap;
+    c.desc_hash = description_hash(n);
     c.det = n->det;
     c.plastic = n->plastic;
     c.poisson = n->poisson_ok;
     c.bucketed = n->bucketed;
+    c.rec_bytes = n->rec_bytes;
+    c.defer = deferred_delivery(n) ? 1u : 0u;
     return c;
 }
 
@@ -2306,9 +2366,20 @@ CheckpointShape shape_of(const naer_net* n)
 struct CheckpointIO {
     FILE* f;
     bool writing;
+    bool dry = false;  // loading: read and validate the file, copy nothing to the device
     std::vector<char> buf = std::vector<char>(64u << 20);
     int copy(void* dev, uint64_t bytes)
     {
+        if (dry) {
+            const long at = std::ftell(f);
+            if (std::fseek(f, 0, SEEK_END) != 0)
+                return fail(NAER_ERR_INVALID, "checkpoint unreadable");
+            const long size = std::ftell(f);
+            if (at < 0 || size < 0 || (uint64_t)(size - at) < bytes ||
+                std::fseek(f, at + (long)bytes, SEEK_SET) != 0)
+                return fail(NAER_ERR_INVALID, "checkpoint truncated");
+            return NAER_OK;
+        }
         for (uint64_t off = 0; off < bytes;) {
             const size_t k = (size_t)std::min<uint64_t>(buf.size(), bytes - off);
             if (writing) {
@@ -2329,13 +2400,58 @@ struct CheckpointIO {
         const size_t k = writing ? std::fwrite(host, 1, bytes, f) : std::fread(host, 1, bytes, f);
         return k == bytes ? NAER_OK : fail(NAER_ERR_INVALID, "checkpoint %s failed", writing ? "write" : "read");
     }
+    // A section the kernels index with: saved from the device; on load read
+    // into host memory, checked by `ok` (which sets the error), and only
+    // then copied to the device.
+    template <typename T, typename Ok>
+    int checked(T* dev, size_t count, std::vector<T>* host, Ok ok)
+    {
+        host->resize(count);
+        if (writing) {
+            HIP_TRY(hipMemcpy(host->data(), dev, count * sizeof(T), hipMemcpyDeviceToHost));
+            return scalar(host->data(), count * sizeof(T));
+        }
+        int rc = scalar(host->data(), count * sizeof(T));
+        if (rc || (rc = ok(*host)) || dry)
+            return rc;
+        HIP_TRY(hipMemcpy(dev, host->data(), count * sizeof(T), hipMemcpyHostToDevice));
+        return NAER_OK;
+    }
 };
 
-// The dynamic state in a fixed order; used for both directions.
+int corrupt(const char* what) { return fail(NAER_ERR_INVALID, "corrupt checkpoint: %s", what); }
+
+// The dynamic state in a fixed order; used for both directions.  Every value
+// the kernels use as an index or a count is validated before it reaches the
+// device.
 int checkpoint_body(naer_net* n, CheckpointIO& io)
 {
     const uint64_t N = n->N;
     int rc;
+    uint64_t steps = n->steps_enqueued, dropped = n->dropped;
+    if ((rc = io.scalar(&steps, 8)) || (rc = io.scalar(&dropped, 8)))
+        return rc;
+    // device clock and bookkeeping
+    DevState mine;
+    HIP_TRY(hipMemcpy(&mine, n->st, sizeof mine, hipMemcpyDeviceToHost));
+    std::vector<DevState> st;
+    rc = io.checked(n->st, 1, &st, [&](const std::vector<DevState>& v) {
+        const DevState& s = v[0];
+        const uint64_t B = std::max<uint32_t>(n->B, 1u);
+        if (s.clock != steps)
+            return corrupt("device clock differs from the step count");
+        if (s.clock > 0 && (s.t0_cur >= s.clock || s.cur != (uint32_t)((s.t0_cur / B) & 1u)))
+            return corrupt("batch in flight");
+        if (s.window != mine.window || s.t_flush > s.clock)
+            return corrupt("flush window");
+        return NAER_OK;
+    });
+    if (rc)
+        return rc;
+    if (!io.dry) {
+        n->steps_enqueued = steps;
+        n->dropped = dropped;
+    }
     if ((rc = io.copy(n->V, N * 4)) || (rc = io.copy(n->ref, N * 4)) || (rc = io.copy(n->last, N * 4)))
         return rc;
     if (n->plastic && ((rc = io.copy(n->lat, N * sizeof(uint4))) || (rc = io.copy(n->pst, 2 * N * sizeof(uint4))) ||
@@ -2345,34 +2461,71 @@ int checkpoint_body(naer_net* n, CheckpointIO& io)
         return rc;
     if (n->pnext && ((rc = io.copy(n->pnext, N * 4)) || (rc = io.copy(n->pcnt, N * 4))))
         return rc;
-    if ((rc = io.copy(n->ring, (uint64_t)n->D * N * (n->det ? 8 : 4))) || (rc = io.copy(n->st, sizeof(DevState))) ||
+    if ((rc = io.copy(n->ring, (uint64_t)n->D * N * (n->det ? 8 : 4))) ||
         (rc = io.copy(n->spk_part, (uint64_t)n->n_seg * 8)) || (rc = io.copy(n->ev_part, (uint64_t)n->n_seg * 8)))
         return rc;
-    if ((rc = io.scalar(&n->steps_enqueued, 8)) || (rc = io.scalar(&n->dropped, 8)))
-        return rc;
     // undrained recorded spikes
-    DevState s;
-    HIP_TRY(hipMemcpy(&s, n->st, sizeof s, hipMemcpyDeviceToHost));
-    if ((rc = io.copy(n->rec, std::min<uint64_t>(s.rec_n, n->rec_cap) * sizeof(naer_spike))))
+    if ((rc = io.copy(n->rec, std::min<uint64_t>(st[0].rec_n, n->rec_cap) * sizeof(naer_spike))))
         return rc;
     if (n->bucketed) {
         // flags and starts of every buffer, then the events of the batches in flight
-        if ((rc = io.copy(n->flags, (uint64_t)n->R * 4)) || (rc = io.copy(n->bs, (uint64_t)n->R * (n->NBK + 1) * 4)))
+        std::vector<uint32_t> flags, bs;
+        rc = io.checked(n->flags, n->R, &flags, [&](const std::vector<uint32_t>& v) {
+            for (uint32_t f : v)
+                if (f > 1)
+                    return corrupt("bucket flags");
+            return NAER_OK;
+        });
+        if (rc)
             return rc;
-        std::vector<uint32_t> flags(n->R), used(n->R);
-        HIP_TRY(hipMemcpy(flags.data(), n->flags, n->R * 4, hipMemcpyDeviceToHost));
+        const size_t row = (size_t)n->NBK + 1;
+        rc = io.checked(n->bs, n->R * row, &bs, [&](const std::vector<uint32_t>& v) {
+            for (uint32_t r = 0; r < n->R; ++r) {
+                if (!flags[r])
+                    continue;
+                const uint32_t* b = v.data() + r * row;
+                if (b[0] != 0 || b[n->NBK] > n->e_cap)
+                    return corrupt("bucket starts");
+                for (uint32_t k = 0; k < n->NBK; ++k)
+                    if (b[k] > b[k + 1])
+                        return corrupt("bucket starts");
+            }
+            return NAER_OK;
+        });
+        if (rc)
+            return rc;
         for (uint32_t r = 0; r < n->R; ++r) {
-            used[r] = 0;
-            if (flags[r])
-                HIP_TRY(hipMemcpy(&used[r], n->bs + (size_t)r * (n->NBK + 1) + n->NBK, 4, hipMemcpyDeviceToHost));
-            if ((rc = io.copy((char*)n->ev + (size_t)r * n->e_cap * n->rec_bytes, (uint64_t)used[r] * n->rec_bytes)))
+            const uint64_t used = flags[r] ? bs[r * row + n->NBK] : 0;
+            if ((rc = io.copy((char*)n->ev + (size_t)r * n->e_cap * n->rec_bytes, used * n->rec_bytes)))
                 return rc;
+        }
+    }
+    if (deferred_delivery(n)) {
+        // the spikes of an aligned batch that is not complete wait in its lists
+        std::vector<uint32_t> cnt;
+        std::vector<unsigned long long> sev;
+        rc = io.checked(n->seg_count, 2 * (size_t)n->n_seg, &cnt, [&](const std::vector<uint32_t>& v) {
+            for (uint32_t c : v)
+                if (c > n->seg_cap)
+                    return corrupt("spike list counts");
+            return NAER_OK;
+        });
+        if (rc || (rc = io.checked(n->seg_ev, 2 * (size_t)n->n_seg, &sev,
+                                   [](const std::vector<unsigned long long>&) { return NAER_OK; })))
+            return rc;
+        if (steps % n->B != 0) {
+            const uint32_t pl = (uint32_t)((steps / n->B) & 1u);
+            naer_spike* list = pl ? n->list1 : n->list0;
+            for (uint32_t g = 0; g < n->n_seg; ++g)
+                if ((rc = io.copy(list + (size_t)g * n->seg_cap, (uint64_t)cnt[pl * n->n_seg + g] * sizeof(naer_spike))))
+                    return rc;
         }
     }
     return NAER_OK;
 }
 
 }  // namespace
+}  // extern "C++"
 
 int naer_net_save(naer_net* n, const char* path)
 {
@@ -2428,8 +2581,19 @@ int naer_net_load(naer_net* n, const char* path)
         rc = fail(NAER_ERR_INVALID, "%s is not a checkpoint", path);
     if (!rc)
         rc = io.scalar(&c, sizeof c);
+    if (!rc && c.desc_hash != mine.desc_hash)
+        rc = fail(NAER_ERR_INVALID, "the checkpoint was written by a net of another description");
     if (!rc && std::memcmp(&c, &mine, sizeof c) != 0)
         rc = fail(NAER_ERR_INVALID, "the checkpoint was written by a net of another shape");
+    // a dry pass validates the whole file before anything reaches the device
+    const long body = rc ? -1 : std::ftell(f);
+    if (!rc) {
+        io.dry = true;
+        rc = checkpoint_body(n, io);
+        io.dry = false;
+    }
+    if (!rc && (body < 0 || std::fseek(f, body, SEEK_SET) != 0))
+        rc = fail(NAER_ERR_INVALID, "cannot reread %s", path);
     if (!rc && (rc = n->drop_graph()) == NAER_OK)
         rc = checkpoint_body(n, io);
     std::fclose(f);

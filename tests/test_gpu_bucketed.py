@@ -81,6 +81,11 @@ def test_irregular_run_lengths_bit_exact(random_weights):
         net.run(c)
     o.run(sum(chunks))
     assert_same(net, o)
+    # partial launches defer their spikes to the launch that completes the
+    # aligned batch: one bucketed delivery per batch, none through the ring
+    s = net.stats()
+    assert s.kernel_calls[_abi.K_BUCKET_PLACE] == sum(chunks) // 15
+    assert s.kernel_calls[_abi.K_SCATTER] == sum(chunks) // 15
     net.close()
 
 

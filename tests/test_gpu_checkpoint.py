@@ -12,10 +12,10 @@ from test_gpu_bucketed import multi_delay_net
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("plastic", [False, True])
-def test_resume_continues_bit_for_bit(tmp_path, plastic):
+@pytest.mark.parametrize("plastic,random_weights", [(False, False), (True, False), (False, True)])
+def test_resume_continues_bit_for_bit(tmp_path, plastic, random_weights):
     path = tmp_path / "net.ckpt"
-    a = multi_delay_net(plastic=plastic)
+    a = multi_delay_net(plastic=plastic, random_weights=random_weights)
     a.compile()
     a.run(157)  # not a multiple of the 15-step batch
     a.save(path)
@@ -24,7 +24,7 @@ def test_resume_continues_bit_for_bit(tmp_path, plastic):
     sa = a.spikes()
     st_a = a.state()
     w_a = a.csr()[2] if plastic else None
-    b = multi_delay_net(plastic=plastic)
+    b = multi_delay_net(plastic=plastic, random_weights=random_weights)
     b.compile()
     b.load(path)
     first = b.spikes()  # the undrained spikes of the first 157 steps
@@ -55,4 +55,56 @@ def test_load_rejects_another_shape(tmp_path):
     b.compile()
     with pytest.raises(NaerError):
         b.load(path)
+    b.close()
+
+
+def test_load_rejects_another_description(tmp_path):
+    """Same shape, another seed: the description hash in the header differs."""
+    path = tmp_path / "net.ckpt"
+    a = multi_delay_net()
+    a.compile()
+    a.run(30)
+    a.save(path)
+    a.close()
+    b = multi_delay_net()
+    b.seed += 1
+    b.compile()
+    with pytest.raises(NaerError, match="description"):
+        b.load(path)
+    b.close()
+
+
+# byte offset of the saved device clock: magic (8), shape (11 x 8 + 6 x 4),
+# step count (8), dropped spikes (8)
+CLOCK_OFFSET = 8 + 112 + 16
+
+
+@pytest.mark.parametrize("field,delta", [(0, 1), (8, 10**6), (28, 1)])
+def test_load_rejects_corrupt_state_before_touching_the_device(tmp_path, field, delta):
+    """Corrupt clock, batch start or list parity: the load fails on the dry
+    pass, and the net keeps its own state."""
+    path = tmp_path / "net.ckpt"
+    a = multi_delay_net()
+    a.compile()
+    a.run(157)
+    a.save(path)
+    a.close()
+    raw = bytearray(path.read_bytes())
+    at = CLOCK_OFFSET + field
+    width = 8 if field < 24 else 4
+    v = int.from_bytes(raw[at:at + width], "little") + delta
+    raw[at:at + width] = v.to_bytes(width, "little")
+    bad = tmp_path / "bad.ckpt"
+    bad.write_bytes(bytes(raw))
+    b = multi_delay_net()
+    b.compile()
+    b.run(20)
+    before = b.state()
+    with pytest.raises(NaerError, match="corrupt"):
+        b.load(bad)
+    after = b.state()
+    np.testing.assert_array_equal(before["v"].view(np.uint32), after["v"].view(np.uint32))
+    assert b.stats().step == 20
+    b.load(path)  # the intact file still loads
+    assert b.stats().step == 157
     b.close()
