@@ -6,25 +6,38 @@ Workload (default): the 1M-neuron balanced LIF network (800k E / 200k I, in-degr
 2e9 synapses, delays uniform over whole ms in 1-8 ms, J_E = 0.05 mV, g = 5,
 Poisson drive for ~5 Hz), fp32 state and weights, no plasticity.  The
 connectivity is built on the device before the timed region; inputs are
-resident in HBM.  A "step" is one dt = 0.1 ms update of all neurons plus the
-scatter of their spikes; K steps are timed (default 10,000 = 1 s biological).
+resident in HBM.
+
+A "step" is one aligned batch of B steps of dt = 0.1 ms (B = 9 on the 1M
+network: 0.9 ms of biological time): one update launch over all neurons for
+the B steps and the delivery of the batch's spikes.  W batches of warm-up
+(with --autotune the delivery schedule is tuned on real steps inside it) are followed by
+exactly K timed batches, so the timed region starts and ends on batch
+boundaries and holds K updates and K deliveries.
 
 Prints one JSON line: synaptic events/s (value), realtime factor, the roofline
 of the spike delivery (the kernels that carry a batch's spikes to their
-targets) measured with HIP events on the net's stream, per-kernel times, and
-the CPU oracle timed on a bounded sample of the same network on the host cores.
+targets) and of the whole batch, measured with HIP events on the net's stream
+over whole batches, per-kernel times, a reduced batched sweep (packed against
+one network per launch sequence, rasters compared), the served stream at one
+arrival rate (p50/p99 latency), and the CPU oracle timed on a bounded sample
+of the same network on all host cores.
 
     python bench.py --copy-probe   # a plain device copy of known size, the
                                    # calibration run for rocprofv3 --pmc counters
     python bench.py --workload plastic_100k   # the plasticity network: N = 1e5,
                                    # in-degree 1000 (1e8 synapses), pair STDP on E->E
     python bench.py --workload balanced_10k   # the small network: N = 1e4, in-degree 1000
+    python bench.py --workload brunel_event   # the reference's event-driven step; a step
+                                   # is one 1500 us window
 """
 import argparse
 import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -79,17 +92,17 @@ EVENT_WINDOW_US = 1500
 
 
 def bench_event(args):
-    """brunel in the event-driven mode: --steps steps of 0.1 ms become whole
-    1500 us windows; the oracle's event-driven restatement on brunel at 1e6
-    synapses is the CPU baseline."""
+    """brunel in the event-driven mode: a step is one 1500 us window; the
+    oracle's event-driven restatement on brunel at 1e6 synapses is the CPU
+    baseline."""
     from naer_amd import _abi, models
     synapses = args.n or 100_000_000
     t = time.perf_counter()
     net = models.brunel(synapses, mode=_abi.MODE_EVENT)
     net.compile()
     build_s = time.perf_counter() - t
-    windows = max(1, round(args.steps * 100 / EVENT_WINDOW_US))
-    net.run(max(1, round(args.warmup * 100 / EVENT_WINDOW_US)))
+    windows = max(1, args.steps)
+    net.run(max(1, args.warmup))
     net.finish()
     s0 = net.stats()
     t0 = time.perf_counter()
@@ -117,10 +130,11 @@ def bench_event(args):
     out = {
         "metric": "synaptic events/s & realtime factor, brunel_event",
         "value": round(events / wall, 1), "unit": "synaptic events/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / (windows * EVENT_WINDOW_US / 100), 5),
+        "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / windows, 5),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 (f64 decay products)",
         "data": "synthetic (device-built Bernoulli connectivity and the reference's Poisson trains)",
         "config": {"workload": f"brunel_event: {WORKLOADS['brunel_event']}, {windows} windows = {bio_s:g} s",
+                   "step": "one 1500 us window (the reference's Network::step)",
                    "n_neurons": net.getNeurons(), "n_synapses": int(net.getSynapses()), "mode": "event"},
         "realtime_factor": round(bio_s / wall, 3),
         "rate_hz": round((s1.spikes - s0.spikes) / net.getNeurons() / bio_s, 3),
@@ -143,48 +157,112 @@ def make_workload(name, n=None, indegree=None, **kw):
     return models.balanced_10k(**size, **kw)
 
 
+def host_cores():
+    """Cores this process may run on (the box's share), and the CPU model."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return n, model
+
+
 def cpu_baseline(args):
-    """The oracle (clean-room C restatement, OpenMP) on a bounded sample: the
-    same network parameters at N = args.cpu_n, args.cpu_steps steps."""
+    """The oracle (clean-room C restatement, OpenMP on every host core) on a
+    bounded sample: the same network parameters at N = args.cpu_n, stepped in
+    chunks of 1000 steps until args.cpu_seconds have passed."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
-    threads = min(16, os.cpu_count() or 1)
+    threads, model = host_cores()
     net = make_workload(args.workload, n=min(args.cpu_n, args.n or WORKLOAD_N[args.workload]),
                         indegree=args.indegree if args.workload == "large_1m" else None)
     o = O.OracleNet(net.description(), O.FIXED, threads=threads)
     o.run(200)  # leave the initial transient
     e0 = o.event_count
     t = time.perf_counter()
-    o.run(args.cpu_steps)
+    steps = 0
+    while steps < 100_000 and (time.perf_counter() - t < args.cpu_seconds or steps == 0):
+        o.run(1000)
+        steps += 1000
     dt = time.perf_counter() - t
     ev = o.event_count - e0
     o.close()
     return {"value": ev / dt, "unit": "synaptic events/s", "cores": threads, "kind": "port",
-            "sample": f"oracle (fp32 + fixed-point ring, OpenMP) on the same network at "
-                      f"N={net.getNeurons()}, {args.cpu_steps} steps after 200 warm-up steps, {dt:.1f} s",
-            "rtf": args.cpu_steps * 1e-4 / dt}
+            "cpu": model,
+            "sample": f"oracle (fp32 + fixed-point ring, OpenMP, {threads} threads) on the same network at "
+                      f"N={net.getNeurons()}, {steps} steps after 200 warm-up steps, {dt:.1f} s",
+            "rtf": steps * 1e-4 / dt}
+
+
+def sweep_sample(args):
+    """The batched sweep reduced to args.sweep_nets networks x args.sweep_steps
+    steps in deterministic mode: packed (graph-captured chunks, one stream
+    per pack) against the plain one-network-per-launch-sequence loop, jobs/s
+    of each (build excluded) and the count of bit-identical rasters."""
+    from naer_amd import sweep
+    specs = sweep.sweep_specs(args.sweep_nets)
+    packed = sweep.run_packed(specs, args.sweep_steps)
+    seq = sweep.run_sequential(specs, args.sweep_steps)
+    ident = sum(int(np.array_equal(packed["rasters"][s["index"]], seq["rasters"][s["index"]])) for s in specs)
+    return {"nets": len(specs), "steps": args.sweep_steps, "mode": "det",
+            "packed_jobs_per_s": round(len(specs) / packed["run_s"], 2),
+            "sequential_jobs_per_s": round(len(specs) / seq["run_s"], 2),
+            "speedup": round(seq["run_s"] / packed["run_s"], 3), "rasters_identical": ident,
+            "sample": f"configs' sweep networks (N log-uniform 1k-50k, K = 0.1 N, half with STDP), reduced to "
+                      f"{len(specs)} networks x {args.sweep_steps} steps; the full sweep is "
+                      f"python -m naer_amd.sweep --nets 256 --steps 5000"}
+
+
+def serving_sample(args):
+    """The served stream at one arrival rate: args.serve_jobs jobs at
+    args.serve_rate jobs/s through the continuous-batching server (200 GB
+    budget, deterministic mode)."""
+    from naer_amd import serving
+    srv = serving.Server(200e9)
+    try:
+        r = srv.serve(serving.served_jobs(args.serve_jobs, args.serve_rate))
+    finally:
+        srv.close()
+    return {"rate_jobs_per_s": args.serve_rate, "jobs": args.serve_jobs, "completed": r["completed"],
+            "jobs_per_s": round(r["jobs_per_s"], 3), "p50_s": round(r["p50_s"], 3), "p99_s": round(r["p99_s"], 3),
+            "makespan_s": round(r["makespan_s"], 2), "rejected": len(r["rejected"])}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10_000)
-    ap.add_argument("--warmup", type=int, default=1_000)
+    ap.add_argument("--steps", type=int, default=1111, help="timed batches (1111 x 0.9 ms = 1 s on the 1M network)")
+    ap.add_argument("--warmup", type=int, default=120, help="warm-up batches (with --autotune, the autotuner runs inside)")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="large_1m")
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--indegree", type=int, default=None)
     ap.add_argument("--mode", choices=["fast", "det"], default="fast")
     ap.add_argument("--cpu-n", type=int, default=50_000)
-    ap.add_argument("--cpu-steps", type=int, default=30_000)  # ~10-15 s of oracle work
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=200, help="batches of the profiled stretch (>= 20)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="let the autotuner pick the delivery schedule during the warm-up (default: the "
+                         "library's schedule for the network)")
+    ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--sweep-nets", type=int, default=256)
+    ap.add_argument("--sweep-steps", type=int, default=600)
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=32)
+    ap.add_argument("--serve-rate", type=float, default=10.0)
     ap.add_argument("--copy-probe", action="store_true")
-    ap.add_argument("--profile-steps", type=int, default=2_000)
-    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--graph-steps", type=int, default=0,
-                    help="steps per HIP-graph-captured chunk of the timed run (0: plain launches; measured "
-                         "on large_1m: no difference, the launches already overlap)")
+                    help="batches per HIP-graph-captured chunk of the timed run (0: plain launches)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the simulation step of one network is not "
@@ -197,34 +275,38 @@ def main():
         bench_event(args)
         return
 
-    from naer_amd import _abi, models
+    from naer_amd import _abi
 
     mode = _abi.MODE_FAST if args.mode == "fast" else _abi.MODE_DETERMINISTIC
     t = time.perf_counter()
     net = make_workload(args.workload, n=args.n, indegree=args.indegree, mode=mode)
     net.compile()
     build_s = time.perf_counter() - t
-    print(f"# built N={net.getNeurons()} S={net.getSynapses()} in {build_s:.1f} s", file=sys.stderr,
-          flush=True)
+    B = int(net.stats().batch)
+    print(f"# built N={net.getNeurons()} S={net.getSynapses()} in {build_s:.1f} s, batches of {B} steps",
+          file=sys.stderr, flush=True)
 
-    # warm-up: half of it to leave the initial transient, then the autotuner
-    # picks the delivery schedule on real steps, then the rest of the warm-up
-    net.run(args.warmup // 2)
-    schedule = net.autotune(2) if args.warmup > 0 and not args.no_autotune else net.schedule()
+    # warm-up: half of it to leave the initial transient, then (--autotune)
+    # the autotuner picks the delivery schedule on real (aligned) batches,
+    # then the rest; the warm-up ends on a batch boundary
+    net.run(B * (args.warmup // 2))
+    schedule = net.autotune(2) if args.warmup > 0 and args.autotune else net.schedule()
     done = int(net.stats().step)
-    if args.warmup > done:
-        net.run(args.warmup - done)
+    target = max(B * args.warmup, -(-done // B) * B)
+    if target > done:
+        net.run(target - done)
     net.finish()
     if args.graph_steps:
-        # the timed steps replay captured chunks (whole batches); the capture
-        # itself happens on the first chunk, so run one before timing
-        net.set_graph(args.graph_steps)
-        net.run(net.stats().batch * ((args.graph_steps + net.stats().batch - 1) // net.stats().batch))
+        # the timed batches replay captured chunks; the capture happens on the
+        # first chunk, so run one before timing
+        net.set_graph(B * args.graph_steps)
+        net.run(B * args.graph_steps)
         net.finish()
     s0 = net.stats()
     warmup_run = int(s0.step)
+    assert warmup_run % B == 0
     t0 = time.perf_counter()
-    net.run(args.steps)
+    net.run(B * args.steps)
     net.finish()
     wall = time.perf_counter() - t0
     s1 = net.stats()
@@ -233,32 +315,33 @@ def main():
     events = s1.events - s0.events
     spikes = s1.spikes - s0.spikes
     dt_ms = net.dt_ms
-    bio_s = steps * dt_ms * 1e-3
+    bio_s = steps * B * dt_ms * 1e-3
     value = events / wall
     rtf = bio_s / wall
     N = net.getNeurons()
 
     roofline = None
+    roofline_batch = None
     kernels = {}
     if not args.no_profile:
-        # kernel times: a further profiled stretch (HIP events around every
-        # launch on the net's stream), kept out of the timed region above
-        prof_steps = min(steps, args.profile_steps)
+        # kernel times: a further profiled stretch of whole batches (HIP
+        # events around every launch on the net's stream), kept out of the
+        # timed region above
+        prof = max(20, min(steps, args.profile_steps))
         net.set_graph(0)  # per-launch HIP events need plain launches
         net.set_profiling(True)
-        s0 = net.stats()
-        net.run(prof_steps)
+        p0 = net.stats()
+        net.run(B * prof)
         net.finish()
-        s1 = net.stats()
+        p1 = net.stats()
         net.set_profiling(False)
-        prof_events = s1.events - s0.events
+        prof_events = p1.events - p0.events
 
         def part(k):
-            ms = s1.kernel_ms[k] - s0.kernel_ms[k]
-            n = s1.kernel_calls[k] - s0.kernel_calls[k]
+            ms = p1.kernel_ms[k] - p0.kernel_ms[k]
+            n = p1.kernel_calls[k] - p0.kernel_calls[k]
             return ms, n
 
-        B = s1.batch
         up_ms, up_n = part(_abi.K_UPDATE)
         dl_ms, dl_n = part(_abi.K_SCATTER)  # delivered batches, all delivery kernels
         kernels["k_update"] = {"avg_us": round(up_ms * 1e3 / max(up_n, 1), 2), "launches": int(up_n),
@@ -276,24 +359,55 @@ def main():
         dl_bytes = prof_events * BYTES_PER_EVENT / max(dl_n, 1)  # per delivered batch
         dl_us = dl_ms * 1e3 / max(dl_n, 1)
         ach = dl_bytes / (dl_us * 1e-6) / 1e9 if dl_us > 0 else 0.0
+        traffic = measured_traffic() if args.workload == "large_1m" and args.n is None else None
         roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(ach / HBM_PEAK_GBS, 4),
-                    "traffic": measured_traffic() if args.workload == "large_1m" and args.n is None else None,
+                    "traffic": traffic,
                     "kernel": ("spike delivery: k_bucket_hist + k_bucket_colscan + k_bucket_starts + "
                                "k_bucket_place + k_bucket_refine") if bucketed else "k_scatter",
                     "avg_us": round(dl_us, 2), "bytes_per_launch": round(dl_bytes),
-                    "unit_of_work": f"{BYTES_PER_EVENT} B per synaptic event, one launch = one batch of {B} steps"}
+                    "events_per_launch": round(prof_events / max(dl_n, 1)),
+                    "profiled_batches": int(dl_n),
+                    "unit_of_work": f"{BYTES_PER_EVENT} B per synaptic event, one launch = one aligned batch of "
+                                    f"{B} steps"}
+        # the whole batch: update + delivery, per-step algorithmic bytes of
+        # events and neuron-steps over the summed kernel time
+        b_bytes = dl_bytes + N * B * BYTES_PER_NEURON_STEP
+        b_us = dl_us + up_ms * 1e3 / max(up_n, 1)
+        roofline_batch = {"achieved": round(b_bytes / (b_us * 1e-6) / 1e9, 1), "unit": "GB/s",
+                          "frac": round(b_bytes / (b_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+                          "avg_us": round(b_us, 2), "bytes_per_batch": round(b_bytes),
+                          "unit_of_work": f"{BYTES_PER_EVENT} B per event + {BYTES_PER_NEURON_STEP} B per "
+                                          f"neuron-step, summed kernel time of one batch"}
+
+    schedule_out = dict(schedule)
+    net.close()  # the samples below need the device memory
+
+    sweep = serving = None
+    if args.workload == "large_1m" and not args.no_sweep:
+        print("# sweep sample", file=sys.stderr, flush=True)
+        try:
+            sweep = sweep_sample(args)
+        except Exception as e:  # a sample, reported beside the headline
+            sweep = {"error": str(e)}
+    if args.workload == "large_1m" and not args.no_serving:
+        print("# serving sample", file=sys.stderr, flush=True)
+        try:
+            serving = serving_sample(args)
+        except Exception as e:
+            serving = {"error": str(e)}
 
     cpu = None
     if not args.no_cpu:
+        print("# cpu baseline", file=sys.stderr, flush=True)
         try:
             cpu = cpu_baseline(args)
         except Exception as e:  # the baseline is reported, never required
             cpu = {"error": str(e)}
 
     out = {
-        "metric": "synaptic events/s & realtime factor, 1M LIF net" if args.workload == "large_1m"
-                  else f"synaptic events/s & realtime factor, {args.workload}",
+        "metric": "synaptic events/s & realtime factor, 1M LIF net; sweep jobs/s + p99 latency"
+                  if args.workload == "large_1m" else f"synaptic events/s & realtime factor, {args.workload}",
         "value": round(value, 1),
         "unit": "synaptic events/s",
         "n_gpus": 1,
@@ -305,19 +419,22 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (device-built connectivity and Poisson drive from fixed seeds)",
-        "config": {"workload": f"{args.workload}: {WORKLOADS[args.workload]}, {steps * dt_ms * 1e-3:g} s at "
-                               f"dt={dt_ms:g} ms",
-                   "n_neurons": N, "n_synapses": int(net.getSynapses()), "mode": args.mode,
+        "config": {"workload": f"{args.workload}: {WORKLOADS[args.workload]}, {bio_s:g} s at dt={dt_ms:g} ms",
+                   "step": f"one aligned batch of {B} steps of {dt_ms:g} ms (update + delivery)",
+                   "batch_steps": B, "n_neurons": N, "n_synapses": int(s1.n_synapses), "mode": args.mode,
                    "dt_ms": dt_ms},
         "realtime_factor": round(rtf, 3),
         "rate_hz": round(spikes / N / bio_s, 3),
         "events": int(events),
         "build_s": round(build_s, 2),
-        "schedule": schedule,
+        "schedule": schedule_out,
         "warmup_steps_run": warmup_run,
         "graph_steps": args.graph_steps,
         "roofline": roofline,
+        "roofline_batch": roofline_batch,
         "kernels": kernels,
+        "sweep": sweep,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -854,15 +854,18 @@ __global__ __launch_bounds__(kBlock) void k_flush(const FlushArgs a)
 // places exactly the events it counted.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kBucketThreads = 1024;
-constexpr uint32_t kArrivalLut = 256;  // offsets r + d < kMaxBatch + 128
+constexpr uint32_t kArrivalLut = 256;  // bound of the offsets r + d < kMaxBatch + 128
 
-__device__ __forceinline__ void bucket_setup(const BucketArgs& a, uint8_t* lut_m, uint8_t* lut_o)
-{
-    for (uint32_t v = threadIdx.x; v < kArrivalLut; v += blockDim.x) {
-        lut_m[v] = (uint8_t)(v / a.Bfull);
-        lut_o[v] = (uint8_t)(v % a.Bfull);
-    }
-}
+// Arrival batch and slot of an event at offset v = r + d (< kArrivalLut)
+// from its aligned batch's first step: v / B and v % B by a multiply-shift,
+// exact for v < 256 and B <= 16 (the rounding error of ceil(2^16 / B) adds
+// less than 256 / 2^16 < 1 / B).
+struct Arrival {
+    uint32_t B, magic;
+    __device__ __forceinline__ explicit Arrival(uint32_t b) : B(b), magic((65535u + b) / b) {}
+    __device__ __forceinline__ uint32_t batch(uint32_t v) const { return (v * magic) >> 16; }
+    __device__ __forceinline__ uint32_t slot(uint32_t v, uint32_t m) const { return v - m * B; }
+};
 
 // Segment ranges of the histogram/placement blocks, balanced by the batch's
 // synaptic events: segment s goes to block
@@ -1016,7 +1019,6 @@ template <uint32_t U, bool AGG>
 __global__ __launch_bounds__(kBucketThreads) void k_bucket_hist(const BucketArgs a)
 {
     extern __shared__ uint32_t s_hist[];
-    __shared__ uint8_t lut_m[kArrivalLut], lut_o[kArrivalLut];
     __shared__ uint32_t s_off[kMaxSegPerBlock + 1];
     __shared__ RowStage rs;
     const DevState* st = a.st;
@@ -1024,7 +1026,7 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_hist(const BucketArgs
     const unsigned long long t0 = st->t0_cur - st->t0_cur % a.Bfull;  // the aligned batch (deferred launches)
     for (uint32_t i = threadIdx.x; i < a.NBK; i += blockDim.x)
         s_hist[i] = 0;
-    bucket_setup(a, lut_m, lut_o);
+    const Arrival arr(a.Bfull);
     __syncthreads();
     const uint32_t lane = lane_id();
     for_block_chunks<U, false>(a, cur, s_off, t0, rs, [&](uint32_t r, uint32_t, unsigned long long beg,
@@ -1041,7 +1043,7 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_hist(const BucketArgs
             if (q < end) {
                 const uint32_t v = r + ((c[u] >> 24) & 0x7Fu);
                 const uint32_t T = (c[u] & kTargetMask) >> a.tw_shift;
-                run_add<false, AGG>(s_hist, (lut_m[v] - 1u) * a.NT + T);
+                run_add<false, AGG>(s_hist, (arr.batch(v) - 1u) * a.NT + T);
             }
         }
     });
@@ -1172,7 +1174,6 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_place(const BucketArg
     using RingT = typename std::conditional<DET, unsigned long long, float>::type;
     using Stage = typename std::conditional<REC4, RowStageCls, RowStage>::type;
     extern __shared__ uint32_t s_cur[];
-    __shared__ uint8_t lut_m[kArrivalLut], lut_o[kArrivalLut];
     __shared__ uint32_t s_off[kMaxSegPerBlock + 1];
     __shared__ Stage rs;
     DevState* st = a.st;
@@ -1189,7 +1190,7 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_place(const BucketArg
         for (uint32_t i = threadIdx.x; i < nbc; i += blockDim.x)
             s_cur[i] = bs[i << a.gt_shift] + h[i];
     }
-    bucket_setup(a, lut_m, lut_o);
+    const Arrival arr(a.Bfull);
     TraceTabs tt{ nullptr, nullptr, 0u };
     if (PLASTIC) {
         __shared__ float s_tp[kLdsTrace], s_tm[kLdsTrace];
@@ -1252,25 +1253,37 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_place(const BucketArg
                 }
             }
         }
+        if (ok) {
+            // (storing each record right after its position was measured
+            // faster than all positions first, then all stores)
 #pragma unroll
-        for (uint32_t u = 0; u < U; ++u) {
-            const unsigned long long q = beg + u * kWave + lane;
-            if (q < end) {
-                const uint32_t d = (c[u] >> 24) & 0x7Fu;
-                const uint32_t tgt = c[u] & kTargetMask;
-                if (ok) {
+            for (uint32_t u = 0; u < U; ++u) {
+                const unsigned long long q = beg + u * kWave + lane;
+                if (q < end) {
+                    const uint32_t d = (c[u] >> 24) & 0x7Fu;
+                    const uint32_t tgt = c[u] & kTargetMask;
                     const uint32_t v = r + d;
                     const uint32_t g = tgt >> g_shift;
-                    const uint32_t pos = run_add<true, AGG, kPlaceRunHeads>(s_cur, (lut_m[v] - 1u) * ng + g);
+                    const uint32_t m = arr.batch(v);
+                    const uint32_t pos = run_add<true, AGG, kPlaceRunHeads>(s_cur, (m - 1u) * ng + g);
+                    const uint32_t rec = (tgt - (g << g_shift)) | (arr.slot(v, m) << (REC4 ? kRec4SlotShift : 16u)) |
+                                         (REC4 ? cls << kRec4ClassShift : 0u);
                     if constexpr (REC4)
-                        stage4[pos] = (tgt - (g << g_shift)) | ((uint32_t)lut_o[v] << kRec4SlotShift) |
-                                      (cls << kRec4ClassShift);
+                        stage4[pos] = rec;
                     else
-                        stage[pos] = make_uint2((tgt - (g << g_shift)) | ((uint32_t)lut_o[v] << 16),
-                                                __float_as_uint(w[u]));
-                } else {
-                    // the batch does not fit the event buffer: deliver through the ring
-                    uint32_t sl = (uint32_t)((t0 + r) % a.D) + d;
+                        stage[pos] = make_uint2(rec, __float_as_uint(w[u]));
+                }
+            }
+        } else {
+            // the batch does not fit the event buffer: deliver through the ring
+            const uint32_t slot0 = (uint32_t)((t0 + r) % a.D);
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u) {
+                const unsigned long long q = beg + u * kWave + lane;
+                if (q < end) {
+                    const uint32_t d = (c[u] >> 24) & 0x7Fu;
+                    const uint32_t tgt = c[u] & kTargetMask;
+                    uint32_t sl = slot0 + d;
                     sl = sl >= a.D ? sl - a.D : sl;
                     RingT* dst = ring + (size_t)sl * a.N + tgt;
                     if (DET)
@@ -1361,10 +1374,25 @@ __global__ __launch_bounds__(kRefineThreads) void k_bucket_refine(const BucketAr
             if (threadIdx.x < gt)
                 s_cnt[threadIdx.x] = 0;
             __syncthreads();
-            for (uint32_t i = threadIdx.x; i < n; i += kRefineThreads) {
-                const Rec e = stage[r0 + i];
-                s_ev[i] = e;
-                s_rank[i] = (uint16_t)run_add<true, AGG>(s_cnt, offset(e) >> a.tw_shift);
+            {
+                // the chunk's records into registers first (all loads in
+                // flight together), then ranked per tile
+                constexpr uint32_t kPer = kRefineChunk / kRefineThreads;
+                Rec e[kPer];
+#pragma unroll
+                for (uint32_t k = 0; k < kPer; ++k) {
+                    const uint32_t i = threadIdx.x + k * kRefineThreads;
+                    if (i < n)
+                        e[k] = stage[r0 + i];
+                }
+#pragma unroll
+                for (uint32_t k = 0; k < kPer; ++k) {
+                    const uint32_t i = threadIdx.x + k * kRefineThreads;
+                    if (i < n) {
+                        s_ev[i] = e[k];
+                        s_rank[i] = (uint16_t)run_add<true, AGG>(s_cnt, offset(e[k]) >> a.tw_shift);
+                    }
+                }
             }
             __syncthreads();
             if (threadIdx.x < gt && s_cnt[threadIdx.x])
