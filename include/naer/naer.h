@@ -477,6 +477,17 @@ int naer_ref_poisson_epoch(naer_ctx* ctx, uint32_t n, uint32_t n_pois_total, uin
                            float rate_hz, uint32_t t0, uint32_t t1, uint32_t* times,
                            uint32_t* count);
 
+/* naer_ref_poisson_epoch without any device allocation: the same kernel and
+ * results, staged through a static device array of 4 MiB per GPU instead of
+ * buffers allocated per call.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (tight serving loops, harnesses
+ * that forbid hipMalloc).  n may be at most 95,325 (n * 11 words of the
+ * array), else NAER_ERR_CAPACITY.  Every call on a GPU shares that GPU's
+ * array: calls that overlap on one GPU (from several host threads) race. */
+int naer_ref_poisson_epoch_noalloc(naer_ctx* ctx, uint32_t n, uint32_t n_pois_total, uint32_t seed,
+                                   float rate_hz, uint32_t t0, uint32_t t1, uint32_t* times,
+                                   uint32_t* count);
+
 /* Message of the last failure on the calling thread ("" if none). */
 const char* naer_last_error(void);
 

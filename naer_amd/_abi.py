@@ -156,6 +156,8 @@ _SIGS = {
     "naer_philox": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, u64],
     "naer_ref_poisson_epoch": [ctypes.c_void_p, u32, u32, u32, f32, u32, u32, ctypes.c_void_p,
                                ctypes.c_void_p],
+    "naer_ref_poisson_epoch_noalloc": [ctypes.c_void_p, u32, u32, u32, f32, u32, u32, ctypes.c_void_p,
+                               ctypes.c_void_p],
     "naer_model_create": [ctypes.c_char_p, u32, ctypes.POINTER(ctypes.c_void_p)],
     "naer_model_destroy": [ctypes.c_void_p],
     "naer_model_source": [ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p)],

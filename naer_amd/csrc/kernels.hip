@@ -1642,6 +1642,15 @@ __global__ __launch_bounds__(kBlock) void k_ref_poisson_epoch(uint32_t n, uint32
     count[idx] = c;
 }
 
+// Fills n words with v (clears memory without hipMemset, which a captured
+// graph must not contain).
+__global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* p, unsigned long long n, uint32_t v)
+{
+    for (unsigned long long i = (unsigned long long)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (unsigned long long)gridDim.x * kBlock)
+        p[i] = v;
+}
+
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
@@ -1839,6 +1848,15 @@ hipError_t launch_scatter(const ScatterArgs& a, bool det, bool plastic, uint32_t
         else
             hipLaunchKernelGGL((k_scatter<false, false>), grid, dim3(kBlock), 0, s, a);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t* p, unsigned long long n, uint32_t v, hipStream_t s)
+{
+    if (n == 0)
+        return hipSuccess;
+    const uint32_t grid = (uint32_t)std::min<unsigned long long>(4096, (n + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid), dim3(kBlock), 0, s, p, n, v);
     return hipGetLastError();
 }
 

@@ -18,6 +18,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1114,6 +1115,32 @@ const char* naer_last_error(void) { return g_err.c_str(); }
 
 const char* naer_version(void) { return "naer-mi355x 0.2 (gfx950)"; }
 
+// Static scratch of naer_ref_poisson_epoch_noalloc: staging for its
+// outputs, one copy per GPU (held until the process ends).  Host code uses
+// only the device address hipGetSymbolAddress gives for the current device,
+// never the array's name.
+constexpr uint64_t kScratchWords = 1u << 20;  // 4 MiB
+static __device__ __align__(256) uint32_t scratch[kScratchWords];
+static std::mutex g_scratch_mu;
+static std::map<int, uint32_t*> g_scratch;  // device -> its copy of `scratch`
+
+// The device address of `scratch` on `device` (current), looked up once per device.
+static int scratch_of(int device, uint32_t** out)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch.find(device);
+    if (it == g_scratch.end()) {
+        uint32_t* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(NAER_ERR_HIP, "no address of the static scratch on device %d: %s", device,
+                        hipGetErrorString(e));
+        it = g_scratch.emplace(device, ptr).first;
+    }
+    *out = it->second;
+    return NAER_OK;
+}
+
 int naer_ctx_create(int device, naer_ctx** out)
 {
     if (!out)
@@ -1125,6 +1152,11 @@ int naer_ctx_create(int device, naer_ctx** out)
     if (device < 0 || device >= count)
         return fail(NAER_ERR_NODEVICE, "device %d out of range (%d devices)", device, count);
     HIP_TRY(hipSetDevice(device));
+    // the static scratch's address on this device, with the device current
+    uint32_t* sp = nullptr;
+    const int rc = scratch_of(device, &sp);
+    if (rc)
+        return rc;
     naer_ctx* c = new naer_ctx;
     c->device = device;
     *out = c;
@@ -2770,6 +2802,40 @@ int naer_philox(naer_ctx* ctx, const uint32_t* ctr, const uint32_t* key, uint32_
     (void)hipFree(dout);
     if (e != hipSuccess)
         return fail(NAER_ERR_HIP, "naer_philox: %s", hipGetErrorString(e));
+    return NAER_OK;
+}
+
+int naer_ref_poisson_epoch_noalloc(naer_ctx* ctx, uint32_t n, uint32_t n_pois_total, uint32_t seed,
+                                   float rate_hz, uint32_t t0, uint32_t t1, uint32_t* times, uint32_t* count)
+{
+    if (!ctx || (n && (!times || !count)))
+        return fail(NAER_ERR_INVALID, "NULL argument");
+    if (n == 0)
+        return NAER_OK;
+    const uint64_t words = (uint64_t)n * (kMaxPoissonPerEpoch + 1);
+    if (words > kScratchWords)
+        return fail(NAER_ERR_CAPACITY,
+                    "naer_ref_poisson_epoch_noalloc: %u neurons need %llu words of its static scratch, which holds "
+                    "%llu (at most %llu neurons)",
+                    n, (unsigned long long)words, (unsigned long long)kScratchWords,
+                    (unsigned long long)(kScratchWords / (kMaxPoissonPerEpoch + 1)));
+    HIP_TRY(hipSetDevice(ctx->device));
+    uint32_t* base = nullptr;
+    int rc = scratch_of(ctx->device, &base);
+    if (rc)
+        return rc;
+    uint32_t* dt = base;
+    uint32_t* dc = base + (size_t)n * kMaxPoissonPerEpoch;
+    // the scratch keeps the previous call's contents: clear the times
+    hipError_t e = launch_fill_u32(dt, (unsigned long long)n * kMaxPoissonPerEpoch, 0u, nullptr);
+    if (e == hipSuccess)
+        e = launch_ref_poisson_epoch(n, n_pois_total, seed, -((double)rate_hz / 1000000.0), t0, t1, dt, dc, nullptr);
+    if (e == hipSuccess)
+        e = hipMemcpy(times, dt, (size_t)n * kMaxPoissonPerEpoch * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess)
+        e = hipMemcpy(count, dc, (size_t)n * 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess)
+        return fail(NAER_ERR_HIP, "naer_ref_poisson_epoch_noalloc: %s", hipGetErrorString(e));
     return NAER_OK;
 }
 
