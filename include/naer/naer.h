@@ -352,10 +352,17 @@ typedef struct naer_arena_desc {
 
 int naer_arena_create(naer_ctx* ctx, const naer_arena_desc* desc, naer_net** out);
 /* NAER_ERR_CAPACITY when the job does not fit now (the caller may retry
- * after retiring jobs); *slot identifies the job in naer_pack_layout and
- * naer_arena_retire.  Synchronises the arena. */
+ * after retiring jobs); NAER_ERR_INVALID when it never can (more neurons or
+ * synapses than the arena holds, rows longer than max_row, settings the
+ * arena was not created with).  *slot identifies the job in naer_pack_layout
+ * and naer_arena_retire.  Synchronises the arena. */
 int naer_arena_admit(naer_net* arena, const naer_net_desc* job, uint32_t* slot);
 int naer_arena_retire(naer_net* arena, uint32_t slot);
+/* Watchdog (no counterpart in the reference): the slots (networks of a
+ * pack, jobs of an arena; 0 for a single net) holding a neuron whose
+ * potential is not finite, e.g. a job whose parameters drive its state out of
+ * fp32 range.  Writes the first min(cap, *n_bad) slots.  Synchronises the net. */
+int naer_net_check_finite(naer_net* net, uint32_t* bad_slots, uint32_t cap, uint32_t* n_bad);
 
 /* Device connectivity build + state initialisation; replaces
  * Network::compile (src/Network.cpp:267-308) and allocateConnections

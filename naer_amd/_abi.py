@@ -140,6 +140,7 @@ _SIGS = {
     "naer_arena_create": [ctypes.c_void_p, ctypes.POINTER(ArenaDesc), ctypes.POINTER(ctypes.c_void_p)],
     "naer_arena_admit": [ctypes.c_void_p, ctypes.POINTER(NetDesc), ctypes.POINTER(u32)],
     "naer_arena_retire": [ctypes.c_void_p, u32],
+    "naer_net_check_finite": [ctypes.c_void_p, ctypes.c_void_p, u32, ctypes.POINTER(u32)],
     "naer_net_build": [ctypes.c_void_p],
     "naer_net_step": [ctypes.c_void_p, u32],
     "naer_net_sync": [ctypes.c_void_p],

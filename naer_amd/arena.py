@@ -47,7 +47,10 @@ class Arena:
 
     def admit(self, network):
         """Build `network` (an uncompiled naer_amd.network.Network) into the
-        arena; returns its slot, or None when it does not fit now."""
+        arena; returns its slot, or None when it does not fit now.  A job
+        that can never be admitted (larger than the arena, settings the arena
+        was not created with) raises NaerError (NAER_ERR_INVALID), as does a
+        failed build (e.g. NAER_ERR_NOMEM)."""
         d = network.description()
         s = ctypes.c_uint32()
         rc = _abi.lib().naer_arena_admit(self._handle, ctypes.byref(d), ctypes.byref(s))
@@ -56,6 +59,15 @@ class Arena:
         check(rc)
         self.slots[s.value] = network
         return s.value
+
+    def check(self):
+        """Slots of the active jobs holding a potential that is not finite
+        (naer_net_check_finite)."""
+        cap = max(1, len(self.slots))
+        out = (ctypes.c_uint32 * cap)()
+        n = ctypes.c_uint32()
+        check(_abi.lib().naer_net_check_finite(self._handle, out, cap, ctypes.byref(n)))
+        return [int(out[i]) for i in range(min(cap, n.value))]
 
     def retire(self, slot):
         """Free the job's ranges.  Drain first: spikes still in the record

@@ -1651,6 +1651,22 @@ __global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* p, unsigned long 
         p[i] = v;
 }
 
+// flags[c] = 1 when a neuron of [256 c, 256 c + 256) that belongs to a
+// population of an active network holds a potential that is not finite
+// (free ranges of an arena are not looked at).
+__global__ __launch_bounds__(kBlock) void k_check_finite(const UpdateArgs a, uint32_t* flags)
+{
+    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    bool bad = false;
+    if (j < a.N && a.n_pops) {
+        const uint32_t p = find_pop(a.pops, a.n_pops, a.blk_pop[j >> a.tw_shift] & ~kTileIdle, j);
+        bad = j >= a.pops[p].begin && j < a.pops[p].end && !isfinite(a.V[j]);
+    }
+    const int any = __syncthreads_or(bad ? 1 : 0);
+    if (threadIdx.x == 0)
+        flags[blockIdx.x] = any ? 1u : 0u;
+}
+
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
@@ -1848,6 +1864,14 @@ hipError_t launch_scatter(const ScatterArgs& a, bool det, bool plastic, uint32_t
         else
             hipLaunchKernelGGL((k_scatter<false, false>), grid, dim3(kBlock), 0, s, a);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_check_finite(const UpdateArgs& a, uint32_t* flags, hipStream_t s)
+{
+    if (a.N == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_check_finite, dim3(cdiv(a.N, kBlock)), dim3(kBlock), 0, s, a, flags);
     return hipGetLastError();
 }
 

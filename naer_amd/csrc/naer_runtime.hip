@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <initializer_list>
 #include <map>
 #include <mutex>
 #include <string>
@@ -258,6 +259,7 @@ struct naer_net {
     uint32_t rec_bytes = 8;   // 4: class-weight delivery (kRec4* records)
     uint2* ev = nullptr;      // [R][e_cap] records of rec_bytes
     uint2* stage = nullptr;   // [e_cap] records of rec_bytes
+    uint32_t* finite_flags = nullptr;  // naer_net_check_finite: per 256-neuron chunk
     // class-weight delivery: class (population) of every neuron, weight of every class
     uint32_t* ncls = nullptr;
     float* cls_w = nullptr;
@@ -398,9 +400,23 @@ int check_net(naer_net* n, const naer_net_desc& d, NetInfo* info)
     info->desc.pops = nullptr;
     info->desc.conns = nullptr;
     uint64_t nn = 0;
+    auto finite = [](std::initializer_list<double> v) {
+        for (double x : v)
+            if (!std::isfinite(x))
+                return false;
+        return true;
+    };
+    if (!finite({ d.dt_ms }) || !(d.dt_ms > 0.0f))
+        return fail(NAER_ERR_INVALID, "dt must be finite and > 0");
     for (uint32_t p = 0; p < d.n_pops; ++p) {
         const naer_pop_desc& q = d.pops[p];
         nn += q.size;
+        if (!finite({ q.tau_m_ms, q.v_th, q.v_reset, q.v_rest, q.t_ref_ms, q.v0_min, q.v0_max, q.ext_rate_hz,
+                      q.ext_weight, q.src_rate_hz }))
+            return fail(NAER_ERR_INVALID, "population %u: parameters must be finite", p);
+        for (uint32_t k = 0; q.kind == NAER_POP_MODEL && k < NAER_MODEL_PARAMS; ++k)
+            if (!std::isfinite(q.params[k]))
+                return fail(NAER_ERR_INVALID, "population %u: model parameter %u is not finite", p, k);
         if (q.kind == NAER_POP_POISSON) {
             if (kEpochUs % n->dt_us != 0)
                 return fail(NAER_ERR_INVALID, "POISSON sources need dt dividing the 1500 us epoch");
@@ -435,12 +451,15 @@ int check_net(naer_net* n, const naer_net_desc& d, NetInfo* info)
             return fail(NAER_ERR_INVALID, "connection %u: unknown rule %u", c, k.rule);
         if (k.rule == NAER_RULE_BERNOULLI && !(k.probability >= 0.0 && k.probability <= 1.0))
             return fail(NAER_ERR_INVALID, "connection %u: probability must be in [0, 1]", c);
+        if (!finite({ k.weight, k.wmin, k.wmax }))
+            return fail(NAER_ERR_INVALID, "connection %u: weights must be finite", c);
         info->dmin = std::min(info->dmin, k.delay_min);
         info->dmax = std::max(info->dmax, k.delay_max);
         if (k.plastic) {
             const naer_stdp_desc& s = d.stdp;
-            if (!(s.tau_plus_ms > 0.0f) || !(s.tau_minus_ms > 0.0f) || !(s.w_max >= s.w_min))
-                return fail(NAER_ERR_INVALID, "STDP needs tau_plus, tau_minus > 0 and w_max >= w_min");
+            if (!(s.tau_plus_ms > 0.0f) || !(s.tau_minus_ms > 0.0f) || !(s.w_max >= s.w_min) ||
+                !finite({ s.tau_plus_ms, s.tau_minus_ms, s.a_plus, s.a_minus, s.w_min, s.w_max }))
+                return fail(NAER_ERR_INVALID, "STDP needs finite parameters, tau_plus, tau_minus > 0 and w_max >= w_min");
             info->plastic = true;
         }
     }
@@ -1620,6 +1639,12 @@ int naer_arena_admit(naer_net* n, const naer_net_desc* desc, uint32_t* slot)
             std::llround((double)q.t_ref_ms / (double)n->dt_ms) < (long long)n->nref_min)
             return fail(NAER_ERR_INVALID, "job refractory period below the arena's %u steps", n->nref_min);
     }
+    // a job larger than the whole arena can never be admitted: that is an
+    // invalid job for this arena, not a capacity that retiring others frees
+    const uint64_t footprint = ((uint64_t)net.n_neurons + 1 + kBlock - 1) / kBlock * kBlock;
+    if (footprint > n->N)
+        return fail(NAER_ERR_INVALID, "a job of %u neurons (%llu with its alignment) exceeds the arena's %u",
+                    net.n_neurons, (unsigned long long)footprint, n->N);
     if ((rc = naer_net_sync(n)))
         return rc;
     // one extra entry: row_ptr[begin + n] (the end of the job's last
@@ -1642,9 +1667,12 @@ int naer_arena_admit(naer_net* n, const naer_net_desc* desc, uint32_t* slot)
     if ((rc = count_net(n, net)) || (rc = read_prefix(n, net.neuron_begin, net.n_neurons, &prefix, &net.max_row)))
         return undo(rc);
     if (net.max_row > n->max_row)
-        return undo(fail(NAER_ERR_CAPACITY, "a source has %u synapses; the arena admits rows up to %u", net.max_row,
+        return undo(fail(NAER_ERR_INVALID, "a source has %u synapses; the arena admits rows up to %u", net.max_row,
                          n->max_row));
     net.n_syn = prefix.back();
+    if (net.n_syn > n->S_cap)
+        return undo(fail(NAER_ERR_INVALID, "a job of %llu synapses exceeds the arena's %llu",
+                         (unsigned long long)net.n_syn, (unsigned long long)n->S_cap));
     if (!n->syn_alloc.take(net.n_syn, 64, &net.syn_begin))
         return undo(fail(NAER_ERR_CAPACITY, "no free range of %llu synapses", (unsigned long long)net.n_syn));
     syn_taken = true;
@@ -1670,6 +1698,47 @@ int naer_arena_admit(naer_net* n, const naer_net_desc* desc, uint32_t* slot)
     }
     HIP_TRY(hipStreamSynchronize(n->stream));
     *slot = s;
+    return NAER_OK;
+}
+
+int naer_net_check_finite(naer_net* n, uint32_t* bad_slots, uint32_t cap, uint32_t* n_bad)
+{
+    if (!n || !n_bad || (cap && !bad_slots))
+        return fail(NAER_ERR_INVALID, "NULL argument");
+    if (!n->built)
+        return fail(NAER_ERR_STATE, "net not built");
+    if (n->event)
+        return fail(NAER_ERR_INVALID, "event-driven nets are not checked");
+    int rc = naer_net_sync(n);
+    if (rc)
+        return rc;
+    // one flag per 256-neuron chunk: arena jobs start on 256-neuron
+    // boundaries, so a chunk belongs to one job at most (the networks of a
+    // pack lie back to back, so a chunk may flag two neighbours)
+    const uint32_t chunks = (n->N + kBlock - 1) / kBlock;
+    if (!n->finite_flags && (rc = n->alloc(&n->finite_flags, chunks)))
+        return rc;
+    const UpdateArgs ua = make_update_args(n);
+    HIP_TRY(launch_check_finite(ua, n->finite_flags, n->stream));
+    std::vector<uint32_t> flags(chunks);
+    HIP_TRY(hipMemcpyAsync(flags.data(), n->finite_flags, chunks * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           n->stream));
+    HIP_TRY(hipStreamSynchronize(n->stream));
+    uint32_t k = 0;
+    for (uint32_t s = 0; s < n->nets.size(); ++s) {
+        const NetInfo& net = n->nets[s];
+        if (!net.active || net.n_neurons == 0)
+            continue;
+        bool bad = false;
+        for (uint32_t c = net.neuron_begin / kBlock; c <= (net.neuron_begin + net.n_neurons - 1) / kBlock && !bad; ++c)
+            bad = flags[c] != 0;
+        if (bad) {
+            if (k < cap)
+                bad_slots[k] = s;
+            ++k;
+        }
+    }
+    *n_bad = k;
     return NAER_OK;
 }
 

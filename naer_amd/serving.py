@@ -14,6 +14,15 @@ retires the jobs that have reached their duration.  A job's raster depends only
 on its own description: alone or in any mix it is the same, bit for bit in
 deterministic mode.
 
+Failure isolation: a job that fails is retired or rejected on its own and
+the others continue unchanged.  Every job ends with a status: "done";
+"rejected" (it can never fit an arena, or its admission failed, e.g. an
+allocation error; the message is kept); "cancelled" (the client cancelled
+it, `cancel_at`); "timeout" (not done `deadline_s` after its arrival); or
+"failed" (the watchdog found a potential that is not finite after a chunk).
+(The reference's convention is to report and carry on, src/main.cpp:106-110;
+a server needs it per job.)
+
     python -m naer_amd.serving --rates 2,5,10,20 --jobs 48
 """
 import argparse
@@ -81,11 +90,16 @@ def neuron_bytes(mode, plastic):
     return ring + lists + state
 
 
+def arena_neurons(job):
+    """Neurons a job takes in its arena: its own, one more (the end of its
+    last row), rounded up to the arena's 256-neuron alignment."""
+    return (job.n + 1 + 255) // 256 * 256
+
+
 def footprint(job, mode=_abi.MODE_DETERMINISTIC):
     """Device bytes a job occupies in its arena."""
     nb = neuron_bytes(mode, job.plastic)
-    n = (job.n + 1 + 255) // 256 * 256
-    return n * nb + job.n_synapses() * BYTES_PER_SYNAPSE
+    return arena_neurons(job) * nb + job.n_synapses() * BYTES_PER_SYNAPSE
 
 
 @dataclass
@@ -116,6 +130,8 @@ class JobResult:
     arena: int = -1
     spikes: int = 0
     raster: list = field(default_factory=list)
+    status: str = "waiting"  # done, rejected, cancelled, timeout, failed
+    error: str = ""
 
     @property
     def latency_s(self):
@@ -159,16 +175,35 @@ class Server:
         for a in self.arenas:
             a.close()
 
-    def serve(self, jobs, clock=time.perf_counter, sleep=time.sleep):
+    def serve(self, jobs, clock=time.perf_counter, sleep=time.sleep, cancel_at=None, deadline_s=None,
+              watchdog=True):
         """Serve the job stream (arrival times relative to the start) and
-        return per-job results and the summary."""
+        return per-job results and the summary.  cancel_at: {job index:
+        time} at which the client cancels a job; deadline_s: the longest a
+        job may take from its arrival; watchdog: after every chunk, retire
+        the jobs whose state is no longer finite."""
         jobs = sorted(jobs, key=lambda j: j.arrival_s)
         res = {j.index: JobResult(j.index, j.arrival_s) for j in jobs}
+        cancel_at = dict(cancel_at or {})
         pending = collections.deque(jobs)
         waiting = []
         active = [dict(), dict()]  # per arena: slot -> (job, steps done)
         rejected = []
         t0 = clock()
+
+        def finish(job, status, error=""):
+            r = res[job.index]
+            r.status, r.error, r.done_s = status, error, clock() - t0
+            if status == "rejected":
+                rejected.append(job.index)
+
+        def expired(job, now):
+            if job.index in cancel_at and now >= cancel_at[job.index]:
+                return "cancelled"
+            if deadline_s is not None and now - job.arrival_s >= deadline_s:
+                return "timeout"
+            return None
+
         while pending or waiting or any(active):
             now = clock() - t0
             while pending and pending[0].arrival_s <= now:
@@ -177,22 +212,38 @@ class Server:
             # waits while later jobs may pass it
             still = []
             for job in waiting:
+                why = expired(job, now)
+                if why:
+                    finish(job, why)
+                    continue
                 ai = 1 if job.plastic else 0
                 cap = self.capacity[ai]
-                if job.n > cap.neurons or job.n_synapses() > cap.synapses or job.indegree * 4 > MAX_ROW:
-                    rejected.append(job.index)  # can never fit
+                if arena_neurons(job) > cap.neurons or job.n_synapses() > cap.synapses or \
+                        job.indegree * 4 > MAX_ROW:
+                    finish(job, "rejected", "larger than the arena")  # can never fit
                     continue
-                slot = self.arenas[ai].admit(job.network(self.mode))
+                try:
+                    slot = self.arenas[ai].admit(job.network(self.mode))
+                except _abi.NaerError as e:  # never admissible, or the build failed
+                    finish(job, "rejected", str(e))
+                    continue
                 if slot is None:
                     still.append(job)
                     continue
                 active[ai][slot] = [job, 0]
                 r = res[job.index]
-                r.admitted_s, r.slot, r.arena = clock() - t0, slot, ai
+                r.admitted_s, r.slot, r.arena, r.status = clock() - t0, slot, ai, "running"
             waiting = still
             if not any(active):
-                if pending:
+                if waiting and not pending:
+                    # the arenas are empty and still cannot take them
+                    for job in waiting:
+                        finish(job, "rejected", "does not fit an empty arena")
+                    waiting = []
+                elif pending:
                     sleep(max(0.0, min(0.05, pending[0].arrival_s - (clock() - t0))))
+                elif waiting:
+                    sleep(0.001)
                 continue
             for ai, arena in enumerate(self.arenas):
                 if active[ai]:
@@ -201,6 +252,8 @@ class Server:
                 if not active[ai]:
                     continue
                 drained = arena.drain()
+                bad = set(arena.check()) if watchdog and hasattr(arena, "check") else set()
+                now = clock() - t0
                 done = []
                 for slot, entry in active[ai].items():
                     job, steps = entry
@@ -212,20 +265,28 @@ class Server:
                         if self.keep:
                             res[job.index].raster.append(sp)
                     entry[1] = steps + self.chunk
-                    if entry[1] >= total:
-                        done.append(slot)
-                for slot in done:
+                    if slot in bad:
+                        done.append((slot, "failed", "a potential is not finite"))
+                    elif entry[1] >= total:
+                        done.append((slot, "done", ""))
+                    else:
+                        why = expired(job, now)
+                        if why:
+                            done.append((slot, why, ""))
+                for slot, status, error in done:
                     job = active[ai].pop(slot)[0]
                     arena.retire(slot)
-                    res[job.index].done_s = clock() - t0
+                    finish(job, status, error)
         makespan = clock() - t0
-        finished = [r for r in res.values() if r.done_s >= 0]
+        finished = [r for r in res.values() if r.status == "done"]
         lat = np.array([r.latency_s for r in finished]) if finished else np.zeros(1)
         for r in finished:
             if self.keep:
                 r.raster = (np.sort(np.concatenate(r.raster), order=["step", "neuron", "time_us"])
                             if r.raster else np.zeros(0, dtype=_abi.SPIKE_DTYPE))
+        statuses = collections.Counter(r.status for r in res.values())
         return dict(results=res, rejected=rejected, makespan_s=makespan, completed=len(finished),
+                    statuses=dict(statuses),
                     jobs_per_s=len(finished) / makespan if makespan > 0 else 0.0,
                     p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
                     mean_s=float(lat.mean()))

@@ -155,3 +155,67 @@ def test_job_in_large_arena_equals_alone(plastic):
     assert len(alone) > 300
     assert first_divergence(mixed, alone) is None, (len(mixed), len(alone))
     a.close()
+
+
+def test_watchdog_and_cancel_leave_the_other_jobs_unchanged(monkeypatch):
+    """One job's drive (-3e38 mV per event, finite) drives its potentials to
+    -inf: the watchdog retires it as failed after a chunk.  Another job is
+    cancelled while it runs.  Every other job finishes with the raster it
+    has alone."""
+    jobs = serving.served_jobs(8, rate_per_s=400.0, seed=123, n_range=(300, 2000), k_range=(50, 200),
+                               dur_range=(0.02, 0.05), stdp_fraction=0.25)
+    poisoned, cancelled = jobs[2].index, jobs[4].index
+    jobs[4].duration_s = 2.0  # long enough to be running when it is cancelled
+    build = serving.JobSpec.network
+
+    def network(self, mode=_abi.MODE_DETERMINISTIC, record_capacity=0):
+        net = build(self, mode, record_capacity)
+        if self.index == poisoned:
+            for p in net._pops:
+                p.params["ext_weight"] = -3e38
+        return net
+    monkeypatch.setattr(serving.JobSpec, "network", network)
+    budget = 6 * max(serving.footprint(j) for j in jobs)
+    srv = serving.Server(budget, mode=_abi.MODE_DETERMINISTIC, chunk=50, keep_rasters=True,
+                         record_per_chunk=1 << 20, synapses_per_neuron=40, stdp_fraction=0.25)
+    out = srv.serve(jobs, cancel_at={cancelled: jobs[4].arrival_s + 0.05})
+    srv.close()
+    res = out["results"]
+    assert res[poisoned].status == "failed", res[poisoned]
+    assert res[cancelled].status == "cancelled" and res[cancelled].admitted_s >= 0
+    monkeypatch.setattr(serving.JobSpec, "network", build)
+    for job in jobs:
+        if job.index in (poisoned, cancelled):
+            continue
+        assert res[job.index].status == "done", res[job.index]
+        alone = serving.run_alone(job, _abi.MODE_DETERMINISTIC)
+        assert first_divergence(res[job.index].raster, alone) is None, job
+
+
+def test_admission_errors_tell_permanent_from_transient():
+    from naer_amd.arena import Arena
+    from naer_amd.network import NaerError
+    jobs = serving.served_jobs(2, rate_per_s=1.0, seed=8, n_range=(600, 700), k_range=(50, 60),
+                               dur_range=(0.01, 0.01), stdp_fraction=0.0)
+    a = Arena(1024, 10 * 800 * 60, dt_ms=0.1, mode=_abi.MODE_DETERMINISTIC, delay_steps=(15, 15))
+    big = serving.JobSpec(9, 0.0, 1024, 50, 0.01, False, 3)  # 1024 + 1 neurons round up to 1280
+    with pytest.raises(NaerError, match="exceeds the arena"):
+        a.admit(big.network())
+    assert a.admit(jobs[0].network()) is not None
+    assert a.admit(jobs[1].network()) is None  # fits once the first retires
+    a.close()
+
+
+@pytest.mark.parametrize("what", ["weight", "v_rest", "ext_weight", "tau"])
+def test_non_finite_parameters_are_rejected(what):
+    from naer_amd import models
+    from naer_amd.network import NaerError
+    net = models.balanced(1000, 100, 0.1, 5.0, 20_000.0, 0.1, mode=_abi.MODE_DETERMINISTIC)
+    if what == "weight":
+        net._conns[0].defaultWeights = float("inf")
+    elif what == "tau":
+        net._pops[0].params["tau_m_ms"] = float("nan")
+    else:
+        net._pops[1].params[what] = float("nan")
+    with pytest.raises(NaerError, match="finite"):
+        net.compile()

@@ -12,36 +12,63 @@ from naer_amd import _abi, serving
 
 
 class FakeArena:
-    def __init__(self, cap, plastic):
+    """Host stand-in of naer_amd.arena.Arena.  Each resident job 'fires'
+    deterministically from its own seed and local steps, so its raster
+    depends on nothing else; `alloc_fail` jobs fail to build (as a device
+    allocation error would), `poison` jobs report a non-finite state after
+    their first chunk."""
+
+    def __init__(self, cap, plastic, alloc_fail=(), poison=()):
         self.cap = cap
         self.plastic = plastic
         self.used_n = self.used_s = 0
         self.slots = {}
         self.steps = 0
         self.admitted = []
+        self.alloc_fail = set(alloc_fail)
+        self.poison = set(poison)
 
     def admit(self, net):
         job = net._job
-        n = (job.n + 1 + 255) // 256 * 256
+        if job.index in self.alloc_fail:
+            raise _abi.NaerError(_abi.NAER_ERR_NOMEM, "hipMalloc failed (injected)")
+        n = serving.arena_neurons(job)
+        if n > self.cap.neurons or job.n_synapses() > self.cap.synapses:
+            raise _abi.NaerError(_abi.NAER_ERR_INVALID, "larger than the arena")
         if self.used_n + n > self.cap.neurons or self.used_s + job.n_synapses() > self.cap.synapses:
             return None
         slot = max(self.slots, default=-1) + 1
-        self.slots[slot] = (n, job.n_synapses())
+        self.slots[slot] = dict(n=n, syn=job.n_synapses(), job=job, t=0, start=self.steps)
         self.used_n += n
         self.used_s += job.n_synapses()
         self.admitted.append(job.index)
         return slot
 
     def retire(self, slot):
-        n, s = self.slots.pop(slot)
-        self.used_n -= n
-        self.used_s -= s
+        e = self.slots.pop(slot)
+        self.used_n -= e["n"]
+        self.used_s -= e["syn"]
 
     def run(self, steps):
         self.steps += steps
 
     def drain(self):
-        return {}
+        out = {}
+        for slot, e in self.slots.items():
+            local = self.steps - e["start"]
+            steps = np.arange(e["t"], local, dtype=np.uint32)
+            rng = np.random.default_rng(e["job"].seed)
+            fire = rng.integers(0, 2**32, size=local, dtype=np.uint64)[e["t"]:]
+            sel = steps[(fire % 7) == 0]
+            sp = np.zeros(len(sel), dtype=_abi.SPIKE_DTYPE)
+            sp["step"] = sel
+            sp["neuron"] = (fire[(fire % 7) == 0] >> np.uint64(8)) % np.uint64(e["job"].n)
+            out[slot] = sp
+            e["t"] = local
+        return out
+
+    def check(self):
+        return [s for s, e in self.slots.items() if e["job"].index in self.poison and e["t"] > 0]
 
     def close(self):
         pass
@@ -71,8 +98,9 @@ def no_device_networks(monkeypatch):
     monkeypatch.setattr(serving.JobSpec, "network", network)
 
 
-def make_server(budget):
-    return serving.Server(budget, arena_factory=lambda cap, plastic: FakeArena(cap, plastic), graphs=False)
+def make_server(budget, **fake):
+    return serving.Server(budget, arena_factory=lambda cap, plastic: FakeArena(cap, plastic, **fake), graphs=False,
+                          keep_rasters=True)
 
 
 def test_budget_split_respects_budget():
@@ -107,8 +135,7 @@ def test_jobs_wait_for_capacity_and_all_complete():
     assert any(r.admitted_s > r.arrival_s + 0.01 for r in out["results"].values() if r.admitted_s >= 0)
     assert static.used_n == 0 and plastic.used_n == 0  # everything retired
     for r in out["results"].values():
-        if r.done_s >= 0:
-            job = jobs[r.index]
+        if r.status == "done":
             assert r.done_s >= r.admitted_s >= r.arrival_s
     assert out["p99_s"] >= out["p50_s"] > 0
 
@@ -133,3 +160,75 @@ def test_job_stream_statistics():
     assert 500 <= k.min() and k.max() <= 2000
     assert abs(gaps.mean() - 0.1) < 0.01          # Poisson arrivals at 10 jobs/s
     assert abs(np.mean([j.plastic for j in jobs]) - 0.3) < 0.03
+
+
+# --- failure isolation -------------------------------------------------------
+ISO_JOBS = serving.served_jobs(24, rate_per_s=200.0, n_range=(2000, 20000), k_range=(500, 1500))
+
+
+def serve_iso(**kw):
+    fake = {k: kw.pop(k) for k in ("alloc_fail", "poison") if k in kw}
+    budget = 4 * max(serving.footprint(j) for j in ISO_JOBS)
+    srv = make_server(budget, **fake)
+    clock = FakeClock()
+    return srv.serve(ISO_JOBS, clock=clock, sleep=clock.sleep, **kw)
+
+
+def assert_others_unchanged(out, base, affected):
+    for i, r in out["results"].items():
+        if i in affected:
+            continue
+        b = base["results"][i]
+        assert r.status == b.status == "done", (i, r.status)
+        assert np.array_equal(r.raster, b.raster), i
+
+
+def test_injected_allocation_failure_rejects_that_job_only():
+    base = serve_iso()
+    assert base["completed"] == len(ISO_JOBS)
+    out = serve_iso(alloc_fail={3, 11})
+    for i in (3, 11):
+        assert out["results"][i].status == "rejected" and "injected" in out["results"][i].error
+    assert sorted(out["rejected"]) == [3, 11]
+    assert_others_unchanged(out, base, {3, 11})
+
+
+def test_cancel_and_deadline():
+    base = serve_iso()
+    # cancel one job before it can start and one while it runs; a deadline
+    # shorter than the slowest jobs times those out
+    first = min(ISO_JOBS, key=lambda j: j.arrival_s)
+    out = serve_iso(cancel_at={first.index: first.arrival_s + 1e-4, 5: 0.0})
+    assert out["results"][5].status == "cancelled" and out["results"][5].admitted_s < 0
+    assert out["results"][first.index].status == "cancelled"
+    assert_others_unchanged(out, base, {first.index, 5})
+    lat = sorted(r.latency_s for r in base["results"].values())
+    limit = lat[len(lat) // 2]
+    out = serve_iso(deadline_s=limit)
+    late = {i for i, r in out["results"].items() if r.status == "timeout"}
+    assert late and len(late) < len(ISO_JOBS)
+    assert all(base["results"][i].latency_s >= limit * 0.5 for i in late)
+    for i, r in out["results"].items():
+        if i not in late:
+            assert np.array_equal(r.raster, base["results"][i].raster), i
+
+
+def test_watchdog_retires_a_non_finite_job():
+    base = serve_iso()
+    out = serve_iso(poison={7})
+    assert out["results"][7].status == "failed" and "finite" in out["results"][7].error
+    assert_others_unchanged(out, base, {7})
+    assert out["statuses"]["failed"] == 1
+
+
+def test_job_of_exactly_the_arena_capacity_is_rejected():
+    """The arena takes n + 1 neurons rounded up to 256: a job of n ==
+    capacity neurons can never be admitted and must not wait forever."""
+    srv = make_server(50e9)
+    cap = srv.capacity[0]
+    jobs = [serving.JobSpec(0, 0.0, cap.neurons, 500, 0.2, False, 1),
+            serving.JobSpec(1, 0.0, 2000, 500, 0.2, False, 2)]
+    clock = FakeClock()
+    out = srv.serve(jobs, clock=clock, sleep=clock.sleep)
+    assert out["results"][0].status == "rejected"
+    assert out["results"][1].status == "done"
