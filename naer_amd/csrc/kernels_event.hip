@@ -17,11 +17,17 @@
 //   k_ev_process   a group of 8 lanes per LIF neuron finds its range of sorted
 //                  events (8-ary search), applies them in order and appends
 //                  its spikes to the next window's input list
-// The sorts and the scan are rocPRIM's (through hipCUB).
+// These replace histogram_scan / scatter / distribute_offsets
+// (spike_aggregation.cl:3-65), which order a window's spikes by time.  The
+// sorts and the scan are this file's own: an LSD radix sort of 32-bit keys
+// (and 32-bit values) in passes of up to 8 bits (11 for up to 2^20 items) —
+// per tile of 4096 items a digit histogram, one flat scan of the histograms,
+// and a stable scatter that ranks equal digits with wave ballots and writes
+// each tile out in digit order — and a three-kernel exclusive scan of 64-bit
+// row lengths.
 #include <algorithm>
 
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include "naer_device.hpp"
 #include "naer_kernels.hpp"
@@ -53,11 +59,12 @@ __device__ __forceinline__ void ev_record(const EvArgs& a, uint32_t j, uint32_t 
     }
 }
 
-__device__ __forceinline__ void ev_append(const EvArgs& a, uint32_t j, uint32_t time)
+// An input of the window starting at t_window: time offset << kEvSrcBits | source.
+__device__ __forceinline__ void ev_append(const EvArgs& a, uint32_t j, uint32_t time, uint32_t t_window)
 {
     const uint32_t k = atomicAdd(a.n_out, 1u);
     if (k < a.in_cap)
-        a.out_keys[k] = (unsigned long long)time << 32 | j;
+        a.out_keys[k] = (time - t_window) << kEvSrcBits | j;
     else
         atomicAdd(a.lost, 1u);
 }
@@ -78,7 +85,7 @@ __global__ __launch_bounds__(256) void k_ev_poisson(const EvArgs a)
     uint32_t time = t0 + ref_isi_us(pp.neg_rate_per_us, gid, pp.pois_seed, t0 * pp.n_pois + idx);
     uint32_t spikes = 0;
     for (uint32_t m = 0; m < kMaxPoissonPerEpoch && time < t1; ++m) {
-        ev_append(a, j, time);
+        ev_append(a, j, time, t0);
         ev_record(a, j, time);
         ++spikes;
         time += ref_isi_us(pp.neg_rate_per_us, gid, pp.pois_seed, time * pp.n_pois + idx);
@@ -94,7 +101,7 @@ __global__ __launch_bounds__(256) void k_ev_rowlen(const EvArgs a, uint32_t n_in
         return;
     unsigned long long len = 0;
     if (i < n_in) {
-        const uint32_t s = (uint32_t)a.in_keys[i];
+        const uint32_t s = a.in_keys[i] & ((1u << kEvSrcBits) - 1u);
         len = a.row_ptr[s + 1] - a.row_ptr[s];
     }
     a.row_len[i] = len;
@@ -103,9 +110,9 @@ __global__ __launch_bounds__(256) void k_ev_rowlen(const EvArgs a, uint32_t n_in
 // One block per input spike walks its row.
 __global__ __launch_bounds__(256) void k_ev_expand(const EvArgs a)
 {
-    const unsigned long long key = a.in_keys[blockIdx.x];
-    const uint32_t s = (uint32_t)key;
-    const uint32_t dt = (uint32_t)(key >> 32) - a.t0;  // < 1500
+    const uint32_t key = a.in_keys[blockIdx.x];
+    const uint32_t s = key & ((1u << kEvSrcBits) - 1u);
+    const uint32_t dt = key >> kEvSrcBits;  // < 1500
     const unsigned long long r0 = a.row_ptr[s], r1 = a.row_ptr[s + 1];
     const unsigned long long e0 = a.row_off[blockIdx.x];
     for (unsigned long long r = r0 + threadIdx.x; r < r1; r += 256) {
@@ -258,7 +265,7 @@ __global__ __launch_bounds__(256) void k_ev_process(const EvArgs a)
                         v = v_reset;
                         t_last = tk + t_ref;
                         if (gl == 0) {
-                            ev_append(a, j, tk + kEpochUs);
+                            ev_append(a, j, tk + kEpochUs, a.t0 + kEpochUs);
                             ev_record(a, j, tk);
                         }
                         ++fired;
@@ -307,30 +314,336 @@ hipError_t ev_launch(const EvArgs& a, int which, uint32_t n, hipStream_t s)
     return hipGetLastError();
 }
 
-// Temporary storage of the sorts and the scan for up to n_in inputs and
-// n_ev events (null `temp`: the size only).
-hipError_t ev_sort_inputs(void* temp, size_t* bytes, const unsigned long long* in, unsigned long long* out,
-                          uint32_t n, hipStream_t s)
+namespace {
+
+// ---------------------------------------------------------------------------
+// LSD radix sort of 32-bit keys (optionally with 32-bit values), stable.  A
+// pass sorts on `bits` bits (at most log2(DMAX)) at `shift`, over tiles of
+// kRsTile items (4 waves x 16 rounds of 64 items; wave w holds the tile's
+// w-th quarter, so the order (wave, round, lane) is the item order):
+//   rs_count    per tile, the count of every digit (LDS), stored digit-major
+//               cnt[d * tiles + t]
+//   (exclusive scan of cnt as one flat array: cnt[d * tiles + t] becomes the
+//    output position of tile t's first item of digit d)
+//   rs_scatter  ranks every item within its wave (lanes of equal digit found
+//               with one ballot per digit bit; per-wave digit counters in
+//               LDS), orders the tile by digit in LDS, then writes it out in
+//               that order, so consecutive lanes store consecutive positions
+//               within each digit's run
+// ---------------------------------------------------------------------------
+constexpr uint32_t kRsThreads = 256, kRsWaves = kRsThreads / kWave, kRsRounds = 16;
+constexpr uint32_t kRsTile = kRsThreads * kRsRounds;  // 4096 items
+constexpr uint32_t kRsSmallDigits = 256, kRsLargeDigits = 2048;
+// 11-bit digits (fewer passes) while the counts stay small: up to 2^20 items
+constexpr uint32_t kRsLargeDigitsMaxItems = 1u << 20;
+
+template <uint32_t DMAX>
+__global__ __launch_bounds__(kRsThreads) void rs_count(const uint32_t* keys, uint32_t n, uint32_t shift,
+                                                      uint32_t bits, uint32_t tiles, uint32_t* cnt)
 {
-    // keys: time << 32 | source
-    return hipcub::DeviceRadixSort::SortKeys(temp, *bytes, in, out, (int)n, 0, 64, s);
+    __shared__ uint32_t h[DMAX];
+    const uint32_t digits = 1u << bits;
+    for (uint32_t d = threadIdx.x; d < digits; d += kRsThreads)
+        h[d] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * kRsTile;
+    for (uint32_t k = 0; k < kRsRounds; ++k) {
+        const uint32_t i = base + k * kRsThreads + threadIdx.x;
+        if (i < n)
+            atomicAdd(&h[(keys[i] >> shift) & (digits - 1u)], 1u);
+    }
+    __syncthreads();
+    for (uint32_t d = threadIdx.x; d < digits; d += kRsThreads)
+        cnt[(size_t)d * tiles + blockIdx.x] = h[d];
 }
 
-hipError_t ev_scan_rows(void* temp, size_t* bytes, const unsigned long long* in, unsigned long long* out, uint32_t n,
-                        hipStream_t s)
+// Exclusive prefix over the block's DMAX digit slots of v (each thread owns
+// DMAX / kRsThreads consecutive slots); s is kRsThreads words of scratch.
+template <uint32_t DMAX>
+__device__ __forceinline__ void rs_block_digit_scan(uint32_t* v, uint32_t* s)
 {
-    return hipcub::DeviceScan::ExclusiveSum(temp, *bytes, in, out, (int)n, s);
+    constexpr uint32_t per = DMAX / kRsThreads;
+    uint32_t sum = 0;
+    for (uint32_t k = 0; k < per; ++k)
+        sum += v[threadIdx.x * per + k];
+    // wave-inclusive scan of the per-thread sums, then the waves' totals
+    const uint32_t lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+    uint32_t x = sum;
+    for (uint32_t off = 1; off < kWave; off <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)x, off);
+        if (lane >= off)
+            x += y;
+    }
+    if (lane == kWave - 1)
+        s[w] = x;
+    __syncthreads();
+    uint32_t run = x - sum;
+    for (uint32_t i = 0; i < w; ++i)
+        run += s[i];
+    for (uint32_t k = 0; k < per; ++k) {
+        const uint32_t c = v[threadIdx.x * per + k];
+        v[threadIdx.x * per + k] = run;
+        run += c;
+    }
+    __syncthreads();
 }
 
-hipError_t ev_sort_events(void* temp, size_t* bytes, const uint32_t* kin, uint32_t* kout, const float* vin,
-                          float* vout, uint32_t n, uint32_t n_neurons, hipStream_t s)
+template <uint32_t DMAX, bool VALS>
+__global__ __launch_bounds__(kRsThreads) void rs_scatter(const uint32_t* kin, uint32_t* kout, const uint32_t* vin,
+                                                        uint32_t* vout, uint32_t n, uint32_t shift, uint32_t bits,
+                                                        uint32_t tiles, const uint32_t* cnt)
 {
-    // keys: target << 11 | time offset (< 2048), sorted on the target bits
-    // only (radix sort is stable, so the expansion order survives)
-    int bits = 11;
-    for (uint32_t m = n_neurons ? n_neurons - 1 : 0; m; m >>= 1)
-        ++bits;
-    return hipcub::DeviceRadixSort::SortPairs(temp, *bytes, kin, kout, vin, vout, (int)n, 11, std::max(bits, 12), s);
+    __shared__ uint32_t wc[kRsWaves][DMAX];  // per-wave digit counts, then the wave's offset within the digit
+    __shared__ uint32_t ls[DMAX];            // the tile's digit counts, then their exclusive prefix
+    __shared__ uint32_t gs[DMAX];            // output position of the tile's first item of each digit
+    __shared__ uint32_t sk[kRsTile];
+    __shared__ uint32_t sv[VALS ? kRsTile : 1];
+    __shared__ uint32_t s_scan[kRsWaves];
+    const uint32_t digits = 1u << bits;
+    const uint32_t lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+    for (uint32_t d = threadIdx.x; d < DMAX; d += kRsThreads) {
+        for (uint32_t q = 0; q < kRsWaves; ++q)
+            wc[q][d] = 0;
+        gs[d] = d < digits ? cnt[(size_t)d * tiles + blockIdx.x] : 0u;
+    }
+    const uint32_t base = blockIdx.x * kRsTile;
+    const uint32_t wbase = base + w * (kRsTile / kRsWaves);
+    uint32_t key[kRsRounds], val[kRsRounds], rank[kRsRounds];
+    for (uint32_t r = 0; r < kRsRounds; ++r) {
+        const uint32_t i = wbase + r * kWave + lane;
+        key[r] = i < n ? kin[i] : 0u;
+        val[r] = (VALS && i < n) ? vin[i] : 0u;
+    }
+    __syncthreads();
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (uint32_t r = 0; r < kRsRounds; ++r) {
+        const bool valid = wbase + r * kWave + lane < n;
+        const uint32_t d = (key[r] >> shift) & (digits - 1u);
+        unsigned long long same = __ballot(valid);
+        for (uint32_t b = 0; b < bits; ++b) {
+            const unsigned long long set = __ballot((d >> b) & 1u);
+            same &= ((d >> b) & 1u) ? set : ~set;
+        }
+        const uint32_t leader = valid ? (uint32_t)__builtin_ctzll(same) : 0u;
+        uint32_t at = 0;
+        if (valid && lane == leader) {
+            at = wc[w][d];
+            wc[w][d] = at + (uint32_t)__popcll(same);
+        }
+        at = (uint32_t)__shfl((int)at, (int)leader);
+        rank[r] = at + (uint32_t)__popcll(same & below);
+    }
+    __syncthreads();
+    for (uint32_t d = threadIdx.x; d < DMAX; d += kRsThreads) {
+        uint32_t run = 0;
+        for (uint32_t q = 0; q < kRsWaves; ++q) {
+            const uint32_t c = wc[q][d];
+            wc[q][d] = run;
+            run += c;
+        }
+        ls[d] = run;
+    }
+    __syncthreads();
+    rs_block_digit_scan<DMAX>(ls, s_scan);
+    for (uint32_t r = 0; r < kRsRounds; ++r) {
+        if (wbase + r * kWave + lane < n) {
+            const uint32_t d = (key[r] >> shift) & (digits - 1u);
+            const uint32_t lp = ls[d] + wc[w][d] + rank[r];
+            sk[lp] = key[r];
+            if (VALS)
+                sv[lp] = val[r];
+        }
+    }
+    __syncthreads();
+    const uint32_t items = min(kRsTile, n - base);
+    for (uint32_t k = threadIdx.x; k < items; k += kRsThreads) {
+        const uint32_t x = sk[k];
+        const uint32_t d = (x >> shift) & (digits - 1u);
+        const uint32_t pos = gs[d] + (k - ls[d]);
+        kout[pos] = x;
+        if (VALS)
+            vout[pos] = sv[k];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive scan of n values (32- or 64-bit) in chunks of kScanChunk: chunk
+// sums, their exclusive prefix (one block), then every chunk's own prefix.
+// A chunk is staged in LDS so that the loads and stores coalesce.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kScanThreads = 256, kScanPer = 16, kScanChunk = kScanThreads * kScanPer;
+
+// exclusive prefix of x over the block; *total: the block's sum
+template <typename T>
+__device__ __forceinline__ T block_excl_scan(T x, T* s_wave, T* total)
+{
+    const uint32_t lane = threadIdx.x % kWave, w = threadIdx.x / kWave, waves = blockDim.x / kWave;
+    T v = x;
+    for (uint32_t off = 1; off < kWave; off <<= 1) {
+        const T y = __shfl_up(v, off);
+        if (lane >= off)
+            v += y;
+    }
+    if (lane == kWave - 1)
+        s_wave[w] = v;
+    __syncthreads();
+    T before = 0, sum = 0;
+    for (uint32_t i = 0; i < waves; ++i) {
+        before += i < w ? s_wave[i] : T(0);
+        sum += s_wave[i];
+    }
+    *total = sum;
+    __syncthreads();
+    return before + v - x;
+}
+
+__device__ __forceinline__ uint32_t scan_slot(uint32_t i) { return i + (i >> 5); }  // padded against bank conflicts
+
+template <typename T>
+__global__ __launch_bounds__(kScanThreads) void sc_partials(const T* in, uint32_t n, T* part)
+{
+    __shared__ T s_wave[kScanThreads / kWave];
+    const uint32_t base = blockIdx.x * kScanChunk;
+    T sum = 0;
+    for (uint32_t k = 0; k < kScanPer; ++k) {
+        const uint32_t i = base + k * kScanThreads + threadIdx.x;
+        sum += i < n ? in[i] : T(0);
+    }
+    T total;
+    (void)block_excl_scan<T>(sum, s_wave, &total);
+    if (threadIdx.x == 0)
+        part[blockIdx.x] = total;
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void sc_partial_prefix(T* part, uint32_t chunks)
+{
+    __shared__ T s_wave[1024 / kWave];
+    T carry = 0;
+    for (uint32_t c0 = 0; c0 < chunks; c0 += 1024) {
+        const uint32_t c = c0 + threadIdx.x;
+        const T x = c < chunks ? part[c] : T(0);
+        T total;
+        const T ex = block_excl_scan<T>(x, s_wave, &total);
+        if (c < chunks)
+            part[c] = carry + ex;
+        carry += total;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kScanThreads) void sc_final(const T* in, uint32_t n, const T* part, T* out)
+{
+    __shared__ T s[kScanChunk + kScanChunk / 32];
+    __shared__ T s_wave[kScanThreads / kWave];
+    const uint32_t base = blockIdx.x * kScanChunk;
+    for (uint32_t k = 0; k < kScanPer; ++k) {
+        const uint32_t j = k * kScanThreads + threadIdx.x;
+        s[scan_slot(j)] = base + j < n ? in[base + j] : T(0);
+    }
+    __syncthreads();
+    T v[kScanPer];
+    T sum = 0;
+    for (uint32_t k = 0; k < kScanPer; ++k) {
+        v[k] = s[scan_slot(threadIdx.x * kScanPer + k)];
+        sum += v[k];
+    }
+    T total;
+    T run = part[blockIdx.x] + block_excl_scan<T>(sum, s_wave, &total);
+    for (uint32_t k = 0; k < kScanPer; ++k) {
+        s[scan_slot(threadIdx.x * kScanPer + k)] = run;
+        run += v[k];
+    }
+    __syncthreads();
+    for (uint32_t k = 0; k < kScanPer; ++k) {
+        const uint32_t j = k * kScanThreads + threadIdx.x;
+        if (base + j < n)
+            out[base + j] = s[scan_slot(j)];
+    }
+}
+
+template <typename T>
+hipError_t scan_launch(const T* in, T* out, uint32_t n, T* part, hipStream_t s)
+{
+    if (n == 0)
+        return hipSuccess;
+    const uint32_t chunks = (n + kScanChunk - 1) / kScanChunk;
+    hipLaunchKernelGGL(sc_partials<T>, dim3(chunks), dim3(kScanThreads), 0, s, in, n, part);
+    hipLaunchKernelGGL(sc_partial_prefix<T>, dim3(1), dim3(1024), 0, s, part, chunks);
+    hipLaunchKernelGGL(sc_final<T>, dim3(chunks), dim3(kScanThreads), 0, s, in, n, part, out);
+    return hipGetLastError();
+}
+
+uint64_t rs_tiles(uint64_t n) { return (n + kRsTile - 1) / kRsTile; }
+
+// counts of a pass over n items and the partial sums of their scan
+uint64_t rs_count_words(uint64_t n)
+{
+    const uint64_t c = std::max(rs_tiles(n) * kRsSmallDigits,
+                                rs_tiles(std::min<uint64_t>(n, kRsLargeDigitsMaxItems)) * kRsLargeDigits);
+    return c + (c + kScanChunk - 1) / kScanChunk + 1;
+}
+
+}  // namespace
+
+uint64_t ev_sort_space(uint32_t n_max) { return rs_count_words(n_max); }
+
+uint64_t ev_scan_space(uint32_t n_max) { return ((uint64_t)n_max + kScanChunk - 1) / kScanChunk + 1; }
+
+hipError_t ev_radix_sort(uint32_t* keys, uint32_t* keys_alt, uint32_t* vals, uint32_t* vals_alt, uint32_t n,
+                         uint32_t lo_bit, uint32_t hi_bit, uint32_t* space, bool* in_alt, hipStream_t s)
+{
+    *in_alt = false;
+    if (n == 0 || hi_bit <= lo_bit)
+        return hipSuccess;
+    const uint32_t tiles = (uint32_t)rs_tiles(n);
+    const bool large = n <= kRsLargeDigitsMaxItems;
+    const uint32_t max_bits = large ? 11 : 8;
+    // equal passes: e.g. 18 bits as 6 + 6 + 6 rather than 8 + 8 + 2
+    const uint32_t total = hi_bit - lo_bit;
+    const uint32_t passes = (total + max_bits - 1) / max_bits;
+    const uint32_t per = (total + passes - 1) / passes;
+    uint32_t* src = keys;
+    uint32_t* dst = keys_alt;
+    uint32_t* vsrc = vals;
+    uint32_t* vdst = vals_alt;
+    for (uint32_t shift = lo_bit; shift < hi_bit; shift += per) {
+        const uint32_t bits = std::min(per, hi_bit - shift);
+        const uint32_t n_cnt = (1u << bits) * tiles;
+        uint32_t* cnt = space;
+        uint32_t* part = space + n_cnt;
+        if (large)
+            hipLaunchKernelGGL(rs_count<kRsLargeDigits>, dim3(tiles), dim3(kRsThreads), 0, s, src, n, shift, bits,
+                               tiles, cnt);
+        else
+            hipLaunchKernelGGL(rs_count<kRsSmallDigits>, dim3(tiles), dim3(kRsThreads), 0, s, src, n, shift, bits,
+                               tiles, cnt);
+        hipError_t e = scan_launch<uint32_t>(cnt, cnt, n_cnt, part, s);
+        if (e != hipSuccess)
+            return e;
+        if (large && vals)
+            hipLaunchKernelGGL((rs_scatter<kRsLargeDigits, true>), dim3(tiles), dim3(kRsThreads), 0, s, src, dst,
+                               vsrc, vdst, n, shift, bits, tiles, cnt);
+        else if (large)
+            hipLaunchKernelGGL((rs_scatter<kRsLargeDigits, false>), dim3(tiles), dim3(kRsThreads), 0, s, src, dst,
+                               nullptr, nullptr, n, shift, bits, tiles, cnt);
+        else if (vals)
+            hipLaunchKernelGGL((rs_scatter<kRsSmallDigits, true>), dim3(tiles), dim3(kRsThreads), 0, s, src, dst,
+                               vsrc, vdst, n, shift, bits, tiles, cnt);
+        else
+            hipLaunchKernelGGL((rs_scatter<kRsSmallDigits, false>), dim3(tiles), dim3(kRsThreads), 0, s, src, dst,
+                               nullptr, nullptr, n, shift, bits, tiles, cnt);
+        std::swap(src, dst);
+        std::swap(vsrc, vdst);
+        *in_alt = !*in_alt;
+    }
+    return hipGetLastError();
+}
+
+hipError_t ev_scan_u64(const unsigned long long* in, unsigned long long* out, uint32_t n, unsigned long long* space,
+                       hipStream_t s)
+{
+    return scan_launch<unsigned long long>(in, out, n, space, s);
 }
 
 }  // namespace naer

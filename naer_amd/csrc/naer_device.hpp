@@ -225,6 +225,7 @@ constexpr uint32_t kRec4StagedClasses = 64;  // class weights the update keeps i
 
 // Event-driven mode (kernels_event.hip): one 1500 us window per epoch.
 constexpr uint32_t kEvLo = 4096;    // entries of a population's L decay table
+constexpr uint32_t kEvSrcBits = 21; // an input key: time offset in the window (< 2^11) << 21 | source (< 2^21)
 constexpr uint32_t kEvHiCap = 8192; // most entries of its H table
 enum { kEvPoisson = 0, kEvRowLen, kEvExpand, kEvProcess };
 
@@ -238,8 +239,8 @@ struct EvArgs {
     const unsigned long long* row_ptr;
     const uint32_t* col;
     const float* w;
-    const unsigned long long* in_keys;  // the window's inputs (time << 32 | source), sorted
-    unsigned long long* out_keys;       // input list being appended to
+    const uint32_t* in_keys;            // the window's inputs (time offset << kEvSrcBits | source), sorted
+    uint32_t* out_keys;                 // input list being appended to (offsets from its window's start)
     uint32_t* n_out;                    // its count
     uint32_t* lost;                     // inputs beyond in_cap
     unsigned long long* row_len;        // [in_cap + 1]

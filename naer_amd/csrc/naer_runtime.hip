@@ -225,7 +225,7 @@ struct naer_net {
     bool event = false;
     uint32_t ev_in_cap = 0;
     uint64_t ev_e_cap = 0, ev_epoch = 0, ev_events = 0;
-    unsigned long long *ev_list[2] = { nullptr, nullptr }, *ev_sorted = nullptr;
+    uint32_t *ev_list[2] = { nullptr, nullptr }, *ev_sorted = nullptr;
     unsigned long long *ev_len = nullptr, *ev_off = nullptr;
     uint32_t *ev_keys = nullptr, *ev_keys_s = nullptr;
     float *ev_w = nullptr, *ev_w_s = nullptr;
@@ -234,8 +234,8 @@ struct naer_net {
     uint32_t *ev_nhi = nullptr, *ev_tref = nullptr;
     // [0], [1]: counts of the two input lists, [2]: inputs lost; spikes at [4] (u64)
     uint32_t* ev_ctr = nullptr;
-    void* ev_temp = nullptr;
-    size_t ev_temp_bytes = 0;
+    uint32_t* ev_sort_ws = nullptr;            // radix sort counts
+    unsigned long long* ev_scan_ws = nullptr;  // scan partial sums
 
     uint32_t scatter_chunks = 1;
 
@@ -1287,13 +1287,8 @@ static int setup_events(naer_net* n)
 {
     if (!n->event)
         return NAER_OK;
-    // validated up to 5e8 synapses: a run of brunel at 1e9 synapses reported
-    // a corrupt event count, cause not yet found
-    if (n->S_cap > 500000000ull)
-        return fail(NAER_ERR_CAPACITY, "event-driven mode: at most 5e8 synapses (this net has %llu)",
-                    (unsigned long long)n->S_cap);
-    // event keys hold target << 11 in 32 bits
-    if (n->N >= (1u << 21))
+    // event keys hold target << 11 and input keys the source in 21 bits
+    if (n->N >= (1u << kEvSrcBits))
         return fail(NAER_ERR_CAPACITY, "event-driven mode: fewer than 2^21 neurons");
     int rc;
     const uint32_t np = (uint32_t)n->pp.size();
@@ -1311,8 +1306,9 @@ static int setup_events(naer_net* n)
     n->ev_in_cap = (uint32_t)in_cap;
     size_t free_b = 0, total_b = 0;
     HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    // a quarter of the free memory for the events (key and weight, sorted
+    // and not); event indices are 32-bit
     const double fit = (double)free_b / 4.0 / 16.0;
-    // the library sorts take an int count
     n->ev_e_cap = (uint64_t)std::min(std::min(std::max((double)n->S_cap, (double)(1u << 22)), fit), 2147483647.0);
     if ((rc = n->alloc(&n->ev_list[0], in_cap)) || (rc = n->alloc(&n->ev_list[1], in_cap)) ||
         (rc = n->alloc(&n->ev_sorted, in_cap)) || (rc = n->alloc(&n->ev_len, in_cap + 1)) ||
@@ -1352,17 +1348,11 @@ static int setup_events(naer_net* n)
     HIP_TRY(hipMemcpy(n->ev_tref, tref.data(), np * sizeof(uint32_t), hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(n->ev_tlast, 0, n->N * sizeof(uint32_t)));
     HIP_TRY(hipMemset(n->ev_ctr, 0, 8 * sizeof(uint32_t)));
-    // one temporary buffer for the three library passes at full size
-    size_t b0 = 0, b1 = 0, b2 = 0;
-    HIP_TRY(ev_sort_inputs(nullptr, &b0, n->ev_list[0], n->ev_sorted, n->ev_in_cap, n->stream));
-    HIP_TRY(ev_scan_rows(nullptr, &b1, n->ev_len, n->ev_off, n->ev_in_cap + 1, n->stream));
-    HIP_TRY(ev_sort_events(nullptr, &b2, n->ev_keys, n->ev_keys_s, n->ev_w, n->ev_w_s,
-                           (uint32_t)std::min<uint64_t>(n->ev_e_cap, 0xFFFFFFFFull), n->N, n->stream));
-    n->ev_temp_bytes = std::max(b0, std::max(b1, b2));
-    unsigned char* t = nullptr;
-    if ((rc = n->alloc(&t, n->ev_temp_bytes)))
+    // workspaces of the sorts (inputs and events) and of the row scan
+    const uint32_t sort_max = (uint32_t)std::max<uint64_t>(n->ev_in_cap, n->ev_e_cap);
+    if ((rc = n->alloc(&n->ev_sort_ws, ev_sort_space(sort_max))) ||
+        (rc = n->alloc(&n->ev_scan_ws, ev_scan_space(n->ev_in_cap + 1))))
         return rc;
-    n->ev_temp = t;
     return NAER_OK;
 }
 
@@ -1381,7 +1371,6 @@ static EvArgs ev_args(const naer_net* n)
     a.row_ptr = n->row_ptr;
     a.col = n->col;
     a.w = n->w;
-    a.in_keys = n->ev_sorted;
     a.lost = n->ev_ctr + 2;
     a.row_len = n->ev_len;
     a.row_off = n->ev_off;
@@ -1399,9 +1388,14 @@ static EvArgs ev_args(const naer_net* n)
 }
 
 // Runs `epochs` windows; two host reads per window (the input and event
-// counts size the sorts).
+// counts size the sorts).  Inputs are sorted on all 32 key bits (time offset,
+// source); events on their target bits only, stably, so that each target's
+// events keep the expansion order (time, source, row position).
 static int ev_step(naer_net* n, uint32_t epochs)
 {
+    uint32_t tgt_bits = 1;
+    while (tgt_bits < 21 && (n->N - 1) >> tgt_bits)
+        ++tgt_bits;
     for (uint32_t k = 0; k < epochs; ++k) {
         if (n->ev_epoch * kEpochUs + 2 * kEpochUs > 0xFFFFFFFFull)
             return fail(NAER_ERR_CAPACITY, "event-driven mode: microsecond clock exhausted");
@@ -1417,12 +1411,12 @@ static int ev_step(naer_net* n, uint32_t epochs)
             return fail(NAER_ERR_CAPACITY, "event-driven mode: %u inputs beyond the list capacity %u", c[2],
                         n->ev_in_cap);
         const uint32_t n_in = c[cur];
-        size_t tb = n->ev_temp_bytes;
-        if (n_in)
-            HIP_TRY(ev_sort_inputs(n->ev_temp, &tb, n->ev_list[cur], n->ev_sorted, n_in, n->stream));
+        bool alt = false;
+        HIP_TRY(ev_radix_sort(n->ev_list[cur], n->ev_sorted, nullptr, nullptr, n_in, 0, 32, n->ev_sort_ws, &alt,
+                              n->stream));
+        a.in_keys = alt ? n->ev_sorted : n->ev_list[cur];
         HIP_TRY(ev_launch(a, kEvRowLen, n_in, n->stream));
-        tb = n->ev_temp_bytes;
-        HIP_TRY(ev_scan_rows(n->ev_temp, &tb, n->ev_len, n->ev_off, n_in + 1, n->stream));
+        HIP_TRY(ev_scan_u64(n->ev_len, n->ev_off, n_in + 1, n->ev_scan_ws, n->stream));
         unsigned long long n_ev = 0;
         HIP_TRY(hipMemcpyAsync(&n_ev, n->ev_off + n_in, sizeof n_ev, hipMemcpyDeviceToHost, n->stream));
         HIP_TRY(hipStreamSynchronize(n->stream));
@@ -1432,14 +1426,12 @@ static int ev_step(naer_net* n, uint32_t epochs)
         a.ev_keys = n->ev_keys;
         a.ev_w = n->ev_w;
         HIP_TRY(ev_launch(a, kEvExpand, n_in, n->stream));
-        if (n_ev) {
-            tb = n->ev_temp_bytes;
-            HIP_TRY(ev_sort_events(n->ev_temp, &tb, n->ev_keys, n->ev_keys_s, n->ev_w, n->ev_w_s, (uint32_t)n_ev,
-                                   n->N, n->stream));
-        }
+        HIP_TRY(ev_radix_sort(n->ev_keys, n->ev_keys_s, reinterpret_cast<uint32_t*>(n->ev_w),
+                              reinterpret_cast<uint32_t*>(n->ev_w_s), (uint32_t)n_ev, 11, 11 + tgt_bits,
+                              n->ev_sort_ws, &alt, n->stream));
         EvArgs b = a;
-        b.ev_keys = n->ev_keys_s;
-        b.ev_w = n->ev_w_s;
+        b.ev_keys = alt ? n->ev_keys_s : n->ev_keys;
+        b.ev_w = alt ? n->ev_w_s : n->ev_w;
         b.out_keys = n->ev_list[cur ^ 1u];
         b.n_out = n->ev_ctr + (cur ^ 1u);
         b.n_ev = (uint32_t)n_ev;
