@@ -131,9 +131,23 @@ typedef struct naer_model naer_model;
  * the input; else V = v_rest + (V - v_rest) e^(-dt/tau_m) (product in double),
  * V += w, t_last = t; V > v_th fires: V = v_reset, t_last = t + t_ref.
  * e^(-dt/tau_m) comes from two tables, exp(-4096 h / tau) * exp(-l / tau)
- * with dt = 4096 h + l.  Populations: LIF without drive, POISSON; no STDP,
- * no models, no graphs.  Recorded spikes carry step = the window and
- * time_us = the firing time. */
+ * with dt = 4096 h + l.  Populations: LIF without drive, POISSON; no models,
+ * no graphs.  Recorded spikes carry step = the window and time_us = the
+ * firing time.
+ * Plastic connections follow stdp.cl:8-140 with basic_stdp.cl's rule and
+ * constants (T = 20 ms, alpha = 2.02, lambda = 0.01, bounds +-0.3 mV (in V),
+ * the 100 us constant, ten 4 us inter-spike entries); naer_stdp_desc is not
+ * used.  A neuron of a population with a plastic connection keeps a post
+ * trace and the inter-spike history; per input at t, last = t_last -
+ * 4 hist[0]: a plastic input first updates its synapse (LTP over the history
+ * in the reference's integer types, the trace decayed from last, LTD, hard
+ * bounds; the CSR weight keeps the result), then the neuron steps with that
+ * weight; hist[0] = (t - last) / 4 in 16 bits; a spike shifts the history,
+ * adds 1 to the trace and carries {last, trace} to its targets.  The
+ * exponentials are one single-precision e^x built from IEEE operations
+ * (identical on host and device).  The source population of a plastic
+ * connection needs t_ref >= 1.5 ms (it fires at most once per window, as
+ * with the reference's 2 ms), and a plastic net fewer than 2^31 synapses. */
 #define NAER_MODE_EVENT 2u
 
 /* ---- network description -------------------------------------------------- */
@@ -453,7 +467,8 @@ int naer_net_raster_close(naer_net* net, uint64_t* n_total);
 /* Copy neuron state.  Any pointer may be NULL.  Arrays have n_neurons
  * entries; POISSON neurons report v = 0 and refractory = 0.  pre/post are the
  * STDP traces the next step sees (zeros when the net has no plastic
- * connection). */
+ * connection); in the event-driven mode post is stdp.cl's post trace and pre
+ * is zero. */
 int naer_net_read_state(naer_net* net, float* v, int32_t* refractory, float* pre_trace,
                         float* post_trace);
 

@@ -30,6 +30,7 @@
 #include <hip/hip_runtime.h>
 
 #include "naer_device.hpp"
+#include "naer_expf.hpp"
 #include "naer_kernels.hpp"
 
 namespace naer {
@@ -59,14 +60,24 @@ __device__ __forceinline__ void ev_record(const EvArgs& a, uint32_t j, uint32_t 
     }
 }
 
-// An input of the window starting at t_window: time offset << kEvSrcBits | source.
-__device__ __forceinline__ void ev_append(const EvArgs& a, uint32_t j, uint32_t time, uint32_t t_window)
+// An input of the window starting at t_window: time offset << kEvSrcBits |
+// source; plastic nets: its slot, and a plastic neuron's payload (stdp.cl:123-127).
+__device__ __forceinline__ void ev_append(const EvArgs& a, uint32_t j, uint32_t time, uint32_t t_window,
+                                          bool payload = false, uint32_t prev = 0, float trace = 0.0f)
 {
     const uint32_t k = atomicAdd(a.n_out, 1u);
-    if (k < a.in_cap)
+    if (k < a.in_cap) {
         a.out_keys[k] = (time - t_window) << kEvSrcBits | j;
-    else
+        if (a.out_vals) {
+            a.out_vals[k] = k;
+            if (payload) {
+                a.out_prev[k] = prev;
+                a.out_trace[k] = trace;
+            }
+        }
+    } else {
         atomicAdd(a.lost, 1u);
+    }
 }
 
 // POISSON sources: the reference train of the window [t0, t0 + 1500).
@@ -107,19 +118,29 @@ __global__ __launch_bounds__(256) void k_ev_rowlen(const EvArgs a, uint32_t n_in
     a.row_len[i] = len;
 }
 
-// One block per input spike walks its row.
+// One block per input spike walks its row.  Plastic nets also write every
+// event's synapse (r < 2^31, | kPlasticBit for a plastic one) and its input's
+// slot; a plastic synapse occurs at most once per window (its source fires at
+// most once), so the weight read here is the one its event updates.
+template <bool PL>
 __global__ __launch_bounds__(256) void k_ev_expand(const EvArgs a)
 {
     const uint32_t key = a.in_keys[blockIdx.x];
     const uint32_t s = key & ((1u << kEvSrcBits) - 1u);
     const uint32_t dt = key >> kEvSrcBits;  // < 1500
+    const uint32_t slot = PL ? a.in_vals[blockIdx.x] : 0u;
     const unsigned long long r0 = a.row_ptr[s], r1 = a.row_ptr[s + 1];
     const unsigned long long e0 = a.row_off[blockIdx.x];
     for (unsigned long long r = r0 + threadIdx.x; r < r1; r += 256) {
-        const uint32_t tgt = a.col[r] & 0xFFFFFFu;
+        const uint32_t c = a.col[r];
+        const uint32_t tgt = c & 0xFFFFFFu;
         const unsigned long long e = e0 + (r - r0);
         a.ev_keys[e] = tgt << 11 | dt;
         a.ev_w[e] = a.w[r];
+        if (PL) {
+            a.ev_syn[e] = (uint32_t)r | (c & kPlasticBit);
+            a.ev_slot[e] = slot;
+        }
     }
 }
 
@@ -173,6 +194,43 @@ __device__ __forceinline__ uint32_t group_lower_bound(const uint32_t* keys, uint
     return bits ? lo + (uint32_t)__ffs(bits) - 1 : hi;
 }
 
+// stdp.cl:8-30 for one plastic event (the reference's update_synapse with
+// basic_stdp.cl's rule), in the reference's integer and float types:
+//   LTP over the inter-spike history while the post spike time (int) is
+//   above the input's previous time (int); each step evaluates
+//   exp((float)(uint)(prev - post) / T) (the unsigned difference of a later
+//   post time wraps, so the factor overflows to +inf and the increment is 0);
+//   the post trace decays from the last spike time (uint difference; a
+//   negative one wraps and clears the trace); LTD with alpha * lambda in
+//   double; hard bounds +-W_MAX.
+__device__ __forceinline__ float ev_stdp_update(float w, const uint32_t (&hh)[kEvHist], uint32_t last,
+                                                uint32_t t, uint32_t prev, float pre_trace, float& trace)
+{
+    int32_t post = (int32_t)last;
+#pragma unroll
+    for (uint32_t h = 1; h < kEvHist; ++h) {
+        if (!(post > (int32_t)prev && hh[h] > 0u))
+            break;
+        const float pre = naer_expf((float)(prev - (uint32_t)post) / kEvStdpTau) * pre_trace + 1.0f;
+        const double inc = kEvStdpLambda * (double)(1.0f - w) * (double)naer_expf(-pre / kEvStdpConstDt);
+        w = (float)((double)w + inc);
+        post -= (int32_t)(hh[h] * kEvHistUnit);
+    }
+    const float dt = (float)(t - last);
+    trace = dt > 0.0f ? naer_expf(-dt / kEvStdpTau) * trace : trace;
+    const double dec = (double)kEvStdpAlpha * kEvStdpLambda * (double)w * (double)naer_expf(-trace / kEvStdpConstDt);
+    w = (float)((double)w - dec);
+    return fmaxf(fminf(w, kEvStdpWMax), -kEvStdpWMax);
+}
+
+// PL: the net has plastic connections.  A neuron of a population with a
+// plastic connection then follows stdp.cl:94-128: last = t_last - 4 hist[0];
+// a plastic event first updates its synapse (written back), then the neuron
+// steps with the updated weight; hist[0] = (t - last) / 4 (16 bits); a spike
+// shifts the history, clears hist[0], adds 1 to the trace and sends
+// {prev = last, trace} with the spike.  Its other inputs (static synapses)
+// step and keep the history the same way.
+template <bool PL>
 __global__ __launch_bounds__(256) void k_ev_process(const EvArgs a)
 {
     const uint32_t gl = threadIdx.x & (kEvGroup - 1);
@@ -196,6 +254,12 @@ __global__ __launch_bounds__(256) void k_ev_process(const EvArgs a)
     const double* H = a.dec_hi + (size_t)p * kEvHiCap;
     uint32_t n_hi = 0, t_ref = 0, t_last = 0;
     float v_rest = 0.0f, v_th = 0.0f, v_reset = 0.0f, V = 0.0f;
+    bool plastic = false;
+    float trace = 0.0f;
+    uint32_t hh[kEvHist];
+#pragma unroll
+    for (uint32_t h = 0; h < kEvHist; ++h)
+        hh[h] = 0;
     if (lo != hi) {
         n_hi = a.n_hi[p];
         t_ref = a.t_ref[p];
@@ -204,34 +268,64 @@ __global__ __launch_bounds__(256) void k_ev_process(const EvArgs a)
         v_reset = a.pops[p].v_reset;
         V = a.V[j];
         t_last = a.t_last[j];
+        if (PL && a.pop_plastic[p]) {
+            plastic = true;
+            trace = a.trace[j];
+#pragma unroll
+            for (uint32_t h = 0; h < kEvHist / 2; ++h) {
+                const uint32_t x = a.hist[(size_t)j * (kEvHist / 2) + h];
+                hh[2 * h] = x & 0xFFFFu;
+                hh[2 * h + 1] = x >> 16;
+            }
+        }
     }
     // Software pipeline over chunks of kEvGroup events: while chunk c is
-    // stepped, chunk c + 1's decays are looked up and chunk c + 2's events
-    // are loaded.
-    auto load = [&](uint32_t c, uint32_t& t, float& w) {
+    // stepped, chunk c + 1's decays (and, plastic, its inputs' payloads) are
+    // looked up and chunk c + 2's events are loaded.
+    struct Ev {
+        uint32_t t;
+        float w;
+        uint32_t syn, slot;
+    };
+    auto load = [&](uint32_t c, Ev& e) {
         const uint32_t i = c + gl;
-        t = 0;
-        w = 0.0f;
+        e.t = 0;
+        e.w = 0.0f;
+        e.syn = 0;
+        e.slot = 0;
         if (i < hi) {
-            t = a.t0 + (a.ev_keys[i] & 2047u);
-            w = a.ev_w[i];
+            e.t = a.t0 + (a.ev_keys[i] & 2047u);
+            e.w = a.ev_w[i];
+            if (PL && plastic) {
+                e.syn = a.ev_syn[i];
+                e.slot = a.ev_slot[i];
+            }
         }
     };
     // the decay of each event of chunk c from the previous event's time
     // (t_before: the last event of the chunk before)
-    auto lookup = [&](uint32_t c, uint32_t t, uint32_t t_before, uint32_t& tp, double& d) {
-        tp = (uint32_t)__shfl_up((int)t, 1, kEvGroup);
+    auto lookup = [&](uint32_t c, const Ev& e, uint32_t t_before, uint32_t& tp, double& d, uint32_t& prev,
+                      float& ptr) {
+        tp = (uint32_t)__shfl_up((int)e.t, 1, kEvGroup);
         if (gl == 0)
             tp = t_before;
-        const long long dtp = (long long)t - (long long)tp;
+        const long long dtp = (long long)e.t - (long long)tp;
         d = (c + gl < hi && dtp >= 0) ? ev_decay(H, L, n_hi, dtp) : 0.0;
+        prev = 0;
+        ptr = 0.0f;
+        if (PL && c + gl < hi && (e.syn & kPlasticBit)) {
+            prev = a.in_prev[e.slot];
+            // stdp.cl:84 holds the pre trace in a uint
+            ptr = (float)(uint32_t)a.in_trace[e.slot];
+        }
     };
-    uint32_t t0c, t1c, tp0;
-    float w0c, w1c;
+    Ev e0, e1;
+    uint32_t tp0, prev0 = 0;
+    float ptr0 = 0.0f;
     double d0;
-    load(lo, t0c, w0c);
-    lookup(lo, t0c, t_last, tp0, d0);
-    load(lo + kEvGroup, t1c, w1c);
+    load(lo, e0);
+    lookup(lo, e0, t_last, tp0, d0, prev0, ptr0);
+    load(lo + kEvGroup, e1);
     uint32_t fired = 0;
     for (uint32_t c = lo;; c += kEvGroup) {
         const bool more = c < hi;
@@ -239,20 +333,37 @@ __global__ __launch_bounds__(256) void k_ev_process(const EvArgs a)
             break;
         const uint32_t n = more ? min(kEvGroup, hi - c) : 0u;
         // next chunk's decays start from this chunk's last event
-        const uint32_t t_end = (uint32_t)__shfl((int)t0c, (int)(n ? n - 1 : 0), kEvGroup);
-        uint32_t tp1;
+        const uint32_t t_end = (uint32_t)__shfl((int)e0.t, (int)(n ? n - 1 : 0), kEvGroup);
+        uint32_t tp1, prev1;
+        float ptr1;
         double d1;
-        lookup(c + kEvGroup, t1c, t_end, tp1, d1);
-        uint32_t t2c;
-        float w2c;
-        load(c + 2 * kEvGroup, t2c, w2c);
+        lookup(c + kEvGroup, e1, t_end, tp1, d1, prev1, ptr1);
+        Ev e2;
+        load(c + 2 * kEvGroup, e2);
         for (uint32_t k = 0; k < kEvGroup; ++k) {
             // every lane takes part in the shuffles; the step is predicated
-            const uint32_t tk = (uint32_t)__shfl((int)t0c, (int)k, kEvGroup);
+            const uint32_t tk = (uint32_t)__shfl((int)e0.t, (int)k, kEvGroup);
             const uint32_t tpk = (uint32_t)__shfl((int)tp0, (int)k, kEvGroup);
-            const float wk = __shfl(w0c, (int)k, kEvGroup);
+            float wk = __shfl(e0.w, (int)k, kEvGroup);
             const double dpk = __shfl(d0, (int)k, kEvGroup);
+            uint32_t synk = 0, prevk = 0;
+            float ptrk = 0.0f;
+            if (PL) {
+                synk = (uint32_t)__shfl((int)e0.syn, (int)k, kEvGroup);
+                prevk = (uint32_t)__shfl((int)prev0, (int)k, kEvGroup);
+                ptrk = __shfl(ptr0, (int)k, kEvGroup);
+            }
             if (k < n) {
+                uint32_t last = 0;
+                if (PL && plastic) {
+                    last = t_last - hh[0] * kEvHistUnit;
+                    if (synk & kPlasticBit) {
+                        wk = ev_stdp_update(wk, hh, last, tk, prevk, ptrk, trace);
+                        if (gl == 0)
+                            a.w_mut[synk & ~kPlasticBit] = wk;
+                    }
+                }
+                bool spiked = false;
                 const long long dt = (long long)tk - (long long)t_last;
                 if (dt >= 0) {
                     const double dk = tpk == t_last ? dpk : ev_decay(H, L, n_hi, dt);
@@ -264,26 +375,45 @@ __global__ __launch_bounds__(256) void k_ev_process(const EvArgs a)
                     if (v > v_th) {
                         v = v_reset;
                         t_last = tk + t_ref;
-                        if (gl == 0) {
-                            ev_append(a, j, tk + kEpochUs, a.t0 + kEpochUs);
-                            ev_record(a, j, tk);
-                        }
-                        ++fired;
+                        spiked = true;
                     }
                     V = v;
                 }
+                if (PL && plastic) {
+                    hh[0] = ((tk - last) / kEvHistUnit) & 0xFFFFu;
+                    if (spiked) {
+#pragma unroll
+                        for (uint32_t h = kEvHist - 1; h > 0; --h)
+                            hh[h] = hh[h - 1];
+                        hh[0] = 0;
+                        trace += 1.0f;
+                    }
+                }
+                if (spiked) {
+                    if (gl == 0) {
+                        ev_append(a, j, tk + kEpochUs, a.t0 + kEpochUs, PL && plastic, last, trace);
+                        ev_record(a, j, tk);
+                    }
+                    ++fired;
+                }
             }
         }
-        t0c = t1c;
-        w0c = w1c;
+        e0 = e1;
         tp0 = tp1;
         d0 = d1;
-        t1c = t2c;
-        w1c = w2c;
+        prev0 = prev1;
+        ptr0 = ptr1;
+        e1 = e2;
     }
     if (gl == 0 && lo != hi) {
         a.V[j] = V;
         a.t_last[j] = t_last;
+        if (PL && plastic) {
+            a.trace[j] = trace;
+#pragma unroll
+            for (uint32_t h = 0; h < kEvHist / 2; ++h)
+                a.hist[(size_t)j * (kEvHist / 2) + h] = hh[2 * h] | hh[2 * h + 1] << 16;
+        }
         if (fired)
             atomicAdd(a.spikes, (unsigned long long)fired);
     }
@@ -302,11 +432,18 @@ hipError_t ev_launch(const EvArgs& a, int which, uint32_t n, hipStream_t s)
         hipLaunchKernelGGL(k_ev_rowlen, dim3(n / 256 + 1), dim3(256), 0, s, a, n);
         break;
     case kEvExpand:
-        if (n)
-            hipLaunchKernelGGL(k_ev_expand, dim3(n), dim3(256), 0, s, a);
+        if (n && a.ev_syn)
+            hipLaunchKernelGGL(k_ev_expand<true>, dim3(n), dim3(256), 0, s, a);
+        else if (n)
+            hipLaunchKernelGGL(k_ev_expand<false>, dim3(n), dim3(256), 0, s, a);
         break;
     case kEvProcess:
-        hipLaunchKernelGGL(k_ev_process, dim3((a.N + 255 / kEvGroup) / (256 / kEvGroup)), dim3(256), 0, s, a);
+        if (a.ev_syn)
+            hipLaunchKernelGGL(k_ev_process<true>, dim3((a.N + 255 / kEvGroup) / (256 / kEvGroup)), dim3(256), 0, s,
+                               a);
+        else
+            hipLaunchKernelGGL(k_ev_process<false>, dim3((a.N + 255 / kEvGroup) / (256 / kEvGroup)), dim3(256), 0, s,
+                               a);
         break;
     default:
         return hipErrorInvalidValue;
@@ -336,6 +473,7 @@ constexpr uint32_t kRsTile = kRsThreads * kRsRounds;  // 4096 items
 constexpr uint32_t kRsSmallDigits = 256, kRsLargeDigits = 2048;
 // 11-bit digits (fewer passes) while the counts stay small: up to 2^20 items
 constexpr uint32_t kRsLargeDigitsMaxItems = 1u << 20;
+constexpr uint32_t kRsMaxVals = 3;
 
 template <uint32_t DMAX>
 __global__ __launch_bounds__(kRsThreads) void rs_count(const uint32_t* keys, uint32_t n, uint32_t shift,
@@ -388,16 +526,22 @@ __device__ __forceinline__ void rs_block_digit_scan(uint32_t* v, uint32_t* s)
     __syncthreads();
 }
 
-template <uint32_t DMAX, bool VALS>
-__global__ __launch_bounds__(kRsThreads) void rs_scatter(const uint32_t* kin, uint32_t* kout, const uint32_t* vin,
-                                                        uint32_t* vout, uint32_t n, uint32_t shift, uint32_t bits,
-                                                        uint32_t tiles, const uint32_t* cnt)
+// the value arrays a sort moves along with its keys
+struct RsVals {
+    const uint32_t* in[kRsMaxVals];
+    uint32_t* out[kRsMaxVals];
+};
+
+template <uint32_t DMAX, uint32_t NV>
+__global__ __launch_bounds__(kRsThreads) void rs_scatter(const uint32_t* kin, uint32_t* kout, const RsVals vals,
+                                                        uint32_t n, uint32_t shift, uint32_t bits, uint32_t tiles,
+                                                        const uint32_t* cnt)
 {
     __shared__ uint32_t wc[kRsWaves][DMAX];  // per-wave digit counts, then the wave's offset within the digit
     __shared__ uint32_t ls[DMAX];            // the tile's digit counts, then their exclusive prefix
     __shared__ uint32_t gs[DMAX];            // output position of the tile's first item of each digit
     __shared__ uint32_t sk[kRsTile];
-    __shared__ uint32_t sv[VALS ? kRsTile : 1];
+    __shared__ uint32_t sv[NV ? NV : 1][NV ? kRsTile : 1];
     __shared__ uint32_t s_scan[kRsWaves];
     const uint32_t digits = 1u << bits;
     const uint32_t lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
@@ -408,11 +552,13 @@ __global__ __launch_bounds__(kRsThreads) void rs_scatter(const uint32_t* kin, ui
     }
     const uint32_t base = blockIdx.x * kRsTile;
     const uint32_t wbase = base + w * (kRsTile / kRsWaves);
-    uint32_t key[kRsRounds], val[kRsRounds], rank[kRsRounds];
+    uint32_t key[kRsRounds], rank[kRsRounds];
+    uint32_t val[NV ? NV : 1][kRsRounds];
     for (uint32_t r = 0; r < kRsRounds; ++r) {
         const uint32_t i = wbase + r * kWave + lane;
         key[r] = i < n ? kin[i] : 0u;
-        val[r] = (VALS && i < n) ? vin[i] : 0u;
+        for (uint32_t v = 0; v < NV; ++v)
+            val[v][r] = i < n ? vals.in[v][i] : 0u;
     }
     __syncthreads();
     const unsigned long long below = (1ull << lane) - 1ull;
@@ -450,8 +596,8 @@ __global__ __launch_bounds__(kRsThreads) void rs_scatter(const uint32_t* kin, ui
             const uint32_t d = (key[r] >> shift) & (digits - 1u);
             const uint32_t lp = ls[d] + wc[w][d] + rank[r];
             sk[lp] = key[r];
-            if (VALS)
-                sv[lp] = val[r];
+            for (uint32_t v = 0; v < NV; ++v)
+                sv[v][lp] = val[v][r];
         }
     }
     __syncthreads();
@@ -461,8 +607,8 @@ __global__ __launch_bounds__(kRsThreads) void rs_scatter(const uint32_t* kin, ui
         const uint32_t d = (x >> shift) & (digits - 1u);
         const uint32_t pos = gs[d] + (k - ls[d]);
         kout[pos] = x;
-        if (VALS)
-            vout[pos] = sv[k];
+        for (uint32_t v = 0; v < NV; ++v)
+            vals.out[v][pos] = sv[v][k];
     }
 }
 
@@ -590,10 +736,33 @@ uint64_t ev_sort_space(uint32_t n_max) { return rs_count_words(n_max); }
 
 uint64_t ev_scan_space(uint32_t n_max) { return ((uint64_t)n_max + kScanChunk - 1) / kScanChunk + 1; }
 
-hipError_t ev_radix_sort(uint32_t* keys, uint32_t* keys_alt, uint32_t* vals, uint32_t* vals_alt, uint32_t n,
-                         uint32_t lo_bit, uint32_t hi_bit, uint32_t* space, bool* in_alt, hipStream_t s)
+template <uint32_t DMAX>
+void rs_scatter_launch(uint32_t nv, const uint32_t* src, uint32_t* dst, const RsVals& v, uint32_t n, uint32_t shift,
+                       uint32_t bits, uint32_t tiles, const uint32_t* cnt, hipStream_t s)
+{
+    switch (nv) {
+    case 0:
+        hipLaunchKernelGGL((rs_scatter<DMAX, 0>), dim3(tiles), dim3(kRsThreads), 0, s, src, dst, v, n, shift, bits,
+                           tiles, cnt);
+        break;
+    case 1:
+        hipLaunchKernelGGL((rs_scatter<DMAX, 1>), dim3(tiles), dim3(kRsThreads), 0, s, src, dst, v, n, shift, bits,
+                           tiles, cnt);
+        break;
+    default:
+        hipLaunchKernelGGL((rs_scatter<DMAX, 3>), dim3(tiles), dim3(kRsThreads), 0, s, src, dst, v, n, shift, bits,
+                           tiles, cnt);
+        break;
+    }
+}
+
+hipError_t ev_radix_sort(uint32_t* keys, uint32_t* keys_alt, uint32_t* const* vals, uint32_t* const* vals_alt,
+                         uint32_t nv, uint32_t n, uint32_t lo_bit, uint32_t hi_bit, uint32_t* space, bool* in_alt,
+                         hipStream_t s)
 {
     *in_alt = false;
+    if (nv == 2 || nv > kRsMaxVals)
+        return hipErrorInvalidValue;
     if (n == 0 || hi_bit <= lo_bit)
         return hipSuccess;
     const uint32_t tiles = (uint32_t)rs_tiles(n);
@@ -605,8 +774,11 @@ hipError_t ev_radix_sort(uint32_t* keys, uint32_t* keys_alt, uint32_t* vals, uin
     const uint32_t per = (total + passes - 1) / passes;
     uint32_t* src = keys;
     uint32_t* dst = keys_alt;
-    uint32_t* vsrc = vals;
-    uint32_t* vdst = vals_alt;
+    RsVals v{};
+    for (uint32_t q = 0; q < nv; ++q) {
+        v.in[q] = vals[q];
+        v.out[q] = vals_alt[q];
+    }
     for (uint32_t shift = lo_bit; shift < hi_bit; shift += per) {
         const uint32_t bits = std::min(per, hi_bit - shift);
         const uint32_t n_cnt = (1u << bits) * tiles;
@@ -621,20 +793,16 @@ hipError_t ev_radix_sort(uint32_t* keys, uint32_t* keys_alt, uint32_t* vals, uin
         hipError_t e = scan_launch<uint32_t>(cnt, cnt, n_cnt, part, s);
         if (e != hipSuccess)
             return e;
-        if (large && vals)
-            hipLaunchKernelGGL((rs_scatter<kRsLargeDigits, true>), dim3(tiles), dim3(kRsThreads), 0, s, src, dst,
-                               vsrc, vdst, n, shift, bits, tiles, cnt);
-        else if (large)
-            hipLaunchKernelGGL((rs_scatter<kRsLargeDigits, false>), dim3(tiles), dim3(kRsThreads), 0, s, src, dst,
-                               nullptr, nullptr, n, shift, bits, tiles, cnt);
-        else if (vals)
-            hipLaunchKernelGGL((rs_scatter<kRsSmallDigits, true>), dim3(tiles), dim3(kRsThreads), 0, s, src, dst,
-                               vsrc, vdst, n, shift, bits, tiles, cnt);
+        if (large)
+            rs_scatter_launch<kRsLargeDigits>(nv, src, dst, v, n, shift, bits, tiles, cnt, s);
         else
-            hipLaunchKernelGGL((rs_scatter<kRsSmallDigits, false>), dim3(tiles), dim3(kRsThreads), 0, s, src, dst,
-                               nullptr, nullptr, n, shift, bits, tiles, cnt);
+            rs_scatter_launch<kRsSmallDigits>(nv, src, dst, v, n, shift, bits, tiles, cnt, s);
         std::swap(src, dst);
-        std::swap(vsrc, vdst);
+        for (uint32_t q = 0; q < nv; ++q) {
+            uint32_t* const was_in = const_cast<uint32_t*>(v.in[q]);
+            v.in[q] = v.out[q];
+            v.out[q] = was_in;
+        }
         *in_alt = !*in_alt;
     }
     return hipGetLastError();

@@ -228,6 +228,14 @@ constexpr uint32_t kEvLo = 4096;    // entries of a population's L decay table
 constexpr uint32_t kEvSrcBits = 21; // an input key: time offset in the window (< 2^11) << 21 | source (< 2^21)
 constexpr uint32_t kEvHiCap = 8192; // most entries of its H table
 enum { kEvPoisson = 0, kEvRowLen, kEvExpand, kEvProcess };
+// stdp.cl / stdp_group.cl / basic_stdp.cl constants of the event-driven STDP
+constexpr uint32_t kEvHist = 10;          // NAER_HIST_LENGTH
+constexpr uint32_t kEvHistUnit = 4;       // NAER_HIST_UNIT (us)
+constexpr float kEvStdpTau = 20000.0f;    // NAER_T_STDP (us)
+constexpr float kEvStdpAlpha = 2.02f;     // NAER_ALPHA
+constexpr double kEvStdpLambda = 0.01;    // NAER_LAMBDA (a double literal there)
+constexpr float kEvStdpWMax = 0.0003f;    // NAER_W_MAX
+constexpr float kEvStdpConstDt = 100.0f;  // NAER_CONST_DT (us)
 
 struct EvArgs {
     const PopParams* pops;
@@ -248,6 +256,22 @@ struct EvArgs {
     uint32_t* ev_keys;                  // target << 11 | time offset (N < 2^21)
     float* ev_w;
     uint32_t n_ev;                      // events of the window
+    // plastic nets (stdp.cl): every input carries the slot of its list entry
+    // (sorted along with its key), whose payload a plastic neuron's spike
+    // writes: its previous spike time and post trace; every event carries its
+    // synapse (index | kPlasticBit for an STDP -> STDP synapse) and its input's slot
+    const uint32_t* in_vals;            // slots of the sorted inputs
+    uint32_t* out_vals;                 // slots of the list being appended to
+    const uint32_t* in_prev;            // payload of the window's list, by slot
+    const float* in_trace;
+    uint32_t* out_prev;                 // payload of the list being appended to
+    float* out_trace;
+    uint32_t* ev_syn;
+    uint32_t* ev_slot;
+    float* w_mut;                       // the CSR weights the plastic events update
+    float* trace;                       // [N] post trace (stdp_traces.stdp)
+    uint32_t* hist;                     // [N][kEvHist / 2] inter-spike history, 16 bits per entry
+    const uint32_t* pop_plastic;        // [n_pops] 1: a population with a plastic connection
     float* V;
     uint32_t* t_last;                   // [N] time of the last input taken (us)
     const double* dec_lo;               // [n_pops][kEvLo]

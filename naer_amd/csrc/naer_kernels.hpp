@@ -46,12 +46,14 @@ const char* model_source(const char* name);
 // Event-driven mode (kernels_event.hip): kernel `which` (kEv*), with n the
 // input or event count where the kernel needs one.
 hipError_t ev_launch(const EvArgs& a, int which, uint32_t n, hipStream_t s);
-// Stable LSD radix sort of n 32-bit keys on bits [lo_bit, hi_bit), values
-// (or null) moved along; `space` holds ev_sort_space(n) words; the result
-// is in keys_alt/vals_alt when *in_alt, else in keys/vals.
+// Stable LSD radix sort of n 32-bit keys on bits [lo_bit, hi_bit), with nv
+// (0, 1 or 3) arrays of 32-bit values moved along; `space` holds
+// ev_sort_space(n) words; the result is in keys_alt/vals_alt when *in_alt,
+// else in keys/vals.
 uint64_t ev_sort_space(uint32_t n_max);
-hipError_t ev_radix_sort(uint32_t* keys, uint32_t* keys_alt, uint32_t* vals, uint32_t* vals_alt, uint32_t n,
-                         uint32_t lo_bit, uint32_t hi_bit, uint32_t* space, bool* in_alt, hipStream_t s);
+hipError_t ev_radix_sort(uint32_t* keys, uint32_t* keys_alt, uint32_t* const* vals, uint32_t* const* vals_alt,
+                         uint32_t nv, uint32_t n, uint32_t lo_bit, uint32_t hi_bit, uint32_t* space, bool* in_alt,
+                         hipStream_t s);
 // Exclusive scan of n 64-bit values; `space` holds ev_scan_space(n) entries.
 uint64_t ev_scan_space(uint32_t n_max);
 hipError_t ev_scan_u64(const unsigned long long* in, unsigned long long* out, uint32_t n, unsigned long long* space,

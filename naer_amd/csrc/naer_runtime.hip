@@ -234,6 +234,14 @@ struct naer_net {
     uint32_t *ev_nhi = nullptr, *ev_tref = nullptr;
     // [0], [1]: counts of the two input lists, [2]: inputs lost; spikes at [4] (u64)
     uint32_t* ev_ctr = nullptr;
+    // plastic nets: input slots and payloads, event synapses and slots,
+    // per-neuron trace and history, plastic populations
+    uint32_t *ev_vals[2] = { nullptr, nullptr }, *ev_vals_s = nullptr;
+    uint32_t* ev_prev[2] = { nullptr, nullptr };
+    float* ev_ptr[2] = { nullptr, nullptr };
+    uint32_t *ev_syn = nullptr, *ev_syn_s = nullptr, *ev_slot = nullptr, *ev_slot_s = nullptr;
+    float* ev_trace = nullptr;
+    uint32_t *ev_hist = nullptr, *ev_pop_plastic = nullptr;
     uint32_t* ev_sort_ws = nullptr;            // radix sort counts
     unsigned long long* ev_scan_ws = nullptr;  // scan partial sums
 
@@ -475,8 +483,16 @@ int check_net(naer_net* n, const naer_net_desc& d, NetInfo* info)
             if (q.kind == NAER_POP_LIF && (!(q.t_ref_ms >= 0.0f) || q.tau_m_ms * 1000.0f < 1.0f))
                 return fail(NAER_ERR_INVALID, "population %u: tau_m must be >= 1 us and t_ref >= 0", p);
         }
-        if (info->plastic)
-            return fail(NAER_ERR_INVALID, "the event-driven mode has no plasticity");
+        // stdp.cl's update reads the weight the window's expansion carried:
+        // a plastic synapse's source fires at most once per window
+        for (uint32_t c = 0; c < d.n_conns; ++c) {
+            const naer_conn_desc& k = d.conns[c];
+            if (k.plastic && !(d.pops[k.src_pop].t_ref_ms * 1000.0f >= (float)kEpochUs))
+                return fail(NAER_ERR_INVALID,
+                            "connection %u: plastic synapses in the event-driven mode need a source refractory period "
+                            "of at least 1.5 ms",
+                            c);
+        }
     }
     return NAER_OK;
 }
@@ -1290,6 +1306,9 @@ static int setup_events(naer_net* n)
     // event keys hold target << 11 and input keys the source in 21 bits
     if (n->N >= (1u << kEvSrcBits))
         return fail(NAER_ERR_CAPACITY, "event-driven mode: fewer than 2^21 neurons");
+    // plastic nets: an event's synapse index is 31 bits
+    if (n->plastic && n->S_cap >= 0x80000000ull)
+        return fail(NAER_ERR_CAPACITY, "event-driven mode: plastic nets of fewer than 2^31 synapses");
     int rc;
     const uint32_t np = (uint32_t)n->pp.size();
     // a POISSON source emits at most 10 spikes per window, a LIF neuron one
@@ -1306,9 +1325,9 @@ static int setup_events(naer_net* n)
     n->ev_in_cap = (uint32_t)in_cap;
     size_t free_b = 0, total_b = 0;
     HIP_TRY(hipMemGetInfo(&free_b, &total_b));
-    // a quarter of the free memory for the events (key and weight, sorted
-    // and not); event indices are 32-bit
-    const double fit = (double)free_b / 4.0 / 16.0;
+    // a quarter of the free memory for the events (key and weight, plastic
+    // nets also synapse and slot; sorted and not); event indices are 32-bit
+    const double fit = (double)free_b / 4.0 / (n->plastic ? 32.0 : 16.0);
     n->ev_e_cap = (uint64_t)std::min(std::min(std::max((double)n->S_cap, (double)(1u << 22)), fit), 2147483647.0);
     if ((rc = n->alloc(&n->ev_list[0], in_cap)) || (rc = n->alloc(&n->ev_list[1], in_cap)) ||
         (rc = n->alloc(&n->ev_sorted, in_cap)) || (rc = n->alloc(&n->ev_len, in_cap + 1)) ||
@@ -1319,6 +1338,36 @@ static int setup_events(naer_net* n)
         (rc = n->alloc(&n->ev_dhi, (size_t)std::max(1u, np) * kEvHiCap)) || (rc = n->alloc(&n->ev_nhi, np)) ||
         (rc = n->alloc(&n->ev_tref, np)) || (rc = n->alloc(&n->ev_ctr, 8)))
         return rc;
+    if (n->plastic) {
+        if ((rc = n->alloc(&n->ev_vals[0], in_cap)) || (rc = n->alloc(&n->ev_vals[1], in_cap)) ||
+            (rc = n->alloc(&n->ev_vals_s, in_cap)) || (rc = n->alloc(&n->ev_prev[0], in_cap)) ||
+            (rc = n->alloc(&n->ev_prev[1], in_cap)) || (rc = n->alloc(&n->ev_ptr[0], in_cap)) ||
+            (rc = n->alloc(&n->ev_ptr[1], in_cap)) || (rc = n->alloc(&n->ev_syn, n->ev_e_cap)) ||
+            (rc = n->alloc(&n->ev_syn_s, n->ev_e_cap)) || (rc = n->alloc(&n->ev_slot, n->ev_e_cap)) ||
+            (rc = n->alloc(&n->ev_slot_s, n->ev_e_cap)) || (rc = n->alloc(&n->ev_trace, n->N)) ||
+            (rc = n->alloc(&n->ev_hist, (size_t)n->N * (kEvHist / 2))) ||
+            (rc = n->alloc(&n->ev_pop_plastic, std::max(1u, np))))
+            return rc;
+        // populations with a plastic connection (stdp.cl's neurons), in the
+        // population table's order (non-empty populations)
+        std::vector<uint32_t> pl(np, 0);
+        uint32_t base = 0;
+        for (auto& net : n->nets) {
+            std::vector<uint32_t> at(net.pops.size(), 0);
+            uint32_t q = base;
+            for (size_t k = 0; k < net.pops.size(); ++k) {
+                at[k] = q;
+                q += net.pops[k].size ? 1u : 0u;
+            }
+            for (auto& k : net.conns)
+                if (k.plastic && net.pops[k.src_pop].size && net.pops[k.dst_pop].size)
+                    pl[at[k.src_pop]] = pl[at[k.dst_pop]] = 1;
+            base = q;
+        }
+        HIP_TRY(hipMemcpy(n->ev_pop_plastic, pl.data(), np * sizeof(uint32_t), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemset(n->ev_trace, 0, n->N * sizeof(float)));
+        HIP_TRY(hipMemset(n->ev_hist, 0, (size_t)n->N * (kEvHist / 2) * sizeof(uint32_t)));
+    }
     std::vector<double> lo((size_t)np * kEvLo, 0.0), hi((size_t)np * kEvHiCap, 0.0);
     std::vector<uint32_t> nhi(np, 0), tref(np, 0);
     uint32_t p = 0;
@@ -1384,6 +1433,12 @@ static EvArgs ev_args(const naer_net* n)
     a.rec_cap = n->rec_cap;
     a.rec_n = (unsigned long long*)((char*)n->st + offsetof(DevState, rec_n));
     a.spikes = (unsigned long long*)(n->ev_ctr + 4);
+    if (n->plastic) {
+        a.w_mut = n->w;
+        a.trace = n->ev_trace;
+        a.hist = n->ev_hist;
+        a.pop_plastic = n->ev_pop_plastic;
+    }
     return a;
 }
 
@@ -1403,6 +1458,8 @@ static int ev_step(naer_net* n, uint32_t epochs)
         EvArgs a = ev_args(n);
         a.out_keys = n->ev_list[cur];
         a.n_out = n->ev_ctr + cur;
+        if (n->plastic)
+            a.out_vals = n->ev_vals[cur];  // POISSON inputs carry no payload
         HIP_TRY(ev_launch(a, kEvPoisson, 0, n->stream));
         uint32_t c[3];
         HIP_TRY(hipMemcpyAsync(c, n->ev_ctr, sizeof c, hipMemcpyDeviceToHost, n->stream));
@@ -1412,9 +1469,17 @@ static int ev_step(naer_net* n, uint32_t epochs)
                         n->ev_in_cap);
         const uint32_t n_in = c[cur];
         bool alt = false;
-        HIP_TRY(ev_radix_sort(n->ev_list[cur], n->ev_sorted, nullptr, nullptr, n_in, 0, 32, n->ev_sort_ws, &alt,
-                              n->stream));
+        uint32_t* const in_v[1] = { n->ev_vals[cur] };
+        uint32_t* const in_v_alt[1] = { n->ev_vals_s };
+        HIP_TRY(ev_radix_sort(n->ev_list[cur], n->ev_sorted, in_v, in_v_alt, n->plastic ? 1u : 0u, n_in, 0, 32,
+                              n->ev_sort_ws, &alt, n->stream));
         a.in_keys = alt ? n->ev_sorted : n->ev_list[cur];
+        a.out_vals = nullptr;
+        if (n->plastic) {
+            a.in_vals = alt ? n->ev_vals_s : n->ev_vals[cur];
+            a.in_prev = n->ev_prev[cur];
+            a.in_trace = n->ev_ptr[cur];
+        }
         HIP_TRY(ev_launch(a, kEvRowLen, n_in, n->stream));
         HIP_TRY(ev_scan_u64(n->ev_len, n->ev_off, n_in + 1, n->ev_scan_ws, n->stream));
         unsigned long long n_ev = 0;
@@ -1425,13 +1490,23 @@ static int ev_step(naer_net* n, uint32_t epochs)
                         (unsigned long long)n->ev_e_cap);
         a.ev_keys = n->ev_keys;
         a.ev_w = n->ev_w;
+        a.ev_syn = n->ev_syn;
+        a.ev_slot = n->ev_slot;
         HIP_TRY(ev_launch(a, kEvExpand, n_in, n->stream));
-        HIP_TRY(ev_radix_sort(n->ev_keys, n->ev_keys_s, reinterpret_cast<uint32_t*>(n->ev_w),
-                              reinterpret_cast<uint32_t*>(n->ev_w_s), (uint32_t)n_ev, 11, 11 + tgt_bits,
-                              n->ev_sort_ws, &alt, n->stream));
+        uint32_t* const ev_v[3] = { reinterpret_cast<uint32_t*>(n->ev_w), n->ev_syn, n->ev_slot };
+        uint32_t* const ev_v_alt[3] = { reinterpret_cast<uint32_t*>(n->ev_w_s), n->ev_syn_s, n->ev_slot_s };
+        HIP_TRY(ev_radix_sort(n->ev_keys, n->ev_keys_s, ev_v, ev_v_alt, n->plastic ? 3u : 1u, (uint32_t)n_ev, 11,
+                              11 + tgt_bits, n->ev_sort_ws, &alt, n->stream));
         EvArgs b = a;
         b.ev_keys = alt ? n->ev_keys_s : n->ev_keys;
         b.ev_w = alt ? n->ev_w_s : n->ev_w;
+        b.ev_syn = alt ? n->ev_syn_s : n->ev_syn;
+        b.ev_slot = alt ? n->ev_slot_s : n->ev_slot;
+        if (n->plastic) {
+            b.out_vals = n->ev_vals[cur ^ 1u];
+            b.out_prev = n->ev_prev[cur ^ 1u];
+            b.out_trace = n->ev_ptr[cur ^ 1u];
+        }
         b.out_keys = n->ev_list[cur ^ 1u];
         b.n_out = n->ev_ctr + (cur ^ 1u);
         b.n_ev = (uint32_t)n_ev;
@@ -2711,7 +2786,9 @@ int naer_net_read_state(naer_net* n, float* v, int32_t* refr, float* pre, float*
         std::memset(pre, 0, N * sizeof(float));
     if (post)
         std::memset(post, 0, N * sizeof(float));
-    if ((pre || post) && n->plastic) {
+    if (post && n->plastic && n->event) {
+        HIP_TRY(hipMemcpy(post, n->ev_trace, N * sizeof(float), hipMemcpyDeviceToHost));
+    } else if ((pre || post) && n->plastic) {
         // the traces the next step sees: (a_s + 1) dec^(clock - s) from the last spike s
         std::vector<uint4> lat(N), pst(2 * N);
         DevState st;

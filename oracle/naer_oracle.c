@@ -1117,15 +1117,89 @@ void orc_net_read_probe(const orc_net* n, float* used, double* exact, double* ab
 #define EV_WINDOW 1500u
 #define EV_LO 4096u
 
+/* Plastic connections (STDP population onto STDP population) follow
+ * stdp.cl:8-140 with basic_stdp.cl:4-27's rule and constants.  A neuron of a
+ * population with a plastic connection keeps a post trace (stdp_traces.stdp,
+ * from 0) and ten 16-bit inter-spike entries in 4 us units (stdp_group.cl:
+ * 4-16, from 0).  Per input at t (in the order above), with
+ * last = t_last - 4 hist[0] (uint):
+ *   if the synapse is plastic (update_synapse, stdp.cl:8-30, in its types):
+ *     post = (int)last; for h = 1..9 while post > (int)prev and hist[h] > 0:
+ *       pre = e^((float)(uint)(prev - post) / 20000) * pre_trace + 1,
+ *       w += 0.01 (1 - w) e^(-pre / 100) (in double, rounded to float),
+ *       post -= 4 hist[h];
+ *     trace = e^(-(float)(uint)(t - last) / 20000) trace when that float is > 0;
+ *     w -= (double)2.02f * 0.01 * w * e^(-trace / 100) (double, to float);
+ *     w = max(min(w, 0.0003), -0.0003); the synapse keeps w
+ *   the neuron step above with w;
+ *   hist[0] = (uint16)((t - last) / 4);
+ *   on a spike: hist[h] = hist[h - 1] for h = 9..1, hist[0] = 0, trace += 1,
+ *   and the spike carries prev = last and its trace, which the target side
+ *   reads as an integer (stdp.cl:84 holds it in a uint) for pre_trace.
+ * prev and pre_trace belong to the input spike.  The exponentials are
+ * orc_expf's.  (For times below 2^31 us the unsigned difference of a later
+ * post time wraps, e^(...) overflows to +inf and the LTP increment is 0.) */
+#define EV_HIST 10u
+#define EV_HIST_UNIT 4u
+
 typedef struct {
     uint32_t src, time;
+    uint32_t prev; /* plastic source: its last spike time before this spike */
+    float trace;   /* plastic source: its trace after the spike */
 } ev_input;
 
 typedef struct {
     uint32_t tgt, time;
     uint64_t seq;
     float w;
+    uint64_t r; /* synapse */
+    uint32_t prev;
+    float trace;
 } ev_event;
+
+/* e^x in single precision with IEEE single operations only: x = k ln2 + r,
+ * k = floor(x log2(e) + 1/2), ln2 as the 9-bit head 0.693359375 plus its tail
+ * (Cody and Waite), e^r by its degree-7 Taylor polynomial (Horner), scaled by
+ * 2^k (ldexpf).  NaN propagates, x > 88.72283935546875 gives +inf, x <
+ * -103.972084045410156 gives 0.  Within 2 ulp of e^x. */
+float orc_expf(float x)
+{
+    if (x != x)
+        return x;
+    if (x > 88.72283935546875f)
+        return INFINITY;
+    if (x < -103.972084045410156f)
+        return 0.0f;
+    const float kf = floorf(x * 1.44269502162933349609375f + 0.5f);
+    const float r = (x - kf * 0.693359375f) - kf * -2.12194440e-4f;
+    float p = 1.0f / 5040.0f;
+    p = p * r + 1.0f / 720.0f;
+    p = p * r + 1.0f / 120.0f;
+    p = p * r + 1.0f / 24.0f;
+    p = p * r + 1.0f / 6.0f;
+    p = p * r + 0.5f;
+    p = p * r + 1.0f;
+    p = p * r + 1.0f;
+    return ldexpf(p, (int)kf);
+}
+
+/* update_synapse (stdp.cl:8-30) of one plastic input */
+static float ev_stdp_update(float w, const uint16_t* hist, uint32_t last, uint32_t t, uint32_t prev,
+                            float pre_trace, float* trace)
+{
+    int32_t post = (int32_t)last;
+    for (uint32_t h = 1; h < EV_HIST && post > (int32_t)prev && hist[h] > 0; ++h) {
+        const float pre = orc_expf((float)(prev - (uint32_t)post) / 20000.0f) * pre_trace + 1.0f;
+        const double inc = 0.01 * (double)(1.0f - w) * (double)orc_expf(-pre / 100.0f);
+        w = (float)((double)w + inc);
+        post -= (int32_t)(hist[h] * EV_HIST_UNIT);
+    }
+    const float dt = (float)(t - last);
+    *trace = dt > 0.0f ? orc_expf(-dt / 20000.0f) * *trace : *trace;
+    const double dec = (double)2.02f * 0.01 * (double)w * (double)orc_expf(-*trace / 100.0f);
+    w = (float)((double)w - dec);
+    return fmaxf(fminf(w, 0.0003f), -0.0003f);
+}
 
 static int cmp_ev_input(const void* a, const void* b)
 {
@@ -1164,6 +1238,9 @@ void orc_ev_tables(double tau_us, double* lo, double* hi, uint32_t cap_hi, uint3
 typedef struct {
     float* V;
     uint32_t* t_last;
+    float* trace;      /* plastic neurons: post trace */
+    uint16_t* hist;    /* [N][EV_HIST] */
+    uint8_t* plastic;  /* per population: has a plastic connection */
     ev_input* in;
     uint64_t n_in, cap_in;
     double* lo; /* EV_LO per population */
@@ -1181,6 +1258,9 @@ void orc_ev_free(void* p)
         return;
     free(e->V);
     free(e->t_last);
+    free(e->trace);
+    free(e->hist);
+    free(e->plastic);
     free(e->in);
     free(e->lo);
     free(e->hi);
@@ -1188,7 +1268,7 @@ void orc_ev_free(void* p)
     free(e);
 }
 
-static void ev_push(ev_state* e, uint32_t src, uint32_t time)
+static void ev_push(ev_state* e, uint32_t src, uint32_t time, uint32_t prev, float trace)
 {
     if (e->n_in == e->cap_in) {
         e->cap_in = e->cap_in ? 2 * e->cap_in : 1024;
@@ -1196,6 +1276,8 @@ static void ev_push(ev_state* e, uint32_t src, uint32_t time)
     }
     e->in[e->n_in].src = src;
     e->in[e->n_in].time = time;
+    e->in[e->n_in].prev = prev;
+    e->in[e->n_in].trace = trace;
     e->n_in++;
 }
 
@@ -1225,6 +1307,12 @@ int orc_ev_run(orc_net* n, void** state, uint32_t epochs)
         e = (ev_state*)xcalloc(1, sizeof(ev_state));
         e->V = (float*)xcalloc(n->N, sizeof(float));
         e->t_last = (uint32_t*)xcalloc(n->N, sizeof(uint32_t));
+        e->trace = (float*)xcalloc(n->N, sizeof(float));
+        e->hist = (uint16_t*)xcalloc((size_t)n->N * EV_HIST, sizeof(uint16_t));
+        e->plastic = (uint8_t*)xcalloc(n->desc.n_pops ? n->desc.n_pops : 1, 1);
+        for (uint32_t c = 0; c < n->desc.n_conns; ++c)
+            if (n->conns[c].plastic)
+                e->plastic[n->conns[c].src_pop] = e->plastic[n->conns[c].dst_pop] = 1;
         memcpy(e->V, n->V, n->N * sizeof(float));
         const uint32_t np = n->desc.n_pops;
         e->lo = (double*)xcalloc((size_t)np * EV_LO, sizeof(double));
@@ -1246,7 +1334,7 @@ int orc_ev_run(orc_net* n, void** state, uint32_t epochs)
             int m = orc_ref_poisson_epoch(n->pois_idx[j], n->n_pois, n->desc.poisson_seed, q->src_rate_hz, t0, t1,
                                           times);
             for (int i = 0; i < m; ++i) {
-                ev_push(e, j, times[i]);
+                ev_push(e, j, times[i], 0, 0.0f);
                 ev_record(n, e->epoch, j, times[i]);
             }
         }
@@ -1263,6 +1351,9 @@ int orc_ev_run(orc_net* n, void** state, uint32_t epochs)
                 ev[c].time = e->in[i].time;
                 ev[c].seq = c;
                 ev[c].w = n->w[r];
+                ev[c].r = r;
+                ev[c].prev = e->in[i].prev;
+                ev[c].trace = e->in[i].trace;
             }
         }
         qsort(ev, n_ev, sizeof(ev_event), cmp_ev_event);
@@ -1272,28 +1363,65 @@ int orc_ev_run(orc_net* n, void** state, uint32_t epochs)
             const uint32_t j = ev[i].tgt, t = ev[i].time;
             const uint32_t p = n->pop_of[j];
             const naer_pop_desc* q = &n->pops[p];
+            const int pl = e->plastic[p];
+            uint16_t* hist = e->hist + (size_t)j * EV_HIST;
+            float w = ev[i].w;
+            uint32_t last = 0;
+            if (pl) {
+                last = e->t_last[j] - (uint32_t)hist[0] * EV_HIST_UNIT;
+                if (n->col[ev[i].r] & 0x80000000u) {
+                    /* stdp.cl:84: the pre trace passes through a uint */
+                    const float pre_trace = (float)(uint32_t)ev[i].trace;
+                    w = ev_stdp_update(w, hist, last, t, ev[i].prev, pre_trace, &e->trace[j]);
+                    n->w[ev[i].r] = w;
+                }
+            }
+            int spiked = 0;
             const int64_t dt = (int64_t)t - (int64_t)e->t_last[j];
-            if (dt < 0)
-                continue;
-            const uint64_t h = (uint64_t)dt >> 12;
-            const double d = h < e->n_hi[p] ? e->hi[(size_t)p * EV_HI_CAP + h] * e->lo[(size_t)p * EV_LO + (dt & 4095)]
-                                            : 0.0;
-            float x = e->V[j] - q->v_rest;
-            x = (float)((double)x * d);
-            float v = x + q->v_rest;
-            v = v + ev[i].w;
-            e->t_last[j] = t;
-            if (v > q->v_th) {
-                v = q->v_reset;
-                e->t_last[j] = t + (uint32_t)llround((double)q->t_ref_ms * 1000.0);
-                ev_push(e, j, t + EV_WINDOW);
+            if (dt >= 0) {
+                const uint64_t h = (uint64_t)dt >> 12;
+                const double d = h < e->n_hi[p]
+                                     ? e->hi[(size_t)p * EV_HI_CAP + h] * e->lo[(size_t)p * EV_LO + (dt & 4095)]
+                                     : 0.0;
+                float x = e->V[j] - q->v_rest;
+                x = (float)((double)x * d);
+                float v = x + q->v_rest;
+                v = v + w;
+                e->t_last[j] = t;
+                if (v > q->v_th) {
+                    v = q->v_reset;
+                    e->t_last[j] = t + (uint32_t)llround((double)q->t_ref_ms * 1000.0);
+                    spiked = 1;
+                }
+                e->V[j] = v;
+            }
+            if (pl) {
+                hist[0] = (uint16_t)((t - last) / EV_HIST_UNIT);
+                if (spiked) {
+                    for (uint32_t h = EV_HIST - 1; h > 0; --h)
+                        hist[h] = hist[h - 1];
+                    hist[0] = 0;
+                    e->trace[j] += 1.0f;
+                }
+            }
+            if (spiked) {
+                ev_push(e, j, t + EV_WINDOW, last, pl ? e->trace[j] : 0.0f);
                 ev_record(n, e->epoch, j, t);
             }
-            e->V[j] = v;
         }
         free(ev);
     }
     return 0;
+}
+
+/* plastic neurons' post traces and inter-spike histories ([N][10]) */
+void orc_ev_read_stdp(void* state, uint32_t n_neurons, float* trace, uint16_t* hist)
+{
+    ev_state* e = (ev_state*)state;
+    if (trace)
+        memcpy(trace, e->trace, n_neurons * sizeof(float));
+    if (hist)
+        memcpy(hist, e->hist, (size_t)n_neurons * EV_HIST * sizeof(uint16_t));
 }
 
 void orc_ev_read_state(void* state, uint32_t n_neurons, float* v, uint32_t* t_last)

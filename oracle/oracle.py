@@ -56,11 +56,19 @@ def lib():
         L.orc_ev_run.argtypes = [v, ctypes.POINTER(ctypes.c_void_p), u32]
         L.orc_ev_run.restype = ctypes.c_int
         L.orc_ev_read_state.argtypes = [v, u32, p, p]
+        L.orc_ev_read_stdp.argtypes = [v, u32, p, p]
+        L.orc_expf.argtypes = [ctypes.c_float]
+        L.orc_expf.restype = ctypes.c_float
         L.orc_ev_free.argtypes = [v]
         L.orc_ev_tables.argtypes = [ctypes.c_double, p, p, u32, ctypes.POINTER(u32)]
         L.orc_set_conn_mutation.argtypes = [u32]
         _lib = L
     return _lib
+
+
+def expf(x):
+    """orc_expf: the single-precision e^x of the event-driven STDP."""
+    return float(lib().orc_expf(float(x)))
 
 
 # Planted mistakes of the fixed in-degree rule (naer_oracle.h), for tests
@@ -198,10 +206,15 @@ class OracleNet:
             raise RuntimeError("oracle: " + lib().orc_error().decode())
 
     def event_state(self):
+        """Potentials, last input times and, for neurons of populations with
+        a plastic connection, the post trace and inter-spike history."""
         v = np.zeros(self.N, np.float32)
         t_last = np.zeros(self.N, np.uint32)
+        trace = np.zeros(self.N, np.float32)
+        hist = np.zeros((self.N, 10), np.uint16)
         lib().orc_ev_read_state(self._ev, self.N, v.ctypes.data, t_last.ctypes.data)
-        return dict(v=v, t_last=t_last)
+        lib().orc_ev_read_stdp(self._ev, self.N, trace.ctypes.data, hist.ctypes.data)
+        return dict(v=v, t_last=t_last, trace=trace, hist=hist)
 
     def close(self):
         if getattr(self, "_ev", None):
