@@ -14,9 +14,9 @@
 //                  order, i.e. by (time, source, row position)
 //   (stable radix sort of the events by target alone: each target's events
 //    keep the order (time, source, row position))
-//   k_ev_process   a group of 8 lanes per LIF neuron finds its range of sorted
-//                  events (8-ary search), applies them in order and appends
-//                  its spikes to the next window's input list
+//   k_ev_bounds    each target's run of sorted events
+//   k_ev_process   a lane per LIF neuron applies its events in order and
+//                  appends its spikes to the next window's input list
 // These replace histogram_scan / scatter / distribute_offsets
 // (spike_aggregation.cl:3-65), which order a window's spikes by time.  The
 // sorts and the scan are this file's own: an LSD radix sort of 32-bit keys
@@ -150,49 +150,46 @@ __device__ __forceinline__ double ev_decay(const double* H, const double* L, uin
     return h < n_hi ? H[h] * L[dt & 4095] : 0.0;
 }
 
-// lif.cl:15-34 per event, in time order; the decay e^(-dt/tau) is H[dt >> 12]
-// * L[dt & 4095] from the population's tables.  A target per group of 8
-// lanes (8 per wave): the group loads 8 events at a time (coalesced) and
-// looks up each event's decay from the previous event's time in parallel; the
-// step itself runs in order on values shuffled within the group, and looks
-// the decay up again only when the neuron's t_last is not the previous
-// event's time (after a spike or a dropped input).
-constexpr uint32_t kEvGroup = 8;
-static_assert(kEvGroup >= 2 && kEvGroup < 32 && (kEvGroup & (kEvGroup - 1)) == 0,
-              "a step group is a power of two of lanes below 32 (its ballot mask is 32-bit)");
-
-// First index of keys[0, n) whose target (key >> 11) is >= tgt (n if none),
-// found by the kEvGroup lanes of a group together: each round probes 8 points
-// of the remaining range.  Every lane of the wave takes part (ballots).
-__device__ __forceinline__ uint32_t group_lower_bound(const uint32_t* keys, uint32_t n, uint32_t tgt, uint32_t gl)
+// Boundaries of each target's run of sorted events: begin[t] and end[t] of
+// the targets that have events (the step kernel clears them after use, so
+// targets without events read an empty range).
+// A thread per aligned quad of keys (the key arrays hold whole quads).
+__global__ __launch_bounds__(256) void k_ev_bounds(const EvArgs a)
 {
-    const uint32_t base = (threadIdx.x & 63u) & ~(kEvGroup - 1);
-    uint32_t lo = 0, hi = n;  // keys[< lo] are below tgt; hi == n or keys[hi] is not
-    for (;;) {
-        const bool busy = hi - lo > kEvGroup;
-        if (!__any(busy))
+    const uint32_t i0 = (blockIdx.x * 256 + threadIdx.x) * 4;
+    if (i0 >= a.n_ev)
+        return;
+    const uint4 k = *reinterpret_cast<const uint4*>(a.ev_keys + i0);
+    const uint32_t before = i0 ? a.ev_keys[i0 - 1] >> 11 : 0xFFFFFFFFu;
+    const uint32_t after = i0 + 4 < a.n_ev ? a.ev_keys[i0 + 4] >> 11 : 0xFFFFFFFFu;
+    const uint32_t t[6] = { before, k.x >> 11, k.y >> 11, k.z >> 11, k.w >> 11, after };
+#pragma unroll
+    for (uint32_t m = 0; m < 4; ++m) {
+        const uint32_t i = i0 + m;
+        if (i >= a.n_ev)
             break;
-        const unsigned long long span = hi - lo;
-        // probes q(g) = lo + (g + 1) span / 9, strictly increasing and < hi
-        auto q = [&](uint32_t g) { return lo + (uint32_t)((g + 1) * span / (kEvGroup + 1)); };
-        const bool ge = busy && (keys[q(gl)] >> 11) >= tgt;
-        const uint32_t bits = (uint32_t)(__ballot(ge) >> base) & ((1u << kEvGroup) - 1);
-        if (busy) {
-            if (bits == 0) {
-                lo = q(kEvGroup - 1) + 1;
-            } else {
-                const uint32_t f = (uint32_t)__ffs(bits) - 1;
-                const uint32_t nlo = f ? q(f - 1) + 1 : lo;
-                hi = q(f);
-                lo = nlo;
-            }
-        }
+        const uint32_t nxt = i + 1 < a.n_ev ? t[m + 2] : 0xFFFFFFFFu;
+        if (t[m] != t[m + 1])
+            a.ev_begin[t[m + 1]] = i;
+        if (nxt != t[m + 1])
+            a.ev_end[t[m + 1]] = i + 1;
     }
-    const uint32_t i = lo + gl;
-    const bool ge = i < hi && (keys[i] >> 11) >= tgt;
-    const uint32_t bits = (uint32_t)(__ballot(ge) >> base) & ((1u << kEvGroup) - 1);
-    return bits ? lo + (uint32_t)__ffs(bits) - 1 : hi;
 }
+
+// lif.cl:15-34 per event, in time order; the decay e^(-dt/tau) is H[dt >> 12]
+// * L[dt & 4095] from the population's tables.  One lane per target walks its
+// run of sorted events in batches of aligned quads (16-byte loads) through a
+// three-stage pipeline: batch b + 2 is loaded, batch b + 1 is decoded (event
+// times, each event's predecessor, the decay from it and, plastic, its
+// input's payload) and batch b is stepped.  Loads are unconditional (indices
+// clamped into range) and their results used an iteration later, so none is
+// waited for early.  A block whose LIF targets share one population keeps
+// its L table and the head of its H table in LDS.  The step looks the decay
+// up again only when t_last is not the previous event's time (after a spike
+// or a dropped input).
+constexpr uint32_t kEvHiLds = 64;    // H entries kept in LDS (dt < 2^18 us)
+constexpr uint32_t kEvStepThreads = 64;  // one wave per block: the waves spread over every CU
+constexpr uint32_t kEvSpkLane = 32;       // spikes a lane buffers in LDS (two batches' worth)
 
 // stdp.cl:8-30 for one plastic event (the reference's update_synapse with
 // basic_stdp.cl's rule), in the reference's integer and float types:
@@ -203,18 +200,29 @@ __device__ __forceinline__ uint32_t group_lower_bound(const uint32_t* keys, uint
 //   the post trace decays from the last spike time (uint difference; a
 //   negative one wraps and clears the trace); LTD with alpha * lambda in
 //   double; hard bounds +-W_MAX.
+// The LTP sum adds exactly 0 while every time is below kEvLtpZeroBelow: a
+// step runs only when post > prev as ints, and then (uint)(prev - post) is
+// at least 2^31 - 2^21 (both real times: the difference wraps; either one a
+// wrapped "last" >= 2^32 - 2^18 * 10: it is at least that far from any real
+// time), so e^((float)diff / 20000) overflows to +inf, pre = +inf (pre_trace
+// >= 1: it is a trace just after its +1), e^(-pre / 100) = 0 and w + 0 = w.
+constexpr uint32_t kEvLtpZeroBelow = (1u << 31) - (1u << 21);
+
 __device__ __forceinline__ float ev_stdp_update(float w, const uint32_t (&hh)[kEvHist], uint32_t last,
-                                                uint32_t t, uint32_t prev, float pre_trace, float& trace)
+                                                uint32_t t, uint32_t prev, float pre_trace, float& trace,
+                                                bool ltp_zero)
 {
     int32_t post = (int32_t)last;
+    if (!ltp_zero) {
 #pragma unroll
-    for (uint32_t h = 1; h < kEvHist; ++h) {
-        if (!(post > (int32_t)prev && hh[h] > 0u))
-            break;
-        const float pre = naer_expf((float)(prev - (uint32_t)post) / kEvStdpTau) * pre_trace + 1.0f;
-        const double inc = kEvStdpLambda * (double)(1.0f - w) * (double)naer_expf(-pre / kEvStdpConstDt);
-        w = (float)((double)w + inc);
-        post -= (int32_t)(hh[h] * kEvHistUnit);
+        for (uint32_t h = 1; h < kEvHist; ++h) {
+            if (!(post > (int32_t)prev && hh[h] > 0u))
+                break;
+            const float pre = naer_expf((float)(prev - (uint32_t)post) / kEvStdpTau) * pre_trace + 1.0f;
+            const double inc = kEvStdpLambda * (double)(1.0f - w) * (double)naer_expf(-pre / kEvStdpConstDt);
+            w = (float)((double)w + inc);
+            post -= (int32_t)(hh[h] * kEvHistUnit);
+        }
     }
     const float dt = (float)(t - last);
     trace = dt > 0.0f ? naer_expf(-dt / kEvStdpTau) * trace : trace;
@@ -231,28 +239,63 @@ __device__ __forceinline__ float ev_stdp_update(float w, const uint32_t (&hh)[kE
 // {prev = last, trace} with the spike.  Its other inputs (static synapses)
 // step and keep the history the same way.
 template <bool PL>
-__global__ __launch_bounds__(256) void k_ev_process(const EvArgs a)
+__global__ __launch_bounds__(kEvStepThreads) void k_ev_process(const EvArgs a)
 {
-    const uint32_t gl = threadIdx.x & (kEvGroup - 1);
-    const uint32_t j = blockIdx.x * (256 / kEvGroup) + threadIdx.x / kEvGroup;
-    uint32_t p = 0;
-    // the target's range of sorted events (groups past N search nothing)
-    const uint32_t n_ev = j < a.N ? a.n_ev : 0u;
-    uint32_t lo = group_lower_bound(a.ev_keys, n_ev, j, gl);
-    uint32_t hi = group_lower_bound(a.ev_keys, n_ev, j + 1, gl);
+    constexpr uint32_t QB = PL ? 2 : 4;  // quads per batch
+    constexpr uint32_t EB = 4 * QB;      // events per batch
+    __shared__ double s_lo[kEvLo];
+    __shared__ double s_hi[kEvHiLds];
+    __shared__ uint32_t s_pop[2];
+    // each lane's spikes {time; plastic: last, trace}, written out together
+    // after the stepping (a global atomic per spike inside the serial loop
+    // would stall the wave and drain its prefetched loads).  Every step
+    // stores its candidate entry at the lane's next slot and a spike keeps
+    // it (no branch); a lane that might fill its slots in the next batch
+    // sends them out first.
+    __shared__ uint32_t s_spk_t[kEvSpkLane][kEvStepThreads];
+    __shared__ uint32_t s_spk_last[PL ? kEvSpkLane : 1][kEvStepThreads];
+    __shared__ float s_spk_tr[PL ? kEvSpkLane : 1][kEvStepThreads];
+    __shared__ unsigned long long s_base[2];
+    const uint32_t j = blockIdx.x * kEvStepThreads + threadIdx.x;
+    uint32_t lo = 0, hi = 0, p = 0;
     if (j < a.N) {
+        lo = a.ev_begin[j];
+        hi = a.ev_end[j];
         if (lo != hi) {
+            a.ev_begin[j] = 0;  // ready for the next window
+            a.ev_end[j] = 0;
             p = ev_pop(a, j);
             const PopParams& pp = a.pops[p];
             if (pp.kind != NAER_POP_LIF || j < pp.begin || j >= pp.end)
                 hi = lo;
         }
     }
-    if (!__any(lo != hi))
-        return;
+    // one population for the block's stepping lanes: its tables go to LDS
+    if (threadIdx.x == 0) {
+        s_pop[0] = 0xFFFFFFFFu;
+        s_pop[1] = 0;
+    }
+    __syncthreads();
+    if (lo != hi) {
+        atomicMin(&s_pop[0], p);
+        atomicMax(&s_pop[1], p);
+    }
+    __syncthreads();
+    if (s_pop[0] == 0xFFFFFFFFu)
+        return;  // nothing to step in this block
+    const bool lds = s_pop[0] == s_pop[1];
+    if (lds) {
+        const uint32_t q = s_pop[0];
+        const uint32_t nh = min(a.n_hi[q], kEvHiLds);
+        for (uint32_t x = threadIdx.x; x < kEvLo; x += kEvStepThreads)
+            s_lo[x] = a.dec_lo[(size_t)q * kEvLo + x];
+        for (uint32_t x = threadIdx.x; x < kEvHiLds; x += kEvStepThreads)
+            s_hi[x] = x < nh ? a.dec_hi[(size_t)q * kEvHiCap + x] : 0.0;
+    }
+    __syncthreads();
     const double* L = a.dec_lo + (size_t)p * kEvLo;
     const double* H = a.dec_hi + (size_t)p * kEvHiCap;
-    uint32_t n_hi = 0, t_ref = 0, t_last = 0;
+    uint32_t n_hi = 1, t_ref = 0, t_last = 0;
     float v_rest = 0.0f, v_th = 0.0f, v_reset = 0.0f, V = 0.0f;
     bool plastic = false;
     float trace = 0.0f;
@@ -279,133 +322,157 @@ __global__ __launch_bounds__(256) void k_ev_process(const EvArgs a)
             }
         }
     }
-    // Software pipeline over chunks of kEvGroup events: while chunk c is
-    // stepped, chunk c + 1's decays (and, plastic, its inputs' payloads) are
-    // looked up and chunk c + 2's events are loaded.
-    struct Ev {
-        uint32_t t;
-        float w;
-        uint32_t syn, slot;
+    // e^(-dt/tau) for dt >= 0 (0 past the tables), from LDS where it can
+    auto decay = [&](uint32_t dt) -> double {
+        const uint32_t h = dt >> 12;
+        if (h >= n_hi)
+            return 0.0;
+        if (lds && h < kEvHiLds)
+            return s_hi[h] * s_lo[dt & 4095];
+        return H[h] * L[dt & 4095];
     };
-    auto load = [&](uint32_t c, Ev& e) {
-        const uint32_t i = c + gl;
-        e.t = 0;
-        e.w = 0.0f;
-        e.syn = 0;
-        e.slot = 0;
-        if (i < hi) {
-            e.t = a.t0 + (a.ev_keys[i] & 2047u);
-            e.w = a.ev_w[i];
-            if (PL && plastic) {
-                e.syn = a.ev_syn[i];
-                e.slot = a.ev_slot[i];
-            }
-        }
+    // batches start at the aligned quad holding lo; event e of the lane is
+    // valid when lo <= e < hi
+    const uint32_t first = lo & ~3u;
+    struct Raw {
+        uint4 key[QB];
+        float4 w[QB];
+        uint4 syn[QB], slot[QB];
     };
-    // the decay of each event of chunk c from the previous event's time
-    // (t_before: the last event of the chunk before)
-    auto lookup = [&](uint32_t c, const Ev& e, uint32_t t_before, uint32_t& tp, double& d, uint32_t& prev,
-                      float& ptr) {
-        tp = (uint32_t)__shfl_up((int)e.t, 1, kEvGroup);
-        if (gl == 0)
-            tp = t_before;
-        const long long dtp = (long long)e.t - (long long)tp;
-        d = (c + gl < hi && dtp >= 0) ? ev_decay(H, L, n_hi, dtp) : 0.0;
-        prev = 0;
-        ptr = 0.0f;
-        if (PL && c + gl < hi && (e.syn & kPlasticBit)) {
-            prev = a.in_prev[e.slot];
-            // stdp.cl:84 holds the pre trace in a uint
-            ptr = (float)(uint32_t)a.in_trace[e.slot];
-        }
+    struct Dec {
+        uint32_t t[EB], tp[EB], syn[EB], slot[EB];
+        float w[EB];
+        double d[EB];
     };
-    Ev e0, e1;
-    uint32_t tp0, prev0 = 0;
-    float ptr0 = 0.0f;
-    double d0;
-    load(lo, e0);
-    lookup(lo, e0, t_last, tp0, d0, prev0, ptr0);
-    load(lo + kEvGroup, e1);
-    uint32_t fired = 0;
-    for (uint32_t c = lo;; c += kEvGroup) {
-        const bool more = c < hi;
-        if (!__any(more))
-            break;
-        const uint32_t n = more ? min(kEvGroup, hi - c) : 0u;
-        // next chunk's decays start from this chunk's last event
-        const uint32_t t_end = (uint32_t)__shfl((int)e0.t, (int)(n ? n - 1 : 0), kEvGroup);
-        uint32_t tp1, prev1;
-        float ptr1;
-        double d1;
-        lookup(c + kEvGroup, e1, t_end, tp1, d1, prev1, ptr1);
-        Ev e2;
-        load(c + 2 * kEvGroup, e2);
-        for (uint32_t k = 0; k < kEvGroup; ++k) {
-            // every lane takes part in the shuffles; the step is predicated
-            const uint32_t tk = (uint32_t)__shfl((int)e0.t, (int)k, kEvGroup);
-            const uint32_t tpk = (uint32_t)__shfl((int)tp0, (int)k, kEvGroup);
-            float wk = __shfl(e0.w, (int)k, kEvGroup);
-            const double dpk = __shfl(d0, (int)k, kEvGroup);
-            uint32_t synk = 0, prevk = 0;
-            float ptrk = 0.0f;
-            if (PL) {
-                synk = (uint32_t)__shfl((int)e0.syn, (int)k, kEvGroup);
-                prevk = (uint32_t)__shfl((int)prev0, (int)k, kEvGroup);
-                ptrk = __shfl(ptr0, (int)k, kEvGroup);
-            }
-            if (k < n) {
-                uint32_t last = 0;
-                if (PL && plastic) {
-                    last = t_last - hh[0] * kEvHistUnit;
-                    if (synk & kPlasticBit) {
-                        wk = ev_stdp_update(wk, hh, last, tk, prevk, ptrk, trace);
-                        if (gl == 0)
-                            a.w_mut[synk & ~kPlasticBit] = wk;
-                    }
-                }
-                bool spiked = false;
-                const long long dt = (long long)tk - (long long)t_last;
-                if (dt >= 0) {
-                    const double dk = tpk == t_last ? dpk : ev_decay(H, L, n_hi, dt);
-                    float x = V - v_rest;
-                    x = (float)((double)x * dk);
-                    float v = x + v_rest;
-                    v = v + wk;
-                    t_last = tk;
-                    if (v > v_th) {
-                        v = v_reset;
-                        t_last = tk + t_ref;
-                        spiked = true;
-                    }
-                    V = v;
-                }
-                if (PL && plastic) {
-                    hh[0] = ((tk - last) / kEvHistUnit) & 0xFFFFu;
-                    if (spiked) {
+    auto load = [&](uint32_t c, Raw& r) {
 #pragma unroll
-                        for (uint32_t h = kEvHist - 1; h > 0; --h)
-                            hh[h] = hh[h - 1];
-                        hh[0] = 0;
-                        trace += 1.0f;
-                    }
-                }
-                if (spiked) {
-                    if (gl == 0) {
-                        ev_append(a, j, tk + kEpochUs, a.t0 + kEpochUs, PL && plastic, last, trace);
-                        ev_record(a, j, tk);
-                    }
-                    ++fired;
-                }
+        for (uint32_t q = 0; q < QB; ++q) {
+            // past the lane's run: reread its first quad (in range, unused)
+            const uint32_t e = c + 4 * q < hi ? c + 4 * q : first;
+            r.key[q] = *reinterpret_cast<const uint4*>(a.ev_keys + e);
+            r.w[q] = *reinterpret_cast<const float4*>(a.ev_w + e);
+            if (PL) {
+                r.syn[q] = *reinterpret_cast<const uint4*>(a.ev_syn + e);
+                r.slot[q] = *reinterpret_cast<const uint4*>(a.ev_slot + e);
             }
         }
-        e0 = e1;
-        tp0 = tp1;
-        d0 = d1;
-        prev0 = prev1;
-        ptr0 = ptr1;
-        e1 = e2;
+    };
+    auto pick = [](const uint4& v, uint32_t m) { return m == 0 ? v.x : m == 1 ? v.y : m == 2 ? v.z : v.w; };
+    auto pickf = [](const float4& v, uint32_t m) { return m == 0 ? v.x : m == 1 ? v.y : m == 2 ? v.z : v.w; };
+    // decode batch c: times, predecessors (t_before: the one before the
+    // batch), decays; returns the batch's last valid event time
+    auto decode = [&](uint32_t c, const Raw& r, uint32_t t_before, Dec& k) -> uint32_t {
+        uint32_t tp = t_before;
+#pragma unroll
+        for (uint32_t x = 0; x < EB; ++x) {
+            const uint32_t e = c + x;
+            const bool valid = e >= lo && e < hi;
+            const uint32_t t = valid ? a.t0 + (pick(r.key[x / 4], x % 4) & 2047u) : 0u;
+            k.t[x] = t;
+            k.w[x] = pickf(r.w[x / 4], x % 4);
+            k.tp[x] = tp;
+            k.d[x] = valid && t >= tp ? decay(t - tp) : 0.0;
+            k.syn[x] = PL && plastic && valid ? pick(r.syn[x / 4], x % 4) : 0u;
+            k.slot[x] = PL ? pick(r.slot[x / 4], x % 4) : 0u;
+            if (valid)
+                tp = t;
+        }
+        return tp;
+    };
+    const bool ltp_zero = a.t0 + 2 * kEpochUs <= kEvLtpZeroBelow;
+    Dec k0;
+    Raw r1;
+    uint32_t t_end;
+    {
+        Raw r0;
+        load(first, r0);
+        t_end = decode(first, r0, t_last, k0);
     }
-    if (gl == 0 && lo != hi) {
+    load(first + EB, r1);
+    uint32_t ns = 0;  // the lane's buffered spikes
+    unsigned long long n_spikes = 0;
+    auto send = [&](uint32_t m) {  // the lane's first m buffered spikes, one by one
+        for (uint32_t i = 0; i < m; ++i) {
+            const uint32_t t = s_spk_t[i][threadIdx.x];
+            ev_append(a, j, t + kEpochUs, a.t0 + kEpochUs, PL, PL ? s_spk_last[i][threadIdx.x] : 0u,
+                      PL ? s_spk_tr[i][threadIdx.x] : 0.0f);
+            ev_record(a, j, t);
+        }
+    };
+    for (uint32_t c = first;; c += EB) {
+        if (!__any(c < hi))
+            break;
+        if (ns > kEvSpkLane - EB) {  // rare: only short refractory periods fire this often
+            send(ns);
+            n_spikes += ns;
+            ns = 0;
+        }
+        Dec k1;
+        const uint32_t t_next = decode(c + EB, r1, t_end, k1);
+        Raw r2;
+        load(c + 2 * EB, r2);
+        // plastic: the payloads of this batch's plastic events
+        uint32_t prev[EB];
+        float ptr[EB];
+#pragma unroll
+        for (uint32_t x = 0; x < EB; ++x) {
+            prev[x] = 0;
+            ptr[x] = 0.0f;
+            if (PL && (k0.syn[x] & kPlasticBit)) {
+                const uint32_t sl = k0.slot[x] < a.in_cap ? k0.slot[x] : 0u;
+                prev[x] = a.in_prev[sl];
+                // stdp.cl:84 holds the pre trace in a uint
+                ptr[x] = (float)(uint32_t)a.in_trace[sl];
+            }
+        }
+#pragma unroll
+        for (uint32_t x = 0; x < EB; ++x) {
+            const bool valid = c + x >= lo && c + x < hi;
+            const uint32_t tk = k0.t[x];
+            float wk = k0.w[x];
+            uint32_t last = 0;
+            if (PL && plastic) {
+                last = t_last - hh[0] * kEvHistUnit;
+                if (valid && (k0.syn[x] & kPlasticBit)) {
+                    wk = ev_stdp_update(wk, hh, last, tk, prev[x], ptr[x], trace, ltp_zero);
+                    a.w_mut[k0.syn[x] & ~kPlasticBit] = wk;
+                }
+            }
+            // the step, computed on every lane and kept where it applies
+            // (refractory inputs are dropped)
+            const bool take = valid && tk >= t_last;
+            double dk = k0.d[x];
+            if (take && k0.tp[x] != t_last)  // after a spike or a dropped input
+                dk = decay(tk - t_last);
+            float xv = V - v_rest;
+            xv = (float)((double)xv * dk);
+            float v = xv + v_rest;
+            v = v + wk;
+            const bool spiked = take && v > v_th;
+            V = take ? (spiked ? v_reset : v) : V;
+            t_last = take ? (spiked ? tk + t_ref : tk) : t_last;
+            if (PL && plastic && valid) {
+                hh[0] = ((tk - last) / kEvHistUnit) & 0xFFFFu;
+                if (spiked) {
+#pragma unroll
+                    for (uint32_t h = kEvHist - 1; h > 0; --h)
+                        hh[h] = hh[h - 1];
+                    hh[0] = 0;
+                    trace += 1.0f;
+                }
+            }
+            s_spk_t[ns][threadIdx.x] = tk;
+            if (PL) {
+                s_spk_last[ns][threadIdx.x] = last;
+                s_spk_tr[ns][threadIdx.x] = trace;
+            }
+            ns += spiked ? 1u : 0u;
+        }
+        k0 = k1;
+        r1 = r2;
+        t_end = t_next;
+    }
+    if (lo != hi) {
         a.V[j] = V;
         a.t_last[j] = t_last;
         if (PL && plastic) {
@@ -414,8 +481,55 @@ __global__ __launch_bounds__(256) void k_ev_process(const EvArgs a)
             for (uint32_t h = 0; h < kEvHist / 2; ++h)
                 a.hist[(size_t)j * (kEvHist / 2) + h] = hh[2 * h] | hh[2 * h + 1] << 16;
         }
-        if (fired)
-            atomicAdd(a.spikes, (unsigned long long)fired);
+    }
+    // the buffered spikes: one reservation per block (one wave) in the next
+    // window's input list and in the record
+    n_spikes += ns;
+    uint32_t before = ns;  // inclusive prefix over the wave's lanes
+    for (uint32_t off = 1; off < kWave; off <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)before, off);
+        if (threadIdx.x >= off)
+            before += y;
+    }
+    const uint32_t total = (uint32_t)__shfl((int)before, kWave - 1);
+    before -= ns;
+    unsigned long long all_spikes = n_spikes;
+    for (uint32_t off = kWave / 2; off; off >>= 1)
+        all_spikes += __shfl_xor(all_spikes, off);
+    if (threadIdx.x == 0) {
+        if (total) {
+            s_base[0] = atomicAdd(a.n_out, total);
+            s_base[1] = a.rec_cap ? atomicAdd(a.rec_n, (unsigned long long)total) : 0ull;
+        }
+        if (all_spikes)
+            atomicAdd(a.spikes, all_spikes);
+    }
+    __syncthreads();
+    for (uint32_t q = 0; q < ns; ++q) {
+        const uint32_t t = s_spk_t[q][threadIdx.x];
+        const uint32_t i = before + q;
+        const unsigned long long k = s_base[0] + i;
+        if (k < a.in_cap) {
+            a.out_keys[k] = (t - a.t0) << kEvSrcBits | j;  // (t + 1500) - (t0 + 1500)
+            if (a.out_vals) {
+                a.out_vals[k] = (uint32_t)k;
+                a.out_prev[k] = PL ? s_spk_last[q][threadIdx.x] : 0u;
+                a.out_trace[k] = PL ? s_spk_tr[q][threadIdx.x] : 0.0f;
+            }
+        } else {
+            atomicAdd(a.lost, 1u);
+        }
+        if (a.rec_cap) {
+            const unsigned long long r = s_base[1] + i;
+            if (r < a.rec_cap) {
+                naer_spike sp;
+                sp.step = a.epoch;
+                sp.neuron = j;
+                sp.time_us = t;
+                sp._pad = 0;
+                a.rec[r] = sp;
+            }
+        }
     }
 }
 
@@ -438,12 +552,14 @@ hipError_t ev_launch(const EvArgs& a, int which, uint32_t n, hipStream_t s)
             hipLaunchKernelGGL(k_ev_expand<false>, dim3(n), dim3(256), 0, s, a);
         break;
     case kEvProcess:
+        if (a.n_ev)
+            hipLaunchKernelGGL(k_ev_bounds, dim3((a.n_ev + 1023) / 1024), dim3(256), 0, s, a);
         if (a.ev_syn)
-            hipLaunchKernelGGL(k_ev_process<true>, dim3((a.N + 255 / kEvGroup) / (256 / kEvGroup)), dim3(256), 0, s,
-                               a);
+            hipLaunchKernelGGL(k_ev_process<true>, dim3((a.N + kEvStepThreads - 1) / kEvStepThreads),
+                               dim3(kEvStepThreads), 0, s, a);
         else
-            hipLaunchKernelGGL(k_ev_process<false>, dim3((a.N + 255 / kEvGroup) / (256 / kEvGroup)), dim3(256), 0, s,
-                               a);
+            hipLaunchKernelGGL(k_ev_process<false>, dim3((a.N + kEvStepThreads - 1) / kEvStepThreads),
+                               dim3(kEvStepThreads), 0, s, a);
         break;
     default:
         return hipErrorInvalidValue;
@@ -485,11 +601,17 @@ __global__ __launch_bounds__(kRsThreads) void rs_count(const uint32_t* keys, uin
         h[d] = 0;
     __syncthreads();
     const uint32_t base = blockIdx.x * kRsTile;
+    // all loads in flight before the counting
+    uint32_t key[kRsRounds];
+#pragma unroll
     for (uint32_t k = 0; k < kRsRounds; ++k) {
         const uint32_t i = base + k * kRsThreads + threadIdx.x;
-        if (i < n)
-            atomicAdd(&h[(keys[i] >> shift) & (digits - 1u)], 1u);
+        key[k] = i < n ? keys[i] : 0u;
     }
+#pragma unroll
+    for (uint32_t k = 0; k < kRsRounds; ++k)
+        if (base + k * kRsThreads + threadIdx.x < n)
+            atomicAdd(&h[(key[k] >> shift) & (digits - 1u)], 1u);
     __syncthreads();
     for (uint32_t d = threadIdx.x; d < digits; d += kRsThreads)
         cnt[(size_t)d * tiles + blockIdx.x] = h[d];

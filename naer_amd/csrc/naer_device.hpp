@@ -256,6 +256,8 @@ struct EvArgs {
     uint32_t* ev_keys;                  // target << 11 | time offset (N < 2^21)
     float* ev_w;
     uint32_t n_ev;                      // events of the window
+    uint32_t* ev_begin;                 // [N] each target's run of sorted events (zero when it has none)
+    uint32_t* ev_end;
     // plastic nets (stdp.cl): every input carries the slot of its list entry
     // (sorted along with its key), whose payload a plastic neuron's spike
     // writes: its previous spike time and post trace; every event carries its

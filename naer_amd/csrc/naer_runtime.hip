@@ -230,6 +230,7 @@ struct naer_net {
     uint32_t *ev_keys = nullptr, *ev_keys_s = nullptr;
     float *ev_w = nullptr, *ev_w_s = nullptr;
     uint32_t* ev_tlast = nullptr;
+    uint32_t *ev_begin = nullptr, *ev_end = nullptr;  // each target's run of sorted events
     double *ev_dlo = nullptr, *ev_dhi = nullptr;
     uint32_t *ev_nhi = nullptr, *ev_tref = nullptr;
     // [0], [1]: counts of the two input lists, [2]: inputs lost; spikes at [4] (u64)
@@ -1329,11 +1330,14 @@ static int setup_events(naer_net* n)
     // nets also synapse and slot; sorted and not); event indices are 32-bit
     const double fit = (double)free_b / 4.0 / (n->plastic ? 32.0 : 16.0);
     n->ev_e_cap = (uint64_t)std::min(std::min(std::max((double)n->S_cap, (double)(1u << 22)), fit), 2147483647.0);
+    // the step kernel reads a target's run in aligned quads: up to 3 entries past the last event
+    const uint64_t ev_pad = n->ev_e_cap + 4;
     if ((rc = n->alloc(&n->ev_list[0], in_cap)) || (rc = n->alloc(&n->ev_list[1], in_cap)) ||
         (rc = n->alloc(&n->ev_sorted, in_cap)) || (rc = n->alloc(&n->ev_len, in_cap + 1)) ||
-        (rc = n->alloc(&n->ev_off, in_cap + 1)) || (rc = n->alloc(&n->ev_keys, n->ev_e_cap)) ||
-        (rc = n->alloc(&n->ev_keys_s, n->ev_e_cap)) || (rc = n->alloc(&n->ev_w, n->ev_e_cap)) ||
-        (rc = n->alloc(&n->ev_w_s, n->ev_e_cap)) || (rc = n->alloc(&n->ev_tlast, n->N)) ||
+        (rc = n->alloc(&n->ev_off, in_cap + 1)) || (rc = n->alloc(&n->ev_keys, ev_pad)) ||
+        (rc = n->alloc(&n->ev_keys_s, ev_pad)) || (rc = n->alloc(&n->ev_w, ev_pad)) ||
+        (rc = n->alloc(&n->ev_w_s, ev_pad)) || (rc = n->alloc(&n->ev_tlast, n->N)) ||
+        (rc = n->alloc(&n->ev_begin, n->N)) || (rc = n->alloc(&n->ev_end, n->N)) ||
         (rc = n->alloc(&n->ev_dlo, (size_t)std::max(1u, np) * kEvLo)) ||
         (rc = n->alloc(&n->ev_dhi, (size_t)std::max(1u, np) * kEvHiCap)) || (rc = n->alloc(&n->ev_nhi, np)) ||
         (rc = n->alloc(&n->ev_tref, np)) || (rc = n->alloc(&n->ev_ctr, 8)))
@@ -1342,9 +1346,9 @@ static int setup_events(naer_net* n)
         if ((rc = n->alloc(&n->ev_vals[0], in_cap)) || (rc = n->alloc(&n->ev_vals[1], in_cap)) ||
             (rc = n->alloc(&n->ev_vals_s, in_cap)) || (rc = n->alloc(&n->ev_prev[0], in_cap)) ||
             (rc = n->alloc(&n->ev_prev[1], in_cap)) || (rc = n->alloc(&n->ev_ptr[0], in_cap)) ||
-            (rc = n->alloc(&n->ev_ptr[1], in_cap)) || (rc = n->alloc(&n->ev_syn, n->ev_e_cap)) ||
-            (rc = n->alloc(&n->ev_syn_s, n->ev_e_cap)) || (rc = n->alloc(&n->ev_slot, n->ev_e_cap)) ||
-            (rc = n->alloc(&n->ev_slot_s, n->ev_e_cap)) || (rc = n->alloc(&n->ev_trace, n->N)) ||
+            (rc = n->alloc(&n->ev_ptr[1], in_cap)) || (rc = n->alloc(&n->ev_syn, ev_pad)) ||
+            (rc = n->alloc(&n->ev_syn_s, ev_pad)) || (rc = n->alloc(&n->ev_slot, ev_pad)) ||
+            (rc = n->alloc(&n->ev_slot_s, ev_pad)) || (rc = n->alloc(&n->ev_trace, n->N)) ||
             (rc = n->alloc(&n->ev_hist, (size_t)n->N * (kEvHist / 2))) ||
             (rc = n->alloc(&n->ev_pop_plastic, std::max(1u, np))))
             return rc;
@@ -1396,6 +1400,8 @@ static int setup_events(naer_net* n)
     HIP_TRY(hipMemcpy(n->ev_nhi, nhi.data(), np * sizeof(uint32_t), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(n->ev_tref, tref.data(), np * sizeof(uint32_t), hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(n->ev_tlast, 0, n->N * sizeof(uint32_t)));
+    HIP_TRY(hipMemset(n->ev_begin, 0, n->N * sizeof(uint32_t)));
+    HIP_TRY(hipMemset(n->ev_end, 0, n->N * sizeof(uint32_t)));
     HIP_TRY(hipMemset(n->ev_ctr, 0, 8 * sizeof(uint32_t)));
     // workspaces of the sorts (inputs and events) and of the row scan
     const uint32_t sort_max = (uint32_t)std::max<uint64_t>(n->ev_in_cap, n->ev_e_cap);
@@ -1425,6 +1431,8 @@ static EvArgs ev_args(const naer_net* n)
     a.row_off = n->ev_off;
     a.V = n->V;
     a.t_last = n->ev_tlast;
+    a.ev_begin = n->ev_begin;
+    a.ev_end = n->ev_end;
     a.dec_lo = n->ev_dlo;
     a.dec_hi = n->ev_dhi;
     a.n_hi = n->ev_nhi;
