@@ -182,6 +182,7 @@ struct naer_net {
     uint32_t pop_cap = 0;
 
     hipStream_t stream = nullptr;
+    hipStream_t build_stream = nullptr;  // arenas: admissions build their connectivity here
     std::vector<void*> allocs;
     uint64_t dev_bytes = 0;
 
@@ -306,6 +307,8 @@ struct naer_net {
             std::fclose(raster);  // unfinished: the header keeps 0 records
         if (raster_stage)
             (void)hipHostFree(raster_stage);
+        if (build_stream)
+            (void)hipStreamDestroy(build_stream);
         if (stream)
             (void)hipStreamDestroy(stream);
     }
@@ -1720,8 +1723,22 @@ int naer_arena_admit(naer_net* n, const naer_net_desc* desc, uint32_t* slot)
     if (footprint > n->N)
         return fail(NAER_ERR_INVALID, "a job of %u neurons (%llu with its alignment) exceeds the arena's %u",
                     net.n_neurons, (unsigned long long)footprint, n->N);
-    if ((rc = naer_net_sync(n)))
-        return rc;
+    // The connectivity is built on a second stream, while steps already
+    // enqueued on the arena's stream run: the build writes only the job's
+    // free ranges (row_ptr, col, w) and the build scratch, which no step
+    // reads.  The population table and the job's state are installed on the
+    // arena's stream once those steps are done.
+    if (!n->build_stream) {
+        const hipError_t e = hipStreamCreateWithFlags(&n->build_stream, hipStreamNonBlocking);
+        if (e != hipSuccess)
+            return fail(NAER_ERR_HIP, "stream creation failed: %s", hipGetErrorString(e));
+    }
+    struct BuildStream {  // the build helpers enqueue on n->stream
+        naer_net* n;
+        hipStream_t main;
+        explicit BuildStream(naer_net* net) : n(net), main(net->stream) { n->stream = n->build_stream; }
+        ~BuildStream() { n->stream = main; }
+    };
     // one extra entry: row_ptr[begin + n] (the end of the job's last
     // row) must not be shared with a job placed right behind it
     uint64_t nb = 0;
@@ -1739,7 +1756,13 @@ int naer_arena_admit(naer_net* n, const naer_net_desc* desc, uint32_t* slot)
         return code;
     };
     std::vector<unsigned long long> prefix;
-    if ((rc = count_net(n, net)) || (rc = read_prefix(n, net.neuron_begin, net.n_neurons, &prefix, &net.max_row)))
+    {
+        BuildStream on_build(n);
+        rc = count_net(n, net);
+        if (!rc)
+            rc = read_prefix(n, net.neuron_begin, net.n_neurons, &prefix, &net.max_row);
+    }
+    if (rc)
         return undo(rc);
     if (net.max_row > n->max_row)
         return undo(fail(NAER_ERR_INVALID, "a source has %u synapses; the arena admits rows up to %u", net.max_row,
@@ -1751,9 +1774,20 @@ int naer_arena_admit(naer_net* n, const naer_net_desc* desc, uint32_t* slot)
     if (!n->syn_alloc.take(net.n_syn, 64, &net.syn_begin))
         return undo(fail(NAER_ERR_CAPACITY, "no free range of %llu synapses", (unsigned long long)net.n_syn));
     syn_taken = true;
-    if ((rc = write_ptr(n, n->row_ptr, net.neuron_begin, net.syn_begin, prefix)) || (rc = place_net(n, net)))
+    {
+        BuildStream on_build(n);
+        rc = write_ptr(n, n->row_ptr, net.neuron_begin, net.syn_begin, prefix);
+        if (!rc)
+            rc = place_net(n, net);
+        if (!rc && net.plastic)
+            rc = count_plastic(n, net);
+        if (!rc && hipStreamSynchronize(n->build_stream) != hipSuccess)
+            rc = fail(NAER_ERR_HIP, "connectivity build failed");
+    }
+    if (rc)
         return undo(rc);
-    if (net.plastic && (rc = count_plastic(n, net)))
+    // the steps enqueued before this admission are done before the table changes
+    if ((rc = naer_net_sync(n)))
         return undo(rc);
     // slot: reuse a retired one
     uint32_t s = (uint32_t)n->nets.size();

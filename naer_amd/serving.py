@@ -6,11 +6,13 @@ uniform in [0.2, 2] s of simulated time, 30% with pair STDP on E->E; J_E =
 0.1 mV, g = 5, 20 kHz x 0.1 mV drive, delay 1.5 ms).
 
 The server keeps two arenas (static jobs, plastic jobs) whose capacities split
-the HBM budget.  Between chunks of `chunk` steps it admits waiting jobs that fit
-(first come first served, later jobs may pass a head that does not fit yet),
-runs every resident job in the same launches (the two arenas on their own
-streams, so their work overlaps on the device), drains the recorded spikes and
-retires the jobs that have reached their duration.  A job's raster depends only
+the HBM budget.  Each round it enqueues the next chunk of `chunk` steps of every
+resident job (the two arenas on their own streams, so their work overlaps on
+the device), admits waiting jobs that fit while that chunk runs (first come
+first served, later jobs may pass a head that does not fit yet; an admission
+builds its connectivity on a second stream of the arena, and the job starts
+with the following chunk), then drains the recorded spikes and retires the
+jobs that have reached their duration.  A job's raster depends only
 on its own description: alone or in any mix it is the same, bit for bit in
 deterministic mode.
 
@@ -208,6 +210,15 @@ class Server:
             now = clock() - t0
             while pending and pending[0].arrival_s <= now:
                 waiting.append(pending.popleft())
+            # the next chunk of the running jobs goes to the GPU first: the
+            # admissions below build their connectivity on a second stream
+            # meanwhile, and their jobs start with the chunk after it
+            ran = [False, False]
+            for ai, arena in enumerate(self.arenas):
+                if active[ai]:
+                    arena.run(self.chunk)
+                    ran[ai] = True
+            admitted = [set(), set()]
             # admission: first come first served; a job that does not fit yet
             # waits while later jobs may pass it
             still = []
@@ -231,31 +242,32 @@ class Server:
                     still.append(job)
                     continue
                 active[ai][slot] = [job, 0]
+                admitted[ai].add(slot)
                 r = res[job.index]
                 r.admitted_s, r.slot, r.arena, r.status = clock() - t0, slot, ai, "running"
             waiting = still
-            if not any(active):
-                if waiting and not pending:
-                    # the arenas are empty and still cannot take them
-                    for job in waiting:
-                        finish(job, "rejected", "does not fit an empty arena")
-                    waiting = []
-                elif pending:
-                    sleep(max(0.0, min(0.05, pending[0].arrival_s - (clock() - t0))))
-                elif waiting:
-                    sleep(0.001)
+            if not any(ran):
+                if not any(active):
+                    if waiting and not pending:
+                        # the arenas are empty and still cannot take them
+                        for job in waiting:
+                            finish(job, "rejected", "does not fit an empty arena")
+                        waiting = []
+                    elif pending:
+                        sleep(max(0.0, min(0.05, pending[0].arrival_s - (clock() - t0))))
+                    elif waiting:
+                        sleep(0.001)
                 continue
             for ai, arena in enumerate(self.arenas):
-                if active[ai]:
-                    arena.run(self.chunk)
-            for ai, arena in enumerate(self.arenas):
-                if not active[ai]:
+                if not ran[ai]:
                     continue
                 drained = arena.drain()
                 bad = set(arena.check()) if watchdog and hasattr(arena, "check") else set()
                 now = clock() - t0
                 done = []
                 for slot, entry in active[ai].items():
+                    if slot in admitted[ai]:
+                        continue  # admitted during this chunk: it starts with the next one
                     job, steps = entry
                     total = job.steps(self.dt_ms)
                     sp = drained.get(slot)
