@@ -243,6 +243,8 @@ def main():
     ap.add_argument("--steps", type=int, default=1111, help="timed batches (1111 x 0.9 ms = 1 s on the 1M network)")
     ap.add_argument("--warmup", type=int, default=120, help="warm-up batches (with --autotune, the autotuner runs inside)")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="large_1m")
+    ap.add_argument("--settle-ms", type=float, default=100.0,
+                    help="biological time run before the warm-up to leave the initial transient")
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--indegree", type=int, default=None)
     ap.add_argument("--mode", choices=["fast", "det"], default="fast")
@@ -286,9 +288,14 @@ def main():
     print(f"# built N={net.getNeurons()} S={net.getSynapses()} in {build_s:.1f} s, batches of {B} steps",
           file=sys.stderr, flush=True)
 
-    # warm-up: half of it to leave the initial transient, then (--autotune)
-    # the autotuner picks the delivery schedule on real (aligned) batches,
-    # then the rest; the warm-up ends on a batch boundary
+    # the network first leaves its initial transient (V0 uniform below
+    # threshold: the rate starts near 2 Hz and settles at ~5 Hz within
+    # ~50 ms), so that any --warmup/--steps measure the steady state; then
+    # the warm-up: half of it, (--autotune) the autotuner picking the
+    # delivery schedule on real (aligned) batches, the rest; it ends on a
+    # batch boundary
+    settle = -(-int(round(args.settle_ms / net.dt_ms)) // B) * B
+    net.run(settle)
     net.run(B * (args.warmup // 2))
     schedule = net.autotune(2) if args.warmup > 0 and args.autotune else net.schedule()
     done = int(net.stats().step)
@@ -421,6 +428,8 @@ def main():
         "data": "synthetic (device-built connectivity and Poisson drive from fixed seeds)",
         "config": {"workload": f"{args.workload}: {WORKLOADS[args.workload]}, {bio_s:g} s at dt={dt_ms:g} ms",
                    "step": f"one aligned batch of {B} steps of {dt_ms:g} ms (update + delivery)",
+                   "state": f"steady state: timed after {settle * dt_ms:g} ms of settling and {args.warmup} "
+                            f"warm-up batches",
                    "batch_steps": B, "n_neurons": N, "n_synapses": int(s1.n_synapses), "mode": args.mode,
                    "dt_ms": dt_ms},
         "realtime_factor": round(rtf, 3),
@@ -429,6 +438,7 @@ def main():
         "build_s": round(build_s, 2),
         "schedule": schedule_out,
         "warmup_steps_run": warmup_run,
+        "settle_steps": settle,
         "graph_steps": args.graph_steps,
         "roofline": roofline,
         "roofline_batch": roofline_batch,
