@@ -1204,8 +1204,9 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_place(const BucketArg
     __syncthreads();
     if (a.is_last && blockIdx.x == 0 && threadIdx.x == 0)
         batch_epilogue(st, a.B);
-    uint2* stage = a.stage;
-    uint32_t* stage4 = (uint32_t*)a.stage;
+    // direct: the group buckets are the tile buckets of the batch's buffer
+    uint2* stage = a.direct ? a.ev + (size_t)buf * a.e_cap : a.stage;
+    uint32_t* stage4 = a.direct ? (uint32_t*)a.ev + (size_t)buf * a.e_cap : (uint32_t*)a.stage;
     RingT* ring = (RingT*)a.ring;
     const uint32_t lane = lane_id();
     for_block_chunks<U, REC4>(a, cur, s_off, t0, rs, [&](uint32_t r, uint32_t nrn, unsigned long long beg,
@@ -1803,6 +1804,8 @@ static void launch_bucket_pass(const BucketArgs& a, int step, bool det, size_t l
         break;
     }
     default:
+        if (a.direct)
+            break;  // the placement wrote the tile buckets
         if (a.ncls)
             hipLaunchKernelGGL((k_bucket_refine<AGG, true>), dim3((a.NBK >> a.gt_shift) * a.RS), dim3(kRefineThreads),
                                0, s, a);

@@ -262,7 +262,7 @@ struct naer_net {
     bool bucketed = false;   // full aligned batches are delivered through buckets
     uint32_t R = 0, M = 0, NT = 0, NBK = 0, NB = 0, G = 1;
     uint32_t U = kBucketUnroll;  // schedule: synapses per lane per task
-    uint32_t RS = 1;             // schedule: refine blocks per group bucket
+    uint32_t RS = 1;             // schedule: refine blocks per group bucket; 0: direct placement
     uint32_t AG = 1;             // schedule: run-aggregated LDS counters
     bool balance = false;        // histogram/placement blocks split each batch by events
     uint64_t e_cap = 0;       // records per event buffer
@@ -1054,6 +1054,19 @@ int set_block_ranges(naer_net* n)
     return NAER_OK;
 }
 
+// The placement keeps one LDS counter per group bucket: at most 8192 (32 KB).
+constexpr uint32_t kMaxPlaceCounters = 8192;
+
+// Direct placement (schedule refine_split 0): groups of a single tile, so the
+// placement writes every event straight into its tile bucket and no refine
+// pass runs.  It pays where a placement block holds many events per tile
+// bucket (dense packs: runs of hundreds of records), and needs one placement
+// counter per tile bucket.
+bool direct_ok(const naer_net* n)
+{
+    return n->NBK <= kMaxPlaceCounters;
+}
+
 int alloc_buckets(naer_net* n)
 {
     n->bucketed = false;
@@ -1089,7 +1102,7 @@ int alloc_buckets(naer_net* n)
     n->RS = std::max<uint32_t>(1, std::min<uint32_t>(64, 2048 / std::max<uint32_t>(1, n->NBK >> n->gt_shift)));
     n->AG = run_aggregation(n);
     // placement counters: one per group bucket in LDS (<= 32 KB without raising the limit)
-    if (n->M > kMaxArrivalBatches || (n->NBK >> n->gt_shift) > 8192)
+    if (n->M > kMaxArrivalBatches || (n->NBK >> n->gt_shift) > kMaxPlaceCounters)
         return NAER_OK;
     int rc;
     n->rec_bytes = class_weights(n) ? 4u : 8u;
@@ -1100,7 +1113,8 @@ int alloc_buckets(naer_net* n)
     if ((rc = n->alloc(&n->ev, (size_t)n->R * rec_pairs)) || (rc = n->alloc(&n->stage, rec_pairs)) ||
         (rc = n->alloc(&n->bs, (size_t)n->R * (n->NBK + 1))) || (rc = n->alloc(&n->flags, n->R)) ||
         (rc = n->alloc(&n->hist, (size_t)n->n_seg * n->NBK)) ||  // rows for any G (G = 1: n_seg blocks)
-        (rc = n->alloc(&n->hist_c, (size_t)n->n_seg * (n->NBK >> n->gt_shift))) || (rc = n->alloc(&n->totals, n->NBK)) ||
+        (rc = n->alloc(&n->hist_c, (size_t)n->n_seg * (direct_ok(n) ? n->NBK : n->NBK >> n->gt_shift))) ||
+        (rc = n->alloc(&n->totals, n->NBK)) ||
         (rc = n->alloc(&n->gcur, n->NBK)))
         return rc;
     HIP_TRY(hipMemsetAsync(n->flags, 0, n->R * sizeof(uint32_t), n->stream));
@@ -2066,7 +2080,8 @@ static int enqueue_steps(naer_net* n, uint32_t n_steps, bool timed)
     ba.NT = n->NT;
     ba.NBK = n->NBK;
     ba.tw_shift = n->tw_shift;
-    ba.gt_shift = n->gt_shift;
+    ba.gt_shift = n->RS ? n->gt_shift : 0u;  // direct placement: one tile per group
+    ba.direct = n->RS ? 0u : 1u;
     ba.G = n->G;
     ba.NB = n->NB;
     ba.U = n->U;
@@ -2213,8 +2228,11 @@ int naer_net_set_schedule(naer_net* n, const naer_schedule* s)
         return fail(NAER_ERR_INVALID, "task_synapses must be %u or %u", 4 * kWave, 8 * kWave);
     if (s->segments_per_block == 0 || s->segments_per_block > kMaxSegSchedule)
         return fail(NAER_ERR_INVALID, "segments_per_block must be in [1, %u]", kMaxSegSchedule);
-    if (s->refine_split == 0 || s->refine_split > 64)
-        return fail(NAER_ERR_INVALID, "refine_split must be in [1, 64]");
+    if (s->refine_split > 64)
+        return fail(NAER_ERR_INVALID, "refine_split must be in [0, 64]");
+    if (s->refine_split == 0 && n->bucketed && !direct_ok(n))
+        return fail(NAER_ERR_INVALID, "refine_split 0 (direct placement) needs at most %u buckets (the net has %u)",
+                    kMaxPlaceCounters, n->NBK);
     if (s->run_aggregate > 3)
         return fail(NAER_ERR_INVALID, "run_aggregate must be in [0, 3]");
     int rc = naer_net_sync(n);
@@ -2231,7 +2249,13 @@ int naer_net_set_schedule(naer_net* n, const naer_schedule* s)
 // Tries schedules of the bucketed delivery on the running simulation: each
 // candidate delivers `batches` aligned batches (real steps; in deterministic
 // mode the results do not depend on the schedule) and is scored by its
-// delivery time per synaptic event, measured with HIP events.
+// delivery time per synaptic event, measured with HIP events.  Three stages,
+// each starting from the previous stage's winner: the placement (two-level
+// or direct, where the net allows both), the shape (task size x segments per
+// block x, two-level only, refine blocks per group bucket, each halved or
+// doubled), then each counter mode flipped.  Every candidate of a stage is
+// tried twice, keeping its better score, so that a drift of the activity
+// during the trials does not decide.
 int naer_net_autotune(naer_net* n, uint32_t batches, naer_schedule* chosen)
 {
     if (!n)
@@ -2251,84 +2275,114 @@ int naer_net_autotune(naer_net* n, uint32_t batches, naer_schedule* chosen)
         return rc;
     const bool prof = n->profiling;
     n->profiling = true;
-    naer_schedule base;
-    naer_net_get_schedule(n, &base);
-    std::vector<naer_schedule> cand;
-    for (uint32_t task : { 4 * kWave, 8 * kWave })
-        for (uint32_t g : { std::max<uint32_t>(1, base.segments_per_block / 2), base.segments_per_block,
-                            std::min<uint32_t>(kMaxSegSchedule, base.segments_per_block * 2) })
-            for (uint32_t r : { std::max<uint32_t>(1, base.refine_split / 2), base.refine_split,
-                                std::min<uint32_t>(64, base.refine_split * 2) }) {
-                naer_schedule c = base;
-                c.task_synapses = task;
-                c.segments_per_block = g;
-                c.refine_split = r;
-                bool dup = false;
-                for (auto& x : cand)
-                    dup = dup || (x.task_synapses == task && x.segments_per_block == g && x.refine_split == r);
-                if (!dup)
-                    cand.push_back(c);
-            }
-    // the base schedule with either counter mode flipped; the winner takes
-    // each mode that did better on the base
-    const size_t n_main = cand.size();
-    for (uint32_t bit : { 1u, 2u }) {
-        naer_schedule c = base;
-        c.run_aggregate = base.run_aggregate ^ bit;
-        cand.push_back(c);
-    }
-    size_t base_i = 0;
-    for (size_t i = 0; i < n_main; ++i)
-        if (cand[i].task_synapses == base.task_synapses && cand[i].segments_per_block == base.segments_per_block &&
-            cand[i].refine_split == base.refine_split)
-            base_i = i;
-    // two passes over the candidates, each candidate keeping its better
-    // score, so that a drift of the activity during the trials does not
-    // decide
-    std::vector<double> score(cand.size(), 1e300), per_batch(cand.size(), 0.0);
-    for (int pass = 0; pass < 2 && rc == NAER_OK; ++pass) {
-        for (size_t i = 0; i < cand.size(); ++i) {
-            if ((rc = naer_net_set_schedule(n, &cand[i])))
-                break;
-            naer_stats s0, s1;
-            double ms0 = 0, ms1 = 0;
-            if ((rc = naer_net_query(n, &s0)))
-                break;
-            for (int k = NAER_K_BUCKET_HIST; k <= NAER_K_BUCKET_REFINE; ++k)
-                ms0 += n->kernel_ms[k];
-            if ((rc = naer_net_step(n, batches * n->B)) || (rc = naer_net_sync(n)) ||
-                (rc = naer_net_query(n, &s1)))
-                break;
-            for (int k = NAER_K_BUCKET_HIST; k <= NAER_K_BUCKET_REFINE; ++k)
-                ms1 += n->kernel_ms[k];
-            const double sc = (ms1 - ms0) / std::max(1.0, (double)(s1.events - s0.events));
-            if (sc < score[i]) {
-                score[i] = sc;
-                per_batch[i] = (ms1 - ms0) * 1e3 / batches;
+    // scores (ms per event) and delivery times per batch of a stage's candidates
+    auto trial = [&](const std::vector<naer_schedule>& cand, std::vector<double>& score,
+                     std::vector<double>& per_batch) -> int {
+        score.assign(cand.size(), 1e300);
+        per_batch.assign(cand.size(), 0.0);
+        for (int pass = 0; pass < 2; ++pass) {
+            for (size_t i = 0; i < cand.size(); ++i) {
+                int r;
+                naer_stats s0, s1;
+                double ms0 = 0, ms1 = 0;
+                if ((r = naer_net_set_schedule(n, &cand[i])) || (r = naer_net_query(n, &s0)))
+                    return r;
+                for (int k = NAER_K_BUCKET_HIST; k <= NAER_K_BUCKET_REFINE; ++k)
+                    ms0 += n->kernel_ms[k];
+                if ((r = naer_net_step(n, batches * n->B)) || (r = naer_net_sync(n)) ||
+                    (r = naer_net_query(n, &s1)))
+                    return r;
+                for (int k = NAER_K_BUCKET_HIST; k <= NAER_K_BUCKET_REFINE; ++k)
+                    ms1 += n->kernel_ms[k];
+                const double sc = (ms1 - ms0) / std::max(1.0, (double)(s1.events - s0.events));
+                if (sc < score[i]) {
+                    score[i] = sc;
+                    per_batch[i] = (ms1 - ms0) * 1e3 / batches;
+                }
             }
         }
+        return NAER_OK;
+    };
+    auto argmin = [](const std::vector<double>& v) {
+        size_t b = 0;
+        for (size_t i = 1; i < v.size(); ++i)
+            if (v[i] < v[b])
+                b = i;
+        return b;
+    };
+    std::vector<double> score, per_batch;
+    naer_schedule best;
+    naer_net_get_schedule(n, &best);
+    double best_us = 0.0;
+    // stage 1: the placement
+    if (direct_ok(n)) {
+        naer_schedule two = best, one = best;
+        if (two.refine_split == 0)
+            two.refine_split = std::max<uint32_t>(1, std::min<uint32_t>(64, 2048 / std::max<uint32_t>(
+                                                                               1, n->NBK >> n->gt_shift)));
+        one.refine_split = 0;
+        const std::vector<naer_schedule> cand{ two, one };
+        if ((rc = trial(cand, score, per_batch)) == NAER_OK) {
+            const size_t i = argmin(score);
+            best = cand[i];
+            best_us = per_batch[i];
+        }
     }
-    size_t bi = 0;
-    for (size_t i = 1; i < n_main; ++i)
-        if (score[i] < score[bi])
-            bi = i;
-    naer_schedule best_s = cand[bi];
-    best_s.delivery_us = per_batch[bi];
-    for (size_t f = 0; f < 2; ++f) {
-        const size_t i = n_main + f;
-        if (score[i] < score[base_i]) {
-            best_s.run_aggregate ^= 1u << f;
-            if (bi == base_i)
-                best_s.delivery_us = std::min(best_s.delivery_us, per_batch[i]);
+    // stage 2: the shape
+    if (rc == NAER_OK) {
+        std::vector<naer_schedule> cand;
+        std::vector<uint32_t> splits;
+        if (best.refine_split == 0)
+            splits = { 0u };
+        else
+            splits = { std::max<uint32_t>(1, best.refine_split / 2), best.refine_split,
+                       std::min<uint32_t>(64, best.refine_split * 2) };
+        for (uint32_t task : { 4 * kWave, 8 * kWave })
+            for (uint32_t g : { std::max<uint32_t>(1, best.segments_per_block / 2), best.segments_per_block,
+                                std::min<uint32_t>(kMaxSegSchedule, best.segments_per_block * 2) })
+                for (uint32_t r : splits) {
+                    naer_schedule c = best;
+                    c.task_synapses = task;
+                    c.segments_per_block = g;
+                    c.refine_split = r;
+                    bool dup = false;
+                    for (auto& x : cand)
+                        dup = dup || (x.task_synapses == task && x.segments_per_block == g && x.refine_split == r);
+                    if (!dup)
+                        cand.push_back(c);
+                }
+        if ((rc = trial(cand, score, per_batch)) == NAER_OK) {
+            const size_t i = argmin(score);
+            best = cand[i];
+            best_us = per_batch[i];
+        }
+    }
+    // stage 3: the counter modes, each flipped on the winner; it takes each
+    // flip that did better than the winner itself
+    if (rc == NAER_OK) {
+        std::vector<naer_schedule> cand{ best, best, best };
+        cand[1].run_aggregate ^= 1u;
+        cand[2].run_aggregate ^= 2u;
+        if ((rc = trial(cand, score, per_batch)) == NAER_OK) {
+            naer_schedule pick = best;
+            double us = per_batch[0];
+            for (size_t f = 0; f < 2; ++f)
+                if (score[1 + f] < score[0]) {
+                    pick.run_aggregate ^= 1u << f;
+                    us = std::min(us, per_batch[1 + f]);
+                }
+            best = pick;
+            best_us = us;
         }
     }
     n->profiling = prof;
     n->graph_chunk = graph;
     if (rc)
         return rc;
-    if ((rc = naer_net_set_schedule(n, &best_s)))
+    best.delivery_us = best_us;
+    if ((rc = naer_net_set_schedule(n, &best)))
         return rc;
-    n->tuned_us = best_s.delivery_us;
+    n->tuned_us = best_us;
     if (chosen)
         return naer_net_get_schedule(n, chosen);
     return NAER_OK;

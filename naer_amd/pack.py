@@ -50,6 +50,10 @@ class Pack:
     def schedule(self):
         return _abi.get_schedule(self._handle)
 
+    def set_schedule(self, task_synapses, segments_per_block, refine_split, run_aggregate=None):
+        """refine_split 0: direct placement (events straight into their tile buckets)."""
+        _abi.set_schedule(self._handle, task_synapses, segments_per_block, refine_split, run_aggregate)
+
     def set_graph(self, chunk_steps):
         check(_abi.lib().naer_net_set_graph(self._handle, int(chunk_steps)))
 

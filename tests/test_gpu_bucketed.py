@@ -153,6 +153,38 @@ def test_full_size_events_conserved_and_deterministic():
 
 
 
+@pytest.mark.parametrize("kw", [{}, {"plastic": True}, {"random_weights": True}])
+def test_direct_placement_matches_oracle(kw):
+    """Schedule refine_split 0: the placement writes every event straight into
+    its tile bucket (no refine pass), for 4-byte class records, 8-byte
+    records with weights and plastic rows (lazy STDP in the placement)."""
+    net = multi_delay_net(**kw)
+    net.compile()
+    o = oracle_of(net, O.FIXED)
+    s = net.schedule()
+    net.set_schedule(s["task_synapses"], s["segments_per_block"], 0)
+    assert net.schedule()["refine_split"] == 0
+    net.run(463)  # a partial batch at the end
+    o.run(463)
+    assert_same(net, o)
+    if kw.get("plastic"):
+        np.testing.assert_array_equal(net.csr()[2].view(np.uint32), o.csr()[2].view(np.uint32))
+    net.close()
+
+
+def test_direct_placement_refused_beyond_its_counters():
+    """More tile buckets than the placement's 8192 LDS counters (1M neurons in
+    1024-neuron tiles x 9 arrival batches): direct placement is refused."""
+    net = models.large_1m(indegree=10)
+    net.compile()
+    s = net.schedule()
+    assert s["bucketed"] == 1
+    with pytest.raises(_abi.NaerError):
+        net.set_schedule(s["task_synapses"], s["segments_per_block"], 0)
+    net.set_schedule(s["task_synapses"], s["segments_per_block"], s["refine_split"])
+    net.close()
+
+
 def test_autotune_and_schedules_leave_results_unchanged():
     net = multi_delay_net()
     net.compile()
@@ -168,8 +200,10 @@ def test_autotune_and_schedules_leave_results_unchanged():
     net.run(40)
     net.set_schedule(512, 4, 1, run_aggregate=3)
     net.run(40)
+    net.set_schedule(512, 4, 0, run_aggregate=0)  # direct placement
+    net.run(40)
     net.set_schedule(512, 4, 1, run_aggregate=0)
-    net.run(700 - steps - 140)
+    net.run(700 - steps - 180)
     o.run(700)
     assert_same(net, o)
     with pytest.raises(Exception):

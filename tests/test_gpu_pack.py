@@ -51,6 +51,23 @@ def test_pack_equals_alone_and_oracle():
         assert first_divergence(alone[i], o.spikes()) is None, i
 
 
+def test_direct_placement_pack_equals_alone():
+    """Packs with schedule refine_split 0 (events straight into their tile
+    buckets, no refine pass) against each network alone (default schedule)."""
+    specs = _specs()
+    nets = [sweep.make_network(s, _abi.MODE_DETERMINISTIC, sweep.expected_record(s, STEPS)) for s in specs]
+    p = Pack(nets)
+    p.build()
+    sc = p.schedule()
+    p.set_schedule(sc["task_synapses"], sc["segments_per_block"], 0)
+    assert p.schedule()["refine_split"] == 0
+    p.run(STEPS)
+    packed = p.spikes()
+    p.close()
+    for i, s in enumerate(specs):
+        assert first_divergence(packed[i], _alone(s)) is None, (i, s)
+
+
 @pytest.mark.parametrize("plastic_packs", [1, 2])
 def test_split_packs_with_graphs_equal_alone(plastic_packs):
     specs = _specs()
