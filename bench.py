@@ -204,10 +204,12 @@ def cpu_baseline(args):
 
 
 def sweep_sample(args):
-    """The batched sweep reduced to args.sweep_nets networks x args.sweep_steps
-    steps in deterministic mode: packed (graph-captured chunks, one stream
-    per pack) against the plain one-network-per-launch-sequence loop, jobs/s
-    of each (build excluded) and the count of bit-identical rasters."""
+    """The batched sweep, args.sweep_nets networks x args.sweep_steps steps
+    (default: the whole sweep, 256 networks x 0.5 s) in deterministic mode:
+    packed (graph-captured chunks, one stream per pack, schedules from the
+    autotuner on the run's own steps) against the plain
+    one-network-per-launch-sequence loop, jobs/s of each (build excluded) and
+    the count of bit-identical rasters."""
     from naer_amd import sweep
     specs = sweep.sweep_specs(args.sweep_nets)
     packed = sweep.run_packed(specs, args.sweep_steps)
@@ -217,9 +219,9 @@ def sweep_sample(args):
             "packed_jobs_per_s": round(len(specs) / packed["run_s"], 2),
             "sequential_jobs_per_s": round(len(specs) / seq["run_s"], 2),
             "speedup": round(seq["run_s"] / packed["run_s"], 3), "rasters_identical": ident,
-            "sample": f"configs' sweep networks (N log-uniform 1k-50k, K = 0.1 N, half with STDP), reduced to "
-                      f"{len(specs)} networks x {args.sweep_steps} steps; the full sweep is "
-                      f"python -m naer_amd.sweep --nets 256 --steps 5000"}
+            "schedules": packed["schedules"],
+            "sample": f"the sweep's networks (N log-uniform 1k-50k, K = 0.1 N, g in [3, 7], drive 10-25 kHz, "
+                      f"half with STDP), {len(specs)} networks x {args.sweep_steps} steps"}
 
 
 def serving_sample(args):
@@ -258,7 +260,7 @@ def main():
                          "library's schedule for the network)")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--sweep-nets", type=int, default=256)
-    ap.add_argument("--sweep-steps", type=int, default=600)
+    ap.add_argument("--sweep-steps", type=int, default=5000)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serve-jobs", type=int, default=32)
     ap.add_argument("--serve-rate", type=float, default=10.0)
@@ -266,6 +268,7 @@ def main():
     ap.add_argument("--graph-steps", type=int, default=0,
                     help="batches per HIP-graph-captured chunk of the timed run (0: plain launches)")
     args = ap.parse_args()
+    t_start = time.perf_counter()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the simulation step of one network is not "
               "split across GPUs in this build (--gpus must be 1)", file=sys.stderr)
@@ -392,13 +395,13 @@ def main():
 
     sweep = serving = None
     if args.workload == "large_1m" and not args.no_sweep:
-        print("# sweep sample", file=sys.stderr, flush=True)
+        print(f"# sweep sample ({time.perf_counter() - t_start:.0f} s)", file=sys.stderr, flush=True)
         try:
             sweep = sweep_sample(args)
         except Exception as e:  # a sample, reported beside the headline
             sweep = {"error": str(e)}
     if args.workload == "large_1m" and not args.no_serving:
-        print("# serving sample", file=sys.stderr, flush=True)
+        print(f"# serving sample ({time.perf_counter() - t_start:.0f} s)", file=sys.stderr, flush=True)
         try:
             serving = serving_sample(args)
         except Exception as e:
@@ -406,7 +409,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        print("# cpu baseline", file=sys.stderr, flush=True)
+        print(f"# cpu baseline ({time.perf_counter() - t_start:.0f} s)", file=sys.stderr, flush=True)
         try:
             cpu = cpu_baseline(args)
         except Exception as e:  # the baseline is reported, never required

@@ -109,7 +109,7 @@ class Arena:
         """Recorded spikes since the last drain, split per active slot:
         {slot: array(step, neuron, time_us)} with local neuron ids and local steps."""
         cap = self.record_capacity
-        out = np.zeros(max(1, cap), dtype=_abi.SPIKE_DTYPE)
+        out = np.empty(max(1, cap), dtype=_abi.SPIKE_DTYPE)
         got = ctypes.c_uint64()
         check(_abi.lib().naer_net_read_spikes(self._handle, out.ctypes.data, cap, ctypes.byref(got)))
         out = out[: got.value]

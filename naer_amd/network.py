@@ -364,11 +364,10 @@ class Network:
         """Drain recorded spikes; structured array (step, neuron, time_us) sorted
         by (step, neuron, time_us)."""
         cap = self.record_capacity if cap is None else cap
-        out = np.zeros(max(1, cap), dtype=_abi.SPIKE_DTYPE)
+        out = np.empty(max(1, cap), dtype=_abi.SPIKE_DTYPE)
         n = ctypes.c_uint64()
         check(_abi.lib().naer_net_read_spikes(self._h(), out.ctypes.data, cap, ctypes.byref(n)))
-        out = out[: n.value]
-        return np.sort(out, order=["step", "neuron", "time_us"])
+        return _abi.sort_spikes(out[: n.value])
 
     # --- checkpoint / resume (naer_net_save / naer_net_load) ----------------
     def save(self, path):

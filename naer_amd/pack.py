@@ -71,7 +71,7 @@ class Pack:
     def spikes(self):
         """Per-network rasters (local neuron ids, local steps), each sorted."""
         cap = self.record_capacity
-        out = np.zeros(max(1, cap), dtype=_abi.SPIKE_DTYPE)
+        out = np.empty(max(1, cap), dtype=_abi.SPIKE_DTYPE)
         got = ctypes.c_uint64()
         check(_abi.lib().naer_net_read_spikes(self._handle, out.ctypes.data, cap, ctypes.byref(got)))
         out = out[: got.value]
@@ -82,7 +82,7 @@ class Pack:
             sp = out[net_of == i].copy()
             sp["neuron"] -= self.begin[i]
             sp["step"] -= np.uint32(self.t_start[i])
-            res.append(np.sort(sp, order=["step", "neuron", "time_us"]))
+            res.append(_abi.sort_spikes(sp))
         return res
 
     def close(self):

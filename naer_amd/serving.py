@@ -294,7 +294,7 @@ class Server:
         lat = np.array([r.latency_s for r in finished]) if finished else np.zeros(1)
         for r in finished:
             if self.keep:
-                r.raster = (np.sort(np.concatenate(r.raster), order=["step", "neuron", "time_us"])
+                r.raster = (_abi.sort_spikes(np.concatenate(r.raster))
                             if r.raster else np.zeros(0, dtype=_abi.SPIKE_DTYPE))
         statuses = collections.Counter(r.status for r in res.values())
         return dict(results=res, rejected=rejected, makespan_s=makespan, completed=len(finished),

@@ -35,3 +35,19 @@ def test_reader_rejects_bad_files(tmp_path):
     (tmp_path / "short.naer").write_bytes(raw[:-8])
     with pytest.raises(ValueError):
         read_raster(tmp_path / "short.naer")
+
+
+def test_sort_spikes_equals_the_field_sort():
+    """The library's raster sort (two stable integer argsorts) orders records
+    as numpy's sort by the fields (step, neuron, time_us), including several
+    spikes of one neuron in one step (POISSON sources)."""
+    from naer_amd import _abi
+    rng = np.random.default_rng(7)
+    n = 20_000
+    sp = np.zeros(n, dtype=_abi.SPIKE_DTYPE)
+    sp["step"] = rng.integers(0, 300, n)
+    sp["neuron"] = rng.integers(0, 50, n)
+    sp["time_us"] = rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)
+    want = np.sort(sp, order=["step", "neuron", "time_us"])
+    assert np.array_equal(_abi.sort_spikes(sp), want)
+    assert len(_abi.sort_spikes(sp[:1])) == 1 and len(_abi.sort_spikes(sp[:0])) == 0
