@@ -871,7 +871,8 @@ struct Arrival {
 // synaptic events: segment s goes to block
 //   min(NB - 1, floor((prefix(s) + events(s) / 2) * NB / total)),
 // a nondecreasing map, so every block gets a contiguous (possibly empty)
-// range; blk_seg[b] = first segment of block b, blk_seg[NB] = n_seg.  One
+// range; blk_seg[b] = first segment of block b, blk_seg[NB] = n_act (the
+// segments [n_act, n_seg) are idle and get no block).  One
 // block of 1024 threads, each owning a contiguous run of segments.
 // Walking a segment costs something even without events (its count is read
 // in a window of kMaxSegPerBlock, under a barrier): each segment weighs its
@@ -885,8 +886,9 @@ __global__ __launch_bounds__(1024) void k_bucket_split(const BucketArgs a)
     const uint32_t cur = a.st->cur;
     const unsigned long long* seg_ev = a.seg_ev + (size_t)cur * a.n_seg;
     auto weight = [&](uint32_t k) { return seg_ev[k] + kSegWeight; };
-    const uint32_t per = (a.n_seg + 1023) / 1024;
-    const uint32_t lo = min(threadIdx.x * per, a.n_seg), hi = min(lo + per, a.n_seg);
+    const uint32_t n_act = a.n_act;
+    const uint32_t per = (n_act + 1023) / 1024;
+    const uint32_t lo = min(threadIdx.x * per, n_act), hi = min(lo + per, n_act);
     unsigned long long sum = 0;
     for (uint32_t k = lo; k < hi; ++k)
         sum += weight(k);
@@ -902,7 +904,7 @@ __global__ __launch_bounds__(1024) void k_bucket_split(const BucketArgs a)
     unsigned long long run = s_part[threadIdx.x] - sum;  // exclusive prefix of segment lo
     const unsigned long long nb = a.NB;
     auto block_of = [&](uint32_t k, unsigned long long pre, unsigned long long e) -> uint32_t {
-        const unsigned long long b = total ? (2ull * pre + e) * nb / (2ull * total) : (unsigned long long)k * nb / a.n_seg;
+        const unsigned long long b = total ? (2ull * pre + e) * nb / (2ull * total) : (unsigned long long)k * nb / n_act;
         return (uint32_t)min(b, nb - 1ull);
     };
     uint32_t prev = 0;
@@ -919,9 +921,9 @@ __global__ __launch_bounds__(1024) void k_bucket_split(const BucketArgs a)
         prev = b;
         run += e;
     }
-    if (hi == a.n_seg && lo < hi) {
+    if (hi == n_act && lo < hi) {
         for (uint32_t q = prev + 1u; q <= a.NB; ++q)
-            a.blk_seg[q] = a.n_seg;
+            a.blk_seg[q] = n_act;
     }
 }
 
@@ -1047,12 +1049,19 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_hist(const BucketArgs
             }
         }
     });
+    // count rows over the active tiles' columns only (idle tiles get no
+    // events; the column scan takes their totals as zero)
     uint32_t* h = a.hist + (size_t)blockIdx.x * a.NBK;
-    for (uint32_t i = threadIdx.x; i < a.NBK; i += blockDim.x)
+    const uint32_t nt = a.n_act, ft = a.M * nt;
+    for (uint32_t x = threadIdx.x; x < ft; x += blockDim.x) {
+        const uint32_t m = x / nt, i = m * a.NT + (x - m * nt);
         h[i] = s_hist[i];
+    }
     const uint32_t nbc = a.NBK >> a.gt_shift;
+    const uint32_t ng = a.NT >> a.gt_shift, ng_act = (nt + (1u << a.gt_shift) - 1) >> a.gt_shift;
     uint32_t* hc = a.hist_c + (size_t)blockIdx.x * nbc;
-    for (uint32_t i = threadIdx.x; i < nbc; i += blockDim.x) {
+    for (uint32_t x = threadIdx.x; x < a.M * ng_act; x += blockDim.x) {
+        const uint32_t m = x / ng_act, i = m * ng + (x - m * ng_act);
         uint32_t sum = 0;
         for (uint32_t t = 0; t < (1u << a.gt_shift); ++t)
             sum += s_hist[(i << a.gt_shift) + t];
@@ -1077,8 +1086,13 @@ __global__ __launch_bounds__(kScanCols * kScanGroups) void k_bucket_colscan(cons
     const uint32_t k = (coarse ? blockIdx.x : blockIdx.x - coarse_blocks) * kScanCols + col;
     const uint32_t rows = (a.NB + kScanGroups - 1) / kScanGroups;
     const uint32_t b0 = min(grp * rows, a.NB), b1 = min(b0 + rows, a.NB);
+    // columns of idle tiles (group columns of idle groups) were not written
+    // by the histogram: their totals are zero and their prefixes unused
+    const uint32_t per_m = coarse ? a.NT >> a.gt_shift : a.NT;
+    const uint32_t act = coarse ? (a.n_act + (1u << a.gt_shift) - 1) >> a.gt_shift : a.n_act;
+    const bool live = k < width && k % per_m < act;
     uint32_t sum = 0;
-    if (k < width) {
+    if (live) {
         for (uint32_t b = b0; b < b1; b += 8) {
             uint32_t c[8];
 #pragma unroll
@@ -1104,7 +1118,7 @@ __global__ __launch_bounds__(kScanCols * kScanGroups) void k_bucket_colscan(cons
     if (!coarse)
         return;
     __syncthreads();
-    if (k >= width)
+    if (!live)
         return;
     uint32_t run = s_part[grp][col];
     for (uint32_t b = b0; b < b1; b += 8) {
@@ -1692,7 +1706,7 @@ static hipError_t launch_update_t(const UpdateArgs& a, hipStream_t s)
             lds_set = lds;
         }
     }
-    hipLaunchKernelGGL((k_update<DET, PLASTIC, BUCKET, EXTRA, TW, REC4>), dim3(a.n_seg), dim3(TW), lds, s, a);
+    hipLaunchKernelGGL((k_update<DET, PLASTIC, BUCKET, EXTRA, TW, REC4>), dim3(a.n_act), dim3(TW), lds, s, a);
     return hipGetLastError();
 }
 

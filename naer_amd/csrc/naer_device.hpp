@@ -184,6 +184,7 @@ struct UpdateArgs {
     uint32_t probe_n;
     uint32_t seg_cap;
     uint32_t n_seg;
+    uint32_t n_act;  // tiles launched: [0, n_act) holds every neuron of a population (<= n_seg)
     uint32_t N, D;
     uint32_t B;
     uint32_t dt_us;
@@ -310,6 +311,8 @@ struct BucketArgs {
     const unsigned long long* seg_ev;  // [2][n_seg]
     uint32_t* blk_seg;                 // [NB + 1]: block b takes segments [blk_seg[b], blk_seg[b + 1])
     uint32_t seg_cap, n_seg;
+    uint32_t n_act;    // segments (tiles) [0, n_act) hold every neuron; the rest are idle (an arena's
+                       // free ranges): no block walks them, their count columns stay zero
     DevState* st;
     const unsigned long long* row_ptr;
     const uint32_t* col;

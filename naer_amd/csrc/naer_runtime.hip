@@ -205,6 +205,7 @@ struct naer_net {
     unsigned long long* cursor = nullptr;  // build scratch, N entries
     naer_spike *list0 = nullptr, *list1 = nullptr, *rec = nullptr;
     uint32_t seg_cap = 0, n_seg = 0;
+    uint32_t n_act = 1;  // tiles [0, n_act) hold every neuron of a population (arenas: the occupied prefix)
     uint32_t* seg_count = nullptr;
     unsigned long long* seg_ev = nullptr;
     uint32_t* blk_seg = nullptr;
@@ -608,6 +609,14 @@ int upload_table(naer_net* n)
             active = n->pp[q].end > j;
         n->blk_pop_h[b] = p | (active ? 0u : kTileIdle);
     }
+    // the update, the scatter and the bucketed passes cover the tiles up to
+    // the last one with a population (first-fit keeps an arena's jobs low)
+    n->n_act = 1;
+    for (uint32_t b = n->n_seg; b > 0; --b)
+        if (!(n->blk_pop_h[b - 1] & kTileIdle)) {
+            n->n_act = b;
+            break;
+        }
     std::vector<float> cls_w;
     std::vector<unsigned long long> cls_fx;
     std::vector<uint32_t> ncls;
@@ -740,6 +749,7 @@ UpdateArgs make_update_args(naer_net* n)
     a.probe_n = n->probe_n;
     a.seg_cap = n->seg_cap;
     a.n_seg = n->n_seg;
+    a.n_act = n->n_act;
     a.N = n->N;
     a.D = n->D;
     a.B = n->B;
@@ -1902,7 +1912,7 @@ static hipError_t do_update(naer_net* n, void* arg)
 static hipError_t do_scatter(naer_net* n, void* arg)
 {
     ScatterArgs* a = (ScatterArgs*)arg;
-    return launch_scatter(*a, n->det, n->plastic, n->n_seg, n->scatter_chunks, n->stream);
+    return launch_scatter(*a, n->det, n->plastic, n->n_act, n->scatter_chunks, n->stream);
 }
 
 FlushArgs flush_args(const naer_net* n, bool force)
@@ -2058,6 +2068,7 @@ static int enqueue_steps(naer_net* n, uint32_t n_steps, bool timed)
     ba.blk_seg = n->blk_seg;
     ba.seg_cap = n->seg_cap;
     ba.n_seg = n->n_seg;
+    ba.n_act = n->n_act;
     ba.st = n->st;
     ba.row_ptr = n->row_ptr;
     ba.col = n->col;
