@@ -219,3 +219,42 @@ def test_non_finite_parameters_are_rejected(what):
         net._pops[1].params[what] = float("nan")
     with pytest.raises(NaerError, match="finite"):
         net.compile()
+
+
+def test_active_tiles_shrink_and_grow_with_the_jobs():
+    """The update, the scatter and the bucketed passes cover the arena's tiles
+    up to the last occupied one.  Retiring the highest job shrinks that
+    prefix (its tiles' spike-list counts go stale), and a job admitted into
+    the freed range grows it again: the jobs below it and the new job give
+    their alone rasters."""
+    from naer_amd.arena import Arena
+    a = Arena(16_384, 4 * 3000 * 300, dt_ms=0.1, mode=_abi.MODE_DETERMINISTIC, delay_steps=(15, 15),
+              record_capacity=1 << 20)
+    specs = [serving.JobSpec(i, 0.0, 3000, 300, 0.03, False, 700 + i) for i in range(4)]
+    slots = [a.admit(s.network()) for s in specs[:3]]
+    assert None not in slots
+    begin, size, _ = a.layout()
+    assert begin[slots[2]] > begin[slots[1]] > begin[slots[0]]  # first fit, low to high
+    got = {s: [] for s in slots}
+
+    def run(n):
+        a.run(n)
+        for s, sp in a.drain().items():
+            got.setdefault(s, []).append(sp)
+
+    run(90)
+    a.retire(slots[2])  # the highest job: the covered prefix shrinks
+    got.pop(slots[2])
+    run(120)
+    s3 = a.admit(specs[3].network())  # first fit: into the freed range, the prefix grows again
+    assert a.layout()[0][s3] == begin[slots[2]]
+    got[s3] = []
+    for _ in range(specs[3].steps() // 60 + 1):
+        run(60)
+    for spec, s in ((specs[0], slots[0]), (specs[1], slots[1]), (specs[3], s3)):
+        mixed = np.sort(np.concatenate(got[s]), order=["step", "neuron", "time_us"])
+        mixed = mixed[mixed["step"] < spec.steps()]
+        alone = serving.run_alone(spec, _abi.MODE_DETERMINISTIC)
+        assert len(alone) > 100
+        assert first_divergence(mixed, alone) is None, (spec, len(mixed), len(alone))
+    a.close()
