@@ -98,12 +98,17 @@ __device__ __forceinline__ float dec_pow(const float* tab, uint32_t n, uint32_t 
 
 // The traces' tables as a kernel reads them: the first n_l entries of each
 // from an LDS copy (the placement kernel stages them), the rest from memory.
+// The LDS pointers carry their address space, so the lookups are ds_read
+// (a generic pointer makes them flat loads, which also wait on the
+// outstanding global loads).
+typedef const __attribute__((address_space(3))) float LdsFloat;
+
 struct TraceTabs {
-    const float *tp_l, *tm_l;
+    LdsFloat *tp_l, *tm_l;
     uint32_t n_l;
 };
 
-__device__ __forceinline__ float dec_pow2(const float* lds, uint32_t n_l, const float* tab, uint32_t n, uint32_t k)
+__device__ __forceinline__ float dec_pow2(LdsFloat* lds, uint32_t n_l, const float* tab, uint32_t n, uint32_t k)
 {
     return k < n_l ? lds[k] : (k < n ? tab[k] : 0.0f);
 }
@@ -1213,7 +1218,7 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_place(const BucketArg
             s_tp[i] = a.sd.tp[i];
             s_tm[i] = a.sd.tm[i];
         }
-        tt = TraceTabs{ s_tp, s_tm, n_l };
+        tt = TraceTabs{ (LdsFloat*)s_tp, (LdsFloat*)s_tm, n_l };
     }
     __syncthreads();
     if (a.is_last && blockIdx.x == 0 && threadIdx.x == 0)
