@@ -437,7 +437,9 @@ typedef struct naer_schedule {
     uint32_t segments_per_block; /* mean spike-list segments per histogram/placement block, 1..4096 */
     uint32_t refine_split;       /* refine blocks per group bucket, 1..64; 0: direct placement,
                                     every event written straight into its tile bucket with no
-                                    refine pass (nets of at most 8192 tile buckets) */
+                                    refine pass, for batches whose tile buckets up to the last
+                                    occupied tile number at most 8192 (the default of arenas;
+                                    a net whose whole tile range exceeds that refuses 0) */
     uint32_t run_aggregate;      /* bit 0: the histogram and refine counters (keyed by target
                                     tile), bit 1: the placement counters (keyed by target group):
                                     one LDS counter atomic per run of equal keys in a wave instead

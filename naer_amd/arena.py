@@ -100,6 +100,11 @@ class Arena:
         """Delivery schedule of the arena (naer_net_get_schedule)."""
         return _abi.get_schedule(self._handle)
 
+    def set_schedule(self, task_synapses, segments_per_block, refine_split, run_aggregate=None):
+        """refine_split 0: direct placement for the batches whose placement
+        counters fit the arena's occupied prefix of tiles."""
+        _abi.set_schedule(self._handle, task_synapses, segments_per_block, refine_split, run_aggregate)
+
     def stats(self):
         s = _abi.Stats()
         check(_abi.lib().naer_net_query(self._handle, ctypes.byref(s)))

@@ -1062,6 +1062,8 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_hist(const BucketArgs
         const uint32_t m = x / nt, i = m * a.NT + (x - m * nt);
         h[i] = s_hist[i];
     }
+    if (a.direct)
+        return;  // the column scan takes the placement's prefixes from the tile rows
     const uint32_t nbc = a.NBK >> a.gt_shift;
     const uint32_t ng = a.NT >> a.gt_shift, ng_act = (nt + (1u << a.gt_shift) - 1) >> a.gt_shift;
     uint32_t* hc = a.hist_c + (size_t)blockIdx.x * nbc;
@@ -1076,25 +1078,29 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_hist(const BucketArgs
 
 // Blocks [0, ceil((NBK >> gt_shift) / 64)): hist_c[b][kc] <- sum of
 // hist_c[b'][kc] for b' < b.  The remaining blocks: totals[k] <- sum over b
-// of hist_f[b][k].  A block covers 64 columns with 16 row groups.
+// of hist_f[b][k].  Direct placement: every block does both on the tile
+// rows, hist_f[b][k] <- its prefix, totals[k] <- the column's sum.  A block
+// covers 64 columns with 16 row groups.
 constexpr uint32_t kScanCols = 64, kScanGroups = 16;
 
 __global__ __launch_bounds__(kScanCols * kScanGroups) void k_bucket_colscan(const BucketArgs a)
 {
     __shared__ uint32_t s_part[kScanGroups][kScanCols];
+    // direct placement: one set of blocks over the tile columns, prefixes in
+    // place (the placement's) and totals
     const uint32_t nbc = a.NBK >> a.gt_shift;
-    const uint32_t coarse_blocks = (nbc + kScanCols - 1) / kScanCols;
-    const bool coarse = blockIdx.x < coarse_blocks;
-    uint32_t* hist = coarse ? a.hist_c : a.hist;
-    const uint32_t width = coarse ? nbc : a.NBK;
+    const uint32_t coarse_blocks = a.direct ? 0u : (nbc + kScanCols - 1) / kScanCols;
+    const bool coarse = a.direct || blockIdx.x < coarse_blocks;
+    uint32_t* hist = (coarse && !a.direct) ? a.hist_c : a.hist;
+    const uint32_t width = (coarse && !a.direct) ? nbc : a.NBK;
     const uint32_t col = threadIdx.x % kScanCols, grp = threadIdx.x / kScanCols;
-    const uint32_t k = (coarse ? blockIdx.x : blockIdx.x - coarse_blocks) * kScanCols + col;
+    const uint32_t k = (blockIdx.x - (coarse && !a.direct ? 0u : coarse_blocks)) * kScanCols + col;
     const uint32_t rows = (a.NB + kScanGroups - 1) / kScanGroups;
     const uint32_t b0 = min(grp * rows, a.NB), b1 = min(b0 + rows, a.NB);
     // columns of idle tiles (group columns of idle groups) were not written
     // by the histogram: their totals are zero and their prefixes unused
-    const uint32_t per_m = coarse ? a.NT >> a.gt_shift : a.NT;
-    const uint32_t act = coarse ? (a.n_act + (1u << a.gt_shift) - 1) >> a.gt_shift : a.n_act;
+    const uint32_t per_m = (coarse && !a.direct) ? a.NT >> a.gt_shift : a.NT;
+    const uint32_t act = (coarse && !a.direct) ? (a.n_act + (1u << a.gt_shift) - 1) >> a.gt_shift : a.n_act;
     const bool live = k < width && k % per_m < act;
     uint32_t sum = 0;
     if (live) {
@@ -1117,7 +1123,7 @@ __global__ __launch_bounds__(kScanCols * kScanGroups) void k_bucket_colscan(cons
             s_part[g][col] = run;
             run += x;
         }
-        if (!coarse && k < width)
+        if ((!coarse || a.direct) && k < width)
             a.totals[k] = run;
     }
     if (!coarse)
@@ -1205,8 +1211,8 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_place(const BucketArg
     const uint32_t g_shift = a.tw_shift + a.gt_shift;
     if (ok) {
         const uint32_t* bs = a.bs + (size_t)buf * (a.NBK + 1);
-        const uint32_t* h = a.hist_c + (size_t)blockIdx.x * nbc;
-        for (uint32_t i = threadIdx.x; i < nbc; i += blockDim.x)
+        const uint32_t* h = a.direct ? a.hist + (size_t)blockIdx.x * a.NBK : a.hist_c + (size_t)blockIdx.x * nbc;
+        for (uint32_t i = threadIdx.x; i < a.n_cnt; i += blockDim.x)
             s_cur[i] = bs[i << a.gt_shift] + h[i];
     }
     const Arrival arr(a.Bfull);
@@ -1841,10 +1847,11 @@ hipError_t launch_bucket_step(const BucketArgs& a, int step, bool det, hipStream
     if (e != hipSuccess)
         return e;
     const size_t lds = (size_t)a.NBK * sizeof(uint32_t);
-    const size_t lds_c = (size_t)(a.NBK >> a.gt_shift) * sizeof(uint32_t);
+    const size_t lds_c = (size_t)a.n_cnt * sizeof(uint32_t);
     switch (step) {
     case 1:
-        hipLaunchKernelGGL(k_bucket_colscan, dim3(cdiv(a.NBK >> a.gt_shift, kScanCols) + cdiv(a.NBK, kScanCols)),
+        hipLaunchKernelGGL(k_bucket_colscan,
+                           dim3((a.direct ? 0u : cdiv(a.NBK >> a.gt_shift, kScanCols)) + cdiv(a.NBK, kScanCols)),
                            dim3(kScanCols * kScanGroups), 0, s, a);
         hipLaunchKernelGGL(k_bucket_starts, dim3(1), dim3(1024), lds, s, a);
         break;

@@ -336,7 +336,10 @@ struct BucketArgs {
     uint32_t U;        // synapses per lane per task: 4 or 8
     uint32_t RS;       // refine blocks per group bucket (on average: they split the batch evenly)
     uint32_t direct;   // 1: groups are single tiles (gt_shift 0) and the placement writes each event
-                       // straight into its tile bucket of ev[E % R]; no refine pass
+                       // straight into its tile bucket of ev[E % R]; no refine pass; the column scan
+                       // leaves the per-block prefixes in the tile-level rows (no group rows)
+    uint32_t n_cnt;    // placement LDS counters in use: keys [0, n_cnt) (group buckets, or with direct
+                       // placement the tile buckets up to the last active tile of the last arrival batch)
     uint32_t balance;  // 1: k_bucket_split sets blk_seg per batch; 0: fixed ranges of G segments
     uint32_t AG;       // 1: histogram/refine LDS counters take one atomic per run of equal keys in a wave
     uint32_t AGP;      // 1: the same for the placement counters

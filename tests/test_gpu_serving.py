@@ -221,15 +221,21 @@ def test_non_finite_parameters_are_rejected(what):
         net.compile()
 
 
-def test_active_tiles_shrink_and_grow_with_the_jobs():
+@pytest.mark.parametrize("direct", [False, True])
+def test_active_tiles_shrink_and_grow_with_the_jobs(direct):
     """The update, the scatter and the bucketed passes cover the arena's tiles
     up to the last occupied one.  Retiring the highest job shrinks that
     prefix (its tiles' spike-list counts go stale), and a job admitted into
     the freed range grows it again: the jobs below it and the new job give
-    their alone rasters."""
+    their alone rasters.  direct: the placement writes the events straight
+    into the tile buckets (counters for the occupied prefix only)."""
     from naer_amd.arena import Arena
     a = Arena(16_384, 4 * 3000 * 300, dt_ms=0.1, mode=_abi.MODE_DETERMINISTIC, delay_steps=(15, 15),
               record_capacity=1 << 20)
+    if direct:
+        sc = a.schedule()
+        a.set_schedule(sc["task_synapses"], sc["segments_per_block"], 0)
+        assert a.schedule()["refine_split"] == 0
     specs = [serving.JobSpec(i, 0.0, 3000, 300, 0.03, False, 700 + i) for i in range(4)]
     slots = [a.admit(s.network()) for s in specs[:3]]
     assert None not in slots
