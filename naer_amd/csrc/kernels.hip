@@ -1009,9 +1009,19 @@ __device__ __forceinline__ void for_block_chunks(const BucketArgs& a, uint32_t c
                     rs.cls[i] = a.ncls[sp.neuron];
             }
             __syncthreads();
-            const uint32_t n_tasks = cnt * a.C;
+            // tasks per spike: those of the round's longest row, not of the
+            // longest row the net may hold (an arena's max_row is far above
+            // its jobs' rows, which would leave most tasks empty); every wave
+            // takes the maximum over the staged rows itself
+            uint32_t lmax = 0;
+            for (uint32_t i = lane_id(); i < cnt; i += kWave)
+                lmax = max(lmax, rs.len[i]);
+            for (uint32_t off = kWave / 2; off > 0; off >>= 1)
+                lmax = max(lmax, (uint32_t)__shfl_xor((int)lmax, (int)off));
+            const uint32_t C = rfl((lmax + kTask - 1) / kTask);
+            const uint32_t n_tasks = cnt * C;
             for (uint32_t t = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave); t < n_tasks; t += nwv) {
-                const uint32_t i = t / a.C, c = t - i * a.C;
+                const uint32_t i = t / C, c = t - i * C;
                 const uint32_t off = c * kTask, len = rs.len[i];
                 if (off < len)
                     f(rs.r[i], rs.nrn[i], rs.beg[i] + off, rs.beg[i] + min(len, off + kTask), i);
