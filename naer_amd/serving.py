@@ -6,8 +6,8 @@ uniform in [0.2, 2] s of simulated time, 30% with pair STDP on E->E; J_E =
 0.1 mV, g = 5, 20 kHz x 0.1 mV drive, delay 1.5 ms).
 
 The server keeps two arenas (static jobs, plastic jobs) whose capacities split
-the HBM budget.  Each round it enqueues the next chunk of `chunk` steps of every
-resident job (the two arenas on their own streams, so their work overlaps on
+the HBM budget.  Each round it enqueues the next chunk of `chunk` steps (rounded to
+whole batches of the arenas) of every resident job (the two arenas on their own streams, so their work overlaps on
 the device), admits waiting jobs that fit while that chunk runs (first come
 first served, later jobs may pass a head that does not fit yet; an admission
 builds its connectivity on a second stream of the arena, and the job starts
@@ -168,6 +168,16 @@ class Server:
                              record_capacity=record_per_chunk)
         self.arenas = [arena_factory(static_cap, False), arena_factory(plastic_cap, True)]
         self.capacity = (static_cap, plastic_cap)
+        # chunks of whole batches of every arena: an arena delivers a partial
+        # batch through the delay ring's device atomics (it admits and retires
+        # jobs between chunks, so it does not defer), several times slower
+        # than its buckets
+        batch = 1
+        for a in self.arenas:
+            if hasattr(a, "stats"):
+                batch = math.lcm(batch, max(1, int(a.stats().batch)))
+        self.chunk = max(batch, int(round(chunk / batch)) * batch)
+        chunk = self.chunk
         if graphs:
             for a in self.arenas:
                 if hasattr(a, "set_graph"):
