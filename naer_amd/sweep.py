@@ -11,8 +11,9 @@ Two ways to run it:
               more (a flush of pending potentiation runs over plastic packs
               only; a plastic network steps about 1.6x slower, so its work is
               split to end with the static pack's), every pack on its own
-              stream, steps enqueued in graph-captured chunks, delivery
-              schedules from the autotuner;
+              stream, steps enqueued in graph-captured chunks (100 steps
+              rounded up to whole batches: 105 at the sweep's 15-step
+              batches), delivery schedules from the autotuner;
   sequential  the plain loop: one network per launch sequence, one after the
               other.
 In deterministic mode every network's raster is bit-identical either way.
@@ -75,7 +76,7 @@ def pack_groups(specs, packs_per_kind=1, plastic_packs=None):
 
 
 def run_packed(specs, steps, mode=_abi.MODE_DETERMINISTIC, chunk=100, record=True, graphs=True, verbose=False,
-               autotune=True, packs_per_kind=1, plastic_packs=None):
+               autotune=True, packs_per_kind=1, plastic_packs=2):
     # each pack runs on its own stream
     groups = pack_groups(specs, packs_per_kind, plastic_packs)
     packs = []
@@ -120,10 +121,15 @@ def run_packed(specs, steps, mode=_abi.MODE_DETERMINISTIC, chunk=100, record=Tru
                 done[k] = step
                 if verbose:
                     _progress(f"packed: schedule {sched}")
+    # a captured chunk holds whole batches (naer_net_set_graph rounds 100
+    # steps up to 105 at 15-step batches): stepping in that unit replays the
+    # graph, and no run ends inside a batch (a plastic pack would deliver
+    # such a partial batch through the ring's device atomics)
+    unit = [-(-chunk // int(p.stats().batch)) * int(p.stats().batch) for _, p in packs]
     last = t0
     while min(done) < steps:
         for k, (_, p) in enumerate(packs):
-            c = min(chunk, steps - done[k])
+            c = min(unit[k], steps - done[k])
             if c > 0:
                 p.run(c)
                 done[k] += c
