@@ -1906,11 +1906,13 @@ hipError_t launch_scatter(const ScatterArgs& a, bool det, bool plastic, uint32_t
     return hipGetLastError();
 }
 
-hipError_t launch_check_finite(const UpdateArgs& a, uint32_t* flags, hipStream_t s)
+hipError_t launch_check_finite(const UpdateArgs& a, uint32_t* flags, uint32_t n_check, hipStream_t s)
 {
     if (a.N == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_check_finite, dim3(cdiv(a.N, kBlock)), dim3(kBlock), 0, s, a, flags);
+    if (n_check == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_check_finite, dim3(cdiv(n_check, kBlock)), dim3(kBlock), 0, s, a, flags);
     return hipGetLastError();
 }
 

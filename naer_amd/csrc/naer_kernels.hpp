@@ -22,7 +22,7 @@ hipError_t launch_scatter(const ScatterArgs& a, bool det, bool plastic, uint32_t
 hipError_t launch_flush(const FlushArgs& a, uint32_t grid, hipStream_t s);
 hipError_t launch_clock_advance(DevState* st, uint32_t b, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, unsigned long long n, uint32_t v, hipStream_t s);
-hipError_t launch_check_finite(const UpdateArgs& a, uint32_t* flags, hipStream_t s);
+hipError_t launch_check_finite(const UpdateArgs& a, uint32_t* flags, uint32_t n_check, hipStream_t s);
 hipError_t launch_conn(const ConnArgs& c, uint32_t rule, bool place, uint32_t* deg,
                        unsigned long long* cursor, uint32_t* col, float* w, hipStream_t s);
 hipError_t launch_sort_rows(const unsigned long long* row_ptr, uint32_t* col, float* w, uint32_t row_begin,

@@ -1868,10 +1868,14 @@ int naer_net_check_finite(naer_net* n, uint32_t* bad_slots, uint32_t cap, uint32
     const uint32_t chunks = (n->N + kBlock - 1) / kBlock;
     if (!n->finite_flags && (rc = n->alloc(&n->finite_flags, chunks)))
         return rc;
+    // every active neuron lies in the tiles [0, n_act): only those chunks are
+    // checked and read back
+    const uint32_t act = (uint32_t)std::min<uint64_t>(n->N, (uint64_t)n->n_act << n->tw_shift);
+    const uint32_t act_chunks = (act + kBlock - 1) / kBlock;
     const UpdateArgs ua = make_update_args(n);
-    HIP_TRY(launch_check_finite(ua, n->finite_flags, n->stream));
-    std::vector<uint32_t> flags(chunks);
-    HIP_TRY(hipMemcpyAsync(flags.data(), n->finite_flags, chunks * sizeof(uint32_t), hipMemcpyDeviceToHost,
+    HIP_TRY(launch_check_finite(ua, n->finite_flags, act, n->stream));
+    std::vector<uint32_t> flags(chunks, 0u);
+    HIP_TRY(hipMemcpyAsync(flags.data(), n->finite_flags, act_chunks * sizeof(uint32_t), hipMemcpyDeviceToHost,
                            n->stream));
     HIP_TRY(hipStreamSynchronize(n->stream));
     uint32_t k = 0;
