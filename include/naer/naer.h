@@ -398,6 +398,11 @@ int naer_net_set_graph(naer_net* net, uint32_t chunk_steps);
 /* Wait for all enqueued work (Network::finish, src/Network.cpp:258-265). */
 int naer_net_sync(naer_net* net);
 
+/* Whether all enqueued work is done, without waiting: *idle = 1 or 0.  No
+ * counterpart in the reference (one net, one loop); a server polls its arenas
+ * with it and steps each again as soon as its last chunk is done. */
+int naer_net_idle(naer_net* net, int* idle);
+
 /* Counters and timers (getTime/getSynapses/getNeurons and the public
  * *_time fields, include/naer/Network.h:44-55).  Synchronises the net. */
 int naer_net_query(naer_net* net, naer_stats* out);

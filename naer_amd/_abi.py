@@ -156,6 +156,7 @@ _SIGS = {
     "naer_net_build": [ctypes.c_void_p],
     "naer_net_step": [ctypes.c_void_p, u32],
     "naer_net_sync": [ctypes.c_void_p],
+    "naer_net_idle": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)],
     "naer_net_set_graph": [ctypes.c_void_p, u32],
     "naer_net_query": [ctypes.c_void_p, ctypes.POINTER(Stats)],
     "naer_net_set_profiling": [ctypes.c_void_p, ctypes.c_int],

@@ -93,6 +93,12 @@ class Arena:
     def run(self, n_steps):
         check(_abi.lib().naer_net_step(self._handle, int(n_steps)))
 
+    def idle(self):
+        """True when the arena's enqueued steps are done (naer_net_idle; does not wait)."""
+        v = ctypes.c_int()
+        check(_abi.lib().naer_net_idle(self._handle, ctypes.byref(v)))
+        return v.value != 0
+
     def sync(self):
         check(_abi.lib().naer_net_sync(self._handle))
 

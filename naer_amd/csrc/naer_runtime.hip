@@ -2425,6 +2425,21 @@ int naer_net_autotune(naer_net* n, uint32_t batches, naer_schedule* chosen)
     return NAER_OK;
 }
 
+int naer_net_idle(naer_net* n, int* idle)
+{
+    if (!n || !idle)
+        return fail(NAER_ERR_INVALID, "NULL argument");
+    const hipError_t e = hipStreamQuery(n->stream);
+    if (e == hipErrorNotReady) {
+        *idle = 0;
+        return NAER_OK;
+    }
+    if (e != hipSuccess)
+        return fail(NAER_ERR_HIP, "stream query failed: %s", hipGetErrorString(e));
+    *idle = 1;
+    return NAER_OK;
+}
+
 int naer_net_sync(naer_net* n)
 {
     if (!n)

@@ -6,13 +6,14 @@ uniform in [0.2, 2] s of simulated time, 30% with pair STDP on E->E; J_E =
 0.1 mV, g = 5, 20 kHz x 0.1 mV drive, delay 1.5 ms).
 
 The server keeps two arenas (static jobs, plastic jobs) whose capacities split
-the HBM budget.  Each round it enqueues the next chunk of `chunk` steps (rounded to
-whole batches of the arenas) of every resident job (the two arenas on their own streams, so their work overlaps on
-the device), admits waiting jobs that fit while that chunk runs (first come
-first served, later jobs may pass a head that does not fit yet; an admission
-builds its connectivity on a second stream of the arena, and the job starts
-with the following chunk), then drains the recorded spikes and retires the
-jobs that have reached their duration.  A job's raster depends only
+the HBM budget, each on its own stream, and services each arena as soon as its
+own chunk is done (naer_net_idle): it drains the recorded spikes, runs the
+watchdog, retires the jobs that have reached their duration, admits waiting
+jobs that fit (first come first served, later jobs may pass a head that does
+not fit yet), and enqueues the next chunk of `chunk` steps (rounded to whole
+batches of the arena) of every resident job, the newly admitted ones
+included.  The two arenas thus step independently and their work overlaps on
+the device.  A job's raster depends only
 on its own description: alone or in any mix it is the same, bit for bit in
 deterministic mode.
 
@@ -216,28 +217,25 @@ class Server:
                 return "timeout"
             return None
 
-        while pending or waiting or any(active):
-            now = clock() - t0
-            while pending and pending[0].arrival_s <= now:
-                waiting.append(pending.popleft())
-            # the next chunk of the running jobs goes to the GPU first: the
-            # admissions below build their connectivity on a second stream
-            # meanwhile, and their jobs start with the chunk after it
-            ran = [False, False]
-            for ai, arena in enumerate(self.arenas):
-                if active[ai]:
-                    arena.run(self.chunk)
-                    ran[ai] = True
-            admitted = [set(), set()]
-            # admission: first come first served; a job that does not fit yet
-            # waits while later jobs may pass it
+        # per arena: the slots that run in its enqueued chunk (None: no chunk
+        # in flight).  Each arena is serviced as soon as its own chunk is done
+        # (naer_net_idle), so the two step independently on the device.
+        in_flight = [None, None]
+
+        def admit(ai, now):
+            """Waiting jobs of arena ai, first come first served (a job that
+            does not fit yet waits while later ones may pass it); the arena
+            has no chunk in flight, so an admission does not wait for one."""
+            nonlocal waiting
             still = []
             for job in waiting:
                 why = expired(job, now)
                 if why:
                     finish(job, why)
                     continue
-                ai = 1 if job.plastic else 0
+                if (1 if job.plastic else 0) != ai:
+                    still.append(job)
+                    continue
                 cap = self.capacity[ai]
                 if arena_neurons(job) > cap.neurons or job.n_synapses() > cap.synapses or \
                         job.indegree * 4 > MAX_ROW:
@@ -252,53 +250,73 @@ class Server:
                     still.append(job)
                     continue
                 active[ai][slot] = [job, 0]
-                admitted[ai].add(slot)
                 r = res[job.index]
                 r.admitted_s, r.slot, r.arena, r.status = clock() - t0, slot, ai, "running"
             waiting = still
-            if not any(ran):
-                if not any(active):
-                    if waiting and not pending:
-                        # the arenas are empty and still cannot take them
-                        for job in waiting:
-                            finish(job, "rejected", "does not fit an empty arena")
-                        waiting = []
-                    elif pending:
-                        sleep(max(0.0, min(0.05, pending[0].arrival_s - (clock() - t0))))
-                    elif waiting:
-                        sleep(0.001)
-                continue
+
+        def collect(ai):
+            """Arena ai's chunk is done: its spikes, the watchdog, the jobs
+            that reached their duration or were cancelled retire."""
+            arena = self.arenas[ai]
+            ran = in_flight[ai]
+            in_flight[ai] = None
+            drained = arena.drain()
+            bad = set(arena.check()) if watchdog and hasattr(arena, "check") else set()
+            now = clock() - t0
+            done = []
+            for slot in ran:
+                job, steps = active[ai][slot]
+                total = job.steps(self.dt_ms)
+                sp = drained.get(slot)
+                if sp is not None and len(sp):
+                    sp = sp[sp["step"] < total]
+                    res[job.index].spikes += len(sp)
+                    if self.keep:
+                        res[job.index].raster.append(sp)
+                active[ai][slot][1] = steps + self.chunk
+                if slot in bad:
+                    done.append((slot, "failed", "a potential is not finite"))
+                elif steps + self.chunk >= total:
+                    done.append((slot, "done", ""))
+                else:
+                    why = expired(job, now)
+                    if why:
+                        done.append((slot, why, ""))
+            for slot, status, error in done:
+                job = active[ai].pop(slot)[0]
+                arena.retire(slot)
+                finish(job, status, error)
+
+        while pending or waiting or any(active):
+            now = clock() - t0
+            while pending and pending[0].arrival_s <= now:
+                waiting.append(pending.popleft())
+            serviced = False
             for ai, arena in enumerate(self.arenas):
-                if not ran[ai]:
-                    continue
-                drained = arena.drain()
-                bad = set(arena.check()) if watchdog and hasattr(arena, "check") else set()
-                now = clock() - t0
-                done = []
-                for slot, entry in active[ai].items():
-                    if slot in admitted[ai]:
-                        continue  # admitted during this chunk: it starts with the next one
-                    job, steps = entry
-                    total = job.steps(self.dt_ms)
-                    sp = drained.get(slot)
-                    if sp is not None and len(sp):
-                        sp = sp[sp["step"] < total]
-                        res[job.index].spikes += len(sp)
-                        if self.keep:
-                            res[job.index].raster.append(sp)
-                    entry[1] = steps + self.chunk
-                    if slot in bad:
-                        done.append((slot, "failed", "a potential is not finite"))
-                    elif entry[1] >= total:
-                        done.append((slot, "done", ""))
-                    else:
-                        why = expired(job, now)
-                        if why:
-                            done.append((slot, why, ""))
-                for slot, status, error in done:
-                    job = active[ai].pop(slot)[0]
-                    arena.retire(slot)
-                    finish(job, status, error)
+                if in_flight[ai] is not None:
+                    if hasattr(arena, "idle") and not arena.idle():
+                        continue
+                    collect(ai)
+                    serviced = True
+                admit(ai, clock() - t0)
+                if active[ai]:
+                    # the next chunk of every resident job, admitted ones included
+                    arena.run(self.chunk)
+                    in_flight[ai] = set(active[ai])
+                    serviced = True
+            if serviced:
+                continue
+            if any(f is not None for f in in_flight):
+                sleep(0.0002)  # a chunk is running: poll again shortly
+            elif waiting and not pending:
+                # the arenas are empty and still cannot take them
+                for job in waiting:
+                    finish(job, "rejected", "does not fit an empty arena")
+                waiting = []
+            elif pending:
+                sleep(max(0.0, min(0.05, pending[0].arrival_s - (clock() - t0))))
+            elif waiting:
+                sleep(0.001)
         makespan = clock() - t0
         finished = [r for r in res.values() if r.status == "done"]
         lat = np.array([r.latency_s for r in finished]) if finished else np.zeros(1)
