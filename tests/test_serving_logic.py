@@ -232,3 +232,56 @@ def test_job_of_exactly_the_arena_capacity_is_rejected():
     out = srv.serve(jobs, clock=clock, sleep=clock.sleep)
     assert out["results"][0].status == "rejected"
     assert out["results"][1].status == "done"
+
+
+class BusyArena(FakeArena):
+    """FakeArena whose enqueued chunk stays 'running' for `polls` calls of
+    idle() (naer_net_idle), as a device chunk does until it completes."""
+
+    def __init__(self, cap, plastic, polls, **kw):
+        super().__init__(cap, plastic, **kw)
+        self.polls = polls
+        self.left = 0
+        self.other = None
+        self.overlapped = 0  # chunks enqueued while the other arena's chunk was running
+
+    def run(self, steps):
+        assert self.left == 0, "a chunk was enqueued before the last one was collected"
+        super().run(steps)
+        self.left = self.polls
+        if self.other is not None and self.other.left:
+            self.overlapped += 1
+
+    def idle(self):
+        if self.left:
+            self.left -= 1
+            return False
+        return True
+
+    def drain(self):
+        assert self.left == 0, "drained while the chunk was still running"
+        return super().drain()
+
+
+def test_arenas_are_serviced_independently():
+    """Each arena is collected and stepped again as soon as its own chunk is
+    done: a slow plastic arena does not hold back the static one, and every
+    job's raster is the one it gets with arenas that are always done."""
+    jobs = serving.served_jobs(30, rate_per_s=500.0, n_range=(2000, 20000), k_range=(500, 1500),
+                               stdp_fraction=0.5)
+    budget = 6 * max(serving.footprint(j) for j in jobs)
+    base = make_server(budget)
+    clock = FakeClock()
+    ref = base.serve(jobs, clock=clock, sleep=clock.sleep)
+    srv = serving.Server(budget, graphs=False, keep_rasters=True,
+                         arena_factory=lambda cap, plastic: BusyArena(cap, plastic, polls=12 if plastic else 2))
+    static, plastic = srv.arenas
+    static.other, plastic.other = plastic, static
+    clock = FakeClock()
+    out = srv.serve(jobs, clock=clock, sleep=clock.sleep)
+    assert out["completed"] == ref["completed"] == len(jobs)
+    for i, r in out["results"].items():
+        assert r.status == "done" and np.array_equal(r.raster, ref["results"][i].raster), i
+    # the static arena was stepped again while the plastic one's chunk still ran
+    assert static.overlapped > 10
+    assert static.used_n == 0 and plastic.used_n == 0
