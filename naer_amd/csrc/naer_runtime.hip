@@ -1124,9 +1124,12 @@ int alloc_buckets(naer_net* n)
     n->G = std::max<uint32_t>(1, (n->n_seg + 255) / 256);
     n->NB = (n->n_seg + n->G - 1) / n->G;
     n->U = kBucketUnroll;
-    // arenas place directly whenever their occupied prefix fits the counters
-    // (serving at 10-20 jobs/s: 2.6-5 % more jobs/s than two levels)
-    n->RS = n->arena ? 0u : default_refine_split(n);
+    // packs and arenas place directly whenever their (occupied) tiles fit the
+    // counters: they hold many events per (placement block, tile bucket)
+    // (measured: the sweep's packs alone 13-25 % faster per batch, serving at
+    // 10-20 jobs/s 2.6-5 % more jobs/s); a single net keeps two levels
+    const bool pack_fits = n->nets.size() > 1 && direct_counters(n, n->n_seg) <= kMaxPlaceCounters;
+    n->RS = (n->arena || pack_fits) ? 0u : default_refine_split(n);
     n->AG = run_aggregation(n);
     // placement counters: one per group bucket in LDS (<= 32 KB without raising the limit)
     if (n->M > kMaxArrivalBatches || (n->NBK >> n->gt_shift) > kMaxPlaceCounters)
@@ -2352,7 +2355,9 @@ int naer_net_autotune(naer_net* n, uint32_t batches, naer_schedule* chosen)
     naer_schedule best;
     naer_net_get_schedule(n, &best);
     double best_us = 0.0;
-    // stage 1: the placement
+    // stage 1: the placement; the other one replaces the net's default only
+    // when it is clearly faster (trials of a few batches, possibly beside
+    // other packs tuning on the same device, are noisy)
     if (direct_ok(n)) {
         naer_schedule two = best, one = best;
         if (two.refine_split == 0)
@@ -2360,7 +2365,8 @@ int naer_net_autotune(naer_net* n, uint32_t batches, naer_schedule* chosen)
         one.refine_split = 0;
         const std::vector<naer_schedule> cand{ two, one };
         if ((rc = trial(cand, score, per_batch)) == NAER_OK) {
-            const size_t i = argmin(score);
+            const size_t def = best.refine_split == 0 ? 1 : 0, alt = 1 - def;
+            const size_t i = score[alt] < 0.9 * score[def] ? alt : def;
             best = cand[i];
             best_us = per_batch[i];
         }
