@@ -51,16 +51,20 @@ def test_pack_equals_alone_and_oracle():
         assert first_divergence(alone[i], o.spikes()) is None, i
 
 
-def test_direct_placement_pack_equals_alone():
-    """Packs with schedule refine_split 0 (events straight into their tile
-    buckets, no refine pass) against each network alone (default schedule)."""
+@pytest.mark.parametrize("refine_split", [0, 3])
+def test_pack_placements_equal_alone(refine_split):
+    """Packs place directly by default (refine_split 0: events straight into
+    their tile buckets, no refine pass); refine_split 3: the two-level
+    placement with three refine blocks per group bucket.  Against each
+    network alone (the single-net default, two levels)."""
     specs = _specs()
     nets = [sweep.make_network(s, _abi.MODE_DETERMINISTIC, sweep.expected_record(s, STEPS)) for s in specs]
     p = Pack(nets)
     p.build()
     sc = p.schedule()
-    p.set_schedule(sc["task_synapses"], sc["segments_per_block"], 0)
-    assert p.schedule()["refine_split"] == 0
+    assert sc["refine_split"] == 0  # the packs' default
+    p.set_schedule(sc["task_synapses"], sc["segments_per_block"], refine_split)
+    assert p.schedule()["refine_split"] == refine_split
     p.run(STEPS)
     packed = p.spikes()
     p.close()
