@@ -255,6 +255,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=200, help="batches of the profiled stretch (>= 20)")
+    ap.add_argument("--reserve", type=int, choices=[0, 1], default=None,
+                    help="delivery placement: 1 reserved (no counting passes), 0 counted two-level (default: the "
+                         "library's for the network)")
     ap.add_argument("--autotune", action="store_true",
                     help="let the autotuner pick the delivery schedule during the warm-up (default: the "
                          "library's schedule for the network)")
@@ -300,6 +303,10 @@ def main():
     settle = -(-int(round(args.settle_ms / net.dt_ms)) // B) * B
     net.run(settle)
     net.run(B * (args.warmup // 2))
+    if args.reserve is not None:
+        s = net.schedule()
+        parts = min(4, s["parts_max"]) if args.reserve else s["refine_split"] or 1
+        net.set_schedule(s["task_synapses"], s["segments_per_block"], parts, reserve=args.reserve)
     schedule = net.autotune(2) if args.warmup > 0 and args.autotune else net.schedule()
     done = int(net.stats().step)
     target = max(B * args.warmup, -(-done // B) * B)
@@ -357,7 +364,8 @@ def main():
         kernels["k_update"] = {"avg_us": round(up_ms * 1e3 / max(up_n, 1), 2), "launches": int(up_n),
                                "alg_bytes_per_launch": N * B * BYTES_PER_NEURON_STEP}
         names = {_abi.K_BUCKET_HIST: "k_bucket_hist", _abi.K_BUCKET_SCAN: "k_bucket_colscan+k_bucket_starts",
-                 _abi.K_BUCKET_PLACE: "k_bucket_place", _abi.K_BUCKET_REFINE: "k_bucket_refine"}
+                 _abi.K_BUCKET_PLACE: "k_bucket_place",
+                 _abi.K_BUCKET_REFINE: "k_bucket_refine_res" if schedule.get("reserve") else "k_bucket_refine"}
         bucketed = False
         for k, name in names.items():
             ms, n = part(k)
@@ -373,8 +381,8 @@ def main():
         roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(ach / HBM_PEAK_GBS, 4),
                     "traffic": traffic,
-                    "kernel": ("spike delivery: k_bucket_hist + k_bucket_colscan + k_bucket_starts + "
-                               "k_bucket_place + k_bucket_refine") if bucketed else "k_scatter",
+                    "kernel": ("spike delivery: " + " + ".join(k for k in kernels if k != "k_update")
+                               .replace("+k_bucket_starts", " + k_bucket_starts")) if bucketed else "k_scatter",
                     "avg_us": round(dl_us, 2), "bytes_per_launch": round(dl_bytes),
                     "events_per_launch": round(prof_events / max(dl_n, 1)),
                     "profiled_batches": int(dl_n),

@@ -449,6 +449,15 @@ typedef struct naer_schedule {
                                     tile), bit 1: the placement counters (keyed by target group):
                                     one LDS counter atomic per run of equal keys in a wave instead
                                     of one per synapse */
+    uint32_t reserve;            /* 1: reserved placement (single nets): no histogram, column scan
+                                    or bucket-start passes; each placement block writes into
+                                    regions of the staging buffer sized from the batch's event
+                                    count and the net's synapse shares per (arrival batch, group),
+                                    events beyond them into spill regions (beyond those, the ring);
+                                    refine_split is then the refine parts per group bucket,
+                                    1..parts_max.  Refused where the shares are unknown (packs,
+                                    arenas, nets whose (delay, group) table exceeds 8192 entries) */
+    uint32_t parts_max;          /* out: largest refine_split of reserved placement (0: unavailable) */
     uint32_t record_bytes;       /* out: bytes per bucketed event: 4 when every population's
                                     outgoing synapses share one constant weight (the record
                                     names the source's population), else 8 (the record carries

@@ -93,7 +93,8 @@ class Spike(ctypes.Structure):
 class Schedule(ctypes.Structure):
     _fields_ = [("bucketed", ctypes.c_uint32), ("task_synapses", ctypes.c_uint32),
                 ("segments_per_block", ctypes.c_uint32), ("refine_split", ctypes.c_uint32),
-                ("run_aggregate", ctypes.c_uint32), ("record_bytes", ctypes.c_uint32),
+                ("run_aggregate", ctypes.c_uint32), ("reserve", ctypes.c_uint32),
+                ("parts_max", ctypes.c_uint32), ("record_bytes", ctypes.c_uint32),
                 ("delivery_us", ctypes.c_double)]
 
     def as_dict(self):
@@ -201,17 +202,29 @@ def lib():
     return _lib
 
 
+def steps_arg(n_steps):
+    """A step count for naer_net_step (uint32): negative or larger counts are refused."""
+    n = int(n_steps)
+    if n < 0 or n > 0xFFFFFFFF:
+        raise ValueError(f"step count {n} outside [0, 2^32)")
+    return n
+
+
 def get_schedule(handle):
     s = Schedule()
     check(lib().naer_net_get_schedule(handle, ctypes.byref(s)))
     return s.as_dict()
 
 
-def set_schedule(handle, task_synapses, segments_per_block, refine_split, run_aggregate=None):
-    """run_aggregate None keeps the current counter mode."""
+def set_schedule(handle, task_synapses, segments_per_block, refine_split, run_aggregate=None, reserve=None):
+    """run_aggregate / reserve None keep the current counter mode / placement."""
+    cur = get_schedule(handle)
     if run_aggregate is None:
-        run_aggregate = get_schedule(handle)["run_aggregate"]
-    s = Schedule(0, int(task_synapses), int(segments_per_block), int(refine_split), int(run_aggregate), 0, 0.0)
+        run_aggregate = cur["run_aggregate"]
+    if reserve is None:
+        reserve = cur["reserve"]
+    s = Schedule(0, int(task_synapses), int(segments_per_block), int(refine_split), int(run_aggregate),
+                 int(reserve), 0, 0, 0.0)
     check(lib().naer_net_set_schedule(handle, ctypes.byref(s)))
 
 

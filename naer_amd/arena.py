@@ -91,7 +91,7 @@ class Arena:
         check(_abi.lib().naer_net_set_graph(self._handle, int(chunk_steps)))
 
     def run(self, n_steps):
-        check(_abi.lib().naer_net_step(self._handle, int(n_steps)))
+        check(_abi.lib().naer_net_step(self._handle, _abi.steps_arg(n_steps)))
 
     def idle(self):
         """True when the arena's enqueued steps are done (naer_net_idle; does not wait)."""

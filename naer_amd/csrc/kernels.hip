@@ -102,6 +102,7 @@ __device__ __forceinline__ float dec_pow(const float* tab, uint32_t n, uint32_t 
 // (a generic pointer makes them flat loads, which also wait on the
 // outstanding global loads).
 typedef const __attribute__((address_space(3))) float LdsFloat;
+typedef __attribute__((address_space(3))) void LdsSink;
 
 struct TraceTabs {
     LdsFloat *tp_l, *tm_l;
@@ -318,12 +319,17 @@ __global__ __launch_bounds__(TW, (BUCKET && !PLASTIC && !EXTRA && TW == 1024) ? 
     // bucketed events: the M buckets (batches A-1 .. A-M) of this tile,
     // flattened: event i of the tile lies in bucket m with
     // s_pre[m] <= i < s_pre[m + 1], at ev + s_evbase[m] + i
-    __shared__ uint32_t s_pre[kMaxArrivalBatches + 1];
-    __shared__ long long s_evbase[kMaxArrivalBatches];
-    __shared__ uint32_t s_live[kMaxArrivalBatches];
+    // (reserved placement: `parts` ranges per batch, those of the tile in
+    // the parts of its group bucket)
+    __shared__ uint32_t s_pre[kResMaxRanges + 1];
+    __shared__ long long s_evbase[kResMaxRanges];
+    __shared__ uint32_t s_live[kResMaxRanges];
     __shared__ uint32_t s_ring_live;
     const unsigned long long A = t0 / a.Bfull;
     const uint32_t o0 = (uint32_t)(t0 - A * a.Bfull);
+    // reserved placement: this batch's flags start clear (its placement sets them)
+    if (BUCKET && a.reserve && blockIdx.x == 0 && threadIdx.x == 0 && (!a.defer || o0 == 0))
+        a.flags[(uint32_t)(A % a.R)] = 0;
     // the thread's neuron state, loaded first so that its latency overlaps
     // the event phase
     const uint32_t j = blockIdx.x * TW + threadIdx.x;
@@ -342,43 +348,56 @@ __global__ __launch_bounds__(TW, (BUCKET && !PLASTIC && !EXTRA && TW == 1024) ? 
     if (BUCKET) {
         for (uint32_t i = threadIdx.x; i < a.B * TW; i += TW)
             acc[i] = (AccT)0;
-        if (threadIdx.x < a.M) {
-            const uint32_t m = threadIdx.x + 1;
+        const uint32_t n_rng = a.M * a.parts;
+        for (uint32_t x = threadIdx.x; x < n_rng; x += TW) {
+            const uint32_t m = x / a.parts + 1, y = x - (m - 1) * a.parts;
             uint32_t lo = 0, hi = 0, live = 0;
             uint32_t buf = 0;
             if (A >= m) {
                 buf = (uint32_t)((A - m) % a.R);
                 // bounds are read together with the flag (in bounds either
                 // way) and dropped when the batch went through the ring
-                const uint32_t* bs = a.bs + (size_t)buf * (a.NBK + 1);
-                const uint32_t k = (m - 1) * a.NT + blockIdx.x;
                 const uint32_t f = a.flags[buf];
-                const uint32_t b_lo = bs[k], b_hi = bs[k + 1];
-                if (f) {
+                uint32_t b_lo = 0, b_hi = 0;
+                if (f & 4u) {
+                    const uint32_t gt = 1u << a.gt_shift;
+                    const uint32_t kc = (m - 1) * (a.NT >> a.gt_shift) + (blockIdx.x >> a.gt_shift);
+                    const uint32_t* tp = a.tb +
+                                         (((size_t)buf * (a.NBK >> a.gt_shift) + kc) * a.parts + y) * (gt + 1) +
+                                         (blockIdx.x & (gt - 1));
+                    b_lo = tp[0];
+                    b_hi = tp[1];
+                } else if (y == 0) {
+                    const uint32_t* bs = a.bs + (size_t)buf * (a.NBK + 1);
+                    const uint32_t k = (m - 1) * a.NT + blockIdx.x;
+                    b_lo = bs[k];
+                    b_hi = bs[k + 1];
+                }
+                if (f & 1u) {
                     lo = b_lo;
                     hi = b_hi;
-                } else {
-                    live = 1;
                 }
+                // bit 1: some of the batch's events went through the ring
+                live = (f & 3u) != 1u ? 1u : 0u;
             }
-            s_pre[m] = hi - lo;
-            s_evbase[m - 1] = (long long)buf * (long long)a.e_cap + lo;
-            s_live[m - 1] = live;
+            s_pre[x + 1] = hi - lo;
+            s_evbase[x] = (long long)buf * (long long)a.e_cap + lo;
+            s_live[x] = live;
         }
         __syncthreads();
         if (threadIdx.x == 0) {
             uint32_t live = 0;
             s_pre[0] = 0;
-            for (uint32_t m = 0; m < a.M; ++m) {
-                const uint32_t n = s_pre[m + 1];
-                s_evbase[m] -= s_pre[m];
-                s_pre[m + 1] = s_pre[m] + n;
-                live |= s_live[m];
+            for (uint32_t x = 0; x < n_rng; ++x) {
+                const uint32_t n = s_pre[x + 1];
+                s_evbase[x] -= s_pre[x];
+                s_pre[x + 1] = s_pre[x] + n;
+                live |= s_live[x];
             }
             s_ring_live = live;
         }
         __syncthreads();
-        const uint32_t total = s_pre[a.M];
+        const uint32_t total = s_pre[n_rng];
         constexpr uint32_t U = 8;
         uint32_t m = 0;  // bucket of the thread's current event (its indexes only grow)
         if constexpr (REC4) {
@@ -567,6 +586,8 @@ __global__ __launch_bounds__(TW, (BUCKET && !PLASTIC && !EXTRA && TW == 1024) ? 
                     a.probe[(size_t)(t - a.probe_t0) * a.N + j] = I;
                 return I;
             };
+            unsigned long long row_beg = 0;  // the row of the neuron's latest spike
+            uint32_t row_len = 0;
             if (model) {
                 // the drive is drawn every step (counter-based: the same values
                 // a LIF neuron draws when it is not refractory)
@@ -597,7 +618,9 @@ __global__ __launch_bounds__(TW, (BUCKET && !PLASTIC && !EXTRA && TW == 1024) ? 
                         if (fired) {
                             ref = n_ref;
                             last = (int32_t)t;
-                            my_events += (uint32_t)(a.row_ptr[j + 1] - a.row_ptr[j]);
+                            row_beg = a.row_ptr[j];
+                            row_len = (uint32_t)(a.row_ptr[j + 1] - row_beg);
+                            my_events += row_len;
                             if (PLASTIC) {
                                 // traces at t (spikes before t), from the previous spike
                                 const uint32_t ns = lt.x;
@@ -620,6 +643,24 @@ __global__ __launch_bounds__(TW, (BUCKET && !PLASTIC && !EXTRA && TW == 1024) ? 
                         }
                     }
                     emit(sg, fired, j, (uint32_t)t, (uint32_t)(t0l + s) * a.dt_us);
+                    if (BUCKET && a.prefetch) {
+                        // the delivery's first pass reads the rows of the
+                        // batch's spikes: the wave loads them now, into a
+                        // dead LDS array (the event phase's bases), so that
+                        // they are in the caches by then; nothing waits for
+                        // these loads before the block's final barrier
+                        unsigned long long fm = __ballot(fired);
+                        while (fm) {
+                            const int l = __builtin_ctzll(fm);
+                            fm &= fm - 1ull;
+                            const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)row_beg, l);
+                            const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(row_beg >> 32), l);
+                            const uint32_t len = (uint32_t)__builtin_amdgcn_readlane((int)row_len, l);
+                            const uint32_t* row = a.col + (((unsigned long long)hi << 32) | lo);
+                            for (uint32_t q = lane_id(); q < len; q += kWave)
+                                __builtin_amdgcn_global_load_lds((const void*)(row + q), (LdsSink*)s_evbase, 4, 0, 0);
+                        }
+                    }
                 }
                 a.V[j] = V;
                 a.ref[j] = ref;
@@ -1215,15 +1256,89 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_place(const BucketArg
     const uint32_t cur = st->cur;
     const unsigned long long t0 = st->t0_cur - st->t0_cur % a.Bfull;  // the aligned batch (deferred launches)
     const uint32_t buf = (uint32_t)((t0 / a.Bfull) % a.R);
-    const bool ok = a.flags[buf] != 0;
     const uint32_t nbc = a.NBK >> a.gt_shift;
     const uint32_t ng = a.NT >> a.gt_shift;
     const uint32_t g_shift = a.tw_shift + a.gt_shift;
-    if (ok) {
-        const uint32_t* bs = a.bs + (size_t)buf * (a.NBK + 1);
-        const uint32_t* h = a.direct ? a.hist + (size_t)blockIdx.x * a.NBK : a.hist_c + (size_t)blockIdx.x * nbc;
-        for (uint32_t i = threadIdx.x; i < a.n_cnt; i += blockDim.x)
-            s_cur[i] = bs[i << a.gt_shift] + h[i];
+    // reserved placement: s_cur[kc] runs from the region's start to its end
+    // s_end[kc] (s_beg[kc]: the start)
+    uint32_t* s_end = s_cur + a.n_cnt;
+    uint32_t* s_beg = s_end + a.n_cnt;
+    __shared__ unsigned long long s_E;
+    const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);  // the block's segments (as for_block_chunks)
+    bool ok;
+    if (a.reserve) {
+        // P_b, the block's events and the batch's, from the segments' counts
+        __shared__ unsigned long long s_red[3][kBucketThreads / kWave];
+        const unsigned long long* sev = a.seg_ev + (size_t)cur * a.n_seg;
+        const uint32_t s_lo = a.blk_seg[lb], s_hi = a.blk_seg[lb + 1];
+        // the shares of this thread's counters, loaded with the counts
+        unsigned long long cv[kResCntPerThread], ccv[kResCntPerThread];
+#pragma unroll
+        for (uint32_t k = 0; k < kResCntPerThread; ++k) {
+            const uint32_t i = threadIdx.x + k * kBucketThreads;
+            cv[k] = i < nbc ? a.rc[i] : 0ull;
+            ccv[k] = i < nbc ? a.rcc[i] : 0ull;
+        }
+        const unsigned long long cc_all = a.rcc[nbc];
+        unsigned long long pre = 0, mine = 0, tot = 0;
+        for (uint32_t k = threadIdx.x; k < a.n_act; k += blockDim.x) {
+            const unsigned long long e = sev[k];
+            tot += e;
+            pre += k < s_lo ? e : 0ull;
+            mine += (k >= s_lo && k < s_hi) ? e : 0ull;
+        }
+        for (uint32_t o = kWave / 2; o > 0; o >>= 1) {
+            tot += __shfl_xor(tot, (int)o);
+            pre += __shfl_xor(pre, (int)o);
+            mine += __shfl_xor(mine, (int)o);
+        }
+        if (lane_id() == 0) {
+            s_red[0][threadIdx.x / kWave] = tot;
+            s_red[1][threadIdx.x / kWave] = pre;
+            s_red[2][threadIdx.x / kWave] = mine;
+        }
+        __syncthreads();
+        tot = pre = mine = 0;
+        for (uint32_t w = 0; w < blockDim.x / kWave; ++w) {
+            tot += s_red[0][w];
+            pre += s_red[1][w];
+            mine += s_red[2][w];
+        }
+        ok = tot <= a.e_cap && res_base(cc_all, tot, nbc, a.NB) <= a.e_cap;
+        if (threadIdx.x == 0) {
+            s_E = tot;
+            a.pb[lb] = pre;
+            if (lb == 0) {
+                a.pb[a.NB] = tot;
+                // bit 0: bucketed, bit 2: tile ranges in tb (the update of
+                // the batch cleared the flags)
+                if (ok)
+                    atomicOr(&a.flags[buf], 5u);
+            }
+        }
+        if (ok) {
+#pragma unroll
+            for (uint32_t k = 0; k < kResCntPerThread; ++k) {
+                const uint32_t i = threadIdx.x + k * kBucketThreads;
+                if (i < nbc) {
+                    const unsigned long long c = cv[k];
+                    const unsigned long long base = res_base(ccv[k], tot, i, a.NB);
+                    const uint32_t b0 = (uint32_t)(base + ((c * pre) >> kResShift) + (unsigned long long)kResPad * lb);
+                    s_beg[i] = b0;
+                    s_cur[i] = b0;
+                    s_end[i] = (uint32_t)(base + ((c * (pre + mine)) >> kResShift) +
+                                          (unsigned long long)kResPad * (lb + 1));
+                }
+            }
+        }
+    } else {
+        ok = a.flags[buf] != 0;
+        if (ok) {
+            const uint32_t* bs = a.bs + (size_t)buf * (a.NBK + 1);
+            const uint32_t* h = a.direct ? a.hist + (size_t)blockIdx.x * a.NBK : a.hist_c + (size_t)blockIdx.x * nbc;
+            for (uint32_t i = threadIdx.x; i < a.n_cnt; i += blockDim.x)
+                s_cur[i] = bs[i << a.gt_shift] + h[i];
+        }
     }
     const Arrival arr(a.Bfull);
     TraceTabs tt{ nullptr, nullptr, 0u };
@@ -1301,13 +1416,37 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_place(const BucketArg
                     const uint32_t v = r + d;
                     const uint32_t g = tgt >> g_shift;
                     const uint32_t m = arr.batch(v);
-                    const uint32_t pos = run_add<true, AGG, kPlaceRunHeads>(s_cur, (m - 1u) * ng + g);
+                    const uint32_t key = (m - 1u) * ng + g;
+                    uint32_t pos = run_add<true, AGG, kPlaceRunHeads>(s_cur, key);
                     const uint32_t rec = (tgt - (g << g_shift)) | (arr.slot(v, m) << (REC4 ? kRec4SlotShift : 16u)) |
                                          (REC4 ? cls << kRec4ClassShift : 0u);
-                    if constexpr (REC4)
-                        stage4[pos] = rec;
-                    else
-                        stage[pos] = make_uint2(rec, __float_as_uint(w[u]));
+                    bool store = true;
+                    if (a.reserve && pos >= s_end[key]) {
+                        // the region is full: the bucket's spill region, else the ring
+                        const unsigned long long cr = a.rc[key], E = s_E;
+                        const unsigned long long sp_base = res_base(a.rcc[key], E, key, a.NB) + ((cr * E) >> kResShift) +
+                                                           (unsigned long long)kResPad * a.NB;
+                        const uint32_t sp_cap = (uint32_t)((cr * E) >> (kResShift + 2)) + kResSpillPad;
+                        const uint32_t sp = atomicAdd(&a.spill[key], 1u);
+                        pos = (uint32_t)sp_base + sp;
+                        if (sp >= sp_cap) {
+                            store = false;
+                            uint32_t sl = (uint32_t)((t0 + r) % a.D) + ((c[u] >> 24) & 0x7Fu);
+                            sl = sl >= a.D ? sl - a.D : sl;
+                            RingT* dst = ring + (size_t)sl * a.N + tgt;
+                            if (DET)
+                                atomicAdd((unsigned long long*)dst, (unsigned long long)to_fixed(w[u]));
+                            else
+                                unsafeAtomicAdd((float*)dst, w[u]);
+                            atomicOr(&a.flags[buf], 2u);
+                        }
+                    }
+                    if (store) {
+                        if constexpr (REC4)
+                            stage4[pos] = rec;
+                        else
+                            stage[pos] = make_uint2(rec, __float_as_uint(w[u]));
+                    }
                 }
             }
         } else {
@@ -1330,6 +1469,12 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_place(const BucketArg
             }
         }
     });
+    if (a.reserve && ok) {
+        // the events each region holds (the refine reads them)
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < nbc; i += blockDim.x)
+            a.rcnt[(size_t)i * a.NB + lb] = min(s_cur[i], s_end[i]) - s_beg[i];
+    }
 }
 
 // The staging buffer holds the batch's events in group-bucket order (group
@@ -1441,6 +1586,193 @@ __global__ __launch_bounds__(kRefineThreads) void k_bucket_refine(const BucketAr
             }
             __syncthreads();
         }
+    }
+}
+
+// Reserved placement's refine: block (kc, y) takes part y of group bucket
+// kc, the regions of placement blocks [y NB / RS, (y + 1) NB / RS) (the
+// last part also the bucket's spill region), and writes their events sorted
+// by tile into the part's span of ev[E % R], which starts where its first
+// region starts: counted per tile in LDS, scanned, placed by LDS cursors.  A
+// part of at most kResCache events is read once (kept in LDS), a larger one
+// twice.  The tile ranges go to tb[E % R][kc][y].  Waves take the regions in
+// turn, kResWalk regions of up to 128 events at a time.
+constexpr uint32_t kResThreads = 512;
+constexpr uint32_t kResCacheBytes = 24576;
+constexpr uint32_t kResWalk = 8;  // regions a wave loads at once
+
+template <bool REC4>
+__global__ __launch_bounds__(kResThreads) void k_bucket_refine_res(const BucketArgs a)
+{
+    using Rec = typename std::conditional<REC4, uint32_t, uint2>::type;
+    constexpr uint32_t kCache = kResCacheBytes / sizeof(Rec);
+    __shared__ Rec s_ev[kCache];
+    __shared__ uint32_t s_q[kResMaxPartRegions + 1];  // exclusive prefix of the regions' counts
+    __shared__ uint32_t s_src[kResMaxPartRegions];    // their starts in the staging buffer
+    __shared__ uint32_t s_cnt[kMaxGroupTiles];
+    __shared__ uint32_t s_pos[kMaxGroupTiles + 1];
+    __shared__ uint32_t s_wsum[kResThreads / kWave];
+    const uint32_t ngb = a.NBK >> a.gt_shift;
+    const uint32_t gt = 1u << a.gt_shift;
+    const uint32_t kc = blockIdx.x / a.RS, y = blockIdx.x - kc * a.RS;
+    const uint32_t b0 = (uint32_t)((unsigned long long)a.NB * y / a.RS);
+    const uint32_t b1 = (uint32_t)((unsigned long long)a.NB * (y + 1) / a.RS);
+    const bool last = y + 1 == a.RS;
+    const uint32_t nr = b1 - b0 + (last ? 1u : 0u);
+    // everything below depends only on these loads (issued together); the
+    // placement used the buckets iff the batch fit, which E decides
+    const unsigned long long t0 = a.st->t0_cur;
+    const unsigned long long E = a.pb[a.NB];
+    const unsigned long long c = a.rc[kc];
+    const unsigned long long Cc = a.rcc[kc], cc_all = a.rcc[ngb];
+    if (!(E <= a.e_cap && res_base(cc_all, E, ngb, a.NB) <= a.e_cap))
+        return;  // the batch went through the ring
+    const uint32_t buf = (uint32_t)((t0 / a.Bfull) % a.R);
+    const unsigned long long base = res_base(Cc, E, kc, a.NB);
+    const uint32_t out = (uint32_t)(base + ((c * a.pb[b0]) >> kResShift) + (unsigned long long)kResPad * b0);
+    const uint32_t lane = lane_id(), wave = threadIdx.x / kWave;
+    // region counts and starts; exclusive scan of the counts (two per thread)
+    uint32_t n2[2];
+#pragma unroll
+    for (uint32_t h = 0; h < 2; ++h) {
+        const uint32_t i = 2 * threadIdx.x + h;
+        n2[h] = 0;
+        if (i < nr) {
+            const uint32_t b = b0 + i;
+            if (b < b1) {
+                s_src[i] = (uint32_t)(base + ((c * a.pb[b]) >> kResShift) + (unsigned long long)kResPad * b);
+                n2[h] = a.rcnt[(size_t)kc * a.NB + b];
+            } else {
+                s_src[i] = (uint32_t)(base + ((c * E) >> kResShift) + (unsigned long long)kResPad * a.NB);
+                n2[h] = min(a.spill[kc], (uint32_t)((c * E) >> (kResShift + 2)) + kResSpillPad);
+            }
+        }
+    }
+    uint32_t incl = n2[0] + n2[1];
+    for (uint32_t o = 1; o < kWave; o <<= 1) {
+        const uint32_t v = (uint32_t)__shfl_up((int)incl, o);
+        incl += lane >= o ? v : 0u;
+    }
+    if (lane == kWave - 1)
+        s_wsum[wave] = incl;
+    if (threadIdx.x < gt)
+        s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t wpre = 0;
+    for (uint32_t w = 0; w < wave; ++w)
+        wpre += s_wsum[w];
+    {
+        const uint32_t ex = wpre + incl - n2[0] - n2[1];
+        const uint32_t i = 2 * threadIdx.x;
+        if (i < nr)
+            s_q[i] = ex;
+        if (i + 1 < nr)
+            s_q[i + 1] = ex + n2[0];
+        if (i + 1 == nr || i + 2 == nr)
+            s_q[nr] = ex + n2[0] + n2[1];
+    }
+    if (nr == 0 && threadIdx.x == 0)
+        s_q[0] = 0;
+    __syncthreads();
+    const uint32_t n = s_q[nr];
+    const bool cached = n <= kCache;
+    if (last && threadIdx.x == 0)
+        a.spill[kc] = 0;  // read above (before the barrier); the next batch's placement starts over
+    const Rec* stage = (const Rec*)a.stage;
+    Rec* ev = (Rec*)a.ev + (size_t)buf * a.e_cap;
+    const uint32_t tmask = (1u << a.tw_shift) - 1u;
+    auto offset = [](const Rec& e) -> uint32_t {
+        if constexpr (REC4)
+            return e & kRec4OffsetMask;
+        else
+            return e.x & 0xFFFFu;
+    };
+    auto retile = [&](const Rec& e) -> Rec {
+        if constexpr (REC4)
+            return (e & tmask) | (e & ~kRec4OffsetMask);
+        else
+            return make_uint2((e.x & tmask) | (e.x & 0xFFFF0000u), e.y);
+    };
+    const uint32_t nwv = blockDim.x / kWave;
+    // calls f(event, index in the part) for every event of the part's regions
+    auto walk = [&](auto f) {
+        for (uint32_t r0 = wave * kResWalk; r0 < nr; r0 += nwv * kResWalk) {
+            Rec e[2 * kResWalk];
+            uint32_t at[2 * kResWalk];
+#pragma unroll
+            for (uint32_t k = 0; k < 2 * kResWalk; ++k) {
+                const uint32_t r = r0 + (k >> 1), j = lane + (k & 1) * kWave;
+                at[k] = 0xFFFFFFFFu;
+                if (r < nr && j < s_q[r + 1] - s_q[r]) {
+                    e[k] = stage[s_src[r] + j];
+                    at[k] = s_q[r] + j;
+                }
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < 2 * kResWalk; ++k)
+                if (at[k] != 0xFFFFFFFFu)
+                    f(e[k], at[k]);
+            // regions beyond 128 events: 8 loads in flight at a time
+            for (uint32_t k = 0; k < kResWalk; ++k) {
+                const uint32_t r = r0 + k;
+                if (r >= nr)
+                    break;
+                const uint32_t len = s_q[r + 1] - s_q[r], src = s_src[r], dst = s_q[r];
+                for (uint32_t j0 = 2 * kWave; j0 < len; j0 += 8 * kWave) {
+                    Rec e2[8];
+#pragma unroll
+                    for (uint32_t u = 0; u < 8; ++u) {
+                        const uint32_t j = j0 + u * kWave + lane;
+                        if (j < len)
+                            e2[u] = stage[src + j];
+                    }
+#pragma unroll
+                    for (uint32_t u = 0; u < 8; ++u) {
+                        const uint32_t j = j0 + u * kWave + lane;
+                        if (j < len)
+                            f(e2[u], dst + j);
+                    }
+                }
+            }
+        }
+    };
+    walk([&](const Rec& e, uint32_t i) {
+        if (cached)
+            s_ev[i] = e;
+        atomicAdd(&s_cnt[offset(e) >> a.tw_shift], 1u);
+    });
+    __syncthreads();
+    if (wave == 0) {
+        // exclusive scan of the tile counts (gt <= 64: one wave)
+        const uint32_t x = lane < gt ? s_cnt[lane] : 0u;
+        uint32_t inc = x;
+        for (uint32_t o = 1; o < kWave; o <<= 1) {
+            const uint32_t v = (uint32_t)__shfl_up((int)inc, o);
+            inc += lane >= o ? v : 0u;
+        }
+        uint32_t* tb = a.tb + (((size_t)buf * ngb + kc) * a.parts + y) * (gt + 1);
+        if (lane < gt) {
+            s_pos[lane] = inc - x;
+            tb[lane] = out + inc - x;
+        }
+        if (lane == 0)
+            tb[gt] = out + n;
+        // the part slots this schedule does not use hold empty ranges
+        if (y == 0) {
+            for (uint32_t i = a.RS * (gt + 1) + lane; i < a.parts * (gt + 1); i += kWave)
+                tb[i] = 0;
+        }
+    }
+    __syncthreads();
+    auto put = [&](const Rec& e) {
+        const uint32_t p = atomicAdd(&s_pos[offset(e) >> a.tw_shift], 1u);
+        ev[out + p] = retile(e);
+    };
+    if (cached) {
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
+            put(s_ev[i]);
+    } else {
+        walk([&](const Rec& e, uint32_t) { put(e); });
     }
 }
 
@@ -1603,6 +1935,30 @@ __global__ __launch_bounds__(kBlock) void k_count_plastic(const uint32_t* col, u
         k += __shfl_down(k, off);
     if (lane_id() == 0 && k)
         atomicAdd(count, (unsigned long long)k);
+}
+
+// Synapses of [0, S) per (delay d - dmin, target group): out[(d - dmin) * ng + g]
+// (nd * ng bins, at most kSynStatsBins, counted in LDS and added once per block).
+__global__ __launch_bounds__(kBlock) void k_syn_stats(const uint32_t* col, unsigned long long S, uint32_t dmin,
+                                                      uint32_t nd, uint32_t ng, unsigned long long* out)
+{
+    extern __shared__ uint32_t s_bin[];
+    const uint32_t nb = nd * ng;
+    for (uint32_t i = threadIdx.x; i < nb; i += kBlock)
+        s_bin[i] = 0;
+    __syncthreads();
+    for (unsigned long long i = (unsigned long long)blockIdx.x * kBlock + threadIdx.x; i < S;
+         i += (unsigned long long)gridDim.x * kBlock) {
+        const uint32_t c = col[i];
+        const uint32_t d = ((c >> 24) & 0x7Fu) - dmin;
+        const uint32_t g = (c & kTargetMask) >> kGroupNeuronShift;
+        if (d < nd && g < ng)
+            atomicAdd(&s_bin[d * ng + g], 1u);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nb; i += kBlock)
+        if (s_bin[i])
+            atomicAdd(&out[i], (unsigned long long)s_bin[i]);
 }
 
 // Initial state of neurons [j_begin, j_end): potentials from the V0 stream,
@@ -1817,6 +2173,8 @@ static void launch_bucket_pass(const BucketArgs& a, int step, bool det, size_t l
     case 0:
         if (a.balance)
             hipLaunchKernelGGL(k_bucket_split, dim3(1), dim3(1024), 0, s, a);
+        if (a.reserve)
+            break;  // reserved placement counts nothing
         if (a.U == 4)
             hipLaunchKernelGGL((k_bucket_hist<4, AGG>), grid, block, lds, s, a);
         else
@@ -1841,6 +2199,14 @@ static void launch_bucket_pass(const BucketArgs& a, int step, bool det, size_t l
     default:
         if (a.direct)
             break;  // the placement wrote the tile buckets
+        if (a.reserve) {
+            const dim3 g((a.NBK >> a.gt_shift) * a.RS);
+            if (a.ncls)
+                hipLaunchKernelGGL(k_bucket_refine_res<true>, g, dim3(kResThreads), 0, s, a);
+            else
+                hipLaunchKernelGGL(k_bucket_refine_res<false>, g, dim3(kResThreads), 0, s, a);
+            break;
+        }
         if (a.ncls)
             hipLaunchKernelGGL((k_bucket_refine<AGG, true>), dim3((a.NBK >> a.gt_shift) * a.RS), dim3(kRefineThreads),
                                0, s, a);
@@ -1857,9 +2223,12 @@ hipError_t launch_bucket_step(const BucketArgs& a, int step, bool det, hipStream
     if (e != hipSuccess)
         return e;
     const size_t lds = (size_t)a.NBK * sizeof(uint32_t);
-    const size_t lds_c = (size_t)a.n_cnt * sizeof(uint32_t);
+    // reserved placement: each counter with its region's end and start
+    const size_t lds_c = (size_t)a.n_cnt * sizeof(uint32_t) * (a.reserve ? 3 : 1);
     switch (step) {
     case 1:
+        if (a.reserve)
+            break;
         hipLaunchKernelGGL(k_bucket_colscan,
                            dim3((a.direct ? 0u : cdiv(a.NBK >> a.gt_shift, kScanCols)) + cdiv(a.NBK, kScanCols)),
                            dim3(kScanCols * kScanGroups), 0, s, a);
@@ -1983,6 +2352,16 @@ hipError_t launch_count_plastic(const uint32_t* col, unsigned long long begin, u
 {
     const uint32_t grid = std::max<uint32_t>(1, std::min<uint64_t>(4096, cdiv(end - begin, kBlock)));
     hipLaunchKernelGGL(k_count_plastic, dim3(grid), dim3(kBlock), 0, s, col, begin, end, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_syn_stats(const uint32_t* col, unsigned long long S, uint32_t dmin, uint32_t nd, uint32_t ng,
+                            unsigned long long* out, hipStream_t s)
+{
+    if (nd * ng > kSynStatsBins || S == 0)
+        return hipErrorInvalidValue;
+    const uint32_t grid = std::max<uint32_t>(1, std::min<uint64_t>(2048, cdiv(S, kBlock * 64)));
+    hipLaunchKernelGGL(k_syn_stats, dim3(grid), dim3(kBlock), nd * ng * sizeof(uint32_t), s, col, S, dmin, nd, ng, out);
     return hipGetLastError();
 }
 

@@ -197,9 +197,20 @@ struct UpdateArgs {
     // (arrival batch offset m, target tile)
     const uint2* ev;          // [R][e_cap]
     const uint32_t* bs;       // [R][NBK + 1] bucket starts
-    const uint32_t* flags;    // [R] 1: that batch's events are in the buckets
+    uint32_t* flags;          // [R] bit 0: that batch's events are in the buckets; bit 1 (reserved
+                              // placement): some of them went through the ring
     unsigned long long e_cap;
     uint32_t R, M, NT, NBK;
+    // reserved placement (see BucketArgs): the tile ranges of each part of a
+    // group bucket, for batches whose flags have bit 2 set; the update clears
+    // the flags of the batch it starts when `reserve`
+    const uint32_t* tb;       // [R][groups][parts][tiles per group + 1]
+    uint32_t reserve, parts, gt_shift;
+    // prefetch (reserved placement): the rows of the batch's spikes are
+    // loaded into the caches as the spikes happen (the placement reads them
+    // next)
+    const uint32_t* col;
+    uint32_t prefetch;
     // neurons of source-defined models (NAER_POP_MODEL): their synaptic input
     // and drive per step, [B][N], read by the model's own kernel
     float2* cin;
@@ -347,7 +358,45 @@ struct BucketArgs {
     // class-weight delivery (4-byte records): class and weight of each neuron's population
     const uint32_t* ncls;   // [N]
     const float* cls_w;     // [classes]
+    // reserved placement (reserve = 1; single nets): no histogram passes.
+    // Placement block b writes the events of group bucket kc into its region
+    // of the staging buffer, whose start is
+    //   off(b, kc) = res_base(Cc[kc], E) + (c[kc] * P_b >> 24) + kResPad * b
+    // (P_b: events of the blocks before b, E: of the batch; c[kc]: the share of
+    // kc reserved per event, 2^-24 units, from the network's synapses; Cc: its
+    // prefix), so a region holds its expected events with a margin; events
+    // beyond it go to the bucket's spill region after off(NB, kc), and beyond
+    // that through the ring.  Refine: RS parts per group bucket, each a
+    // counting sort by tile of its blocks' regions into the part's span of
+    // ev[E % R], the tile ranges in tb.
+    uint32_t reserve;
+    const uint32_t* rc;               // [groups] c
+    const unsigned long long* rcc;    // [groups + 1] Cc
+    uint32_t* rcnt;                   // [groups][NB] events each block's region holds
+    uint32_t* spill;                  // [groups] spill cursors (zeroed by the refine that reads them)
+    unsigned long long* pb;           // [NB + 1] P_b, and E at [NB]
+    uint32_t* tb;                     // [R][groups][parts][tiles per group + 1]
+    uint32_t parts;                   // part slots per group bucket in tb (RS <= parts; the refine
+                                      // empties slots [RS, parts))
 };
+
+constexpr uint32_t kResPad = 16;        // records every region holds beyond its share
+constexpr uint32_t kResSpillPad = 64;   // and every spill region beyond a quarter of its bucket's share
+constexpr uint32_t kResShift = 24;      // c in 2^-24 units
+constexpr uint32_t kResMaxShare = 16u << kResShift;  // sum of c: at most 16 records reserved per event
+constexpr uint32_t kResMaxParts = 8;    // RS bound
+constexpr uint32_t kResMaxRanges = 128; // M * RS bound: the update's ranges per tile
+constexpr uint32_t kResMaxPartRegions = 1024;  // regions of a part (ceil(NB / RS) + 1)
+constexpr uint32_t kResCntPerThread = 3;       // placement counters per thread: 3 words each in 32 KB of LDS
+
+// start of group bucket kc's regions for a batch of E events (Cc: its prefix);
+// with kc = groups, the extent of the staging layout
+__host__ __device__ __forceinline__ unsigned long long res_base(unsigned long long Cc, unsigned long long E, uint32_t kc,
+                                                               uint32_t NB)
+{
+    const unsigned long long x = Cc * E;
+    return (x >> kResShift) + (x >> (kResShift + 2)) + (unsigned long long)kc * (kResPad * NB + kResSpillPad);
+}
 
 constexpr uint32_t kMaxSegPerBlock = 64;  // segments a histogram/placement block stages at once
 constexpr uint32_t kMaxSegSchedule = 4096;  // bound of the schedule's mean segments per block

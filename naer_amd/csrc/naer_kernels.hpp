@@ -13,7 +13,7 @@ namespace naer {
 hipError_t launch_update(const UpdateArgs& a, bool det, bool plastic, bool bucket, bool extra, bool rec4,
                          hipStream_t s);
 // the bucketed delivery of one aligned batch: steps 0 split + histogram,
-// 1 scans, 2 place, 3 refine
+// 1 scans, 2 place, 3 refine (reserved placement: 2 and 3 only)
 constexpr int kBucketSteps = 4;
 hipError_t launch_bucket_step(const BucketArgs& a, int step, bool det, hipStream_t s);
 hipError_t launch_bucket_scrub(const UpdateArgs& a, uint32_t j_begin, uint32_t j_end, hipStream_t s);
@@ -29,6 +29,11 @@ hipError_t launch_sort_rows(const unsigned long long* row_ptr, uint32_t* col, fl
                             uint32_t row_end, uint32_t max_row, hipStream_t s);
 hipError_t launch_count_plastic(const uint32_t* col, unsigned long long begin, unsigned long long end,
                                 unsigned long long* count, hipStream_t s);
+// Synapses of [0, S) per (delay - dmin, target group of 2^14 neurons): out
+// has nd * ng <= kSynStatsBins zeroed entries, to which the counts are added.
+constexpr uint32_t kSynStatsBins = 8192;
+hipError_t launch_syn_stats(const uint32_t* col, unsigned long long S, uint32_t dmin, uint32_t nd, uint32_t ng,
+                            unsigned long long* out, hipStream_t s);
 hipError_t launch_init_state(const UpdateArgs& a, bool det, uint32_t j_begin, uint32_t j_end, hipStream_t s);
 hipError_t launch_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out,
                          unsigned long long n, hipStream_t s);

@@ -278,6 +278,16 @@ struct naer_net {
     uint32_t cls_cap = 0;
     uint32_t *bs = nullptr, *flags = nullptr, *hist = nullptr, *hist_c = nullptr, *totals = nullptr;
     uint32_t* gcur = nullptr;
+    // reserved placement (single nets; see BucketArgs)
+    bool res_ok = false;    // shares known: the schedule may ask for it
+    uint32_t RES = 0;       // schedule: 1 reserved placement
+    uint32_t parts = 1;     // part slots per group bucket in res_tb
+    uint32_t* res_c = nullptr;
+    unsigned long long* res_cc = nullptr;
+    uint32_t* res_cnt = nullptr;
+    uint32_t* res_spill = nullptr;
+    unsigned long long* res_pb = nullptr;
+    uint32_t* res_tb = nullptr;
 
     bool profiling = false;
     uint32_t graph_chunk = 0;  // steps per captured graph (0 = plain launches)
@@ -768,6 +778,11 @@ UpdateArgs make_update_args(naer_net* n)
     a.cin = n->cin;
     a.cls_fx = n->cls_fx;
     a.n_cls = n->rec_bytes == 4 ? (uint32_t)n->pp.size() : 0u;
+    a.tb = n->res_tb;
+    a.parts = n->parts;
+    a.gt_shift = n->gt_shift;
+    a.col = n->col;
+    a.prefetch = 0;
     return a;
 }
 
@@ -1092,6 +1107,104 @@ uint32_t default_refine_split(const naer_net* n)
     return std::max<uint32_t>(1, std::min<uint32_t>(64, 2048 / std::max<uint32_t>(1, n->NBK >> n->gt_shift)));
 }
 
+// Reserved placement (see BucketArgs): no counting passes; each placement
+// block writes its events into regions of the staging buffer reserved from
+// the batch's event counts.  The share of events reserved for group bucket
+// (m, g) is 1.5x the largest fraction of the network's synapses that can
+// arrive there (delay d from a spike at step r of the batch arrives in batch
+// m = (r + d) / B; the largest over r), measured once over the built CSR.
+constexpr double kResMargin = 1.5;
+
+// its parts per group bucket: refine blocks of about 4k events or fewer
+uint32_t default_res_parts(const naer_net* n)
+{
+    return std::max<uint32_t>(1, std::min<uint32_t>(n->parts, 4));
+}
+
+// Reserved placement is the default where a batch has at least this many
+// group buckets: its refine has one block per part of a group bucket, so a
+// net with few groups (up to ~10^5 neurons) keeps the counted two-level
+// placement, whose refine splits the whole batch evenly (measured: 1M
+// neurons, 558 group buckets: +6 % events/s; 10^5 neurons with 70 group
+// buckets: -8 %)
+constexpr uint32_t kResDefaultGroups = 512;
+
+bool res_fits(const naer_net* n, uint32_t G, uint32_t RS)
+{
+    const uint32_t NB = (n->n_seg + G - 1) / G;
+    return n->res_ok && RS >= 1 && RS <= n->parts && (NB + RS - 1) / RS + 1 <= kResMaxPartRegions;
+}
+
+int setup_reserve(naer_net* n)
+{
+    n->parts = 1;
+    n->res_ok = false;
+    if (n->arena || n->nets.size() != 1 || n->S_cap == 0 || n->dmax < n->dmin)
+        return NAER_OK;
+    const uint32_t ng = n->NT >> n->gt_shift, ngb = n->NBK >> n->gt_shift, gt = 1u << n->gt_shift;
+    const uint32_t nd = n->dmax - n->dmin + 1;
+    // the update's ranges per tile, and the placement's LDS (three words per counter)
+    const uint32_t parts = std::min(kResMaxParts, kResMaxRanges / n->M);
+    if ((uint64_t)nd * ng > kSynStatsBins || 3ull * ngb * sizeof(uint32_t) > 32768 || parts == 0)
+        return NAER_OK;
+    int rc;
+    unsigned long long* d_h = nullptr;
+    const size_t hb = (size_t)nd * ng * sizeof(unsigned long long);
+    HIP_TRY(hipMalloc(&d_h, hb));
+    std::vector<unsigned long long> h((size_t)nd * ng);
+    hipError_t e = hipMemsetAsync(d_h, 0, hb, n->stream);
+    if (e == hipSuccess)
+        e = launch_syn_stats(n->col, n->S_cap, n->dmin, nd, ng, d_h, n->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(h.data(), d_h, hb, hipMemcpyDeviceToHost, n->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(n->stream);
+    (void)hipFree(d_h);
+    if (e != hipSuccess)
+        return fail(NAER_ERR_HIP, "synapse statistics failed: %s", hipGetErrorString(e));
+    std::vector<uint32_t> c(ngb, 0);
+    std::vector<unsigned long long> cc(ngb + 1, 0);
+    const double S = (double)n->S_cap;
+    // NAER_RESERVE_MARGIN overrides the margin (tests use small ones to send
+    // events into the spill regions and the ring)
+    double margin = kResMargin;
+    if (const char* env = std::getenv("NAER_RESERVE_MARGIN"))
+        margin = std::max(0.0, std::strtod(env, nullptr));
+    for (uint32_t m = 1; m <= n->M; ++m)
+        for (uint32_t g = 0; g < ng; ++g) {
+            unsigned long long most = 0;
+            for (uint32_t r = 0; r < n->B; ++r) {
+                unsigned long long f = 0;
+                for (uint32_t i = 0; i < nd; ++i)
+                    if ((r + n->dmin + i) / n->B == m)
+                        f += h[(size_t)i * ng + g];
+                most = std::max(most, f);
+            }
+            if (most)
+                c[(m - 1) * ng + g] = (uint32_t)std::ceil(margin * (double)most / S * (double)(1u << kResShift));
+        }
+    for (uint32_t k = 0; k < ngb; ++k)
+        cc[k + 1] = cc[k] + c[k];
+    if (cc[ngb] > kResMaxShare)
+        return NAER_OK;
+    if ((rc = n->alloc(&n->res_c, ngb)) || (rc = n->alloc(&n->res_cc, ngb + 1)) ||
+        (rc = n->alloc(&n->res_cnt, (size_t)n->n_seg * ngb)) || (rc = n->alloc(&n->res_spill, ngb)) ||
+        (rc = n->alloc(&n->res_pb, (size_t)n->n_seg + 1)) ||
+        (rc = n->alloc(&n->res_tb, (size_t)n->R * ngb * parts * (gt + 1))))
+        return rc;
+    HIP_TRY(hipMemcpy(n->res_c, c.data(), ngb * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(n->res_cc, cc.data(), (ngb + 1) * sizeof(unsigned long long), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(n->res_spill, 0, ngb * sizeof(uint32_t)));
+    HIP_TRY(hipMemset(n->res_tb, 0, (size_t)n->R * ngb * parts * (gt + 1) * sizeof(uint32_t)));
+    n->parts = parts;
+    n->res_ok = true;
+    if (ngb >= kResDefaultGroups && res_fits(n, n->G, default_res_parts(n))) {
+        n->RES = 1;
+        n->RS = default_res_parts(n);
+    }
+    return NAER_OK;
+}
+
 int alloc_buckets(naer_net* n)
 {
     n->bucketed = false;
@@ -1149,7 +1262,7 @@ int alloc_buckets(naer_net* n)
         return rc;
     HIP_TRY(hipMemsetAsync(n->flags, 0, n->R * sizeof(uint32_t), n->stream));
     n->balance = n->arena || n->nets.size() > 1;
-    if ((rc = set_block_ranges(n)))
+    if ((rc = set_block_ranges(n)) || (rc = setup_reserve(n)))
         return rc;
     n->bucketed = true;
     return NAER_OK;
@@ -2134,6 +2247,22 @@ static int enqueue_steps(naer_net* n, uint32_t n_steps, bool timed)
     ba.is_last = 1;
     ba.ncls = n->rec_bytes == 4 ? n->ncls : nullptr;
     ba.cls_w = n->cls_w;
+    // reserved placement (set_schedule admits it only where it fits)
+    if (!direct && n->RES && res_fits(n, n->G, n->RS)) {
+        ba.reserve = 1;
+        ba.rc = n->res_c;
+        ba.rcc = n->res_cc;
+        ba.rcnt = n->res_cnt;
+        ba.spill = n->res_spill;
+        ba.pb = n->res_pb;
+        ba.tb = n->res_tb;
+        ua.reserve = 1;
+        // the placement reads the rows of the batch's spikes first (no
+        // histogram pass has brought them into the caches): the update loads
+        // them as the spikes happen
+        ua.prefetch = 1;
+    }
+    ba.parts = n->parts;
     FlushArgs fa = flush_args(n, false);
     std::vector<ModelArgs> ma;
     for (const auto& m : n->mpops)
@@ -2162,6 +2291,8 @@ static int enqueue_steps(naer_net* n, uint32_t n_steps, bool timed)
         if (n->bucketed && completes && (ua.defer || off == 0)) {
             // timed per part; the parts add up to NAER_K_SCATTER
             for (int k = 0; k < kBucketSteps; ++k) {
+                if (ba.reserve && k < 2)
+                    continue;  // reserved placement: no counting passes
                 BucketStep bst{ &ba, k };
                 if ((rc = launch_timed(n, NAER_K_BUCKET_HIST + k, do_bucket_step, &bst, timed)))
                     return rc;
@@ -2254,6 +2385,8 @@ int naer_net_get_schedule(naer_net* n, naer_schedule* out)
     out->segments_per_block = n->G;
     out->refine_split = n->RS;
     out->run_aggregate = n->AG;
+    out->reserve = n->RES;
+    out->parts_max = n->res_ok ? n->parts : 0u;
     out->record_bytes = n->bucketed ? n->rec_bytes : 0u;
     out->delivery_us = n->tuned_us;
     return NAER_OK;
@@ -2276,6 +2409,13 @@ int naer_net_set_schedule(naer_net* n, const naer_schedule* s)
                     "(the net has %u)", kMaxPlaceCounters, direct_counters(n, n->n_seg));
     if (s->run_aggregate > 3)
         return fail(NAER_ERR_INVALID, "run_aggregate must be in [0, 3]");
+    if (s->reserve > 1)
+        return fail(NAER_ERR_INVALID, "reserve must be 0 or 1");
+    if (s->reserve && !n->res_ok)
+        return fail(NAER_ERR_INVALID, "reserved placement is not available for this net");
+    if (s->reserve && !res_fits(n, s->segments_per_block, s->refine_split))
+        return fail(NAER_ERR_INVALID, "reserved placement needs refine_split in [1, %u] and at most %u "
+                    "placement blocks per part", n->parts, kResMaxPartRegions - 1);
     int rc = naer_net_sync(n);
     if (rc || (rc = n->drop_graph()))  // captured launches carry the old shapes
         return rc;
@@ -2284,6 +2424,7 @@ int naer_net_set_schedule(naer_net* n, const naer_schedule* s)
     n->NB = (n->n_seg + n->G - 1) / n->G;
     n->RS = s->refine_split;
     n->AG = s->run_aggregate;
+    n->RES = s->reserve;
     return n->bucketed ? set_block_ranges(n) : NAER_OK;
 }
 
@@ -2355,18 +2496,31 @@ int naer_net_autotune(naer_net* n, uint32_t batches, naer_schedule* chosen)
     naer_schedule best;
     naer_net_get_schedule(n, &best);
     double best_us = 0.0;
-    // stage 1: the placement; the other one replaces the net's default only
-    // when it is clearly faster (trials of a few batches, possibly beside
-    // other packs tuning on the same device, are noisy)
-    if (direct_ok(n)) {
-        naer_schedule two = best, one = best;
-        if (two.refine_split == 0)
+    // stage 1: the placement (two-level counted, direct, reserved, where the
+    // net allows them); another one replaces the net's default only when it
+    // is clearly faster (trials of a few batches, possibly beside other packs
+    // tuning on the same device, are noisy)
+    {
+        std::vector<naer_schedule> cand{ best };  // the default first
+        naer_schedule two = best, one = best, res = best;
+        two.reserve = 0;
+        if (two.refine_split == 0 || best.reserve)
             two.refine_split = default_refine_split(n);
+        one.reserve = 0;
         one.refine_split = 0;
-        const std::vector<naer_schedule> cand{ two, one };
-        if ((rc = trial(cand, score, per_batch)) == NAER_OK) {
-            const size_t def = best.refine_split == 0 ? 1 : 0, alt = 1 - def;
-            const size_t i = score[alt] < 0.9 * score[def] ? alt : def;
+        res.reserve = 1;
+        res.refine_split = default_res_parts(n);
+        if (best.reserve || best.refine_split == 0)
+            cand.push_back(two);
+        if (direct_ok(n) && best.refine_split != 0)
+            cand.push_back(one);
+        if (!best.reserve && res_fits(n, res.segments_per_block, res.refine_split))
+            cand.push_back(res);
+        if (cand.size() > 1 && (rc = trial(cand, score, per_batch)) == NAER_OK) {
+            size_t i = 0;
+            for (size_t k = 1; k < cand.size(); ++k)
+                if (score[k] < 0.9 * score[0] && score[k] < score[i])
+                    i = k;
             best = cand[i];
             best_us = per_batch[i];
         }
@@ -2375,11 +2529,12 @@ int naer_net_autotune(naer_net* n, uint32_t batches, naer_schedule* chosen)
     if (rc == NAER_OK) {
         std::vector<naer_schedule> cand;
         std::vector<uint32_t> splits;
+        const uint32_t top = best.reserve ? n->parts : 64u;
         if (best.refine_split == 0)
             splits = { 0u };
         else
             splits = { std::max<uint32_t>(1, best.refine_split / 2), best.refine_split,
-                       std::min<uint32_t>(64, best.refine_split * 2) };
+                       std::min<uint32_t>(top, best.refine_split * 2) };
         for (uint32_t task : { 4 * kWave, 8 * kWave })
             for (uint32_t g : { std::max<uint32_t>(1, best.segments_per_block / 2), best.segments_per_block,
                                 std::min<uint32_t>(kMaxSegSchedule, best.segments_per_block * 2) })
@@ -2388,6 +2543,8 @@ int naer_net_autotune(naer_net* n, uint32_t batches, naer_schedule* chosen)
                     c.task_synapses = task;
                     c.segments_per_block = g;
                     c.refine_split = r;
+                    if (c.reserve && !res_fits(n, g, r))
+                        continue;
                     bool dup = false;
                     for (auto& x : cand)
                         dup = dup || (x.task_synapses == task && x.segments_per_block == g && x.refine_split == r);
@@ -2621,7 +2778,7 @@ int naer_net_raster_close(naer_net* n, uint64_t* n_total)
 extern "C++" {
 namespace {
 
-constexpr char kCheckpointMagic[8] = { 'N', 'A', 'E', 'R', 'C', 'K', 'P', '4' };
+constexpr char kCheckpointMagic[8] = { 'N', 'A', 'E', 'R', 'C', 'K', 'P', '5' };
 
 // FNV-1a over the description of every network (populations, connections,
 // STDP parameters, step, mode, seeds): a checkpoint loads only into a net
@@ -2658,6 +2815,7 @@ uint64_t description_hash(const naer_net* n)
 struct CheckpointShape {
     uint64_t N, S_cap, D, B, R, NBK, e_cap, n_seg, rec_cap, seg_cap, desc_hash;
     uint32_t det, plastic, poisson, bucketed, rec_bytes, defer;
+    uint32_t parts, _pad;  // part slots of the reserved placement's tile ranges
 };
 
 CheckpointShape shape_of(const naer_net* n)
@@ -2681,6 +2839,7 @@ CheckpointShape shape_of(const naer_net* n)
     c.bucketed = n->bucketed;
     c.rec_bytes = n->rec_bytes;
     c.defer = deferred_delivery(n) ? 1u : 0u;
+    c.parts = n->res_tb ? n->parts : 0u;
     return c;
 }
 
@@ -2790,20 +2949,23 @@ int checkpoint_body(naer_net* n, CheckpointIO& io)
     if ((rc = io.copy(n->rec, std::min<uint64_t>(st[0].rec_n, n->rec_cap) * sizeof(naer_spike))))
         return rc;
     if (n->bucketed) {
-        // flags and starts of every buffer, then the events of the batches in flight
-        std::vector<uint32_t> flags, bs;
+        // flags and starts of every buffer (bit 0: bucketed, bit 1: some
+        // events in the ring, bit 2: the reserved placement's tile ranges),
+        // the tile ranges, then the events of the batches in flight
+        std::vector<uint32_t> flags, bs, tb;
         rc = io.checked(n->flags, n->R, &flags, [&](const std::vector<uint32_t>& v) {
             for (uint32_t f : v)
-                if (f > 1)
+                if (f > 7 || ((f & 6u) && !(f & 1u)) || ((f & 4u) && !n->res_tb))
                     return corrupt("bucket flags");
             return NAER_OK;
         });
         if (rc)
             return rc;
         const size_t row = (size_t)n->NBK + 1;
+        std::vector<uint64_t> used(n->R, 0);
         rc = io.checked(n->bs, n->R * row, &bs, [&](const std::vector<uint32_t>& v) {
             for (uint32_t r = 0; r < n->R; ++r) {
-                if (!flags[r])
+                if (flags[r] != 1u && flags[r] != 3u)
                     continue;
                 const uint32_t* b = v.data() + r * row;
                 if (b[0] != 0 || b[n->NBK] > n->e_cap)
@@ -2816,11 +2978,37 @@ int checkpoint_body(naer_net* n, CheckpointIO& io)
         });
         if (rc)
             return rc;
-        for (uint32_t r = 0; r < n->R; ++r) {
-            const uint64_t used = flags[r] ? bs[r * row + n->NBK] : 0;
-            if ((rc = io.copy((char*)n->ev + (size_t)r * n->e_cap * n->rec_bytes, used * n->rec_bytes)))
+        for (uint32_t r = 0; r < n->R; ++r)
+            if (flags[r] == 1u || flags[r] == 3u)
+                used[r] = bs[r * row + n->NBK];
+        if (n->res_tb) {
+            const size_t gt1 = ((size_t)1 << n->gt_shift) + 1;
+            const size_t per = (size_t)(n->NBK >> n->gt_shift) * n->parts;  // ranges per buffer
+            rc = io.checked(n->res_tb, n->R * per * gt1, &tb, [&](const std::vector<uint32_t>& v) {
+                for (uint32_t r = 0; r < n->R; ++r) {
+                    if (!(flags[r] & 4u))
+                        continue;
+                    for (size_t k = 0; k < per; ++k) {
+                        const uint32_t* t = v.data() + (r * per + k) * gt1;
+                        for (size_t i = 0; i + 1 < gt1; ++i)
+                            if (t[i] > t[i + 1])
+                                return corrupt("tile ranges");
+                        if (t[gt1 - 1] > n->e_cap)
+                            return corrupt("tile ranges");
+                    }
+                }
+                return NAER_OK;
+            });
+            if (rc)
                 return rc;
+            for (uint32_t r = 0; r < n->R; ++r)
+                if (flags[r] & 4u)
+                    for (size_t k = 0; k < per; ++k)
+                        used[r] = std::max<uint64_t>(used[r], tb[(r * per + k) * gt1 + gt1 - 1]);
         }
+        for (uint32_t r = 0; r < n->R; ++r)
+            if ((rc = io.copy((char*)n->ev + (size_t)r * n->e_cap * n->rec_bytes, used[r] * n->rec_bytes)))
+                return rc;
     }
     if (deferred_delivery(n)) {
         // the spikes of an aligned batch that is not complete wait in its lists

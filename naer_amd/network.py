@@ -311,7 +311,7 @@ class Network:
     def run(self, n_steps):
         """Advance n_steps steps of dt (asynchronous); in the event-driven mode
         (mode=_abi.MODE_EVENT) n_steps windows of 1500 us."""
-        check(_abi.lib().naer_net_step(self._h(), int(n_steps)))
+        check(_abi.lib().naer_net_step(self._h(), _abi.steps_arg(n_steps)))
 
     def finish(self):
         check(_abi.lib().naer_net_sync(self._h()))
@@ -321,8 +321,9 @@ class Network:
         """The delivery schedule (naer_schedule) as a dict."""
         return _abi.get_schedule(self._h())
 
-    def set_schedule(self, task_synapses, segments_per_block, refine_split, run_aggregate=None):
-        _abi.set_schedule(self._h(), task_synapses, segments_per_block, refine_split, run_aggregate)
+    def set_schedule(self, task_synapses, segments_per_block, refine_split, run_aggregate=None, reserve=None):
+        """reserve 1: reserved placement (refine_split: its parts per group bucket)."""
+        _abi.set_schedule(self._h(), task_synapses, segments_per_block, refine_split, run_aggregate, reserve)
 
     def autotune(self, batches=2):
         """Try delivery schedules on the running simulation (real steps) and

@@ -58,7 +58,7 @@ class Pack:
         check(_abi.lib().naer_net_set_graph(self._handle, int(chunk_steps)))
 
     def run(self, n_steps):
-        check(_abi.lib().naer_net_step(self._handle, int(n_steps)))
+        check(_abi.lib().naer_net_step(self._handle, _abi.steps_arg(n_steps)))
 
     def sync(self):
         check(_abi.lib().naer_net_sync(self._handle))

@@ -129,11 +129,17 @@ def _row_ptr(net):
 
 def test_full_size_events_conserved_and_deterministic():
     """1M neurons, 2e9 synapses: every recorded spike delivers exactly its
-    out-degree, and two runs give the same raster bit for bit."""
+    out-degree, and two runs, the second with the other placement (counted
+    two-level or reserved), give the same raster bit for bit."""
     rasters = []
-    for _ in range(2):
+    for k in range(2):
         net = models.large_1m(mode=_abi.MODE_DETERMINISTIC, record_capacity=2_000_000)
         net.compile()
+        s = net.schedule()
+        assert s["parts_max"] >= 4
+        if k == 1:
+            res = 1 - s["reserve"]
+            net.set_schedule(s["task_synapses"], s["segments_per_block"], 4 if res else 3, reserve=res)
         net.run(400)
         net.finish()
         net.spikes()  # drain the warm-up
@@ -172,6 +178,71 @@ def test_direct_placement_matches_oracle(kw):
     net.close()
 
 
+@pytest.mark.parametrize("kw,margin", [({}, None), ({"plastic": True}, None), ({"random_weights": True}, None),
+                                       ({}, "0.3"), ({"plastic": True}, "0.3"), ({"random_weights": True}, "0")])
+def test_reserved_placement_matches_oracle(kw, margin):
+    """Reserved placement (no counting passes: regions reserved from the
+    batch's event count and the net's synapse shares, a counting sort per
+    part of a group bucket): 4-byte class records, 8-byte records with
+    weights, plastic rows.  A margin of 0.3 sends the events beyond the
+    regions into the spill regions (and some through the ring), 0 sends all
+    but 16 per region and 64 per group bucket through the ring."""
+    if margin is not None:
+        os.environ["NAER_RESERVE_MARGIN"] = margin
+    try:
+        net = multi_delay_net(**kw)
+        net.compile()
+    finally:
+        os.environ.pop("NAER_RESERVE_MARGIN", None)
+    o = oracle_of(net, O.FIXED)
+    s = net.schedule()
+    assert s["parts_max"] >= 4 and s["reserve"] in (0, 1)
+    for parts in (4, 1, s["parts_max"]):
+        net.set_schedule(s["task_synapses"], s["segments_per_block"], parts, reserve=1)
+        assert net.schedule()["reserve"] == 1 and net.schedule()["refine_split"] == parts
+        net.run(151)  # ends inside a batch (plastic nets deliver it through the ring)
+    o.run(453)
+    assert_same(net, o)
+    if kw.get("plastic"):
+        np.testing.assert_array_equal(net.csr()[2].view(np.uint32), o.csr()[2].view(np.uint32))
+    net.close()
+
+
+def test_reserved_placement_switches_and_graphs():
+    """Batches in flight keep the layout they were placed with when the
+    schedule switches between reserved and counted placement; deferred
+    launches and graph-captured chunks with reserved placement."""
+    net = multi_delay_net()
+    net.compile()
+    o = oracle_of(net, O.FIXED)
+    s = net.schedule()
+    total = 0
+    for i, c in enumerate([37, 16, 61, 15, 8, 94, 45]):
+        net.set_schedule(s["task_synapses"], s["segments_per_block"], 2 if i % 2 == 0 else s["refine_split"],
+                         reserve=1 if i % 2 == 0 else 0)
+        net.run(c)
+        total += c
+    net.set_schedule(s["task_synapses"], s["segments_per_block"], 3, reserve=1)
+    net.set_graph(25)  # whole batches: 30 steps
+    for c in [37, 61, 100]:
+        net.run(c)
+        total += c
+    o.run(total)
+    assert_same(net, o)
+    net.close()
+
+
+def test_reserved_placement_refused_where_unavailable():
+    net = multi_delay_net()
+    net.compile()
+    s = net.schedule()
+    with pytest.raises(_abi.NaerError):
+        net.set_schedule(s["task_synapses"], s["segments_per_block"], s["parts_max"] + 1, reserve=1)
+    with pytest.raises(_abi.NaerError):
+        net.set_schedule(s["task_synapses"], s["segments_per_block"], 0, reserve=1)
+    net.close()
+
+
 def test_direct_placement_refused_beyond_its_counters():
     """More tile buckets than the placement's 8192 LDS counters (1M neurons in
     1024-neuron tiles x 9 arrival batches): direct placement is refused."""
@@ -194,17 +265,17 @@ def test_autotune_and_schedules_leave_results_unchanged():
     assert chosen["bucketed"] == 1 and chosen["task_synapses"] in (256, 512)
     steps = int(net.stats().step)
     assert steps > 100
-    net.set_schedule(256, 1, 3, run_aggregate=1)
+    net.set_schedule(256, 1, 3, run_aggregate=1, reserve=0)
     net.run(60)
-    net.set_schedule(512, 2, 2, run_aggregate=2)
+    net.set_schedule(512, 2, 2, run_aggregate=2, reserve=1)  # reserved placement, 2 parts
     net.run(40)
-    net.set_schedule(512, 4, 1, run_aggregate=3)
+    net.set_schedule(512, 4, 1, run_aggregate=3, reserve=0)
     net.run(40)
-    net.set_schedule(512, 4, 0, run_aggregate=0)  # direct placement
+    net.set_schedule(512, 4, 0, run_aggregate=0, reserve=0)  # direct placement
     net.run(40)
-    net.set_schedule(512, 4, 1, run_aggregate=0)
-    net.run(700 - steps - 180)
-    o.run(700)
+    net.set_schedule(512, 4, 1, run_aggregate=0, reserve=0)
+    net.run(200)
+    o.run(steps + 380)
     assert_same(net, o)
     with pytest.raises(Exception):
         net.set_schedule(128, 1, 1)

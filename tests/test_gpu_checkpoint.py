@@ -12,11 +12,19 @@ from test_gpu_bucketed import multi_delay_net
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("plastic,random_weights", [(False, False), (True, False), (False, True)])
-def test_resume_continues_bit_for_bit(tmp_path, plastic, random_weights):
+@pytest.mark.parametrize("plastic,random_weights,reserve",
+                         [(False, False, False), (True, False, False), (False, True, False), (False, False, True),
+                          (True, False, True)])
+def test_resume_continues_bit_for_bit(tmp_path, plastic, random_weights, reserve):
+    """reserve: the saved net places its batches in reserved regions; the
+    loading net keeps its default schedule and reads the batches in flight
+    by the layout they were placed with."""
     path = tmp_path / "net.ckpt"
     a = multi_delay_net(plastic=plastic, random_weights=random_weights)
     a.compile()
+    if reserve:
+        s = a.schedule()
+        a.set_schedule(s["task_synapses"], s["segments_per_block"], 4, reserve=1)
     a.run(157)  # not a multiple of the 15-step batch
     a.save(path)
     a.spikes()  # the saved spikes stay recorded in the checkpoint as well
@@ -74,9 +82,9 @@ def test_load_rejects_another_description(tmp_path):
     b.close()
 
 
-# byte offset of the saved device clock: magic (8), shape (11 x 8 + 6 x 4),
+# byte offset of the saved device clock: magic (8), shape (11 x 8 + 8 x 4),
 # step count (8), dropped spikes (8)
-CLOCK_OFFSET = 8 + 112 + 16
+CLOCK_OFFSET = 8 + 120 + 16
 
 
 @pytest.mark.parametrize("field,delta", [(0, 1), (8, 10**6), (28, 1)])

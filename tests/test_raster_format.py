@@ -51,3 +51,11 @@ def test_sort_spikes_equals_the_field_sort():
     want = np.sort(sp, order=["step", "neuron", "time_us"])
     assert np.array_equal(_abi.sort_spikes(sp), want)
     assert len(_abi.sort_spikes(sp[:1])) == 1 and len(_abi.sort_spikes(sp[:0])) == 0
+
+
+def test_step_counts_outside_uint32_are_refused():
+    from naer_amd import _abi
+    assert _abi.steps_arg(0) == 0 and _abi.steps_arg(2**32 - 1) == 2**32 - 1
+    for bad in (-1, -480, 2**32):
+        with pytest.raises(ValueError):
+            _abi.steps_arg(bad)
