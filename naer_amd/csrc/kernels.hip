@@ -267,6 +267,22 @@ __device__ __forceinline__ void emit(Stage& sg, bool fire, uint32_t neuron, uint
 // EXTRA: the rarely used population kinds (POISSON sources, source-defined
 // models) are compiled in; nets of LIF populations only run without them.
 // REC4: the bucketed events are 4-byte class records (kRec4*).
+// Workgroups are dealt to the 8 XCDs round-robin (block b to XCD b % 8);
+// xcd_block gives XCD x the x-th contiguous share of the logical blocks, so
+// that neighbouring tiles and segments are handled under one L2: the update
+// of a tile and the delivery blocks that walk its spikes' rows (which the
+// update loads into the caches) share an XCD, and the rows of one network,
+// whose targets' STDP records share lines, are walked under one L2.  A
+// bijection of [0, nb).
+constexpr uint32_t kXcds = 8;
+
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb)
+{
+    const uint32_t x = b % kXcds, i = b / kXcds;
+    const uint32_t q = nb / kXcds, r = nb % kXcds;
+    return x * q + min(x, r) + i;
+}
+
 template <bool DET, bool PLASTIC, bool BUCKET, bool EXTRA, uint32_t TW, bool REC4>
 __global__ __launch_bounds__(TW, (BUCKET && !PLASTIC && !EXTRA && TW == 1024) ? 8 : 1) void k_update(const UpdateArgs a)
 {
@@ -282,7 +298,9 @@ __global__ __launch_bounds__(TW, (BUCKET && !PLASTIC && !EXTRA && TW == 1024) ? 
     __shared__ uint32_t s_n;
     __shared__ unsigned long long s_events;
     __shared__ unsigned long long s_base;
-    const uint32_t bp = a.n_pops ? a.blk_pop[blockIdx.x] : kTileIdle;
+    // the block's tile (XCD-contiguous ranges of tiles)
+    const uint32_t tile = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t bp = a.n_pops ? a.blk_pop[tile] : kTileIdle;
     if (bp & kTileIdle) {
         // no population on this tile (an arena's free range): no neuron to
         // update and no spike this batch; block 0 still does the batch's
@@ -292,8 +310,8 @@ __global__ __launch_bounds__(TW, (BUCKET && !PLASTIC && !EXTRA && TW == 1024) ? 
             const unsigned long long t0 = st->clock;
             const uint32_t pl = (uint32_t)((t0 / a.Bfull) & 1u);
             if (!a.defer || t0 % a.Bfull == 0) {
-                a.seg_count[pl * a.n_seg + blockIdx.x] = 0;
-                a.seg_ev[pl * a.n_seg + blockIdx.x] = 0;
+                a.seg_count[pl * a.n_seg + tile] = 0;
+                a.seg_ev[pl * a.n_seg + tile] = 0;
             }
             if (blockIdx.x == 0) {
                 st->cur = pl;
@@ -332,7 +350,7 @@ __global__ __launch_bounds__(TW, (BUCKET && !PLASTIC && !EXTRA && TW == 1024) ? 
         a.flags[(uint32_t)(A % a.R)] = 0;
     // the thread's neuron state, loaded first so that its latency overlaps
     // the event phase
-    const uint32_t j = blockIdx.x * TW + threadIdx.x;
+    const uint32_t j = tile * TW + threadIdx.x;
     float V_in = 0.0f;
     int32_t ref_in = 0, last_in = 0;
     if (j < a.N) {
@@ -361,15 +379,15 @@ __global__ __launch_bounds__(TW, (BUCKET && !PLASTIC && !EXTRA && TW == 1024) ? 
                 uint32_t b_lo = 0, b_hi = 0;
                 if (f & 4u) {
                     const uint32_t gt = 1u << a.gt_shift;
-                    const uint32_t kc = (m - 1) * (a.NT >> a.gt_shift) + (blockIdx.x >> a.gt_shift);
+                    const uint32_t kc = (m - 1) * (a.NT >> a.gt_shift) + (tile >> a.gt_shift);
                     const uint32_t* tp = a.tb +
                                          (((size_t)buf * (a.NBK >> a.gt_shift) + kc) * a.parts + y) * (gt + 1) +
-                                         (blockIdx.x & (gt - 1));
+                                         (tile & (gt - 1));
                     b_lo = tp[0];
                     b_hi = tp[1];
                 } else if (y == 0) {
                     const uint32_t* bs = a.bs + (size_t)buf * (a.NBK + 1);
-                    const uint32_t k = (m - 1) * a.NT + blockIdx.x;
+                    const uint32_t k = (m - 1) * a.NT + tile;
                     b_lo = bs[k];
                     b_hi = bs[k + 1];
                 }
@@ -457,9 +475,9 @@ __global__ __launch_bounds__(TW, (BUCKET && !PLASTIC && !EXTRA && TW == 1024) ? 
     // the list segments of aligned batch A; a launch that continues the
     // batch (deferred delivery) appends to what the earlier launches wrote
     const uint32_t pl = (uint32_t)(A & 1u);
-    naer_spike* seg = (pl ? a.list1 : a.list0) + (size_t)blockIdx.x * a.seg_cap;
+    naer_spike* seg = (pl ? a.list1 : a.list0) + (size_t)tile * a.seg_cap;
     const bool append = a.defer && o0 > 0;
-    const uint32_t base = append ? min(a.seg_count[pl * a.n_seg + blockIdx.x], a.seg_cap) : 0u;
+    const uint32_t base = append ? min(a.seg_count[pl * a.n_seg + tile], a.seg_cap) : 0u;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         st->cur = pl;
         st->t0_cur = t0;
@@ -713,11 +731,11 @@ __global__ __launch_bounds__(TW, (BUCKET && !PLASTIC && !EXTRA && TW == 1024) ? 
     for (uint32_t i = threadIdx.x; i < staged; i += TW)
         seg[base + i] = s_list[i];
     if (threadIdx.x == 0) {
-        a.seg_count[pl * a.n_seg + blockIdx.x] = base + n;
-        unsigned long long* sev = a.seg_ev + pl * a.n_seg + blockIdx.x;
+        a.seg_count[pl * a.n_seg + tile] = base + n;
+        unsigned long long* sev = a.seg_ev + pl * a.n_seg + tile;
         *sev = (append ? *sev : 0ull) + s_events;
-        a.spk_part[blockIdx.x] += total;
-        a.ev_part[blockIdx.x] += s_events;
+        a.spk_part[tile] += total;
+        a.ev_part[tile] += s_events;
         if (total > n)
             atomicAdd(&st->overflow, total - n);
         s_base = (a.rec_cap && n) ? atomicAdd(&st->rec_n, (unsigned long long)n) : 0ull;
@@ -994,18 +1012,6 @@ struct RowStageCls : RowStage {
 // on a whole wave (q range wave-uniform).  CLS: the stage also holds the
 // class of each spike's neuron.
 // The block's segments are staged kMaxSegPerBlock at a time.
-// Workgroups are dealt to the 8 XCDs round-robin (block b to XCD b % 8);
-// xcd_block gives XCD x the x-th contiguous share of the logical blocks, so
-// the neighbouring segments (rows of one network, whose targets' STDP
-// records share lines) are walked under one L2.  A bijection of [0, nb).
-constexpr uint32_t kXcds = 8;
-
-__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb)
-{
-    const uint32_t x = b % kXcds, i = b / kXcds;
-    const uint32_t q = nb / kXcds, r = nb % kXcds;
-    return x * q + min(x, r) + i;
-}
 
 template <uint32_t U, bool CLS, typename RS, typename F>
 __device__ __forceinline__ void for_block_chunks(const BucketArgs& a, uint32_t cur, uint32_t* s_off,
@@ -1599,7 +1605,7 @@ __global__ __launch_bounds__(kRefineThreads) void k_bucket_refine(const BucketAr
 // turn, kResWalk regions of up to 128 events at a time.
 constexpr uint32_t kResThreads = 512;
 constexpr uint32_t kResCacheBytes = 24576;
-constexpr uint32_t kResWalk = 8;  // regions a wave loads at once
+constexpr uint32_t kResWalk = 4;  // regions a wave loads at once (at most 64 VGPRs: four blocks per CU)
 
 template <bool REC4>
 __global__ __launch_bounds__(kResThreads) void k_bucket_refine_res(const BucketArgs a)
