@@ -340,7 +340,8 @@ __global__ __launch_bounds__(TW, (BUCKET && !PLASTIC && !EXTRA && TW == 1024) ? 
     // (reserved placement: `parts` ranges per batch, those of the tile in
     // the parts of its group bucket)
     __shared__ uint32_t s_pre[kResMaxRanges + 1];
-    __shared__ long long s_evbase[kResMaxRanges];
+    __shared__ __attribute__((aligned(16))) long long s_evbase[kResMaxRanges];  // 1 KB: also the prefetch's sink
+    static_assert(sizeof(s_evbase) >= kWave * 16, "the prefetch writes 16 bytes per lane");
     __shared__ uint32_t s_live[kResMaxRanges];
     __shared__ uint32_t s_ring_live;
     const unsigned long long A = t0 / a.Bfull;
@@ -674,9 +675,27 @@ __global__ __launch_bounds__(TW, (BUCKET && !PLASTIC && !EXTRA && TW == 1024) ? 
                             const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)row_beg, l);
                             const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(row_beg >> 32), l);
                             const uint32_t len = (uint32_t)__builtin_amdgcn_readlane((int)row_len, l);
-                            const uint32_t* row = a.col + (((unsigned long long)hi << 32) | lo);
-                            for (uint32_t q = lane_id(); q < len; q += kWave)
-                                __builtin_amdgcn_global_load_lds((const void*)(row + q), (LdsSink*)s_evbase, 4, 0, 0);
+                            // 16-byte loads over the row's aligned interior
+                            // (LDS: 1 KB per wave instruction), 4-byte
+                            // loads for its ends
+                            const unsigned long long b = ((unsigned long long)hi << 32) | lo, e = b + len;
+                            const unsigned long long a0 = (b + 3) & ~3ull, a1 = e & ~3ull;
+                            const uint32_t ln = lane_id();
+                            if (a0 >= a1) {
+                                for (uint32_t q = ln; q < len; q += kWave)
+                                    __builtin_amdgcn_global_load_lds((const void*)(a.col + b + q), (LdsSink*)s_evbase,
+                                                                     4, 0, 0);
+                            } else {
+                                if (ln < a0 - b)
+                                    __builtin_amdgcn_global_load_lds((const void*)(a.col + b + ln), (LdsSink*)s_evbase,
+                                                                     4, 0, 0);
+                                if (ln < e - a1)
+                                    __builtin_amdgcn_global_load_lds((const void*)(a.col + a1 + ln), (LdsSink*)s_evbase,
+                                                                     4, 0, 0);
+                                for (unsigned long long q = a0 + 4ull * ln; q < a1; q += 4ull * kWave)
+                                    __builtin_amdgcn_global_load_lds((const void*)(a.col + q), (LdsSink*)s_evbase, 16,
+                                                                     0, 0);
+                            }
                         }
                     }
                 }
