@@ -1620,11 +1620,11 @@ __global__ __launch_bounds__(kRefineThreads) void k_bucket_refine(const BucketAr
 // by tile into the part's span of ev[E % R], which starts where its first
 // region starts: counted per tile in LDS, scanned, placed by LDS cursors.  A
 // part of at most kResCache events is read once (kept in LDS), a larger one
-// twice.  The tile ranges go to tb[E % R][kc][y].  Waves take the regions in
-// turn, kResWalk regions of up to 128 events at a time.
+// twice.  The tile ranges go to tb[E % R][kc][y].
 constexpr uint32_t kResThreads = 512;
 constexpr uint32_t kResCacheBytes = 24576;
-constexpr uint32_t kResWalk = 4;  // regions a wave loads at once (at most 64 VGPRs: four blocks per CU)
+constexpr uint32_t kResLoads = 8;  // events a thread loads at once
+constexpr uint32_t kResChunks = 256;  // 64-event chunks whose first region is tabled
 
 template <bool REC4>
 __global__ __launch_bounds__(kResThreads) void k_bucket_refine_res(const BucketArgs a)
@@ -1637,6 +1637,7 @@ __global__ __launch_bounds__(kResThreads) void k_bucket_refine_res(const BucketA
     __shared__ uint32_t s_cnt[kMaxGroupTiles];
     __shared__ uint32_t s_pos[kMaxGroupTiles + 1];
     __shared__ uint32_t s_wsum[kResThreads / kWave];
+    __shared__ uint16_t s_reg[kResChunks];
     const uint32_t ngb = a.NBK >> a.gt_shift;
     const uint32_t gt = 1u << a.gt_shift;
     const uint32_t kc = blockIdx.x / a.RS, y = blockIdx.x - kc * a.RS;
@@ -1701,6 +1702,20 @@ __global__ __launch_bounds__(kResThreads) void k_bucket_refine_res(const BucketA
     __syncthreads();
     const uint32_t n = s_q[nr];
     const bool cached = n <= kCache;
+    // region of the first event of each 64-event chunk
+    for (uint32_t ch = threadIdx.x; ch < kResChunks && (ch << 6) < n; ch += blockDim.x) {
+        const uint32_t i = ch << 6;
+        uint32_t lo = 0, hi = nr;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (s_q[mid] <= i)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        s_reg[ch] = lo;
+    }
+    __syncthreads();
     if (last && threadIdx.x == 0)
         a.spill[kc] = 0;  // read above (before the barrier); the next batch's placement starts over
     const Rec* stage = (const Rec*)a.stage;
@@ -1718,46 +1733,42 @@ __global__ __launch_bounds__(kResThreads) void k_bucket_refine_res(const BucketA
         else
             return make_uint2((e.x & tmask) | (e.x & 0xFFFF0000u), e.y);
     };
-    const uint32_t nwv = blockDim.x / kWave;
-    // calls f(event, index in the part) for every event of the part's regions
+    // calls f(event, index in the part) for every event of the part's
+    // regions: thread t takes events t, t + 512, ... (kResLoads loads in
+    // flight), each finding its region by a binary search of the prefix
     auto walk = [&](auto f) {
-        for (uint32_t r0 = wave * kResWalk; r0 < nr; r0 += nwv * kResWalk) {
-            Rec e[2 * kResWalk];
-            uint32_t at[2 * kResWalk];
+        for (uint32_t i0 = threadIdx.x; i0 < n; i0 += blockDim.x * kResLoads) {
+            Rec e[kResLoads];
 #pragma unroll
-            for (uint32_t k = 0; k < 2 * kResWalk; ++k) {
-                const uint32_t r = r0 + (k >> 1), j = lane + (k & 1) * kWave;
-                at[k] = 0xFFFFFFFFu;
-                if (r < nr && j < s_q[r + 1] - s_q[r]) {
-                    e[k] = stage[s_src[r] + j];
-                    at[k] = s_q[r] + j;
+            for (uint32_t u = 0; u < kResLoads; ++u) {
+                const uint32_t i = i0 + u * blockDim.x;
+                if (i < n) {
+                    // the last region r with s_q[r] <= i: from the region of
+                    // the event's 64-event chunk (regions hold ~80 events)
+                    uint32_t lo;
+                    if ((i >> 6) < kResChunks) {
+                        lo = s_reg[i >> 6];
+                        while (s_q[lo + 1] <= i)
+                            ++lo;
+                    } else {
+                        lo = 0;
+                        uint32_t hi = nr;
+                        while (hi - lo > 1) {
+                            const uint32_t mid = (lo + hi) >> 1;
+                            if (s_q[mid] <= i)
+                                lo = mid;
+                            else
+                                hi = mid;
+                        }
+                    }
+                    e[u] = stage[s_src[lo] + (i - s_q[lo])];
                 }
             }
 #pragma unroll
-            for (uint32_t k = 0; k < 2 * kResWalk; ++k)
-                if (at[k] != 0xFFFFFFFFu)
-                    f(e[k], at[k]);
-            // regions beyond 128 events: 8 loads in flight at a time
-            for (uint32_t k = 0; k < kResWalk; ++k) {
-                const uint32_t r = r0 + k;
-                if (r >= nr)
-                    break;
-                const uint32_t len = s_q[r + 1] - s_q[r], src = s_src[r], dst = s_q[r];
-                for (uint32_t j0 = 2 * kWave; j0 < len; j0 += 8 * kWave) {
-                    Rec e2[8];
-#pragma unroll
-                    for (uint32_t u = 0; u < 8; ++u) {
-                        const uint32_t j = j0 + u * kWave + lane;
-                        if (j < len)
-                            e2[u] = stage[src + j];
-                    }
-#pragma unroll
-                    for (uint32_t u = 0; u < 8; ++u) {
-                        const uint32_t j = j0 + u * kWave + lane;
-                        if (j < len)
-                            f(e2[u], dst + j);
-                    }
-                }
+            for (uint32_t u = 0; u < kResLoads; ++u) {
+                const uint32_t i = i0 + u * blockDim.x;
+                if (i < n)
+                    f(e[u], i);
             }
         }
     };
