@@ -269,9 +269,10 @@ __device__ __forceinline__ void emit(Stage& sg, bool fire, uint32_t neuron, uint
 // REC4: the bucketed events are 4-byte class records (kRec4*).
 // Workgroups are dealt to the 8 XCDs round-robin (block b to XCD b % 8);
 // xcd_block gives XCD x the x-th contiguous share of the logical blocks, so
-// that neighbouring tiles and segments are handled under one L2: the update
-// of a tile and the delivery blocks that walk its spikes' rows (which the
-// update loads into the caches) share an XCD, and the rows of one network,
+// that neighbouring tiles and segments are handled under one L2: with the
+// row prefetch, the update of a tile and the delivery blocks that walk its
+// spikes' rows (which the update loads into the caches) share an XCD, and
+// the rows of one network,
 // whose targets' STDP records share lines, are walked under one L2.  A
 // bijection of [0, nb).
 constexpr uint32_t kXcds = 8;
@@ -298,8 +299,11 @@ __global__ __launch_bounds__(TW, (BUCKET && !PLASTIC && !EXTRA && TW == 1024) ? 
     __shared__ uint32_t s_n;
     __shared__ unsigned long long s_events;
     __shared__ unsigned long long s_base;
-    // the block's tile (XCD-contiguous ranges of tiles)
-    const uint32_t tile = xcd_block(blockIdx.x, gridDim.x);
+    // the block's tile: with the row prefetch, XCD-contiguous ranges of tiles
+    // (an XCD's update blocks load the rows its placement blocks walk); else
+    // block b (packs: XCD-contiguous ranges put a few networks' tiles on each
+    // XCD, measured 13 % slower on the sweep)
+    const uint32_t tile = a.prefetch ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
     const uint32_t bp = a.n_pops ? a.blk_pop[tile] : kTileIdle;
     if (bp & kTileIdle) {
         // no population on this tile (an arena's free range): no neuron to
