@@ -89,16 +89,21 @@ def test_irregular_run_lengths_bit_exact(random_weights):
     net.close()
 
 
-@pytest.mark.parametrize("cap,random_weights", [(2_000, False), (150_000, False), (150_000, True)])
-def test_event_buffer_overflow_falls_back_to_ring(cap, random_weights):
-    # a small event buffer: every batch (2_000) or the busier ones (150_000)
-    # go through the ring from the placement kernel
+@pytest.mark.parametrize("cap,random_weights,reserve", [(2_000, False, 0), (150_000, False, 0), (150_000, True, 0),
+                                                        (2_000, False, 1), (400_000, False, 1)])
+def test_event_buffer_overflow_falls_back_to_ring(cap, random_weights, reserve):
+    # a small event buffer: every batch (2_000) or the busier ones (150_000;
+    # 400_000 with reserved placement, whose layout needs about twice the
+    # events) go through the ring from the placement kernel
     os.environ["NAER_BUCKET_EVENTS"] = str(cap)
     try:
         net = multi_delay_net(random_weights=random_weights)
         net.compile()
     finally:
         del os.environ["NAER_BUCKET_EVENTS"]
+    if reserve:
+        s = net.schedule()
+        net.set_schedule(s["task_synapses"], s["segments_per_block"], 4, reserve=1)
     o = oracle_of(net, O.FIXED)
     net.run(450)
     o.run(450)
