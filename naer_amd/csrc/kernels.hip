@@ -605,7 +605,7 @@ __global__ __launch_bounds__(TW, (BUCKET && !PLASTIC && !EXTRA && TW == 1024) ? 
                     I = DET ? from_fixed((long long)cur) : (float)cur;
                 }
                 const unsigned long long t = t0 + s;
-                if (a.probe && t - a.probe_t0 < a.probe_n)
+                if (EXTRA && a.probe && t - a.probe_t0 < a.probe_n)
                     a.probe[(size_t)(t - a.probe_t0) * a.N + j] = I;
                 return I;
             };

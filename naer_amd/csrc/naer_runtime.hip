@@ -2046,8 +2046,10 @@ static bool deferred_delivery(const naer_net* n)
 static hipError_t do_update(naer_net* n, void* arg)
 {
     UpdateArgs* a = (UpdateArgs*)arg;
-    return launch_update(*a, n->det, n->plastic, n->bucketed, n->poisson_ok || !n->mpops.empty(),
-                         n->rec_bytes == 4, n->stream);
+    // the input probe is compiled into the EXTRA variants only (it costs the
+    // plain variant's step loop scalar registers)
+    return launch_update(*a, n->det, n->plastic, n->bucketed,
+                         n->poisson_ok || !n->mpops.empty() || a->probe != nullptr, n->rec_bytes == 4, n->stream);
 }
 
 static hipError_t do_scatter(naer_net* n, void* arg)
