@@ -1198,11 +1198,11 @@ int setup_reserve(naer_net* n)
     HIP_TRY(hipMemset(n->res_tb, 0, (size_t)n->R * ngb * parts * (gt + 1) * sizeof(uint32_t)));
     n->parts = parts;
     n->res_ok = true;
-    // the default only where the reserved layout of a batch of half the event
-    // buffer's capacity still fits it (a batch that does not fit goes through
-    // the ring)
+    // the default only where the reserved layout of a batch of a quarter of
+    // the event buffer's capacity still fits it (the layout takes about twice
+    // a batch's events; a batch that does not fit goes through the ring)
     if (ngb >= kResDefaultGroups && res_fits(n, n->G, default_res_parts(n)) &&
-        res_base(cc[ngb], n->e_cap / 2, ngb, n->NB) <= n->e_cap) {
+        res_base(cc[ngb], n->e_cap / 4, ngb, n->NB) <= n->e_cap) {
         n->RES = 1;
         n->RS = default_res_parts(n);
     }

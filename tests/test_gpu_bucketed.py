@@ -142,6 +142,8 @@ def test_full_size_events_conserved_and_deterministic():
         net.compile()
         s = net.schedule()
         assert s["parts_max"] >= 4
+        if k == 0:
+            assert s["reserve"] == 1  # the default of this net (558 group buckets)
         if k == 1:
             res = 1 - s["reserve"]
             net.set_schedule(s["task_synapses"], s["segments_per_block"], 4 if res else 3, reserve=res)
