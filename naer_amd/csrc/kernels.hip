@@ -1292,7 +1292,8 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_place(const BucketArg
     // s_end[kc] (s_beg[kc]: the start)
     uint32_t* s_end = s_cur + a.n_cnt;
     uint32_t* s_beg = s_end + a.n_cnt;
-    __shared__ unsigned long long s_E;
+    uint32_t* s_spb = s_beg + a.n_cnt;  // the group bucket's spill region [s_spb, s_spe)
+    uint32_t* s_spe = s_spb + a.n_cnt;
     const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);  // the block's segments (as for_block_chunks)
     bool ok;
     if (a.reserve) {
@@ -1335,7 +1336,6 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_place(const BucketArg
         }
         ok = tot <= a.e_cap && res_base(cc_all, tot, nbc, a.NB) <= a.e_cap;
         if (threadIdx.x == 0) {
-            s_E = tot;
             a.pb[lb] = pre;
             if (lb == 0) {
                 a.pb[a.NB] = tot;
@@ -1357,6 +1357,9 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_place(const BucketArg
                     s_cur[i] = b0;
                     s_end[i] = (uint32_t)(base + ((c * (pre + mine)) >> kResShift) +
                                           (unsigned long long)kResPad * (lb + 1));
+                    const uint32_t sb = (uint32_t)(base + ((c * tot) >> kResShift) + (unsigned long long)kResPad * a.NB);
+                    s_spb[i] = sb;
+                    s_spe[i] = sb + (uint32_t)((c * tot) >> (kResShift + 2)) + kResSpillPad;
                 }
             }
         }
@@ -1452,13 +1455,10 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_place(const BucketArg
                     bool store = true;
                     if (a.reserve && pos >= s_end[key]) {
                         // the region is full: the bucket's spill region, else the ring
-                        const unsigned long long cr = a.rc[key], E = s_E;
-                        const unsigned long long sp_base = res_base(a.rcc[key], E, key, a.NB) + ((cr * E) >> kResShift) +
-                                                           (unsigned long long)kResPad * a.NB;
-                        const uint32_t sp_cap = (uint32_t)((cr * E) >> (kResShift + 2)) + kResSpillPad;
                         const uint32_t sp = atomicAdd(&a.spill[key], 1u);
-                        pos = (uint32_t)sp_base + sp;
-                        if (sp >= sp_cap) {
+                        const uint32_t sb = s_spb[key];
+                        pos = sb + sp;
+                        if (sp >= s_spe[key] - sb) {
                             store = false;
                             uint32_t sl = (uint32_t)((t0 + r) % a.D) + ((c[u] >> 24) & 0x7Fu);
                             sl = sl >= a.D ? sl - a.D : sl;
@@ -2263,8 +2263,9 @@ hipError_t launch_bucket_step(const BucketArgs& a, int step, bool det, hipStream
     if (e != hipSuccess)
         return e;
     const size_t lds = (size_t)a.NBK * sizeof(uint32_t);
-    // reserved placement: each counter with its region's end and start
-    const size_t lds_c = (size_t)a.n_cnt * sizeof(uint32_t) * (a.reserve ? 3 : 1);
+    // reserved placement: each counter with its region's end and start and
+    // its spill region
+    const size_t lds_c = (size_t)a.n_cnt * sizeof(uint32_t) * (a.reserve ? kResCounterWords : 1);
     switch (step) {
     case 1:
         if (a.reserve)

@@ -387,7 +387,9 @@ constexpr uint32_t kResMaxShare = 16u << kResShift;  // sum of c: at most 16 rec
 constexpr uint32_t kResMaxParts = 8;    // RS bound
 constexpr uint32_t kResMaxRanges = 128; // M * RS bound: the update's ranges per tile
 constexpr uint32_t kResMaxPartRegions = 1024;  // regions of a part (ceil(NB / RS) + 1)
-constexpr uint32_t kResCntPerThread = 3;       // placement counters per thread: 3 words each in 32 KB of LDS
+constexpr uint32_t kResCounterWords = 5;       // LDS words per placement counter: cursor, region end and start,
+                                               // spill region start and end
+constexpr uint32_t kResCntPerThread = 2;       // placement counters per 1024-thread block: 5 words each in 32 KB
 
 // start of group bucket kc's regions for a batch of E events (Cc: its prefix);
 // with kc = groups, the extent of the staging layout

@@ -1145,7 +1145,7 @@ int setup_reserve(naer_net* n)
     const uint32_t nd = n->dmax - n->dmin + 1;
     // the update's ranges per tile, and the placement's LDS (three words per counter)
     const uint32_t parts = std::min(kResMaxParts, kResMaxRanges / n->M);
-    if ((uint64_t)nd * ng > kSynStatsBins || 3ull * ngb * sizeof(uint32_t) > 32768 || parts == 0)
+    if ((uint64_t)nd * ng > kSynStatsBins || (uint64_t)kResCounterWords * ngb * sizeof(uint32_t) > 32768 || parts == 0)
         return NAER_OK;
     int rc;
     unsigned long long* d_h = nullptr;
