@@ -456,7 +456,8 @@ typedef struct naer_schedule {
                                     events beyond them into spill regions (beyond those, the ring);
                                     refine_split is then the refine parts per group bucket,
                                     1..parts_max.  Refused where the shares are unknown (packs,
-                                    arenas, nets whose (delay, group) table exceeds 8192 entries) */
+                                    arenas, nets whose (delay, group) table exceeds 8192 entries or
+                                    with more than 1638 group buckets) */
     uint32_t parts_max;          /* out: largest refine_split of reserved placement (0: unavailable) */
     uint32_t record_bytes;       /* out: bytes per bucketed event: 4 when every population's
                                     outgoing synapses share one constant weight (the record

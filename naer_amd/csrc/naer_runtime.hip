@@ -1143,7 +1143,7 @@ int setup_reserve(naer_net* n)
         return NAER_OK;
     const uint32_t ng = n->NT >> n->gt_shift, ngb = n->NBK >> n->gt_shift, gt = 1u << n->gt_shift;
     const uint32_t nd = n->dmax - n->dmin + 1;
-    // the update's ranges per tile, and the placement's LDS (three words per counter)
+    // the update's ranges per tile, and the placement's LDS (kResCounterWords words per counter)
     const uint32_t parts = std::min(kResMaxParts, kResMaxRanges / n->M);
     if ((uint64_t)nd * ng > kSynStatsBins || (uint64_t)kResCounterWords * ngb * sizeof(uint32_t) > 32768 || parts == 0)
         return NAER_OK;
