@@ -670,8 +670,9 @@ __global__ __launch_bounds__(TW, (BUCKET && !PLASTIC && !EXTRA && TW == 1024) ? 
                         // the delivery's first pass reads the rows of the
                         // batch's spikes: the wave loads them now, into a
                         // dead LDS array (the event phase's bases), so that
-                        // they are in the caches by then; nothing waits for
-                        // these loads before the block's final barrier
+                        // they are in the caches by then; the wave waits for
+                        // them only at its next LDS atomic (a later spike's
+                        // staging) or the block's final barrier
                         unsigned long long fm = __ballot(fired);
                         while (fm) {
                             const int l = __builtin_ctzll(fm);
