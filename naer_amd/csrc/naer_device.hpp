@@ -380,7 +380,7 @@ struct BucketArgs {
                                       // empties slots [RS, parts))
 };
 
-constexpr uint32_t kResPad = 16;        // records every region holds beyond its share
+constexpr uint32_t kResPad = 8;        // records every region holds beyond its share
 constexpr uint32_t kResSpillPad = 64;   // and every spill region beyond a quarter of its bucket's share
 constexpr uint32_t kResShift = 24;      // c in 2^-24 units
 constexpr uint32_t kResMaxShare = 16u << kResShift;  // sum of c: at most 16 records reserved per event

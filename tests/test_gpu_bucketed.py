@@ -193,7 +193,7 @@ def test_reserved_placement_matches_oracle(kw, margin):
     part of a group bucket): 4-byte class records, 8-byte records with
     weights, plastic rows.  A margin of 0.3 sends the events beyond the
     regions into the spill regions (and some through the ring), 0 sends all
-    but 16 per region and 64 per group bucket through the ring."""
+    but 8 per region and 64 per group bucket through the ring."""
     if margin is not None:
         os.environ["NAER_RESERVE_MARGIN"] = margin
     try:
